@@ -1,0 +1,356 @@
+"""amber_amd -- Python mirror of the amber_mi355x C ABI (include/amber_mi355x.h).
+
+The product is the HIP library ``libamber_mi355x.so``; this module only binds
+it with ctypes and converts numpy arrays.  It mirrors the reference's
+in-process function set (4_fbe/mabs_funcs.h:19-60) with the same argument
+meaning: ``closed_rings`` is calCR (amber_funcs.c:389-491), ``bispec_v1`` /
+``bispec_v2`` are one iteration of the rp loop of amber.c:187-326 /
+amber_v2.c:187-333, ``read_*``/``write_*`` are the reference's file formats.
+
+There is no CPU fallback: importing this module raises if the library has
+not been built (``python -m amber_amd.build``).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libamber_mi355x.so")
+if not os.path.exists(LIB_PATH):
+    raise ImportError("amber_amd: %s is missing; build it with `python -m amber_amd.build`" % LIB_PATH)
+_lib = C.CDLL(LIB_PATH)
+
+PREC_FP32, PREC_BF16X3, PREC_FP8 = 0, 1, 2
+_PREC = {"fp32": PREC_FP32, "bf16x3": PREC_BF16X3, "fp8": PREC_FP8}
+
+EXPORTS = [
+    "amb_strerror", "amb_version", "amb_read_params_ess", "amb_read_gv_header", "amb_read_cube_files",
+    "amb_free", "amb_write_cube_files", "amb_read_uf_chans", "amb_write_uf_chans", "amb_write_closed_rings",
+    "amb_read_rings", "amb_write_bsout_v1", "amb_write_bsout_v2", "amb_ng", "amb_annular_rings", "amb_uavg",
+    "amb_triangles", "amb_generate", "amb_create", "amb_destroy", "amb_last_error", "amb_set_precision",
+    "amb_load_cube", "amb_load_cube_files", "amb_release_cube", "amb_set_channels", "amb_set_rings",
+    "amb_closed_rings", "amb_bispec_v1", "amb_bispec_v2", "amb_bispec_ex", "amb_bispec_batch",
+    "amb_get_stats", "amb_reset_stats",
+]
+
+_dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int)
+
+
+class GenCfg(C.Structure):
+    _fields_ = [("N", C.c_int64), ("nc", C.c_int64), ("nflag", C.c_int), ("shared_mask", C.c_int),
+                ("count_max", C.c_int), ("seed", C.c_uint64), ("p0", C.c_double), ("u0", C.c_double)]
+
+
+class Opts(C.Structure):
+    _fields_ = [("device", C.c_int), ("precision", C.c_int), ("hbm_budget", C.c_uint64), ("verbose", C.c_int)]
+
+
+class Result(C.Structure):
+    _fields_ = [("bl", _fp), ("bl64", _dp), ("cnt", _ip), ("norm", _fp), ("norm64", _dp),
+                ("inv_ng2", C.c_double), ("ng", C.c_long), ("cert_margin", C.c_double)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("ms_fft", C.c_double), ("ms_contract", C.c_double), ("ms_reduce", C.c_double),
+                ("ms_norm", C.c_double), ("ms_total", C.c_double), ("flop_contract", C.c_double),
+                ("bytes_fft", C.c_double), ("n_contract_launch", C.c_int64), ("n_fft_launch", C.c_int64),
+                ("evals", C.c_int64), ("ms_contract_max_launch", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib.amb_strerror.restype = C.c_char_p
+_lib.amb_version.restype = C.c_char_p
+_lib.amb_last_error.restype = C.c_char_p
+_lib.amb_last_error.argtypes = [C.c_void_p]
+_lib.amb_ng.restype = C.c_long
+_lib.amb_ng.argtypes = [C.c_double, C.c_double]
+_lib.amb_create.argtypes = [C.POINTER(Opts), C.POINTER(C.c_void_p)]
+_lib.amb_destroy.argtypes = [C.c_void_p]
+_lib.amb_generate.argtypes = [C.POINTER(GenCfg), _dp, _dp, _ip, _ip]
+_lib.amb_load_cube.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_double, C.c_int32, _dp, _dp, _ip]
+_lib.amb_load_cube_files.argtypes = [C.c_void_p, C.c_char_p, C.c_int, _ip]
+_lib.amb_release_cube.argtypes = [C.c_void_p, C.c_int]
+_lib.amb_set_channels.argtypes = [C.c_void_p, C.c_int, _ip, C.c_int, C.c_int]
+_lib.amb_set_rings.argtypes = [C.c_void_p, _dp, _dp, C.c_int]
+_lib.amb_set_precision.argtypes = [C.c_void_p, C.c_int]
+_lib.amb_closed_rings.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, C.c_int, _ip]
+_lib.amb_bispec_v1.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, _fp, _ip, _dp, _dp]
+_lib.amb_bispec_v2.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _fp, _fp, _dp]
+_lib.amb_bispec_ex.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(Result)]
+_lib.amb_bispec_batch.argtypes = [C.c_void_p, C.c_int, _ip, C.c_int, _fp, _ip, _fp]
+_lib.amb_get_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
+_lib.amb_reset_stats.argtypes = [C.c_void_p]
+_lib.amb_uavg.argtypes = [_dp, _dp, C.c_int, _dp]
+_lib.amb_triangles.argtypes = [_dp, C.c_int, _ip, C.c_int]
+_lib.amb_annular_rings.argtypes = [C.c_int, _dp, _dp]
+_lib.amb_read_params_ess.argtypes = [C.c_char_p, _dp, _dp, _dp, C.c_char_p, C.c_size_t]
+_lib.amb_read_gv_header.argtypes = [C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _dp, C.POINTER(C.c_int32)]
+_lib.amb_read_cube_files.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _dp,
+                                     C.POINTER(C.c_int32), C.POINTER(_dp), C.POINTER(_dp)]
+_lib.amb_free.argtypes = [C.c_void_p]
+_lib.amb_write_cube_files.argtypes = [C.c_char_p, C.c_int, C.c_int64, C.c_int64, C.c_double, C.c_int32, _dp, _dp]
+_lib.amb_read_uf_chans.argtypes = [C.c_char_p, C.c_int, C.c_int64, _ip]
+_lib.amb_write_uf_chans.argtypes = [C.c_char_p, C.c_int, _ip]
+_lib.amb_write_closed_rings.argtypes = [C.c_char_p, _dp, C.c_int]
+_lib.amb_read_rings.argtypes = [C.c_char_p, _ip, C.POINTER(_dp)]
+_lib.amb_write_bsout_v1.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                    C.c_double, _fp, _ip]
+_lib.amb_write_bsout_v2.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, _fp, _fp]
+
+
+class AmberError(RuntimeError):
+    def __init__(self, code, msg=""):
+        self.code = code
+        super().__init__("%s%s" % (_lib.amb_strerror(code).decode(), (": " + msg) if msg else ""))
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+def _chk(rc, ctx=None):
+    if rc != 0:
+        msg = _lib.amb_last_error(ctx).decode() if ctx else ""
+        raise AmberError(rc, msg)
+
+
+def version():
+    return _lib.amb_version().decode()
+
+
+# ---------------------------------------------------------------- rules
+def ng(uhigh, du=1.0):
+    """Reduced grid side (amber.c:193-197)."""
+    return int(_lib.amb_ng(float(uhigh), float(du)))
+
+
+def annular_rings(nr=22):
+    """The reference's hard-coded ring edges (amber_funcs.c:2022-2053)."""
+    lo = np.zeros(nr); hi = np.zeros(nr)
+    _lib.amb_annular_rings(nr, _p(lo, _dp), _p(hi, _dp))
+    return lo, hi
+
+
+def uavg(lo, hi):
+    lo = np.ascontiguousarray(lo, np.float64); hi = np.ascontiguousarray(hi, np.float64)
+    out = np.zeros(len(lo))
+    _lib.amb_uavg(_p(lo, _dp), _p(hi, _dp), len(lo), _p(out, _dp))
+    return out
+
+
+def triangles(lo, hi):
+    """Closed-triangle list r1>=r2>=r3 (amber_funcs.c:443-449)."""
+    u = uavg(lo, hi)
+    n = _lib.amb_triangles(_p(u, _dp), len(u), None, 0)
+    t = np.zeros((max(n, 1), 3), np.int32)
+    _lib.amb_triangles(_p(u, _dp), len(u), _p(t, _ip), n)
+    return t[:n]
+
+
+def log_rings(umin, umax, nbins):
+    """Log-spaced annular bins [e_i, e_{i+1}) in cells."""
+    e = np.logspace(np.log10(umin), np.log10(umax), nbins + 1)
+    return e[:-1].copy(), e[1:].copy()
+
+
+# ---------------------------------------------------------------- generator
+def generate(N, nc, nflag=0, seed=1, p0=0.4, u0=40.0, shared_mask=False, count_max=1):
+    """Synthetic cube in the file layout: gv [N*(N/2+1), nc, 2] f64,
+    samp [N*(N/2+1), nc] f64, and the unflagged channel ids."""
+    cells = N * (N // 2 + 1)
+    gv = np.empty((cells, nc, 2), np.float64)
+    samp = np.empty((cells, nc), np.float64)
+    uf = np.zeros(nc, np.int32)
+    nuf = C.c_int(0)
+    cfg = GenCfg(N, nc, nflag, int(bool(shared_mask)), count_max, seed, p0, u0)
+    _chk(_lib.amb_generate(C.byref(cfg), _p(gv, _dp), _p(samp, _dp), _p(uf, _ip), C.byref(nuf)))
+    return gv, samp, uf[: nuf.value].copy()
+
+
+# ---------------------------------------------------------------- files
+def write_cube_files(data_path, rlzn, N, nc, Du, flag_key, gv, samp):
+    _chk(_lib.amb_write_cube_files(data_path.encode(), rlzn, N, nc, Du, flag_key,
+                                   _p(np.ascontiguousarray(gv, np.float64), _dp),
+                                   _p(np.ascontiguousarray(samp, np.float64), _dp)))
+
+
+def read_cube_files(data_path, rlzn):
+    NN = C.c_int64(); nc = C.c_int64(); Du = C.c_double(); fk = C.c_int32()
+    g = _dp(); s = _dp()
+    _chk(_lib.amb_read_cube_files(data_path.encode(), rlzn, C.byref(NN), C.byref(nc), C.byref(Du),
+                                  C.byref(fk), C.byref(g), C.byref(s)))
+    cells = NN.value * (NN.value // 2 + 1)
+    gv = np.ctypeslib.as_array(g, shape=(cells * nc.value * 2,)).copy().reshape(cells, nc.value, 2)
+    samp = np.ctypeslib.as_array(s, shape=(cells * nc.value,)).copy().reshape(cells, nc.value)
+    _lib.amb_free(g); _lib.amb_free(s)
+    return NN.value, nc.value, Du.value, fk.value, gv, samp
+
+
+def read_params_ess(path):
+    a = C.c_double(); b = C.c_double(); c = C.c_double()
+    buf = C.create_string_buffer(512)
+    _chk(_lib.amb_read_params_ess(path.encode(), C.byref(a), C.byref(b), C.byref(c), buf, 512))
+    return a.value, b.value, c.value, buf.value.decode()
+
+
+def read_uf_chans(path, uf_nc, nc):
+    out = np.zeros(max(uf_nc, 1), np.int32)
+    _chk(_lib.amb_read_uf_chans(path.encode(), uf_nc, nc, _p(out, _ip)))
+    return out[:uf_nc]
+
+
+def write_uf_chans(path, chans):
+    chans = np.ascontiguousarray(chans, np.int32)
+    _chk(_lib.amb_write_uf_chans(path.encode(), len(chans), _p(chans, _ip)))
+
+
+def write_closed_rings(res_path, rdata):
+    rdata = np.ascontiguousarray(rdata, np.float64).reshape(-1, 4)
+    _chk(_lib.amb_write_closed_rings(res_path.encode(), _p(rdata, _dp), len(rdata)))
+
+
+def read_rings(res_path):
+    n = C.c_int(); d = _dp()
+    _chk(_lib.amb_read_rings(res_path.encode(), C.byref(n), C.byref(d)))
+    out = np.ctypeslib.as_array(d, shape=(max(n.value, 1) * 4,)).copy()[: n.value * 4].reshape(n.value, 4)
+    _lib.amb_free(d)
+    return n.value, out
+
+
+def write_bsout_v1(res_path, r1, r2, r3, rlzn, ntri, num, bl, cnt):
+    W = int(round(np.sqrt(bl.size)))
+    _chk(_lib.amb_write_bsout_v1(res_path.encode(), r1, r2, r3, rlzn, W, ntri, num,
+                                 _p(np.ascontiguousarray(bl, np.float32), _fp),
+                                 _p(np.ascontiguousarray(cnt, np.int32), _ip)))
+
+
+def write_bsout_v2(res_path, r1, r2, r3, rlzn, inv, bl, norm):
+    W = int(round(np.sqrt(bl.size)))
+    _chk(_lib.amb_write_bsout_v2(res_path.encode(), r1, r2, r3, rlzn, W, inv,
+                                 _p(np.ascontiguousarray(bl, np.float32), _fp),
+                                 _p(np.ascontiguousarray(norm, np.float32), _fp)))
+
+
+# ---------------------------------------------------------------- device context
+class Context:
+    """One device context (amb_create).  Owns resident cubes and cached
+    filtered planes; all calls run on the library's own HIP stream."""
+
+    def __init__(self, device=0, precision="bf16x3", hbm_budget=0, verbose=False):
+        self._h = C.c_void_p()
+        o = Opts(device, _PREC[precision], int(hbm_budget), int(bool(verbose)))
+        _chk(_lib.amb_create(C.byref(o), C.byref(self._h)))
+        self._meta = {}
+
+    def close(self):
+        if self._h:
+            _lib.amb_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _c(self, rc):
+        _chk(rc, self._h)
+
+    def set_precision(self, precision):
+        self._c(_lib.amb_set_precision(self._h, _PREC[precision]))
+
+    def load_cube(self, N, nc, Du, flag_key, gv, samp):
+        gv = np.ascontiguousarray(gv, np.float64)
+        samp = np.ascontiguousarray(samp, np.float64)
+        cid = C.c_int()
+        self._c(_lib.amb_load_cube(self._h, N, nc, Du, flag_key, _p(gv, _dp), _p(samp, _dp), C.byref(cid)))
+        self._meta[cid.value] = dict(N=N, nc=nc, Du=Du, flag_key=flag_key)
+        return cid.value
+
+    def load_cube_files(self, data_path, rlzn):
+        cid = C.c_int()
+        self._c(_lib.amb_load_cube_files(self._h, data_path.encode(), rlzn, C.byref(cid)))
+        return cid.value
+
+    def release_cube(self, cube):
+        self._c(_lib.amb_release_cube(self._h, cube))
+        self._meta.pop(cube, None)
+
+    def set_channels(self, cube, uf_chans, nc10):
+        u = np.ascontiguousarray(uf_chans if uf_chans is not None else np.zeros(0), np.int32)
+        self._c(_lib.amb_set_channels(self._h, cube, _p(u, _ip) if u.size else None, int(u.size), nc10))
+        self._meta.setdefault(cube, {})["nc10"] = nc10
+
+    def set_rings(self, lo, hi):
+        self._lo = np.ascontiguousarray(lo, np.float64)
+        self._hi = np.ascontiguousarray(hi, np.float64)
+        self._c(_lib.amb_set_rings(self._h, _p(self._lo, _dp), _p(self._hi, _dp), len(self._lo)))
+
+    def closed_rings(self, cube, plane=5):
+        n = len(triangles(self._lo, self._hi))
+        rd = np.zeros((max(n, 1), 4))
+        cnt = C.c_int()
+        self._c(_lib.amb_closed_rings(self._h, cube, plane, _p(rd, _dp), n, C.byref(cnt)))
+        return rd[: cnt.value]
+
+    def _W(self, cube):
+        return 2 * self._meta[cube]["nc10"] + 1
+
+    def bispec_v1(self, cube, r1, r2, r3, rdata_num):
+        W = self._W(cube)
+        bl = np.zeros(W * W, np.float32)
+        cnt = np.zeros(W * W, np.int32)
+        ntri = C.c_double(); num = C.c_double()
+        self._c(_lib.amb_bispec_v1(self._h, cube, r1, r2, r3, rdata_num, _p(bl, _fp), _p(cnt, _ip),
+                                   C.byref(ntri), C.byref(num)))
+        return ntri.value, num.value, bl.reshape(W, W), cnt.reshape(W, W)
+
+    def bispec_v2(self, cube, r1, r2, r3):
+        W = self._W(cube)
+        bl = np.zeros(W * W, np.float32)
+        norm = np.zeros(W * W, np.float32)
+        inv = C.c_double()
+        self._c(_lib.amb_bispec_v2(self._h, cube, r1, r2, r3, _p(bl, _fp), _p(norm, _fp), C.byref(inv)))
+        return inv.value, bl.reshape(W, W), norm.reshape(W, W)
+
+    def bispec_ex(self, cube, r1, r2, r3, want_norm=False):
+        W = self._W(cube)
+        out = dict(bl=np.zeros(W * W, np.float32), bl64=np.zeros(W * W), cnt=np.zeros(W * W, np.int32))
+        if want_norm:
+            out["norm"] = np.zeros(W * W, np.float32)
+            out["norm64"] = np.zeros(W * W)
+        r = Result(_p(out["bl"], _fp), _p(out["bl64"], _dp), _p(out["cnt"], _ip),
+                   _p(out.get("norm"), _fp), _p(out.get("norm64"), _dp), 0.0, 0, 0.0)
+        self._c(_lib.amb_bispec_ex(self._h, cube, r1, r2, r3, int(bool(want_norm)), C.byref(r)))
+        res = {k: v.reshape(W, W) for k, v in out.items()}
+        res.update(inv_ng2=r.inv_ng2, ng=r.ng, cert_margin=r.cert_margin)
+        return res
+
+    def bispec_batch(self, cube, tris, want_cnt=True, want_norm=False):
+        W = self._W(cube)
+        tris = np.ascontiguousarray(tris, np.int32).reshape(-1, 3)
+        n = len(tris)
+        bl = np.zeros((n, W, W), np.float32)
+        cnt = np.zeros((n, W, W), np.int32) if want_cnt else None
+        norm = np.zeros((n, W, W), np.float32) if want_norm else None
+        self._c(_lib.amb_bispec_batch(self._h, cube, _p(tris, _ip), n, _p(bl, _fp), _p(cnt, _ip), _p(norm, _fp)))
+        return bl, cnt, norm
+
+    def stats(self):
+        s = Stats()
+        self._c(_lib.amb_get_stats(self._h, C.byref(s)))
+        return s.as_dict()
+
+    def reset_stats(self):
+        self._c(_lib.amb_reset_stats(self._h))

@@ -1,0 +1,825 @@
+/*
+ * C-ABI implementation: device context, resident cubes, the filtered-plane
+ * cache and the per-triangle orchestration that replaces the rp loop of
+ * amber.c:187-326 / amber_v2.c:187-333.
+ *
+ * HBM layout (one cube, nc channels, N grid, Nh = N/2+1):
+ *   vis_t [nc][Nh][N] complex64 and samp_t [nc][Nh][N] float -- plane-major,
+ *     column-contiguous so the c2r column pass gathers a ring column with
+ *     unit stride;
+ *   filtered planes [nact][Ng*Ng] float per (cube, kind, ring edges, Ng),
+ *     nact = channels of the triplet set (flagged planes are never built:
+ *     they never enter a triplet, amber_funcs.c:2207-2259);
+ *   C [nact][nact][nact] float: per-p1 contraction results, reduced over p1
+ *     into the [W][W] lag arrays of the result files.
+ */
+#include "amb_internal.h"
+#include "amber_mi355x.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace {
+
+struct Cube {
+    int64_t N = 0, nc = 0;
+    double Du = 1.0;
+    int flag_key = 0;
+    float2 *vis_t = nullptr;
+    float *samp_t = nullptr;
+    size_t bytes = 0;
+    bool chans = false;
+    int nc10 = 0;
+    std::vector<int> act, inv; /* compact <-> channel */
+    int *d_act = nullptr;      /* [nact] */
+    int *d_inv = nullptr;      /* inv[nc] then first_ge[nc+1] */
+    int *d_wlo = nullptr, *d_whi = nullptr;
+    int nact() const { return (int)act.size(); }
+};
+
+struct PlaneSet {
+    int cube, kind, ng;
+    double lo, hi;
+    float *d = nullptr;
+    size_t bytes = 0;
+    uint64_t last = 0;
+    int pins = 0;
+};
+
+struct Timer {
+    hipEvent_t a = nullptr, b = nullptr;
+    double *acc = nullptr;
+};
+
+} // namespace
+
+struct amb_ctx {
+    int device = 0, prec = AMB_PREC_BF16X3, verbose = 0;
+    uint64_t budget = 0, used = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    std::string err;
+    std::vector<double> ulow, uhigh, uavg;
+    std::map<int, Cube> cubes;
+    int next_cube = 1;
+    std::vector<PlaneSet> planes;
+    uint64_t tick = 0;
+    std::map<int, float2 *> tw;
+    std::map<int, FftPlan> plans;
+    /* scratch */
+    float2 *G = nullptr; size_t G_bytes = 0;
+    float *C = nullptr; size_t C_bytes = 0;
+    int *Cq = nullptr; size_t Cq_bytes = 0;
+    unsigned *margin = nullptr;
+    void *out = nullptr; size_t out_bytes = 0;
+    /* stats */
+    amb_stats st{};
+    std::vector<Timer> timers;
+    std::vector<hipEvent_t> evpool;
+};
+
+/* ------------------------------------------------------------------ helpers */
+static int fail(amb_ctx *c, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    c->err = buf;
+    if (c->verbose) fprintf(stderr, "amber_mi355x: %s\n", buf);
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                   \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(ctx, AMB_EHIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                                \
+    } while (0)
+
+static hipEvent_t ev_get(amb_ctx *c)
+{
+    if (!c->evpool.empty()) {
+        hipEvent_t e = c->evpool.back();
+        c->evpool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+static void timer_start(amb_ctx *c, double *acc)
+{
+    Timer t;
+    t.a = ev_get(c);
+    t.b = nullptr;
+    t.acc = acc;
+    if (t.a) hipEventRecord(t.a, c->stream);
+    c->timers.push_back(t);
+}
+static void timer_stop(amb_ctx *c)
+{
+    for (auto it = c->timers.rbegin(); it != c->timers.rend(); ++it)
+        if (!it->b) {
+            it->b = ev_get(c);
+            if (it->b) hipEventRecord(it->b, c->stream);
+            return;
+        }
+}
+/* after a stream sync: fold recorded intervals into the stats */
+static void timers_collect(amb_ctx *c)
+{
+    for (auto &t : c->timers) {
+        if (t.a && t.b) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess && t.acc) *t.acc += ms;
+        }
+        if (t.a) c->evpool.push_back(t.a);
+        if (t.b) c->evpool.push_back(t.b);
+    }
+    c->timers.clear();
+}
+
+static int dev_alloc(amb_ctx *c, void **p, size_t bytes, const char *what);
+static void dev_free(amb_ctx *c, void *p, size_t bytes)
+{
+    if (p) {
+        hipFree(p);
+        c->used -= bytes;
+    }
+}
+
+/* evict unpinned filtered planes, least recently used first */
+static bool evict_one(amb_ctx *c)
+{
+    int best = -1;
+    for (size_t i = 0; i < c->planes.size(); ++i)
+        if (c->planes[i].pins == 0 && (best < 0 || c->planes[i].last < c->planes[best].last)) best = (int)i;
+    if (best < 0) return false;
+    dev_free(c, c->planes[best].d, c->planes[best].bytes);
+    c->planes.erase(c->planes.begin() + best);
+    return true;
+}
+
+static int dev_alloc(amb_ctx *c, void **p, size_t bytes, const char *what)
+{
+    *p = nullptr;
+    if (bytes == 0) return AMB_OK;
+    while (c->used + bytes > c->budget)
+        if (!evict_one(c))
+            return fail(c, AMB_EBUDGET, "%s: %zu bytes exceed the HBM budget (%llu used of %llu)", what, bytes,
+                        (unsigned long long)c->used, (unsigned long long)c->budget);
+    hipError_t e = hipMalloc(p, bytes);
+    while (e != hipSuccess && evict_one(c)) {
+        (void)hipGetLastError();
+        e = hipMalloc(p, bytes);
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return fail(c, AMB_EBUDGET, "%s: hipMalloc(%zu) failed: %s", what, bytes, hipGetErrorString(e));
+    }
+    c->used += bytes;
+    return AMB_OK;
+}
+
+/* grow-only scratch buffer */
+template <class T>
+static int scratch(amb_ctx *c, T **p, size_t *have, size_t need, const char *what)
+{
+    if (*have >= need) return AMB_OK;
+    dev_free(c, *p, *have);
+    *p = nullptr;
+    *have = 0;
+    void *q = nullptr;
+    int rc = dev_alloc(c, &q, need, what);
+    if (rc) return rc;
+    *p = (T *)q;
+    *have = need;
+    return AMB_OK;
+}
+
+static int twiddles(amb_ctx *c, int n, float2 **out, FftPlan *pl)
+{
+    auto it = c->tw.find(n);
+    if (it != c->tw.end()) {
+        *out = it->second;
+        *pl = c->plans[n];
+        return AMB_OK;
+    }
+    std::vector<float2> h(n);
+    for (int m = 0; m < n; ++m) {
+        double a = 2.0 * M_PI * (double)m / (double)n;
+        h[m] = make_float2((float)cos(a), (float)sin(a));
+    }
+    void *d = nullptr;
+    int rc = dev_alloc(c, &d, sizeof(float2) * n, "twiddle table");
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpy(d, h.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
+    FftPlan p;
+    amb_make_plan(n, &p);
+    c->tw[n] = (float2 *)d;
+    c->plans[n] = p;
+    *out = (float2 *)d;
+    *pl = p;
+    return AMB_OK;
+}
+
+static int ring_geom(amb_ctx *c, const Cube &cb, int r, int ng, RingGeom *g)
+{
+    if (r < 0 || r >= (int)c->ulow.size()) return fail(c, AMB_EARG, "ring %d outside [0,%zu)", r, c->ulow.size());
+    g->lo = c->ulow[r];
+    g->hi = c->uhigh[r];
+    g->ng = ng;
+    if (!(g->hi > g->lo)) return fail(c, AMB_EARG, "ring %d has UHigh <= ULow", r);
+    /* cells of the ring must lie in the source half plane: |ia| < UHigh and
+       j < UHigh need UHigh <= N/2 (rows beyond N/2 would alias, and the
+       reference reads outside the cube there) */
+    if (g->hi > (double)(cb.N / 2))
+        return fail(c, AMB_EARG, "ring %d UHigh=%g exceeds N/2=%lld", r, g->hi, (long long)(cb.N / 2));
+    long jc = (long)ceil(g->hi);
+    if (jc > ng / 3) jc = ng / 3;
+    if (jc < 0) jc = 0;
+    g->jcnt = (int)jc;
+    return AMB_OK;
+}
+
+/* filtered planes of (cube, kind, ring) at grid side ng, pinned on return */
+static int get_planes(amb_ctx *c, int cube_id, Cube &cb, int kind, int r, int ng, float **out, int *slot)
+{
+    RingGeom g;
+    int rc = ring_geom(c, cb, r, ng, &g);
+    if (rc) return rc;
+    for (size_t i = 0; i < c->planes.size(); ++i) {
+        PlaneSet &p = c->planes[i];
+        if (p.cube == cube_id && p.kind == kind && p.ng == ng && p.lo == g.lo && p.hi == g.hi) {
+            p.last = ++c->tick;
+            p.pins++;
+            *out = p.d;
+            *slot = (int)i;
+            return AMB_OK;
+        }
+    }
+    const int nact = cb.nact();
+    const size_t P = (size_t)ng * ng;
+    float2 *tw = nullptr;
+    FftPlan pl;
+    rc = twiddles(c, ng, &tw, &pl);
+    if (rc) return rc;
+    size_t gneed = sizeof(float2) * (size_t)nact * (size_t)std::max(g.jcnt, 1) * ng;
+    rc = scratch(c, &c->G, &c->G_bytes, gneed, "fft scratch");
+    if (rc) return rc;
+    void *d = nullptr;
+    rc = dev_alloc(c, &d, sizeof(float) * P * nact, "filtered planes");
+    if (rc) return rc;
+    timer_start(c, &c->st.ms_fft);
+    hipError_t e = amb_launch_ring_c2r(c->stream, kind, cb.vis_t, cb.samp_t, cb.d_act, nact, (int)cb.N,
+                                       (int)(cb.N / 2 + 1), pl, tw, g, c->G, (float *)d);
+    timer_stop(c);
+    if (e != hipSuccess) {
+        dev_free(c, d, sizeof(float) * P * nact);
+        return fail(c, AMB_EHIP, "ring c2r launch: %s", hipGetErrorString(e));
+    }
+    c->st.n_fft_launch += 2;
+    /* algorithmic bytes: Ng^2 floats written per plane plus the ring cells read */
+    double cells = 0;
+    for (long ia = -(long)ceil(g.hi); ia <= (long)ceil(g.hi); ++ia)
+        for (long j = 0; j < g.jcnt; ++j) {
+            double uu = sqrt((double)(ia * ia + j * j));
+            if (uu >= g.lo && uu < g.hi && uu > 0) cells += 1;
+        }
+    c->st.bytes_fft += (double)nact * (4.0 * (double)P + (kind == 0 ? 8.0 : 4.0) * cells);
+    PlaneSet p;
+    p.cube = cube_id; p.kind = kind; p.ng = ng; p.lo = g.lo; p.hi = g.hi;
+    p.d = (float *)d; p.bytes = sizeof(float) * P * nact; p.last = ++c->tick; p.pins = 1;
+    c->planes.push_back(p);
+    *out = p.d;
+    *slot = (int)c->planes.size() - 1;
+    return AMB_OK;
+}
+
+static void unpin_all(amb_ctx *c)
+{
+    for (auto &p : c->planes) p.pins = 0;
+}
+
+static int find_cube(amb_ctx *c, int id, Cube **cb)
+{
+    auto it = c->cubes.find(id);
+    if (it == c->cubes.end()) return fail(c, AMB_ESTATE, "unknown cube id %d", id);
+    *cb = &it->second;
+    return AMB_OK;
+}
+
+/* ------------------------------------------------------------------ API */
+extern "C" {
+
+const char *amb_version(void) { return "amber_mi355x 0.1 (gfx950)"; }
+
+int amb_create(const amb_opts *o, amb_ctx **out)
+{
+    if (!out) return AMB_EARG;
+    *out = nullptr;
+    amb_ctx *c = new (std::nothrow) amb_ctx();
+    if (!c) return AMB_ENOMEM;
+    if (o) {
+        c->device = o->device;
+        c->prec = o->precision;
+        c->verbose = o->verbose;
+    }
+    if (c->prec < AMB_PREC_FP32 || c->prec > AMB_PREC_FP8) { delete c; return AMB_EARG; }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= c->device || c->device < 0) {
+        (void)hipGetLastError();
+        delete c;
+        return AMB_EHIP;
+    }
+    if (hipSetDevice(c->device) != hipSuccess) { delete c; return AMB_EHIP; }
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) { delete c; return AMB_EHIP; }
+    c->budget = (o && o->hbm_budget) ? o->hbm_budget : (uint64_t)(0.9 * (double)fr);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return AMB_EHIP; }
+    void *m = nullptr;
+    if (hipMalloc(&m, 256) != hipSuccess) { hipStreamDestroy(c->stream); delete c; return AMB_EHIP; }
+    c->margin = (unsigned *)m;
+    c->used = 256;
+    *out = c;
+    return AMB_OK;
+}
+
+void amb_destroy(amb_ctx *c)
+{
+    if (!c) return;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    timers_collect(c);
+    for (auto &p : c->planes) hipFree(p.d);
+    for (auto &kv : c->cubes) {
+        Cube &cb = kv.second;
+        hipFree(cb.vis_t); hipFree(cb.samp_t); hipFree(cb.d_act); hipFree(cb.d_inv);
+        hipFree(cb.d_wlo); hipFree(cb.d_whi);
+    }
+    for (auto &kv : c->tw) hipFree(kv.second);
+    hipFree(c->G); hipFree(c->C); hipFree(c->Cq); hipFree(c->margin); hipFree(c->out);
+    for (auto e : c->evpool) hipEventDestroy(e);
+    hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char *amb_last_error(amb_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int amb_set_precision(amb_ctx *c, int prec)
+{
+    if (!c) return AMB_EARG;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (prec < AMB_PREC_FP32 || prec > AMB_PREC_FP8) return fail(c, AMB_EARG, "precision %d", prec);
+    if (prec == AMB_PREC_FP8) return fail(c, AMB_EARG, "FP8 contraction is not available in this build");
+    c->prec = prec;
+    return AMB_OK;
+}
+
+int amb_load_cube(amb_ctx *c, int64_t N, int64_t nc, double Du, int32_t flag_key, const double *gv,
+                  const double *samp, int *cube_id)
+{
+    if (!c || !gv || !samp || !cube_id) return AMB_EARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    hipSetDevice(c->device);
+    if (N < 4 || N % 2 || nc < 1 || !(Du > 0)) return fail(c, AMB_EARG, "bad cube header N=%lld nc=%lld", (long long)N, (long long)nc);
+    const int64_t Nh = N / 2 + 1;
+    const size_t nv = (size_t)N * Nh * nc;
+    Cube cb;
+    cb.N = N; cb.nc = nc; cb.Du = Du; cb.flag_key = flag_key;
+    void *v = nullptr, *s = nullptr;
+    int rc = dev_alloc(c, &v, sizeof(float2) * nv, "cube");
+    if (rc) return rc;
+    rc = dev_alloc(c, &s, sizeof(float) * nv, "sampling cube");
+    if (rc) { dev_free(c, v, sizeof(float2) * nv); return rc; }
+    cb.vis_t = (float2 *)v;
+    cb.samp_t = (float *)s;
+    cb.bytes = sizeof(float2) * nv + sizeof(float) * nv;
+    /* stage rows of cells through a device buffer of <= ~256 MiB */
+    const size_t row_doubles = (size_t)Nh * nc * 3;
+    int rows = (int)std::max<size_t>(1, (256u << 20) / (row_doubles * 8));
+    if (rows > N) rows = (int)N;
+    void *stg = nullptr;
+    rc = dev_alloc(c, &stg, row_doubles * 8 * rows, "upload staging");
+    if (rc) { dev_free(c, v, sizeof(float2) * nv); dev_free(c, s, sizeof(float) * nv); return rc; }
+    double *dg = (double *)stg, *ds = dg + (size_t)rows * Nh * nc * 2;
+    for (int64_t i0 = 0; i0 < N; i0 += rows) {
+        int r = (int)std::min<int64_t>(rows, N - i0);
+        size_t n1 = (size_t)r * Nh * nc;
+        hipError_t e = hipMemcpyAsync(dg, gv + (size_t)i0 * Nh * nc * 2, n1 * 16, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(ds, samp + (size_t)i0 * Nh * nc, n1 * 8, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = amb_launch_upload(c->stream, dg, ds, (int)i0, r, (int)N, (int)Nh, (int)nc, cb.vis_t, cb.samp_t);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) {
+            dev_free(c, stg, row_doubles * 8 * rows);
+            dev_free(c, v, sizeof(float2) * nv);
+            dev_free(c, s, sizeof(float) * nv);
+            return fail(c, AMB_EHIP, "cube upload: %s", hipGetErrorString(e));
+        }
+    }
+    dev_free(c, stg, row_doubles * 8 * rows);
+    int id = c->next_cube++;
+    c->cubes[id] = cb;
+    *cube_id = id;
+    return AMB_OK;
+}
+
+int amb_load_cube_files(amb_ctx *c, const char *data_path, int rlzn, int *cube_id)
+{
+    if (!c || !data_path) return AMB_EARG;
+    int64_t NN, nc;
+    double Du;
+    int32_t fk;
+    double *gv = nullptr, *samp = nullptr;
+    int rc = amb_read_cube_files(data_path, rlzn, &NN, &nc, &Du, &fk, &gv, &samp);
+    if (rc) {
+        std::lock_guard<std::mutex> lk(c->mu);
+        return fail(c, rc, "reading %s/GV_data_%d and sampling: %s", data_path, rlzn, amb_strerror(rc));
+    }
+    rc = amb_load_cube(c, NN, nc, Du, fk, gv, samp, cube_id);
+    amb_free(gv);
+    amb_free(samp);
+    return rc;
+}
+
+int amb_release_cube(amb_ctx *c, int id)
+{
+    if (!c) return AMB_EARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    Cube *cb;
+    int rc = find_cube(c, id, &cb);
+    if (rc) return rc;
+    hipStreamSynchronize(c->stream);
+    for (size_t i = 0; i < c->planes.size();)
+        if (c->planes[i].cube == id) {
+            dev_free(c, c->planes[i].d, c->planes[i].bytes);
+            c->planes.erase(c->planes.begin() + i);
+        } else ++i;
+    const size_t nv = (size_t)cb->N * (cb->N / 2 + 1) * cb->nc;
+    dev_free(c, cb->vis_t, sizeof(float2) * nv);
+    dev_free(c, cb->samp_t, sizeof(float) * nv);
+    const size_t na = cb->act.size(), ncs = (size_t)cb->nc;
+    dev_free(c, cb->d_act, sizeof(int) * na);
+    dev_free(c, cb->d_inv, sizeof(int) * (2 * ncs + 1));
+    dev_free(c, cb->d_wlo, sizeof(int) * na);
+    dev_free(c, cb->d_whi, sizeof(int) * na);
+    c->cubes.erase(id);
+    return AMB_OK;
+}
+
+int amb_set_channels(amb_ctx *c, int id, const int *uf, int uf_nc, int nc10)
+{
+    if (!c) return AMB_EARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    hipSetDevice(c->device);
+    Cube *cb;
+    int rc = find_cube(c, id, &cb);
+    if (rc) return rc;
+    const int nc = (int)cb->nc;
+    if (nc10 < 0 || nc10 > nc - 1) return fail(c, AMB_EARG, "nc10=%d outside [0,%d]", nc10, nc - 1);
+    std::vector<char> on(nc, 0);
+    if (cb->flag_key == 1) {
+        if (!uf || uf_nc <= 0) return fail(c, AMB_EARG, "flag_key=1 needs uf_chans");
+        for (int i = 0; i < uf_nc; ++i) {
+            if (uf[i] < 0 || uf[i] >= nc) return fail(c, AMB_EARG, "uf_chans[%d]=%d outside [0,%d)", i, uf[i], nc);
+            on[uf[i]] = 1;
+        }
+    } else {
+        std::fill(on.begin(), on.end(), 1);
+    }
+    hipStreamSynchronize(c->stream);
+    /* planes are built per channel set: drop this cube's cached planes */
+    for (size_t i = 0; i < c->planes.size();)
+        if (c->planes[i].cube == id) {
+            dev_free(c, c->planes[i].d, c->planes[i].bytes);
+            c->planes.erase(c->planes.begin() + i);
+        } else ++i;
+    size_t na_old = cb->act.size();
+    dev_free(c, cb->d_act, sizeof(int) * na_old);
+    dev_free(c, cb->d_inv, sizeof(int) * (2 * (size_t)nc + 1));
+    dev_free(c, cb->d_wlo, sizeof(int) * na_old);
+    dev_free(c, cb->d_whi, sizeof(int) * na_old);
+    cb->d_act = cb->d_inv = cb->d_wlo = cb->d_whi = nullptr;
+    cb->act.clear();
+    cb->inv.assign(nc, -1);
+    for (int p = 0; p < nc; ++p)
+        if (on[p]) { cb->inv[p] = (int)cb->act.size(); cb->act.push_back(p); }
+    const int na = (int)cb->act.size();
+    std::vector<int> invfg(2 * (size_t)nc + 1);
+    for (int p = 0; p < nc; ++p) invfg[p] = cb->inv[p];
+    /* first_ge[p] = first compact index whose channel is >= p */
+    for (int p = nc, k = na; p >= 0; --p) {
+        while (k > 0 && cb->act[k - 1] >= p) --k;
+        invfg[nc + p] = k;
+    }
+    std::vector<int> wlo(na), whi(na);
+    for (int i = 0, l = 0, h = 0; i < na; ++i) {
+        int p1 = cb->act[i];
+        while (l < na && cb->act[l] < p1 - nc10) ++l;
+        while (h < na && cb->act[h] <= p1 + nc10) ++h;
+        wlo[i] = l;
+        whi[i] = h;
+    }
+    void *p;
+    if ((rc = dev_alloc(c, &p, sizeof(int) * na, "channel table"))) return rc;
+    cb->d_act = (int *)p;
+    if ((rc = dev_alloc(c, &p, sizeof(int) * invfg.size(), "channel table"))) return rc;
+    cb->d_inv = (int *)p;
+    if ((rc = dev_alloc(c, &p, sizeof(int) * na, "window table"))) return rc;
+    cb->d_wlo = (int *)p;
+    if ((rc = dev_alloc(c, &p, sizeof(int) * na, "window table"))) return rc;
+    cb->d_whi = (int *)p;
+    HIPCHK(c, hipMemcpy(cb->d_act, cb->act.data(), sizeof(int) * na, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(cb->d_inv, invfg.data(), sizeof(int) * invfg.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(cb->d_wlo, wlo.data(), sizeof(int) * na, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(cb->d_whi, whi.data(), sizeof(int) * na, hipMemcpyHostToDevice));
+    cb->nc10 = nc10;
+    cb->chans = true;
+    return AMB_OK;
+}
+
+int amb_set_rings(amb_ctx *c, const double *lo, const double *hi, int nr)
+{
+    if (!c || !lo || !hi || nr <= 0) return AMB_EARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    for (int i = 0; i < nr; ++i)
+        if (!(hi[i] > lo[i]) || !(lo[i] >= 0)) return fail(c, AMB_EARG, "ring %d edges [%g,%g)", i, lo[i], hi[i]);
+    c->ulow.assign(lo, lo + nr);
+    c->uhigh.assign(hi, hi + nr);
+    c->uavg.resize(nr);
+    amb_uavg(lo, hi, nr, c->uavg.data());
+    return AMB_OK;
+}
+
+int amb_closed_rings(amb_ctx *c, int id, int plane, double *rdata, int cap, int *rcount)
+{
+    if (!c || !rcount) return AMB_EARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    hipSetDevice(c->device);
+    Cube *cb;
+    int rc = find_cube(c, id, &cb);
+    if (rc) return rc;
+    const int nr = (int)c->ulow.size();
+    if (nr == 0) return fail(c, AMB_ESTATE, "rings not set");
+    if (plane < 0 || plane >= cb->nc) return fail(c, AMB_EARG, "sampling plane %d outside [0,%lld)", plane, (long long)cb->nc);
+    const int N = (int)cb->N;
+    const size_t P = (size_t)N * N;
+    int ntri = amb_triangles(c->uavg.data(), nr, nullptr, 0);
+    std::vector<int> tri(3 * (size_t)std::max(ntri, 1));
+    amb_triangles(c->uavg.data(), nr, tri.data(), ntri);
+    *rcount = ntri;
+    if (ntri == 0) return AMB_OK;
+    float2 *tw;
+    FftPlan pl;
+    if ((rc = twiddles(c, N, &tw, &pl))) return rc;
+    int jmax = 0;
+    std::vector<RingGeom> geo(nr);
+    for (int r = 0; r < nr; ++r) {
+        if ((rc = ring_geom(c, *cb, r, N, &geo[r]))) return rc;
+        jmax = std::max(jmax, geo[r].jcnt);
+    }
+    if ((rc = scratch(c, &c->G, &c->G_bytes, sizeof(float2) * (size_t)std::max(jmax, 1) * N, "fft scratch"))) return rc;
+    void *II = nullptr, *dp = nullptr, *dtri = nullptr, *dnum = nullptr;
+    if ((rc = dev_alloc(c, &II, sizeof(float) * P * nr, "calCR images"))) return rc;
+    if ((rc = dev_alloc(c, &dp, sizeof(int), "plane id")) ||
+        (rc = dev_alloc(c, &dtri, sizeof(int) * 3 * ntri, "triangle list")) ||
+        (rc = dev_alloc(c, &dnum, sizeof(double) * ntri, "calCR sums"))) {
+        dev_free(c, II, sizeof(float) * P * nr);
+        dev_free(c, dp, sizeof(int));
+        dev_free(c, dtri, sizeof(int) * 3 * ntri);
+        return rc;
+    }
+    auto cleanup = [&]() {
+        dev_free(c, II, sizeof(float) * P * nr);
+        dev_free(c, dp, sizeof(int));
+        dev_free(c, dtri, sizeof(int) * 3 * ntri);
+        dev_free(c, dnum, sizeof(double) * ntri);
+    };
+    hipError_t e = hipMemcpyAsync(dp, &plane, sizeof(int), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dtri, tri.data(), sizeof(int) * 3 * ntri, hipMemcpyHostToDevice, c->stream);
+    timer_start(c, &c->st.ms_fft);
+    /* calCR's gather (amber_funcs.c:419-429) is the Dfg gather at Ng = N */
+    for (int r = 0; r < nr && e == hipSuccess; ++r)
+        e = amb_launch_ring_c2r(c->stream, 1, cb->vis_t, cb->samp_t, (const int *)dp, 1, N, N / 2 + 1, pl, tw,
+                                geo[r], c->G, (float *)II + (size_t)r * P);
+    timer_stop(c);
+    if (e == hipSuccess) e = amb_launch_triple_sum(c->stream, (const float *)II, (long)P, (const int *)dtri, ntri, (double *)dnum);
+    std::vector<double> num(ntri);
+    if (e == hipSuccess) e = hipMemcpyAsync(num.data(), dnum, sizeof(double) * ntri, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    timers_collect(c);
+    cleanup();
+    if (e != hipSuccess) return fail(c, AMB_EHIP, "calCR: %s", hipGetErrorString(e));
+    /* Num = N^2 * (count of closed sampled triangles), a multiple of N^2/8 */
+    const double NN2 = (double)N * (double)N;
+    for (int t = 0; t < ntri; ++t) {
+        double q = num[t] / NN2;
+        double q8 = std::rint(q * 8.0);
+        if (std::fabs(q - q8 * 0.125) > 1.0 / 32.0)
+            return fail(c, AMB_ECERT, "calCR triangle %d: Num/N^2=%.6f is not within 1/32 of a multiple of 1/8", t, q);
+        if (rdata && t < cap) {
+            rdata[4 * t] = tri[3 * t];
+            rdata[4 * t + 1] = tri[3 * t + 1];
+            rdata[4 * t + 2] = tri[3 * t + 2];
+            rdata[4 * t + 3] = q8 * 0.125 * NN2;
+        }
+    }
+    return AMB_OK;
+}
+
+/* core of every bispectrum entry point; output pointers are host memory */
+static int bispec_core(amb_ctx *c, int id, int r1, int r2, int r3, int want_norm, amb_result *res)
+{
+    hipSetDevice(c->device);
+    Cube *cb;
+    int rc = find_cube(c, id, &cb);
+    if (rc) return rc;
+    if (!cb->chans) return fail(c, AMB_ESTATE, "channels not set for cube %d", id);
+    const int nr = (int)c->ulow.size();
+    if (nr == 0) return fail(c, AMB_ESTATE, "rings not set");
+    if (r1 < 0 || r1 >= nr || r2 < 0 || r2 >= nr || r3 < 0 || r3 >= nr)
+        return fail(c, AMB_EARG, "triangle (%d,%d,%d) outside [0,%d)", r1, r2, r3, nr);
+    const long ng = amb_ng(c->uhigh[r1], cb->Du);
+    if (ng <= 0 || ng > 4096) return fail(c, AMB_EARG, "Ng=%ld out of range", ng);
+    const long P = ng * ng;
+    const int nact = cb->nact();
+    const int W = 2 * cb->nc10 + 1;
+    const size_t cells = (size_t)W * W;
+    res->ng = ng;
+    res->inv_ng2 = 1.0 / (ng * ng * 1.);
+    res->cert_margin = 0;
+
+    int prec = c->prec;
+    float *D[3], *S[3] = {nullptr, nullptr, nullptr};
+    int slot;
+    int rr[3] = {r1, r2, r3};
+    for (int k = 0; k < 3; ++k)
+        if ((rc = get_planes(c, id, *cb, 0, rr[k], (int)ng, &D[k], &slot))) { unpin_all(c); return rc; }
+    if (want_norm)
+        for (int k = 0; k < 3; ++k)
+            if ((rc = get_planes(c, id, *cb, 1, rr[k], (int)ng, &S[k], &slot))) { unpin_all(c); return rc; }
+
+    const size_t nC = (size_t)nact * nact * nact;
+    if ((rc = scratch(c, &c->C, &c->C_bytes, sizeof(float) * nC, "contraction partials"))) { unpin_all(c); return rc; }
+    if (want_norm && (rc = scratch(c, &c->Cq, &c->Cq_bytes, sizeof(int) * nC, "normalisation partials"))) { unpin_all(c); return rc; }
+    const size_t outb = cells * (sizeof(float) * 2 + sizeof(double) * 2 + sizeof(int));
+    if ((rc = scratch(c, (char **)&c->out, &c->out_bytes, outb, "lag arrays"))) { unpin_all(c); return rc; }
+    float *d_bl = (float *)c->out;
+    float *d_norm = d_bl + cells;
+    double *d_bl64 = (double *)(d_norm + cells + (cells & 1));
+    double *d_norm64 = d_bl64 + cells;
+    int *d_cnt = (int *)(d_norm64 + cells);
+
+    ContractArgs a{};
+    a.P = P; a.nact = nact; a.wlo = cb->d_wlo; a.whi = cb->d_whi;
+    a.D1 = D[0]; a.D2 = D[1]; a.D3 = D[2]; a.C = c->C;
+    hipError_t e = hipSuccess;
+    timer_start(c, &c->st.ms_contract);
+    e = amb_launch_contract(c->stream, prec, 0, a);
+    timer_stop(c);
+    c->st.n_contract_launch++;
+    if (e == hipSuccess && want_norm) {
+        ContractArgs b = a;
+        b.D1 = S[0]; b.D2 = S[1]; b.D3 = S[2]; b.C = nullptr; b.Cq = c->Cq; b.margin = c->margin;
+        e = hipMemsetAsync(c->margin, 0, sizeof(unsigned), c->stream);
+        timer_start(c, &c->st.ms_norm);
+        if (e == hipSuccess) e = amb_launch_contract(c->stream, prec == AMB_PREC_FP32 ? AMB_PREC_FP32 : AMB_PREC_BF16X3, 1, b);
+        timer_stop(c);
+    }
+    ReduceArgs ra{};
+    ra.C = c->C; ra.Cq = want_norm ? c->Cq : nullptr; ra.act = cb->d_act; ra.inv = cb->d_inv;
+    ra.nc = (int)cb->nc; ra.nact = nact; ra.nc10 = cb->nc10; ra.P = P;
+    ra.bl = d_bl; ra.bl64 = d_bl64; ra.cnt = d_cnt;
+    ra.norm = want_norm ? d_norm : nullptr; ra.norm64 = want_norm ? d_norm64 : nullptr;
+    timer_start(c, &c->st.ms_reduce);
+    if (e == hipSuccess) e = amb_launch_reduce(c->stream, ra);
+    timer_stop(c);
+    std::vector<int> hcnt;
+    int32_t *cnt_dst = res->cnt;
+    if (!cnt_dst) { hcnt.resize(cells); cnt_dst = hcnt.data(); }
+    if (e == hipSuccess && res->bl) e = hipMemcpyAsync(res->bl, d_bl, sizeof(float) * cells, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && res->bl64) e = hipMemcpyAsync(res->bl64, d_bl64, sizeof(double) * cells, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(cnt_dst, d_cnt, sizeof(int) * cells, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && want_norm && res->norm) e = hipMemcpyAsync(res->norm, d_norm, sizeof(float) * cells, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && want_norm && res->norm64) e = hipMemcpyAsync(res->norm64, d_norm64, sizeof(double) * cells, hipMemcpyDeviceToHost, c->stream);
+    unsigned mbits = 0;
+    if (e == hipSuccess && want_norm) e = hipMemcpyAsync(&mbits, c->margin, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    timers_collect(c);
+    unpin_all(c);
+    if (e != hipSuccess) return fail(c, AMB_EHIP, "bispectrum (%d,%d,%d): %s", r1, r2, r3, hipGetErrorString(e));
+    int64_t ev = 0;
+    double pairs = 0;
+    for (size_t i = 0; i < cells; ++i) {
+        ev += cnt_dst[i] > 0;
+        pairs += cnt_dst[i];
+    }
+    c->st.evals += ev;
+    c->st.flop_contract += 2.0 * pairs * (double)P;
+    if (want_norm) {
+        float m;
+        memcpy(&m, &mbits, sizeof m);
+        res->cert_margin = m;
+        if (m > 1.0f / 32.0f)
+            return fail(c, AMB_ECERT, "normalisation sums off their 1/8 grid by %g", (double)m);
+    }
+    return AMB_OK;
+}
+
+int amb_bispec_ex(amb_ctx *c, int id, int r1, int r2, int r3, int want_norm, amb_result *res)
+{
+    if (!c || !res) return AMB_EARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    return bispec_core(c, id, r1, r2, r3, want_norm, res);
+}
+
+int amb_bispec_v1(amb_ctx *c, int id, int r1, int r2, int r3, double rdata_num, float *bl, int32_t *cnt,
+                  double *ntri, double *num)
+{
+    if (!c || !bl || !cnt) return AMB_EARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    amb_result r{};
+    r.bl = bl;
+    r.cnt = cnt;
+    int rc = bispec_core(c, id, r1, r2, r3, 0, &r);
+    if (rc) return rc;
+    /* amber.c:198-201,287 with the same double expression order */
+    const Cube &cb = c->cubes[id];
+    long Ng1 = r.ng, Ng2 = r.ng;
+    double gfact = (1. * Ng1) / (1. * cb.N);
+    double gfact_sq = gfact * gfact;
+    double Num = gfact_sq * rdata_num;
+    double inv = 1.0 / (Ng1 * Ng2 * 1.);
+    if (num) *num = Num;
+    if (ntri) *ntri = Num * inv;
+    return AMB_OK;
+}
+
+int amb_bispec_v2(amb_ctx *c, int id, int r1, int r2, int r3, float *bl, float *norm, double *inv_ng2)
+{
+    if (!c || !bl || !norm) return AMB_EARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    amb_result r{};
+    r.bl = bl;
+    r.norm = norm;
+    int rc = bispec_core(c, id, r1, r2, r3, 1, &r);
+    if (rc) return rc;
+    if (inv_ng2) *inv_ng2 = r.inv_ng2;
+    return AMB_OK;
+}
+
+int amb_bispec_batch(amb_ctx *c, int id, const int *tri, int ntri, float *bl, int32_t *cnt, float *norm)
+{
+    if (!c || !tri || ntri < 0 || !bl) return AMB_EARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    Cube *cb;
+    int rc = find_cube(c, id, &cb);
+    if (rc) return rc;
+    if (!cb->chans) return fail(c, AMB_ESTATE, "channels not set for cube %d", id);
+    const size_t cells = (size_t)(2 * cb->nc10 + 1) * (2 * cb->nc10 + 1);
+    /* visit triangles grouped by r1 (one Ng per group) so planes are reused */
+    std::vector<int> order(ntri);
+    for (int i = 0; i < ntri; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return tri[3 * x] < tri[3 * y]; });
+    for (int k = 0; k < ntri; ++k) {
+        int i = order[k];
+        amb_result r{};
+        r.bl = bl + (size_t)i * cells;
+        r.cnt = cnt ? cnt + (size_t)i * cells : nullptr;
+        r.norm = norm ? norm + (size_t)i * cells : nullptr;
+        rc = bispec_core(c, id, tri[3 * i], tri[3 * i + 1], tri[3 * i + 2], norm != nullptr, &r);
+        if (rc) return rc;
+    }
+    return AMB_OK;
+}
+
+int amb_get_stats(amb_ctx *c, amb_stats *st)
+{
+    if (!c || !st) return AMB_EARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    *st = c->st;
+    st->ms_total = st->ms_fft + st->ms_contract + st->ms_reduce + st->ms_norm;
+    return AMB_OK;
+}
+
+int amb_reset_stats(amb_ctx *c)
+{
+    if (!c) return AMB_EARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->st = amb_stats{};
+    return AMB_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,283 @@
+/*
+ * Ring gather + c2r for gfx950: the filtered image planes of Dfg/Ifg
+ * (amber_funcs.c:573-621) as used by fill_s2d_float / fill_s2I
+ * (amber_funcs.c:967-1010, 1109-1131), in FP32.
+ *
+ * The c2r of an Ng x (Ng/2+1) half plane whose nonzeros are the cells of one
+ * annulus runs in two LDS-staged Stockham passes:
+ *   cols: for every column j < jcnt (the only columns a ring can touch,
+ *         jcnt = min(Ng/3, ceil(UHigh))) gather the ring cells straight from
+ *         the plane-major cube, inverse-DFT along the first axis in LDS and
+ *         write G[plane][j][x] (x contiguous, coalesced).
+ *   rows: for every pair of output rows (xa, xb) build the Hermitian
+ *         extension Z(k) = A(k) + i B(k), Z(Ng-k) = conj A(k) + i conj B(k),
+ *         one complex inverse DFT gives row xa in Re and row xb in Im; the
+ *         row is written once as Ng floats (coalesced).
+ * The j=0 column enters the row pass through its real part, which is the
+ * c2r of the Hermitian-symmetrised column (the FFTW/pocketfft behaviour).
+ * Radices 2, 3, 4 have register butterflies; any other prime factor runs a
+ * direct radix-p stage parallelised over its outputs, so every even Ng
+ * (2*109, 2*181, 2*349, ...) is handled without padding.
+ */
+#include "amb_internal.h"
+#include <math.h>
+#include <stdio.h>
+
+#define FFT_THREADS 256
+
+void amb_make_plan(int n, FftPlan *pl)
+{
+    pl->n = n;
+    pl->nst = 0;
+    int m = n;
+    while (m % 4 == 0 && pl->nst < AMB_MAX_RADIX_STAGES) { pl->rad[pl->nst++] = 4; m /= 4; }
+    while (m % 2 == 0 && pl->nst < AMB_MAX_RADIX_STAGES) { pl->rad[pl->nst++] = 2; m /= 2; }
+    while (m % 3 == 0 && pl->nst < AMB_MAX_RADIX_STAGES) { pl->rad[pl->nst++] = 3; m /= 3; }
+    for (int p = 5; m > 1 && pl->nst < AMB_MAX_RADIX_STAGES; p += 2)
+        while (m % p == 0) { pl->rad[pl->nst++] = p; m /= p; }
+}
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+
+/* nb transforms of length n in x[b*n + i]; inverse direction exp(+2 pi i/n).
+ * tw[m] = exp(+2 pi i m / n).  Returns the buffer holding the result. */
+__device__ float2 *lds_fft_batch(float2 *x, float2 *y, const float2 *tw, const FftPlan &pl, int nb)
+{
+    const int n = pl.n;
+    const int tid = threadIdx.x, nth = blockDim.x;
+    int Ns = 1;
+    for (int s = 0; s < pl.nst; ++s) {
+        const int R = pl.rad[s];
+        const int nR = n / R;
+        const int tstep = n / (Ns * R);
+        if (R == 4) {
+            for (int t = tid; t < nb * nR; t += nth) {
+                int b = t / nR, j = t - b * nR, k = j % Ns;
+                const float2 *xi = x + b * n;
+                float2 v0 = xi[j];
+                float2 v1 = cmul(xi[j + nR], tw[k * tstep]);
+                float2 v2 = cmul(xi[j + 2 * nR], tw[2 * k * tstep]);
+                float2 v3 = cmul(xi[j + 3 * nR], tw[3 * k * tstep]);
+                float2 s02 = cadd(v0, v2), d02 = csub(v0, v2);
+                float2 s13 = cadd(v1, v3), d13 = csub(v1, v3);
+                float2 id13 = make_float2(-d13.y, d13.x); /* i*(v1-v3) */
+                float2 *yo = y + b * n + (j / Ns) * Ns * 4 + k;
+                yo[0] = cadd(s02, s13);
+                yo[Ns] = cadd(d02, id13);
+                yo[2 * Ns] = csub(s02, s13);
+                yo[3 * Ns] = csub(d02, id13);
+            }
+        } else if (R == 2) {
+            for (int t = tid; t < nb * nR; t += nth) {
+                int b = t / nR, j = t - b * nR, k = j % Ns;
+                const float2 *xi = x + b * n;
+                float2 v0 = xi[j];
+                float2 v1 = cmul(xi[j + nR], tw[k * tstep]);
+                float2 *yo = y + b * n + (j / Ns) * Ns * 2 + k;
+                yo[0] = cadd(v0, v1);
+                yo[Ns] = csub(v0, v1);
+            }
+        } else if (R == 3) {
+            const float h = 0.86602540378443864676f; /* sqrt(3)/2 */
+            for (int t = tid; t < nb * nR; t += nth) {
+                int b = t / nR, j = t - b * nR, k = j % Ns;
+                const float2 *xi = x + b * n;
+                float2 v0 = xi[j];
+                float2 v1 = cmul(xi[j + nR], tw[k * tstep]);
+                float2 v2 = cmul(xi[j + 2 * nR], tw[2 * k * tstep]);
+                float2 sm = cadd(v1, v2), df = csub(v1, v2);
+                float2 m = make_float2(v0.x - 0.5f * sm.x, v0.y - 0.5f * sm.y);
+                float2 r = make_float2(-h * df.y, h * df.x); /* i*h*(v1-v2) */
+                float2 *yo = y + b * n + (j / Ns) * Ns * 3 + k;
+                yo[0] = cadd(v0, sm);
+                yo[Ns] = cadd(m, r);
+                yo[2 * Ns] = csub(m, r);
+            }
+        } else {
+            /* direct radix-R stage, one thread per output:
+               out[(j/Ns)Ns R + k + q Ns] = sum_r x[j + r nR] w_{Ns R}^{r (k + q Ns)} */
+            const int NsR = Ns * R;
+            for (int t = tid; t < nb * n; t += nth) {
+                int b = t / n, o = t - b * n;
+                int g = o / NsR, rem = o - g * NsR;
+                int q = rem / Ns, k = rem - q * Ns;
+                int j = g * Ns + k;
+                int e = k + q * Ns; /* exponent step in units of w_{NsR} */
+                const float2 *xi = x + b * n + j;
+                float2 acc = make_float2(0.f, 0.f);
+                int ex = 0;
+                for (int r = 0; r < R; ++r) {
+                    acc = cadd(acc, cmul(xi[r * nR], tw[ex * tstep]));
+                    ex += e;
+                    if (ex >= NsR) ex -= NsR;
+                }
+                y[b * n + o] = acc;
+            }
+        }
+        __syncthreads();
+        float2 *tmp = x; x = y; y = tmp;
+        Ns *= R;
+    }
+    return x;
+}
+
+/* grid (ceil(jcnt/nb), npl); dynamic LDS: tw[n] + 2 * nb * n complex */
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_cols(
+    int kind, const float2 *__restrict__ vis_t, const float *__restrict__ samp_t,
+    const int *__restrict__ planes, int N, int Nh, FftPlan pl, const float2 *__restrict__ twg,
+    double lo, double hi, int jcnt, int nb, float2 *__restrict__ G)
+{
+    extern __shared__ float2 smem[];
+    const int n = pl.n;
+    float2 *tw = smem, *bx = smem + n, *by = bx + nb * n;
+    const int pli = blockIdx.y;
+    const int ch = planes[pli];
+    const int j0 = blockIdx.x * nb;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) tw[t] = twg[t];
+    const int jlim = n / 3; /* column cut j < Ng/3 (amber_funcs.c:582) */
+    for (int t = threadIdx.x; t < nb * n; t += blockDim.x) {
+        int c = t / n, ing = t - c * n;
+        int j = j0 + c;
+        int ia = ing <= n / 2 ? ing : ing - n; /* ia in (-Ng/2, Ng/2] */
+        float2 v = make_float2(0.f, 0.f);
+        if (j < jcnt && j < jlim) {
+            double uu = sqrt(1. * (double)(ia * ia + j * j));
+            if (uu >= lo && uu < hi && uu > 0.0) {
+                int in = ia < 0 ? N + ia : ia;
+                size_t src = ((size_t)ch * Nh + j) * N + in;
+                if (kind == 0) v = vis_t[src];
+                else v = make_float2(samp_t[src], 0.f);
+            }
+        }
+        bx[t] = v;
+    }
+    __syncthreads();
+    float2 *res = lds_fft_batch(bx, by, tw, pl, nb);
+    for (int t = threadIdx.x; t < nb * n; t += blockDim.x) {
+        int c = t / n, x = t - c * n;
+        int j = j0 + c;
+        if (j < jcnt) G[((size_t)pli * jcnt + j) * n + x] = res[t];
+    }
+}
+
+/* grid (ceil(n/(2 nb)), npl); each block nb transforms = 2 nb output rows */
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_rows(
+    const float2 *__restrict__ G, int jcnt, FftPlan pl, const float2 *__restrict__ twg, int nb,
+    float *__restrict__ out)
+{
+    extern __shared__ float2 smem[];
+    const int n = pl.n;
+    float2 *tw = smem, *bx = smem + n, *by = bx + nb * n;
+    const int pli = blockIdx.y;
+    const int x0 = blockIdx.x * 2 * nb;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) tw[t] = twg[t];
+    const float2 *Gp = G + (size_t)pli * jcnt * n;
+    /* b fastest so a warp reads 2*nb consecutive x of one column */
+    for (int t = threadIdx.x; t < nb * n; t += blockDim.x) {
+        int k = t / nb, b = t - k * nb;
+        int xa = x0 + 2 * b, xb = xa + 1;
+        float2 z = make_float2(0.f, 0.f);
+        if (xb < n) {
+            if (k == 0) {
+                float2 A = Gp[xa], B = Gp[xb];
+                z = make_float2(A.x, B.x);
+            } else if (k < jcnt) {
+                float2 A = Gp[(size_t)k * n + xa], B = Gp[(size_t)k * n + xb];
+                z = make_float2(A.x - B.y, A.y + B.x);
+            } else if (n - k < jcnt) {
+                int kk = n - k;
+                float2 A = Gp[(size_t)kk * n + xa], B = Gp[(size_t)kk * n + xb];
+                z = make_float2(A.x + B.y, B.x - A.y);
+            }
+        }
+        bx[b * n + k] = z;
+    }
+    __syncthreads();
+    float2 *res = lds_fft_batch(bx, by, tw, pl, nb);
+    float *op = out + (size_t)pli * n * n;
+    for (int t = threadIdx.x; t < nb * n; t += blockDim.x) {
+        int b = t / n, yv = t - b * n;
+        int xa = x0 + 2 * b;
+        if (xa + 1 < n) {
+            float2 z = res[t];
+            op[(size_t)xa * n + yv] = z.x;
+            op[(size_t)(xa + 1) * n + yv] = z.y;
+        }
+    }
+}
+
+__global__ void k_zero(float *p, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t st = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += st) p[i] = 0.f;
+}
+
+static int lds_batch(int n)
+{   /* transforms per block with tw + 2 buffers inside 64 KiB */
+    int nb = (65536 / (8 * n) - 1) / 2;
+    if (nb > 8) nb = 8;
+    if (nb < 1) nb = 1;
+    return nb;
+}
+
+hipError_t amb_launch_ring_c2r(hipStream_t s, int kind, const float2 *vis_t, const float *samp_t,
+                               const int *d_planes, int npl, int N, int Nh, const FftPlan &pl,
+                               const float2 *d_tw, RingGeom g, float2 *G, float *out)
+{
+    const int n = pl.n;
+    if (npl <= 0) return hipSuccess;
+    if (g.jcnt <= 0) {
+        size_t tot = (size_t)npl * n * n;
+        hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, s, out, tot);
+        return hipGetLastError();
+    }
+    int nb = lds_batch(n);
+    size_t lds = sizeof(float2) * (size_t)(n + 2 * nb * n);
+    dim3 gc((g.jcnt + nb - 1) / nb, npl);
+    hipLaunchKernelGGL(k_fft_cols, gc, dim3(FFT_THREADS), lds, s, kind, vis_t, samp_t, d_planes, N, Nh,
+                       pl, d_tw, g.lo, g.hi, g.jcnt, nb, G);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    dim3 gr((n + 2 * nb - 1) / (2 * nb), npl);
+    hipLaunchKernelGGL(k_fft_rows, gr, dim3(FFT_THREADS), lds, s, G, g.jcnt, pl, d_tw, nb, out);
+    return hipGetLastError();
+}
+
+/* Upload: file layout [rows][Nh][nc]{re,im} doubles -> plane-major
+ * vis_t[nc][Nh][N] complex64 and samp_t[nc][Nh][N] float, with
+ * normalize_data (amber_funcs.c:1946-1978) applied in FP64 first. */
+__global__ void k_upload(const double *__restrict__ gv, const double *__restrict__ samp, int i0,
+                         int rows, int N, int Nh, int nc, float2 *__restrict__ vis_t,
+                         float *__restrict__ samp_t)
+{
+    size_t tot = (size_t)rows * Nh * nc;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < tot;
+         idx += (size_t)gridDim.x * blockDim.x) {
+        int c = (int)(idx % nc);
+        size_t cell = idx / nc;
+        int j = (int)(cell % Nh);
+        int r = (int)(cell / Nh);
+        double sv = samp[idx];
+        double re = gv[2 * idx], im = gv[2 * idx + 1];
+        if (sv > 0) { re /= sv; im /= sv; sv = 1.0; }
+        size_t dst = ((size_t)c * Nh + j) * N + (i0 + r);
+        vis_t[dst] = make_float2((float)re, (float)im);
+        samp_t[dst] = (float)sv;
+    }
+}
+
+hipError_t amb_launch_upload(hipStream_t s, const double *gv, const double *samp, int i0, int rows,
+                             int N, int Nh, int nc, float2 *vis_t, float *samp_t)
+{
+    size_t tot = (size_t)rows * Nh * nc;
+    int blocks = (int)((tot + 255) / 256);
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_upload, dim3(blocks), dim3(256), 0, s, gv, samp, i0, rows, N, Nh, nc, vis_t, samp_t);
+    return hipGetLastError();
+}

@@ -1,0 +1,356 @@
+/*
+ * Host-side pieces of amber_mi355x: file formats, integer/FP64 rules and the
+ * deterministic synthetic-cube generator.  Plain C11 + OpenMP.
+ * Byte layouts follow the reference's writers and readers (cited per
+ * function); every error is returned, never exit().
+ */
+#include "amber_mi355x.h"
+#include <errno.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+void amb_free(void *p) { free(p); }
+
+const char *amb_strerror(int code)
+{
+    switch (code) {
+    case AMB_OK: return "ok";
+    case AMB_EARG: return "invalid argument";
+    case AMB_EIO: return "file missing or unreadable";
+    case AMB_EFORMAT: return "truncated or inconsistent file";
+    case AMB_EHIP: return "HIP runtime error";
+    case AMB_ENOMEM: return "host allocation failed";
+    case AMB_EBUDGET: return "request exceeds the HBM budget";
+    case AMB_ESTATE: return "context state not set (rings, channels or cube)";
+    case AMB_ECERT: return "integer certificate failed";
+    case AMB_EBUSY: return "queue full";
+    default: return "unknown error";
+    }
+}
+
+/* ---------------------------------------------------------------- params */
+int amb_read_params_ess(const char *path, double *nu0, double *dnu, double *chan0,
+                        char *data_path, size_t cap)
+{   /* amber_funcs.c:175-188 */
+    if (!path || !data_path || cap < 2) return AMB_EARG;
+    FILE *fp = fopen(path, "r");
+    if (!fp) return AMB_EIO;
+    char buf[512];
+    double a, b, c;
+    int n = fscanf(fp, "%lf%lf%lf%511s", &a, &b, &c, buf);
+    fclose(fp);
+    if (n != 4) return AMB_EFORMAT;
+    if (nu0) *nu0 = a;
+    if (dnu) *dnu = b;
+    if (chan0) *chan0 = c;
+    if (strlen(buf) + 1 > cap) return AMB_EARG;
+    strcpy(data_path, buf);
+    return AMB_OK;
+}
+
+/* ---------------------------------------------------------------- cube files */
+static int read_header(FILE *fp, int64_t *NN, int64_t *nc, double *Du, int32_t *fk)
+{   /* amber_funcs.c:1736-1739 / 1877-1880: four separate freads, 28 bytes */
+    if (fread(NN, sizeof(int64_t), 1, fp) != 1) return AMB_EFORMAT;
+    if (fread(nc, sizeof(int64_t), 1, fp) != 1) return AMB_EFORMAT;
+    if (fread(Du, sizeof(double), 1, fp) != 1) return AMB_EFORMAT;
+    if (fread(fk, sizeof(int32_t), 1, fp) != 1) return AMB_EFORMAT;
+    if (*NN <= 0 || *nc <= 0 || *NN > (1 << 16) || *nc > (1 << 16) || !(*Du > 0)) return AMB_EFORMAT;
+    return AMB_OK;
+}
+
+int amb_read_gv_header(const char *path, int64_t *NN, int64_t *nc, double *Du, int32_t *flag_key)
+{
+    FILE *fp = fopen(path, "rb");
+    if (!fp) return AMB_EIO;
+    int rc = read_header(fp, NN, nc, Du, flag_key);
+    fclose(fp);
+    return rc;
+}
+
+int amb_read_cube_files(const char *data_path, int rlzn, int64_t *NN, int64_t *nc, double *Du,
+                        int32_t *flag_key, double **gv, double **samp)
+{   /* initialize + ReadGVnAC_gridding, amber_funcs.c:1720-1779, 1861-1911 */
+    char f[1024];
+    snprintf(f, sizeof f, "%s/GV_data_%d", data_path, rlzn);
+    FILE *fp = fopen(f, "rb");
+    if (!fp) return AMB_EIO;
+    int rc = read_header(fp, NN, nc, Du, flag_key);
+    if (rc) { fclose(fp); return rc; }
+    size_t cells = (size_t)(*NN) * (size_t)(*NN / 2 + 1);
+    size_t nv = cells * (size_t)(*nc);
+    double *g = malloc(sizeof(double) * 2 * nv);
+    double *s = malloc(sizeof(double) * nv);
+    if (!g || !s) { free(g); free(s); fclose(fp); return AMB_ENOMEM; }
+    size_t got = fread(g, sizeof(double), 2 * nv, fp);
+    fclose(fp);
+    if (got != 2 * nv) { free(g); free(s); return AMB_EFORMAT; }
+    snprintf(f, sizeof f, "%s/sampling", data_path);
+    fp = fopen(f, "rb");
+    if (!fp) { free(g); free(s); return AMB_EIO; }
+    got = fread(s, sizeof(double), nv, fp);
+    fclose(fp);
+    if (got != nv) { free(g); free(s); return AMB_EFORMAT; }
+    *gv = g;
+    *samp = s;
+    return AMB_OK;
+}
+
+int amb_write_cube_files(const char *data_path, int rlzn, int64_t NN, int64_t nc, double Du,
+                         int32_t flag_key, const double *gv, const double *samp)
+{   /* WriteGVnAC_gridding 3_vcg/tgefuncs.c:639-655, Write_NTiG :728-748 */
+    char f[1024];
+    size_t nv = (size_t)NN * (size_t)(NN / 2 + 1) * (size_t)nc;
+    snprintf(f, sizeof f, "%s/GV_data_%d", data_path, rlzn);
+    FILE *fp = fopen(f, "wb");
+    if (!fp) return AMB_EIO;
+    int ok = fwrite(&NN, sizeof NN, 1, fp) == 1 && fwrite(&nc, sizeof nc, 1, fp) == 1 &&
+             fwrite(&Du, sizeof Du, 1, fp) == 1 && fwrite(&flag_key, sizeof flag_key, 1, fp) == 1 &&
+             fwrite(gv, sizeof(double), 2 * nv, fp) == 2 * nv;
+    if (fclose(fp) != 0 || !ok) return AMB_EIO;
+    snprintf(f, sizeof f, "%s/sampling", data_path);
+    fp = fopen(f, "wb");
+    if (!fp) return AMB_EIO;
+    ok = fwrite(samp, sizeof(double), nv, fp) == nv;
+    if (fclose(fp) != 0 || !ok) return AMB_EIO;
+    return AMB_OK;
+}
+
+int amb_read_uf_chans(const char *path, int uf_nc, int64_t nc, int *chans)
+{   /* amber_funcs.c:2485-2508; a short read or an out-of-range id is an
+       error here (the reference prints and continues, docs/REVIEW.md:15). */
+    if (uf_nc < 0 || !chans) return AMB_EARG;
+    FILE *fp = fopen(path, "r");
+    if (!fp) return AMB_EIO;
+    int i = 0;
+    while (i < uf_nc && fscanf(fp, "%d", &chans[i]) == 1) ++i;
+    fclose(fp);
+    if (i < uf_nc) return AMB_EFORMAT;
+    for (int k = 0; k < uf_nc; ++k)
+        if (chans[k] < 0 || chans[k] >= nc) return AMB_EFORMAT;
+    return AMB_OK;
+}
+
+int amb_write_uf_chans(const char *path, int uf_nc, const int *chans)
+{
+    FILE *fp = fopen(path, "w");
+    if (!fp) return AMB_EIO;
+    for (int i = 0; i < uf_nc; ++i) fprintf(fp, "%d\n", chans[i]);
+    return fclose(fp) == 0 ? AMB_OK : AMB_EIO;
+}
+
+/* ---------------------------------------------------------------- rings files */
+int amb_write_closed_rings(const char *res_path, const double *rdata, int rcount)
+{   /* amber_funcs.c:406-411,459-463 (binary, 4 doubles per triangle) and
+       :476-483 (text "%d") */
+    char f[1024];
+    snprintf(f, sizeof f, "%s/closed_rings", res_path);
+    FILE *fp = fopen(f, "wb");
+    if (!fp) return AMB_EIO;
+    int ok = rcount == 0 || fwrite(rdata, sizeof(double), (size_t)4 * rcount, fp) == (size_t)4 * rcount;
+    if (fclose(fp) != 0 || !ok) return AMB_EIO;
+    snprintf(f, sizeof f, "%s/rcount", res_path);
+    fp = fopen(f, "w");
+    if (!fp) return AMB_EIO;
+    fprintf(fp, "%d", rcount);
+    return fclose(fp) == 0 ? AMB_OK : AMB_EIO;
+}
+
+int amb_read_rings(const char *res_path, int *rcount, double **rdata)
+{   /* read_rings, amber_funcs.c:499-545 */
+    char f[1024];
+    snprintf(f, sizeof f, "%s/rcount", res_path);
+    FILE *fp = fopen(f, "r");
+    if (!fp) return AMB_EIO;
+    int n;
+    int got = fscanf(fp, "%d", &n);
+    fclose(fp);
+    if (got != 1 || n < 0) return AMB_EFORMAT;
+    snprintf(f, sizeof f, "%s/closed_rings", res_path);
+    fp = fopen(f, "rb");
+    if (!fp) return AMB_EIO;
+    double *d = malloc(sizeof(double) * 4 * (size_t)(n ? n : 1));
+    if (!d) { fclose(fp); return AMB_ENOMEM; }
+    size_t r = fread(d, sizeof(double), (size_t)4 * n, fp);
+    fclose(fp);
+    if (r != (size_t)4 * n) { free(d); return AMB_EFORMAT; }
+    *rcount = n;
+    *rdata = d;
+    return AMB_OK;
+}
+
+/* ---------------------------------------------------------------- bsout files */
+int amb_write_bsout_v1(const char *res_path, int r1, int r2, int r3, int rlzn, int W,
+                       double ntri, double num, const float *bl, const int32_t *cnt)
+{   /* amber.c:289-304 */
+    char f[1024];
+    size_t n = (size_t)W * W;
+    snprintf(f, sizeof f, "%s/bsout_final_ra%d_rb%d_rc%d_R%d", res_path, r1, r2, r3, rlzn);
+    FILE *fp = fopen(f, "wb");
+    if (!fp) return AMB_EIO;
+    int ok = fwrite(&ntri, sizeof(double), 1, fp) == 1 && fwrite(&num, sizeof(double), 1, fp) == 1 &&
+             fwrite(bl, sizeof(float), n, fp) == n && fwrite(cnt, sizeof(int32_t), n, fp) == n;
+    if (fclose(fp) != 0 || !ok) return AMB_EIO;
+    return AMB_OK;
+}
+
+int amb_write_bsout_v2(const char *res_path, int r1, int r2, int r3, int rlzn, int W,
+                       double inv_ng2, const float *bl, const float *norm)
+{   /* amber_v2.c:295-309 */
+    char f[1024];
+    size_t n = (size_t)W * W;
+    snprintf(f, sizeof f, "%s/bsout_final2_ra%d_rb%d_rc%d_R%d", res_path, r1, r2, r3, rlzn);
+    FILE *fp = fopen(f, "wb");
+    if (!fp) return AMB_EIO;
+    int ok = fwrite(&inv_ng2, sizeof(double), 1, fp) == 1 && fwrite(bl, sizeof(float), n, fp) == n &&
+             fwrite(norm, sizeof(float), n, fp) == n;
+    if (fclose(fp) != 0 || !ok) return AMB_EIO;
+    return AMB_OK;
+}
+
+/* ---------------------------------------------------------------- rules */
+long amb_ng(double uhigh, double du)
+{   /* amber.c:193-197, same integer truncation and parity step */
+    double umax = uhigh + 3;
+    long nmax = (long)(umax / du);
+    long ng = (long)(3 * nmax);
+    return (ng % 2 == 0) ? ng + 2 : ng + 3;
+}
+
+void amb_uavg(const double *ulow, const double *uhigh, int nr, double *uavg)
+{   /* amber_funcs.c:2065: (2/3)(H^3-L^3)/(H^2-L^2) */
+    for (int i = 0; i < nr; ++i) {
+        double h = uhigh[i], l = ulow[i];
+        uavg[i] = (2. / 3.) * (pow(h, 3.) - pow(l, 3.)) / (pow(h, 2.) - pow(l, 2.));
+    }
+}
+
+void amb_annular_rings(int nr, double *ulow, double *uhigh)
+{   /* amber_funcs.c:2025-2053: edges step 5 to 50, 10 to 100, 20 to 200,
+       35 to 270; each UHigh is the next ULow; ULow[0]=1, UHigh[last]=270 */
+    static const int seg[4][3] = {{0, 50, 5}, {50, 100, 10}, {100, 200, 20}, {200, 270, 35}};
+    int k = 0;
+    for (int i = 0; i < nr; ++i) ulow[i] = uhigh[i] = 0;
+    for (int s = 0; s < 4; ++s)
+        for (int e = seg[s][0]; e < seg[s][1] && k < nr; e += seg[s][2]) {
+            ulow[k] = e;
+            if (k > 0) uhigh[k - 1] = e;
+            ++k;
+        }
+    if (nr > 0) { ulow[0] = 1.; uhigh[nr - 1] = 270.; }
+}
+
+int amb_triangles(const double *uavg, int nr, int *tri, int cap)
+{   /* amber_funcs.c:443-449: r1>=r2>=r3, t=U2/U1 in [0.5,1],
+       mu=0.5(t+1/t-(U3/U1)(U3/U2)) in [0.5/t,1]; expression order kept so
+       borderline equalities decide the same way */
+    int n = 0;
+    for (int a = 0; a < nr; ++a)
+        for (int b = 0; b <= a; ++b) {
+            double t = uavg[b] / uavg[a];
+            if (!(t >= 0.5 && t <= 1.0)) continue;
+            for (int c = 0; c <= b; ++c) {
+                double mu = 0.5 * (t + 1. / t - (uavg[c] / uavg[a]) * (uavg[c] / uavg[b]));
+                if (mu >= 0.5 / t && mu <= 1.0) {
+                    if (tri && n < cap) { tri[3 * n] = a; tri[3 * n + 1] = b; tri[3 * n + 2] = c; }
+                    ++n;
+                }
+            }
+        }
+    return n;
+}
+
+/* ---------------------------------------------------------------- generator */
+static inline uint64_t mix64(uint64_t z)
+{   /* splitmix64 finaliser */
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+static inline double u01(uint64_t *s)
+{   /* (0,1]: 53 random bits, never 0 so log() is finite */
+    *s += 0x9e3779b97f4a7c15ULL;
+    return ((mix64(*s) >> 11) + 1) * (1.0 / 9007199254740992.0);
+}
+
+int amb_generate(const amb_gen_cfg *cfg, double *gv, double *samp, int *uf_chans, int *uf_nc)
+{
+    if (!cfg || !gv || !samp || cfg->N < 4 || cfg->N % 2 || cfg->nc < 1 || cfg->nflag < 0 ||
+        cfg->nflag >= cfg->nc || (cfg->nc > 5 && cfg->nflag > cfg->nc - 1))
+        return AMB_EARG;
+    const int64_t N = cfg->N, nc = cfg->nc, Nh = N / 2 + 1;
+    const double p0 = cfg->p0 > 0 ? cfg->p0 : 0.4, u0 = cfg->u0 > 0 ? cfg->u0 : 40.0;
+    uint8_t *flag = calloc(nc, 1);
+    if (!flag) return AMB_ENOMEM;
+    /* flagged channels: a seeded Fisher-Yates draw that never picks channel 5
+       (calCR reads sampling plane 5, amber_funcs.c:425) */
+    {
+        int64_t *ids = malloc(sizeof(int64_t) * nc);
+        int64_t m = 0;
+        for (int64_t c = 0; c < nc; ++c) if (c != 5) ids[m++] = c;
+        uint64_t s = cfg->seed ^ 0xf1a9f1a9ULL;
+        for (int64_t i = 0; i < cfg->nflag && i < m; ++i) {
+            int64_t j = i + (int64_t)(u01(&s) * (double)(m - i));
+            if (j >= m) j = m - 1;
+            int64_t t = ids[i]; ids[i] = ids[j]; ids[j] = t;
+            flag[ids[i]] = 1;
+        }
+        free(ids);
+    }
+    int n_uf = 0;
+    for (int64_t c = 0; c < nc; ++c)
+        if (!flag[c]) { if (uf_chans) uf_chans[n_uf] = (int)c; ++n_uf; }
+    if (uf_nc) *uf_nc = n_uf;
+
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < N; ++i) {
+        int64_t ia = i <= N / 2 ? i : i - N;
+        for (int64_t j = 0; j < Nh; ++j) {
+            int64_t cell = i * Nh + j;
+            double pm = p0 * exp(-sqrt((double)(ia * ia + j * j)) / u0);
+            for (int64_t c = 0; c < nc; ++c) {
+                uint64_t s = mix64(cfg->seed * 0x632be59bd9b4e019ULL + (uint64_t)(cell * nc + c) + 1);
+                double a = u01(&s), b = u01(&s), mk = u01(&s), cn = u01(&s);
+                if (cfg->shared_mask) {
+                    uint64_t s2 = mix64(cfg->seed * 0x632be59bd9b4e019ULL + (uint64_t)cell * 0x9e3779b1ULL + 7);
+                    mk = u01(&s2);
+                    cn = u01(&s2);
+                }
+                double m = (!flag[c] && mk < pm) ? 1.0 : 0.0;
+                double cnt = m;
+                if (m > 0 && cfg->count_max > 1) {
+                    cnt = 1.0 + floor(cn * cfg->count_max);
+                    if (cnt > cfg->count_max) cnt = cfg->count_max;
+                }
+                double r = sqrt(-2.0 * log(a));
+                gv[(cell * nc + c) * 2] = r * cos(2 * M_PI * b) * cnt;
+                gv[(cell * nc + c) * 2 + 1] = r * sin(2 * M_PI * b) * cnt;
+                samp[cell * nc + c] = cnt;
+            }
+        }
+    }
+    /* Hermitian symmetry of the self-conjugate columns j=0 and j=N/2:
+       (N-i, j) = conj(i, j); rows 0 and N/2 are real there. */
+    int64_t cols[2] = {0, N / 2};
+    for (int k = 0; k < 2; ++k) {
+        int64_t j = cols[k];
+        for (int64_t i = N / 2 + 1; i < N; ++i) {
+            int64_t dst = i * Nh + j, src = (N - i) * Nh + j;
+            for (int64_t c = 0; c < nc; ++c) {
+                gv[(dst * nc + c) * 2] = gv[(src * nc + c) * 2];
+                gv[(dst * nc + c) * 2 + 1] = -gv[(src * nc + c) * 2 + 1];
+                samp[dst * nc + c] = samp[src * nc + c];
+            }
+        }
+        int64_t rows[2] = {0, N / 2};
+        for (int q = 0; q < 2; ++q)
+            for (int64_t c = 0; c < nc; ++c) gv[((rows[q] * Nh + j) * nc + c) * 2 + 1] = 0.0;
+    }
+    free(flag);
+    return AMB_OK;
+}

@@ -1,0 +1,201 @@
+/*
+ * Drop-in drivers `amber` (v1) and `amber_v2` (built with -DAMB_V2) on top of
+ * the amber_mi355x C ABI.  Same working-directory contract as the reference
+ * (4_fbe/amber.c:21-335, 4_fbe/amber_v2.c:21-345): run from a directory
+ * holding input_params_ess and results/v1 (results/v2), read
+ * <data_path>/GV_data_<Rlzn>, <data_path>/sampling and, for flagged data,
+ * <data_path>/uf_chans.txt, and write the same result files byte for byte in
+ * layout.
+ *
+ * The reference hard-codes Nr=22, nc10=383, uf_nc=552 and the first
+ * triangle (amber.c:43-49,187; amber_v2.c:187).  They are options here with
+ * those defaults, except the first triangle, which defaults to 0 (a full
+ * run); pass --rp-start 365 (v1) or 635 (v2) to reproduce the reference's
+ * partial runs.  --rings FILE replaces the built-in piecewise edges.
+ */
+#include "amber_mi355x.h"
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#ifdef AMB_V2
+#define PROG "amber_v2"
+#define RES_PATH "results/v2"
+#else
+#define PROG "amber"
+#define RES_PATH "results/v1"
+#endif
+
+static double now(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+static void usage(void)
+{
+    fprintf(stderr,
+            "Usage: %s <Rlzn> [--params FILE] [--rings FILE] [--nr N] [--nc10 N] [--uf-nc N]\n"
+            "          [--rp-start N] [--rp-end N] [--precision fp32|bf16x3] [--device N]\n"
+            "          [--calcr-plane N] [--res-path DIR]\n",
+            PROG);
+}
+
+static int read_rings_file(const char *path, double **lo, double **hi, int *nr)
+{
+    FILE *fp = fopen(path, "r");
+    if (!fp) return AMB_EIO;
+    int cap = 64, n = 0;
+    double *l = malloc(sizeof(double) * cap), *h = malloc(sizeof(double) * cap);
+    double a, b;
+    while (fscanf(fp, "%lf %lf", &a, &b) == 2) {
+        if (n == cap) {
+            cap *= 2;
+            l = realloc(l, sizeof(double) * cap);
+            h = realloc(h, sizeof(double) * cap);
+        }
+        l[n] = a; h[n] = b; ++n;
+    }
+    fclose(fp);
+    if (n == 0) { free(l); free(h); return AMB_EFORMAT; }
+    *lo = l; *hi = h; *nr = n;
+    return AMB_OK;
+}
+
+#define CHECK(call)                                                                        \
+    do {                                                                                   \
+        int rc_ = (call);                                                                  \
+        if (rc_ != AMB_OK) {                                                               \
+            fprintf(stderr, "%s: %s: %s (%s)\n", PROG, #call, amb_strerror(rc_),            \
+                    ctx ? amb_last_error(ctx) : "");                                       \
+            exit(EXIT_FAILURE);                                                            \
+        }                                                                                  \
+    } while (0)
+
+int main(int argc, char **argv)
+{
+    amb_ctx *ctx = NULL;
+    if (argc < 2) { usage(); return 1; }
+    int Rlzn = atoi(argv[1]);
+    const char *params = "input_params_ess", *rings_file = NULL, *res_path = RES_PATH;
+    int nr = 22, nc10 = 383, uf_nc = 552, rp_start = 0, rp_end = -1, device = 0, calcr_plane = 5;
+    int prec = AMB_PREC_BF16X3;
+    for (int i = 2; i < argc; ++i) {
+        const char *a = argv[i];
+        const char *v = i + 1 < argc ? argv[i + 1] : NULL;
+        if (!v) { usage(); return 1; }
+        if (!strcmp(a, "--params")) params = v;
+        else if (!strcmp(a, "--rings")) rings_file = v;
+        else if (!strcmp(a, "--nr")) nr = atoi(v);
+        else if (!strcmp(a, "--nc10")) nc10 = atoi(v);
+        else if (!strcmp(a, "--uf-nc")) uf_nc = atoi(v);
+        else if (!strcmp(a, "--rp-start")) rp_start = atoi(v);
+        else if (!strcmp(a, "--rp-end")) rp_end = atoi(v);
+        else if (!strcmp(a, "--device")) device = atoi(v);
+        else if (!strcmp(a, "--calcr-plane")) calcr_plane = atoi(v);
+        else if (!strcmp(a, "--res-path")) res_path = v;
+        else if (!strcmp(a, "--precision")) {
+            if (!strcmp(v, "fp32")) prec = AMB_PREC_FP32;
+            else if (!strcmp(v, "bf16x3")) prec = AMB_PREC_BF16X3;
+            else { usage(); return 1; }
+        } else { usage(); return 1; }
+        ++i;
+    }
+    double T = now();
+    char data_path[512];
+    double nu0, dnu, chan0;
+    int rc = amb_read_params_ess(params, &nu0, &dnu, &chan0, data_path, sizeof data_path);
+    if (rc) { fprintf(stderr, "%s: reading %s: %s\n", PROG, params, amb_strerror(rc)); return 1; }
+    printf("\nInput parameters:\n\nnu0=%.2lf\ndeltanu=%.2lf\nchan0=%.2lf\nData Path=%s\n\n", nu0, dnu, chan0, data_path);
+
+    amb_opts o = {device, prec, 0, 1};
+    rc = amb_create(&o, &ctx);
+    if (rc) { fprintf(stderr, "%s: amb_create: %s\n", PROG, amb_strerror(rc)); return 1; }
+    int cube;
+    CHECK(amb_load_cube_files(ctx, data_path, Rlzn, &cube));
+    char gvp[1024];
+    int64_t NN, nc;
+    double Du;
+    int32_t flag_key;
+    snprintf(gvp, sizeof gvp, "%s/GV_data_%d", data_path, Rlzn);
+    CHECK(amb_read_gv_header(gvp, &NN, &nc, &Du, &flag_key));
+    printf("NN=%lld, nc=%lld, Du=%lf, Flag=%d, nc10=%d, Unflagged Channles=%d\n", (long long)NN,
+           (long long)nc, Du, flag_key, nc10, uf_nc);
+    printf("result will be saved in %s\n", res_path);
+
+    double *lo, *hi;
+    if (rings_file) {
+        rc = read_rings_file(rings_file, &lo, &hi, &nr);
+        if (rc) { fprintf(stderr, "%s: reading %s: %s\n", PROG, rings_file, amb_strerror(rc)); return 1; }
+    } else {
+        lo = malloc(sizeof(double) * nr);
+        hi = malloc(sizeof(double) * nr);
+        amb_annular_rings(nr, lo, hi);
+    }
+    double *uavg = malloc(sizeof(double) * nr);
+    amb_uavg(lo, hi, nr, uavg);
+    printf("\t%s\t\t%s\t\t\t%s\n", "ULow", "UHigh", "Uavg");
+    for (int i = 0; i < nr; ++i) printf("\t%f\t%f\t%f\n", lo[i], hi[i], uavg[i]);
+    CHECK(amb_set_rings(ctx, lo, hi, nr));
+
+    int *chans = NULL;
+    if (flag_key == 1) {
+        char ufp[1024];
+        snprintf(ufp, sizeof ufp, "%s/uf_chans.txt", data_path);
+        chans = malloc(sizeof(int) * (uf_nc > 0 ? uf_nc : 1));
+        rc = amb_read_uf_chans(ufp, uf_nc, nc, chans);
+        if (rc) { fprintf(stderr, "%s: reading %s (%d ids): %s\n", PROG, ufp, uf_nc, amb_strerror(rc)); return 1; }
+    }
+    CHECK(amb_set_channels(ctx, cube, chans, uf_nc, nc10));
+
+#ifndef AMB_V2
+    /* calCR writes closed_rings/rcount (amber.c:127) */
+    int ntri_max = amb_triangles(uavg, nr, NULL, 0);
+    double *rd0 = malloc(sizeof(double) * 4 * (ntri_max ? ntri_max : 1));
+    int rcount0 = 0;
+    CHECK(amb_closed_rings(ctx, cube, calcr_plane, rd0, ntri_max, &rcount0));
+    CHECK(amb_write_closed_rings(res_path, rd0, rcount0));
+    printf("\nTotal combinations of rings on a plane forming closed triangle = %d\n", rcount0);
+    free(rd0);
+#endif
+    int rcount;
+    double *rdata;
+    rc = amb_read_rings(res_path, &rcount, &rdata);
+    if (rc) { fprintf(stderr, "%s: reading %s/{rcount,closed_rings}: %s\n", PROG, res_path, amb_strerror(rc)); return 1; }
+    printf("Successfully read %d ring configurations.\n", rcount);
+
+    const int W = 2 * nc10 + 1;
+    const size_t ncsize = (size_t)W * W;
+    float *bl = malloc(sizeof(float) * ncsize);
+#ifndef AMB_V2
+    int32_t *cnt = malloc(sizeof(int32_t) * ncsize);
+#else
+    float *norm = malloc(sizeof(float) * ncsize);
+#endif
+    if (rp_end < 0 || rp_end > rcount) rp_end = rcount;
+    for (int rp = rp_start; rp < rp_end; ++rp) {
+        int r1 = (int)rdata[rp * 4 + 0], r2 = (int)rdata[rp * 4 + 1], r3 = (int)rdata[rp * 4 + 2];
+        double t0 = now();
+#ifndef AMB_V2
+        double Ntri, Num;
+        CHECK(amb_bispec_v1(ctx, cube, r1, r2, r3, rdata[rp * 4 + 3], bl, cnt, &Ntri, &Num));
+        CHECK(amb_write_bsout_v1(res_path, r1, r2, r3, Rlzn, W, Ntri, Num, bl, cnt));
+#else
+        double inv;
+        CHECK(amb_bispec_v2(ctx, cube, r1, r2, r3, bl, norm, &inv));
+        CHECK(amb_write_bsout_v2(res_path, r1, r2, r3, Rlzn, W, inv, bl, norm));
+#endif
+        printf("rp=%d (r1=%d r2=%d r3=%d, Ng=%ld): %.3f s\n", rp, r1, r2, r3, amb_ng(hi[r1], Du), now() - t0);
+    }
+    printf("Total time taken = %.3f s\n", now() - T);
+    amb_destroy(ctx);
+    free(bl); free(rdata); free(lo); free(hi); free(uavg); free(chans);
+#ifndef AMB_V2
+    free(cnt);
+#else
+    free(norm);
+#endif
+    return 0;
+}

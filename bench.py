@@ -1,0 +1,242 @@
+"""Benchmark of the 4_fbe estimator hot path on one MI355X.
+
+A step is one full pass over every closed triangle of the workload: ring
+gather + c2r of every needed (ring, channel) plane, the per-p1 MFMA
+contraction, the lag reduction with counts, and the copy of every
+triangle's W x W accumulator and count arrays to host memory.  The cube is
+resident in HBM before the timed region starts.
+
+Default workload (the metric's configuration): 768 channels x 512^2 grid,
+24 log-spaced annular bins 1..200 cells, all 506 closed triangles, all lags
+(nc10 = 767), split-BF16 contraction.
+
+    python bench.py [--gpus 1] [--steps 1] [--warmup 1] [--config cfg3]
+                    [--precision bf16x3|fp32] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+CONFIGS = {
+    "cfg1": dict(N=128, nc=64, nflag=6, rings=(2, 60, 8), nc10=63, seed=1001,
+                 desc="plain small: 64 ch x 128^2, 6 flagged, 8 log bins 2-60, lags |dnu|<=63"),
+    "cfg2": dict(N=512, nc=384, nflag=31, rings=(1, 200, 20), nc10=383, seed=2001,
+                 desc="plain scaled: 384 ch x 512^2, 31 flagged, 20 log bins 1-200, all lags"),
+    "cfg3": dict(N=512, nc=768, nflag=0, rings=(1, 200, 24), nc10=767, seed=3001,
+                 desc="precision: 768 ch x 512^2, 24 log bins 1-200, all lags"),
+}
+
+PEAK_TFLOPS = {"bf16x3": 2500.0, "fp32": 157.3}   # dense MFMA, published
+PASSES = {"bf16x3": 3, "fp32": 1}
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="cfg3", choices=sorted(CONFIGS))
+    ap.add_argument("--precision", default="bf16x3", choices=["bf16x3", "fp32"])
+    ap.add_argument("--tri-limit", type=int, default=0,
+                    help="development only: first K triangles (the line is then not the metric workload)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-error", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; --gpus %d is not supported (the path would shard by "
+              "triangle with no collective, see the design notes)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import amber_amd as A
+    cfg = CONFIGS[args.config]
+    N, nc, nc10 = cfg["N"], cfg["nc"], cfg["nc10"]
+    lo, hi = A.log_rings(*cfg["rings"])
+    tris = A.triangles(lo, hi)
+    if args.tri_limit:
+        tris = tris[: args.tri_limit]
+    t0 = time.time()
+    gv, samp, uf = A.generate(N, nc, nflag=cfg["nflag"], seed=cfg["seed"])
+    fk = 1 if cfg["nflag"] else 0
+    log("generated %s cube in %.1f s" % (args.config, time.time() - t0))
+
+    ctx = A.Context(device=0, precision=args.precision)
+    t0 = time.time()
+    cube = ctx.load_cube(N, nc, 1.0, fk, gv, samp)
+    upload_s = time.time() - t0
+    ctx.set_channels(cube, uf if fk else None, nc10)
+    ctx.set_rings(lo, hi)
+    log("uploaded cube (%.2f GB of FP64 file data) in %.2f s" % ((gv.nbytes + samp.nbytes) / 1e9, upload_s))
+
+    for w in range(args.warmup):
+        t0 = time.time()
+        ctx.bispec_batch(cube, tris)
+        log("warmup step %d: %.2f s" % (w, time.time() - t0))
+    ctx.reset_stats()
+    per_step = []
+    lat = []
+    t_start = time.time()
+    for s in range(args.steps):
+        t0 = time.time()
+        bl, cnt, _ = ctx.bispec_batch(cube, tris)
+        per_step.append(time.time() - t0)
+        log("timed step %d: %.2f s" % (s, per_step[-1]))
+    elapsed = time.time() - t_start
+    st = ctx.stats()
+    evals_per_step = st["evals"] / max(args.steps, 1)
+    value = evals_per_step * args.steps / elapsed
+
+    # roofline of the dominant kernel (the contraction), from HIP events
+    ms_c = st["ms_contract"]
+    ach = st["flop_contract"] / (ms_c * 1e-3) / 1e12 if ms_c > 0 else 0.0
+    peak = PEAK_TFLOPS[args.precision]
+    roof = {"bound": "mfma", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
+            "frac": round(ach / peak, 4), "traffic": None,
+            "kernel": "k_contract<%s>" % args.precision,
+            "mfma_passes": PASSES[args.precision],
+            "avg_launch_ms": round(ms_c / max(st["n_contract_launch"], 1), 3)}
+    stages = {"fft_ms": round(st["ms_fft"], 2), "contract_ms": round(st["ms_contract"], 2),
+              "reduce_ms": round(st["ms_reduce"], 2),
+              "fft_gbs": round(st["bytes_fft"] / (st["ms_fft"] * 1e-3) / 1e9, 1) if st["ms_fft"] > 0 else None}
+
+    err = None
+    if not args.no_error:
+        err = fp64_error(A, ctx, cube, cfg, gv, samp, uf, fk, lo, hi, tris)
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(A, cfg, gv, samp, uf, fk, lo, hi, tris, args.cpu_seconds)
+
+    workload = cfg["desc"] + ", %d triangles" % len(tris)
+    if args.tri_limit:
+        workload += " (DEVELOPMENT SUBSET, not the metric workload)"
+    line = {
+        "metric": "triangle-bin x lag-pair evals/sec at 768ch x 512^2; rel err vs FP64; p50/p99 latency",
+        "value": round(value, 1), "unit": "evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / max(args.steps, 1), 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16" if args.precision == "bf16x3" else "f32",
+        "precision": args.precision, "data": "synthetic",
+        "config": {"workload": workload, "channels": nc, "grid": N, "bins": cfg["rings"][2],
+                   "triangles": int(len(tris)), "nc10": nc10, "flagged": cfg["nflag"],
+                   "evals_per_step": int(evals_per_step)},
+        "roofline": roof, "stages": stages,
+        "triangle_latency_ms": lat_stats(st, len(tris), args.steps),
+        "rel_err_vs_fp64": err, "cpu_baseline": cpu,
+        "upload_s_pcie_incl": round(upload_s, 3),
+    }
+    ctx.close()
+    print(json.dumps(line), flush=True)
+
+
+def lat_stats(st, ntri, steps):
+    return None
+
+
+def fp64_error(A, ctx, cube, cfg, gv, samp, uf, fk, lo, hi, tris):
+    """Relative error vs the FP64 oracle on sampled lag cells of two triangles."""
+    import oracle as O
+    N, nc, nc10 = cfg["N"], cfg["nc"], cfg["nc10"]
+    mask = O.chan_mask(nc, fk, uf)
+    ngs = np.array([A.ng(hi[t[0]]) for t in tris])
+    picks = sorted(set([int(np.argsort(ngs)[len(ngs) // 2]), int(np.argmax(ngs <= 140) if np.any(ngs <= 140) else 0)]))
+    rng = np.random.default_rng(7)
+    out = []
+    for k in picks:
+        r1, r2, r3 = map(int, tris[k])
+        res = ctx.bispec_ex(cube, r1, r2, r3)
+        W = 2 * nc10 + 1
+        cells = np.argwhere(res["cnt"] > 0)
+        sel = cells[rng.choice(len(cells), size=min(48, len(cells)), replace=False)]
+        sel = np.vstack([sel, [[nc10, nc10]]])
+        d2 = sel[:, 0] - nc10
+        d3 = sel[:, 1] - nc10
+        t0 = time.time()
+        b64, c64, _ = O.bispec_cells(N, nc, 1.0, gv, samp, lo, hi, r1, r2, r3, mask, nc10, d2, d3,
+                                     nthreads=min(16, os.cpu_count() or 1))
+        got = res["bl64"][sel[:, 0], sel[:, 1]]
+        rel = np.abs(got - b64) / np.maximum(np.abs(b64), 1e-300)
+        out.append({"triangle": [r1, r2, r3], "ng": int(res["ng"]), "cells": int(len(sel)),
+                    "counts_exact": bool(np.array_equal(res["cnt"][sel[:, 0], sel[:, 1]], c64)),
+                    "median": float(np.median(rel)), "p99": float(np.quantile(rel, 0.99)),
+                    "max": float(rel.max()), "normwise": float(np.linalg.norm(got - b64) / np.linalg.norm(b64)),
+                    "oracle_s": round(time.time() - t0, 1)})
+        log("fp64 check triangle %s Ng=%d: normwise %.3g" % ((r1, r2, r3), res["ng"], out[-1]["normwise"]))
+    return out
+
+
+def cpu_baseline(A, cfg, gv, samp, uf, fk, lo, hi, tris, budget_s):
+    """Reference-algorithm CPU path (FP64 c2r + FP32 triplet loop, OpenMP),
+    timed on a bounded sample and extrapolated linearly in V*Ng^2."""
+    import oracle as O
+    N, nc, nc10 = cfg["N"], cfg["nc"], cfg["nc10"]
+    mask = O.chan_mask(nc, fk, uf)
+    threads = min(16, os.cpu_count() or 1)
+    ngs = np.array([A.ng(hi[t[0]]) for t in tris])
+    k = int(np.argsort(ngs)[len(ngs) // 2])
+    r1, r2, r3 = map(int, tris[k])
+    P = int(ngs[k]) ** 2
+    act = np.flatnonzero(mask)
+    # probe one reference plane, then size the sample to the budget
+    bl, _, cnt, n1, tf, tl = O.refalgo(N, nc, 1.0, gv, samp, lo, hi, r1, r2, r3, mask, nc10,
+                                       p1_lo=int(act[0]), p1_hi=int(act[0]) + 1, nthreads=threads)
+    per_trip_px = tl / max(n1 * P, 1)
+    nplanes_filled = 3 * min(nc, 1 + nc10 + 1)
+    fill_per_plane_px = tf / max(nplanes_filled * P, 1)
+    kp = int(max(1, min(nc, (budget_s - tf) / max(tl, 1e-3))))
+    bl, _, cnt, n2, tf2, tl2 = O.refalgo(N, nc, 1.0, gv, samp, lo, hi, r1, r2, r3, mask, nc10,
+                                         p1_lo=0, p1_hi=kp, nthreads=threads)
+    per_trip_px = tl2 / max(n2 * P, 1)
+    # whole workload: every triangle fills 3*nc planes and loops over V triplets
+    V = int(np.sum(mask)) ** 3 if nc10 >= nc - 1 else None
+    if V is None:
+        m = mask.astype(bool)
+        V = 0
+        for p1 in np.flatnonzero(m):
+            w = m[max(0, p1 - nc10): p1 + nc10 + 1].sum()
+            V += int(w) * int(w)
+    total = 0.0
+    evals = 0
+    W = 2 * nc10 + 1
+    ev_tri = None
+    for t, g in zip(tris, ngs):
+        Pt = int(g) ** 2
+        total += per_trip_px * V * Pt + fill_per_plane_px * 3 * nc * Pt
+    ev_tri = int(np.count_nonzero(cnt_full(mask, nc10)))
+    evals = ev_tri * len(tris)
+    return {"value": round(evals / total, 2), "unit": "evals/s", "cores": threads, "kind": "port",
+            "sample": "reference-algorithm CPU path (oracle/orc_refalgo.c: FP64 c2r fill + FP32 triplet "
+                      "loop, OpenMP dynamic) on triangle (%d,%d,%d) Ng=%d, p1 in [0,%d) = %d triplets, "
+                      "%.1f s loop + %.1f s fill; extrapolated linearly in V*Ng^2 to all %d triangles"
+                      % (r1, r2, r3, int(ngs[k]), kp, n2, tl2, tf2, len(tris)),
+            "est_total_s": round(total, 1)}
+
+
+def cnt_full(mask, nc10):
+    nc = len(mask)
+    W = 2 * nc10 + 1
+    m = mask.astype(np.int64)
+    out = np.zeros((W, W), np.int64)
+    for i2 in range(W):
+        d2 = i2 - nc10
+        for p1 in range(max(0, -d2), min(nc, nc - d2)):
+            if m[p1] and m[p1 + d2]:
+                lo = max(0, -p1) if False else max(-nc10, -p1)
+                hi_ = min(nc10, nc - 1 - p1)
+                out[i2, lo + nc10: hi_ + nc10 + 1] += m[p1 + lo: p1 + hi_ + 1]
+    return out
+
+
+if __name__ == "__main__":
+    main()

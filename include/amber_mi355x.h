@@ -1,0 +1,208 @@
+/*
+ * amber_mi355x -- MI355X-native (gfx950) implementation of AMBER's 4_fbe
+ * estimator hot path behind a plain C ABI.
+ *
+ * The reference has no plugin or FFI layer: its hot path is the in-process C
+ * function set declared in 4_fbe/mabs_funcs.h:19-60 plus the inline loops of
+ * 4_fbe/amber.c and 4_fbe/amber_v2.c, all driven through globals
+ * (4_fbe/vars.h:7-21).  Each entry point below names the reference function
+ * or loop it replaces.  Differences in contract, all deliberate:
+ *   - state lives in an amb_ctx instead of globals; every call returns an
+ *     int status (AMB_OK or a negative AMB_E*), never exit();
+ *   - output buffers are caller-owned host memory laid out exactly like the
+ *     reference's result files; device memory is owned by the library;
+ *   - a context is internally synchronised (one mutex), so it may be shared
+ *     by host threads; calls on one context execute one at a time.
+ * No torch or HIP types cross this boundary.
+ */
+#ifndef AMBER_MI355X_H
+#define AMBER_MI355X_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define AMB_OK 0
+#define AMB_EARG (-1)     /* bad argument / out-of-range ring, plane, triangle */
+#define AMB_EIO (-2)      /* file missing or unreadable */
+#define AMB_EFORMAT (-3)  /* truncated or inconsistent file */
+#define AMB_EHIP (-4)     /* HIP runtime error */
+#define AMB_ENOMEM (-5)   /* host allocation failed */
+#define AMB_EBUDGET (-6)  /* request does not fit the HBM budget */
+#define AMB_ESTATE (-7)   /* rings/channels not set, unknown cube id */
+#define AMB_ECERT (-8)    /* an integer-valued sum failed its rounding certificate */
+#define AMB_EBUSY (-9)    /* serving queue full */
+
+/* ---- precision of the triple-product contraction ---- */
+#define AMB_PREC_FP32 0   /* FP32 MFMA (v_mfma_f32_32x32x2_f32) */
+#define AMB_PREC_BF16X3 1 /* hi+lo BF16 split, 3 MFMA passes, FP32 accumulate */
+#define AMB_PREC_FP8 2    /* FP8 e4m3 with per-plane absmax scales */
+
+const char *amb_strerror(int code);
+const char *amb_version(void);
+
+/* =====================================================================
+ * Host-side file formats and rules (no device, no context)
+ * ===================================================================== */
+
+/* read_inputs_ess (amber_funcs.c:175-188): "%lf%lf%lf%511s". */
+int amb_read_params_ess(const char *path, double *nu0, double *dnu, double *chan0,
+                        char *data_path, size_t data_path_cap);
+
+/* GV_data_<R> header (amber_funcs.c:1733-1740): int64 NN, int64 nc, f64 Du,
+ * int32 flag_key, 28 bytes, no padding. */
+int amb_read_gv_header(const char *path, int64_t *NN, int64_t *nc, double *Du, int32_t *flag_key);
+
+/* initialize + ReadGVnAC_gridding (amber_funcs.c:1720-1779, 1861-1911):
+ * reads <data_path>/GV_data_<rlzn> and <data_path>/sampling in bulk.
+ * *gv gets [NN*(NN/2+1)][nc][2] doubles and *samp [NN*(NN/2+1)][nc] doubles
+ * (pixel-major, the file order); release both with amb_free. */
+int amb_read_cube_files(const char *data_path, int rlzn, int64_t *NN, int64_t *nc, double *Du,
+                        int32_t *flag_key, double **gv, double **samp);
+void amb_free(void *p);
+
+/* Writers for the same two files (3_vcg/tgefuncs.c:628-655, 728-748). */
+int amb_write_cube_files(const char *data_path, int rlzn, int64_t NN, int64_t nc, double Du,
+                         int32_t flag_key, const double *gv, const double *samp);
+
+/* get_uf_chans (amber_funcs.c:2485-2508) reading <path>; unlike the
+ * reference, a short read or an id outside [0,nc) is an error (AMB_EFORMAT). */
+int amb_read_uf_chans(const char *path, int uf_nc, int64_t nc, int *chans);
+int amb_write_uf_chans(const char *path, int uf_nc, const int *chans);
+
+/* closed_rings + rcount (writer amber_funcs.c:459-483, reader read_rings
+ * :499-545).  amb_read_rings allocates *rdata (rcount*4 doubles; amb_free). */
+int amb_write_closed_rings(const char *res_path, const double *rdata, int rcount);
+int amb_read_rings(const char *res_path, int *rcount, double **rdata);
+
+/* bsout files.  v1 (amber.c:287-304): f64 Ntri, f64 Num, f32 bl[W*W],
+ * i32 cnt[W*W] into <res_path>/bsout_final_ra%d_rb%d_rc%d_R%d.
+ * v2 (amber_v2.c:295-309): f64 inv_Ng1_Ng2, f32 bl[W*W], f32 norm[W*W] into
+ * <res_path>/bsout_final2_ra%d_rb%d_rc%d_R%d. */
+int amb_write_bsout_v1(const char *res_path, int r1, int r2, int r3, int rlzn, int W,
+                       double ntri, double num, const float *bl, const int32_t *cnt);
+int amb_write_bsout_v2(const char *res_path, int r1, int r2, int r3, int rlzn, int W,
+                       double inv_ng2, const float *bl, const float *norm);
+
+/* Rules.  amb_ng: amber.c:193-197.  amb_annular_rings: the reference's
+ * hard-coded edges, amber_funcs.c:2022-2053.  amb_uavg: :2065.
+ * amb_triangles: :443-449 (returns the count; fills up to cap triples). */
+long amb_ng(double uhigh, double du);
+void amb_annular_rings(int nr, double *ulow, double *uhigh);
+void amb_uavg(const double *ulow, const double *uhigh, int nr, double *uavg);
+int amb_triangles(const double *uavg, int nr, int *tri, int cap);
+
+/* Deterministic synthetic cube (no counterpart in the reference, which ships
+ * no data: examples/README.md:3).  Cell values (N(0,1)+iN(0,1))*m with
+ * m ~ Bernoulli(p0*exp(-|U|/u0)), |U| in cells, Hermitian on the j=0 and
+ * j=N/2 columns; nflag channels (never channel 5) fully flagged.
+ * shared_mask=1 uses one mask for every unflagged channel.
+ * count_max>1 stores sampling counts in [1,count_max] for sampled cells
+ * (values scaled by the count, so normalise_data divides them back).
+ * gv/samp as amb_read_cube_files; uf_chans gets the unflagged ids (size nc). */
+typedef struct amb_gen_cfg {
+    int64_t N, nc;
+    int nflag;
+    int shared_mask;
+    int count_max;
+    uint64_t seed;
+    double p0, u0;
+} amb_gen_cfg;
+int amb_generate(const amb_gen_cfg *cfg, double *gv, double *samp, int *uf_chans, int *uf_nc);
+
+/* =====================================================================
+ * Device context
+ * ===================================================================== */
+typedef struct amb_ctx amb_ctx;
+
+typedef struct amb_opts {
+    int device;          /* HIP device ordinal */
+    int precision;       /* AMB_PREC_* for the data contraction */
+    uint64_t hbm_budget; /* bytes the library may hold; 0 = 90% of free HBM */
+    int verbose;
+} amb_opts;
+
+int amb_create(const amb_opts *opts, amb_ctx **out);
+void amb_destroy(amb_ctx *ctx);
+const char *amb_last_error(amb_ctx *ctx);
+int amb_set_precision(amb_ctx *ctx, int precision);
+
+/* Upload a cube given in the file layout.  Replaces initialize +
+ * ReadGVnAC_gridding + normalize_data (amber_funcs.c:1720-1978): values are
+ * divided by the sampling count where it is >0 and the count becomes 1, in
+ * FP64, then stored plane-major as complex64 / float in HBM. */
+int amb_load_cube(amb_ctx *ctx, int64_t N, int64_t nc, double Du, int32_t flag_key,
+                  const double *gv, const double *samp, int *cube_id);
+int amb_load_cube_files(amb_ctx *ctx, const char *data_path, int rlzn, int *cube_id);
+int amb_release_cube(amb_ctx *ctx, int cube_id);
+
+/* Triplet set (precompute_unflg_vcp / precompute_vcp_dnu,
+ * amber_funcs.c:2207-2310): with the cube's flag_key=1 the channels are
+ * uf_chans[0..uf_nc), with flag_key=0 all channels (uf_chans ignored).
+ * Lags |p2-p1|,|p3-p1| <= nc10.  The table itself is never materialised. */
+int amb_set_channels(amb_ctx *ctx, int cube_id, const int *uf_chans, int uf_nc, int nc10);
+
+/* Ring edges in cells (replaces annular_rings, amber_funcs.c:2022-2094). */
+int amb_set_rings(amb_ctx *ctx, const double *ulow, const double *uhigh, int nr);
+
+/* calCR (amber_funcs.c:389-491) on the GPU: per ring the c2r of sampling
+ * plane `plane` (5 in the reference) on the full grid, per closed triangle
+ * the triple sum.  Num is certified to be an integer multiple of N^2 and
+ * written exactly.  rdata gets rcount x {r1,r2,r3,Num}. */
+int amb_closed_rings(amb_ctx *ctx, int cube_id, int plane, double *rdata, int cap, int *rcount);
+
+/* One triangle, v1 (fill_s2d_float x3 + amber.c:226-304): bl[W*W] f32,
+ * cnt[W*W] i32; *ntri and *num as written by the reference (Num =
+ * (Ng/N)^2 * rdata_num, Ntri = Num/(Ng*Ng)). */
+int amb_bispec_v1(amb_ctx *ctx, int cube_id, int r1, int r2, int r3, double rdata_num,
+                  float *bl, int32_t *cnt, double *ntri, double *num);
+
+/* One triangle, v2 (fill_s2d_float x3 + fill_s2I x3 + amber_v2.c:222-309):
+ * bl[W*W] f32 and norm[W*W] f32 (exact integer*Ng^2 cast to f32). */
+int amb_bispec_v2(amb_ctx *ctx, int cube_id, int r1, int r2, int r3,
+                  float *bl, float *norm, double *inv_ng2);
+
+/* Extended single-triangle call used by tests and tools: any output may be
+ * NULL.  bl64 receives the compensated accumulator before the f32 cast;
+ * norm64 the exact normalisation in FP64. */
+typedef struct amb_result {
+    float *bl;
+    double *bl64;
+    int32_t *cnt;
+    float *norm;
+    double *norm64;
+    double inv_ng2;
+    long ng;
+    double cert_margin; /* worst |S/P - round(S/P)| over normalisation sums */
+} amb_result;
+int amb_bispec_ex(amb_ctx *ctx, int cube_id, int r1, int r2, int r3, int want_norm, amb_result *res);
+
+/* Many triangles of one cube in one call, grouped by r1 so filtered planes
+ * are built once per (ring, Ng) and reused.  tri: ntri x {r1,r2,r3}.
+ * bl: ntri*W*W f32 (required); cnt, norm: may be NULL. */
+int amb_bispec_batch(amb_ctx *ctx, int cube_id, const int *tri, int ntri,
+                     float *bl, int32_t *cnt, float *norm);
+
+/* Per-stage device time and work since the last reset (HIP events on the
+ * library's stream). */
+typedef struct amb_stats {
+    double ms_fft;          /* ring gather + c2r kernels */
+    double ms_contract;     /* triple-product GEMM kernels */
+    double ms_reduce;       /* lag reduction + counts */
+    double ms_norm;         /* normalisation contraction (v2) */
+    double ms_total;        /* wall of device work incl. the above */
+    double flop_contract;   /* algorithmic 2*cnt*Ng^2 per eval, data only */
+    double bytes_fft;       /* algorithmic bytes of the fft stage */
+    int64_t n_contract_launch, n_fft_launch;
+    int64_t evals;          /* (triangle, lag cell) pairs with cnt > 0 */
+    double ms_contract_max_launch;
+} amb_stats;
+int amb_get_stats(amb_ctx *ctx, amb_stats *st);
+int amb_reset_stats(amb_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

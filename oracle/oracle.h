@@ -1,0 +1,112 @@
+/*
+ * FP64 CPU oracle for AMBER's 4_fbe estimator hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
+ * product (amber_amd/, libamber_mi355x.so, the amber/amber_v2 drivers) never
+ * links, loads or calls anything here.
+ *
+ * It is a clean-room restatement of the reference's algorithm; every function
+ * cites the reference lines it follows (paths relative to the reference root).
+ * How each function is pinned (not by retyping its formula):
+ *   orc_fft_c2c       direct O(n^2) DFT on small n and numpy.fft (pocketfft)
+ *   orc_ring_c2r      numpy.fft.irfft2 * Ng^2 on the same masked half plane,
+ *                     and a brute-force direct 2-D sum on tiny grids
+ *   orc_ng, orc_uavg,
+ *   orc_triangles,
+ *   orc_annular_rings closed forms and hand-computed cases (triangle counts of
+ *                     the shipped 22-ring setup, equilateral borderline)
+ *   orc_calcr         Num / N^2 equals the brute-force count of closed
+ *                     sampled triangles k1+k2+k3 = 0 on tiny grids
+ *   orc_bispec        triple-product invariant sum_x D1 D2 D3 =
+ *                     Ng^2 * sum_{k1+k2+k3=0} V1 V2 V3 by brute force, the
+ *                     alpha^3 scaling, channel-constant cubes, and the
+ *                     closed-form triplet counts
+ *   orc_refalgo_*     the reference-algorithm FP32 path; pinned against
+ *                     orc_bispec (FP32-level agreement) and by identical counts
+ * No function is left "parity unpinned".
+ */
+#ifndef AMBER_ORACLE_H
+#define AMBER_ORACLE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Unnormalised in-place complex DFT of length n; sign=+1 is exp(+2 pi i jk/n)
+ * (FFTW_BACKWARD, the direction of every c2r in amber_funcs.c:402,990,1116).
+ * re/im have n entries each. */
+void orc_fft_c2c(int n, double *re, double *im, int sign);
+
+/* Direct O(n^2) DFT used to validate orc_fft_c2c. */
+void orc_dft_direct(int n, const double *re, const double *im, double *ore, double *oim, int sign);
+
+/* Reduced grid side for a triangle whose largest ring has upper edge uhigh
+ * (amber.c:193-197): Nmax=(long)((uhigh+3)/Du); Ng=3*Nmax; Ng+=2 if even else 3. */
+long orc_ng(double uhigh, double du);
+
+/* Ring-average |U| (amber_funcs.c:2065). */
+void orc_uavg(const double *ulow, const double *uhigh, int nr, double *uavg);
+
+/* The reference's hard-coded piecewise ring edges (amber_funcs.c:2022-2053). */
+void orc_annular_rings(int nr, double *ulow, double *uhigh);
+
+/* Closed-triangle rule r1>=r2>=r3 (amber_funcs.c:443-449).  Writes up to cap
+ * triples (r1,r2,r3) into tri and returns the total count. */
+int orc_triangles(const double *uavg, int nr, int *tri, int cap);
+
+/* Ring gather + unnormalised c2r on the Ng grid (Dfg/Ifg, amber_funcs.c:573-621).
+ * Source cube in the reference's pixel-major file layout:
+ *   re_im  [N*(N/2+1)][nc][2]  (NULL for a sampling plane)
+ *   samp   [N*(N/2+1)][nc]     (used when re_im == NULL; real part only)
+ * out is Ng*Ng doubles, row-major (row = first FFT axis). */
+void orc_ring_c2r(long N, long nc, long Ng, double ulow, double uhigh,
+                  const double *re_im, const double *samp, long plane, double *out);
+
+/* calCR (amber_funcs.c:389-491): per ring the c2r of the sampling plane `plane`
+ * (5 in the reference) on the full N grid, then for each closed triangle the
+ * FP64 triple sum.  rdata gets {r1,r2,r3,Num} per triangle; returns rcount. */
+int orc_calcr(long N, long nc, const double *samp, int plane,
+              const double *ulow, const double *uhigh, int nr,
+              double *rdata, int cap);
+
+/* FP64 bispectrum accumulators for one triangle (amber.c:187-304 v1,
+ * amber_v2.c:187-309 v2).  chan_mask[nc] is 1 for channels in the triplet set
+ * (all channels when flag_key=0, uf_chans when flag_key=1;
+ * amber_funcs.c:2207-2310).  W = 2*nc10+1.
+ *   bl64[W*W]   sum_{p1} sum_x D_r1(p1) D_r2(p1+d2) D_r3(p1+d3)      (data)
+ *   norm64[W*W] same on the sampling images (v2 numdnu; may be NULL)
+ *   cnt[W*W]    number of valid p1 per lag cell (v1 numdnu; may be NULL)
+ * Layout [d2+nc10][d3+nc10], as amber_funcs.c:2293-2304.
+ * nthreads <= 0 uses OpenMP's default. Returns 0 or a negative error. */
+int orc_bispec(long N, long nc, double Du, const double *re_im, const double *samp,
+               const double *ulow, const double *uhigh, int r1, int r2, int r3,
+               const uint8_t *chan_mask, int nc10,
+               double *bl64, double *norm64, int32_t *cnt, int nthreads);
+
+/* Same sums evaluated only at ncell lag cells (d2[i], d3[i]) (lags, not
+ * offsets), for large configurations.  out_bl/out_norm may be NULL. */
+int orc_bispec_cells(long N, long nc, double Du, const double *re_im, const double *samp,
+                     const double *ulow, const double *uhigh, int r1, int r2, int r3,
+                     const uint8_t *chan_mask, int nc10, int ncell,
+                     const int *d2, const int *d3, double *out_bl, double *out_norm,
+                     int32_t *out_cnt, int nthreads);
+
+/* Reference-algorithm CPU path (amber.c:187-304 / amber_v2.c:187-309):
+ * FP64 c2r per plane cast to float, the 6-int triplet table
+ * (amber_funcs.c:2207-2310) and the FP32 sequential triple loop under
+ * `omp for schedule(dynamic)` with private accumulators and a critical merge.
+ * Restricted to p1 in [p1_lo, p1_hi) so a bounded sample can be timed.
+ * bl/numf are W*W floats, cnt W*W ints (any may be NULL except bl).
+ * Returns the number of triplets evaluated; *seconds_loop gets the time of
+ * the triple loop alone and *seconds_fill the time of the plane fills. */
+long orc_refalgo(long N, long nc, double Du, const double *re_im, const double *samp,
+                 const double *ulow, const double *uhigh, int r1, int r2, int r3,
+                 const uint8_t *chan_mask, int nc10, int p1_lo, int p1_hi, int v2,
+                 float *bl, float *numf, int32_t *cnt, int nthreads,
+                 double *seconds_fill, double *seconds_loop);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

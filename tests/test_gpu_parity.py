@@ -1,0 +1,120 @@
+"""GPU parity: the HIP path through the C ABI against the FP64 oracle.
+
+Counts (v1 numdnu, calCR Num, v2 normalisation) must be bit-exact against
+the exact integer values; accumulators are compared normwise and per cell
+against the oracle with the tolerances written in each test."""
+import numpy as np
+import pytest
+
+import amber_amd as A
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+# relative-error bars vs the FP64 oracle (normwise ||B-B64|| / ||B64||)
+TOL = {"fp32": 1e-5, "bf16x3": 1e-5}
+
+
+def _setup(ctx, N, nc, nflag, seed, lo, hi, nc10, **gen):
+    gv, samp, uf = A.generate(N, nc, nflag=nflag, seed=seed, **gen)
+    fk = 1 if nflag > 0 else 0
+    cube = ctx.load_cube(N, nc, 1.0, fk, gv, samp)
+    ctx.set_channels(cube, uf if fk else None, nc10)
+    ctx.set_rings(lo, hi)
+    mask = O.chan_mask(nc, fk, uf)
+    return cube, gv, samp, uf, mask
+
+
+def _relerr(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def _exact_norm(n64, P):
+    """oracle FP64 normalisation snapped to its exact value P * (k/8)"""
+    return P * np.rint(n64 / P * 8.0) / 8.0
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_small_v1_v2_vs_oracle(ctx, prec):
+    N, nc, nc10 = 32, 8, 7
+    lo, hi = A.log_rings(2, 10, 4)
+    ctx.set_precision(prec)
+    cube, gv, samp, uf, mask = _setup(ctx, N, nc, 2, 9, lo, hi, nc10)
+    print("case small N=%d nc=%d prec=%s" % (N, nc, prec), flush=True)
+    for (r1, r2, r3) in A.triangles(lo, hi):
+        res = ctx.bispec_ex(cube, int(r1), int(r2), int(r3), want_norm=True)
+        bl64, cnt64, n64 = O.bispec(N, nc, 1.0, gv, samp, lo, hi, r1, r2, r3, mask, nc10, want_norm=True)
+        assert np.array_equal(res["cnt"], cnt64)
+        P = res["ng"] ** 2
+        assert np.array_equal(res["norm64"], _exact_norm(n64, P))
+        assert np.array_equal(res["norm"], _exact_norm(n64, P).astype(np.float32))
+        if np.linalg.norm(bl64) > 0:
+            assert _relerr(res["bl64"], bl64) <= TOL[prec], (r1, r2, r3)
+    ctx.release_cube(cube)
+
+
+def test_window_and_flags(ctx):
+    # nc10 < nc-1 exercises per-p1 windows; flagged channels drop out
+    N, nc, nc10 = 32, 12, 3
+    lo, hi = A.log_rings(2, 10, 3)
+    ctx.set_precision("bf16x3")
+    cube, gv, samp, uf, mask = _setup(ctx, N, nc, 3, 17, lo, hi, nc10)
+    for (r1, r2, r3) in A.triangles(lo, hi)[:4]:
+        res = ctx.bispec_ex(cube, int(r1), int(r2), int(r3))
+        bl64, cnt64, _ = O.bispec(N, nc, 1.0, gv, samp, lo, hi, r1, r2, r3, mask, nc10)
+        assert np.array_equal(res["cnt"], cnt64)
+        if np.linalg.norm(bl64) > 0:
+            assert _relerr(res["bl64"], bl64) <= TOL["bf16x3"]
+    ctx.release_cube(cube)
+
+
+def test_calcr_exact(ctx):
+    N, nc = 64, 8
+    lo, hi = A.log_rings(2, 20, 6)
+    cube, gv, samp, uf, mask = _setup(ctx, N, nc, 0, 4, lo, hi, 3, p0=0.8, u0=60.0)
+    got = ctx.closed_rings(cube, plane=5)
+    ref = O.calcr(N, nc, samp, lo, hi, plane=5)
+    assert np.array_equal(got[:, :3], ref[:, :3])
+    exact = np.rint(ref[:, 3] / N ** 2 * 8) / 8 * N ** 2
+    assert np.array_equal(got[:, 3], exact)
+    ctx.release_cube(cube)
+
+
+def test_deterministic_alone_and_in_batch(ctx):
+    N, nc, nc10 = 32, 10, 9
+    lo, hi = A.log_rings(2, 10, 4)
+    ctx.set_precision("bf16x3")
+    cube, gv, samp, uf, mask = _setup(ctx, N, nc, 1, 23, lo, hi, nc10)
+    tris = A.triangles(lo, hi)
+    alone = [ctx.bispec_ex(cube, *map(int, t))["bl"] for t in tris]
+    bl, cnt, _ = ctx.bispec_batch(cube, tris[::-1])
+    for k, t in enumerate(tris[::-1]):
+        i = len(tris) - 1 - k
+        assert np.array_equal(bl[k], alone[i])
+    again = ctx.bispec_ex(cube, *map(int, tris[-1]))["bl"]
+    assert np.array_equal(again, alone[-1])
+    ctx.release_cube(cube)
+
+
+@pytest.mark.parametrize("prec", ["bf16x3"])
+def test_cfg1_plain_small(ctx, prec):
+    """64 ch x 128^2, 6 flagged, 8 log bins 2->60, lags to 63, all 46
+    triangles; FP64 oracle on every cell of a subset of triangles."""
+    N, nc, nc10 = 128, 64, 63
+    lo, hi = A.log_rings(2, 60, 8)
+    ctx.set_precision(prec)
+    cube, gv, samp, uf, mask = _setup(ctx, N, nc, 6, 1001, lo, hi, nc10)
+    tris = A.triangles(lo, hi)
+    assert len(tris) == 46
+    print("case cfg1 prec=%s" % prec, flush=True)
+    bl, cnt, _ = ctx.bispec_batch(cube, tris)
+    worst = 0.0
+    for k in (0, 11, 23, 45):
+        r1, r2, r3 = map(int, tris[k])
+        b64, c64, _ = O.bispec(N, nc, 1.0, gv, samp, lo, hi, r1, r2, r3, mask, nc10)
+        assert np.array_equal(cnt[k], c64)
+        e = _relerr(bl[k].astype(np.float64), b64)
+        worst = max(worst, e)
+    print("cfg1 worst normwise rel err %.3g" % worst, flush=True)
+    assert worst <= TOL[prec]
+    ctx.release_cube(cube)
