@@ -31,7 +31,7 @@ EXPORTS = [
     "amb_triangles", "amb_generate", "amb_create", "amb_destroy", "amb_last_error", "amb_set_precision",
     "amb_load_cube", "amb_load_cube_files", "amb_release_cube", "amb_set_channels", "amb_set_rings",
     "amb_closed_rings", "amb_bispec_v1", "amb_bispec_v2", "amb_bispec_ex", "amb_bispec_batch",
-    "amb_get_stats", "amb_reset_stats",
+    "amb_get_stats", "amb_reset_stats", "amb_drop_planes", "amb_batch_times",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -83,6 +83,8 @@ _lib.amb_bispec_v1.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C
 _lib.amb_bispec_v2.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _fp, _fp, _dp]
 _lib.amb_bispec_ex.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(Result)]
 _lib.amb_bispec_batch.argtypes = [C.c_void_p, C.c_int, _ip, C.c_int, _fp, _ip, _fp]
+_lib.amb_drop_planes.argtypes = [C.c_void_p, C.c_int]
+_lib.amb_batch_times.argtypes = [C.c_void_p, _dp, C.c_int, _ip]
 _lib.amb_get_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
 _lib.amb_reset_stats.argtypes = [C.c_void_p]
 _lib.amb_uavg.argtypes = [_dp, _dp, C.c_int, _dp]
@@ -337,15 +339,31 @@ class Context:
         res.update(inv_ng2=r.inv_ng2, ng=r.ng, cert_margin=r.cert_margin)
         return res
 
-    def bispec_batch(self, cube, tris, want_cnt=True, want_norm=False):
+    def drop_planes(self, cube):
+        self._c(_lib.amb_drop_planes(self._h, cube))
+
+    def bispec_batch(self, cube, tris, want_cnt=True, want_norm=False, out=None):
+        """All triangles of one cube; out=(bl, cnt, norm) reuses caller arrays."""
         W = self._W(cube)
         tris = np.ascontiguousarray(tris, np.int32).reshape(-1, 3)
         n = len(tris)
-        bl = np.zeros((n, W, W), np.float32)
-        cnt = np.zeros((n, W, W), np.int32) if want_cnt else None
-        norm = np.zeros((n, W, W), np.float32) if want_norm else None
+        if out is not None:
+            bl, cnt, norm = out
+            assert bl.shape == (n, W, W) and bl.dtype == np.float32 and bl.flags.c_contiguous
+        else:
+            bl = np.zeros((n, W, W), np.float32)
+            cnt = np.zeros((n, W, W), np.int32) if want_cnt else None
+            norm = np.zeros((n, W, W), np.float32) if want_norm else None
         self._c(_lib.amb_bispec_batch(self._h, cube, _p(tris, _ip), n, _p(bl, _fp), _p(cnt, _ip), _p(norm, _fp)))
         return bl, cnt, norm
+
+    def batch_times(self):
+        """completion time (ms from the call's start) of each triangle of the last batch"""
+        n = C.c_int()
+        self._c(_lib.amb_batch_times(self._h, None, 0, C.byref(n)))
+        out = np.zeros(max(n.value, 1))
+        self._c(_lib.amb_batch_times(self._h, _p(out, _dp), n.value, C.byref(n)))
+        return out[: n.value]
 
     def stats(self):
         s = Stats()

@@ -17,9 +17,11 @@
 #include "amber_mi355x.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -47,7 +49,7 @@ struct Cube {
 struct PlaneSet {
     int cube, kind, ng;
     double lo, hi;
-    float *d = nullptr;
+    void *d = nullptr;
     size_t bytes = 0;
     uint64_t last = 0;
     int pins = 0;
@@ -61,7 +63,7 @@ struct Timer {
 } // namespace
 
 struct amb_ctx {
-    int device = 0, prec = AMB_PREC_BF16X3, verbose = 0;
+    int device = 0, prec = AMB_PREC_BF16X3, verbose = 0, sync_debug = 0, kvar = 0, no_sym = 0;
     uint64_t budget = 0, used = 0;
     hipStream_t stream = nullptr;
     std::mutex mu;
@@ -81,6 +83,7 @@ struct amb_ctx {
     void *out = nullptr; size_t out_bytes = 0;
     /* stats */
     amb_stats st{};
+    std::vector<double> batch_ms; /* per-triangle completion time of the last batch */
     std::vector<Timer> timers;
     std::vector<hipEvent_t> evpool;
 };
@@ -254,8 +257,42 @@ static int ring_geom(amb_ctx *c, const Cube &cb, int r, int ng, RingGeom *g)
     return AMB_OK;
 }
 
-/* filtered planes of (cube, kind, ring) at grid side ng, pinned on return */
-static int get_planes(amb_ctx *c, int cube_id, Cube &cb, int kind, int r, int ng, float **out, int *slot)
+static long plane_pitch(long ng) { return (ng * ng + 63) / 64 * 64; }
+
+/* Plane sets of (cube, ring) at grid side ng, pinned on return.
+ * kind 0: data images (FP32), 1: sampling images (FP32),
+ * 2 / 3: hi and lo BF16 halves of kind 0 / 1, stored hi planes then lo,
+ * 4: e4m3 data images times their B-side scales,
+ * 5: per-plane FP8 scales of kind 0 (A side [nact], then B side [nact]). */
+static int get_planes(amb_ctx *c, int cube_id, Cube &cb, int kind, int r, int ng, void **out);
+
+static int get_fp8(amb_ctx *c, int cube_id, Cube &cb, int kind, int r, int ng, void **out, const RingGeom &g)
+{
+    const int nact = cb.nact();
+    const size_t pitch = plane_pitch(ng);
+    void *src = nullptr, *sc = nullptr;
+    int rc = get_planes(c, cube_id, cb, 0, r, ng, &src);
+    if (rc) return rc;
+    if (kind == 4 && (rc = get_planes(c, cube_id, cb, 5, r, ng, &sc))) return rc;
+    const size_t bytes = kind == 4 ? pitch * nact : sizeof(float) * 2 * nact;
+    void *d = nullptr;
+    if ((rc = dev_alloc(c, &d, bytes, "fp8 planes"))) return rc;
+    hipError_t e = kind == 4
+        ? amb_launch_fp8_quant(c->stream, (const float *)src, (long)pitch, nact, (const float *)sc + nact, (uint8_t *)d)
+        : amb_launch_fp8_scales(c->stream, (const float *)src, (long)pitch, nact, (float *)d);
+    if (e != hipSuccess) {
+        dev_free(c, d, bytes);
+        return fail(c, AMB_EHIP, "fp8 preparation: %s", hipGetErrorString(e));
+    }
+    PlaneSet p;
+    p.cube = cube_id; p.kind = kind; p.ng = ng; p.lo = g.lo; p.hi = g.hi;
+    p.d = d; p.bytes = bytes; p.last = ++c->tick; p.pins = 1;
+    c->planes.push_back(p);
+    *out = d;
+    return AMB_OK;
+}
+
+static int get_planes(amb_ctx *c, int cube_id, Cube &cb, int kind, int r, int ng, void **out)
 {
     RingGeom g;
     int rc = ring_geom(c, cb, r, ng, &g);
@@ -266,45 +303,52 @@ static int get_planes(amb_ctx *c, int cube_id, Cube &cb, int kind, int r, int ng
             p.last = ++c->tick;
             p.pins++;
             *out = p.d;
-            *slot = (int)i;
             return AMB_OK;
         }
     }
+    if (kind >= 4) return get_fp8(c, cube_id, cb, kind, r, ng, out, g);
     const int nact = cb.nact();
-    const size_t P = (size_t)ng * ng;
+    const size_t P = (size_t)ng * ng, pitch = plane_pitch(ng);
+    const size_t bytes = sizeof(float) * pitch * nact; /* FP32, or hi+lo BF16 */
+    void *src = nullptr;
+    if (kind >= 2 && (rc = get_planes(c, cube_id, cb, kind - 2, r, ng, &src))) return rc;
     float2 *tw = nullptr;
     FftPlan pl;
-    rc = twiddles(c, ng, &tw, &pl);
-    if (rc) return rc;
-    size_t gneed = sizeof(float2) * (size_t)nact * (size_t)std::max(g.jcnt, 1) * ng;
-    rc = scratch(c, &c->G, &c->G_bytes, gneed, "fft scratch");
-    if (rc) return rc;
-    void *d = nullptr;
-    rc = dev_alloc(c, &d, sizeof(float) * P * nact, "filtered planes");
-    if (rc) return rc;
-    timer_start(c, &c->st.ms_fft);
-    hipError_t e = amb_launch_ring_c2r(c->stream, kind, cb.vis_t, cb.samp_t, cb.d_act, nact, (int)cb.N,
-                                       (int)(cb.N / 2 + 1), pl, tw, g, c->G, (float *)d);
-    timer_stop(c);
-    if (e != hipSuccess) {
-        dev_free(c, d, sizeof(float) * P * nact);
-        return fail(c, AMB_EHIP, "ring c2r launch: %s", hipGetErrorString(e));
+    if (kind < 2) {
+        if ((rc = twiddles(c, ng, &tw, &pl))) return rc;
+        size_t gneed = sizeof(float2) * (size_t)nact * (size_t)std::max(g.jcnt, 1) * ng;
+        if ((rc = scratch(c, &c->G, &c->G_bytes, gneed, "fft scratch"))) return rc;
     }
-    c->st.n_fft_launch += 2;
-    /* algorithmic bytes: Ng^2 floats written per plane plus the ring cells read */
-    double cells = 0;
-    for (long ia = -(long)ceil(g.hi); ia <= (long)ceil(g.hi); ++ia)
-        for (long j = 0; j < g.jcnt; ++j) {
-            double uu = sqrt((double)(ia * ia + j * j));
-            if (uu >= g.lo && uu < g.hi && uu > 0) cells += 1;
-        }
-    c->st.bytes_fft += (double)nact * (4.0 * (double)P + (kind == 0 ? 8.0 : 4.0) * cells);
+    void *d = nullptr;
+    if ((rc = dev_alloc(c, &d, bytes, "filtered planes"))) return rc;
+    hipError_t e;
+    if (kind < 2) {
+        timer_start(c, &c->st.ms_fft);
+        e = amb_launch_ring_c2r(c->stream, kind, cb.vis_t, cb.samp_t, cb.d_act, nact, (int)cb.N,
+                                (int)(cb.N / 2 + 1), pl, tw, g, c->G, (float *)d, (long)pitch);
+        timer_stop(c);
+        c->st.n_fft_launch += 2;
+        /* algorithmic bytes: Ng^2 floats written per plane plus the ring cells read */
+        double cells = 0;
+        for (long ia = -(long)ceil(g.hi); ia <= (long)ceil(g.hi); ++ia)
+            for (long j = 0; j < g.jcnt; ++j) {
+                double uu = sqrt((double)(ia * ia + j * j));
+                if (uu >= g.lo && uu < g.hi && uu > 0) cells += 1;
+            }
+        c->st.bytes_fft += (double)nact * (4.0 * (double)P + (kind == 0 ? 8.0 : 4.0) * cells);
+    } else {
+        __bf16 *h = (__bf16 *)d;
+        e = amb_launch_split(c->stream, (const float *)src, pitch * nact, h, h + pitch * nact);
+    }
+    if (e != hipSuccess) {
+        dev_free(c, d, bytes);
+        return fail(c, AMB_EHIP, "plane build (kind %d): %s", kind, hipGetErrorString(e));
+    }
     PlaneSet p;
     p.cube = cube_id; p.kind = kind; p.ng = ng; p.lo = g.lo; p.hi = g.hi;
-    p.d = (float *)d; p.bytes = sizeof(float) * P * nact; p.last = ++c->tick; p.pins = 1;
+    p.d = d; p.bytes = bytes; p.last = ++c->tick; p.pins = 1;
     c->planes.push_back(p);
-    *out = p.d;
-    *slot = (int)c->planes.size() - 1;
+    *out = d;
     return AMB_OK;
 }
 
@@ -347,6 +391,9 @@ int amb_create(const amb_opts *o, amb_ctx **out)
     if (hipSetDevice(c->device) != hipSuccess) { delete c; return AMB_EHIP; }
     size_t fr = 0, tot = 0;
     if (hipMemGetInfo(&fr, &tot) != hipSuccess) { delete c; return AMB_EHIP; }
+    c->sync_debug = getenv("AMB_SYNC_DEBUG") != nullptr;
+    if (const char *v = getenv("AMB_KVAR")) c->kvar = atoi(v) & (AMB_NVAR - 1);
+    c->no_sym = getenv("AMB_NO_SYM") != nullptr; /* for A/B checks of the symmetry */
     c->budget = (o && o->hbm_budget) ? o->hbm_budget : (uint64_t)(0.9 * (double)fr);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return AMB_EHIP; }
     void *m = nullptr;
@@ -383,7 +430,6 @@ int amb_set_precision(amb_ctx *c, int prec)
     if (!c) return AMB_EARG;
     std::lock_guard<std::mutex> g(c->mu);
     if (prec < AMB_PREC_FP32 || prec > AMB_PREC_FP8) return fail(c, AMB_EARG, "precision %d", prec);
-    if (prec == AMB_PREC_FP8) return fail(c, AMB_EARG, "FP8 contraction is not available in this build");
     c->prec = prec;
     return AMB_OK;
 }
@@ -613,7 +659,7 @@ int amb_closed_rings(amb_ctx *c, int id, int plane, double *rdata, int cap, int 
     /* calCR's gather (amber_funcs.c:419-429) is the Dfg gather at Ng = N */
     for (int r = 0; r < nr && e == hipSuccess; ++r)
         e = amb_launch_ring_c2r(c->stream, 1, cb->vis_t, cb->samp_t, (const int *)dp, 1, N, N / 2 + 1, pl, tw,
-                                geo[r], c->G, (float *)II + (size_t)r * P);
+                                geo[r], c->G, (float *)II + (size_t)r * P, (long)P);
     timer_stop(c);
     if (e == hipSuccess) e = amb_launch_triple_sum(c->stream, (const float *)II, (long)P, (const int *)dtri, ntri, (double *)dnum);
     std::vector<double> num(ntri);
@@ -661,20 +707,33 @@ static int bispec_core(amb_ctx *c, int id, int r1, int r2, int r3, int want_norm
     res->inv_ng2 = 1.0 / (ng * ng * 1.);
     res->cert_margin = 0;
 
-    int prec = c->prec;
-    float *D[3], *S[3] = {nullptr, nullptr, nullptr};
-    int slot;
-    int rr[3] = {r1, r2, r3};
-    for (int k = 0; k < 3; ++k)
-        if ((rc = get_planes(c, id, *cb, 0, rr[k], (int)ng, &D[k], &slot))) { unpin_all(c); return rc; }
+    const int prec = c->prec;
+    const int nprec = prec == AMB_PREC_FP32 ? AMB_PREC_FP32 : AMB_PREC_BF16X3; /* normalisation form */
+    const long pitch = plane_pitch(ng);
+    /* D/S: FP32 images of r1, r2, r3; the third operand in the form the
+       contraction reads (FP32, or the hi/lo BF16 split) */
+    void *D[3], *S[3] = {nullptr, nullptr, nullptr}, *SC[3] = {nullptr, nullptr, nullptr};
+    const int rr[3] = {r1, r2, r3};
+    const int d3kind = prec == AMB_PREC_FP32 ? 0 : prec == AMB_PREC_BF16X3 ? 2 : 4;
+    for (int k = 0; k < 3; ++k) {
+        const int kind = k == 2 ? d3kind : 0;
+        if ((rc = get_planes(c, id, *cb, kind, rr[k], (int)ng, &D[k]))) { unpin_all(c); return rc; }
+        if (prec == AMB_PREC_FP8 && (rc = get_planes(c, id, *cb, 5, rr[k], (int)ng, &SC[k]))) {
+            unpin_all(c);
+            return rc;
+        }
+    }
     if (want_norm)
-        for (int k = 0; k < 3; ++k)
-            if ((rc = get_planes(c, id, *cb, 1, rr[k], (int)ng, &S[k], &slot))) { unpin_all(c); return rc; }
+        for (int k = 0; k < 3; ++k) {
+            const int kind = (k == 2 && nprec != AMB_PREC_FP32) ? 3 : 1;
+            if ((rc = get_planes(c, id, *cb, kind, rr[k], (int)ng, &S[k]))) { unpin_all(c); return rc; }
+        }
 
     const size_t nC = (size_t)nact * nact * nact;
     if ((rc = scratch(c, &c->C, &c->C_bytes, sizeof(float) * nC, "contraction partials"))) { unpin_all(c); return rc; }
     if (want_norm && (rc = scratch(c, &c->Cq, &c->Cq_bytes, sizeof(int) * nC, "normalisation partials"))) { unpin_all(c); return rc; }
-    const size_t outb = cells * (sizeof(float) * 2 + sizeof(double) * 2 + sizeof(int));
+    /* bl, norm | pad to 8 B | bl64, norm64 | cnt */
+    const size_t outb = cells * (sizeof(float) * 2 + sizeof(double) * 2 + sizeof(int)) + sizeof(float) * (cells & 1);
     if ((rc = scratch(c, (char **)&c->out, &c->out_bytes, outb, "lag arrays"))) { unpin_all(c); return rc; }
     float *d_bl = (float *)c->out;
     float *d_norm = d_bl + cells;
@@ -683,43 +742,67 @@ static int bispec_core(amb_ctx *c, int id, int r1, int r2, int r3, int want_norm
     int *d_cnt = (int *)(d_norm64 + cells);
 
     ContractArgs a{};
-    a.P = P; a.nact = nact; a.wlo = cb->d_wlo; a.whi = cb->d_whi;
-    a.D1 = D[0]; a.D2 = D[1]; a.D3 = D[2]; a.C = c->C;
+    /* r1 == r2 symmetry needs every p3 of p1's window inside p2's window */
+    const bool full_window = nact > 0 && cb->act.back() - cb->act.front() <= cb->nc10;
+    const int sym12 = (r1 == r2 && full_window && !c->no_sym) ? 1 : 0;
+    a.P = P; a.pitch = pitch; a.nact = nact; a.wlo = cb->d_wlo; a.whi = cb->d_whi; a.sym12 = sym12;
+    a.D1 = (const float *)D[0]; a.D2 = (const float *)D[1]; a.C = c->C;
+    if (prec == AMB_PREC_FP32) a.D3 = (const float *)D[2];
+    else if (prec == AMB_PREC_BF16X3) { a.H3 = (const __bf16 *)D[2]; a.L3 = a.H3 + (size_t)pitch * nact; }
+    else {
+        a.Q3 = (const uint8_t *)D[2];
+        a.s1 = (const float *)SC[0];
+        a.s2 = (const float *)SC[1];
+        a.t3 = (const float *)SC[2] + nact;
+    }
     hipError_t e = hipSuccess;
+    const char *step = "none";
+#define STEP(name, expr)                                              \
+    if (e == hipSuccess) {                                            \
+        step = name;                                                  \
+        e = (expr);                                                   \
+        if (e == hipSuccess && c->sync_debug) e = hipStreamSynchronize(c->stream); \
+    }
     timer_start(c, &c->st.ms_contract);
-    e = amb_launch_contract(c->stream, prec, 0, a);
+    STEP("contract", amb_launch_contract(c->stream, prec, 0, c->kvar, a));
     timer_stop(c);
     c->st.n_contract_launch++;
-    if (e == hipSuccess && want_norm) {
+    if (want_norm) {
         ContractArgs b = a;
-        b.D1 = S[0]; b.D2 = S[1]; b.D3 = S[2]; b.C = nullptr; b.Cq = c->Cq; b.margin = c->margin;
-        e = hipMemsetAsync(c->margin, 0, sizeof(unsigned), c->stream);
+        b.D1 = (const float *)S[0]; b.D2 = (const float *)S[1]; b.C = nullptr; b.Cq = c->Cq; b.margin = c->margin;
+        b.D3 = nullptr; b.H3 = b.L3 = nullptr;
+        if (nprec == AMB_PREC_FP32) b.D3 = (const float *)S[2];
+        else { b.H3 = (const __bf16 *)S[2]; b.L3 = b.H3 + (size_t)pitch * nact; }
+        STEP("margin reset", hipMemsetAsync(c->margin, 0, sizeof(unsigned), c->stream));
         timer_start(c, &c->st.ms_norm);
-        if (e == hipSuccess) e = amb_launch_contract(c->stream, prec == AMB_PREC_FP32 ? AMB_PREC_FP32 : AMB_PREC_BF16X3, 1, b);
+        STEP("norm contract", amb_launch_contract(c->stream, nprec, 1, c->kvar, b));
         timer_stop(c);
     }
     ReduceArgs ra{};
     ra.C = c->C; ra.Cq = want_norm ? c->Cq : nullptr; ra.act = cb->d_act; ra.inv = cb->d_inv;
-    ra.nc = (int)cb->nc; ra.nact = nact; ra.nc10 = cb->nc10; ra.P = P;
+    ra.nc = (int)cb->nc; ra.nact = nact; ra.nc10 = cb->nc10; ra.P = P; ra.sym12 = sym12;
     ra.bl = d_bl; ra.bl64 = d_bl64; ra.cnt = d_cnt;
     ra.norm = want_norm ? d_norm : nullptr; ra.norm64 = want_norm ? d_norm64 : nullptr;
     timer_start(c, &c->st.ms_reduce);
-    if (e == hipSuccess) e = amb_launch_reduce(c->stream, ra);
+    STEP("reduce", amb_launch_reduce(c->stream, ra));
     timer_stop(c);
     std::vector<int> hcnt;
     int32_t *cnt_dst = res->cnt;
     if (!cnt_dst) { hcnt.resize(cells); cnt_dst = hcnt.data(); }
-    if (e == hipSuccess && res->bl) e = hipMemcpyAsync(res->bl, d_bl, sizeof(float) * cells, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess && res->bl64) e = hipMemcpyAsync(res->bl64, d_bl64, sizeof(double) * cells, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(cnt_dst, d_cnt, sizeof(int) * cells, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess && want_norm && res->norm) e = hipMemcpyAsync(res->norm, d_norm, sizeof(float) * cells, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess && want_norm && res->norm64) e = hipMemcpyAsync(res->norm64, d_norm64, sizeof(double) * cells, hipMemcpyDeviceToHost, c->stream);
+    if (res->bl) STEP("copy bl", hipMemcpyAsync(res->bl, d_bl, sizeof(float) * cells, hipMemcpyDeviceToHost, c->stream));
+    if (res->bl64) STEP("copy bl64", hipMemcpyAsync(res->bl64, d_bl64, sizeof(double) * cells, hipMemcpyDeviceToHost, c->stream));
+    STEP("copy cnt", hipMemcpyAsync(cnt_dst, d_cnt, sizeof(int) * cells, hipMemcpyDeviceToHost, c->stream));
+    if (want_norm && res->norm) STEP("copy norm", hipMemcpyAsync(res->norm, d_norm, sizeof(float) * cells, hipMemcpyDeviceToHost, c->stream));
+    if (want_norm && res->norm64) STEP("copy norm64", hipMemcpyAsync(res->norm64, d_norm64, sizeof(double) * cells, hipMemcpyDeviceToHost, c->stream));
     unsigned mbits = 0;
-    if (e == hipSuccess && want_norm) e = hipMemcpyAsync(&mbits, c->margin, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (want_norm) STEP("copy margin", hipMemcpyAsync(&mbits, c->margin, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+    STEP("sync", hipStreamSynchronize(c->stream));
+#undef STEP
     timers_collect(c);
     unpin_all(c);
-    if (e != hipSuccess) return fail(c, AMB_EHIP, "bispectrum (%d,%d,%d): %s", r1, r2, r3, hipGetErrorString(e));
+    if (e != hipSuccess)
+        return fail(c, AMB_EHIP, "bispectrum (%d,%d,%d) Ng=%ld nact=%d step %s: %s", r1, r2, r3, ng, nact, step,
+                    hipGetErrorString(e));
     int64_t ev = 0;
     double pairs = 0;
     for (size_t i = 0; i < cells; ++i) {
@@ -793,6 +876,8 @@ int amb_bispec_batch(amb_ctx *c, int id, const int *tri, int ntri, float *bl, in
     std::vector<int> order(ntri);
     for (int i = 0; i < ntri; ++i) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return tri[3 * x] < tri[3 * y]; });
+    const auto t0 = std::chrono::steady_clock::now();
+    c->batch_ms.assign(ntri, 0.0);
     for (int k = 0; k < ntri; ++k) {
         int i = order[k];
         amb_result r{};
@@ -801,7 +886,33 @@ int amb_bispec_batch(amb_ctx *c, int id, const int *tri, int ntri, float *bl, in
         r.norm = norm ? norm + (size_t)i * cells : nullptr;
         rc = bispec_core(c, id, tri[3 * i], tri[3 * i + 1], tri[3 * i + 2], norm != nullptr, &r);
         if (rc) return rc;
+        c->batch_ms[i] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
+    return AMB_OK;
+}
+
+int amb_batch_times(amb_ctx *c, double *ms, int cap, int *n)
+{
+    if (!c || !n) return AMB_EARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    *n = (int)c->batch_ms.size();
+    for (int i = 0; i < *n && i < cap && ms; ++i) ms[i] = c->batch_ms[i];
+    return AMB_OK;
+}
+
+int amb_drop_planes(amb_ctx *c, int id)
+{
+    if (!c) return AMB_EARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    Cube *cb;
+    int rc = find_cube(c, id, &cb);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < c->planes.size();)
+        if (c->planes[i].cube == id) {
+            dev_free(c, c->planes[i].d, c->planes[i].bytes);
+            c->planes.erase(c->planes.begin() + i);
+        } else ++i;
     return AMB_OK;
 }
 

@@ -5,18 +5,32 @@
  *   C[p1][p2][p3] = sum_x (D_r1[p1](x) * D_r2[p2](x)) * D_r3[p3](x)
  *
  * i.e. (D_r1[p1] o D_r2) . D_r3^T over pixels, for p2, p3 in the lag window
- * of p1.  A workgroup owns a 128x128 (p2, p3) tile of one p1; its 4 waves
- * each hold 64x64 of the tile in MFMA accumulators (2x2 blocks of 32x32).
- * The Hadamard product with D_r1[p1] is formed while staging the D_r2 rows
- * into LDS, so the A operand is never materialised in HBM.  Workgroups are
- * placed so that the ones resident on an XCD share their (p2, p3) tile and
- * differ only in p1: the D_r2/D_r3 K-chunks they stream are then L2 hits.
+ * of p1 (compact channel indices).
+ *
+ * Work decomposition.  A workgroup owns one 128x128 (p2, p3) tile and 4
+ * reference planes p1; each plane's 128x128 block lives in the MFMA
+ * accumulators of one wave (4x4 blocks of 32x32) or of two waves (2x4 each,
+ * the 8-wave variant).  Per 32-pixel chunk the
+ * workgroup stages the D_r2 rows (FP32), the D_r3 rows (pre-split hi/lo BF16
+ * or FP32) and its 4 D_r1 rows into LDS once; every wave then forms its own
+ * A operand D_r1[p1] o D_r2 in registers (multiply, split) and runs the MFMAs
+ * against the shared B operand, so the staged bytes feed 4x the MFMA work of
+ * a one-plane tile.  LDS is double-buffered and the next chunk's global loads
+ * are in flight while the current chunk is multiplied.  Workgroups are
+ * numbered tile-major and dealt to the 8 XCDs in contiguous runs, so the
+ * workgroups resident on one XCD share their (p2, p3) tile and stream the
+ * same D_r2 / D_r3 chunks through that XCD's L2.
  *
  * Precisions (the accumulate is FP32 in every case):
  *   FP32   v_mfma_f32_32x32x2_f32 on FP32 operands
- *   BF16x3 operands split a = hi + lo (hi = bf16(a), lo = bf16(a - hi));
- *          3 passes lo*hi + hi*lo + hi*hi of v_mfma_f32_32x32x16_bf16
- *   FP8    e4m3 with per-plane absmax scales, v_mfma_scale_f32_32x32x64_f8f6f4
+ *   BF16x3 a = hi + lo (hi = bf16(a), lo = bf16(a - hi)); three passes
+ *          lo*hi + hi*lo + hi*hi of v_mfma_f32_32x32x16_bf16
+ *   FP8    e4m3 operands with per-plane power-of-two absmax scales: the A
+ *          operand D_r1[p1] o D_r2[p2] is quantised as
+ *          (D_r1 s1[p1]) (D_r2 s2[p2]) with s = 2^floor(log2(sqrt(448)/max|D|)),
+ *          the B operand as D_r3 t[p3] with t = 2^floor(log2(448/max|D|)),
+ *          one v_mfma_f32_32x32x16_fp8_fp8 per 16 pixels, and the epilogue
+ *          divides by s1 s2 t exactly
  *
  * The sums over p1 (amber.c:265, private_bldnu[idx] += Bispec) happen in the
  * reduction kernel below, in ascending p1 order with compensated FP32
@@ -25,166 +39,461 @@
  * with an epilogue that rounds S/P to the nearest 1/8 (the exact value is a
  * count of closed sampled triangles, a multiple of 1/8 even when a j=0 column
  * is not Hermitian) and records the worst rounding distance as a certificate.
+ *
+ * Planes are stored with a pitch that is a multiple of 32 elements and a
+ * zero tail, so every chunk load is aligned and in bounds.
  */
 #include "amb_internal.h"
 #include "amber_mi355x.h"
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
-typedef int i32x8 __attribute__((ext_vector_type(8)));
 
-#define TM 128
-#define TN 128
-#define KT 32
+#define QP 4     /* reference planes per workgroup */
+#define TT 128   /* p2 rows = p3 columns per workgroup */
+#define KC 32    /* pixels per LDS chunk */
+#define PADF 4   /* FP32 row padding (floats) */
+#define PADH 8   /* BF16 row padding (elements) */
 
-template <int MODE, int NORM>
-__global__ __launch_bounds__(256) void k_contract(ContractArgs a, int nt3, int per_xcd, int nwork)
+/* Variants (chosen per shape by the autotuner):
+ *   NWR = 1: 4 waves, wave w holds plane w and all 128 rows (4x4 blocks)
+ *   NWR = 2: 8 waves, wave w holds plane w%4 and rows 64*(w/4)..+64 (2x4)
+ *   PIPE = 0: a k-step forms all its fragments, then multiplies
+ *   PIPE = 1: (BF16x3) a k-step forms the A fragment of row block i+1
+ *             while the 12 MFMAs of block i issue; scheduling hints
+ *             interleave MFMA with the LDS reads and VALU work */
+template <int NWR>
+struct Geo {
+    static constexpr int NT = 256 * NWR; /* threads */
+    static constexpr int RB = 4 / NWR;   /* 32-row blocks per wave */
+    static constexpr int NA = 1024 / NT; /* float4 of a 128x32 FP32 tile per thread */
+    static constexpr int NH = 512 / NT;  /* uint4 of a 128x32 BF16 tile per thread */
+};
+
+#define A16 __attribute__((aligned(16)))
+
+template <int MODE>
+struct Stage;
+
+template <>
+struct A16 Stage<AMB_PREC_BF16X3> {
+    A16 float a[TT][KC + PADF];         /* D_r2 rows */
+    A16 __bf16 bh[TT][KC + PADH];       /* D_r3 hi */
+    A16 __bf16 bl[TT][KC + PADH];       /* D_r3 lo */
+    A16 float d1[QP][KC];               /* D_r1 rows of the 4 planes */
+};
+
+template <>
+struct A16 Stage<AMB_PREC_FP32> {
+    A16 float a[TT][KC + PADF];
+    A16 float b[TT][KC + PADF];
+    A16 float d1[QP][KC];
+};
+
+template <>
+struct A16 Stage<AMB_PREC_FP8> {
+    A16 float a[TT][KC + PADF];         /* D_r2 rows times s2 */
+    A16 uint8_t b[TT][KC + 16];         /* D_r3 rows times t3, e4m3 */
+    A16 float d1[QP][KC];               /* D_r1 rows times s1 */
+};
+
+/* global -> register prefetch of one chunk (per thread) */
+template <int MODE, int NWR>
+struct Pref {
+    float4 a[Geo<NWR>::NA];
+    float4 b[Geo<NWR>::NA];    /* FP32 form */
+    uint4 bh[Geo<NWR>::NH];    /* BF16x3 form */
+    uint4 bl[Geo<NWR>::NH];
+    uint4 q;                   /* FP8 form */
+    float4 d1;
+};
+
+__device__ __forceinline__ float4 ldg4(const float *p) { return *(const float4 *)p; }
+__device__ __forceinline__ uint4 ldg4u(const void *p) { return *(const uint4 *)p; }
+
+/* thread -> staged element maps (NT threads):
+ *   FP32 rows (D_r2; D_r3 in the FP32 form): 128 rows x 8 float4,
+ *     row = p*(NT/8) + tid/8, float4 slot tid%8;
+ *   BF16 rows: 128 rows x 4 uint4 per half, row = p*(NT/4) + tid/4, slot tid%4;
+ *   e4m3 rows: 128 rows x 2 uint4, threads 0..255, row = tid/2, slot tid%2;
+ *   D_r1: 4 rows x 8 float4, threads 0..31. */
+template <int MODE, int NWR>
+__device__ __forceinline__ void prefetch(Pref<MODE, NWR> &r, const ContractArgs &a, long k0, int r0, int c0,
+                                         int g0, int tid)
 {
-    const int b = blockIdx.x;
-    const int s = b >> 3;
-    const int w = (b & 7) * per_xcd + s;
-    if (s >= per_xcd || w >= nwork) return;
+    using G = Geo<NWR>;
+    const long pitch = a.pitch;
     const int nact = a.nact;
-    const int t = w / nact, i1 = w - t * nact;
-    const int t2 = t / nt3, t3 = t - t2 * nt3;
-    const int r0 = t2 * TM, c0 = t3 * TN;
-    const int lo = a.wlo[i1], hi = a.whi[i1];
-    if (r0 >= hi || r0 + TM <= lo || c0 >= hi || c0 + TN <= lo) return;
-
-    const long P = a.P;
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
-    const int wm = wave >> 1, wn = wave & 1;
-    const int srow = tid >> 3, kq = (tid & 7) * 4;
-
-    f32x16 acc[2][2];
+    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+    const uint4 zu = make_uint4(0, 0, 0, 0);
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int p = 0; p < G::NA; ++p) {
+        const int ga = r0 + p * (G::NT / 8) + (tid >> 3);
+        r.a[p] = ga < nact ? ldg4(a.D2 + (size_t)ga * pitch + k0 + (tid & 7) * 4) : z;
+    }
+    if constexpr (MODE == AMB_PREC_BF16X3) {
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int p = 0; p < G::NH; ++p) {
+            const int gb = c0 + p * (G::NT / 4) + (tid >> 2);
+            const size_t off = (size_t)gb * pitch + k0 + (tid & 3) * 8;
+            r.bh[p] = gb < nact ? ldg4u(a.H3 + off) : zu;
+            r.bl[p] = gb < nact ? ldg4u(a.L3 + off) : zu;
+        }
+    } else if constexpr (MODE == AMB_PREC_FP32) {
+#pragma unroll
+        for (int p = 0; p < G::NA; ++p) {
+            const int gb = c0 + p * (G::NT / 8) + (tid >> 3);
+            r.b[p] = gb < nact ? ldg4(a.D3 + (size_t)gb * pitch + k0 + (tid & 7) * 4) : z;
+        }
+    } else {
+        const int gb = c0 + (tid >> 1);
+        r.q = (tid < 256 && gb < nact) ? ldg4u(a.Q3 + (size_t)gb * pitch + k0 + (tid & 1) * 16) : zu;
+    }
+    if (tid < QP * KC / 4) {
+        const int g = g0 + (tid >> 3);
+        r.d1 = g < nact ? ldg4(a.D1 + (size_t)g * pitch + k0 + (tid & 7) * 4) : z;
+    }
+}
+
+template <int MODE, int NWR>
+__device__ __forceinline__ void stage_store(Stage<MODE> &s, const Pref<MODE, NWR> &r, int tid, const float *rsc,
+                                            const float *s1v)
+{
+    using G = Geo<NWR>;
+#pragma unroll
+    for (int p = 0; p < G::NA; ++p) {
+        const int row = p * (G::NT / 8) + (tid >> 3);
+        float4 v = r.a[p];
+        if constexpr (MODE == AMB_PREC_FP8) { /* fold the plane scale s2 in */
+            const float f = rsc[row];
+            v.x *= f; v.y *= f; v.z *= f; v.w *= f;
+        }
+        *(float4 *)&s.a[row][(tid & 7) * 4] = v;
+    }
+    if constexpr (MODE == AMB_PREC_BF16X3) {
+#pragma unroll
+        for (int p = 0; p < G::NH; ++p) {
+            const int row = p * (G::NT / 4) + (tid >> 2);
+            *(uint4 *)&s.bh[row][(tid & 3) * 8] = r.bh[p];
+            *(uint4 *)&s.bl[row][(tid & 3) * 8] = r.bl[p];
+        }
+    } else if constexpr (MODE == AMB_PREC_FP32) {
+#pragma unroll
+        for (int p = 0; p < G::NA; ++p) *(float4 *)&s.b[p * (G::NT / 8) + (tid >> 3)][(tid & 7) * 4] = r.b[p];
+    } else {
+        if (tid < 256) *(uint4 *)&s.b[tid >> 1][(tid & 1) * 16] = r.q;
+    }
+    if (tid < QP * KC / 4) {
+        float4 v = r.d1;
+        if constexpr (MODE == AMB_PREC_FP8) { /* and s1 of the reference plane */
+            const float f = s1v[tid >> 3];
+            v.x *= f; v.y *= f; v.z *= f; v.w *= f;
+        }
+        *(float4 *)&s.d1[tid >> 3][(tid & 7) * 4] = v;
+    }
+}
+
+/* Operand fragments of one k-step (16 pixels; 8 for FP32 which runs four
+ * k=2 MFMAs per 8 pixels and lane half) for RB row blocks and 4 column
+ * blocks. */
+template <int MODE, int RB>
+struct Frag;
+template <int RB>
+struct Frag<AMB_PREC_BF16X3, RB> {
+    bf16x8 ah[RB], al[RB], bh[4], bl[4];
+};
+template <int RB>
+struct Frag<AMB_PREC_FP8, RB> {
+    long aq[RB], bq[4];
+};
+template <int RB>
+struct Frag<AMB_PREC_FP32, RB> {
+    float av[RB][4], bv[4][4];
+};
+
+/* k-steps per chunk */
+template <int MODE>
+constexpr int ksteps() { return MODE == AMB_PREC_FP32 ? 4 : 2; }
+
+template <int MODE, int RB>
+__device__ __forceinline__ void load_frag(const Stage<MODE> &s, int step, int q, int rb, int lane, Frag<MODE, RB> &f)
+{
+    const int rl = lane & 31, hl = lane >> 5;
+    if constexpr (MODE == AMB_PREC_BF16X3) {
+        const int kc = step * 16 + hl * 8;
+        float d1v[8];
+        *(float4 *)&d1v[0] = *(const float4 *)&s.d1[q][kc];
+        *(float4 *)&d1v[4] = *(const float4 *)&s.d1[q][kc + 4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            f.bh[j] = *(const bf16x8 *)&s.bh[j * 32 + rl][kc];
+            f.bl[j] = *(const bf16x8 *)&s.bl[j * 32 + rl][kc];
+        }
+#pragma unroll
+        for (int i = 0; i < RB; ++i) {
+            float x[8];
+            *(float4 *)&x[0] = *(const float4 *)&s.a[rb + i * 32 + rl][kc];
+            *(float4 *)&x[4] = *(const float4 *)&s.a[rb + i * 32 + rl][kc + 4];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const float v = x[e] * d1v[e];
+                const __bf16 h = (__bf16)v;
+                f.ah[i][e] = h;
+                f.al[i][e] = (__bf16)(v - (float)h);
+            }
+        }
+    } else if constexpr (MODE == AMB_PREC_FP8) {
+        const int kc = step * 16 + hl * 8;
+        float d1v[8];
+        *(float4 *)&d1v[0] = *(const float4 *)&s.d1[q][kc];
+        *(float4 *)&d1v[4] = *(const float4 *)&s.d1[q][kc + 4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) f.bq[j] = *(const long *)&s.b[j * 32 + rl][kc];
+#pragma unroll
+        for (int i = 0; i < RB; ++i) {
+            const float4 x0 = *(const float4 *)&s.a[rb + i * 32 + rl][kc];
+            const float4 x1 = *(const float4 *)&s.a[rb + i * 32 + rl][kc + 4];
+            int lo4 = __builtin_amdgcn_cvt_pk_fp8_f32(x0.x * d1v[0], x0.y * d1v[1], 0, false);
+            lo4 = __builtin_amdgcn_cvt_pk_fp8_f32(x0.z * d1v[2], x0.w * d1v[3], lo4, true);
+            int hi4 = __builtin_amdgcn_cvt_pk_fp8_f32(x1.x * d1v[4], x1.y * d1v[5], 0, false);
+            hi4 = __builtin_amdgcn_cvt_pk_fp8_f32(x1.z * d1v[6], x1.w * d1v[7], hi4, true);
+            f.aq[i] = (long)(((unsigned long)(unsigned)hi4 << 32) | (unsigned)lo4);
+        }
+    } else {
+        /* k-permuted: lane half hl takes pixels hl*16 + 4*step .. +4, so a
+           lane reads 4 consecutive floats per row with one b128 load */
+        const int kc = hl * 16 + step * 4;
+        const float4 d1 = *(const float4 *)&s.d1[q][kc];
+#pragma unroll
+        for (int i = 0; i < RB; ++i) {
+            const float4 t = *(const float4 *)&s.a[rb + i * 32 + rl][kc];
+            f.av[i][0] = t.x * d1.x; f.av[i][1] = t.y * d1.y; f.av[i][2] = t.z * d1.z; f.av[i][3] = t.w * d1.w;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float4 t = *(const float4 *)&s.b[j * 32 + rl][kc];
+            f.bv[j][0] = t.x; f.bv[j][1] = t.y; f.bv[j][2] = t.z; f.bv[j][3] = t.w;
+        }
+    }
+}
+
+template <int MODE, int RB>
+__device__ __forceinline__ void mfma_frag(const Frag<MODE, RB> &f, f32x16 (&acc)[RB][4])
+{
+    if constexpr (MODE == AMB_PREC_BF16X3) {
+        /* small terms first; the three passes of one block are 4*RB MFMAs apart */
+#pragma unroll
+        for (int i = 0; i < RB; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.al[i], f.bh[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < RB; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.ah[i], f.bl[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < RB; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.ah[i], f.bh[j], acc[i][j], 0, 0, 0);
+    } else if constexpr (MODE == AMB_PREC_FP8) {
+#pragma unroll
+        for (int i = 0; i < RB; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(f.aq[i], f.bq[j], acc[i][j], 0, 0, 0);
+    } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+#pragma unroll
+            for (int i = 0; i < RB; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(f.av[i][e], f.bv[j][e], acc[i][j], 0, 0, 0);
+    }
+}
+
+/* LDS of one workgroup */
+template <int MODE, int NST>
+struct A16 Smem {
+    Stage<MODE> sm[NST];
+    float rsc[TT], csc[TT]; /* FP8: s2 of the tile rows, 1/t of its columns */
+    float s1v[QP];          /* FP8: s1 of the 4 reference planes */
+};
+
+template <int PIPE>
+constexpr int nstages() { return 2; }
+
+/* BF16x3 k-step with the A formation of block i+1 overlapping the MFMAs of
+ * block i (one extra A fragment live). */
+template <int RB>
+__device__ __forceinline__ void bf16_step_ilv(const Stage<AMB_PREC_BF16X3> &s, int step, int q, int rb, int lane,
+                                              f32x16 (&acc)[RB][4])
+{
+    const int rl = lane & 31, hl = lane >> 5;
+    const int kc = step * 16 + hl * 8;
+    float d1v[8];
+    *(float4 *)&d1v[0] = *(const float4 *)&s.d1[q][kc];
+    *(float4 *)&d1v[4] = *(const float4 *)&s.d1[q][kc + 4];
+    bf16x8 bh[4], bl[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        bh[j] = *(const bf16x8 *)&s.bh[j * 32 + rl][kc];
+        bl[j] = *(const bf16x8 *)&s.bl[j * 32 + rl][kc];
+    }
+    auto form = [&](int i, bf16x8 &ah, bf16x8 &al) {
+        float x[8];
+        *(float4 *)&x[0] = *(const float4 *)&s.a[rb + i * 32 + rl][kc];
+        *(float4 *)&x[4] = *(const float4 *)&s.a[rb + i * 32 + rl][kc + 4];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const float v = x[e] * d1v[e];
+            const __bf16 h = (__bf16)v;
+            ah[e] = h;
+            al[e] = (__bf16)(v - (float)h);
+        }
+    };
+    bf16x8 ah0, al0, ah1, al1;
+    form(0, ah0, al0);
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+        bf16x8 &ah = (i & 1) ? ah1 : ah0, &al = (i & 1) ? al1 : al0;
+        if (i + 1 < RB) {
+            if (i & 1) form(i + 1, ah0, al0);
+            else form(i + 1, ah1, al1);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh[j], acc[i][j], 0, 0, 0);
+        if (i + 1 < RB) {
+            /* 12 MFMAs, each followed by a share of the next block's work:
+               2 LDS reads and 40 VALU instructions spread over them */
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0); /* MFMA */
+            __builtin_amdgcn_sched_group_barrier(0x100, 2, 0); /* DS read */
+#pragma unroll
+            for (int k = 0; k < 11; ++k) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0); /* MFMA */
+                __builtin_amdgcn_sched_group_barrier(0x002, 4, 0); /* VALU */
+            }
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        }
+    }
+}
+
+/* One work item: the (tile, group) product of one triangle. */
+template <int MODE, int NORM, int NWR, int PIPE>
+__device__ __forceinline__ void contract_item(const ContractArgs &a, int tile, int grp,
+                                              Smem<MODE, nstages<PIPE>()> &L)
+{
+    using G = Geo<NWR>;
+    constexpr int RB = G::RB;
+    constexpr int NS = ksteps<MODE>();
+    const int nact = a.nact;
+    const int nt3 = (nact + TT - 1) / TT;
+    const int t2 = tile / nt3, t3 = tile - t2 * nt3;
+    const int r0 = t2 * TT, c0 = t3 * TT, g0 = grp * QP;
+    /* skip a tile outside every window of the group */
+    int lo = a.wlo[g0], hi = a.whi[min(g0 + QP, nact) - 1];
+    if (r0 >= hi || r0 + TT <= lo || c0 >= hi || c0 + TT <= lo) return;
+    /* r1 == r2: C[p1][p2][.] = C[p2][p1][.] bit for bit (the A operand is a
+       product of the same two floats), so rows below every reference plane
+       of the group are read transposed and never computed */
+    if (a.sym12 && r0 + TT <= g0) return;
+
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int q = wv & 3, rb = (wv >> 2) * (TT / NWR);
+    const bool live = g0 + q < nact;
+    float s1 = 1.f;
+    if constexpr (MODE == AMB_PREC_FP8) {
+        if (tid < TT) {
+            L.rsc[tid] = r0 + tid < nact ? a.s2[r0 + tid] : 1.f;
+            L.csc[tid] = c0 + tid < nact ? 1.f / a.t3[c0 + tid] : 1.f;
+        }
+        s1 = live ? a.s1[g0 + q] : 1.f;
+        if (tid < QP) L.s1v[tid] = g0 + tid < nact ? a.s1[g0 + tid] : 1.f;
+        __syncthreads();
+    }
+    f32x16 acc[RB][4];
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.f;
 
-    const float *d1p = a.D1 + (size_t)i1 * P;
-
-    if constexpr (MODE == AMB_PREC_FP32) {
-        __shared__ float As[TM][KT + 4];
-        __shared__ float Bs[TN][KT + 4];
-        for (long k0 = 0; k0 < P; k0 += KT) {
-            const long kk = k0 + kq;
-            const bool kin = kk < P;
-            float4 d1 = kin ? *(const float4 *)(d1p + kk) : make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-            for (int pass = 0; pass < 4; ++pass) {
-                const int row = pass * 32 + srow;
-                const int ga = r0 + row, gb = c0 + row;
-                float4 av = make_float4(0.f, 0.f, 0.f, 0.f), bv = av;
-                if (kin && ga < nact) av = *(const float4 *)(a.D2 + (size_t)ga * P + kk);
-                if (kin && gb < nact) bv = *(const float4 *)(a.D3 + (size_t)gb * P + kk);
-                As[row][kq] = av.x * d1.x; As[row][kq + 1] = av.y * d1.y;
-                As[row][kq + 2] = av.z * d1.z; As[row][kq + 3] = av.w * d1.w;
-                Bs[row][kq] = bv.x; Bs[row][kq + 1] = bv.y; Bs[row][kq + 2] = bv.z; Bs[row][kq + 3] = bv.w;
+    const long nchunk = a.pitch / KC;
+    Pref<MODE, NWR> pf;
+    if constexpr (PIPE == 0) {
+        prefetch<MODE, NWR>(pf, a, 0, r0, c0, g0, tid);
+        stage_store<MODE, NWR>(L.sm[0], pf, tid, L.rsc, L.s1v);
+        __syncthreads();
+        for (long c = 0; c < nchunk; ++c) {
+            const int cur = (int)(c & 1);
+            if (c + 1 < nchunk) prefetch<MODE, NWR>(pf, a, (c + 1) * KC, r0, c0, g0, tid);
+            if (live) {
+#pragma unroll 1
+                for (int st = 0; st < NS; ++st) {
+                    Frag<MODE, RB> f;
+                    load_frag<MODE, RB>(L.sm[cur], st, q, rb, lane, f);
+                    mfma_frag<MODE, RB>(f, acc);
+                }
             }
-            __syncthreads();
-#pragma unroll
-            for (int k = 0; k < KT; k += 2) {
-                float af[2], bf[2];
-#pragma unroll
-                for (int i = 0; i < 2; ++i) af[i] = As[wm * 64 + i * 32 + (lane & 31)][k + (lane >> 5)];
-#pragma unroll
-                for (int j = 0; j < 2; ++j) bf[j] = Bs[wn * 64 + j * 32 + (lane & 31)][k + (lane >> 5)];
-#pragma unroll
-                for (int i = 0; i < 2; ++i)
-#pragma unroll
-                    for (int j = 0; j < 2; ++j)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(af[i], bf[j], acc[i][j], 0, 0, 0);
-            }
+            if (c + 1 < nchunk) stage_store<MODE, NWR>(L.sm[cur ^ 1], pf, tid, L.rsc, L.s1v);
             __syncthreads();
         }
     } else {
-        /* BF16x3: hi/lo planes of A = D1 o D2 and of B = D3 */
-        __shared__ __attribute__((aligned(16))) __bf16 Ah[TM][KT + 8];
-        __shared__ __attribute__((aligned(16))) __bf16 Al[TM][KT + 8];
-        __shared__ __attribute__((aligned(16))) __bf16 Bh[TN][KT + 8];
-        __shared__ __attribute__((aligned(16))) __bf16 Bl[TN][KT + 8];
-        for (long k0 = 0; k0 < P; k0 += KT) {
-            const long kk = k0 + kq;
-            const bool kin = kk < P;
-            float4 d1 = kin ? *(const float4 *)(d1p + kk) : make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-            for (int pass = 0; pass < 4; ++pass) {
-                const int row = pass * 32 + srow;
-                const int ga = r0 + row, gb = c0 + row;
-                float4 av = make_float4(0.f, 0.f, 0.f, 0.f), bv = av;
-                if (kin && ga < nact) av = *(const float4 *)(a.D2 + (size_t)ga * P + kk);
-                if (kin && gb < nact) bv = *(const float4 *)(a.D3 + (size_t)gb * P + kk);
-                float x[4] = {av.x * d1.x, av.y * d1.y, av.z * d1.z, av.w * d1.w};
-                float y[4] = {bv.x, bv.y, bv.z, bv.w};
-                bf16x4 xh, xl, yh, yl;
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    __bf16 h = (__bf16)x[e];
-                    xh[e] = h;
-                    xl[e] = (__bf16)(x[e] - (float)h);
-                    __bf16 g = (__bf16)y[e];
-                    yh[e] = g;
-                    yl[e] = (__bf16)(y[e] - (float)g);
-                }
-                *(bf16x4 *)&Ah[row][kq] = xh;
-                *(bf16x4 *)&Al[row][kq] = xl;
-                *(bf16x4 *)&Bh[row][kq] = yh;
-                *(bf16x4 *)&Bl[row][kq] = yl;
-            }
-            __syncthreads();
-#pragma unroll
-            for (int k = 0; k < KT; k += 16) {
-                bf16x8 ah[2], al[2], bh[2], bl[2];
-                const int kc = k + (lane >> 5) * 8;
-#pragma unroll
-                for (int i = 0; i < 2; ++i) {
-                    const int r = wm * 64 + i * 32 + (lane & 31);
-                    ah[i] = *(const bf16x8 *)&Ah[r][kc];
-                    al[i] = *(const bf16x8 *)&Al[r][kc];
-                }
-#pragma unroll
-                for (int j = 0; j < 2; ++j) {
-                    const int r = wn * 64 + j * 32 + (lane & 31);
-                    bh[j] = *(const bf16x8 *)&Bh[r][kc];
-                    bl[j] = *(const bf16x8 *)&Bl[r][kc];
-                }
-#pragma unroll
-                for (int i = 0; i < 2; ++i)
-#pragma unroll
-                    for (int j = 0; j < 2; ++j) {
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+        prefetch<MODE, NWR>(pf, a, 0, r0, c0, g0, tid);
+        stage_store<MODE, NWR>(L.sm[0], pf, tid, L.rsc, L.s1v);
+        __syncthreads();
+        for (long c = 0; c < nchunk; ++c) {
+            const int cur = (int)(c & 1);
+            if (c + 1 < nchunk) prefetch<MODE, NWR>(pf, a, (c + 1) * KC, r0, c0, g0, tid);
+            if (live) {
+#pragma unroll 1
+                for (int st = 0; st < NS; ++st) {
+                    if constexpr (MODE == AMB_PREC_BF16X3) {
+                        bf16_step_ilv<RB>(L.sm[cur], st, q, rb, lane, acc);
+                    } else {
+                        Frag<MODE, RB> f;
+                        load_frag<MODE, RB>(L.sm[cur], st, q, rb, lane, f);
+                        mfma_frag<MODE, RB>(f, acc);
                     }
+                }
             }
+            if (c + 1 < nchunk) stage_store<MODE, NWR>(L.sm[cur ^ 1], pf, tid, L.rsc, L.s1v);
             __syncthreads();
         }
     }
+    if (!live) return;
 
-    /* epilogue: C layout of a 32x32 MFMA block, lane l register v holds
-       row (v/4)*8 + (l/32)*4 + v%4, column l%32 */
-    const double invP = 1.0 / (double)P;
+    /* epilogue: a 32x32 MFMA block leaves row (v/4)*8 + (l/32)*4 + v%4,
+       column l%32 in register v of lane l */
+    const int i1 = g0 + q;
+    lo = a.wlo[i1];
+    hi = a.whi[i1];
+    const double invP = 1.0 / (double)a.P;
     float worst = 0.f;
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < RB; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < 4; ++j)
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
-                const int gi = r0 + wm * 64 + i * 32 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
-                const int gj = c0 + wn * 64 + j * 32 + (lane & 31);
+                const int li = rb + i * 32 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+                const int lj = j * 32 + (lane & 31);
+                const int gi = r0 + li, gj = c0 + lj;
                 if (gi < lo || gi >= hi || gj < lo || gj >= hi) continue;
                 const size_t idx = ((size_t)i1 * nact + gi) * nact + gj;
-                if constexpr (NORM) {
+                if constexpr (MODE == AMB_PREC_FP8) {
+                    a.C[idx] = acc[i][j][v] * (1.f / (s1 * L.rsc[li])) * L.csc[lj];
+                } else if constexpr (NORM) {
                     double val = (double)acc[i][j][v] * invP;
                     double q8 = rint(val * 8.0);
-                    float d = (float)fabs(val - q8 * 0.125);
-                    worst = fmaxf(worst, d);
+                    worst = fmaxf(worst, (float)fabs(val - q8 * 0.125));
                     a.Cq[idx] = (int)q8;
                 } else {
                     a.C[idx] = acc[i][j][v];
@@ -196,20 +505,208 @@ __global__ __launch_bounds__(256) void k_contract(ContractArgs a, int nt3, int p
     }
 }
 
-hipError_t amb_launch_contract(hipStream_t s, int prec, int norm, const ContractArgs &a)
+/* One triangle: workgroups numbered tile-major and dealt to the 8 XCDs in
+ * contiguous runs (hardware places workgroup b on XCD b mod 8). */
+template <int MODE, int NORM, int NWR, int PIPE>
+__global__ __launch_bounds__(256 * NWR, 1) void k_contract(ContractArgs a, int ngroups, int per_xcd, int nwork)
 {
-    const int nt2 = (a.nact + TM - 1) / TM, nt3 = (a.nact + TN - 1) / TN;
-    const int nwork = nt2 * nt3 * a.nact;
-    const int per_xcd = (nwork + 7) / 8;
-    dim3 grid(8 * per_xcd), block(256);
-    if (a.P % 4 != 0) return hipErrorInvalidValue;
-    if (prec == AMB_PREC_FP32) {
-        if (norm) hipLaunchKernelGGL((k_contract<AMB_PREC_FP32, 1>), grid, block, 0, s, a, nt3, per_xcd, nwork);
-        else hipLaunchKernelGGL((k_contract<AMB_PREC_FP32, 0>), grid, block, 0, s, a, nt3, per_xcd, nwork);
-    } else {
-        if (norm) hipLaunchKernelGGL((k_contract<AMB_PREC_BF16X3, 1>), grid, block, 0, s, a, nt3, per_xcd, nwork);
-        else hipLaunchKernelGGL((k_contract<AMB_PREC_BF16X3, 0>), grid, block, 0, s, a, nt3, per_xcd, nwork);
+    __shared__ Smem<MODE, nstages<PIPE>()> L;
+    const int b = blockIdx.x;
+    const int slot = b >> 3;
+    const int w = (b & 7) * per_xcd + slot;
+    if (slot >= per_xcd || w >= nwork) return;
+    contract_item<MODE, NORM, NWR, PIPE>(a, w / ngroups, w % ngroups, L);
+}
+
+/* Many triangles in one persistent launch: each workgroup takes the next
+ * item from a global counter until the queue is empty.  Items are ordered
+ * by the host (largest triangles first, tile-major within a triangle).
+ * begin[t] is the first item of triangle t, begin[ntri] the total. */
+template <int MODE, int NORM, int NWR, int PIPE>
+__global__ __launch_bounds__(256 * NWR, 1) void k_contract_grouped(const ContractArgs *__restrict__ descs,
+                                                                   const long *__restrict__ begin, int ntri,
+                                                                   unsigned long long *counter)
+{
+    __shared__ Smem<MODE, nstages<PIPE>()> L;
+    __shared__ long item;
+    const long total = begin[ntri];
+    for (;;) {
+        if (threadIdx.x == 0) item = (long)atomicAdd(counter, 1ull);
+        __syncthreads();
+        const long w = item;
+        __syncthreads();
+        if (w >= total) return;
+        int lo = 0, hi = ntri - 1;
+        while (lo < hi) { /* last t with begin[t] <= w */
+            const int mid = (lo + hi + 1) >> 1;
+            if (begin[mid] <= w) lo = mid;
+            else hi = mid - 1;
+        }
+        const ContractArgs &a = descs[lo]; /* uniform: scalar loads */
+        const int ngroups = (a.nact + QP - 1) / QP;
+        const long r = w - begin[lo];
+        contract_item<MODE, NORM, NWR, PIPE>(a, (int)(r / ngroups), (int)(r % ngroups), L);
+        __syncthreads();
     }
+}
+
+__global__ void k_set_u64(unsigned long long *p, unsigned long long v) { *p = v; }
+
+static bool args_ok(int prec, int norm, const ContractArgs &a)
+{
+    if (a.pitch % KC != 0 || a.nact <= 0) return false;
+    if (prec == AMB_PREC_BF16X3 && (!a.H3 || !a.L3)) return false;
+    if (prec == AMB_PREC_FP32 && !a.D3) return false;
+    if (prec == AMB_PREC_FP8 && (norm || !a.Q3 || !a.s1 || !a.s2 || !a.t3)) return false;
+    return true;
+}
+
+long amb_contract_items(int nact)
+{
+    const long nt = (nact + TT - 1) / TT;
+    return nt * nt * ((nact + QP - 1) / QP);
+}
+
+template <int MODE, int NORM>
+static void launch_grouped_var(int var, dim3 grid, hipStream_t s, const ContractArgs *d, const long *b, int n,
+                               unsigned long long *ctr)
+{
+    switch (var) {
+    case 1: hipLaunchKernelGGL((k_contract_grouped<MODE, NORM, 2, 0>), grid, dim3(512), 0, s, d, b, n, ctr); break;
+    case 2: hipLaunchKernelGGL((k_contract_grouped<MODE, NORM, 1, 1>), grid, dim3(256), 0, s, d, b, n, ctr); break;
+    case 3: hipLaunchKernelGGL((k_contract_grouped<MODE, NORM, 2, 1>), grid, dim3(512), 0, s, d, b, n, ctr); break;
+    default: hipLaunchKernelGGL((k_contract_grouped<MODE, NORM, 1, 0>), grid, dim3(256), 0, s, d, b, n, ctr); break;
+    }
+}
+
+template <int MODE, int NORM>
+static void launch_var(int var, dim3 grid, hipStream_t s, const ContractArgs &a, int ngroups, int per_xcd, int nwork)
+{
+    switch (var) {
+    case 1: hipLaunchKernelGGL((k_contract<MODE, NORM, 2, 0>), grid, dim3(512), 0, s, a, ngroups, per_xcd, nwork); break;
+    case 2: hipLaunchKernelGGL((k_contract<MODE, NORM, 1, 1>), grid, dim3(256), 0, s, a, ngroups, per_xcd, nwork); break;
+    case 3: hipLaunchKernelGGL((k_contract<MODE, NORM, 2, 1>), grid, dim3(512), 0, s, a, ngroups, per_xcd, nwork); break;
+    default: hipLaunchKernelGGL((k_contract<MODE, NORM, 1, 0>), grid, dim3(256), 0, s, a, ngroups, per_xcd, nwork); break;
+    }
+}
+
+hipError_t amb_launch_contract_grouped(hipStream_t s, int prec, int norm, int var, const ContractArgs *d_descs,
+                                       const long *d_begin, int ntri, long total, unsigned long long *d_counter,
+                                       int nblocks)
+{
+    if (ntri <= 0 || total <= 0) return hipSuccess;
+    if (prec == AMB_PREC_FP8 && norm) return hipErrorInvalidValue;
+    /* a kernel fill, not a memset, so the reset can live in a captured graph */
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, s, d_counter, 0ull);
+    if ((long)nblocks > total) nblocks = (int)total;
+    dim3 grid(nblocks);
+    if (prec == AMB_PREC_FP8) launch_grouped_var<AMB_PREC_FP8, 0>(var, grid, s, d_descs, d_begin, ntri, d_counter);
+    else if (prec == AMB_PREC_FP32) {
+        if (norm) launch_grouped_var<AMB_PREC_FP32, 1>(var, grid, s, d_descs, d_begin, ntri, d_counter);
+        else launch_grouped_var<AMB_PREC_FP32, 0>(var, grid, s, d_descs, d_begin, ntri, d_counter);
+    } else {
+        if (norm) launch_grouped_var<AMB_PREC_BF16X3, 1>(var, grid, s, d_descs, d_begin, ntri, d_counter);
+        else launch_grouped_var<AMB_PREC_BF16X3, 0>(var, grid, s, d_descs, d_begin, ntri, d_counter);
+    }
+    return hipGetLastError();
+}
+
+bool amb_contract_args_ok(int prec, int norm, const ContractArgs &a) { return args_ok(prec, norm, a); }
+
+hipError_t amb_launch_contract(hipStream_t s, int prec, int norm, int var, const ContractArgs &a)
+{
+    if (!args_ok(prec, norm, a)) return hipErrorInvalidValue;
+    const int nt = (a.nact + TT - 1) / TT;
+    const int ngroups = (a.nact + QP - 1) / QP;
+    const int nwork = nt * nt * ngroups;
+    const int per_xcd = (nwork + 7) / 8;
+    dim3 grid(8 * per_xcd);
+    if (prec == AMB_PREC_FP8) launch_var<AMB_PREC_FP8, 0>(var, grid, s, a, ngroups, per_xcd, nwork);
+    else if (prec == AMB_PREC_FP32) {
+        if (norm) launch_var<AMB_PREC_FP32, 1>(var, grid, s, a, ngroups, per_xcd, nwork);
+        else launch_var<AMB_PREC_FP32, 0>(var, grid, s, a, ngroups, per_xcd, nwork);
+    } else {
+        if (norm) launch_var<AMB_PREC_BF16X3, 1>(var, grid, s, a, ngroups, per_xcd, nwork);
+        else launch_var<AMB_PREC_BF16X3, 0>(var, grid, s, a, ngroups, per_xcd, nwork);
+    }
+    return hipGetLastError();
+}
+
+/* hi/lo BF16 split of FP32 planes (the B operand of the BF16x3 form) */
+__global__ void k_split(const float *__restrict__ D, size_t n, __bf16 *__restrict__ H, __bf16 *__restrict__ L)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float x = D[i];
+        const __bf16 h = (__bf16)x;
+        H[i] = h;
+        L[i] = (__bf16)(x - (float)h);
+    }
+}
+
+hipError_t amb_launch_split(hipStream_t s, const float *D, size_t n, __bf16 *H, __bf16 *L)
+{
+    int blocks = (int)std::min<size_t>((n + 255) / 256, 65536);
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_split, dim3(blocks), dim3(256), 0, s, D, n, H, L);
+    return hipGetLastError();
+}
+
+/* Per-plane power-of-two scales for the FP8 form from max|D| over each
+ * plane: sc[i] = 2^floor(log2(sqrt(448)/m)) (A-side factors) and
+ * sc[n+i] = 2^floor(log2(448/m)) (B side); 1 for an all-zero plane. */
+__global__ void k_fp8_scales(const float *__restrict__ D, long pitch, float *__restrict__ sc, int n)
+{
+    __shared__ float red[256];
+    const float *p = D + (size_t)blockIdx.x * pitch;
+    float m = 0.f;
+    for (long x = threadIdx.x; x < pitch; x += blockDim.x) m = fmaxf(m, fabsf(p[x]));
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        m = red[0];
+        float a = 1.f, b = 1.f;
+        if (m > 0.f) {
+            a = exp2f(floorf(log2f(21.166010488516726f / m)));
+            b = exp2f(floorf(log2f(448.f / m)));
+            /* log2f may round up across an integer: keep |x| s within range */
+            while (m * a > 21.166010488516726f) a *= 0.5f;
+            while (m * b > 448.f) b *= 0.5f;
+        }
+        sc[blockIdx.x] = a;
+        sc[n + blockIdx.x] = b;
+    }
+}
+
+__global__ void k_fp8_quant(const float *__restrict__ D, long pitch, int n, const float *__restrict__ t,
+                            uint8_t *__restrict__ Q)
+{
+    const size_t tot4 = (size_t)n * pitch / 4;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < tot4; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t e = 4 * i;
+        const float f = t[e / pitch];
+        const float4 v = *(const float4 *)(D + e);
+        int w = __builtin_amdgcn_cvt_pk_fp8_f32(v.x * f, v.y * f, 0, false);
+        w = __builtin_amdgcn_cvt_pk_fp8_f32(v.z * f, v.w * f, w, true);
+        *(int *)(Q + e) = w;
+    }
+}
+
+hipError_t amb_launch_fp8_scales(hipStream_t s, const float *D, long pitch, int n, float *sc)
+{
+    hipLaunchKernelGGL(k_fp8_scales, dim3(n), dim3(256), 0, s, D, pitch, sc, n);
+    return hipGetLastError();
+}
+
+hipError_t amb_launch_fp8_quant(hipStream_t s, const float *D, long pitch, int n, const float *t, uint8_t *Q)
+{
+    if (pitch % 4 != 0) return hipErrorInvalidValue;
+    size_t tot4 = (size_t)n * pitch / 4;
+    int blocks = (int)std::min<size_t>((tot4 + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_fp8_quant, dim3(blocks), dim3(256), 0, s, D, pitch, n, t, Q);
     return hipGetLastError();
 }
 
@@ -217,12 +714,10 @@ hipError_t amb_launch_contract(hipStream_t s, int prec, int norm, const Contract
  * ascending order with Neumaier-compensated FP32 (data) and exact int64
  * (normalisation, in units of 1/8), and counts valid p1 (v1 numdnu,
  * amber.c:266). */
-__global__ void k_reduce(ReduceArgs a, const int *__restrict__ first_ge)
+__device__ __forceinline__ void reduce_cell(const ReduceArgs &a, long cell)
 {
     const int W = 2 * a.nc10 + 1;
-    const long cells = (long)W * W;
-    const long cell = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (cell >= cells) return;
+    const int *__restrict__ first_ge = a.inv + a.nc; /* inv[nc] then first_ge[nc+1] */
     const int d2 = (int)(cell / W) - a.nc10, d3 = (int)(cell % W) - a.nc10;
     int plo = 0;
     if (-d2 > plo) plo = -d2;
@@ -234,23 +729,44 @@ __global__ void k_reduce(ReduceArgs a, const int *__restrict__ first_ge)
     long long q = 0;
     int n = 0;
     const size_t na = (size_t)a.nact;
-    if (plo < phi) {
-        for (int i1 = first_ge[plo]; i1 < a.nact; ++i1) {
-            const int p1 = a.act[i1];
-            if (p1 >= phi) break;
-            const int j2 = a.inv[p1 + d2], j3 = a.inv[p1 + d3];
-            if (j2 < 0 || j3 < 0) continue;
-            const size_t idx = ((size_t)i1 * na + j2) * na + j3;
-            if (a.C) {
-                const float x = a.C[idx];
-                const float tsum = sum + x;
-                if (fabsf(sum) >= fabsf(x)) comp += (sum - tsum) + x;
-                else comp += (x - tsum) + sum;
-                sum = tsum;
-            }
-            if (a.Cq) q += a.Cq[idx];
+    /* eight p1 at a time: the index loads and the C loads of a batch are
+       independent, the compensated sums still run in ascending p1 order */
+    constexpr int U = 8;
+    int i1 = plo < phi ? first_ge[plo] : a.nact;
+    while (i1 < a.nact) {
+        size_t idx[U];
+        bool ok[U];
+        bool past = false;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = i1 + u;
+            const int p1 = i < a.nact ? a.act[i] : phi;
+            past |= p1 >= phi;
+            ok[u] = p1 < phi;
+            const int j2 = ok[u] ? a.inv[p1 + d2] : -1, j3 = ok[u] ? a.inv[p1 + d3] : -1;
+            ok[u] = ok[u] && j2 >= 0 && j3 >= 0;
+            const bool sw = a.sym12 && j2 < i;
+            idx[u] = ok[u] ? ((size_t)(sw ? j2 : i) * na + (sw ? i : j2)) * na + j3 : 0;
+        }
+        float x[U];
+        int qx[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            x[u] = (a.C && ok[u]) ? a.C[idx[u]] : 0.f;
+            qx[u] = (a.Cq && ok[u]) ? a.Cq[idx[u]] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (!ok[u]) continue;
+            const float tsum = sum + x[u];
+            if (fabsf(sum) >= fabsf(x[u])) comp += (sum - tsum) + x[u];
+            else comp += (x[u] - tsum) + sum;
+            sum = tsum;
+            q += qx[u];
             ++n;
         }
+        if (past) break;
+        i1 += U;
     }
     if (a.bl) a.bl[cell] = sum + comp;
     if (a.bl64) a.bl64[cell] = (double)sum + (double)comp;
@@ -262,12 +778,41 @@ __global__ void k_reduce(ReduceArgs a, const int *__restrict__ first_ge)
     }
 }
 
+__global__ void k_reduce(ReduceArgs a)
+{
+    const int W = 2 * a.nc10 + 1;
+    const long cell = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell < (long)W * W) reduce_cell(a, cell);
+}
+
+/* many triangles: cbegin[t] is the first global cell of triangle t */
+__global__ void k_reduce_grouped(const ReduceArgs *__restrict__ ra, const long *__restrict__ cbegin, int ntri)
+{
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= cbegin[ntri]) return;
+    int lo = 0, hi = ntri - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cbegin[mid] <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    reduce_cell(ra[lo], g - cbegin[lo]);
+}
+
+hipError_t amb_launch_reduce_grouped(hipStream_t s, const ReduceArgs *d_ra, const long *d_cbegin, int ntri,
+                                     long total_cells)
+{
+    if (ntri <= 0 || total_cells <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reduce_grouped, dim3((unsigned)((total_cells + 255) / 256)), dim3(256), 0, s, d_ra,
+                       d_cbegin, ntri);
+    return hipGetLastError();
+}
+
 hipError_t amb_launch_reduce(hipStream_t s, const ReduceArgs &a)
 {
-    /* first_ge is appended after inv by the caller: inv[nc] then first_ge[nc+1] */
     const int W = 2 * a.nc10 + 1;
     const long cells = (long)W * W;
-    hipLaunchKernelGGL(k_reduce, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s, a, a.inv + a.nc);
+    hipLaunchKernelGGL(k_reduce, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
@@ -294,27 +839,5 @@ hipError_t amb_launch_triple_sum(hipStream_t s, const float *II, long P, const i
 {
     if (ntri <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_triple_sum, dim3(ntri), dim3(256), 0, s, II, P, tri, out);
-    return hipGetLastError();
-}
-
-__global__ void k_absmax(const float *__restrict__ D, long P, float *out)
-{
-    __shared__ float red[256];
-    const float *p = D + (size_t)blockIdx.x * P;
-    float m = 0.f;
-    for (long x = threadIdx.x; x < P; x += blockDim.x) m = fmaxf(m, fabsf(p[x]));
-    red[threadIdx.x] = m;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
-}
-
-hipError_t amb_launch_absmax(hipStream_t s, const float *D, long P, int npl, float *out)
-{
-    if (npl <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_absmax, dim3(npl), dim3(256), 0, s, D, P, out);
     return hipGetLastError();
 }

@@ -168,7 +168,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_cols(
 /* grid (ceil(n/(2 nb)), npl); each block nb transforms = 2 nb output rows */
 __global__ __launch_bounds__(FFT_THREADS) void k_fft_rows(
     const float2 *__restrict__ G, int jcnt, FftPlan pl, const float2 *__restrict__ twg, int nb,
-    float *__restrict__ out)
+    float *__restrict__ out, long pitch)
 {
     extern __shared__ float2 smem[];
     const int n = pl.n;
@@ -199,7 +199,9 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_rows(
     }
     __syncthreads();
     float2 *res = lds_fft_batch(bx, by, tw, pl, nb);
-    float *op = out + (size_t)pli * n * n;
+    float *op = out + (size_t)pli * pitch;
+    if (blockIdx.x == 0)
+        for (long t = (long)n * n + threadIdx.x; t < pitch; t += blockDim.x) op[t] = 0.f;
     for (int t = threadIdx.x; t < nb * n; t += blockDim.x) {
         int b = t / n, yv = t - b * n;
         int xa = x0 + 2 * b;
@@ -228,12 +230,13 @@ static int lds_batch(int n)
 
 hipError_t amb_launch_ring_c2r(hipStream_t s, int kind, const float2 *vis_t, const float *samp_t,
                                const int *d_planes, int npl, int N, int Nh, const FftPlan &pl,
-                               const float2 *d_tw, RingGeom g, float2 *G, float *out)
+                               const float2 *d_tw, RingGeom g, float2 *G, float *out, long pitch)
 {
     const int n = pl.n;
     if (npl <= 0) return hipSuccess;
+    if (pitch < (long)n * n) return hipErrorInvalidValue;
     if (g.jcnt <= 0) {
-        size_t tot = (size_t)npl * n * n;
+        size_t tot = (size_t)npl * pitch;
         hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, s, out, tot);
         return hipGetLastError();
     }
@@ -245,7 +248,7 @@ hipError_t amb_launch_ring_c2r(hipStream_t s, int kind, const float2 *vis_t, con
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     dim3 gr((n + 2 * nb - 1) / (2 * nb), npl);
-    hipLaunchKernelGGL(k_fft_rows, gr, dim3(FFT_THREADS), lds, s, G, g.jcnt, pl, d_tw, nb, out);
+    hipLaunchKernelGGL(k_fft_rows, gr, dim3(FFT_THREADS), lds, s, G, g.jcnt, pl, d_tw, nb, out, pitch);
     return hipGetLastError();
 }
 

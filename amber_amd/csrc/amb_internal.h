@@ -21,10 +21,11 @@ struct RingGeom {
 
 /* ---- FFT stage (amb_fft.hip) ---- */
 /* kind 0: data (complex vis), kind 1: sampling (0/1 real).
- * G scratch: [npl][jcnt][n] complex64.  out: [npl][n][n] float. */
+ * G scratch: [npl][jcnt][n] complex64.  out: npl planes of n*n floats at
+ * `pitch` floats apart; the pitch - n*n tail of each plane is zeroed. */
 hipError_t amb_launch_ring_c2r(hipStream_t s, int kind, const float2 *vis_t, const float *samp_t,
                                const int *d_planes, int npl, int N, int Nh, const FftPlan &pl,
-                               const float2 *d_tw, RingGeom g, float2 *G, float *out);
+                               const float2 *d_tw, RingGeom g, float2 *G, float *out, long pitch);
 void amb_make_plan(int n, FftPlan *pl);
 
 /* ---- cube upload (amb_fft.hip) ---- */
@@ -33,22 +34,36 @@ hipError_t amb_launch_upload(hipStream_t s, const double *gv, const double *samp
 
 /* ---- contraction (amb_contract.hip) ---- */
 struct ContractArgs {
-    const float *D1, *D2, *D3; /* [nact][P] */
-    long P;
+    const float *D1, *D2;      /* [nact][pitch] FP32 */
+    const float *D3;           /* FP32 form: [nact][pitch] */
+    const __bf16 *H3, *L3;     /* BF16x3 form: hi / lo split of D3 */
+    const uint8_t *Q3;         /* FP8 form: e4m3 D3 times t3 */
+    const float *s1, *s2, *t3; /* FP8 form: per-plane scales of D1, D2, D3 */
+    long P, pitch;             /* pixels, plane pitch (multiple of 32) */
     int nact;
+    int sym12;                 /* r1 == r2 and full windows: rows below the
+                                  reference planes are never needed */
     const int *wlo, *whi;      /* per-i1 window in compact channel indices */
     float *C;                  /* data: [nact][nact][nact] f32 */
     int *Cq;                   /* norm: [nact][nact][nact] int32 of 8*count */
     unsigned *margin;          /* norm: max |S/P - q| as float bits */
-    const float *scale;        /* fp8: per-plane scale, [3][nact] */
 };
-hipError_t amb_launch_contract(hipStream_t s, int prec, int norm, const ContractArgs &a);
+/* var: kernel variant (bit 0: 8 waves with half tiles per wave, bit 1:
+ * three-stage pipelined k-steps); chosen per shape by the autotuner */
+#define AMB_NVAR 4
+hipError_t amb_launch_contract(hipStream_t s, int prec, int norm, int var, const ContractArgs &a);
+hipError_t amb_launch_split(hipStream_t s, const float *D, size_t n, __bf16 *H, __bf16 *L);
+/* FP8 preparation of n planes: sc[0..n) A-side scales, sc[n..2n) B-side
+ * scales; quant writes the e4m3 planes D*t[i] */
+hipError_t amb_launch_fp8_scales(hipStream_t s, const float *D, long pitch, int n, float *sc);
+hipError_t amb_launch_fp8_quant(hipStream_t s, const float *D, long pitch, int n, const float *t, uint8_t *Q);
 
 struct ReduceArgs {
     const float *C;     /* data partials or NULL */
     const int *Cq;      /* norm partials or NULL */
     const int *act, *inv;
     int nc, nact, nc10;
+    int sym12;          /* read C[p2][p1][p3] when p2 < p1 */
     long P;
     float *bl;
     double *bl64;
@@ -58,9 +73,18 @@ struct ReduceArgs {
 };
 hipError_t amb_launch_reduce(hipStream_t s, const ReduceArgs &a);
 
+/* ---- many triangles per launch (batches, serving) ---- */
+long amb_contract_items(int nact); /* work items of one triangle */
+bool amb_contract_args_ok(int prec, int norm, const ContractArgs &a);
+/* persistent contraction over ntri triangles: descs/begin in device memory,
+ * begin[t] = first item of triangle t (largest first), begin[ntri] = total;
+ * d_counter is reset by a kernel inside the launch sequence */
+hipError_t amb_launch_contract_grouped(hipStream_t s, int prec, int norm, int var, const ContractArgs *d_descs,
+                                       const long *d_begin, int ntri, long total, unsigned long long *d_counter,
+                                       int nblocks);
+hipError_t amb_launch_reduce_grouped(hipStream_t s, const ReduceArgs *d_ra, const long *d_cbegin, int ntri,
+                                     long total_cells);
+
 /* calCR triple sums: one FP64 sum per triangle over P pixels. */
 hipError_t amb_launch_triple_sum(hipStream_t s, const float *II, long P, const int *tri, int ntri,
                                  double *out);
-
-/* per-plane absmax for FP8 scaling: out[i] = max |x| over plane i */
-hipError_t amb_launch_absmax(hipStream_t s, const float *D, long P, int npl, float *out);

@@ -185,6 +185,15 @@ int amb_bispec_ex(amb_ctx *ctx, int cube_id, int r1, int r2, int r3, int want_no
 int amb_bispec_batch(amb_ctx *ctx, int cube_id, const int *tri, int ntri,
                      float *bl, int32_t *cnt, float *norm);
 
+/* Host wall time (ms) from the start of amb_bispec_batch to each
+ * triangle's results being in host memory, for the last batch call, in the
+ * order the triangles were given.  *n gets the count (<= cap written). */
+int amb_batch_times(amb_ctx *ctx, double *ms, int cap, int *n);
+
+/* Drop every cached filtered plane of a cube (they are rebuilt on demand).
+ * Planes stay resident across calls otherwise, as the serving loop wants. */
+int amb_drop_planes(amb_ctx *ctx, int cube_id);
+
 /* Per-stage device time and work since the last reset (HIP events on the
  * library's stream). */
 typedef struct amb_stats {

@@ -31,7 +31,7 @@ _lib.orc_bispec.argtypes = [C.c_long, C.c_long, C.c_double, _dp, _dp, _dp, _dp, 
                             _u8, C.c_int, _dp, _dp, C.POINTER(C.c_int32), C.c_int]
 _lib.orc_bispec_cells.argtypes = [C.c_long, C.c_long, C.c_double, _dp, _dp, _dp, _dp, C.c_int, C.c_int,
                                   C.c_int, _u8, C.c_int, C.c_int, _ip, _ip, _dp, _dp,
-                                  C.POINTER(C.c_int32), C.c_int]
+                                  C.POINTER(C.c_int32), _dp, C.c_int]
 _lib.orc_refalgo.restype = C.c_long
 _lib.orc_refalgo.argtypes = [C.c_long, C.c_long, C.c_double, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int,
                              _u8, C.c_int, C.c_int, C.c_int, C.c_int, _fp, _fp, C.POINTER(C.c_int32),
@@ -123,18 +123,22 @@ def bispec(N, nc, Du, gv, samp, lo, hi, r1, r2, r3, mask, nc10, want_norm=False,
 
 
 def bispec_cells(N, nc, Du, gv, samp, lo, hi, r1, r2, r3, mask, nc10, d2, d3, want_bl=True,
-                 want_norm=False, nthreads=0):
+                 want_norm=False, nthreads=0, want_abs=False):
+    """Returns (bl, cnt, norm) or, with want_abs, (bl, cnt, norm, sum|terms|)."""
     d2 = np.ascontiguousarray(d2, np.int32); d3 = np.ascontiguousarray(d3, np.int32)
     n = len(d2)
     bl = np.zeros(n) if want_bl else None
     norm = np.zeros(n) if want_norm else None
+    ab = np.zeros(n) if (want_abs and want_bl) else None
     cnt = np.zeros(n, np.int32)
     rc = _lib.orc_bispec_cells(N, nc, Du, _p(_f64(gv), _dp), _p(_f64(samp), _dp), _p(_f64(lo), _dp),
                                _p(_f64(hi), _dp), r1, r2, r3, _p(np.ascontiguousarray(mask, np.uint8), _u8),
                                nc10, n, _p(d2, _ip), _p(d3, _ip), _p(bl, _dp), _p(norm, _dp),
-                               cnt.ctypes.data_as(C.POINTER(C.c_int32)), nthreads)
+                               cnt.ctypes.data_as(C.POINTER(C.c_int32)), _p(ab, _dp), nthreads)
     if rc != 0:
         raise RuntimeError("orc_bispec_cells failed")
+    if want_abs:
+        return bl, cnt, norm, ab
     return bl, cnt, norm
 
 

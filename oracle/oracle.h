@@ -85,12 +85,13 @@ int orc_bispec(long N, long nc, double Du, const double *re_im, const double *sa
                double *bl64, double *norm64, int32_t *cnt, int nthreads);
 
 /* Same sums evaluated only at ncell lag cells (d2[i], d3[i]) (lags, not
- * offsets), for large configurations.  out_bl/out_norm may be NULL. */
+ * offsets), for large configurations.  out_bl/out_norm/out_abs may be NULL;
+ * out_abs gets sum |D1 D2 D3| over the same terms (the scale of the sum). */
 int orc_bispec_cells(long N, long nc, double Du, const double *re_im, const double *samp,
                      const double *ulow, const double *uhigh, int r1, int r2, int r3,
                      const uint8_t *chan_mask, int nc10, int ncell,
                      const int *d2, const int *d3, double *out_bl, double *out_norm,
-                     int32_t *out_cnt, int nthreads);
+                     int32_t *out_cnt, double *out_abs, int nthreads);
 
 /* Reference-algorithm CPU path (amber.c:187-304 / amber_v2.c:187-309):
  * FP64 c2r per plane cast to float, the 6-int triplet table

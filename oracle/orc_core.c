@@ -339,7 +339,7 @@ int orc_bispec_cells(long N, long nc, double Du, const double *re_im, const doub
                      const double *ulow, const double *uhigh, int r1, int r2, int r3,
                      const uint8_t *chan_mask, int nc10, int ncell,
                      const int *d2s, const int *d3s, double *out_bl, double *out_norm,
-                     int32_t *out_cnt, int nthreads)
+                     int32_t *out_cnt, double *out_abs, int nthreads)
 {
     long Ng = orc_ng(uhigh[r1], Du), P = Ng * Ng;
     nthreads = threads_or_default(nthreads);
@@ -361,7 +361,7 @@ int orc_bispec_cells(long N, long nc, double Du, const double *re_im, const doub
 #pragma omp parallel for schedule(dynamic) num_threads(nthreads)
     for (int c = 0; c < ncell; ++c) {
         long d2 = d2s[c], d3 = d3s[c];
-        double sb = 0, sn = 0;
+        double sb = 0, sn = 0, sa = 0;
         int n = 0;
         if (d2 < -nc10 || d2 > nc10 || d3 < -nc10 || d3 > nc10) goto done;
         for (long p1 = 0; p1 < nc; ++p1) {
@@ -370,9 +370,14 @@ int orc_bispec_cells(long N, long nc, double Du, const double *re_im, const doub
                 p3 < 0 || p3 >= nc || !chan_mask[p3]) continue;
             if (out_bl) {
                 const double *a = D[0] + p1 * P, *b = D[1] + p2 * P, *e = D[2] + p3 * P;
-                double s = 0;
-                for (long x = 0; x < P; ++x) s += a[x] * b[x] * e[x];
+                double s = 0, t = 0;
+                for (long x = 0; x < P; ++x) {
+                    const double v = a[x] * b[x] * e[x];
+                    s += v;
+                    t += fabs(v);
+                }
                 sb += s;
+                sa += t;
             }
             if (out_norm) {
                 const double *a = S[0] + p1 * P, *b = S[1] + p2 * P, *e = S[2] + p3 * P;
@@ -384,6 +389,7 @@ int orc_bispec_cells(long N, long nc, double Du, const double *re_im, const doub
         }
     done:
         if (out_bl) out_bl[c] = sb;
+        if (out_abs) out_abs[c] = sa;
         if (out_norm) out_norm[c] = sn;
         if (out_cnt) out_cnt[c] = n;
     }

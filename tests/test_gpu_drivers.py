@@ -1,0 +1,95 @@
+"""Drop-in drivers end to end: `amber` and `amber_v2` run in a working
+directory laid out like the reference's 4_fbe/ (input_params_ess,
+results/v1, results/v2) over files written in the reference formats, and
+their result files are checked with the reference readers' byte layouts
+(mabs_funcs.py:94-205) against the FP64 oracle."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import amber_amd as A
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+BIN = os.path.join(os.path.dirname(A.LIB_PATH), "bin")
+
+
+def _layout(tmp, N, nc, nflag, seed, lo, hi):
+    data = tmp / "data"
+    data.mkdir()
+    for d in ("results/v1", "results/v2"):
+        (tmp / d).mkdir(parents=True)
+    gv, samp, uf = A.generate(N, nc, nflag=nflag, seed=seed)
+    A.write_cube_files(str(data), 1, N, nc, 1.0, 1 if nflag else 0, gv, samp)
+    A.write_uf_chans(str(data / "uf_chans.txt"), uf)
+    (tmp / "input_params_ess").write_text("154.255e6\t40.00e03\t385\t%s\n" % data)
+    with open(tmp / "rings.txt", "w") as f:
+        for a, b in zip(lo, hi):
+            f.write("%.17g %.17g\n" % (a, b))
+    return gv, samp, uf
+
+
+def _run(tmp, prog, *args):
+    r = subprocess.run([os.path.join(BIN, prog), "1", "--rings", "rings.txt"] + list(args), cwd=tmp,
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:]
+    return r.stdout
+
+
+def test_amber_v1_and_v2_drivers(tmp_path):
+    N, nc, nflag, nc10 = 64, 10, 2, 6
+    lo, hi = A.log_rings(2, 20, 5)
+    gv, samp, uf = _layout(tmp_path, N, nc, nflag, 31, lo, hi)
+    print("case drivers N=%d nc=%d" % (N, nc), flush=True)
+    _run(tmp_path, "amber", "--nc10", str(nc10), "--uf-nc", str(len(uf)))
+    # closed_rings / rcount exactly as calCR would write them (exact integers * N^2)
+    n, rd = A.read_rings(str(tmp_path / "results/v1"))
+    ref = O.calcr(N, nc, samp, lo, hi, plane=5)
+    assert n == len(ref) and np.array_equal(rd[:, :3], ref[:, :3])
+    assert np.array_equal(rd[:, 3], np.rint(ref[:, 3] / N ** 2 * 8) / 8 * N ** 2)
+    mask = O.chan_mask(nc, 1, uf)
+    W = 2 * nc10 + 1
+    for (r1, r2, r3, num) in rd[::3]:
+        r1, r2, r3 = int(r1), int(r2), int(r3)
+        f = tmp_path / "results/v1" / ("bsout_final_ra%d_rb%d_rc%d_R1" % (r1, r2, r3))
+        with open(f, "rb") as fh:
+            ntri = np.fromfile(fh, np.float64, 1)[0]
+            Num = np.fromfile(fh, np.float64, 1)[0]
+            bl = np.fromfile(fh, np.float32, W * W).reshape(W, W)
+            cnt = np.fromfile(fh, np.int32, W * W).reshape(W, W)
+        Ng = A.ng(hi[r1])
+        gfact = (1.0 * Ng) / (1.0 * N)
+        assert Num == gfact * gfact * num            # amber.c:198-200
+        assert ntri == Num * (1.0 / (Ng * Ng * 1.0))  # amber.c:201,287
+        b64, c64, _ = O.bispec(N, nc, 1.0, gv, samp, lo, hi, r1, r2, r3, mask, nc10)
+        assert np.array_equal(cnt, c64)
+        assert np.linalg.norm(bl - b64) <= 1e-5 * np.linalg.norm(b64)
+    # amber_v2 reads results/v2/{rcount,closed_rings}, which it never writes
+    for fn in ("rcount", "closed_rings"):
+        shutil.copy(tmp_path / "results/v1" / fn, tmp_path / "results/v2" / fn)
+    _run(tmp_path, "amber_v2", "--nc10", str(nc10), "--uf-nc", str(len(uf)))
+    for (r1, r2, r3, num) in rd[::4]:
+        r1, r2, r3 = int(r1), int(r2), int(r3)
+        f = tmp_path / "results/v2" / ("bsout_final2_ra%d_rb%d_rc%d_R1" % (r1, r2, r3))
+        with open(f, "rb") as fh:
+            inv = np.fromfile(fh, np.float64, 1)[0]
+            bl = np.fromfile(fh, np.float32, W * W).reshape(W, W)
+            norm = np.fromfile(fh, np.float32, W * W).reshape(W, W)
+        Ng = A.ng(hi[r1])
+        assert inv == 1.0 / (Ng * Ng * 1.0)
+        b64, c64, n64 = O.bispec(N, nc, 1.0, gv, samp, lo, hi, r1, r2, r3, mask, nc10, want_norm=True)
+        P = Ng * Ng
+        assert np.array_equal(norm, (P * np.rint(n64 / P * 8) / 8).astype(np.float32))
+        assert np.linalg.norm(bl - b64) <= 1e-5 * np.linalg.norm(b64)
+
+
+def test_driver_errors_are_reported(tmp_path):
+    # missing cube file: the driver exits non-zero with a message, no crash
+    (tmp_path / "results/v1").mkdir(parents=True)
+    (tmp_path / "input_params_ess").write_text("1 1 1 %s\n" % (tmp_path / "nowhere"))
+    r = subprocess.run([os.path.join(BIN, "amber"), "1"], cwd=tmp_path, stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, text=True, timeout=60)
+    assert r.returncode != 0 and "GV_data_1" in r.stdout

@@ -53,6 +53,57 @@ def test_small_v1_v2_vs_oracle(ctx, prec):
     ctx.release_cube(cube)
 
 
+def test_fp8_form(ctx):
+    # e4m3 carries 3 mantissa bits: the contraction error is a few 1e-2 of
+    # |B|; the counts and the (BF16x3) normalisation stay exact
+    N, nc, nc10 = 48, 10, 9
+    lo, hi = A.log_rings(2, 14, 4)
+    ctx.set_precision("fp8")
+    cube, gv, samp, uf, mask = _setup(ctx, N, nc, 1, 41, lo, hi, nc10)
+    worst = 0.0
+    for (r1, r2, r3) in A.triangles(lo, hi)[::3]:
+        res = ctx.bispec_ex(cube, int(r1), int(r2), int(r3), want_norm=True)
+        bl64, cnt64, n64 = O.bispec(N, nc, 1.0, gv, samp, lo, hi, r1, r2, r3, mask, nc10, want_norm=True)
+        assert np.array_equal(res["cnt"], cnt64)
+        P = res["ng"] ** 2
+        assert np.array_equal(res["norm64"], _exact_norm(n64, P))
+        if np.linalg.norm(bl64) > 0:
+            worst = max(worst, _relerr(res["bl64"], bl64))
+    print("fp8 worst normwise rel err %.3g" % worst, flush=True)
+    assert worst <= 0.15
+    ctx.set_precision("bf16x3")
+    ctx.release_cube(cube)
+
+
+@pytest.mark.parametrize("prec", ["bf16x3", "fp32", "fp8"])
+def test_r1_eq_r2_symmetry_is_bit_exact(ctx, prec, monkeypatch):
+    """With r1 == r2 the kernel skips the row tiles below the reference
+    planes and the reduction reads C[p2][p1][p3]; the A operand is the same
+    product of the same two floats, so results match the full computation
+    bit for bit."""
+    N, nc, nc10 = 48, 300, 299
+    lo, hi = A.log_rings(2, 14, 4)
+    ctx.set_precision(prec)
+    cube, gv, samp, uf, mask = _setup(ctx, N, nc, 5, 57, lo, hi, nc10)
+    monkeypatch.setenv("AMB_NO_SYM", "1")
+    full = A.Context(device=0, precision=prec)
+    monkeypatch.delenv("AMB_NO_SYM")
+    cube2 = full.load_cube(N, nc, 1.0, 1, gv, samp)
+    full.set_channels(cube2, uf, nc10)
+    full.set_rings(lo, hi)
+    tris = [t for t in A.triangles(lo, hi) if t[0] == t[1]]
+    assert tris
+    for (r1, r2, r3) in tris[-3:]:
+        a = ctx.bispec_ex(cube, int(r1), int(r2), int(r3), want_norm=(prec != "fp8"))
+        b = full.bispec_ex(cube2, int(r1), int(r2), int(r3), want_norm=(prec != "fp8"))
+        assert np.array_equal(a["bl64"], b["bl64"]) and np.array_equal(a["cnt"], b["cnt"])
+        if prec != "fp8":
+            assert np.array_equal(a["norm64"], b["norm64"])
+    full.close()
+    ctx.set_precision("bf16x3")
+    ctx.release_cube(cube)
+
+
 def test_window_and_flags(ctx):
     # nc10 < nc-1 exercises per-p1 windows; flagged channels drop out
     N, nc, nc10 = 32, 12, 3
