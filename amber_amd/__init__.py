@@ -31,7 +31,7 @@ EXPORTS = [
     "amb_triangles", "amb_generate", "amb_create", "amb_destroy", "amb_last_error", "amb_set_precision",
     "amb_load_cube", "amb_load_cube_files", "amb_release_cube", "amb_set_channels", "amb_set_rings",
     "amb_closed_rings", "amb_bispec_v1", "amb_bispec_v2", "amb_bispec_ex", "amb_bispec_batch",
-    "amb_get_stats", "amb_reset_stats", "amb_drop_planes", "amb_batch_times",
+    "amb_get_stats", "amb_reset_stats", "amb_drop_planes", "amb_batch_times", "amb_ring_image",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -57,7 +57,7 @@ class Stats(C.Structure):
     _fields_ = [("ms_fft", C.c_double), ("ms_contract", C.c_double), ("ms_reduce", C.c_double),
                 ("ms_norm", C.c_double), ("ms_total", C.c_double), ("flop_contract", C.c_double),
                 ("bytes_fft", C.c_double), ("n_contract_launch", C.c_int64), ("n_fft_launch", C.c_int64),
-                ("evals", C.c_int64), ("ms_contract_max_launch", C.c_double)]
+                ("evals", C.c_int64), ("flop_contract_exec", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -84,6 +84,7 @@ _lib.amb_bispec_v2.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _
 _lib.amb_bispec_ex.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(Result)]
 _lib.amb_bispec_batch.argtypes = [C.c_void_p, C.c_int, _ip, C.c_int, _fp, _ip, _fp]
 _lib.amb_drop_planes.argtypes = [C.c_void_p, C.c_int]
+_lib.amb_ring_image.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int64, C.c_int, C.c_int, _fp, C.c_int64]
 _lib.amb_batch_times.argtypes = [C.c_void_p, _dp, C.c_int, _ip]
 _lib.amb_get_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
 _lib.amb_reset_stats.argtypes = [C.c_void_p]
@@ -338,6 +339,16 @@ class Context:
         res = {k: v.reshape(W, W) for k, v in out.items()}
         res.update(inv_ng2=r.inv_ng2, ng=r.ng, cert_margin=r.cert_margin)
         return res
+
+    def ring_image(self, cube, ring, plane, kind=0, ng=0):
+        """Ring-filtered image of one plane (kind 0 data, 1 sampling) on an
+        ng x ng grid (ng=0: the ring's own Ng), as fill_s2d_float /
+        fill_s2I (amber_funcs.c:967-1010, 1109-1131) produce it."""
+        du = float(self._meta.get(cube, {}).get("Du", 1.0))
+        n = int(ng) if ng > 0 else int(_lib.amb_ng(float(self._hi[ring]), du))
+        out = np.zeros(n * n, np.float32)
+        self._c(_lib.amb_ring_image(self._h, cube, int(ring), n, int(plane), int(kind), _p(out, _fp), n * n))
+        return out.reshape(n, n)
 
     def drop_planes(self, cube):
         self._c(_lib.amb_drop_planes(self._h, cube))

@@ -43,6 +43,7 @@ struct Cube {
     int *d_act = nullptr;      /* [nact] */
     int *d_inv = nullptr;      /* inv[nc] then first_ge[nc+1] */
     int *d_wlo = nullptr, *d_whi = nullptr;
+    std::vector<int> wlo, whi; /* host copies of the per-plane windows */
     int nact() const { return (int)act.size(); }
 };
 
@@ -266,7 +267,7 @@ static long plane_pitch(long ng) { return (ng * ng + 63) / 64 * 64; }
  * 5: per-plane FP8 scales of kind 0 (A side [nact], then B side [nact]). */
 static int get_planes(amb_ctx *c, int cube_id, Cube &cb, int kind, int r, int ng, void **out);
 
-static int get_fp8(amb_ctx *c, int cube_id, Cube &cb, int kind, int r, int ng, void **out, const RingGeom &g)
+static int get_scaled(amb_ctx *c, int cube_id, Cube &cb, int kind, int r, int ng, void **out, const RingGeom &g)
 {
     const int nact = cb.nact();
     const size_t pitch = plane_pitch(ng);
@@ -276,13 +277,14 @@ static int get_fp8(amb_ctx *c, int cube_id, Cube &cb, int kind, int r, int ng, v
     if (kind == 4 && (rc = get_planes(c, cube_id, cb, 5, r, ng, &sc))) return rc;
     const size_t bytes = kind == 4 ? pitch * nact : sizeof(float) * 2 * nact;
     void *d = nullptr;
-    if ((rc = dev_alloc(c, &d, bytes, "fp8 planes"))) return rc;
+    if ((rc = dev_alloc(c, &d, bytes, "scaled planes"))) return rc;
+    const float *D0 = (const float *)src;
     hipError_t e = kind == 4
-        ? amb_launch_fp8_quant(c->stream, (const float *)src, (long)pitch, nact, (const float *)sc + nact, (uint8_t *)d)
-        : amb_launch_fp8_scales(c->stream, (const float *)src, (long)pitch, nact, (float *)d);
+        ? amb_launch_fp8_quant(c->stream, D0, (long)pitch, nact, (const float *)sc + nact, (uint8_t *)d)
+        : amb_launch_fp8_scales(c->stream, D0, (long)pitch, nact, (float *)d);
     if (e != hipSuccess) {
         dev_free(c, d, bytes);
-        return fail(c, AMB_EHIP, "fp8 preparation: %s", hipGetErrorString(e));
+        return fail(c, AMB_EHIP, "scaled plane preparation (kind %d): %s", kind, hipGetErrorString(e));
     }
     PlaneSet p;
     p.cube = cube_id; p.kind = kind; p.ng = ng; p.lo = g.lo; p.hi = g.hi;
@@ -306,7 +308,7 @@ static int get_planes(amb_ctx *c, int cube_id, Cube &cb, int kind, int r, int ng
             return AMB_OK;
         }
     }
-    if (kind >= 4) return get_fp8(c, cube_id, cb, kind, r, ng, out, g);
+    if (kind >= 4) return get_scaled(c, cube_id, cb, kind, r, ng, out, g);
     const int nact = cb.nact();
     const size_t P = (size_t)ng * ng, pitch = plane_pitch(ng);
     const size_t bytes = sizeof(float) * pitch * nact; /* FP32, or hi+lo BF16 */
@@ -315,6 +317,9 @@ static int get_planes(amb_ctx *c, int cube_id, Cube &cb, int kind, int r, int ng
     float2 *tw = nullptr;
     FftPlan pl;
     if (kind < 2) {
+        /* one transform of ng complex values plus twiddles and a second
+           buffer must fit a workgroup's 64 KiB of LDS */
+        if (24L * ng > 65536) return fail(c, AMB_EARG, "grid %d exceeds the LDS transform size (2730)", ng);
         if ((rc = twiddles(c, ng, &tw, &pl))) return rc;
         size_t gneed = sizeof(float2) * (size_t)nact * (size_t)std::max(g.jcnt, 1) * ng;
         if ((rc = scratch(c, &c->G, &c->G_bytes, gneed, "fft scratch"))) return rc;
@@ -591,6 +596,8 @@ int amb_set_channels(amb_ctx *c, int id, const int *uf, int uf_nc, int nc10)
     HIPCHK(c, hipMemcpy(cb->d_inv, invfg.data(), sizeof(int) * invfg.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(cb->d_wlo, wlo.data(), sizeof(int) * na, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(cb->d_whi, whi.data(), sizeof(int) * na, hipMemcpyHostToDevice));
+    cb->wlo = wlo;
+    cb->whi = whi;
     cb->nc10 = nc10;
     cb->chans = true;
     return AMB_OK;
@@ -715,10 +722,11 @@ static int bispec_core(amb_ctx *c, int id, int r1, int r2, int r3, int want_norm
     void *D[3], *S[3] = {nullptr, nullptr, nullptr}, *SC[3] = {nullptr, nullptr, nullptr};
     const int rr[3] = {r1, r2, r3};
     const int d3kind = prec == AMB_PREC_FP32 ? 0 : prec == AMB_PREC_BF16X3 ? 2 : 4;
+    const int sckind = prec == AMB_PREC_FP8 ? 5 : -1;
     for (int k = 0; k < 3; ++k) {
         const int kind = k == 2 ? d3kind : 0;
         if ((rc = get_planes(c, id, *cb, kind, rr[k], (int)ng, &D[k]))) { unpin_all(c); return rc; }
-        if (prec == AMB_PREC_FP8 && (rc = get_planes(c, id, *cb, 5, rr[k], (int)ng, &SC[k]))) {
+        if (sckind >= 0 && (rc = get_planes(c, id, *cb, sckind, rr[k], (int)ng, &SC[k]))) {
             unpin_all(c);
             return rc;
         }
@@ -748,8 +756,10 @@ static int bispec_core(amb_ctx *c, int id, int r1, int r2, int r3, int want_norm
     a.P = P; a.pitch = pitch; a.nact = nact; a.wlo = cb->d_wlo; a.whi = cb->d_whi; a.sym12 = sym12;
     a.D1 = (const float *)D[0]; a.D2 = (const float *)D[1]; a.C = c->C;
     if (prec == AMB_PREC_FP32) a.D3 = (const float *)D[2];
-    else if (prec == AMB_PREC_BF16X3) { a.H3 = (const __bf16 *)D[2]; a.L3 = a.H3 + (size_t)pitch * nact; }
-    else {
+    else if (prec == AMB_PREC_BF16X3) {
+        a.H3 = (const __bf16 *)D[2];
+        a.L3 = a.H3 + (size_t)pitch * nact;
+    } else {
         a.Q3 = (const uint8_t *)D[2];
         a.s1 = (const float *)SC[0];
         a.s2 = (const float *)SC[1];
@@ -811,6 +821,8 @@ static int bispec_core(amb_ctx *c, int id, int r1, int r2, int r3, int want_norm
     }
     c->st.evals += ev;
     c->st.flop_contract += 2.0 * pairs * (double)P;
+    c->st.flop_contract_exec += (double)amb_contract_live_items(nact, cb->wlo.data(), cb->whi.data(), sym12) *
+                                amb_contract_item_flops(prec, pitch);
     if (want_norm) {
         float m;
         memcpy(&m, &mbits, sizeof m);
@@ -897,6 +909,35 @@ int amb_batch_times(amb_ctx *c, double *ms, int cap, int *n)
     std::lock_guard<std::mutex> lk(c->mu);
     *n = (int)c->batch_ms.size();
     for (int i = 0; i < *n && i < cap && ms; ++i) ms[i] = c->batch_ms[i];
+    return AMB_OK;
+}
+
+int amb_ring_image(amb_ctx *c, int id, int ring, int64_t ng, int plane, int kind, float *out, int64_t cap)
+{
+    if (!c || !out) return AMB_EARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    hipSetDevice(c->device);
+    Cube *cb;
+    int rc = find_cube(c, id, &cb);
+    if (rc) return rc;
+    if (!cb->chans) return fail(c, AMB_ESTATE, "channels not set for cube %d", id);
+    if (ring < 0 || ring >= (int)c->ulow.size()) return fail(c, AMB_EARG, "ring %d outside the ring set", ring);
+    if (kind != 0 && kind != 1) return fail(c, AMB_EARG, "image kind %d (0 data, 1 sampling)", kind);
+    if (plane < 0 || plane >= cb->nc) return fail(c, AMB_EARG, "plane %d outside [0,%lld)", plane, (long long)cb->nc);
+    const int ia = cb->inv[plane];
+    if (ia < 0) return fail(c, AMB_EARG, "plane %d is not in the channel set", plane);
+    if (ng <= 0) ng = amb_ng(c->uhigh[ring], cb->Du);
+    if (ng < 1 || ng > 2730) return fail(c, AMB_EARG, "grid %lld outside [1,2730]", (long long)ng);
+    const size_t P = (size_t)ng * ng;
+    if (cap < (int64_t)P) return fail(c, AMB_EARG, "output holds %lld floats, %zu needed", (long long)cap, P);
+    void *d = nullptr;
+    rc = get_planes(c, id, *cb, kind, ring, (int)ng, &d);
+    unpin_all(c);
+    if (rc) return rc;
+    const float *src = (const float *)d + (size_t)ia * plane_pitch((int)ng);
+    HIPCHK(c, hipMemcpyAsync(out, src, sizeof(float) * P, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    timers_collect(c);
     return AMB_OK;
 }
 

@@ -48,9 +48,10 @@ struct ContractArgs {
     int *Cq;                   /* norm: [nact][nact][nact] int32 of 8*count */
     unsigned *margin;          /* norm: max |S/P - q| as float bits */
 };
-/* var: kernel variant (bit 0: 8 waves with half tiles per wave, bit 1:
- * three-stage pipelined k-steps); chosen per shape by the autotuner */
-#define AMB_NVAR 4
+/* var: kernel variant (0: plain k-steps, 1: BF16x3 k-steps interleaving
+ * MFMA with the next block's A formation); chosen per shape by the
+ * autotuner */
+#define AMB_NVAR 2
 hipError_t amb_launch_contract(hipStream_t s, int prec, int norm, int var, const ContractArgs &a);
 hipError_t amb_launch_split(hipStream_t s, const float *D, size_t n, __bf16 *H, __bf16 *L);
 /* FP8 preparation of n planes: sc[0..n) A-side scales, sc[n..2n) B-side
@@ -75,6 +76,11 @@ hipError_t amb_launch_reduce(hipStream_t s, const ReduceArgs &a);
 
 /* ---- many triangles per launch (batches, serving) ---- */
 long amb_contract_items(int nact); /* work items of one triangle */
+/* items the kernel computes (not skipped by its window and r1 == r2 tests),
+ * from host copies of the per-plane windows */
+long amb_contract_live_items(int nact, const int *wlo, const int *whi, int sym12);
+/* MFMA flops one computed item issues at pixel pitch `pitch` */
+double amb_contract_item_flops(int prec, long pitch);
 bool amb_contract_args_ok(int prec, int norm, const ContractArgs &a);
 /* persistent contraction over ntri triangles: descs/begin in device memory,
  * begin[t] = first item of triangle t (largest first), begin[ntri] = total;

@@ -11,7 +11,7 @@ Default workload (the metric's configuration): 768 channels x 512^2 grid,
 (nc10 = 767), split-BF16 contraction.
 
     python bench.py [--gpus 1] [--steps 1] [--warmup 1] [--config cfg3]
-                    [--precision bf16x3|fp32] [--no-cpu-baseline]
+                    [--precision bf16x3|fp32|fp8] [--no-cpu-baseline]
 """
 import argparse
 import json
@@ -33,7 +33,9 @@ CONFIGS = {
                  desc="precision: 768 ch x 512^2, 24 log bins 1-200, all lags"),
 }
 
-PEAK_TFLOPS = {"bf16x3": 2500.0, "fp32": 157.3, "fp8": 5000.0}   # dense MFMA, published
+# published dense MFMA peaks of the instruction each form issues; one
+# algorithmic BF16x3 flop costs three BF16 MFMA flops (mfma_passes)
+PEAK_TFLOPS = {"bf16x3": 2500.0, "fp32": 157.3, "fp8": 5000.0}
 PASSES = {"bf16x3": 3, "fp32": 1, "fp8": 1}
 HBM_PEAK_GBS = 8000.0
 
@@ -55,7 +57,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB three times (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calib:
+        copy_calib(args.copy_calib)
+        return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported (the path would shard by "
               "triangle with no collective, see the design notes)" % args.gpus, file=sys.stderr)
@@ -121,7 +128,12 @@ def main():
             "frac": round(ach / peak, 4), "traffic": None,
             "kernel": "k_contract<%s>" % args.precision,
             "mfma_passes": PASSES[args.precision],
-            "avg_launch_ms": round(ms_c / max(st["n_contract_launch"], 1), 3)}
+            "avg_launch_ms": round(ms_c / max(st["n_contract_launch"], 1), 3),
+            # MFMA work actually issued (whole tiles, padded pitch, all passes;
+            # r1 == r2 tiles skipped) against the dense peak of the instruction
+            "issued_tflops": round(st["flop_contract_exec"] / (ms_c * 1e-3) / 1e12, 2) if ms_c > 0 else None,
+            "issued_frac_of_instr_peak": round(st["flop_contract_exec"] / (ms_c * 1e-3) / 1e12 /
+                                               peak, 4) if ms_c > 0 else None}
     stages = {"fft_ms": round(st["ms_fft"], 2), "contract_ms": round(st["ms_contract"], 2),
               "reduce_ms": round(st["ms_reduce"], 2),
               "fft_gbs": round(st["bytes_fft"] / (st["ms_fft"] * 1e-3) / 1e9, 1) if st["ms_fft"] > 0 else None}
@@ -157,6 +169,19 @@ def main():
     }
     ctx.close()
     print(json.dumps(line), flush=True)
+
+
+def copy_calib(mib):
+    """A plain HBM copy of known size, for scaling the FETCH_SIZE/WRITE_SIZE
+    counters of the contraction: each dispatch reads and writes mib MiB."""
+    import torch
+    n = mib << 20
+    src = torch.ones(n, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calib_bytes": n, "dispatches": 3}), flush=True)
 
 
 def fp64_error(A, ctx, cube, cfg, gv, samp, uf, fk, lo, hi, tris):

@@ -190,6 +190,16 @@ int amb_bispec_batch(amb_ctx *ctx, int cube_id, const int *tri, int ntri,
  * order the triangles were given.  *n gets the count (<= cap written). */
 int amb_batch_times(amb_ctx *ctx, double *ms, int cap, int *n);
 
+/* One ring-filtered real-space image of plane `plane`, as the triple
+ * product reads it: kind 0 is the data image of fill_s2d_float
+ * (amber_funcs.c:967-1010), kind 1 the sampling image of fill_s2I
+ * (amber_funcs.c:1109-1131).  ng <= 0 takes Ng of ring `ring` itself
+ * (amber.c:193-197 computes Ng from the triangle's r1).  out: ng*ng f32,
+ * row-major [x][y]; cap is its length in floats.  The plane must belong to
+ * the channel set (flagged planes are never built). */
+int amb_ring_image(amb_ctx *ctx, int cube_id, int ring, int64_t ng, int plane, int kind,
+                   float *out, int64_t cap);
+
 /* Drop every cached filtered plane of a cube (they are rebuilt on demand).
  * Planes stay resident across calls otherwise, as the serving loop wants. */
 int amb_drop_planes(amb_ctx *ctx, int cube_id);
@@ -206,7 +216,11 @@ typedef struct amb_stats {
     double bytes_fft;       /* algorithmic bytes of the fft stage */
     int64_t n_contract_launch, n_fft_launch;
     int64_t evals;          /* (triangle, lag cell) pairs with cnt > 0 */
-    double ms_contract_max_launch;
+    double flop_contract_exec; /* MFMA work issued by the data contraction:
+                                  whole 128x128x4 tiles over the padded pixel
+                                  pitch, times the passes of the form (3 for
+                                  BF16x3); below flop_contract x passes when
+                                  r1 == r2 tiles are skipped */
 } amb_stats;
 int amb_get_stats(amb_ctx *ctx, amb_stats *st);
 int amb_reset_stats(amb_ctx *ctx);

@@ -169,3 +169,70 @@ def test_cfg1_plain_small(ctx, prec):
     print("cfg1 worst normwise rel err %.3g" % worst, flush=True)
     assert worst <= TOL[prec]
     ctx.release_cube(cube)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_large_grid_cells(ctx, prec):
+    """A 150-200 ring bin on a 512^2 grid (Ng = 612, 375k pixels per
+    plane, |B| about 1e-3 of the sum of |terms|): the FP64 oracle on a
+    spread of lag cells, compared normwise over the cells and per cell
+    against the sum of |terms|.  The reference's own FP32 arithmetic (the
+    loop restated in oracle/orc_refalgo.c) is scored on the same cells:
+    the FP32 form must do at least as well."""
+    N, nc, nc10 = 512, 24, 23
+    lo = np.array([150.0]); hi = np.array([200.0])
+    ctx.set_precision(prec)
+    cube, gv, samp, uf, mask = _setup(ctx, N, nc, 2, 77, lo, hi, nc10)
+    res = ctx.bispec_ex(cube, 0, 0, 0)
+    W = 2 * nc10 + 1
+    rng = np.random.default_rng(5)
+    i2 = rng.integers(0, W, 24)
+    i3 = rng.integers(0, W, 24)
+    b64, c64, _, ab = O.bispec_cells(N, nc, 1.0, gv, samp, lo, hi, 0, 0, 0, mask, nc10, i2 - nc10, i3 - nc10,
+                                     want_abs=True)
+    got = res["bl64"][i2, i3]
+    assert np.array_equal(res["cnt"][i2, i3], c64)
+    live = c64 > 0
+    e = _relerr(got[live], b64[live])
+    cell = np.max(np.abs(got[live] - b64[live]) / np.maximum(ab[live], 1e-300))
+    rb = O.refalgo(N, nc, 1.0, gv, samp, lo, hi, 0, 0, 0, mask, nc10)[0][i2, i3].astype(np.float64)
+    e_ref = _relerr(rb[live], b64[live])
+    print("large grid Ng=%d prec=%s normwise %.3g per-cell/|terms| %.3g (reference FP32 loop: %.3g)"
+          % (res["ng"], prec, e, cell, e_ref), flush=True)
+    assert cell <= 2e-8
+    assert e <= TOL[prec]
+    if prec == "fp32":
+        assert e <= e_ref
+    ctx.set_precision("bf16x3")
+    ctx.release_cube(cube)
+
+
+@pytest.mark.parametrize("N,nc,lo,hi,ng", [(32, 6, 2.0, 9.0, 0), (64, 6, 5.0, 13.0, 0),
+                                           (64, 6, 3.0, 6.0, 48), (512, 4, 150.0, 200.0, 0)])
+def test_ring_image_vs_oracle(ctx, N, nc, lo, hi, ng):
+    """The c2r stage alone: data and sampling images of every active plane
+    against the FP64 ring gather + c2r, normwise per plane.  Ng=612
+    (2^2 3^2 17) runs the radix-17 stage; ng=48 a grid above the ring's
+    own."""
+    lo_a, hi_a = np.array([lo]), np.array([hi])
+    cube, gv, samp, uf, mask = _setup(ctx, N, nc, 1, 31, lo_a, hi_a, nc - 1)
+    n = ng if ng > 0 else O.ng(hi)
+    print("case ring image N=%d Ng=%d" % (N, n), flush=True)
+    worst = 0.0
+    for p in range(nc):
+        if not mask[p]:
+            with pytest.raises(RuntimeError):
+                ctx.ring_image(cube, 0, p, kind=0, ng=ng)
+            continue
+        for kind in (0, 1):
+            got = ctx.ring_image(cube, 0, p, kind=kind, ng=ng)
+            ref = O.ring_c2r(N, nc, n, lo, hi, p, gv=gv if kind == 0 else None,
+                             samp=samp if kind == 1 else None)
+            assert got.shape == (n, n)
+            if np.linalg.norm(ref) == 0:
+                assert not np.any(got)
+                continue
+            worst = max(worst, _relerr(got.astype(np.float64), ref))
+    print("ring image worst normwise rel err %.3g" % worst, flush=True)
+    assert worst <= 1e-6
+    ctx.release_cube(cube)
