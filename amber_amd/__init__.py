@@ -32,7 +32,11 @@ EXPORTS = [
     "amb_load_cube", "amb_load_cube_files", "amb_release_cube", "amb_set_channels", "amb_set_rings",
     "amb_closed_rings", "amb_bispec_v1", "amb_bispec_v2", "amb_bispec_ex", "amb_bispec_batch",
     "amb_get_stats", "amb_reset_stats", "amb_drop_planes", "amb_batch_times", "amb_ring_image",
+    "amb_bispec_multi", "amb_set_batch_mode", "amb_update_cube", "amb_graph_create", "amb_graph_run",
+    "amb_graph_destroy", "amb_serve_start", "amb_submit", "amb_poll", "amb_wait", "amb_serve_stop",
 ]
+
+BATCH_PER_TRIANGLE, BATCH_GROUPED = 0, 1
 
 _dp = C.POINTER(C.c_double)
 _fp = C.POINTER(C.c_float)
@@ -53,11 +57,17 @@ class Result(C.Structure):
                 ("inv_ng2", C.c_double), ("ng", C.c_long), ("cert_margin", C.c_double)]
 
 
+class TriReq(C.Structure):
+    _fields_ = [("cube_id", C.c_int), ("r1", C.c_int), ("r2", C.c_int), ("r3", C.c_int),
+                ("bl", _fp), ("cnt", _ip), ("norm", _fp)]
+
+
 class Stats(C.Structure):
     _fields_ = [("ms_fft", C.c_double), ("ms_contract", C.c_double), ("ms_reduce", C.c_double),
                 ("ms_norm", C.c_double), ("ms_total", C.c_double), ("flop_contract", C.c_double),
                 ("bytes_fft", C.c_double), ("n_contract_launch", C.c_int64), ("n_fft_launch", C.c_int64),
-                ("evals", C.c_int64), ("flop_contract_exec", C.c_double)]
+                ("evals", C.c_int64), ("flop_contract_exec", C.c_double),
+                ("ms_graph", C.c_double), ("n_graph_launch", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -86,6 +96,18 @@ _lib.amb_bispec_batch.argtypes = [C.c_void_p, C.c_int, _ip, C.c_int, _fp, _ip, _
 _lib.amb_drop_planes.argtypes = [C.c_void_p, C.c_int]
 _lib.amb_ring_image.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int64, C.c_int, C.c_int, _fp, C.c_int64]
 _lib.amb_batch_times.argtypes = [C.c_void_p, _dp, C.c_int, _ip]
+_lib.amb_bispec_multi.argtypes = [C.c_void_p, C.POINTER(TriReq), C.c_int]
+_lib.amb_set_batch_mode.argtypes = [C.c_void_p, C.c_int]
+_lib.amb_update_cube.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+_lib.amb_graph_create.argtypes = [C.c_void_p, C.POINTER(TriReq), C.c_int, C.POINTER(C.c_void_p)]
+_lib.amb_graph_run.argtypes = [C.c_void_p, C.c_void_p]
+_lib.amb_graph_destroy.argtypes = [C.c_void_p, C.c_void_p]
+_lib.amb_graph_destroy.restype = None
+_lib.amb_serve_start.argtypes = [C.c_void_p, C.c_int]
+_lib.amb_submit.argtypes = [C.c_void_p, C.POINTER(TriReq), C.c_int, C.POINTER(C.c_uint64)]
+_lib.amb_poll.argtypes = [C.c_void_p, C.c_uint64, _ip]
+_lib.amb_wait.argtypes = [C.c_void_p, C.c_uint64, _dp]
+_lib.amb_serve_stop.argtypes = [C.c_void_p]
 _lib.amb_get_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
 _lib.amb_reset_stats.argtypes = [C.c_void_p]
 _lib.amb_uavg.argtypes = [_dp, _dp, C.c_int, _dp]
@@ -368,6 +390,70 @@ class Context:
         self._c(_lib.amb_bispec_batch(self._h, cube, _p(tris, _ip), n, _p(bl, _fp), _p(cnt, _ip), _p(norm, _fp)))
         return bl, cnt, norm
 
+    def set_batch_mode(self, mode):
+        """"grouped" (one persistent launch per wave) or "per_triangle" """
+        m = {"grouped": BATCH_GROUPED, "per_triangle": BATCH_PER_TRIANGLE}[mode]
+        self._c(_lib.amb_set_batch_mode(self._h, m))
+
+    def update_cube(self, cube, gv, samp):
+        gv = np.ascontiguousarray(gv, np.float64)
+        samp = np.ascontiguousarray(samp, np.float64)
+        self._c(_lib.amb_update_cube(self._h, cube, _p(gv, _dp), _p(samp, _dp)))
+
+    def _reqs(self, reqs, want_cnt=True, want_norm=False):
+        """reqs: (cube, r1, r2, r3) tuples -> TriReq array plus owned output arrays"""
+        n = len(reqs)
+        arr = (TriReq * max(n, 1))()
+        outs = []
+        for i, (cube, r1, r2, r3) in enumerate(reqs):
+            W = self._W(cube)
+            bl = np.zeros((W, W), np.float32)
+            cnt = np.zeros((W, W), np.int32) if want_cnt else None
+            norm = np.zeros((W, W), np.float32) if want_norm else None
+            arr[i] = TriReq(int(cube), int(r1), int(r2), int(r3), _p(bl, _fp), _p(cnt, _ip), _p(norm, _fp))
+            outs.append((bl, cnt, norm))
+        return arr, outs
+
+    def bispec_multi(self, reqs, want_cnt=True, want_norm=False):
+        """Triangles of any resident cubes: reqs = [(cube, r1, r2, r3), ...];
+        returns [(bl, cnt, norm), ...] in the same order."""
+        arr, outs = self._reqs(reqs, want_cnt, want_norm)
+        self._c(_lib.amb_bispec_multi(self._h, arr, len(reqs)))
+        return outs
+
+    def graph(self, reqs, want_cnt=True, want_norm=False):
+        """A captured, replayable batch (amb_graph_create)."""
+        return Graph(self, reqs, want_cnt, want_norm)
+
+    def serve_start(self, max_wave=64):
+        self._tickets = {}
+        self._c(_lib.amb_serve_start(self._h, int(max_wave)))
+
+    def submit(self, reqs, want_cnt=True, want_norm=False):
+        """Queue a request of triangles; returns a ticket.  The outputs are
+        filled once wait(ticket) returns."""
+        arr, outs = self._reqs(reqs, want_cnt, want_norm)
+        t = C.c_uint64()
+        self._c(_lib.amb_submit(self._h, arr, len(reqs), C.byref(t)))
+        self._tickets[t.value] = (arr, outs)
+        return t.value
+
+    def poll(self, ticket):
+        d = C.c_int()
+        self._c(_lib.amb_poll(self._h, C.c_uint64(ticket), C.byref(d)))
+        return bool(d.value)
+
+    def wait(self, ticket):
+        """-> (outputs, latency_ms)"""
+        lat = C.c_double()
+        rc = _lib.amb_wait(self._h, C.c_uint64(ticket), C.byref(lat))
+        arr, outs = self._tickets.pop(ticket)
+        self._c(rc)
+        return outs, lat.value
+
+    def serve_stop(self):
+        self._c(_lib.amb_serve_stop(self._h))
+
     def batch_times(self):
         """completion time (ms from the call's start) of each triangle of the last batch"""
         n = C.c_int()
@@ -383,3 +469,24 @@ class Context:
 
     def reset_stats(self):
         self._c(_lib.amb_reset_stats(self._h))
+
+
+class Graph:
+    """amb_graph_create / amb_graph_run / amb_graph_destroy.  run() replays
+    the captured batch and returns the outputs [(bl, cnt, norm), ...]; the
+    arrays are reused by every replay."""
+
+    def __init__(self, ctx, reqs, want_cnt=True, want_norm=False):
+        self._ctx = ctx
+        self._arr, self.outputs = ctx._reqs(reqs, want_cnt, want_norm)
+        self._g = C.c_void_p()
+        ctx._c(_lib.amb_graph_create(ctx._h, self._arr, len(reqs), C.byref(self._g)))
+
+    def run(self):
+        self._ctx._c(_lib.amb_graph_run(self._ctx._h, self._g))
+        return self.outputs
+
+    def close(self):
+        if self._g and self._ctx._h:
+            _lib.amb_graph_destroy(self._ctx._h, self._g)
+        self._g = C.c_void_p()

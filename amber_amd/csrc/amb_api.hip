@@ -1,20 +1,21 @@
 /*
  * C-ABI implementation: device context, resident cubes, the filtered-plane
  * cache and the per-triangle orchestration that replaces the rp loop of
- * amber.c:187-326 / amber_v2.c:187-333.
+ * amber.c:187-326 / amber_v2.c:187-333.  Grouped waves, captured graphs
+ * and the serving queue live in amb_wave.hip.
  *
  * HBM layout (one cube, nc channels, N grid, Nh = N/2+1):
  *   vis_t [nc][Nh][N] complex64 and samp_t [nc][Nh][N] float -- plane-major,
  *     column-contiguous so the c2r column pass gathers a ring column with
  *     unit stride;
- *   filtered planes [nact][Ng*Ng] float per (cube, kind, ring edges, Ng),
- *     nact = channels of the triplet set (flagged planes are never built:
- *     they never enter a triplet, amber_funcs.c:2207-2259);
+ *   filtered planes [nact][pitch] float per (cube, kind, ring edges, Ng),
+ *     pitch = Ng*Ng rounded up to 64 with a zero tail, nact = channels of
+ *     the triplet set (flagged planes are never built: they never enter a
+ *     triplet, amber_funcs.c:2207-2259);
  *   C [nact][nact][nact] float: per-p1 contraction results, reduced over p1
  *     into the [W][W] lag arrays of the result files.
  */
-#include "amb_internal.h"
-#include "amber_mi355x.h"
+#include "amb_ctx.h"
 
 #include <algorithm>
 #include <chrono>
@@ -23,74 +24,13 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
-#include <map>
-#include <mutex>
-#include <string>
-#include <vector>
 
-namespace {
+using Lock = std::lock_guard<std::recursive_mutex>;
 
-struct Cube {
-    int64_t N = 0, nc = 0;
-    double Du = 1.0;
-    int flag_key = 0;
-    float2 *vis_t = nullptr;
-    float *samp_t = nullptr;
-    size_t bytes = 0;
-    bool chans = false;
-    int nc10 = 0;
-    std::vector<int> act, inv; /* compact <-> channel */
-    int *d_act = nullptr;      /* [nact] */
-    int *d_inv = nullptr;      /* inv[nc] then first_ge[nc+1] */
-    int *d_wlo = nullptr, *d_whi = nullptr;
-    std::vector<int> wlo, whi; /* host copies of the per-plane windows */
-    int nact() const { return (int)act.size(); }
-};
-
-struct PlaneSet {
-    int cube, kind, ng;
-    double lo, hi;
-    void *d = nullptr;
-    size_t bytes = 0;
-    uint64_t last = 0;
-    int pins = 0;
-};
-
-struct Timer {
-    hipEvent_t a = nullptr, b = nullptr;
-    double *acc = nullptr;
-};
-
-} // namespace
-
-struct amb_ctx {
-    int device = 0, prec = AMB_PREC_BF16X3, verbose = 0, sync_debug = 0, kvar = 0, no_sym = 0;
-    uint64_t budget = 0, used = 0;
-    hipStream_t stream = nullptr;
-    std::mutex mu;
-    std::string err;
-    std::vector<double> ulow, uhigh, uavg;
-    std::map<int, Cube> cubes;
-    int next_cube = 1;
-    std::vector<PlaneSet> planes;
-    uint64_t tick = 0;
-    std::map<int, float2 *> tw;
-    std::map<int, FftPlan> plans;
-    /* scratch */
-    float2 *G = nullptr; size_t G_bytes = 0;
-    float *C = nullptr; size_t C_bytes = 0;
-    int *Cq = nullptr; size_t Cq_bytes = 0;
-    unsigned *margin = nullptr;
-    void *out = nullptr; size_t out_bytes = 0;
-    /* stats */
-    amb_stats st{};
-    std::vector<double> batch_ms; /* per-triangle completion time of the last batch */
-    std::vector<Timer> timers;
-    std::vector<hipEvent_t> evpool;
-};
+namespace amb {
 
 /* ------------------------------------------------------------------ helpers */
-static int fail(amb_ctx *c, int code, const char *fmt, ...)
+int fail(amb_ctx *c, int code, const char *fmt, ...)
 {
     char buf[512];
     va_list ap;
@@ -101,14 +41,6 @@ static int fail(amb_ctx *c, int code, const char *fmt, ...)
     if (c->verbose) fprintf(stderr, "amber_mi355x: %s\n", buf);
     return code;
 }
-
-#define HIPCHK(ctx, expr)                                                                   \
-    do {                                                                                    \
-        hipError_t e_ = (expr);                                                             \
-        if (e_ != hipSuccess)                                                               \
-            return fail(ctx, AMB_EHIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
-                        __FILE__, __LINE__);                                                \
-    } while (0)
 
 static hipEvent_t ev_get(amb_ctx *c)
 {
@@ -122,7 +54,7 @@ static hipEvent_t ev_get(amb_ctx *c)
     return e;
 }
 
-static void timer_start(amb_ctx *c, double *acc)
+void timer_start(amb_ctx *c, double *acc)
 {
     Timer t;
     t.a = ev_get(c);
@@ -131,7 +63,8 @@ static void timer_start(amb_ctx *c, double *acc)
     if (t.a) hipEventRecord(t.a, c->stream);
     c->timers.push_back(t);
 }
-static void timer_stop(amb_ctx *c)
+
+void timer_stop(amb_ctx *c)
 {
     for (auto it = c->timers.rbegin(); it != c->timers.rend(); ++it)
         if (!it->b) {
@@ -140,8 +73,9 @@ static void timer_stop(amb_ctx *c)
             return;
         }
 }
+
 /* after a stream sync: fold recorded intervals into the stats */
-static void timers_collect(amb_ctx *c)
+void timers_collect(amb_ctx *c)
 {
     for (auto &t : c->timers) {
         if (t.a && t.b) {
@@ -154,8 +88,7 @@ static void timers_collect(amb_ctx *c)
     c->timers.clear();
 }
 
-static int dev_alloc(amb_ctx *c, void **p, size_t bytes, const char *what);
-static void dev_free(amb_ctx *c, void *p, size_t bytes)
+void dev_free(amb_ctx *c, void *p, size_t bytes)
 {
     if (p) {
         hipFree(p);
@@ -175,7 +108,7 @@ static bool evict_one(amb_ctx *c)
     return true;
 }
 
-static int dev_alloc(amb_ctx *c, void **p, size_t bytes, const char *what)
+int dev_alloc(amb_ctx *c, void **p, size_t bytes, const char *what)
 {
     *p = nullptr;
     if (bytes == 0) return AMB_OK;
@@ -197,23 +130,45 @@ static int dev_alloc(amb_ctx *c, void **p, size_t bytes, const char *what)
     return AMB_OK;
 }
 
-/* grow-only scratch buffer */
-template <class T>
-static int scratch(amb_ctx *c, T **p, size_t *have, size_t need, const char *what)
+/* grow-only device scratch */
+int grow(amb_ctx *c, void **p, size_t *have, size_t need, const char *what)
 {
     if (*have >= need) return AMB_OK;
     dev_free(c, *p, *have);
     *p = nullptr;
     *have = 0;
-    void *q = nullptr;
-    int rc = dev_alloc(c, &q, need, what);
+    int rc = dev_alloc(c, p, need, what);
     if (rc) return rc;
-    *p = (T *)q;
     *have = need;
     return AMB_OK;
 }
 
-static int twiddles(amb_ctx *c, int n, float2 **out, FftPlan *pl)
+/* grow-only pinned host staging */
+int grow_host(amb_ctx *c, void **p, size_t *have, size_t need)
+{
+    if (*have >= need) return AMB_OK;
+    if (*p) hipHostFree(*p);
+    *p = nullptr;
+    *have = 0;
+    if (hipHostMalloc(p, need, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return fail(c, AMB_ENOMEM, "pinned host staging of %zu bytes", need);
+    }
+    *have = need;
+    return AMB_OK;
+}
+
+template <class T>
+static int scratch(amb_ctx *c, T **p, size_t *have, size_t need, const char *what)
+{
+    void *q = *p;
+    int rc = grow(c, &q, have, need, what);
+    *p = (T *)q;
+    return rc;
+}
+
+int twiddles(amb_ctx *c, int n, float2 **out, FftPlan *pl)
 {
     auto it = c->tw.find(n);
     if (it != c->tw.end()) {
@@ -258,111 +213,143 @@ static int ring_geom(amb_ctx *c, const Cube &cb, int r, int ng, RingGeom *g)
     return AMB_OK;
 }
 
-static long plane_pitch(long ng) { return (ng * ng + 63) / 64 * 64; }
+long plane_pitch(long ng) { return (ng * ng + 63) / 64 * 64; }
 
-/* Plane sets of (cube, ring) at grid side ng, pinned on return.
- * kind 0: data images (FP32), 1: sampling images (FP32),
- * 2 / 3: hi and lo BF16 halves of kind 0 / 1, stored hi planes then lo,
- * 4: e4m3 data images times their B-side scales,
- * 5: per-plane FP8 scales of kind 0 (A side [nact], then B side [nact]). */
-static int get_planes(amb_ctx *c, int cube_id, Cube &cb, int kind, int r, int ng, void **out);
-
-static int get_scaled(amb_ctx *c, int cube_id, Cube &cb, int kind, int r, int ng, void **out, const RingGeom &g)
+static size_t plane_bytes(int kind, int nact, size_t pitch)
 {
-    const int nact = cb.nact();
-    const size_t pitch = plane_pitch(ng);
-    void *src = nullptr, *sc = nullptr;
-    int rc = get_planes(c, cube_id, cb, 0, r, ng, &src);
-    if (rc) return rc;
-    if (kind == 4 && (rc = get_planes(c, cube_id, cb, 5, r, ng, &sc))) return rc;
-    const size_t bytes = kind == 4 ? pitch * nact : sizeof(float) * 2 * nact;
-    void *d = nullptr;
-    if ((rc = dev_alloc(c, &d, bytes, "scaled planes"))) return rc;
-    const float *D0 = (const float *)src;
-    hipError_t e = kind == 4
-        ? amb_launch_fp8_quant(c->stream, D0, (long)pitch, nact, (const float *)sc + nact, (uint8_t *)d)
-        : amb_launch_fp8_scales(c->stream, D0, (long)pitch, nact, (float *)d);
-    if (e != hipSuccess) {
-        dev_free(c, d, bytes);
-        return fail(c, AMB_EHIP, "scaled plane preparation (kind %d): %s", kind, hipGetErrorString(e));
-    }
-    PlaneSet p;
-    p.cube = cube_id; p.kind = kind; p.ng = ng; p.lo = g.lo; p.hi = g.hi;
-    p.d = d; p.bytes = bytes; p.last = ++c->tick; p.pins = 1;
-    c->planes.push_back(p);
-    *out = d;
-    return AMB_OK;
+    if (kind == 4) return pitch * nact;
+    if (kind == 5) return sizeof(float) * 2 * nact;
+    return sizeof(float) * pitch * nact; /* FP32, or hi + lo BF16 */
 }
 
-static int get_planes(amb_ctx *c, int cube_id, Cube &cb, int kind, int r, int ng, void **out)
+size_t build_scratch_bytes(amb_ctx *c, const BuildOp &op)
 {
-    RingGeom g;
-    int rc = ring_geom(c, cb, r, ng, &g);
+    if (op.kind > 1) return 0;
+    const Cube &cb = c->cubes[op.cube];
+    return sizeof(float2) * (size_t)cb.nact() * (size_t)std::max(op.g.jcnt, 1) * op.g.ng;
+}
+
+hipError_t launch_build(amb_ctx *c, hipStream_t s, const BuildOp &op, float2 *G)
+{
+    const Cube &cb = c->cubes[op.cube];
+    const int nact = cb.nact();
+    const long pitch = plane_pitch(op.g.ng);
+    switch (op.kind) {
+    case 0:
+    case 1: {
+        const float2 *tw = c->tw[op.g.ng];
+        const FftPlan &pl = c->plans[op.g.ng];
+        return amb_launch_ring_c2r(s, op.kind, cb.vis_t, cb.samp_t, cb.d_act, nact, (int)cb.N, (int)(cb.N / 2 + 1),
+                                   pl, tw, op.g, G, (float *)op.dst, pitch);
+    }
+    case 2:
+    case 3: {
+        __bf16 *h = (__bf16 *)op.dst;
+        return amb_launch_split(s, (const float *)op.src, (size_t)pitch * nact, h, h + (size_t)pitch * nact);
+    }
+    case 4:
+        return amb_launch_fp8_quant(s, (const float *)op.src, pitch, nact, (const float *)op.sc + nact,
+                                    (uint8_t *)op.dst);
+    default:
+        return amb_launch_fp8_scales(s, (const float *)op.src, pitch, nact, (float *)op.dst);
+    }
+}
+
+/* algorithmic bytes of a c2r build: Ng^2 floats written per plane plus the
+   ring cells read */
+double build_fft_bytes(amb_ctx *c, const BuildOp &op)
+{
+    if (op.kind > 1) return 0.0;
+    const Cube &cb = c->cubes[op.cube];
+    const RingGeom &g = op.g;
+    double cells = 0;
+    for (long ia = -(long)ceil(g.hi); ia <= (long)ceil(g.hi); ++ia)
+        for (long j = 0; j < g.jcnt; ++j) {
+            double uu = sqrt((double)(ia * ia + j * j));
+            if (uu >= g.lo && uu < g.hi && uu > 0) cells += 1;
+        }
+    return (double)cb.nact() * (4.0 * (double)g.ng * g.ng + (op.kind == 0 ? 8.0 : 4.0) * cells);
+}
+
+int get_planes(amb_ctx *c, const PlaneSrc &src, int cube_id, int kind, int r, int ng, void **out)
+{
+    Cube *cb;
+    int rc = find_cube(c, cube_id, &cb);
     if (rc) return rc;
-    for (size_t i = 0; i < c->planes.size(); ++i) {
-        PlaneSet &p = c->planes[i];
+    RingGeom g;
+    if ((rc = ring_geom(c, *cb, r, ng, &g))) return rc;
+    for (auto &p : *src.list)
         if (p.cube == cube_id && p.kind == kind && p.ng == ng && p.lo == g.lo && p.hi == g.hi) {
             p.last = ++c->tick;
             p.pins++;
             *out = p.d;
             return AMB_OK;
         }
-    }
-    if (kind >= 4) return get_scaled(c, cube_id, cb, kind, r, ng, out, g);
-    const int nact = cb.nact();
-    const size_t P = (size_t)ng * ng, pitch = plane_pitch(ng);
-    const size_t bytes = sizeof(float) * pitch * nact; /* FP32, or hi+lo BF16 */
-    void *src = nullptr;
-    if (kind >= 2 && (rc = get_planes(c, cube_id, cb, kind - 2, r, ng, &src))) return rc;
-    float2 *tw = nullptr;
-    FftPlan pl;
-    if (kind < 2) {
+    BuildOp op;
+    op.kind = kind;
+    op.cube = cube_id;
+    op.g = g;
+    void *dep = nullptr, *sc = nullptr;
+    if (kind == 2 || kind == 3) {
+        if ((rc = get_planes(c, src, cube_id, kind - 2, r, ng, &dep))) return rc;
+    } else if (kind == 4 || kind == 5) {
+        if ((rc = get_planes(c, src, cube_id, 0, r, ng, &dep))) return rc;
+        if (kind == 4 && (rc = get_planes(c, src, cube_id, 5, r, ng, &sc))) return rc;
+    } else {
         /* one transform of ng complex values plus twiddles and a second
            buffer must fit a workgroup's 64 KiB of LDS */
         if (24L * ng > 65536) return fail(c, AMB_EARG, "grid %d exceeds the LDS transform size (2730)", ng);
+        float2 *tw;
+        FftPlan pl;
         if ((rc = twiddles(c, ng, &tw, &pl))) return rc;
-        size_t gneed = sizeof(float2) * (size_t)nact * (size_t)std::max(g.jcnt, 1) * ng;
-        if ((rc = scratch(c, &c->G, &c->G_bytes, gneed, "fft scratch"))) return rc;
     }
+    op.src = dep;
+    op.sc = sc;
+    const size_t bytes = plane_bytes(kind, cb->nact(), plane_pitch(ng));
     void *d = nullptr;
-    if ((rc = dev_alloc(c, &d, bytes, "filtered planes"))) return rc;
-    hipError_t e;
-    if (kind < 2) {
-        timer_start(c, &c->st.ms_fft);
-        e = amb_launch_ring_c2r(c->stream, kind, cb.vis_t, cb.samp_t, cb.d_act, nact, (int)cb.N,
-                                (int)(cb.N / 2 + 1), pl, tw, g, c->G, (float *)d, (long)pitch);
-        timer_stop(c);
-        c->st.n_fft_launch += 2;
-        /* algorithmic bytes: Ng^2 floats written per plane plus the ring cells read */
-        double cells = 0;
-        for (long ia = -(long)ceil(g.hi); ia <= (long)ceil(g.hi); ++ia)
-            for (long j = 0; j < g.jcnt; ++j) {
-                double uu = sqrt((double)(ia * ia + j * j));
-                if (uu >= g.lo && uu < g.hi && uu > 0) cells += 1;
-            }
-        c->st.bytes_fft += (double)nact * (4.0 * (double)P + (kind == 0 ? 8.0 : 4.0) * cells);
-    } else {
-        __bf16 *h = (__bf16 *)d;
-        e = amb_launch_split(c->stream, (const float *)src, pitch * nact, h, h + pitch * nact);
+    if (src.bump) {
+        const size_t need = (bytes + 255) / 256 * 256;
+        if (src.bump->off + need > src.bump->cap)
+            return fail(c, AMB_EBUDGET, "plane arena: %zu + %zu bytes exceed %zu", src.bump->off, need, src.bump->cap);
+        d = (void *)((uintptr_t)src.bump->base + src.bump->off); /* base 0: measuring only */
+        src.bump->off += need;
+    } else if ((rc = dev_alloc(c, &d, bytes, "filtered planes"))) {
+        return rc;
     }
-    if (e != hipSuccess) {
-        dev_free(c, d, bytes);
-        return fail(c, AMB_EHIP, "plane build (kind %d): %s", kind, hipGetErrorString(e));
+    op.dst = d;
+    if (src.record) {
+        src.record->push_back(op);
+    } else {
+        if (kind <= 1 && (rc = scratch(c, &c->G, &c->G_bytes, build_scratch_bytes(c, op), "fft scratch"))) {
+            dev_free(c, d, bytes);
+            return rc;
+        }
+        if (kind <= 1) {
+            timer_start(c, &c->st.ms_fft);
+            c->st.n_fft_launch += 2;
+            c->st.bytes_fft += build_fft_bytes(c, op);
+        }
+        hipError_t e = launch_build(c, c->stream, op, c->G);
+        if (kind <= 1) timer_stop(c);
+        if (e != hipSuccess) {
+            dev_free(c, d, bytes);
+            return fail(c, AMB_EHIP, "plane build (kind %d): %s", kind, hipGetErrorString(e));
+        }
     }
     PlaneSet p;
     p.cube = cube_id; p.kind = kind; p.ng = ng; p.lo = g.lo; p.hi = g.hi;
     p.d = d; p.bytes = bytes; p.last = ++c->tick; p.pins = 1;
-    c->planes.push_back(p);
+    src.list->push_back(p);
     *out = d;
     return AMB_OK;
 }
 
-static void unpin_all(amb_ctx *c)
+void unpin_all(amb_ctx *c)
 {
     for (auto &p : c->planes) p.pins = 0;
 }
 
-static int find_cube(amb_ctx *c, int id, Cube **cb)
+int find_cube(amb_ctx *c, int id, Cube **cb)
 {
     auto it = c->cubes.find(id);
     if (it == c->cubes.end()) return fail(c, AMB_ESTATE, "unknown cube id %d", id);
@@ -370,10 +357,183 @@ static int find_cube(amb_ctx *c, int id, Cube **cb)
     return AMB_OK;
 }
 
+static void drop_cube_planes(amb_ctx *c, int id)
+{
+    for (size_t i = 0; i < c->planes.size();)
+        if (c->planes[i].cube == id) {
+            dev_free(c, c->planes[i].d, c->planes[i].bytes);
+            c->planes.erase(c->planes.begin() + i);
+        } else ++i;
+}
+
+int prepare_tri(amb_ctx *c, const PlaneSrc &src, int id, int r1, int r2, int r3, int want_norm, TriJob &j)
+{
+    Cube *cb;
+    int rc = find_cube(c, id, &cb);
+    if (rc) return rc;
+    if (!cb->chans) return fail(c, AMB_ESTATE, "channels not set for cube %d", id);
+    const int nr = (int)c->ulow.size();
+    if (nr == 0) return fail(c, AMB_ESTATE, "rings not set");
+    if (r1 < 0 || r1 >= nr || r2 < 0 || r2 >= nr || r3 < 0 || r3 >= nr)
+        return fail(c, AMB_EARG, "triangle (%d,%d,%d) outside [0,%d)", r1, r2, r3, nr);
+    const long ng = amb_ng(c->uhigh[r1], cb->Du);
+    if (ng <= 0 || ng > 2730) return fail(c, AMB_EARG, "Ng=%ld out of range", ng);
+    j = TriJob{};
+    j.cube = id; j.r1 = r1; j.r2 = r2; j.r3 = r3; j.want_norm = want_norm;
+    j.ng = ng; j.P = ng * ng; j.pitch = plane_pitch(ng);
+    j.nact = cb->nact();
+    j.W = 2 * cb->nc10 + 1;
+    j.cells = (size_t)j.W * j.W;
+    const int prec = c->prec, nact = j.nact;
+    const int nprec = prec == AMB_PREC_FP32 ? AMB_PREC_FP32 : AMB_PREC_BF16X3; /* normalisation form */
+    /* D/S: FP32 images of r1, r2, r3; the third operand in the form the
+       contraction reads (FP32, the hi/lo BF16 split, or scaled e4m3) */
+    void *D[3], *S[3] = {nullptr, nullptr, nullptr}, *SC[3] = {nullptr, nullptr, nullptr};
+    const int rr[3] = {r1, r2, r3};
+    const int d3kind = prec == AMB_PREC_FP32 ? 0 : prec == AMB_PREC_BF16X3 ? 2 : 4;
+    for (int k = 0; k < 3; ++k) {
+        if ((rc = get_planes(c, src, id, k == 2 ? d3kind : 0, rr[k], (int)ng, &D[k]))) return rc;
+        if (prec == AMB_PREC_FP8 && (rc = get_planes(c, src, id, 5, rr[k], (int)ng, &SC[k]))) return rc;
+    }
+    if (want_norm)
+        for (int k = 0; k < 3; ++k) {
+            const int kind = (k == 2 && nprec != AMB_PREC_FP32) ? 3 : 1;
+            if ((rc = get_planes(c, src, id, kind, rr[k], (int)ng, &S[k]))) return rc;
+        }
+    /* r1 == r2 symmetry needs every p3 of p1's window inside p2's window */
+    const bool full_window = nact > 0 && cb->act.back() - cb->act.front() <= cb->nc10;
+    j.sym12 = (r1 == r2 && full_window && !c->no_sym) ? 1 : 0;
+    ContractArgs &a = j.a;
+    a.P = j.P; a.pitch = j.pitch; a.nact = nact; a.wlo = cb->d_wlo; a.whi = cb->d_whi; a.sym12 = j.sym12;
+    a.D1 = (const float *)D[0]; a.D2 = (const float *)D[1];
+    if (prec == AMB_PREC_FP32) a.D3 = (const float *)D[2];
+    else if (prec == AMB_PREC_BF16X3) {
+        a.H3 = (const __bf16 *)D[2];
+        a.L3 = a.H3 + (size_t)j.pitch * nact;
+    } else {
+        a.Q3 = (const uint8_t *)D[2];
+        a.s1 = (const float *)SC[0];
+        a.s2 = (const float *)SC[1];
+        a.t3 = (const float *)SC[2] + nact;
+    }
+    if (want_norm) {
+        ContractArgs &b = j.b;
+        b = a;
+        b.D1 = (const float *)S[0]; b.D2 = (const float *)S[1];
+        b.D3 = nullptr; b.H3 = b.L3 = nullptr; b.Q3 = nullptr; b.s1 = b.s2 = b.t3 = nullptr;
+        if (nprec == AMB_PREC_FP32) b.D3 = (const float *)S[2];
+        else { b.H3 = (const __bf16 *)S[2]; b.L3 = b.H3 + (size_t)j.pitch * nact; }
+    }
+    ReduceArgs &ra = j.ra;
+    ra.act = cb->d_act; ra.inv = cb->d_inv;
+    ra.nc = (int)cb->nc; ra.nact = nact; ra.nc10 = cb->nc10; ra.P = j.P; ra.sym12 = j.sym12;
+    j.flop_exec = (double)amb_contract_live_items(nact, cb->wlo.data(), cb->whi.data(), j.sym12) *
+                  amb_contract_item_flops(prec, j.pitch);
+    return AMB_OK;
+}
+
+void account_tri(amb_ctx *c, const TriJob &j, const int32_t *cnt)
+{
+    int64_t ev = 0;
+    double pairs = 0;
+    for (size_t i = 0; i < j.cells; ++i) {
+        ev += cnt[i] > 0;
+        pairs += cnt[i];
+    }
+    c->st.evals += ev;
+    c->st.flop_contract += 2.0 * pairs * (double)j.P;
+    c->st.flop_contract_exec += j.flop_exec;
+}
+
+/* one triangle with its own launches; output pointers are host memory */
+int bispec_core(amb_ctx *c, int id, int r1, int r2, int r3, int want_norm, amb_result *res)
+{
+    hipSetDevice(c->device);
+    TriJob j;
+    const PlaneSrc src{&c->planes};
+    int rc = prepare_tri(c, src, id, r1, r2, r3, want_norm, j);
+    if (rc) { unpin_all(c); return rc; }
+    res->ng = j.ng;
+    res->inv_ng2 = 1.0 / (j.ng * j.ng * 1.);
+    res->cert_margin = 0;
+    const size_t cells = j.cells, nC = (size_t)j.nact * j.nact * j.nact;
+    if ((rc = scratch(c, &c->C, &c->C_bytes, sizeof(float) * nC, "contraction partials"))) { unpin_all(c); return rc; }
+    if (want_norm && (rc = scratch(c, &c->Cq, &c->Cq_bytes, sizeof(int) * nC, "normalisation partials"))) { unpin_all(c); return rc; }
+    /* bl, norm | pad to 8 B | bl64, norm64 | cnt */
+    const size_t outb = cells * (sizeof(float) * 2 + sizeof(double) * 2 + sizeof(int)) + sizeof(float) * (cells & 1);
+    if ((rc = grow(c, &c->out, &c->out_bytes, outb, "lag arrays"))) { unpin_all(c); return rc; }
+    float *d_bl = (float *)c->out;
+    float *d_norm = d_bl + cells;
+    double *d_bl64 = (double *)(d_norm + cells + (cells & 1));
+    double *d_norm64 = d_bl64 + cells;
+    int *d_cnt = (int *)(d_norm64 + cells);
+
+    const int prec = c->prec;
+    const int nprec = prec == AMB_PREC_FP32 ? AMB_PREC_FP32 : AMB_PREC_BF16X3;
+    j.a.C = c->C;
+    hipError_t e = hipSuccess;
+    const char *step = "none";
+#define STEP(name, expr)                                              \
+    if (e == hipSuccess) {                                            \
+        step = name;                                                  \
+        e = (expr);                                                   \
+        if (e == hipSuccess && c->sync_debug) e = hipStreamSynchronize(c->stream); \
+    }
+    timer_start(c, &c->st.ms_contract);
+    STEP("contract", amb_launch_contract(c->stream, prec, 0, c->kvar, j.a));
+    timer_stop(c);
+    c->st.n_contract_launch++;
+    if (want_norm) {
+        j.b.Cq = c->Cq;
+        j.b.margin = c->margin;
+        STEP("margin reset", amb_launch_fill_u32(c->stream, c->margin, 1, 0u));
+        timer_start(c, &c->st.ms_norm);
+        STEP("norm contract", amb_launch_contract(c->stream, nprec, 1, c->kvar, j.b));
+        timer_stop(c);
+    }
+    ReduceArgs ra = j.ra;
+    ra.C = c->C; ra.Cq = want_norm ? c->Cq : nullptr;
+    ra.bl = d_bl; ra.bl64 = d_bl64; ra.cnt = d_cnt;
+    ra.norm = want_norm ? d_norm : nullptr; ra.norm64 = want_norm ? d_norm64 : nullptr;
+    timer_start(c, &c->st.ms_reduce);
+    STEP("reduce", amb_launch_reduce(c->stream, ra));
+    timer_stop(c);
+    std::vector<int> hcnt;
+    int32_t *cnt_dst = res->cnt;
+    if (!cnt_dst) { hcnt.resize(cells); cnt_dst = hcnt.data(); }
+    if (res->bl) STEP("copy bl", hipMemcpyAsync(res->bl, d_bl, sizeof(float) * cells, hipMemcpyDeviceToHost, c->stream));
+    if (res->bl64) STEP("copy bl64", hipMemcpyAsync(res->bl64, d_bl64, sizeof(double) * cells, hipMemcpyDeviceToHost, c->stream));
+    STEP("copy cnt", hipMemcpyAsync(cnt_dst, d_cnt, sizeof(int) * cells, hipMemcpyDeviceToHost, c->stream));
+    if (want_norm && res->norm) STEP("copy norm", hipMemcpyAsync(res->norm, d_norm, sizeof(float) * cells, hipMemcpyDeviceToHost, c->stream));
+    if (want_norm && res->norm64) STEP("copy norm64", hipMemcpyAsync(res->norm64, d_norm64, sizeof(double) * cells, hipMemcpyDeviceToHost, c->stream));
+    unsigned mbits = 0;
+    if (want_norm) STEP("copy margin", hipMemcpyAsync(&mbits, c->margin, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+    STEP("sync", hipStreamSynchronize(c->stream));
+#undef STEP
+    timers_collect(c);
+    unpin_all(c);
+    if (e != hipSuccess)
+        return fail(c, AMB_EHIP, "bispectrum (%d,%d,%d) Ng=%ld nact=%d step %s: %s", r1, r2, r3, j.ng, j.nact, step,
+                    hipGetErrorString(e));
+    account_tri(c, j, cnt_dst);
+    if (want_norm) {
+        float m;
+        memcpy(&m, &mbits, sizeof m);
+        res->cert_margin = m;
+        if (m > 1.0f / 32.0f)
+            return fail(c, AMB_ECERT, "normalisation sums off their 1/8 grid by %g", (double)m);
+    }
+    return AMB_OK;
+}
+
+} // namespace amb
+
+using namespace amb;
+
 /* ------------------------------------------------------------------ API */
 extern "C" {
 
-const char *amb_version(void) { return "amber_mi355x 0.1 (gfx950)"; }
+const char *amb_version(void) { return "amber_mi355x 0.2 (gfx950)"; }
 
 int amb_create(const amb_opts *o, amb_ctx **out)
 {
@@ -405,6 +565,7 @@ int amb_create(const amb_opts *o, amb_ctx **out)
     if (hipMalloc(&m, 256) != hipSuccess) { hipStreamDestroy(c->stream); delete c; return AMB_EHIP; }
     c->margin = (unsigned *)m;
     c->used = 256;
+    c->grouped_blocks = amb_contract_grouped_blocks(c->device);
     *out = c;
     return AMB_OK;
 }
@@ -412,7 +573,9 @@ int amb_create(const amb_opts *o, amb_ctx **out)
 void amb_destroy(amb_ctx *c)
 {
     if (!c) return;
+    amb_serve_stop(c);
     hipSetDevice(c->device);
+    while (!c->graphs.empty()) amb_graph_destroy(c, (amb_graph *)c->graphs.back());
     hipStreamSynchronize(c->stream);
     timers_collect(c);
     for (auto &p : c->planes) hipFree(p.d);
@@ -422,7 +585,8 @@ void amb_destroy(amb_ctx *c)
         hipFree(cb.d_wlo); hipFree(cb.d_whi);
     }
     for (auto &kv : c->tw) hipFree(kv.second);
-    hipFree(c->G); hipFree(c->C); hipFree(c->Cq); hipFree(c->margin); hipFree(c->out);
+    hipFree(c->G); hipFree(c->C); hipFree(c->Cq); hipFree(c->margin); hipFree(c->out); hipFree(c->wdesc);
+    if (c->hstage) hipHostFree(c->hstage);
     for (auto e : c->evpool) hipEventDestroy(e);
     hipStreamDestroy(c->stream);
     delete c;
@@ -433,9 +597,44 @@ const char *amb_last_error(amb_ctx *c) { return c ? c->err.c_str() : "null conte
 int amb_set_precision(amb_ctx *c, int prec)
 {
     if (!c) return AMB_EARG;
-    std::lock_guard<std::mutex> g(c->mu);
+    Lock g(c->mu);
     if (prec < AMB_PREC_FP32 || prec > AMB_PREC_FP8) return fail(c, AMB_EARG, "precision %d", prec);
     c->prec = prec;
+    return AMB_OK;
+}
+
+int amb_set_batch_mode(amb_ctx *c, int mode)
+{
+    if (!c) return AMB_EARG;
+    Lock g(c->mu);
+    if (mode != AMB_BATCH_PER_TRIANGLE && mode != AMB_BATCH_GROUPED) return fail(c, AMB_EARG, "batch mode %d", mode);
+    c->batch_mode = mode;
+    return AMB_OK;
+}
+
+/* normalize_data (amber_funcs.c:1946-1978) fused into the transposing
+   upload; rows of cells are staged through a device buffer of <= ~256 MiB */
+static int upload_cube(amb_ctx *c, Cube &cb, const double *gv, const double *samp)
+{
+    const int64_t N = cb.N, nc = cb.nc, Nh = N / 2 + 1;
+    const size_t row_doubles = (size_t)Nh * nc * 3;
+    int rows = (int)std::max<size_t>(1, (256u << 20) / (row_doubles * 8));
+    if (rows > N) rows = (int)N;
+    void *stg = nullptr;
+    int rc = dev_alloc(c, &stg, row_doubles * 8 * rows, "upload staging");
+    if (rc) return rc;
+    double *dg = (double *)stg, *ds = dg + (size_t)rows * Nh * nc * 2;
+    hipError_t e = hipSuccess;
+    for (int64_t i0 = 0; i0 < N && e == hipSuccess; i0 += rows) {
+        int r = (int)std::min<int64_t>(rows, N - i0);
+        size_t n1 = (size_t)r * Nh * nc;
+        e = hipMemcpyAsync(dg, gv + (size_t)i0 * Nh * nc * 2, n1 * 16, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(ds, samp + (size_t)i0 * Nh * nc, n1 * 8, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = amb_launch_upload(c->stream, dg, ds, (int)i0, r, (int)N, (int)Nh, (int)nc, cb.vis_t, cb.samp_t);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    }
+    dev_free(c, stg, row_doubles * 8 * rows);
+    if (e != hipSuccess) return fail(c, AMB_EHIP, "cube upload: %s", hipGetErrorString(e));
     return AMB_OK;
 }
 
@@ -443,11 +642,10 @@ int amb_load_cube(amb_ctx *c, int64_t N, int64_t nc, double Du, int32_t flag_key
                   const double *samp, int *cube_id)
 {
     if (!c || !gv || !samp || !cube_id) return AMB_EARG;
-    std::lock_guard<std::mutex> lk(c->mu);
+    Lock lk(c->mu);
     hipSetDevice(c->device);
     if (N < 4 || N % 2 || nc < 1 || !(Du > 0)) return fail(c, AMB_EARG, "bad cube header N=%lld nc=%lld", (long long)N, (long long)nc);
-    const int64_t Nh = N / 2 + 1;
-    const size_t nv = (size_t)N * Nh * nc;
+    const size_t nv = (size_t)N * (N / 2 + 1) * nc;
     Cube cb;
     cb.N = N; cb.nc = nc; cb.Du = Du; cb.flag_key = flag_key;
     void *v = nullptr, *s = nullptr;
@@ -458,33 +656,28 @@ int amb_load_cube(amb_ctx *c, int64_t N, int64_t nc, double Du, int32_t flag_key
     cb.vis_t = (float2 *)v;
     cb.samp_t = (float *)s;
     cb.bytes = sizeof(float2) * nv + sizeof(float) * nv;
-    /* stage rows of cells through a device buffer of <= ~256 MiB */
-    const size_t row_doubles = (size_t)Nh * nc * 3;
-    int rows = (int)std::max<size_t>(1, (256u << 20) / (row_doubles * 8));
-    if (rows > N) rows = (int)N;
-    void *stg = nullptr;
-    rc = dev_alloc(c, &stg, row_doubles * 8 * rows, "upload staging");
-    if (rc) { dev_free(c, v, sizeof(float2) * nv); dev_free(c, s, sizeof(float) * nv); return rc; }
-    double *dg = (double *)stg, *ds = dg + (size_t)rows * Nh * nc * 2;
-    for (int64_t i0 = 0; i0 < N; i0 += rows) {
-        int r = (int)std::min<int64_t>(rows, N - i0);
-        size_t n1 = (size_t)r * Nh * nc;
-        hipError_t e = hipMemcpyAsync(dg, gv + (size_t)i0 * Nh * nc * 2, n1 * 16, hipMemcpyHostToDevice, c->stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(ds, samp + (size_t)i0 * Nh * nc, n1 * 8, hipMemcpyHostToDevice, c->stream);
-        if (e == hipSuccess) e = amb_launch_upload(c->stream, dg, ds, (int)i0, r, (int)N, (int)Nh, (int)nc, cb.vis_t, cb.samp_t);
-        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-        if (e != hipSuccess) {
-            dev_free(c, stg, row_doubles * 8 * rows);
-            dev_free(c, v, sizeof(float2) * nv);
-            dev_free(c, s, sizeof(float) * nv);
-            return fail(c, AMB_EHIP, "cube upload: %s", hipGetErrorString(e));
-        }
+    if ((rc = upload_cube(c, cb, gv, samp))) {
+        dev_free(c, v, sizeof(float2) * nv);
+        dev_free(c, s, sizeof(float) * nv);
+        return rc;
     }
-    dev_free(c, stg, row_doubles * 8 * rows);
     int id = c->next_cube++;
     c->cubes[id] = cb;
     *cube_id = id;
     return AMB_OK;
+}
+
+int amb_update_cube(amb_ctx *c, int id, const double *gv, const double *samp)
+{
+    if (!c || !gv || !samp) return AMB_EARG;
+    Lock lk(c->mu);
+    hipSetDevice(c->device);
+    Cube *cb;
+    int rc = find_cube(c, id, &cb);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    drop_cube_planes(c, id); /* filtered from the old values */
+    return upload_cube(c, *cb, gv, samp);
 }
 
 int amb_load_cube_files(amb_ctx *c, const char *data_path, int rlzn, int *cube_id)
@@ -496,7 +689,7 @@ int amb_load_cube_files(amb_ctx *c, const char *data_path, int rlzn, int *cube_i
     double *gv = nullptr, *samp = nullptr;
     int rc = amb_read_cube_files(data_path, rlzn, &NN, &nc, &Du, &fk, &gv, &samp);
     if (rc) {
-        std::lock_guard<std::mutex> lk(c->mu);
+        Lock lk(c->mu);
         return fail(c, rc, "reading %s/GV_data_%d and sampling: %s", data_path, rlzn, amb_strerror(rc));
     }
     rc = amb_load_cube(c, NN, nc, Du, fk, gv, samp, cube_id);
@@ -508,16 +701,13 @@ int amb_load_cube_files(amb_ctx *c, const char *data_path, int rlzn, int *cube_i
 int amb_release_cube(amb_ctx *c, int id)
 {
     if (!c) return AMB_EARG;
-    std::lock_guard<std::mutex> lk(c->mu);
+    Lock lk(c->mu);
     Cube *cb;
     int rc = find_cube(c, id, &cb);
     if (rc) return rc;
+    if (cb->graphs) return fail(c, AMB_ESTATE, "cube %d is read by %d captured graph(s)", id, cb->graphs);
     hipStreamSynchronize(c->stream);
-    for (size_t i = 0; i < c->planes.size();)
-        if (c->planes[i].cube == id) {
-            dev_free(c, c->planes[i].d, c->planes[i].bytes);
-            c->planes.erase(c->planes.begin() + i);
-        } else ++i;
+    drop_cube_planes(c, id);
     const size_t nv = (size_t)cb->N * (cb->N / 2 + 1) * cb->nc;
     dev_free(c, cb->vis_t, sizeof(float2) * nv);
     dev_free(c, cb->samp_t, sizeof(float) * nv);
@@ -533,11 +723,12 @@ int amb_release_cube(amb_ctx *c, int id)
 int amb_set_channels(amb_ctx *c, int id, const int *uf, int uf_nc, int nc10)
 {
     if (!c) return AMB_EARG;
-    std::lock_guard<std::mutex> lk(c->mu);
+    Lock lk(c->mu);
     hipSetDevice(c->device);
     Cube *cb;
     int rc = find_cube(c, id, &cb);
     if (rc) return rc;
+    if (cb->graphs) return fail(c, AMB_ESTATE, "cube %d is read by %d captured graph(s)", id, cb->graphs);
     const int nc = (int)cb->nc;
     if (nc10 < 0 || nc10 > nc - 1) return fail(c, AMB_EARG, "nc10=%d outside [0,%d]", nc10, nc - 1);
     std::vector<char> on(nc, 0);
@@ -552,17 +743,14 @@ int amb_set_channels(amb_ctx *c, int id, const int *uf, int uf_nc, int nc10)
     }
     hipStreamSynchronize(c->stream);
     /* planes are built per channel set: drop this cube's cached planes */
-    for (size_t i = 0; i < c->planes.size();)
-        if (c->planes[i].cube == id) {
-            dev_free(c, c->planes[i].d, c->planes[i].bytes);
-            c->planes.erase(c->planes.begin() + i);
-        } else ++i;
+    drop_cube_planes(c, id);
     size_t na_old = cb->act.size();
     dev_free(c, cb->d_act, sizeof(int) * na_old);
     dev_free(c, cb->d_inv, sizeof(int) * (2 * (size_t)nc + 1));
     dev_free(c, cb->d_wlo, sizeof(int) * na_old);
     dev_free(c, cb->d_whi, sizeof(int) * na_old);
     cb->d_act = cb->d_inv = cb->d_wlo = cb->d_whi = nullptr;
+    cb->chans = false;
     cb->act.clear();
     cb->inv.assign(nc, -1);
     for (int p = 0; p < nc; ++p)
@@ -606,7 +794,7 @@ int amb_set_channels(amb_ctx *c, int id, const int *uf, int uf_nc, int nc10)
 int amb_set_rings(amb_ctx *c, const double *lo, const double *hi, int nr)
 {
     if (!c || !lo || !hi || nr <= 0) return AMB_EARG;
-    std::lock_guard<std::mutex> lk(c->mu);
+    Lock lk(c->mu);
     for (int i = 0; i < nr; ++i)
         if (!(hi[i] > lo[i]) || !(lo[i] >= 0)) return fail(c, AMB_EARG, "ring %d edges [%g,%g)", i, lo[i], hi[i]);
     c->ulow.assign(lo, lo + nr);
@@ -619,7 +807,7 @@ int amb_set_rings(amb_ctx *c, const double *lo, const double *hi, int nr)
 int amb_closed_rings(amb_ctx *c, int id, int plane, double *rdata, int cap, int *rcount)
 {
     if (!c || !rcount) return AMB_EARG;
-    std::lock_guard<std::mutex> lk(c->mu);
+    Lock lk(c->mu);
     hipSetDevice(c->device);
     Cube *cb;
     int rc = find_cube(c, id, &cb);
@@ -634,6 +822,7 @@ int amb_closed_rings(amb_ctx *c, int id, int plane, double *rdata, int cap, int 
     amb_triangles(c->uavg.data(), nr, tri.data(), ntri);
     *rcount = ntri;
     if (ntri == 0) return AMB_OK;
+    if (24L * N > 65536) return fail(c, AMB_EARG, "grid %d exceeds the LDS transform size (2730)", N);
     float2 *tw;
     FftPlan pl;
     if ((rc = twiddles(c, N, &tw, &pl))) return rc;
@@ -692,151 +881,10 @@ int amb_closed_rings(amb_ctx *c, int id, int plane, double *rdata, int cap, int 
     return AMB_OK;
 }
 
-/* core of every bispectrum entry point; output pointers are host memory */
-static int bispec_core(amb_ctx *c, int id, int r1, int r2, int r3, int want_norm, amb_result *res)
-{
-    hipSetDevice(c->device);
-    Cube *cb;
-    int rc = find_cube(c, id, &cb);
-    if (rc) return rc;
-    if (!cb->chans) return fail(c, AMB_ESTATE, "channels not set for cube %d", id);
-    const int nr = (int)c->ulow.size();
-    if (nr == 0) return fail(c, AMB_ESTATE, "rings not set");
-    if (r1 < 0 || r1 >= nr || r2 < 0 || r2 >= nr || r3 < 0 || r3 >= nr)
-        return fail(c, AMB_EARG, "triangle (%d,%d,%d) outside [0,%d)", r1, r2, r3, nr);
-    const long ng = amb_ng(c->uhigh[r1], cb->Du);
-    if (ng <= 0 || ng > 4096) return fail(c, AMB_EARG, "Ng=%ld out of range", ng);
-    const long P = ng * ng;
-    const int nact = cb->nact();
-    const int W = 2 * cb->nc10 + 1;
-    const size_t cells = (size_t)W * W;
-    res->ng = ng;
-    res->inv_ng2 = 1.0 / (ng * ng * 1.);
-    res->cert_margin = 0;
-
-    const int prec = c->prec;
-    const int nprec = prec == AMB_PREC_FP32 ? AMB_PREC_FP32 : AMB_PREC_BF16X3; /* normalisation form */
-    const long pitch = plane_pitch(ng);
-    /* D/S: FP32 images of r1, r2, r3; the third operand in the form the
-       contraction reads (FP32, or the hi/lo BF16 split) */
-    void *D[3], *S[3] = {nullptr, nullptr, nullptr}, *SC[3] = {nullptr, nullptr, nullptr};
-    const int rr[3] = {r1, r2, r3};
-    const int d3kind = prec == AMB_PREC_FP32 ? 0 : prec == AMB_PREC_BF16X3 ? 2 : 4;
-    const int sckind = prec == AMB_PREC_FP8 ? 5 : -1;
-    for (int k = 0; k < 3; ++k) {
-        const int kind = k == 2 ? d3kind : 0;
-        if ((rc = get_planes(c, id, *cb, kind, rr[k], (int)ng, &D[k]))) { unpin_all(c); return rc; }
-        if (sckind >= 0 && (rc = get_planes(c, id, *cb, sckind, rr[k], (int)ng, &SC[k]))) {
-            unpin_all(c);
-            return rc;
-        }
-    }
-    if (want_norm)
-        for (int k = 0; k < 3; ++k) {
-            const int kind = (k == 2 && nprec != AMB_PREC_FP32) ? 3 : 1;
-            if ((rc = get_planes(c, id, *cb, kind, rr[k], (int)ng, &S[k]))) { unpin_all(c); return rc; }
-        }
-
-    const size_t nC = (size_t)nact * nact * nact;
-    if ((rc = scratch(c, &c->C, &c->C_bytes, sizeof(float) * nC, "contraction partials"))) { unpin_all(c); return rc; }
-    if (want_norm && (rc = scratch(c, &c->Cq, &c->Cq_bytes, sizeof(int) * nC, "normalisation partials"))) { unpin_all(c); return rc; }
-    /* bl, norm | pad to 8 B | bl64, norm64 | cnt */
-    const size_t outb = cells * (sizeof(float) * 2 + sizeof(double) * 2 + sizeof(int)) + sizeof(float) * (cells & 1);
-    if ((rc = scratch(c, (char **)&c->out, &c->out_bytes, outb, "lag arrays"))) { unpin_all(c); return rc; }
-    float *d_bl = (float *)c->out;
-    float *d_norm = d_bl + cells;
-    double *d_bl64 = (double *)(d_norm + cells + (cells & 1));
-    double *d_norm64 = d_bl64 + cells;
-    int *d_cnt = (int *)(d_norm64 + cells);
-
-    ContractArgs a{};
-    /* r1 == r2 symmetry needs every p3 of p1's window inside p2's window */
-    const bool full_window = nact > 0 && cb->act.back() - cb->act.front() <= cb->nc10;
-    const int sym12 = (r1 == r2 && full_window && !c->no_sym) ? 1 : 0;
-    a.P = P; a.pitch = pitch; a.nact = nact; a.wlo = cb->d_wlo; a.whi = cb->d_whi; a.sym12 = sym12;
-    a.D1 = (const float *)D[0]; a.D2 = (const float *)D[1]; a.C = c->C;
-    if (prec == AMB_PREC_FP32) a.D3 = (const float *)D[2];
-    else if (prec == AMB_PREC_BF16X3) {
-        a.H3 = (const __bf16 *)D[2];
-        a.L3 = a.H3 + (size_t)pitch * nact;
-    } else {
-        a.Q3 = (const uint8_t *)D[2];
-        a.s1 = (const float *)SC[0];
-        a.s2 = (const float *)SC[1];
-        a.t3 = (const float *)SC[2] + nact;
-    }
-    hipError_t e = hipSuccess;
-    const char *step = "none";
-#define STEP(name, expr)                                              \
-    if (e == hipSuccess) {                                            \
-        step = name;                                                  \
-        e = (expr);                                                   \
-        if (e == hipSuccess && c->sync_debug) e = hipStreamSynchronize(c->stream); \
-    }
-    timer_start(c, &c->st.ms_contract);
-    STEP("contract", amb_launch_contract(c->stream, prec, 0, c->kvar, a));
-    timer_stop(c);
-    c->st.n_contract_launch++;
-    if (want_norm) {
-        ContractArgs b = a;
-        b.D1 = (const float *)S[0]; b.D2 = (const float *)S[1]; b.C = nullptr; b.Cq = c->Cq; b.margin = c->margin;
-        b.D3 = nullptr; b.H3 = b.L3 = nullptr;
-        if (nprec == AMB_PREC_FP32) b.D3 = (const float *)S[2];
-        else { b.H3 = (const __bf16 *)S[2]; b.L3 = b.H3 + (size_t)pitch * nact; }
-        STEP("margin reset", hipMemsetAsync(c->margin, 0, sizeof(unsigned), c->stream));
-        timer_start(c, &c->st.ms_norm);
-        STEP("norm contract", amb_launch_contract(c->stream, nprec, 1, c->kvar, b));
-        timer_stop(c);
-    }
-    ReduceArgs ra{};
-    ra.C = c->C; ra.Cq = want_norm ? c->Cq : nullptr; ra.act = cb->d_act; ra.inv = cb->d_inv;
-    ra.nc = (int)cb->nc; ra.nact = nact; ra.nc10 = cb->nc10; ra.P = P; ra.sym12 = sym12;
-    ra.bl = d_bl; ra.bl64 = d_bl64; ra.cnt = d_cnt;
-    ra.norm = want_norm ? d_norm : nullptr; ra.norm64 = want_norm ? d_norm64 : nullptr;
-    timer_start(c, &c->st.ms_reduce);
-    STEP("reduce", amb_launch_reduce(c->stream, ra));
-    timer_stop(c);
-    std::vector<int> hcnt;
-    int32_t *cnt_dst = res->cnt;
-    if (!cnt_dst) { hcnt.resize(cells); cnt_dst = hcnt.data(); }
-    if (res->bl) STEP("copy bl", hipMemcpyAsync(res->bl, d_bl, sizeof(float) * cells, hipMemcpyDeviceToHost, c->stream));
-    if (res->bl64) STEP("copy bl64", hipMemcpyAsync(res->bl64, d_bl64, sizeof(double) * cells, hipMemcpyDeviceToHost, c->stream));
-    STEP("copy cnt", hipMemcpyAsync(cnt_dst, d_cnt, sizeof(int) * cells, hipMemcpyDeviceToHost, c->stream));
-    if (want_norm && res->norm) STEP("copy norm", hipMemcpyAsync(res->norm, d_norm, sizeof(float) * cells, hipMemcpyDeviceToHost, c->stream));
-    if (want_norm && res->norm64) STEP("copy norm64", hipMemcpyAsync(res->norm64, d_norm64, sizeof(double) * cells, hipMemcpyDeviceToHost, c->stream));
-    unsigned mbits = 0;
-    if (want_norm) STEP("copy margin", hipMemcpyAsync(&mbits, c->margin, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
-    STEP("sync", hipStreamSynchronize(c->stream));
-#undef STEP
-    timers_collect(c);
-    unpin_all(c);
-    if (e != hipSuccess)
-        return fail(c, AMB_EHIP, "bispectrum (%d,%d,%d) Ng=%ld nact=%d step %s: %s", r1, r2, r3, ng, nact, step,
-                    hipGetErrorString(e));
-    int64_t ev = 0;
-    double pairs = 0;
-    for (size_t i = 0; i < cells; ++i) {
-        ev += cnt_dst[i] > 0;
-        pairs += cnt_dst[i];
-    }
-    c->st.evals += ev;
-    c->st.flop_contract += 2.0 * pairs * (double)P;
-    c->st.flop_contract_exec += (double)amb_contract_live_items(nact, cb->wlo.data(), cb->whi.data(), sym12) *
-                                amb_contract_item_flops(prec, pitch);
-    if (want_norm) {
-        float m;
-        memcpy(&m, &mbits, sizeof m);
-        res->cert_margin = m;
-        if (m > 1.0f / 32.0f)
-            return fail(c, AMB_ECERT, "normalisation sums off their 1/8 grid by %g", (double)m);
-    }
-    return AMB_OK;
-}
-
 int amb_bispec_ex(amb_ctx *c, int id, int r1, int r2, int r3, int want_norm, amb_result *res)
 {
     if (!c || !res) return AMB_EARG;
-    std::lock_guard<std::mutex> lk(c->mu);
+    Lock lk(c->mu);
     return bispec_core(c, id, r1, r2, r3, want_norm, res);
 }
 
@@ -844,7 +892,7 @@ int amb_bispec_v1(amb_ctx *c, int id, int r1, int r2, int r3, double rdata_num, 
                   double *ntri, double *num)
 {
     if (!c || !bl || !cnt) return AMB_EARG;
-    std::lock_guard<std::mutex> lk(c->mu);
+    Lock lk(c->mu);
     amb_result r{};
     r.bl = bl;
     r.cnt = cnt;
@@ -865,7 +913,7 @@ int amb_bispec_v1(amb_ctx *c, int id, int r1, int r2, int r3, double rdata_num, 
 int amb_bispec_v2(amb_ctx *c, int id, int r1, int r2, int r3, float *bl, float *norm, double *inv_ng2)
 {
     if (!c || !bl || !norm) return AMB_EARG;
-    std::lock_guard<std::mutex> lk(c->mu);
+    Lock lk(c->mu);
     amb_result r{};
     r.bl = bl;
     r.norm = norm;
@@ -875,38 +923,10 @@ int amb_bispec_v2(amb_ctx *c, int id, int r1, int r2, int r3, float *bl, float *
     return AMB_OK;
 }
 
-int amb_bispec_batch(amb_ctx *c, int id, const int *tri, int ntri, float *bl, int32_t *cnt, float *norm)
-{
-    if (!c || !tri || ntri < 0 || !bl) return AMB_EARG;
-    std::lock_guard<std::mutex> lk(c->mu);
-    Cube *cb;
-    int rc = find_cube(c, id, &cb);
-    if (rc) return rc;
-    if (!cb->chans) return fail(c, AMB_ESTATE, "channels not set for cube %d", id);
-    const size_t cells = (size_t)(2 * cb->nc10 + 1) * (2 * cb->nc10 + 1);
-    /* visit triangles grouped by r1 (one Ng per group) so planes are reused */
-    std::vector<int> order(ntri);
-    for (int i = 0; i < ntri; ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return tri[3 * x] < tri[3 * y]; });
-    const auto t0 = std::chrono::steady_clock::now();
-    c->batch_ms.assign(ntri, 0.0);
-    for (int k = 0; k < ntri; ++k) {
-        int i = order[k];
-        amb_result r{};
-        r.bl = bl + (size_t)i * cells;
-        r.cnt = cnt ? cnt + (size_t)i * cells : nullptr;
-        r.norm = norm ? norm + (size_t)i * cells : nullptr;
-        rc = bispec_core(c, id, tri[3 * i], tri[3 * i + 1], tri[3 * i + 2], norm != nullptr, &r);
-        if (rc) return rc;
-        c->batch_ms[i] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    }
-    return AMB_OK;
-}
-
 int amb_batch_times(amb_ctx *c, double *ms, int cap, int *n)
 {
     if (!c || !n) return AMB_EARG;
-    std::lock_guard<std::mutex> lk(c->mu);
+    Lock lk(c->mu);
     *n = (int)c->batch_ms.size();
     for (int i = 0; i < *n && i < cap && ms; ++i) ms[i] = c->batch_ms[i];
     return AMB_OK;
@@ -915,7 +935,7 @@ int amb_batch_times(amb_ctx *c, double *ms, int cap, int *n)
 int amb_ring_image(amb_ctx *c, int id, int ring, int64_t ng, int plane, int kind, float *out, int64_t cap)
 {
     if (!c || !out) return AMB_EARG;
-    std::lock_guard<std::mutex> lk(c->mu);
+    Lock lk(c->mu);
     hipSetDevice(c->device);
     Cube *cb;
     int rc = find_cube(c, id, &cb);
@@ -931,7 +951,7 @@ int amb_ring_image(amb_ctx *c, int id, int ring, int64_t ng, int plane, int kind
     const size_t P = (size_t)ng * ng;
     if (cap < (int64_t)P) return fail(c, AMB_EARG, "output holds %lld floats, %zu needed", (long long)cap, P);
     void *d = nullptr;
-    rc = get_planes(c, id, *cb, kind, ring, (int)ng, &d);
+    rc = get_planes(c, PlaneSrc{&c->planes}, id, kind, ring, (int)ng, &d);
     unpin_all(c);
     if (rc) return rc;
     const float *src = (const float *)d + (size_t)ia * plane_pitch((int)ng);
@@ -944,23 +964,19 @@ int amb_ring_image(amb_ctx *c, int id, int ring, int64_t ng, int plane, int kind
 int amb_drop_planes(amb_ctx *c, int id)
 {
     if (!c) return AMB_EARG;
-    std::lock_guard<std::mutex> lk(c->mu);
+    Lock lk(c->mu);
     Cube *cb;
     int rc = find_cube(c, id, &cb);
     if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    for (size_t i = 0; i < c->planes.size();)
-        if (c->planes[i].cube == id) {
-            dev_free(c, c->planes[i].d, c->planes[i].bytes);
-            c->planes.erase(c->planes.begin() + i);
-        } else ++i;
+    drop_cube_planes(c, id);
     return AMB_OK;
 }
 
 int amb_get_stats(amb_ctx *c, amb_stats *st)
 {
     if (!c || !st) return AMB_EARG;
-    std::lock_guard<std::mutex> lk(c->mu);
+    Lock lk(c->mu);
     *st = c->st;
     st->ms_total = st->ms_fft + st->ms_contract + st->ms_reduce + st->ms_norm;
     return AMB_OK;
@@ -969,7 +985,7 @@ int amb_get_stats(amb_ctx *c, amb_stats *st)
 int amb_reset_stats(amb_ctx *c)
 {
     if (!c) return AMB_EARG;
-    std::lock_guard<std::mutex> lk(c->mu);
+    Lock lk(c->mu);
     c->st = amb_stats{};
     return AMB_OK;
 }

@@ -603,6 +603,19 @@ __global__ __launch_bounds__(256 * NWR, 1) void k_contract_grouped(const Contrac
 
 __global__ void k_set_u64(unsigned long long *p, unsigned long long v) { *p = v; }
 
+__global__ void k_fill_u32(unsigned *p, long n, unsigned v)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
+}
+
+hipError_t amb_launch_fill_u32(hipStream_t s, unsigned *p, long n, unsigned v)
+{
+    if (n <= 0) return hipSuccess;
+    const int blocks = (int)std::min<long>((n + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
 static bool args_ok(int prec, int norm, const ContractArgs &a)
 {
     if (a.pitch % KC != 0 || a.nact <= 0) return false;
@@ -612,6 +625,19 @@ static bool args_ok(int prec, int norm, const ContractArgs &a)
     /* scaled forms carry no normalisation epilogue */
     if (prec == AMB_PREC_FP8 && (norm || !a.s1 || !a.s2 || !a.t3)) return false;
     return true;
+}
+
+int amb_contract_grouped_blocks(int dev)
+{
+    int cus = 0, occ = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 256;
+    /* every launched form shares the 8-wave geometry; the BF16x3 data form
+       has the largest LDS footprint */
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_contract_grouped<AMB_PREC_BF16X3, 0, 2, 0>, 512, 0) !=
+            hipSuccess || occ < 1)
+        occ = 1;
+    (void)hipGetLastError();
+    return cus * occ;
 }
 
 long amb_contract_items(int nact)

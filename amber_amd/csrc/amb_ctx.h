@@ -1,0 +1,171 @@
+/*
+ * Host-side state of a device context, shared by the C-ABI translation
+ * units (amb_api.hip: context, cubes, planes, one triangle; amb_wave.hip:
+ * grouped waves of many triangles, captured graphs and the serving queue).
+ * Not part of the public interface.
+ */
+#pragma once
+#include "amb_internal.h"
+#include "amber_mi355x.h"
+
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+struct AmbGraph;
+struct AmbServer;
+
+namespace amb {
+
+struct Cube {
+    int64_t N = 0, nc = 0;
+    double Du = 1.0;
+    int flag_key = 0;
+    float2 *vis_t = nullptr;
+    float *samp_t = nullptr;
+    size_t bytes = 0;
+    bool chans = false;
+    int nc10 = 0;
+    int graphs = 0;            /* captured graphs that read this cube's buffers */
+    std::vector<int> act, inv; /* compact <-> channel */
+    int *d_act = nullptr;      /* [nact] */
+    int *d_inv = nullptr;      /* inv[nc] then first_ge[nc+1] */
+    int *d_wlo = nullptr, *d_whi = nullptr;
+    std::vector<int> wlo, whi; /* host copies of the per-plane windows */
+    int nact() const { return (int)act.size(); }
+};
+
+/* One set of filtered planes: every active plane of (cube, kind, ring, Ng).
+ * kind 0: data images (FP32), 1: sampling images (FP32),
+ * 2 / 3: hi and lo BF16 halves of kind 0 / 1, stored hi planes then lo,
+ * 4: e4m3 data images times their B-side scales,
+ * 5: per-plane FP8 scales of kind 0 (A side [nact], then B side [nact]). */
+struct PlaneSet {
+    int cube, kind, ng;
+    double lo, hi;
+    void *d = nullptr;
+    size_t bytes = 0;
+    uint64_t last = 0;
+    int pins = 0;
+};
+
+/* What fills one plane set, recorded so it can be launched now or inside
+ * a captured graph. */
+struct BuildOp {
+    int kind = 0, cube = 0;
+    RingGeom g{};
+    void *dst = nullptr;
+    const void *src = nullptr; /* kinds 2-5: the FP32 images */
+    const void *sc = nullptr;  /* kind 4: the FP8 scales */
+};
+
+/* Bump allocation inside one device arena (a graph wave's planes); with
+ * base == nullptr it only measures. */
+struct Bump {
+    char *base = nullptr;
+    size_t off = 0, cap = ~(size_t)0;
+};
+
+/* Where planes come from: the context's LRU cache (builds launched at
+ * once, memory from the budget) or a graph wave's list (builds recorded
+ * for capture, memory from a bump arena). */
+struct PlaneSrc {
+    std::vector<PlaneSet> *list;
+    std::vector<BuildOp> *record = nullptr; /* null: launch now */
+    Bump *bump = nullptr;                   /* null: dev_alloc */
+};
+
+struct Timer {
+    hipEvent_t a = nullptr, b = nullptr;
+    double *acc = nullptr;
+};
+
+/* One triangle resolved against a cube: planes pinned, kernel arguments
+ * filled except the scratch pointers an executor assigns. */
+struct TriJob {
+    int cube = 0, r1 = 0, r2 = 0, r3 = 0, want_norm = 0;
+    long ng = 0, P = 0, pitch = 0;
+    int nact = 0, W = 0, sym12 = 0;
+    size_t cells = 0;
+    ContractArgs a{}; /* data contraction */
+    ContractArgs b{}; /* normalisation contraction (want_norm) */
+    ReduceArgs ra{};
+    double flop_exec = 0; /* MFMA work the data contraction issues */
+};
+
+} // namespace amb
+
+struct amb_ctx {
+    int device = 0, prec = AMB_PREC_BF16X3, verbose = 0, sync_debug = 0, kvar = 0, no_sym = 0;
+    int batch_mode = AMB_BATCH_GROUPED;
+    uint64_t budget = 0, used = 0;
+    hipStream_t stream = nullptr;
+    std::recursive_mutex mu;
+    std::string err;
+    std::vector<double> ulow, uhigh, uavg;
+    std::map<int, amb::Cube> cubes;
+    int next_cube = 1;
+    std::vector<amb::PlaneSet> planes;
+    uint64_t tick = 0;
+    std::map<int, float2 *> tw;
+    std::map<int, FftPlan> plans;
+    int grouped_blocks = 0; /* persistent grid of the grouped contraction */
+    /* scratch (grow-only) */
+    float2 *G = nullptr; size_t G_bytes = 0;
+    float *C = nullptr; size_t C_bytes = 0;
+    int *Cq = nullptr; size_t Cq_bytes = 0;
+    unsigned *margin = nullptr;
+    void *out = nullptr; size_t out_bytes = 0;
+    void *wdesc = nullptr; size_t wdesc_bytes = 0; /* wave descriptors */
+    void *hstage = nullptr; size_t hstage_bytes = 0; /* pinned host staging */
+    /* stats */
+    amb_stats st{};
+    std::vector<double> batch_ms; /* per-triangle completion time of the last batch */
+    std::vector<amb::Timer> timers;
+    std::vector<hipEvent_t> evpool;
+    std::vector<AmbGraph *> graphs;
+    AmbServer *server = nullptr;
+};
+
+namespace amb {
+
+int fail(amb_ctx *c, int code, const char *fmt, ...);
+void timer_start(amb_ctx *c, double *acc);
+void timer_stop(amb_ctx *c);
+void timers_collect(amb_ctx *c);
+int dev_alloc(amb_ctx *c, void **p, size_t bytes, const char *what);
+void dev_free(amb_ctx *c, void *p, size_t bytes);
+int grow(amb_ctx *c, void **p, size_t *have, size_t need, const char *what);
+int grow_host(amb_ctx *c, void **p, size_t *have, size_t need);
+int twiddles(amb_ctx *c, int n, float2 **out, FftPlan *pl);
+int find_cube(amb_ctx *c, int id, Cube **cb);
+long plane_pitch(long ng);
+void unpin_all(amb_ctx *c);
+/* planes of (cube, kind, ring r) at grid ng from src, pinned on return */
+int get_planes(amb_ctx *c, const PlaneSrc &src, int cube_id, int kind, int r, int ng, void **out);
+/* launches one recorded build on stream s with FFT scratch G */
+hipError_t launch_build(amb_ctx *c, hipStream_t s, const BuildOp &op, float2 *G);
+size_t build_scratch_bytes(amb_ctx *c, const BuildOp &op);
+/* resolve a triangle: validate, pin its planes, fill kernel arguments */
+int prepare_tri(amb_ctx *c, const PlaneSrc &src, int cube_id, int r1, int r2, int r3, int want_norm, TriJob &j);
+/* evals and algorithmic flops of one finished triangle from its counts */
+void account_tri(amb_ctx *c, const TriJob &j, const int32_t *cnt);
+/* algorithmic HBM bytes of one c2r build (0 for derived kinds) */
+double build_fft_bytes(amb_ctx *c, const BuildOp &op);
+/* one triangle with its own launches, results in host memory */
+int bispec_core(amb_ctx *c, int id, int r1, int r2, int r3, int want_norm, amb_result *res);
+
+} // namespace amb
+
+#define HIPCHK(ctx, expr)                                                                    \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return amb::fail(ctx, AMB_EHIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                             __FILE__, __LINE__);                                            \
+    } while (0)

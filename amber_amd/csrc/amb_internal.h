@@ -90,6 +90,11 @@ hipError_t amb_launch_contract_grouped(hipStream_t s, int prec, int norm, int va
                                        int nblocks);
 hipError_t amb_launch_reduce_grouped(hipStream_t s, const ReduceArgs *d_ra, const long *d_cbegin, int ntri,
                                      long total_cells);
+/* workgroups of the persistent grouped contraction that are resident at
+ * once on device `dev` (CUs x occupancy of the launched form) */
+int amb_contract_grouped_blocks(int dev);
+/* n words set to v by a kernel (graph-capturable, unlike a memset) */
+hipError_t amb_launch_fill_u32(hipStream_t s, unsigned *p, long n, unsigned v);
 
 /* calCR triple sums: one FP64 sum per triangle over P pixels. */
 hipError_t amb_launch_triple_sum(hipStream_t s, const float *II, long P, const int *tri, int ntri,

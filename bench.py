@@ -49,7 +49,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="cfg3", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="cfg3", choices=sorted(CONFIGS) + ["cfg4", "cfg5"])
+    ap.add_argument("--qps", type=float, default=0.0, help="cfg5: mean arrival rate (0: 60%% of the measured capacity)")
+    ap.add_argument("--duration", type=float, default=30.0, help="cfg5: seconds of arrivals")
     ap.add_argument("--precision", default="bf16x3", choices=["bf16x3", "fp32", "fp8"])
     ap.add_argument("--tri-limit", type=int, default=0,
                     help="development only: K triangles (the line is then not the metric workload)")
@@ -69,6 +71,10 @@ def main():
         sys.exit(2)
 
     import amber_amd as A
+    if args.config == "cfg4":
+        return bench_cfg4(A, args)
+    if args.config == "cfg5":
+        return bench_cfg5(A, args)
     cfg = CONFIGS[args.config]
     N, nc, nc10 = cfg["N"], cfg["nc"], cfg["nc10"]
     lo, hi = A.log_rings(*cfg["rings"])
@@ -167,6 +173,186 @@ def main():
         "rel_err_vs_fp64": err, "cpu_baseline": cpu,
         "upload_s_pcie_incl": round(upload_s, 3),
     }
+    ctx.close()
+    print(json.dumps(line), flush=True)
+
+
+def mixed_workload(A, ctx, shapes, ntri, seed, max_r_frac=1.0 / 3.0):
+    """Resident cubes of the given (N, nc) shapes (no flags, all lags) and
+    ntri random closed triangles over them: ring widths uniform in 1..16
+    cells, radii below N/3, each triangle's three rings its own.  Returns
+    (cubes by shape, [(cube, r1, r2, r3)], ring lo, ring hi, triangles by cube)."""
+    rng = np.random.default_rng(seed)
+    cubes = {}
+    for k, (N, nc) in enumerate(shapes):
+        gv, samp, _ = A.generate(N, nc, nflag=0, seed=seed + 17 * k)
+        cid = ctx.load_cube(N, nc, 1.0, 0, gv, samp)
+        ctx.set_channels(cid, None, nc - 1)
+        cubes[(N, nc)] = cid
+        del gv, samp
+    lo, hi, reqs = [], [], []
+    by_cube = {c: [] for c in cubes.values()}
+    while len(reqs) < ntri:
+        N, nc = shapes[rng.integers(len(shapes))]
+        umax = N * max_r_frac - 1
+        w = rng.integers(1, 17, 3)
+        h1 = rng.uniform(8 + w[0], umax)
+        h2 = min(h1, h1 * rng.uniform(0.5, 1.0) + w[1] / 2)
+        if h2 <= w[2] + 1:
+            continue
+        h3 = rng.uniform(w[2] + 1, h2)
+        rh = np.array([h3, h2, h1])
+        rl = np.maximum(rh - w[[2, 1, 0]], 0.5)
+        if not np.all(rh[:-1] <= rh[1:]):
+            continue
+        if not any((t == [2, 1, 0]).all() for t in A.triangles(rl, rh)):
+            continue
+        base = len(lo)
+        lo.extend(rl.tolist())
+        hi.extend(rh.tolist())
+        cid = cubes[(N, nc)]
+        reqs.append((cid, base + 2, base + 1, base))
+        by_cube[cid].append((cid, base + 2, base + 1, base))
+    lo, hi = np.array(lo), np.array(hi)
+    ctx.set_rings(lo, hi)
+    return cubes, reqs, lo, hi, by_cube
+
+
+def bench_cfg4(A, args):
+    """200 uneven triangles over 9 cubes {256,512,1024}^2 x {128,384,768}:
+    per-triangle launches vs grouped waves vs one captured graph."""
+    shapes = [(N, nc) for N in (256, 512, 1024) for nc in (128, 384, 768)]
+    ctx = A.Context(device=0, precision=args.precision)
+    t0 = time.time()
+    cubes, reqs, lo, hi, _ = mixed_workload(A, ctx, shapes, 200, 4001)
+    log("cfg4: 9 cubes resident, 200 triangles, %d rings, setup %.1f s" % (len(lo), time.time() - t0))
+    res = {}
+
+    def timed(label, fn, reps):
+        ts = []
+        for _ in range(reps):
+            ctx.reset_stats()
+            t = time.time()
+            out = fn()
+            ts.append(time.time() - t)
+        st = ctx.stats()
+        log("cfg4 %s: %s s" % (label, ", ".join("%.2f" % x for x in ts)))
+        return out, min(ts), st
+
+    ctx.set_batch_mode("grouped")
+    ref, t_grp, st = timed("grouped", lambda: [drop_all(ctx, cubes), ctx.bispec_multi(reqs)][1],
+                           1 + max(args.steps, 1))
+    evals = st["evals"]
+    ctx.set_batch_mode("per_triangle")
+    per, t_per, _ = timed("per_triangle", lambda: [drop_all(ctx, cubes), ctx.bispec_multi(reqs)][1], 1)
+    same_per = all(np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) for a, b in zip(ref, per))
+    ctx.set_batch_mode("grouped")
+    t = time.time()
+    g = ctx.graph(reqs)
+    t_cap = time.time() - t
+    gout, t_graph, _ = timed("graph replay", g.run, 1 + max(args.steps, 1))
+    same_graph = all(np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) for a, b in zip(ref, gout))
+    g.close()
+    line = {"metric": "cfg4 mixed-batch evals/sec (200 uneven triangles, 9 cubes)", "unit": "evals/s",
+            "precision": args.precision, "evals": int(evals),
+            "per_triangle": {"s": round(t_per, 3), "evals_per_s": round(evals / t_per, 1)},
+            "grouped": {"s": round(t_grp, 3), "evals_per_s": round(evals / t_grp, 1)},
+            "graph": {"s": round(t_graph, 3), "evals_per_s": round(evals / t_graph, 1),
+                      "capture_s": round(t_cap, 3)},
+            "grouped_speedup": round(t_per / t_grp, 3), "graph_speedup": round(t_per / t_graph, 3),
+            "bit_identical": {"per_triangle": same_per, "graph": same_graph},
+            "note": "each timed pass rebuilds every filtered plane (eager: planes dropped first; graph: "
+                    "the plane builds are nodes of the graph)"}
+    ctx.close()
+    print(json.dumps(line), flush=True)
+
+
+def drop_all(ctx, cubes):
+    for c in cubes.values():
+        ctx.drop_planes(c)
+
+
+def bench_cfg5(A, args):
+    """Serving: Poisson arrivals of requests of 1-64 triangles on resident
+    cubes, continuous batching into grouped waves, p50/p99 latency."""
+    import threading
+    shapes = [(N, nc) for N in (256, 512) for nc in (128, 384)]
+    ctx = A.Context(device=0, precision=args.precision)
+    t0 = time.time()
+    cubes, reqs, lo, hi, by_cube = mixed_workload(A, ctx, shapes, 400, 5001)
+    log("cfg5: 4 cubes resident, %d triangle bins, setup %.1f s" % (len(reqs), time.time() - t0))
+    rng = np.random.default_rng(5002)
+    cube_ids = sorted(by_cube)
+
+    def make_request():
+        cid = cube_ids[rng.integers(len(cube_ids))]
+        pool = by_cube[cid]
+        k = int(rng.integers(1, 65))
+        pick = rng.choice(len(pool), size=min(k, len(pool)), replace=False)
+        return [pool[i] for i in pick]
+
+    # capacity probe: a closed loop of requests, one at a time
+    ctx.serve_start(max_wave=64)
+    t = time.time()
+    nprobe = 0
+    while time.time() - t < 5.0:
+        ctx.wait(ctx.submit(make_request()))
+        nprobe += 1
+    cap_rps = nprobe / (time.time() - t)
+    qps = args.qps if args.qps > 0 else 0.6 * cap_rps
+    log("cfg5: offered load %.2f requests/s for %.0f s" % (qps, args.duration))
+    ctx.reset_stats()
+    tickets = []
+    lock = threading.Lock()
+    done = []
+
+    def waiter():
+        i = 0
+        while True:
+            with lock:
+                if i >= len(tickets):
+                    if stop[0]:
+                        return
+                    tk = None
+                else:
+                    tk = tickets[i]
+            if tk is None:
+                time.sleep(0.001)
+                continue
+            outs, lat = ctx.wait(tk[0])
+            done.append((lat, tk[1]))
+            i += 1
+
+    stop = [False]
+    th = threading.Thread(target=waiter)
+    th.start()
+    t_start = time.time()
+    next_t = t_start
+    while next_t - t_start < args.duration:
+        now = time.time()
+        if now < next_t:
+            time.sleep(next_t - now)
+        r = make_request()
+        tk = ctx.submit(r)
+        with lock:
+            tickets.append((tk, len(r)))
+        next_t += rng.exponential(1.0 / qps)
+    stop[0] = True
+    th.join()
+    wall = time.time() - t_start
+    ctx.serve_stop()
+    st = ctx.stats()
+    lat = np.array([d[0] for d in done])
+    ntri = sum(d[1] for d in done)
+    line = {"metric": "cfg5 serving latency and throughput", "precision": args.precision,
+            "offered_qps": round(qps, 3), "capacity_rps_one_at_a_time": round(cap_rps, 3),
+            "requests": len(done), "triangles": int(ntri), "wall_s": round(wall, 2),
+            "throughput": {"requests_per_s": round(len(done) / wall, 3), "triangles_per_s": round(ntri / wall, 2),
+                           "evals_per_s": round(st["evals"] / wall, 1)},
+            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),
+                           "p99": round(float(np.percentile(lat, 99)), 2), "max": round(float(lat.max()), 2)},
+            "workload": "cubes {256,512}^2 x {128,384} ch resident, 400 triangle bins (widths 1-16, radii < N/3), "
+                        "requests of 1-64 bins of one cube, Poisson arrivals, waves of <= 64 triangles"}
     ctx.close()
     print(json.dumps(line), flush=True)
 

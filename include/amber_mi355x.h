@@ -181,14 +181,72 @@ int amb_bispec_ex(amb_ctx *ctx, int cube_id, int r1, int r2, int r3, int want_no
 
 /* Many triangles of one cube in one call, grouped by r1 so filtered planes
  * are built once per (ring, Ng) and reused.  tri: ntri x {r1,r2,r3}.
- * bl: ntri*W*W f32 (required); cnt, norm: may be NULL. */
+ * bl: ntri*W*W f32 (required); cnt, norm: may be NULL.  This replaces the
+ * whole rp loop of amber.c:187-326 / amber_v2.c:187-333 for one cube. */
 int amb_bispec_batch(amb_ctx *ctx, int cube_id, const int *tri, int ntri,
                      float *bl, int32_t *cnt, float *norm);
 
-/* Host wall time (ms) from the start of amb_bispec_batch to each
- * triangle's results being in host memory, for the last batch call, in the
- * order the triangles were given.  *n gets the count (<= cap written). */
+/* One triangle of one resident cube with caller-owned host outputs
+ * (bl: W*W f32, required; cnt: W*W i32 and norm: W*W f32, may be NULL; a
+ * non-NULL norm asks for the v2 normalisation).  No counterpart in the
+ * reference, which runs one cube per process. */
+typedef struct amb_tri_req {
+    int cube_id, r1, r2, r3;
+    float *bl;
+    int32_t *cnt;
+    float *norm;
+} amb_tri_req;
+
+/* Triangles of any resident cubes in one call.  Results are bit-identical
+ * to one amb_bispec_ex per triangle in every batch mode. */
+int amb_bispec_multi(amb_ctx *ctx, const amb_tri_req *req, int n);
+
+/* How amb_bispec_batch / amb_bispec_multi execute:
+ *   AMB_BATCH_GROUPED (default): waves of triangles, each wave one
+ *     persistent contraction launch over the work items of all its
+ *     triangles (largest first), one grouped reduction and one copy-out;
+ *   AMB_BATCH_PER_TRIANGLE: launches and a host sync per triangle. */
+#define AMB_BATCH_PER_TRIANGLE 0
+#define AMB_BATCH_GROUPED 1
+int amb_set_batch_mode(amb_ctx *ctx, int mode);
+
+/* Host wall time (ms) from the start of the last amb_bispec_batch /
+ * amb_bispec_multi call to each triangle's results being in host memory,
+ * in the order the triangles were given (a triangle of a grouped wave
+ * completes with its wave).  *n gets the count (<= cap written). */
 int amb_batch_times(amb_ctx *ctx, double *ms, int cap, int *n);
+
+/* Re-upload new values into a resident cube of the same shape (the next
+ * realisation, GV_data_<R>); cached planes of the cube are dropped. */
+int amb_update_cube(amb_ctx *ctx, int cube_id, const double *gv, const double *samp);
+
+/* A fixed batch captured once into a HIP graph and replayed: the graph
+ * rebuilds every filtered plane it reads from the cube buffers (so a
+ * replay after amb_update_cube sees the new realisation), runs one grouped
+ * contraction per wave, the grouped reduction and one copy-out, and the
+ * host scatters the results into the requests' output pointers, which must
+ * stay valid for the graph's life.  The graph owns its planes and scratch;
+ * its cubes cannot be released or re-channelled while it lives. */
+typedef struct amb_graph amb_graph;
+int amb_graph_create(amb_ctx *ctx, const amb_tri_req *req, int n, amb_graph **out);
+int amb_graph_run(amb_ctx *ctx, amb_graph *g);
+void amb_graph_destroy(amb_ctx *ctx, amb_graph *g);
+
+/* Serving queue: a library thread takes submitted requests in arrival
+ * order and runs them as grouped waves of at most max_wave triangles
+ * (continuous batching: a wave starts as soon as the previous one ends,
+ * with whatever has arrived).  Output pointers of a request must stay
+ * valid until amb_wait returns for its ticket.  Other calls on the context
+ * wait while a wave runs. */
+typedef uint64_t amb_ticket;
+int amb_serve_start(amb_ctx *ctx, int max_wave);
+int amb_submit(amb_ctx *ctx, const amb_tri_req *req, int n, amb_ticket *ticket);
+/* *done = 1 once every triangle of the ticket is in host memory */
+int amb_poll(amb_ctx *ctx, amb_ticket ticket, int *done);
+/* blocks until done; *latency_ms = submit to completion (may be NULL);
+ * returns the status the ticket's wave ended with */
+int amb_wait(amb_ctx *ctx, amb_ticket ticket, double *latency_ms);
+int amb_serve_stop(amb_ctx *ctx);
 
 /* One ring-filtered real-space image of plane `plane`, as the triple
  * product reads it: kind 0 is the data image of fill_s2d_float
@@ -221,6 +279,8 @@ typedef struct amb_stats {
                                   pitch, times the passes of the form (3 for
                                   BF16x3); below flop_contract x passes when
                                   r1 == r2 tiles are skipped */
+    double ms_graph;        /* device time of amb_graph_run replays */
+    int64_t n_graph_launch;
 } amb_stats;
 int amb_get_stats(amb_ctx *ctx, amb_stats *st);
 int amb_reset_stats(amb_ctx *ctx);
