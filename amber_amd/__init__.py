@@ -16,7 +16,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libamber_mi355x.so")
+# AMB_LIB names another build of the same library (A/B timing of builds)
+LIB_PATH = os.environ.get("AMB_LIB") or os.path.join(_HERE, "libamber_mi355x.so")
 if not os.path.exists(LIB_PATH):
     raise ImportError("amber_amd: %s is missing; build it with `python -m amber_amd.build`" % LIB_PATH)
 _lib = C.CDLL(LIB_PATH)

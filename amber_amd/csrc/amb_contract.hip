@@ -226,6 +226,31 @@ struct Frag<AMB_PREC_FP32, RB> {
 template <int MODE>
 constexpr int ksteps() { return MODE == AMB_PREC_FP32 ? 4 : 2; }
 
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+/* x*d split into hi + lo BF16 for 8 elements, two at a time: hi rounds to
+ * nearest even, lo is the BF16 of the exact residual x*d - hi (one FMA).
+ * Converting pairs needs one v_cvt_pk_bf16_f32 per two values, and the
+ * FP32 value of each hi comes back from the packed word by a shift or a
+ * mask. */
+__device__ __forceinline__ void split8(const float (&x)[8], const float (&d)[8], bf16x8 &hi, bf16x8 &lo)
+{
+    u32x4 hp, lp;
+#pragma unroll
+    for (int e = 0; e < 8; e += 2) {
+        const f32x2 v = {x[e] * d[e], x[e + 1] * d[e + 1]};
+        const unsigned hb = __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2));
+        const float h0 = __uint_as_float(hb << 16), h1 = __uint_as_float(hb & 0xffff0000u);
+        const f32x2 r = {fmaf(x[e], d[e], -h0), fmaf(x[e + 1], d[e + 1], -h1)};
+        hp[e / 2] = hb;
+        lp[e / 2] = __builtin_bit_cast(unsigned, __builtin_convertvector(r, bf16x2));
+    }
+    hi = __builtin_bit_cast(bf16x8, hp);
+    lo = __builtin_bit_cast(bf16x8, lp);
+}
+
 template <int MODE, int RB>
 __device__ __forceinline__ void load_frag(const Stage<MODE> &s, int step, int q, int rb, int lane, Frag<MODE, RB> &f)
 {
@@ -245,13 +270,7 @@ __device__ __forceinline__ void load_frag(const Stage<MODE> &s, int step, int q,
             float x[8];
             *(float4 *)&x[0] = *(const float4 *)&s.a[rb + i * 32 + rl][kc];
             *(float4 *)&x[4] = *(const float4 *)&s.a[rb + i * 32 + rl][kc + 4];
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const float v = x[e] * d1v[e];
-                const __bf16 h = (__bf16)v;
-                f.ah[i][e] = h;
-                f.al[i][e] = (__bf16)(v - (float)h);
-            }
+            split8(x, d1v, f.ah[i], f.al[i]);
         }
     } else if constexpr (MODE == AMB_PREC_FP8) {
         const int kc = step * 16 + hl * 8;
@@ -357,13 +376,7 @@ __device__ __forceinline__ void bf16_step_ilv(const Stage<AMB_PREC_BF16X3> &s, i
         float x[8];
         *(float4 *)&x[0] = *(const float4 *)&s.a[rb + i * 32 + rl][kc];
         *(float4 *)&x[4] = *(const float4 *)&s.a[rb + i * 32 + rl][kc + 4];
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const float v = x[e] * d1v[e];
-            const __bf16 h = (__bf16)v;
-            ah[e] = h;
-            al[e] = (__bf16)(v - (float)h);
-        }
+        split8(x, d1v, ah, al);
     };
     bf16x8 ah0, al0, ah1, al1;
     form(0, ah0, al0);
