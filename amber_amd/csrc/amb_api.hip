@@ -557,7 +557,10 @@ int amb_create(const amb_opts *o, amb_ctx **out)
     size_t fr = 0, tot = 0;
     if (hipMemGetInfo(&fr, &tot) != hipSuccess) { delete c; return AMB_EHIP; }
     c->sync_debug = getenv("AMB_SYNC_DEBUG") != nullptr;
-    if (const char *v = getenv("AMB_KVAR")) c->kvar = atoi(v) & (AMB_NVAR - 1);
+    if (const char *v = getenv("AMB_KVAR")) {
+        const int k = atoi(v);
+        c->kvar = (k >= 0 && k < AMB_NVAR) ? k : 0;
+    }
     c->no_sym = getenv("AMB_NO_SYM") != nullptr; /* for A/B checks of the symmetry */
     c->budget = (o && o->hbm_budget) ? o->hbm_budget : (uint64_t)(0.9 * (double)fr);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return AMB_EHIP; }

@@ -569,6 +569,289 @@ __device__ __forceinline__ void contract_item(const ContractArgs &a, int tile, i
     }
 }
 
+/* ---------------------------------------------------------------------------
+ * Pipelined BF16x3 data contraction (variant 2).  Same work item and the
+ * same arithmetic in the same order as the form above, so its results are
+ * identical bit for bit; only the data movement differs:
+ *   - 64-pixel chunks in a two-stage LDS ring filled by direct
+ *     global -> LDS loads (global_load_lds_dwordx4: lane i's 16 bytes land
+ *     at the instruction's LDS base + 16 i), one chunk ahead, so staging
+ *     costs neither registers nor LDS store instructions;
+ *   - rows stored without padding; a row's 16-byte slots are XOR-permuted by
+ *     the low 3 bits of the row, which makes the fragment reads of 8
+ *     consecutive rows hit 8 different bank groups;
+ *   - one barrier per chunk, and within a chunk the LDS reads and A
+ *     formation of k-step s+1 are interleaved with the MFMAs of k-step s.
+ * Rows and columns past nact and planes past nact read clamped (valid)
+ * rows; their results fall outside every window and are never stored.
+ * ------------------------------------------------------------------------- */
+#define KC2 64
+typedef __attribute__((address_space(3))) void lds_void_t;
+
+struct A16 V2Stage {
+    float a[TT * KC2];     /* D_r2 row r, float4 slot c at r*64 + 4*(c ^ (r & 7)) */
+    __bf16 bh[TT * KC2];   /* D_r3 hi col j, 16-byte slot c at j*64 + 8*(c ^ (j & 7)) */
+    __bf16 bl[TT * KC2];   /* D_r3 lo, same layout */
+    float d1[QP * KC2];    /* D_r1 of the 4 planes, plain rows */
+};
+struct A16 V2Smem {
+    V2Stage st[2];
+    float rsc[TT], csc[TT]; /* unused by this form; flush_acc takes them */
+};
+
+struct F2 {
+    bf16x8 ah[2], al[2], bh[4], bl[4];
+};
+
+__device__ __forceinline__ void lds_load16(const void *g, void *l)
+{
+    __builtin_amdgcn_global_load_lds(g, (lds_void_t *)l, 16, 0, 0);
+}
+
+/* One chunk into stage s; wave w issues A rows 16w..16w+15 (4 loads), hi and
+ * lo columns 16w..16w+15 (2 + 2 loads), wave 0 also the 4 D_r1 rows. */
+__device__ __forceinline__ void v2_issue(V2Stage &s, const ContractArgs &a, long k0, int r0, int c0, int g0,
+                                         int wv, int lane)
+{
+    const long pitch = a.pitch;
+    const int last = a.nact - 1;
+    /* opaque lane: the nine per-lane addresses are recomputed each chunk
+       instead of being hoisted out of the chunk loop into 18 registers */
+    asm volatile("" : "+v"(lane));
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int t = wv * 4 + u;                  /* 1 KB block of the A region */
+        const int r = t * 4 + (lane >> 4), ph = lane & 15;
+        const int c = ph ^ (r & 7);
+        const int gr = min(r0 + r, last);
+        lds_load16(a.D2 + (size_t)gr * pitch + k0 + c * 4, &s.a[t * 256]);
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int t = wv * 2 + u;
+        const int j = t * 8 + (lane >> 3), ph = lane & 7;
+        const int c = ph ^ (j & 7);
+        const size_t off = (size_t)min(c0 + j, last) * pitch + k0 + c * 8;
+        lds_load16(a.H3 + off, &s.bh[t * 512]);
+        lds_load16(a.L3 + off, &s.bl[t * 512]);
+    }
+    if (wv == 0) {
+        const int q = lane >> 4, c = lane & 15;
+        lds_load16(a.D1 + (size_t)min(g0 + q, last) * pitch + k0 + c * 4, &s.d1[0]);
+    }
+}
+
+/* fragments of k-step st (16 pixels) of a chunk */
+__device__ __forceinline__ void v2_frag(const V2Stage &s, int st, int q, int rb, int lane, F2 &f)
+{
+    const int rl = lane & 31, hl = lane >> 5;
+    const int kc = st * 16 + hl * 8;
+    float d1v[8];
+    *(float4 *)&d1v[0] = *(const float4 *)&s.d1[q * KC2 + kc];
+    *(float4 *)&d1v[4] = *(const float4 *)&s.d1[q * KC2 + kc + 4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int col = j * 32 + rl;
+        const int ph = (kc >> 3) ^ (col & 7);
+        f.bh[j] = *(const bf16x8 *)&s.bh[col * KC2 + ph * 8];
+        f.bl[j] = *(const bf16x8 *)&s.bl[col * KC2 + ph * 8];
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int r = rb + i * 32 + rl;
+        const int p0 = (kc >> 2) ^ (r & 7);
+        float x[8];
+        *(float4 *)&x[0] = *(const float4 *)&s.a[r * KC2 + p0 * 4];
+        *(float4 *)&x[4] = *(const float4 *)&s.a[r * KC2 + (p0 ^ 1) * 4];
+        split8(x, d1v, f.ah[i], f.al[i]);
+    }
+}
+
+/* the three passes of one k-step, in the order of mfma_frag */
+__device__ __forceinline__ void v2_mfma(const F2 &f, f32x16 (&acc)[2][4])
+{
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.al[i], f.bh[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.ah[i], f.bl[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.ah[i], f.bh[j], acc[i][j], 0, 0, 0);
+}
+
+/* A-side operands of a k-step as read from LDS: the D_r2 rows still to be
+ * multiplied by D_r1 and split. */
+struct R2 {
+    float x[2][8];
+    float d1v[8];
+};
+
+__device__ __forceinline__ void v2_read_a(const V2Stage &s, int st, int q, int rb, int lane, R2 &r)
+{
+    const int rl = lane & 31, hl = lane >> 5;
+    const int kc = st * 16 + hl * 8;
+    *(float4 *)&r.d1v[0] = *(const float4 *)&s.d1[q * KC2 + kc];
+    *(float4 *)&r.d1v[4] = *(const float4 *)&s.d1[q * KC2 + kc + 4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int row = rb + i * 32 + rl;
+        const int p0 = (kc >> 2) ^ (row & 7);
+        *(float4 *)&r.x[i][0] = *(const float4 *)&s.a[row * KC2 + p0 * 4];
+        *(float4 *)&r.x[i][4] = *(const float4 *)&s.a[row * KC2 + (p0 ^ 1) * 4];
+    }
+}
+
+__device__ __forceinline__ void v2_read_b(const V2Stage &s, int st, int lane, F2 &f)
+{
+    const int rl = lane & 31, hl = lane >> 5;
+    const int kc = st * 16 + hl * 8;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int col = j * 32 + rl;
+        const int ph = (kc >> 3) ^ (col & 7);
+        f.bh[j] = *(const bf16x8 *)&s.bh[col * KC2 + ph * 8];
+        f.bl[j] = *(const bf16x8 *)&s.bl[col * KC2 + ph * 8];
+    }
+}
+
+/* pair e (0..7) of the hi/lo split of a k-step's A operand: row block e/4,
+ * elements 2(e%4), 2(e%4)+1; the same arithmetic as split8 */
+__device__ __forceinline__ void v2_split_pair(const R2 &r, int e, u32x4 (&hp)[2], u32x4 (&lp)[2])
+{
+    const int i = e >> 2, k = (e & 3) * 2;
+    const f32x2 v = {r.x[i][k] * r.d1v[k], r.x[i][k + 1] * r.d1v[k + 1]};
+    const unsigned hb = __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2));
+    const float h0 = __uint_as_float(hb << 16), h1 = __uint_as_float(hb & 0xffff0000u);
+    const f32x2 res = {fmaf(r.x[i][k], r.d1v[k], -h0), fmaf(r.x[i][k + 1], r.d1v[k + 1], -h1)};
+    hp[i][e & 3] = hb;
+    lp[i][e & 3] = __builtin_bit_cast(unsigned, __builtin_convertvector(res, bf16x2));
+}
+
+/* MFMA m (0..23) of a k-step in the order of mfma_frag: 8 lo.hi, 8 hi.lo,
+ * 8 hi.hi, each over (row block, column block) */
+__device__ __forceinline__ void v2_mfma_one(const F2 &f, f32x16 (&acc)[2][4], int m)
+{
+    const int pass = m >> 3, i = (m >> 2) & 1, j = m & 3;
+    if (pass == 0) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.al[i], f.bh[j], acc[i][j], 0, 0, 0);
+    else if (pass == 1) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.ah[i], f.bl[j], acc[i][j], 0, 0, 0);
+    else acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.ah[i], f.bh[j], acc[i][j], 0, 0, 0);
+}
+
+/* The 24 MFMAs of one k-step with the next k-step's work placed between
+ * them in a fixed order: after MFMA 0 the 6 LDS reads of the next A side,
+ * after MFMAs 8..15 one pair each of its hi/lo split (the reads have had 7
+ * MFMAs to land), after MFMA 16 (the current lo halves are dead by then)
+ * the 8 reads of the next B side.  sched_barrier keeps the compiler from
+ * clustering the MFMAs again, so a wave's VALU and LDS work runs beside its
+ * own MFMAs. */
+__device__ __forceinline__ void v2_step(const F2 &cur, f32x16 (&acc)[2][4], const V2Stage &s, int st_next, int q,
+                                        int rb, int lane, F2 &nxt)
+{
+    R2 r;
+    u32x4 hp[2], lp[2];
+#pragma unroll
+    for (int m = 0; m < 24; ++m) {
+        v2_mfma_one(cur, acc, m);
+        if (m == 0) v2_read_a(s, st_next, q, rb, lane, r);
+        if (m >= 8 && m < 16) v2_split_pair(r, m - 8, hp, lp);
+        if (m == 16) v2_read_b(s, st_next, lane, nxt);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        nxt.ah[i] = __builtin_bit_cast(bf16x8, hp[i]);
+        nxt.al[i] = __builtin_bit_cast(bf16x8, lp[i]);
+    }
+}
+
+template <int NORM>
+__device__ __forceinline__ void contract_item_v2(const ContractArgs &a, int tile, int grp, V2Smem &L)
+{
+    const int nact = a.nact;
+    const int nt3 = (nact + TT - 1) / TT;
+    const int t2 = tile / nt3, t3 = tile - t2 * nt3;
+    const int r0 = t2 * TT, c0 = t3 * TT, g0 = grp * QP;
+    int lo = a.wlo[g0], hi = a.whi[min(g0 + QP, nact) - 1];
+    if (r0 >= hi || r0 + TT <= lo || c0 >= hi || c0 + TT <= lo) return;
+    if (a.sym12 && r0 + TT <= g0) return;
+
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int q = wv & 3, rb = (wv >> 2) * 64;
+    const bool live = g0 + q < nact;
+    f32x16 acc[2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.f;
+
+    const long nchunk = a.pitch / KC2;
+    constexpr int SEG2 = SEGC * KC / KC2; /* the same 8192-pixel segments */
+    v2_issue(L.st[0], a, 0, r0, c0, g0, wv, lane);
+    for (long c = 0; c < nchunk; ++c) {
+        /* chunk c has landed for this wave; after the barrier, for every
+           wave, and every wave is done reading the other stage */
+        __builtin_amdgcn_s_waitcnt(0x0F70); /* vmcnt(0) */
+        __syncthreads();
+        if (c + 1 < nchunk) v2_issue(L.st[(c + 1) & 1], a, (c + 1) * KC2, r0, c0, g0, wv, lane);
+        const V2Stage &S = L.st[c & 1];
+        if (live) {
+            F2 f0, f1;
+            v2_frag(S, 0, q, rb, lane, f0);
+            v2_step(f0, acc, S, 1, q, rb, lane, f1);
+            v2_step(f1, acc, S, 2, q, rb, lane, f0);
+            v2_step(f0, acc, S, 3, q, rb, lane, f1);
+            v2_mfma(f1, acc);
+            if ((c + 1) % SEG2 == 0 && c + 1 < nchunk)
+                flush_acc<AMB_PREC_BF16X3, 2>(a, acc, c + 1 == SEG2, g0 + q, r0, c0, rb, lane, 1.f, L.rsc, L.csc);
+        }
+    }
+    if (live) flush_acc<AMB_PREC_BF16X3, 2>(a, acc, nchunk <= SEG2, g0 + q, r0, c0, rb, lane, 1.f, L.rsc, L.csc);
+}
+
+__global__ __launch_bounds__(512, 1) void k_contract_v2(ContractArgs a, int ngroups, int per_xcd, int nwork)
+{
+    __shared__ V2Smem L;
+    const int b = blockIdx.x;
+    const int slot = b >> 3;
+    const int w = (b & 7) * per_xcd + slot;
+    if (slot >= per_xcd || w >= nwork) return;
+    contract_item_v2<0>(a, w / ngroups, w % ngroups, L);
+}
+
+__global__ __launch_bounds__(512, 1) void k_contract_grouped_v2(const ContractArgs *__restrict__ descs,
+                                                                const long *__restrict__ begin, int ntri,
+                                                                unsigned long long *counter)
+{
+    __shared__ V2Smem L;
+    __shared__ long item;
+    const long total = begin[ntri];
+    for (;;) {
+        if (threadIdx.x == 0) item = (long)atomicAdd(counter, 1ull);
+        __syncthreads();
+        const long w = item;
+        __syncthreads();
+        if (w >= total) return;
+        int lo = 0, hi = ntri - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (begin[mid] <= w) lo = mid;
+            else hi = mid - 1;
+        }
+        const ContractArgs &a = descs[lo];
+        const int ngroups = (a.nact + QP - 1) / QP;
+        const long r = w - begin[lo];
+        contract_item_v2<0>(a, (int)(r / ngroups), (int)(r % ngroups), L);
+        __syncthreads();
+    }
+}
+
 /* One triangle: workgroups numbered tile-major and dealt to the 8 XCDs in
  * contiguous runs (hardware places workgroup b on XCD b mod 8). */
 template <int MODE, int NORM, int NWR, int PIPE>
@@ -685,14 +968,18 @@ template <int MODE, int NORM>
 static void launch_grouped_var(int var, dim3 grid, hipStream_t s, const ContractArgs *d, const long *b, int n,
                                unsigned long long *ctr)
 {
-    if (var == 1) hipLaunchKernelGGL((k_contract_grouped<MODE, NORM, 2, 1>), grid, dim3(512), 0, s, d, b, n, ctr);
+    if (var == 2 && MODE == AMB_PREC_BF16X3 && NORM == 0)
+        hipLaunchKernelGGL(k_contract_grouped_v2, grid, dim3(512), 0, s, d, b, n, ctr);
+    else if (var == 1) hipLaunchKernelGGL((k_contract_grouped<MODE, NORM, 2, 1>), grid, dim3(512), 0, s, d, b, n, ctr);
     else hipLaunchKernelGGL((k_contract_grouped<MODE, NORM, 2, 0>), grid, dim3(512), 0, s, d, b, n, ctr);
 }
 
 template <int MODE, int NORM>
 static void launch_var(int var, dim3 grid, hipStream_t s, const ContractArgs &a, int ngroups, int per_xcd, int nwork)
 {
-    if (var == 1) hipLaunchKernelGGL((k_contract<MODE, NORM, 2, 1>), grid, dim3(512), 0, s, a, ngroups, per_xcd, nwork);
+    if (var == 2 && MODE == AMB_PREC_BF16X3 && NORM == 0)
+        hipLaunchKernelGGL(k_contract_v2, grid, dim3(512), 0, s, a, ngroups, per_xcd, nwork);
+    else if (var == 1) hipLaunchKernelGGL((k_contract<MODE, NORM, 2, 1>), grid, dim3(512), 0, s, a, ngroups, per_xcd, nwork);
     else hipLaunchKernelGGL((k_contract<MODE, NORM, 2, 0>), grid, dim3(512), 0, s, a, ngroups, per_xcd, nwork);
 }
 

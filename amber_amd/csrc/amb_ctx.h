@@ -101,7 +101,7 @@ struct TriJob {
 } // namespace amb
 
 struct amb_ctx {
-    int device = 0, prec = AMB_PREC_BF16X3, verbose = 0, sync_debug = 0, kvar = 0, no_sym = 0;
+    int device = 0, prec = AMB_PREC_BF16X3, verbose = 0, sync_debug = 0, kvar = 2, no_sym = 0;
     int batch_mode = AMB_BATCH_GROUPED;
     uint64_t budget = 0, used = 0;
     hipStream_t stream = nullptr;

@@ -49,9 +49,9 @@ struct ContractArgs {
     unsigned *margin;          /* norm: max |S/P - q| as float bits */
 };
 /* var: kernel variant (0: plain k-steps, 1: BF16x3 k-steps interleaving
- * MFMA with the next block's A formation); chosen per shape by the
- * autotuner */
-#define AMB_NVAR 2
+ * MFMA with the next block's A formation, 2: BF16x3 data form with direct
+ * global->LDS loads and k-step pipelining; other forms run variant 0) */
+#define AMB_NVAR 3
 hipError_t amb_launch_contract(hipStream_t s, int prec, int norm, int var, const ContractArgs &a);
 hipError_t amb_launch_split(hipStream_t s, const float *D, size_t n, __bf16 *H, __bf16 *L);
 /* FP8 preparation of n planes: sc[0..n) A-side scales, sc[n..2n) B-side

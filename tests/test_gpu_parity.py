@@ -236,3 +236,37 @@ def test_ring_image_vs_oracle(ctx, N, nc, lo, hi, ng):
     print("ring image worst normwise rel err %.3g" % worst, flush=True)
     assert worst <= 1e-6
     ctx.release_cube(cube)
+
+
+@pytest.mark.parametrize("N,nc,nflag,nc10,rings", [
+    (64, 200, 7, 199, (2, 20, 5)),    # partial 128-row tiles, flags, one segment
+    (256, 40, 0, 12, (20, 60, 3)),    # lag windows, Ng up to 186: several 8192-pixel segments
+])
+def test_pipelined_variant_is_bit_identical(ctx, N, nc, nflag, nc10, rings, monkeypatch):
+    """The BF16x3 kernel with direct global->LDS loads and pipelined k-steps
+    (variant 2) runs the same arithmetic in the same order as variant 0: its
+    results, per triangle and in grouped waves, must match bit for bit."""
+    lo, hi = A.log_rings(*rings)
+    gv, samp, uf = A.generate(N, nc, nflag=nflag, seed=61)
+    fk = 1 if nflag else 0
+    ctxs, cubes = [], []
+    for var in ("0", "2"):
+        monkeypatch.setenv("AMB_KVAR", var)
+        c = A.Context(device=0, precision="bf16x3")
+        cb = c.load_cube(N, nc, 1.0, fk, gv, samp)
+        c.set_channels(cb, uf if fk else None, nc10)
+        c.set_rings(lo, hi)
+        ctxs.append(c)
+        cubes.append(cb)
+    monkeypatch.delenv("AMB_KVAR")
+    tris = A.triangles(lo, hi)
+    print("case pipelined variant N=%d nc=%d ntri=%d" % (N, nc, len(tris)), flush=True)
+    for mode in ("per_triangle", "grouped"):
+        outs = []
+        for c, cb in zip(ctxs, cubes):
+            c.set_batch_mode(mode)
+            outs.append(c.bispec_batch(cb, tris))
+        assert np.array_equal(outs[0][1], outs[1][1])
+        assert np.array_equal(outs[0][0], outs[1][0]), mode
+    for c in ctxs:
+        c.close()
