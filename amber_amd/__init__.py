@@ -35,6 +35,7 @@ EXPORTS = [
     "amb_get_stats", "amb_reset_stats", "amb_drop_planes", "amb_batch_times", "amb_ring_image",
     "amb_bispec_multi", "amb_set_batch_mode", "amb_update_cube", "amb_graph_create", "amb_graph_run",
     "amb_graph_destroy", "amb_serve_start", "amb_submit", "amb_poll", "amb_wait", "amb_serve_stop",
+    "amb_cyl_bispec",
 ]
 
 BATCH_PER_TRIANGLE, BATCH_GROUPED = 0, 1
@@ -61,6 +62,24 @@ class Result(C.Structure):
 class TriReq(C.Structure):
     _fields_ = [("cube_id", C.c_int), ("r1", C.c_int), ("r2", C.c_int), ("r3", C.c_int),
                 ("bl", _fp), ("cnt", _ip), ("norm", _fp)]
+
+
+class CylParams(C.Structure):
+    _fields_ = [("ncut", C.c_int), ("mabs_norm", C.c_double), ("rc", C.c_double), ("rc_prime", C.c_double),
+                ("dnu", C.c_double), ("window", C.c_int)]
+
+
+def _analysis_constants():
+    """the constants of the reference's analysis chain (mabs_funcs.py:7-27)"""
+    import math
+    theta0 = 0.6 * 24.68 * math.pi / 180
+    fact = math.pi * theta0 * theta0 / 3
+    nu0, k_b, c = 154.255e6, 1.38e3, 3e8
+    rjfac = 2. * k_b * nu0 * nu0 / (c * c) / 1e3
+    return dict(ncut=8, mabs_norm=fact * rjfac ** 3, rc=9210.11291939594, rc_prime=17., dnu=0.04)
+
+
+ANALYSIS = _analysis_constants()
 
 
 class Stats(C.Structure):
@@ -109,6 +128,7 @@ _lib.amb_submit.argtypes = [C.c_void_p, C.POINTER(TriReq), C.c_int, C.POINTER(C.
 _lib.amb_poll.argtypes = [C.c_void_p, C.c_uint64, _ip]
 _lib.amb_wait.argtypes = [C.c_void_p, C.c_uint64, _dp]
 _lib.amb_serve_stop.argtypes = [C.c_void_p]
+_lib.amb_cyl_bispec.argtypes = [C.c_void_p, C.c_int, C.c_int, _fp, C.c_void_p, C.c_int, C.POINTER(CylParams), _dp]
 _lib.amb_get_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
 _lib.amb_reset_stats.argtypes = [C.c_void_p]
 _lib.amb_uavg.argtypes = [_dp, _dp, C.c_int, _dp]
@@ -454,6 +474,29 @@ class Context:
 
     def serve_stop(self):
         self._c(_lib.amb_serve_stop(self._h))
+
+    def cyl_bispec(self, bl, den, window=False, **kw):
+        """Lag-space post-processing (get_cyl_BS / cylbs_win) of one or more
+        triangles' W x W arrays: den is the int32 count array (v1 files) or
+        the float32 normalisation (v2 files).  Keyword arguments override the
+        reference analysis constants (ncut, mabs_norm, rc, rc_prime, dnu).
+        Returns float64 (n, N, N) (or (N, N) for one triangle), N = W - 2 ncut."""
+        bl = np.ascontiguousarray(bl, np.float32)
+        one = bl.ndim == 2
+        bl = bl.reshape((-1,) + bl.shape[-2:])
+        den = np.asarray(den)
+        is_int = np.issubdtype(den.dtype, np.integer)
+        den = np.ascontiguousarray(den, np.int32 if is_int else np.float32).reshape(bl.shape)
+        k = dict(ANALYSIS)
+        k.update(kw)
+        n, W = bl.shape[0], bl.shape[1]
+        N = W - 2 * int(k["ncut"])
+        out = np.zeros((n, N, N))
+        p = CylParams(int(k["ncut"]), float(k["mabs_norm"]), float(k["rc"]), float(k["rc_prime"]),
+                      float(k["dnu"]), int(bool(window)))
+        self._c(_lib.amb_cyl_bispec(self._h, n, W, _p(bl, _fp), den.ctypes.data_as(C.c_void_p), int(is_int),
+                                    C.byref(p), _p(out, _dp)))
+        return out[0] if one else out
 
     def batch_times(self):
         """completion time (ms from the call's start) of each triangle of the last batch"""

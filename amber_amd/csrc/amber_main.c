@@ -168,30 +168,52 @@ int main(int argc, char **argv)
 
     const int W = 2 * nc10 + 1;
     const size_t ncsize = (size_t)W * W;
-    float *bl = malloc(sizeof(float) * ncsize);
+    /* the rp loop (amber.c:187-304) as batches of triangles: each
+       amb_bispec_batch call runs grouped waves; blocks keep the host
+       buffers near 1 GiB */
+    int blk = (int)((1u << 30) / (ncsize * 8));
+    if (blk < 1) blk = 1;
+    if (blk > 64) blk = 64;
+    float *bl = malloc(sizeof(float) * ncsize * blk);
 #ifndef AMB_V2
-    int32_t *cnt = malloc(sizeof(int32_t) * ncsize);
+    int32_t *cnt = malloc(sizeof(int32_t) * ncsize * blk);
 #else
-    float *norm = malloc(sizeof(float) * ncsize);
+    float *norm = malloc(sizeof(float) * ncsize * blk);
 #endif
+    int *tri = malloc(sizeof(int) * 3 * blk);
     if (rp_end < 0 || rp_end > rcount) rp_end = rcount;
-    for (int rp = rp_start; rp < rp_end; ++rp) {
-        int r1 = (int)rdata[rp * 4 + 0], r2 = (int)rdata[rp * 4 + 1], r3 = (int)rdata[rp * 4 + 2];
+    for (int rp0 = rp_start; rp0 < rp_end; rp0 += blk) {
+        const int nb = rp_end - rp0 < blk ? rp_end - rp0 : blk;
+        for (int k = 0; k < nb; ++k)
+            for (int e = 0; e < 3; ++e) tri[3 * k + e] = (int)rdata[(rp0 + k) * 4 + e];
         double t0 = now();
 #ifndef AMB_V2
-        double Ntri, Num;
-        CHECK(amb_bispec_v1(ctx, cube, r1, r2, r3, rdata[rp * 4 + 3], bl, cnt, &Ntri, &Num));
-        CHECK(amb_write_bsout_v1(res_path, r1, r2, r3, Rlzn, W, Ntri, Num, bl, cnt));
+        CHECK(amb_bispec_batch(ctx, cube, tri, nb, bl, cnt, NULL));
 #else
-        double inv;
-        CHECK(amb_bispec_v2(ctx, cube, r1, r2, r3, bl, norm, &inv));
-        CHECK(amb_write_bsout_v2(res_path, r1, r2, r3, Rlzn, W, inv, bl, norm));
+        CHECK(amb_bispec_batch(ctx, cube, tri, nb, bl, NULL, norm));
 #endif
-        printf("rp=%d (r1=%d r2=%d r3=%d, Ng=%ld): %.3f s\n", rp, r1, r2, r3, amb_ng(hi[r1], Du), now() - t0);
+        const double dt = now() - t0;
+        for (int k = 0; k < nb; ++k) {
+            const int rp = rp0 + k, r1 = tri[3 * k], r2 = tri[3 * k + 1], r3 = tri[3 * k + 2];
+            const long Ng1 = amb_ng(hi[r1], Du), Ng2 = Ng1;
+#ifndef AMB_V2
+            /* amber.c:198-201,287 with the same double expression order */
+            double gfact = (1. * Ng1) / (1. * NN);
+            double gfact_sq = gfact * gfact;
+            double Num = gfact_sq * rdata[rp * 4 + 3];
+            double inv = 1.0 / (Ng1 * Ng2 * 1.);
+            CHECK(amb_write_bsout_v1(res_path, r1, r2, r3, Rlzn, W, Num * inv, Num, bl + ncsize * k, cnt + ncsize * k));
+#else
+            double inv = 1.0 / (Ng1 * Ng2 * 1.);
+            CHECK(amb_write_bsout_v2(res_path, r1, r2, r3, Rlzn, W, inv, bl + ncsize * k, norm + ncsize * k));
+#endif
+            printf("rp=%d (r1=%d r2=%d r3=%d, Ng=%ld)\n", rp, r1, r2, r3, Ng1);
+        }
+        printf("triangles %d..%d: %.3f s\n", rp0, rp0 + nb - 1, dt);
     }
     printf("Total time taken = %.3f s\n", now() - T);
     amb_destroy(ctx);
-    free(bl); free(rdata); free(lo); free(hi); free(uavg); free(chans);
+    free(bl); free(tri); free(rdata); free(lo); free(hi); free(uavg); free(chans);
 #ifndef AMB_V2
     free(cnt);
 #else

@@ -38,6 +38,11 @@ CONFIGS = {
 PEAK_TFLOPS = {"bf16x3": 2500.0, "fp32": 157.3, "fp8": 5000.0}
 PASSES = {"bf16x3": 3, "fp32": 1, "fp8": 1}
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per contraction launch of the default workload (grouped waves,
+# pipelined BF16x3 form): FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 from two PMC
+# runs of this workload, the factors being what the same counters read for
+# a plain 1 GiB device copy.  Counters cannot be read inside a timed run.
+MEASURED_TRAFFIC = {("cfg3", "bf16x3", "grouped"): 2.842e11}
 
 
 def log(msg):
@@ -49,7 +54,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="cfg3", choices=sorted(CONFIGS) + ["cfg4", "cfg5"])
+    ap.add_argument("--config", default="cfg3", choices=sorted(CONFIGS) + ["cfg4", "cfg5", "post"])
     ap.add_argument("--qps", type=float, default=0.0, help="cfg5: mean arrival rate (0: 60%% of the measured capacity)")
     ap.add_argument("--duration", type=float, default=30.0, help="cfg5: seconds of arrivals")
     ap.add_argument("--precision", default="bf16x3", choices=["bf16x3", "fp32", "fp8"])
@@ -76,6 +81,8 @@ def main():
         return bench_cfg4(A, args)
     if args.config == "cfg5":
         return bench_cfg5(A, args)
+    if args.config == "post":
+        return bench_post(A, args)
     cfg = CONFIGS[args.config]
     N, nc, nc10 = cfg["N"], cfg["nc"], cfg["nc10"]
     lo, hi = A.log_rings(*cfg["rings"])
@@ -132,7 +139,9 @@ def main():
     ach = st["flop_contract"] / (ms_c * 1e-3) / 1e12 if ms_c > 0 else 0.0
     peak = PEAK_TFLOPS[args.precision]
     roof = {"bound": "mfma", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
-            "frac": round(ach / peak, 4), "traffic": None,
+            "frac": round(ach / peak, 4),
+            "traffic": (MEASURED_TRAFFIC.get((args.config, args.precision, args.batch_mode))
+                        if not args.tri_limit and os.environ.get("AMB_KVAR", "2") == "2" else None),
             "kernel": "k_contract<%s>" % args.precision,
             "mfma_passes": PASSES[args.precision],
             "avg_launch_ms": round(ms_c / max(st["n_contract_launch"], 1), 3),
@@ -355,6 +364,32 @@ def bench_cfg5(A, args):
                            "p99": round(float(np.percentile(lat, 99)), 2), "max": round(float(lat.max()), 2)},
             "workload": "cubes {256,512}^2 x {128,384} ch resident, 400 triangle bins (widths 1-16, radii < N/3), "
                         "requests of 1-64 bins of one cube, Poisson arrivals, waves of <= 64 triangles"}
+    ctx.close()
+    print(json.dumps(line), flush=True)
+
+
+def bench_post(A, args):
+    """Lag-space post-processing (get_cyl_BS) of cfg3-sized result arrays
+    (W = 1535, N = 1519 after the 8-lag cut): triangles/s and the FP64 rate
+    of the two direct-sum transforms (8 M N^2 flop per triangle)."""
+    nc10, n = 767, 32
+    W = 2 * nc10 + 1
+    rng = np.random.default_rng(6001)
+    bl = rng.standard_normal((n, W, W)).astype(np.float32)
+    cnt = rng.integers(1, 768, (n, W, W)).astype(np.int32)
+    ctx = A.Context(device=0)
+    ctx.cyl_bispec(bl[:2], cnt[:2])
+    t = time.time()
+    for _ in range(max(args.steps, 1)):
+        ctx.cyl_bispec(bl, cnt)
+    dt = (time.time() - t) / max(args.steps, 1)
+    N = W - 2 * A.ANALYSIS["ncut"]
+    M = nc10 + 1 - A.ANALYSIS["ncut"]
+    fl = 8.0 * M * N * N * n
+    line = {"metric": "lag-space post-processing (fold, normalise, window, lag->k_par transform)",
+            "value": round(n / dt, 2), "unit": "triangles/s", "triangles": n, "W": W, "N": N,
+            "s_per_batch": round(dt, 4), "fp64_tflops": round(fl / dt / 1e12, 2), "fp64_peak_tflops": 78.6,
+            "note": "host->device copy of the W x W arrays and device->host copy of the N x N results included"}
     ctx.close()
     print(json.dumps(line), flush=True)
 

@@ -258,6 +258,25 @@ int amb_serve_stop(amb_ctx *ctx);
 int amb_ring_image(amb_ctx *ctx, int cube_id, int ring, int64_t ng, int plane, int kind,
                    float *out, int64_t cap);
 
+/* Lag-space post-processing of result arrays (the consumer of the bsout
+ * files): get_cyl_BS (mabs_funcs.py:375-388), or with window=1 cylbs_win
+ * (mfunc_winds.py:113-127).  For each of n triangles: fold the W x W arrays
+ * (fold_quadrants, mabs_funcs.py:210-222), divide by mabs_norm and the
+ * folded denominator, cut ncut rows/columns (cut_nan, :259-260), apply the
+ * Blackman-Nuttall window if asked, and transform to k_par
+ * (Bdnu2Bk_fft, :325-334).  den: W x W int32 counts (v1, den_is_int = 1)
+ * or float32 norms (v2).  out: n x N x N float64, N = W - 2 ncut. */
+typedef struct amb_cyl_params {
+    int ncut;         /* n_nancut, mabs_funcs.py:16 */
+    double mabs_norm; /* mabs_funcs.py:22 */
+    double rc;        /* mabs_funcs.py:24 */
+    double rc_prime;  /* mabs_funcs.py:25 */
+    double dnu;       /* dnu_mwa, mabs_funcs.py:27 */
+    int window;
+} amb_cyl_params;
+int amb_cyl_bispec(amb_ctx *ctx, int n, int W, const float *bl, const void *den, int den_is_int,
+                   const amb_cyl_params *p, double *out);
+
 /* Drop every cached filtered plane of a cube (they are rebuilt on demand).
  * Planes stay resident across calls otherwise, as the serving loop wants. */
 int amb_drop_planes(amb_ctx *ctx, int cube_id);
