@@ -1,0 +1,68 @@
+"""GPU at the metric workload's own size (768 channels x 512^2 grid, all
+767 lags, rings up to 200 cells, Ng = 612) through properties that hold at
+any size, since the FP64 oracle cannot run whole triangles here:
+  - counts equal the closed form nc - (max(0,d2,d3) - min(0,d2,d3));
+  - doubling the cube scales B by exactly 8: a power-of-two scale passes
+    unchanged through the FFT, the hi/lo split, the MFMA products, the
+    segmented sums and the compensated reduction, so the identity is bit
+    for bit.
+A few lag cells of the largest triangle are also checked against the FP64
+oracle (per cell relative to the sum of |terms|)."""
+import numpy as np
+import pytest
+
+import amber_amd as A
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+N, NC, NC10 = 512, 768, 767
+
+
+@pytest.fixture(scope="module")
+def big():
+    gv, samp, uf = A.generate(N, NC, nflag=0, seed=3001)
+    return gv, samp, {}
+
+
+@pytest.mark.parametrize("prec", ["bf16x3", "fp32"])
+def test_fullsize_counts_and_scaling(ctx, big, prec):
+    gv, samp, cache = big
+    lo, hi = A.log_rings(1, 200, 24)
+    tris = A.triangles(lo, hi)
+    pick = [t for t in tris if t[0] == 23][:1] + [tris[len(tris) // 2]]
+    ctx.set_precision(prec)
+    ctx.set_rings(lo, hi)
+    c1 = ctx.load_cube(N, NC, 1.0, 0, gv, samp)
+    ctx.set_channels(c1, None, NC10)
+    c2 = ctx.load_cube(N, NC, 1.0, 0, 2.0 * gv, samp)
+    ctx.set_channels(c2, None, NC10)
+    print("case fullsize prec=%s triangles=%s" % (prec, [tuple(t) for t in pick]), flush=True)
+    d = np.arange(-NC10, NC10 + 1)
+    d2, d3 = np.meshgrid(d, d, indexing="ij")
+    spread = np.maximum(0, np.maximum(d2, d3)) - np.minimum(0, np.minimum(d2, d3))
+    closed = np.maximum(NC - spread, 0)
+    for t in pick:
+        a = ctx.bispec_ex(c1, *map(int, t))
+        b = ctx.bispec_ex(c2, *map(int, t))
+        assert np.array_equal(a["cnt"], closed)
+        assert np.array_equal(b["bl64"], 8.0 * a["bl64"])
+        assert np.array_equal(b["bl"], 8.0 * a["bl"])
+    # FP64 oracle on a few cells of the largest triangle
+    t = pick[0]
+    res = ctx.bispec_ex(c1, *map(int, t))
+    rng = np.random.default_rng(9)
+    i2 = rng.integers(0, 2 * NC10 + 1, 6)
+    i3 = rng.integers(0, 2 * NC10 + 1, 6)
+    if "cells" not in cache:  # FP64 planes of 768 channels at Ng = 612: built once for both forms
+        mask = np.ones(NC, np.uint8)
+        cache["cells"] = O.bispec_cells(N, NC, 1.0, gv, samp, lo, hi, *map(int, t), mask, NC10, i2 - NC10,
+                                        i3 - NC10, want_abs=True)
+    b64, c64, _, ab = cache["cells"]
+    got = res["bl64"][i2, i3]
+    cell = np.max(np.abs(got - b64) / np.maximum(ab, 1e-300))
+    print("fullsize Ng=%d prec=%s per-cell/|terms| %.3g" % (res["ng"], prec, cell), flush=True)
+    assert cell <= 2e-8
+    ctx.release_cube(c1)
+    ctx.release_cube(c2)
+    ctx.set_precision("bf16x3")
