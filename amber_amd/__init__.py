@@ -35,7 +35,7 @@ EXPORTS = [
     "amb_get_stats", "amb_reset_stats", "amb_drop_planes", "amb_batch_times", "amb_ring_image",
     "amb_bispec_multi", "amb_set_batch_mode", "amb_update_cube", "amb_graph_create", "amb_graph_run",
     "amb_graph_destroy", "amb_serve_start", "amb_submit", "amb_poll", "amb_wait", "amb_serve_stop",
-    "amb_cyl_bispec",
+    "amb_cyl_bispec", "amb_maps_cldnu", "amb_write_cldnu", "amb_read_params",
 ]
 
 BATCH_PER_TRIANGLE, BATCH_GROUPED = 0, 1
@@ -87,7 +87,9 @@ class Stats(C.Structure):
                 ("ms_norm", C.c_double), ("ms_total", C.c_double), ("flop_contract", C.c_double),
                 ("bytes_fft", C.c_double), ("n_contract_launch", C.c_int64), ("n_fft_launch", C.c_int64),
                 ("evals", C.c_int64), ("flop_contract_exec", C.c_double),
-                ("ms_graph", C.c_double), ("n_graph_launch", C.c_int64)]
+                ("ms_graph", C.c_double), ("n_graph_launch", C.c_int64),
+                ("ms_maps", C.c_double), ("ms_maps_total", C.c_double), ("flop_maps", C.c_double),
+                ("n_maps_launch", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -129,6 +131,11 @@ _lib.amb_poll.argtypes = [C.c_void_p, C.c_uint64, _ip]
 _lib.amb_wait.argtypes = [C.c_void_p, C.c_uint64, _dp]
 _lib.amb_serve_stop.argtypes = [C.c_void_p]
 _lib.amb_cyl_bispec.argtypes = [C.c_void_p, C.c_int, C.c_int, _fp, C.c_void_p, C.c_int, C.POINTER(CylParams), _dp]
+_lib.amb_maps_cldnu.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_double, _dp, _dp, C.c_int, C.c_double,
+                                C.c_double, _dp, _dp, _dp, _ip]
+_lib.amb_write_cldnu.argtypes = [C.c_char_p, C.c_int, C.c_int64, _dp, _dp, _dp, _ip]
+_lib.amb_read_params.argtypes = [C.c_char_p, C.POINTER(C.c_int64), _dp, _dp, _dp, _dp, C.POINTER(C.c_int64),
+                                 C.c_char_p, C.c_size_t]
 _lib.amb_get_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
 _lib.amb_reset_stats.argtypes = [C.c_void_p]
 _lib.amb_uavg.argtypes = [_dp, _dp, C.c_int, _dp]
@@ -497,6 +504,19 @@ class Context:
         self._c(_lib.amb_cyl_bispec(self._h, n, W, _p(bl, _fp), den.ctypes.data_as(C.c_void_p), int(is_int),
                                     C.byref(p), _p(out, _dp)))
         return out[0] if one else out
+
+    def maps_cldnu(self, N, nc, Du, gv, samp, NB=15, umin=1.0, umax=500.0):
+        """MAPS C_l(dnu) of a cube in the file layout (maps.c:100-229 over
+        cl_dnu): returns (cldnu[NB][nc], numdnu[nc], binu[NB], num[NB])."""
+        gv = np.ascontiguousarray(gv, np.float64)
+        samp = np.ascontiguousarray(samp, np.float64)
+        npix = N * (N // 2 + 1)
+        if gv.size != npix * nc * 2 or samp.size != npix * nc:
+            raise ValueError("cube arrays do not match N=%d nc=%d" % (N, nc))
+        cl = np.zeros((NB, nc)); nd = np.zeros(nc); bu = np.zeros(NB); nm = np.zeros(NB, np.int32)
+        self._c(_lib.amb_maps_cldnu(self._h, N, nc, float(Du), _p(gv, _dp), _p(samp, _dp), NB, float(umin),
+                                    float(umax), _p(cl, _dp), _p(nd, _dp), _p(bu, _dp), _p(nm, _ip)))
+        return cl, nd, bu, nm
 
     def batch_times(self):
         """completion time (ms from the call's start) of each triangle of the last batch"""

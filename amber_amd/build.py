@@ -1,5 +1,6 @@
 """Builds libamber_mi355x.so (HIP for gfx950 + host C) and the drop-in
-``amber`` / ``amber_v2`` executables, in-tree, with hipcc and gcc.
+``amber`` / ``amber_v2`` / ``amber_maps`` executables, in-tree, with hipcc
+and gcc.
 
     python -m amber_amd.build            # library + drivers
 """
@@ -15,7 +16,7 @@ INC = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libamber_mi355x.so")
 BIN = os.path.join(HERE, "bin")
 
-HIP_SOURCES = ["amb_fft.hip", "amb_contract.hip", "amb_api.hip", "amb_wave.hip", "amb_post.hip"]
+HIP_SOURCES = ["amb_fft.hip", "amb_contract.hip", "amb_api.hip", "amb_wave.hip", "amb_post.hip", "amb_maps.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # packed-FP32 instructions are kept out of every HIP object
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", "-Wno-unused-value",
@@ -46,9 +47,10 @@ def build(verbose=False):
             print(out)
         objs.append(o)
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-o", LIB] + objs + ["-lgomp", "-lm"])
-    for name, extra in (("amber", []), ("amber_v2", ["-DAMB_V2"])):
+    for name, src, extra in (("amber", "amber_main.c", []), ("amber_v2", "amber_main.c", ["-DAMB_V2"]),
+                             ("amber_maps", "amber_maps.c", [])):
         _run(["gcc", "-O2", "-std=gnu11", "-Wall", "-I" + INC] + extra +
-             [os.path.join(CSRC, "amber_main.c"), "-o", os.path.join(BIN, name),
+             [os.path.join(CSRC, src), "-o", os.path.join(BIN, name),
               "-L" + HERE, "-lamber_mi355x", "-Wl,-rpath,$ORIGIN/.."])
     return LIB
 

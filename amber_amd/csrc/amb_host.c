@@ -213,6 +213,44 @@ int amb_write_bsout_v2(const char *res_path, int r1, int r2, int r3, int rlzn, i
     return AMB_OK;
 }
 
+int amb_read_params(const char *path, int64_t *NN, double *Du, double *nu0, double *dnu, double *chan0,
+                    int64_t *nc, char *data_path, size_t cap)
+{   /* read_inputs, amber_funcs.c:159-172: seven fields, the path at most 511 chars */
+    if (!path || !data_path || cap < 2) return AMB_EARG;
+    FILE *fp = fopen(path, "r");
+    if (!fp) return AMB_EIO;
+    char buf[512];
+    long nn, c;
+    double du, a, b, ch;
+    int n = fscanf(fp, "%ld%lf%lf%lf%lf%ld%511s", &nn, &du, &a, &b, &ch, &c, buf);
+    fclose(fp);
+    if (n != 7) return AMB_EFORMAT;
+    if (NN) *NN = nn;
+    if (Du) *Du = du;
+    if (nu0) *nu0 = a;
+    if (dnu) *dnu = b;
+    if (chan0) *chan0 = ch;
+    if (nc) *nc = c;
+    if (strlen(buf) + 1 > cap) return AMB_EARG;
+    strcpy(data_path, buf);
+    return AMB_OK;
+}
+
+int amb_write_cldnu(const char *path, int NB, int64_t nc, const double *binu, const double *cldnu,
+                    const double *numdnu, const int32_t *num)
+{   /* maps.c:213-222: bin-major, one line per (bin, lag) */
+    if (!path || NB < 0 || nc < 0 || (NB && nc && (!binu || !cldnu || !numdnu || !num))) return AMB_EARG;
+    FILE *fp = fopen(path, "w");
+    if (!fp) return AMB_EIO;
+    int ok = 1;
+    for (int i = 0; i < NB && ok; ++i)
+        for (int64_t j = 0; j < nc && ok; ++j)
+            ok = fprintf(fp, "%d\t%e\t%d\t%e\t%e\t%d\n", i, binu[i], (int)j, cldnu[(size_t)i * nc + j], numdnu[j],
+                         num[i]) > 0;
+    if (fclose(fp) != 0 || !ok) return AMB_EIO;
+    return AMB_OK;
+}
+
 /* ---------------------------------------------------------------- rules */
 long amb_ng(double uhigh, double du)
 {   /* amber.c:193-197, same integer truncation and parity step */

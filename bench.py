@@ -54,7 +54,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="cfg3", choices=sorted(CONFIGS) + ["cfg4", "cfg5", "post"])
+    ap.add_argument("--config", default="cfg3", choices=sorted(CONFIGS) + ["cfg4", "cfg5", "post", "maps"])
     ap.add_argument("--qps", type=float, default=0.0, help="cfg5: mean arrival rate (0: 60%% of the measured capacity)")
     ap.add_argument("--duration", type=float, default=30.0, help="cfg5: seconds of arrivals")
     ap.add_argument("--precision", default="bf16x3", choices=["bf16x3", "fp32", "fp8"])
@@ -83,6 +83,8 @@ def main():
         return bench_cfg5(A, args)
     if args.config == "post":
         return bench_post(A, args)
+    if args.config == "maps":
+        return bench_maps(A, args)
     cfg = CONFIGS[args.config]
     N, nc, nc10 = cfg["N"], cfg["nc"], cfg["nc10"]
     lo, hi = A.log_rings(*cfg["rings"])
@@ -391,6 +393,47 @@ def bench_post(A, args):
             "s_per_batch": round(dt, 4), "fp64_tflops": round(fl / dt / 1e12, 2), "fp64_peak_tflops": 78.6,
             "note": "host->device copy of the W x W arrays and device->host copy of the N x N results included"}
     ctx.close()
+    print(json.dumps(line), flush=True)
+
+
+def bench_maps(A, args):
+    """MAPS C_l(dnu) (the maps program over cl_dnu) on the 768-channel 512^2
+    cube of the precision workload, 15 log bins of |U| in [1, 500): every
+    channel pair n1 <= n2 over every binned cell in FP64.  The host->device
+    copy of the FP64 file arrays is inside each call and reported apart from
+    the device time; value is channel-pair cells per second of device time."""
+    import oracle as O
+    cfg = CONFIGS["cfg3"]
+    N, nc, Du = cfg["N"], cfg["nc"], 1.0
+    gv, samp, _ = A.generate(N, nc, nflag=0, seed=cfg["seed"])
+    ctx = A.Context(device=0)
+    ctx.maps_cldnu(N, nc, Du, gv, samp)
+    ctx.reset_stats()
+    steps = max(args.steps, 1)
+    t = time.time()
+    for _ in range(steps):
+        ctx.maps_cldnu(N, nc, Du, gv, samp)
+    wall = (time.time() - t) / steps
+    st = ctx.stats()
+    ms_pair, ms_all = st["ms_maps"] / steps, st["ms_maps_total"] / steps
+    flop = st["flop_maps"] / steps
+    pair_cells = flop / 4.0
+    line = {"metric": "MAPS C_l(dnu): channel-pair cells per second", "value": round(pair_cells / (ms_all * 1e-3)),
+            "unit": "pair-cells/s", "steps": steps, "ms_per_step": round(ms_all, 3), "dtype": "f64",
+            "data": "synthetic", "config": {"workload": "maps 768 ch x 512^2, 15 log |U| bins 1-500"},
+            "roofline": {"bound": "fp64", "kernel": "k_maps_pairs", "achieved": round(flop / (ms_pair * 1e-3) / 1e12, 2),
+                         "peak": 78.6, "unit": "TFLOP/s", "frac": round(flop / (ms_pair * 1e-3) / 1e12 / 78.6, 4)},
+            "ms_pair_kernel": round(ms_pair, 3), "s_per_call_with_upload": round(wall, 3)}
+    ctx.close()
+    if not args.no_cpu_baseline:
+        ncs = 48
+        sub = np.ascontiguousarray(gv.reshape(-1, nc, 2)[:, :ncs]), np.ascontiguousarray(samp[:, :ncs])
+        t = time.time()
+        O.cldnu(N, ncs, Du, sub[0], sub[1])
+        dt = time.time() - t
+        line["cpu_baseline"] = {"value": round(pair_cells / (nc * (nc + 1) / 2) * ncs * (ncs + 1) / 2 / dt),
+                                "unit": "pair-cells/s", "cores": 1, "kind": "port",
+                                "sample": "the FP64 oracle's pair loop over the first %d channels of the same cube" % ncs}
     print(json.dumps(line), flush=True)
 
 

@@ -277,6 +277,29 @@ typedef struct amb_cyl_params {
 int amb_cyl_bispec(amb_ctx *ctx, int n, int W, const float *bl, const void *den, int den_is_int,
                    const amb_cyl_params *p, double *out);
 
+/* MAPS C_l(dnu): replaces the maps program's pair loop (4_fbe/maps.c:100-229)
+ * over cl_dnu (4_fbe/amber_funcs.c:1418-1488) for one cube given in the file
+ * layout (gv [N(N/2+1)][nc][2], samp [N(N/2+1)][nc], normalised inside as
+ * maps.c:100-125 does).  NB log bins of |U| = sqrt(ia^2+j^2) Du over
+ * [umin, umax) (the reference uses 15, 1, 500).  Outputs, as the reference
+ * leaves them before writing results/cldnu:
+ *   cldnu[NB][nc]  sum over n1 of the binned C(n1, n1 + d) / no_b(n1)
+ *   numdnu[nc]     number of channel pairs at lag d (nc - d)
+ *   binu[NB]       mean |U| of the bin's sampled cells of channel nc-1
+ *   num[NB]        number of those cells
+ * FP64 throughout; counts exact. */
+int amb_maps_cldnu(amb_ctx *ctx, int64_t N, int64_t nc, double Du, const double *gv, const double *samp,
+                   int NB, double umin, double umax, double *cldnu, double *numdnu, double *binu,
+                   int32_t *num);
+
+/* results/cldnu text (maps.c:213-222): NB*nc lines "b binu d cldnu numdnu num". */
+int amb_write_cldnu(const char *path, int NB, int64_t nc, const double *binu, const double *cldnu,
+                    const double *numdnu, const int32_t *num);
+/* input_params of the maps program (read_inputs, amber_funcs.c:159-172):
+ * NN Du nu0 dnu chan0 nc data_path; data_path gets at most cap-1 chars. */
+int amb_read_params(const char *path, int64_t *NN, double *Du, double *nu0, double *dnu, double *chan0,
+                    int64_t *nc, char *data_path, size_t cap);
+
 /* Drop every cached filtered plane of a cube (they are rebuilt on demand).
  * Planes stay resident across calls otherwise, as the serving loop wants. */
 int amb_drop_planes(amb_ctx *ctx, int cube_id);
@@ -300,6 +323,10 @@ typedef struct amb_stats {
                                   r1 == r2 tiles are skipped */
     double ms_graph;        /* device time of amb_graph_run replays */
     int64_t n_graph_launch;
+    double ms_maps;         /* MAPS pair-contraction kernel */
+    double ms_maps_total;   /* all MAPS kernels (normalise .. lag sums) */
+    double flop_maps;       /* algorithmic 4 flops per (channel pair n1<=n2, binned cell) */
+    int64_t n_maps_launch;
 } amb_stats;
 int amb_get_stats(amb_ctx *ctx, amb_stats *st);
 int amb_reset_stats(amb_ctx *ctx);

@@ -157,3 +157,26 @@ def refalgo(N, nc, Du, gv, samp, lo, hi, r1, r2, r3, mask, nc10, p1_lo=0, p1_hi=
                          cnt.ctypes.data_as(C.POINTER(C.c_int32)), nthreads, C.byref(tf), C.byref(tl))
     return (bl.reshape(W, W), None if numf is None else numf.reshape(W, W), cnt.reshape(W, W), n,
             tf.value, tl.value)
+
+
+_lib.orc_maps_bin.restype = C.c_int
+_lib.orc_maps_bin.argtypes = [C.c_long, C.c_double, C.c_long, C.c_long, C.c_int, C.c_double, C.c_double, _dp]
+_lib.orc_cldnu.argtypes = [C.c_long, C.c_long, C.c_double, _dp, _dp, C.c_int, C.c_double, C.c_double, _dp, _dp,
+                           _dp, C.POINTER(C.c_int32)]
+
+
+def maps_bin(N, Du, i, j, NB=15, umin=1.0, umax=500.0):
+    """(bin, |U|) of half-plane cell (i, j); bin -1 outside [umin, umax)"""
+    uu = C.c_double()
+    b = _lib.orc_maps_bin(N, Du, i, j, NB, umin, umax, C.byref(uu))
+    return b, uu.value
+
+
+def cldnu(N, nc, Du, gv, samp, NB=15, umin=1.0, umax=500.0):
+    """MAPS run: returns (cldnu[NB][nc], numdnu[nc], binu[NB], num[NB])"""
+    cl = np.zeros((NB, nc)); nd = np.zeros(nc); bu = np.zeros(NB); nm = np.zeros(NB, np.int32)
+    rc = _lib.orc_cldnu(N, nc, Du, _p(_f64(gv), _dp), _p(_f64(samp), _dp), NB, umin, umax, _p(cl, _dp),
+                        _p(nd, _dp), _p(bu, _dp), nm.ctypes.data_as(C.POINTER(C.c_int32)))
+    if rc != 0:
+        raise RuntimeError("orc_cldnu failed")
+    return cl, nd, bu, nm

@@ -24,6 +24,13 @@
  *                     closed-form triplet counts
  *   orc_refalgo_*     the reference-algorithm FP32 path; pinned against
  *                     orc_bispec (FP32-level agreement) and by identical counts
+ *   orc_maps_bin      hand-computed bins of chosen cells (3-4-5 radii,
+ *                     bin edges), the row fold ia = N - i
+ *   orc_cldnu         closed forms: a cube V(x,n) = c(x) e^{i phi n} gives
+ *                     cldnu[b][d] = (nc - d) cos(phi d) <|c|^2>_b, with
+ *                     channel masks sum_n1 sum_x [S1>0][S2>0]|c|^2 / no_b(n1);
+ *                     numdnu = nc - d; counts and mean |U| per bin by brute
+ *                     enumeration of the cells
  * No function is left "parity unpinned".
  */
 #ifndef AMBER_ORACLE_H
@@ -106,6 +113,18 @@ long orc_refalgo(long N, long nc, double Du, const double *re_im, const double *
                  const uint8_t *chan_mask, int nc10, int p1_lo, int p1_hi, int v2,
                  float *bl, float *numf, int32_t *cnt, int nthreads,
                  double *seconds_fill, double *seconds_loop);
+
+/* MAPS C_l(dnu) (4_fbe/maps.c:100-229 with cl_dnu, amber_funcs.c:1418-1488).
+ * Bin of half-plane cell (i, j): -1 outside [umin, umax); *uu_out gets |U|. */
+int orc_maps_bin(long N, double Du, long i, long j, int NB, double umin, double umax, double *uu_out);
+
+/* The whole MAPS run on a cube in the file layout (re_im [N(N/2+1)][nc][2],
+ * samp [N(N/2+1)][nc], normalised here as maps.c:100-125 does):
+ * cldnu[NB][nc] = sum over nn1 of binned C(nn1, nn1 + d), numdnu[nc] the
+ * number of pairs per lag d, binu[NB] / num[NB] the mean |U| and cell count
+ * of the last pair (nc-1, nc-1).  Returns 0, or -1 when out of memory. */
+int orc_cldnu(long N, long nc, double Du, const double *re_im, const double *samp, int NB, double umin,
+              double umax, double *cldnu, double *numdnu, double *binu, int32_t *num);
 
 #ifdef __cplusplus
 }
