@@ -15,25 +15,32 @@
  * inside a bin) and cuts every bin into chunks of at most MAPS_KCH cells.
  *
  * Kernels, in stream order:
- *   k_maps_norm   V /= NTiG and NTiG := 1 where NTiG > 0 (maps.c:100-125),
- *                 in place, FP64 division as written;
+ *   k_maps_prep   per (chunk, 64 channels): V /= NTiG where NTiG > 0
+ *                 (maps.c:100-125), in place, FP64 division as written, for
+ *                 the binned cells (the only ones read later); one ballot
+ *                 word of sampled cells per (cell, 64 channels), from which
+ *                 the A operand NTiG V is formed on load; counts the
+ *                 sampled cells per channel (exact); flags NTiG < 0 (a
+ *                 sampling count is never negative; the call then fails);
  *   k_maps_pairs  per (chunk, 128x128 channel tile with tile(n1) <=
- *                 tile(n2)) the partial Gram block over the chunk's cells:
- *                 8-cell stages through LDS (the next stage's loads in
- *                 flight during the products), 8x8 FP64 outputs per thread,
- *                 written to its own slice of the partial array (no atomics);
- *   k_maps_counts per (chunk, channel) the number of sampled cells (exact);
+ *                 tile(n2)) the partial Gram block over the chunk's cells on
+ *                 the FP64 matrix cores, 16-cell stages through LDS with the
+ *                 next stage's loads (and the stage after's cell index) in
+ *                 flight during the products; each block writes its own
+ *                 slice of the partial array (no atomics).  All tiles of a
+ *                 chunk run on one XCD, so a cell's channels come from that
+ *                 XCD's L2 after the first tile reads them.  The first
+ *                 blocks of the same launch compute, per bin and in cell
+ *                 order, no and usum of channel nc-1 (what the reference's
+ *                 last call leaves in binu/num) alongside;
  *   k_maps_sum    per bin, the chunk partials and counts added in chunk
  *                 order into the bin's first chunk;
- *   k_maps_last   per bin, in cell order, no and usum of the last pair's
- *                 channel nc-1, as the reference's last call leaves them in
- *                 binu/num (terms fetched in parallel, added by one thread);
- *   k_maps_lag    per (bin, lag d) the sum over n1 ascending of
- *                 C_b(n1, n1 + d) / no_b(n1), skipping the division where
- *                 no_b(n1) = 0 as the reference does.
+ *   k_maps_lag    per (bin, lag d) the sum over n1 of C_b(n1, n1 + d) /
+ *                 no_b(n1), skipping the division where no_b(n1) = 0 as the
+ *                 reference does; 8 fixed n1 segments added in order.
  * Every sum has a fixed order, so results are deterministic run to run; the
- * Gram sums differ from the reference's sequential cell order by FP64
- * rounding only.  Counts are exact.
+ * Gram and lag sums differ from the reference's sequential order by FP64
+ * rounding only.  Counts, binu and num are the reference's exactly.
  *
  * Bins are computed on the host with the reference's expression and libm
  * (sqrt(ia^2 + j^2) Du, floor(log10(uu/umin)/binsiz)), so the cell-to-bin
@@ -41,9 +48,8 @@
  * (possible only within an ulp of umax, an out-of-bounds write in the
  * reference) is counted in bin NB - 1.
  *
- * Roofline: the pair kernel does 4 FP64 flops per (pair, cell) on the
- * VALU, whose FP64 rate on gfx950 equals the FP64 MFMA rate (78.6 TFLOP/s),
- * and reads each cell's channels once per tile row and column from cache.
+ * Roofline: 4 FP64 flops per (pair n1 <= n2, binned cell); the FP64 MFMA
+ * peak of gfx950 is 78.6 TFLOP/s.
  */
 #include "amb_ctx.h"
 
@@ -52,140 +58,85 @@
 
 namespace {
 
-constexpr int MT = 128;         /* channel tile */
-constexpr int MKP = 8;          /* cells per LDS stage */
-constexpr int MAPS_KCH = 2048;  /* cells per chunk */
+constexpr int MT = 128;           /* channel tile */
+constexpr int MKC = 16;           /* cells per LDS stage */
+constexpr int MROW = MT + 16;     /* padded LDS row, doubles */
+constexpr int MAPS_KCH = 2048;    /* cells per chunk */
+constexpr int NXCD = 8;           /* XCDs; workgroup i runs on XCD i % 8 */
+typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 struct MapsChunk {
     int bin, start, count, pad;
 };
 
-__global__ void k_maps_norm(double2 *__restrict__ v, double *__restrict__ s, size_t n)
+/* per (64-channel slab, chunk): normalise the chunk's cells in place as
+ * maps.c:100-125 does (V /= NTiG where NTiG > 0, FP64 division as written),
+ * record which cells are sampled as one 64-bit ballot word per (cell, slab)
+ * (the A operand is NTiG V = V where the bit is set, else 0), count the
+ * sampled cells per channel (exact; 4 waves reduced in LDS) and flag a
+ * negative count.  The chunk's cell indices are staged in LDS first so the
+ * data loads of consecutive cells are independent.  Only binned cells are
+ * read afterwards, so only they are normalised; NTiG itself is left as
+ * read (readers test NTiG > 0). */
+__global__ __launch_bounds__(256) void k_maps_prep(double2 *__restrict__ v, const double *__restrict__ s,
+                                                   unsigned long long *__restrict__ mask, int nslab,
+                                                   const int *__restrict__ order,
+                                                   const MapsChunk *__restrict__ chunks, int nc, int *__restrict__ nop,
+                                                   unsigned *__restrict__ bad)
 {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    __shared__ int idx[MAPS_KCH];
+    __shared__ int cs[4][64];
+    const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int slab = blockIdx.x, n = slab * 64 + lane;
+    const MapsChunk ch = chunks[blockIdx.y];
+    for (int k = threadIdx.x; k < ch.count; k += 256) idx[k] = order[ch.start + k];
+    __syncthreads();
+    int cnt = 0;
+    const int nn = min(n, nc - 1);
+#pragma unroll 8
+    for (int k = g; k < ch.count; k += 4) {
+        const int x = idx[k];
+        const size_t i = (size_t)x * nc + nn;
         const double sv = s[i];
-        if (sv > 0) {
-            double2 x = v[i];
-            x.x /= sv;
-            x.y /= sv;
-            v[i] = x;
-            s[i] = 1.0;
+        const bool on = n < nc && sv > 0;
+        if (on) {
+            double2 t = v[i];
+            t.x /= sv;
+            t.y /= sv;
+            v[i] = t;
+            ++cnt;
         }
+        if (n < nc && sv < 0) atomicOr(bad, 1u);
+        const unsigned long long m = __ballot(on);
+        if (lane == 0) mask[(size_t)x * nslab + slab] = m;
     }
+    cs[g][lane] = cnt;
+    __syncthreads();
+    if (g == 0 && n < nc) nop[(size_t)blockIdx.y * nc + n] = cs[0][lane] + cs[1][lane] + cs[2][lane] + cs[3][lane];
 }
 
-__global__ __launch_bounds__(256) void k_maps_pairs(const double2 *__restrict__ v, const double *__restrict__ s,
-                                                    const int *__restrict__ order,
-                                                    const MapsChunk *__restrict__ chunks, int nc, int ntile,
-                                                    double *__restrict__ part)
-{
-    __shared__ double2 As[MKP][MT], Bs[MKP][MT];
-    int p = blockIdx.x, ti = 0;
-    while (p >= ntile - ti) {
-        p -= ntile - ti;
-        ++ti;
-    }
-    const int tj = ti + p;
-    const MapsChunk ch = chunks[blockIdx.y];
-    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-    const int lp = tid >> 5, lc = (tid & 31) * 4; /* loader: cell lp, channels lc..lc+3 */
-    const int r0 = ti * MT, c0 = tj * MT;
-    double acc[8][8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i)
-#pragma unroll
-        for (int j = 0; j < 8; ++j) acc[i][j] = 0.0;
+struct MapsSmem {
+    double A[2 * MKC][MROW], B[2 * MKC][MROW];
+};
 
-    /* stage k0's operands into registers: A = NTiG V, B = V, zero past the
-     * chunk or the channel count */
-    double2 a[4], b[4];
-    auto fetch = [&](int k0) {
-        if (k0 + lp < ch.count) {
-            const size_t x = (size_t)order[ch.start + k0 + lp] * nc;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int n1 = r0 + lc + q, n2 = c0 + lc + q;
-                if (n1 < nc) {
-                    const double2 t = v[x + n1];
-                    const double w = s[x + n1];
-                    a[q] = make_double2(t.x * w, t.y * w);
-                } else {
-                    a[q] = make_double2(0.0, 0.0);
-                }
-                b[q] = n2 < nc ? v[x + n2] : make_double2(0.0, 0.0);
-            }
-        } else {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) a[q] = b[q] = make_double2(0.0, 0.0);
-        }
-    };
-    fetch(0);
-    for (int k0 = 0; k0 < ch.count; k0 += MKP) {
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            As[lp][lc + q] = a[q];
-            Bs[lp][lc + q] = b[q];
-        }
-        __syncthreads();
-        if (k0 + MKP < ch.count) fetch(k0 + MKP); /* in flight during this stage's products */
-#pragma unroll
-        for (int kk = 0; kk < MKP; ++kk) {
-            double2 ra[8], rb[8];
-#pragma unroll
-            for (int i = 0; i < 8; ++i) ra[i] = As[kk][ty + 16 * i];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) rb[j] = Bs[kk][tx + 16 * j];
-#pragma unroll
-            for (int i = 0; i < 8; ++i)
-#pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    acc[i][j] = fma(ra[i].x, rb[j].x, acc[i][j]);
-                    acc[i][j] = fma(ra[i].y, rb[j].y, acc[i][j]);
-                }
-        }
-    }
-    double *out = part + (size_t)blockIdx.y * nc * nc;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int n1 = r0 + ty + 16 * i;
-        if (n1 >= nc) continue;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int n2 = c0 + tx + 16 * j;
-            if (n2 < nc) out[(size_t)n1 * nc + n2] = acc[i][j];
-        }
-    }
-}
-
-__global__ void k_maps_counts(const double *__restrict__ s, const int *__restrict__ order,
-                              const MapsChunk *__restrict__ chunks, int nc, int *__restrict__ nop)
-{
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= nc) return;
-    const MapsChunk ch = chunks[blockIdx.y];
-    int no = 0;
-    for (int k = 0; k < ch.count; ++k) no = (int)(no + s[(size_t)order[ch.start + k] * nc + n]);
-    nop[(size_t)blockIdx.y * nc + n] = no;
-}
-
-/* one block per bin: 256 cells' terms are fetched in parallel into LDS and
- * thread 0 adds them in cell order, the reference's sequential order and
- * expressions (no += NTiG as int, usum += uu * NTiG) */
-__global__ __launch_bounds__(256) void k_maps_last(const double *__restrict__ s, const int *__restrict__ order,
-                                                   const double *__restrict__ uu, const int *__restrict__ bstart,
-                                                   int nc, int *__restrict__ no_last, double *__restrict__ us_last)
+/* no and usum of channel nc-1 over bin b in cell order, the reference's
+ * expressions (no += NTiG as int, usum += uu * NTiG): 256 terms at a time
+ * fetched in parallel into LDS, added by one thread */
+__device__ void maps_last(MapsSmem &L, const double *__restrict__ s, const int *__restrict__ order,
+                          const double *__restrict__ uu, const int *__restrict__ bstart, int nc, int b,
+                          int *__restrict__ no_last, double *__restrict__ us_last)
 {
 #pragma clang fp contract(off)
-    __shared__ double wv[256], pv[256];
-    const int b = blockIdx.x, t = threadIdx.x;
+    double *wv = &L.A[0][0], *pv = &L.B[0][0];
+    const int t = threadIdx.x;
     const int k0 = bstart[b], k1 = bstart[b + 1];
     int no = 0;
     double us = 0.0;
     for (int base = k0; base < k1; base += 256) {
         const int m = min(256, k1 - base);
         if (t < m) {
-            const double w = s[(size_t)order[base + t] * nc + (nc - 1)];
+            const double r = s[(size_t)order[base + t] * nc + (nc - 1)];
+            const double w = r > 0 ? 1.0 : r; /* NTiG after normalisation */
             wv[t] = w;
             pv[t] = uu[base + t] * w;
         }
@@ -201,6 +152,137 @@ __global__ __launch_bounds__(256) void k_maps_last(const double *__restrict__ s,
         no_last[b] = no;
         us_last[b] = us;
     }
+}
+
+/* Wave w owns the 64x64 quarter (w>>1, w&1) of the 128x128 tile as 4x4
+ * blocks of v_mfma_f64_16x16x4f64: A[m][k] = lane m + 16k, B[k][n] = lane
+ * n + 16k, and the four results of lane l are rows 4r + l/16 (r = 0..3) of
+ * column l%16 (rows interleaved across the lane groups, unlike the
+ * contiguous rows of the FP32 16x16x4 form).  The K dimension is (cell,
+ * re/im): LDS holds 16 cells as 32 rows of 128 channels per operand (row 2c
+ * real parts of cell c, row 2c+1 imaginary parts), rows padded by 16
+ * doubles so the two half-waves of a read hit disjoint banks. */
+__device__ void maps_tile(MapsSmem &L, const double2 *__restrict__ v, const unsigned long long *__restrict__ mask,
+                          int nslab, const int *__restrict__ order, const MapsChunk ch, int chunk, int ti, int tj,
+                          int nc, double *__restrict__ part)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wr = (wv >> 1) * 64, wc = (wv & 1) * 64;
+    const int lc = tid >> 4, lch = (tid & 15) * 8; /* loader: cell lc, channels lch..lch+7 */
+    const int r0 = ti * MT, c0 = tj * MT;
+    f64x4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = (f64x4){0.0, 0.0, 0.0, 0.0};
+
+    /* raw operands of one cell, loaded without branches (indices clamped
+     * into the cube, out-of-range values zeroed at the LDS store) so every
+     * load of a stage is in flight at once and none is waited for before
+     * the next stage's store */
+    double2 tv[8], tb[8];
+    unsigned long long tm = 0; /* sample bits of the A channels (bit lch % 64 + q) */
+    bool okc = false;
+    /* k: the cell's place in the chunk; idx: its cube cell (any valid cell
+     * when k is past the chunk) */
+    auto fetch = [&](int k, int idx) {
+        okc = k < ch.count;
+        const size_t xb = (size_t)idx * nc;
+        tm = mask[(size_t)idx * nslab + min(r0 + lch, nc - 1) / 64];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int n1 = min(r0 + lch + q, nc - 1), n2 = min(c0 + lch + q, nc - 1);
+            tv[q] = v[xb + n1];
+            tb[q] = v[xb + n2];
+        }
+    };
+    /* the index load is not waited for until the fetch that uses it */
+    auto cell = [&](int k) { return order[ch.start + min(k, ch.count - 1)]; };
+    fetch(lc, cell(lc));
+    int xn = cell(MKC + lc);
+    const int fm = lane & 15, fk = lane >> 4;
+    for (int k0 = 0; k0 < ch.count; k0 += MKC) {
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 8; q += 2) {
+            double2 a0, a1, b0, b1;
+            const unsigned bits = (unsigned)(tm >> (lch & 63));
+            const bool o1a = okc && r0 + lch + q < nc && (bits >> q & 1u);
+            const bool o1b = okc && r0 + lch + q + 1 < nc && (bits >> (q + 1) & 1u);
+            const bool o2a = okc && c0 + lch + q < nc, o2b = okc && c0 + lch + q + 1 < nc;
+            a0 = o1a ? tv[q] : make_double2(0.0, 0.0);
+            a1 = o1b ? tv[q + 1] : make_double2(0.0, 0.0);
+            b0 = o2a ? tb[q] : make_double2(0.0, 0.0);
+            b1 = o2b ? tb[q + 1] : make_double2(0.0, 0.0);
+            *(double2 *)&L.A[2 * lc][lch + q] = make_double2(a0.x, a1.x);
+            *(double2 *)&L.A[2 * lc + 1][lch + q] = make_double2(a0.y, a1.y);
+            *(double2 *)&L.B[2 * lc][lch + q] = make_double2(b0.x, b1.x);
+            *(double2 *)&L.B[2 * lc + 1][lch + q] = make_double2(b0.y, b1.y);
+        }
+        __syncthreads();
+        if (k0 + MKC < ch.count) { /* in flight during this stage's products */
+            fetch(k0 + MKC + lc, xn);
+            xn = cell(k0 + 2 * MKC + lc);
+        }
+#pragma unroll
+        for (int ks = 0; ks < 2 * MKC / 4; ++ks) {
+            const int row = 4 * ks + fk;
+            double fa[4], fb[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) fa[i] = L.A[row][wr + 16 * i + fm];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) fb[j] = L.B[row][wc + 16 * j + fm];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[i], fb[j], acc[i][j], 0, 0, 0);
+        }
+    }
+    double *out = part + (size_t)chunk * nc * nc;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int n1 = r0 + wr + 16 * i + 4 * r + fk;
+            if (n1 >= nc) continue;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int n2 = c0 + wc + 16 * j + fm;
+                if (n2 < nc) out[(size_t)n1 * nc + n2] = acc[i][j][r];
+            }
+        }
+}
+
+/* grid: nlast blocks (a multiple of 8) for the per-bin last-channel sums,
+ * then 8 * ceil(nchunk / 8) * npair tile blocks; tile block w runs on XCD
+ * w % 8 and takes chunk w % 8 + 8 * (slot / npair), pair slot % npair,
+ * slot = w / 8 */
+__global__ __launch_bounds__(256, 2) void k_maps_pairs(const double2 *__restrict__ v,
+                                                       const unsigned long long *__restrict__ mask, int nslab,
+                                                       const double *__restrict__ s, const int *__restrict__ order,
+                                                       const MapsChunk *__restrict__ chunks, int nchunk, int nc,
+                                                       int ntile, int nlast, double *__restrict__ part,
+                                                       const double *__restrict__ uu,
+                                                       const int *__restrict__ bstart, int NB,
+                                                       int *__restrict__ no_last, double *__restrict__ us_last)
+{
+    __shared__ MapsSmem L;
+    const int id = blockIdx.x;
+    if (id < nlast) {
+        if (id < NB) maps_last(L, s, order, uu, bstart, nc, id, no_last, us_last);
+        return;
+    }
+    const int w = id - nlast, xcd = w % NXCD, slot = w / NXCD;
+    const int npair = ntile * (ntile + 1) / 2;
+    const int chunk = xcd + NXCD * (slot / npair);
+    if (chunk >= nchunk) return;
+    int p = slot % npair, ti = 0;
+    while (p >= ntile - ti) {
+        p -= ntile - ti;
+        ++ti;
+    }
+    maps_tile(L, v, mask, nslab, order, chunks[chunk], chunk, ti, ti + p, nc, part);
 }
 
 /* fold a bin's chunk partials into its first chunk, chunk order kept:
@@ -226,25 +308,37 @@ __global__ void k_maps_sum(double *__restrict__ part, int *__restrict__ nop, con
     }
 }
 
-__global__ void k_maps_lag(const double *__restrict__ part, const int *__restrict__ nop,
-                           const int *__restrict__ bchunk, int nc, double *__restrict__ cl)
+/* block (32 lags, bin): thread (lag dl, segment g) sums n1 in
+ * [g S, (g+1) S) sequentially, S = ceil(nc / 8); the 8 partials are then
+ * added in segment order */
+__global__ __launch_bounds__(256) void k_maps_lag(const double *__restrict__ part, const int *__restrict__ nop,
+                                                  const int *__restrict__ bchunk, int nc, double *__restrict__ cl)
 {
-    const int d = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;
-    if (d >= nc) return;
+    __shared__ double ps[8][32];
+    const int dl = threadIdx.x & 31, g = threadIdx.x >> 5;
+    const int d = blockIdx.x * 32 + dl, b = blockIdx.y;
     const int c0 = bchunk[b], c1 = bchunk[b + 1];
+    const int S = (nc + 7) / 8;
     double acc = 0.0;
-    if (c1 > c0) {
+    if (d < nc && c1 > c0) {
         const double *P = part + (size_t)c0 * nc * nc;
         const int *no = nop + (size_t)c0 * nc;
+        const int e = min((g + 1) * S, nc - d);
 #pragma unroll 8
-        for (int n1 = 0; n1 + d < nc; ++n1) {
+        for (int n1 = g * S; n1 < e; ++n1) {
             double v = P[(size_t)n1 * nc + n1 + d];
             const int k = no[n1];
             if (k != 0) v /= (1. * k);
             acc += v;
         }
     }
-    cl[(size_t)b * nc + d] = acc;
+    ps[g][dl] = acc;
+    __syncthreads();
+    if (g == 0 && d < nc) {
+        double t = ps[0][dl];
+        for (int i = 1; i < 8; ++i) t += ps[i][dl];
+        cl[(size_t)b * nc + d] = t;
+    }
 }
 
 } // namespace
@@ -258,7 +352,7 @@ extern "C" int amb_maps_cldnu(amb_ctx *c, int64_t N, int64_t nc, double Du, cons
     if (!c || !gv || !samp || !cldnu || !numdnu || !binu || !num) return AMB_EARG;
     std::lock_guard<std::recursive_mutex> lk(c->mu);
     hipSetDevice(c->device);
-    if (N < 2 || nc < 1 || nc > (1 << 20) || !(Du > 0) || NB < 1 || !(umin > 0) || !(umax > umin))
+    if (N < 2 || nc < 1 || nc > (1 << 20) || !(Du > 0) || NB < 1 || NB > 65535 || !(umin > 0) || !(umax > umin))
         return fail(c, AMB_EARG, "MAPS: bad arguments N=%lld nc=%lld NB=%d umin=%g umax=%g", (long long)N,
                     (long long)nc, NB, umin, umax);
     const int64_t Nh = N / 2 + 1, npix = N * Nh;
@@ -306,13 +400,16 @@ extern "C" int amb_maps_cldnu(amb_ctx *c, int64_t N, int64_t nc, double Du, cons
     }
     const int nchunk = (int)chunks.size();
     const int ntile = (int)((nc + MT - 1) / MT), npair = ntile * (ntile + 1) / 2;
-    if (nchunk > 65535) return fail(c, AMB_EARG, "MAPS: %d cell chunks exceed the launch grid", nchunk);
+    if ((double)(nchunk + NXCD) * npair + NB + NXCD > 2e9)
+        return fail(c, AMB_EARG, "MAPS: %d cell chunks x %d tile pairs exceed the launch grid", nchunk, npair);
 
     /* one device block: cube, sampling, tables, partials, results */
     const size_t nv = (size_t)npix * nc;
     size_t off = 0;
     auto place = [&](size_t bytes) { size_t o = off; off += (bytes + 255) / 256 * 256; return o; };
     const size_t o_v = place(sizeof(double2) * nv), o_s = place(sizeof(double) * nv);
+    const int nslab = (int)((nc + 63) / 64);
+    const size_t o_mk = place(sizeof(unsigned long long) * (size_t)npix * nslab);
     const size_t o_ord = place(sizeof(int) * order.size()), o_uu = place(sizeof(double) * ouu.size());
     const size_t o_ch = place(sizeof(MapsChunk) * std::max(nchunk, 1));
     const size_t o_bs = place(sizeof(int) * (NB + 1)), o_bc = place(sizeof(int) * (NB + 1));
@@ -320,16 +417,19 @@ extern "C" int amb_maps_cldnu(amb_ctx *c, int64_t N, int64_t nc, double Du, cons
     const size_t o_nop = place(sizeof(int) * (size_t)std::max(nchunk, 1) * nc);
     const size_t o_cl = place(sizeof(double) * NB * nc);
     const size_t o_nl = place(sizeof(int) * NB), o_ul = place(sizeof(double) * NB);
+    const size_t o_bad = place(sizeof(unsigned));
     void *base = nullptr;
     int rc = dev_alloc(c, &base, off, "MAPS cube and partials");
     if (rc) return rc;
     char *d = (char *)base;
     double2 *d_v = (double2 *)(d + o_v);
+    unsigned long long *d_mk = (unsigned long long *)(d + o_mk);
     double *d_s = (double *)(d + o_s), *d_uu = (double *)(d + o_uu), *d_part = (double *)(d + o_part);
     double *d_cl = (double *)(d + o_cl), *d_ul = (double *)(d + o_ul);
     int *d_ord = (int *)(d + o_ord), *d_bs = (int *)(d + o_bs), *d_bc = (int *)(d + o_bc);
     int *d_nop = (int *)(d + o_nop), *d_nl = (int *)(d + o_nl);
     MapsChunk *d_ch = (MapsChunk *)(d + o_ch);
+    unsigned *d_bad = (unsigned *)(d + o_bad);
 
     std::vector<double> cl_h((size_t)NB * nc), ul(NB);
     std::vector<int> nl(NB);
@@ -344,33 +444,36 @@ extern "C" int amb_maps_cldnu(amb_ctx *c, int64_t N, int64_t nc, double Du, cons
         e = hipMemcpyAsync(d_ch, chunks.data(), sizeof(MapsChunk) * nchunk, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(d_bs, bstart.data(), sizeof(int) * (NB + 1), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(d_bc, bchunk.data(), sizeof(int) * (NB + 1), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = amb_launch_fill_u32(st, d_bad, 1, 0u);
     if (e == hipSuccess) {
         timer_start(c, &c->st.ms_maps_total);
-        const int nb_norm = (int)std::min<size_t>((nv + 255) / 256, 8192);
-        hipLaunchKernelGGL(k_maps_norm, dim3(nb_norm), dim3(256), 0, st, d_v, d_s, nv);
-        if (nchunk) {
-            timer_start(c, &c->st.ms_maps);
-            hipLaunchKernelGGL(k_maps_pairs, dim3(npair, nchunk), dim3(256), 0, st, d_v, d_s, d_ord, d_ch, (int)nc,
-                               ntile, d_part);
-            timer_stop(c);
-            hipLaunchKernelGGL(k_maps_counts, dim3((unsigned)((nc + 255) / 256), nchunk), dim3(256), 0, st, d_s,
-                               d_ord, d_ch, (int)nc, d_nop);
+        if (nchunk)
+            hipLaunchKernelGGL(k_maps_prep, dim3((unsigned)nslab, nchunk), dim3(256), 0, st, d_v, d_s, d_mk, nslab,
+                               d_ord, d_ch, (int)nc, d_nop, d_bad);
+        const int nlast = (NB + NXCD - 1) / NXCD * NXCD;
+        const long ntb = (long)(nchunk + NXCD - 1) / NXCD * NXCD * npair;
+        timer_start(c, &c->st.ms_maps);
+        hipLaunchKernelGGL(k_maps_pairs, dim3((unsigned)(nlast + ntb)), dim3(256), 0, st, d_v, d_mk, nslab, d_s,
+                           d_ord, d_ch, nchunk, (int)nc, ntile, nlast, d_part, d_uu, d_bs, NB, d_nl, d_ul);
+        timer_stop(c);
+        if (nchunk)
             hipLaunchKernelGGL(k_maps_sum, dim3((unsigned)(((size_t)nc * nc + 255) / 256), NB), dim3(256), 0, st,
                                d_part, d_nop, d_bc, (int)nc);
-        }
-        hipLaunchKernelGGL(k_maps_last, dim3(NB), dim3(256), 0, st, d_s, d_ord, d_uu, d_bs, (int)nc, d_nl, d_ul);
-        hipLaunchKernelGGL(k_maps_lag, dim3((unsigned)((nc + 255) / 256), NB), dim3(256), 0, st, d_part, d_nop,
-                           d_bc, (int)nc, d_cl);
+        hipLaunchKernelGGL(k_maps_lag, dim3((unsigned)((nc + 31) / 32), NB), dim3(256), 0, st, d_part, d_nop, d_bc,
+                           (int)nc, d_cl);
         timer_stop(c);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(cl_h.data(), d_cl, sizeof(double) * NB * nc, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipMemcpyAsync(nl.data(), d_nl, sizeof(int) * NB, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipMemcpyAsync(ul.data(), d_ul, sizeof(double) * NB, hipMemcpyDeviceToHost, st);
+    unsigned bad = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&bad, d_bad, sizeof(unsigned), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     timers_collect(c);
     dev_free(c, base, off);
     if (e != hipSuccess) return fail(c, AMB_EHIP, "MAPS: %s", hipGetErrorString(e));
+    if (bad) return fail(c, AMB_EARG, "MAPS: negative sampling count in the cube");
     c->st.flop_maps += 4.0 * ((double)nc * (nc + 1) / 2) * ncell;
     c->st.n_maps_launch += nchunk ? 1 : 0;
     for (size_t k = 0; k < cl_h.size(); ++k) cldnu[k] = cl_h[k];

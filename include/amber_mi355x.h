@@ -287,7 +287,8 @@ int amb_cyl_bispec(amb_ctx *ctx, int n, int W, const float *bl, const void *den,
  *   numdnu[nc]     number of channel pairs at lag d (nc - d)
  *   binu[NB]       mean |U| of the bin's sampled cells of channel nc-1
  *   num[NB]        number of those cells
- * FP64 throughout; counts exact. */
+ * FP64 throughout; counts exact.  Sampling counts must be >= 0 (AMB_EARG
+ * otherwise; the gridder never writes a negative count). */
 int amb_maps_cldnu(amb_ctx *ctx, int64_t N, int64_t nc, double Du, const double *gv, const double *samp,
                    int NB, double umin, double umax, double *cldnu, double *numdnu, double *binu,
                    int32_t *num);

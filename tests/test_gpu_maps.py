@@ -85,6 +85,11 @@ def test_maps_edges():
         samp[:, 2] = 0                                          # an unsampled channel
         check(ctx, 32, 6, 9.0, gv, samp)
         check(ctx, 32, 6, 9.0, gv, samp, NB=1)
+        samp[5, 1] = -1.0                                       # a sampling count is never negative
+        with pytest.raises(A.AmberError):
+            ctx.maps_cldnu(32, 6, 9.0, gv, samp)
+        samp[5, 1] = 1.0
+        check(ctx, 32, 6, 9.0, gv, samp)                        # the context still works
 
 
 def test_maps_deterministic():
