@@ -35,7 +35,8 @@ EXPORTS = [
     "amb_get_stats", "amb_reset_stats", "amb_drop_planes", "amb_batch_times", "amb_ring_image",
     "amb_bispec_multi", "amb_set_batch_mode", "amb_update_cube", "amb_graph_create", "amb_graph_run",
     "amb_graph_destroy", "amb_serve_start", "amb_submit", "amb_poll", "amb_wait", "amb_serve_stop",
-    "amb_cyl_bispec", "amb_maps_cldnu", "amb_write_cldnu", "amb_read_params",
+    "amb_cyl_bispec", "amb_maps_cldnu", "amb_write_cldnu", "amb_read_params", "amb_write_bsw", "amb_read_bsw",
+    "amb_read_bsout", "amb_crc32",
 ]
 
 BATCH_PER_TRIANGLE, BATCH_GROUPED = 0, 1
@@ -136,6 +137,11 @@ _lib.amb_maps_cldnu.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_double, _d
 _lib.amb_write_cldnu.argtypes = [C.c_char_p, C.c_int, C.c_int64, _dp, _dp, _dp, _ip]
 _lib.amb_read_params.argtypes = [C.c_char_p, C.POINTER(C.c_int64), _dp, _dp, _dp, _dp, C.POINTER(C.c_int64),
                                  C.c_char_p, C.c_size_t]
+_lib.amb_write_bsw.argtypes = [C.c_char_p, C.c_void_p, _fp, C.c_void_p]
+_lib.amb_read_bsw.argtypes = [C.c_char_p, C.c_void_p, _fp, C.c_void_p, C.c_int64]
+_lib.amb_read_bsout.argtypes = [C.c_char_p, _ip, _ip, _dp, _dp, _fp, C.c_void_p, C.c_int64]
+_lib.amb_crc32.restype = C.c_uint32
+_lib.amb_crc32.argtypes = [C.c_void_p, C.c_size_t]
 _lib.amb_get_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
 _lib.amb_reset_stats.argtypes = [C.c_void_p]
 _lib.amb_uavg.argtypes = [_dp, _dp, C.c_int, _dp]
@@ -287,6 +293,65 @@ def write_bsout_v2(res_path, r1, r2, r3, rlzn, inv, bl, norm):
     _chk(_lib.amb_write_bsout_v2(res_path.encode(), r1, r2, r3, rlzn, W, inv,
                                  _p(np.ascontiguousarray(bl, np.float32), _fp),
                                  _p(np.ascontiguousarray(norm, np.float32), _fp)))
+
+
+class BswHeader(C.Structure):
+    """amb_bsw_header: the 104-byte header of a hardened result file"""
+    _fields_ = [("magic", C.c_char * 8), ("version", C.c_uint32), ("header_bytes", C.c_uint32),
+                ("kind", C.c_uint32), ("endian", C.c_uint32), ("nc10", C.c_int32), ("W", C.c_int32),
+                ("r1", C.c_int32), ("r2", C.c_int32), ("r3", C.c_int32), ("rlzn", C.c_int32),
+                ("ng", C.c_int64), ("nc", C.c_int64), ("precision", C.c_int32), ("payload_crc", C.c_uint32),
+                ("h0", C.c_double), ("h1", C.c_double), ("payload_bytes", C.c_uint64),
+                ("reserved", C.c_uint8 * 8)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k not in ("magic", "reserved")}
+
+
+assert C.sizeof(BswHeader) == 104
+BSW_KIND_V1, BSW_KIND_V2 = 1, 2
+
+
+def crc32(data):
+    b = bytes(data)
+    return int(_lib.amb_crc32(b, len(b)))
+
+
+def write_bsw(res_path, kind, r1, r2, r3, rlzn, ng, nc, precision, h0, h1, bl, den):
+    """hardened result file bsw<kind>_ra.._R<rlzn>; den: int32 counts (kind 1) or float32 norm (kind 2)"""
+    bl = np.ascontiguousarray(bl, np.float32)
+    W = int(round(np.sqrt(bl.size)))
+    den = np.ascontiguousarray(den, np.int32 if kind == BSW_KIND_V1 else np.float32)
+    h = BswHeader()
+    h.kind, h.nc10, h.W = kind, (W - 1) // 2, W
+    h.r1, h.r2, h.r3, h.rlzn, h.ng, h.nc = r1, r2, r3, rlzn, ng, nc
+    h.precision = _PREC[precision] if isinstance(precision, str) else int(precision)
+    h.h0, h.h1 = h0, h1
+    _chk(_lib.amb_write_bsw(res_path.encode(), C.byref(h), _p(bl, _fp), den.ctypes.data_as(C.c_void_p)))
+
+
+def read_bsw(path):
+    """(header dict, bl[W][W] float32, den[W][W] int32 or float32) of a hardened file"""
+    h = BswHeader()
+    _chk(_lib.amb_read_bsw(path.encode(), C.byref(h), None, None, 0))
+    W = h.W
+    bl = np.empty((W, W), np.float32)
+    den = np.empty((W, W), np.int32 if h.kind == BSW_KIND_V1 else np.float32)
+    _chk(_lib.amb_read_bsw(path.encode(), C.byref(h), _p(bl, _fp), den.ctypes.data_as(C.c_void_p), W * W))
+    return h.as_dict(), bl, den
+
+
+def read_bsout(path):
+    """a legacy bsout file of either kind, W from its size:
+    (kind, h0, h1, bl[W][W], den[W][W]) as the reference writes them"""
+    k = C.c_int(); W = C.c_int(); h0 = C.c_double(); h1 = C.c_double()
+    _chk(_lib.amb_read_bsout(path.encode(), C.byref(k), C.byref(W), C.byref(h0), C.byref(h1), None, None, 0))
+    n = W.value
+    bl = np.empty((n, n), np.float32)
+    den = np.empty((n, n), np.int32 if k.value == 1 else np.float32)
+    _chk(_lib.amb_read_bsout(path.encode(), C.byref(k), C.byref(W), C.byref(h0), C.byref(h1), _p(bl, _fp),
+                             den.ctypes.data_as(C.c_void_p), n * n))
+    return k.value, h0.value, h1.value, bl, den
 
 
 # ---------------------------------------------------------------- device context

@@ -12,6 +12,9 @@
  * those defaults, except the first triangle, which defaults to 0 (a full
  * run); pass --rp-start 365 (v1) or 635 (v2) to reproduce the reference's
  * partial runs.  --rings FILE replaces the built-in piecewise edges.
+ * --wire hardened (or both) also writes each triangle as a self-describing
+ * bsw1_/bsw2_ file (versioned header with nc10, Ng, precision, triangle id
+ * and a CRC-32 of the payload); the legacy files stay the default.
  */
 #include "amber_mi355x.h"
 #include <stdio.h>
@@ -39,7 +42,7 @@ static void usage(void)
     fprintf(stderr,
             "Usage: %s <Rlzn> [--params FILE] [--rings FILE] [--nr N] [--nc10 N] [--uf-nc N]\n"
             "          [--rp-start N] [--rp-end N] [--precision fp32|bf16x3] [--device N]\n"
-            "          [--calcr-plane N] [--res-path DIR]\n",
+            "          [--calcr-plane N] [--res-path DIR] [--wire legacy|hardened|both]\n",
             PROG);
 }
 
@@ -82,6 +85,7 @@ int main(int argc, char **argv)
     const char *params = "input_params_ess", *rings_file = NULL, *res_path = RES_PATH;
     int nr = 22, nc10 = 383, uf_nc = 552, rp_start = 0, rp_end = -1, device = 0, calcr_plane = 5;
     int prec = AMB_PREC_BF16X3;
+    int wire_legacy = 1, wire_hardened = 0; /* --wire: which result files to write */
     for (int i = 2; i < argc; ++i) {
         const char *a = argv[i];
         const char *v = i + 1 < argc ? argv[i + 1] : NULL;
@@ -99,6 +103,11 @@ int main(int argc, char **argv)
         else if (!strcmp(a, "--precision")) {
             if (!strcmp(v, "fp32")) prec = AMB_PREC_FP32;
             else if (!strcmp(v, "bf16x3")) prec = AMB_PREC_BF16X3;
+            else { usage(); return 1; }
+        } else if (!strcmp(a, "--wire")) {
+            if (!strcmp(v, "legacy")) { wire_legacy = 1; wire_hardened = 0; }
+            else if (!strcmp(v, "hardened")) { wire_legacy = 0; wire_hardened = 1; }
+            else if (!strcmp(v, "both")) { wire_legacy = 1; wire_hardened = 1; }
             else { usage(); return 1; }
         } else { usage(); return 1; }
         ++i;
@@ -202,10 +211,19 @@ int main(int argc, char **argv)
             double gfact_sq = gfact * gfact;
             double Num = gfact_sq * rdata[rp * 4 + 3];
             double inv = 1.0 / (Ng1 * Ng2 * 1.);
-            CHECK(amb_write_bsout_v1(res_path, r1, r2, r3, Rlzn, W, Num * inv, Num, bl + ncsize * k, cnt + ncsize * k));
+            if (wire_legacy)
+                CHECK(amb_write_bsout_v1(res_path, r1, r2, r3, Rlzn, W, Num * inv, Num, bl + ncsize * k,
+                                         cnt + ncsize * k));
+            amb_bsw_header h = {{0}, 0, 0, AMB_BSW_KIND_V1, 0, nc10, W, r1, r2, r3, Rlzn, Ng1, nc, prec, 0,
+                                Num * inv, Num, 0, {0}};
+            if (wire_hardened) CHECK(amb_write_bsw(res_path, &h, bl + ncsize * k, cnt + ncsize * k));
 #else
             double inv = 1.0 / (Ng1 * Ng2 * 1.);
-            CHECK(amb_write_bsout_v2(res_path, r1, r2, r3, Rlzn, W, inv, bl + ncsize * k, norm + ncsize * k));
+            if (wire_legacy)
+                CHECK(amb_write_bsout_v2(res_path, r1, r2, r3, Rlzn, W, inv, bl + ncsize * k, norm + ncsize * k));
+            amb_bsw_header h = {{0}, 0, 0, AMB_BSW_KIND_V2, 0, nc10, W, r1, r2, r3, Rlzn, Ng1, nc, prec, 0,
+                                inv, 0.0, 0, {0}};
+            if (wire_hardened) CHECK(amb_write_bsw(res_path, &h, bl + ncsize * k, norm + ncsize * k));
 #endif
             printf("rp=%d (r1=%d r2=%d r3=%d, Ng=%ld)\n", rp, r1, r2, r3, Ng1);
         }

@@ -86,6 +86,45 @@ int amb_write_bsout_v1(const char *res_path, int r1, int r2, int r3, int rlzn, i
 int amb_write_bsout_v2(const char *res_path, int r1, int r2, int r3, int rlzn, int W,
                        double inv_ng2, const float *bl, const float *norm);
 
+/* Hardened result files (no counterpart in the reference, whose files carry
+ * no dimensions or version tag, README.md:120).  A fixed 104-byte header,
+ * then the legacy payload unchanged: f32 bl[W*W], then i32 counts (kind 1)
+ * or f32 norm (kind 2).  Written as <res_path>/bsw1_ra%d_rb%d_rc%d_R%d
+ * (kind 1) or bsw2_... (kind 2), a prefix the legacy readers' bsout_final*
+ * patterns do not match.  The writer fills magic, version, header_bytes,
+ * endian, payload_bytes and payload_crc; the reader checks every one of
+ * them, W = 2 nc10 + 1 and the file size, and returns AMB_EFORMAT on any
+ * mismatch (a truncated, corrupted or foreign-endian file). */
+#define AMB_BSW_VERSION 1
+#define AMB_BSW_KIND_V1 1
+#define AMB_BSW_KIND_V2 2
+typedef struct amb_bsw_header {
+    char magic[8];          /* "AMBRBSW" + NUL */
+    uint32_t version;       /* AMB_BSW_VERSION */
+    uint32_t header_bytes;  /* 104 */
+    uint32_t kind;          /* AMB_BSW_KIND_V1 / _V2 */
+    uint32_t endian;        /* 0x01020304 in the writer's byte order */
+    int32_t nc10, W;        /* W = 2 nc10 + 1 */
+    int32_t r1, r2, r3, rlzn;
+    int64_t ng;             /* reduced grid side of the triangle (amber.c:193-197) */
+    int64_t nc;             /* channels of the cube */
+    int32_t precision;      /* AMB_PREC_* of the data contraction */
+    uint32_t payload_crc;   /* CRC-32 (IEEE 802.3, reflected 0xEDB88320) of the payload */
+    double h0, h1;          /* kind 1: Ntri, Num (amber.c:287); kind 2: inv_Ng1_Ng2, 0 */
+    uint64_t payload_bytes; /* 8 W^2 */
+    uint8_t reserved[8];    /* zero */
+} amb_bsw_header;
+int amb_write_bsw(const char *res_path, const amb_bsw_header *h, const float *bl, const void *den);
+/* Reads a hardened file: *h always (when the header is valid); bl and den
+ * (W*W elements each) when non-NULL and cap >= W*W. */
+int amb_read_bsw(const char *path, amb_bsw_header *h, float *bl, void *den, int64_t cap);
+/* Reads a legacy bsout file of either kind, W inferred from the size
+ * (16 + 8 W^2 for v1, 8 + 8 W^2 for v2; exactly one can hold for odd W).
+ * *kind, *W always; h0/h1 as in amb_bsw_header; arrays when cap >= W*W. */
+int amb_read_bsout(const char *path, int *kind, int *W, double *h0, double *h1, float *bl, void *den,
+                   int64_t cap);
+uint32_t amb_crc32(const void *p, size_t n);
+
 /* Rules.  amb_ng: amber.c:193-197.  amb_annular_rings: the reference's
  * hard-coded edges, amber_funcs.c:2022-2053.  amb_uavg: :2065.
  * amb_triangles: :443-449 (returns the count; fills up to cap triples). */

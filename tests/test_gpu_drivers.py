@@ -93,3 +93,28 @@ def test_driver_errors_are_reported(tmp_path):
     r = subprocess.run([os.path.join(BIN, "amber"), "1"], cwd=tmp_path, stdout=subprocess.PIPE,
                        stderr=subprocess.STDOUT, text=True, timeout=60)
     assert r.returncode != 0 and "GV_data_1" in r.stdout
+
+
+def test_hardened_wire_matches_legacy(tmp_path):
+    """--wire both: each bsw file carries the legacy payload unchanged plus a
+    header naming the triangle, nc10, Ng, nc and the precision"""
+    N, nc, nflag, nc10 = 64, 10, 2, 6
+    lo, hi = A.log_rings(2, 20, 5)
+    gv, samp, uf = _layout(tmp_path, N, nc, nflag, 32, lo, hi)
+    print("case hardened wire N=%d nc=%d" % (N, nc), flush=True)
+    _run(tmp_path, "amber", "--nc10", str(nc10), "--uf-nc", str(len(uf)), "--wire", "both", "--precision", "fp32")
+    for fn in ("rcount", "closed_rings"):
+        shutil.copy(tmp_path / "results/v1" / fn, tmp_path / "results/v2" / fn)
+    _run(tmp_path, "amber_v2", "--nc10", str(nc10), "--uf-nc", str(len(uf)), "--wire", "both")
+    n, rd = A.read_rings(str(tmp_path / "results/v1"))
+    W = 2 * nc10 + 1
+    for (r1, r2, r3, _) in rd:
+        r1, r2, r3 = int(r1), int(r2), int(r3)
+        for kind, pre, d, prec in ((1, "bsout_final", "v1", A.PREC_FP32), (2, "bsout_final2", "v2", A.PREC_BF16X3)):
+            legacy = A.read_bsout(str(tmp_path / "results" / d / ("%s_ra%d_rb%d_rc%d_R1" % (pre, r1, r2, r3))))
+            h, bl, den = A.read_bsw(str(tmp_path / "results" / d / ("bsw%d_ra%d_rb%d_rc%d_R1" % (kind, r1, r2, r3))))
+            assert legacy[0] == kind and h["kind"] == kind
+            assert (h["r1"], h["r2"], h["r3"], h["rlzn"], h["nc10"], h["W"], h["nc"]) == (r1, r2, r3, 1, nc10, W, nc)
+            assert h["ng"] == A.ng(hi[r1]) and h["precision"] == prec
+            assert (h["h0"], h["h1"]) == (legacy[1], legacy[2])
+            assert np.array_equal(bl, legacy[3]) and np.array_equal(den, legacy[4])
