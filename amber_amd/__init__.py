@@ -36,7 +36,8 @@ EXPORTS = [
     "amb_bispec_multi", "amb_set_batch_mode", "amb_update_cube", "amb_graph_create", "amb_graph_run",
     "amb_graph_destroy", "amb_serve_start", "amb_submit", "amb_poll", "amb_wait", "amb_serve_stop",
     "amb_cyl_bispec", "amb_maps_cldnu", "amb_write_cldnu", "amb_read_params", "amb_write_bsw", "amb_read_bsw",
-    "amb_read_bsout", "amb_crc32",
+    "amb_read_bsout", "amb_crc32", "amb_read_vcg_inputs", "amb_grid_setup", "amb_grid_cells", "amb_grid",
+    "amb_grid_cube",
 ]
 
 BATCH_PER_TRIANGLE, BATCH_GROUPED = 0, 1
@@ -90,7 +91,7 @@ class Stats(C.Structure):
                 ("evals", C.c_int64), ("flop_contract_exec", C.c_double),
                 ("ms_graph", C.c_double), ("n_graph_launch", C.c_int64),
                 ("ms_maps", C.c_double), ("ms_maps_total", C.c_double), ("flop_maps", C.c_double),
-                ("n_maps_launch", C.c_int64)]
+                ("n_maps_launch", C.c_int64), ("ms_grid", C.c_double), ("n_grid_groups", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -142,6 +143,18 @@ _lib.amb_read_bsw.argtypes = [C.c_char_p, C.c_void_p, _fp, C.c_void_p, C.c_int64
 _lib.amb_read_bsout.argtypes = [C.c_char_p, _ip, _ip, _dp, _dp, _fp, C.c_void_p, C.c_int64]
 _lib.amb_crc32.restype = C.c_uint32
 _lib.amb_crc32.argtypes = [C.c_void_p, C.c_size_t]
+class GridCfg(C.Structure):
+    _fields_ = [("NN", C.c_int64), ("nc", C.c_int64), ("nchan", C.c_int64), ("nstokes", C.c_int64),
+                ("chan1", C.c_int), ("npar", C.c_int), ("flag_key", C.c_int), ("n_collapse", C.c_int),
+                ("nu_chan0", C.c_float), ("dU", C.c_double), ("Umax", C.c_double)]
+
+
+_lib.amb_read_vcg_inputs.argtypes = [C.c_char_p, _ip, _dp, _dp, _dp]
+_lib.amb_grid_setup.argtypes = [C.c_double, C.c_double, _dp, C.POINTER(C.c_int64)]
+_lib.amb_grid_cells.restype = C.c_int64
+_lib.amb_grid_cells.argtypes = [C.POINTER(GridCfg), C.c_int64, _fp, C.POINTER(C.c_int64), _fp]
+_lib.amb_grid.argtypes = [C.c_void_p, C.POINTER(GridCfg), C.c_int64, _fp, _fp, _dp, _dp, C.POINTER(C.c_int64)]
+_lib.amb_grid_cube.argtypes = [C.c_void_p, C.POINTER(GridCfg), C.c_int64, _fp, _fp, _ip, C.POINTER(C.c_int64)]
 _lib.amb_get_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
 _lib.amb_reset_stats.argtypes = [C.c_void_p]
 _lib.amb_uavg.argtypes = [_dp, _dp, C.c_int, _dp]
@@ -352,6 +365,42 @@ def read_bsout(path):
     _chk(_lib.amb_read_bsout(path.encode(), C.byref(k), C.byref(W), C.byref(h0), C.byref(h1), _p(bl, _fp),
                              den.ctypes.data_as(C.c_void_p), n * n))
     return k.value, h0.value, h1.value, bl, den
+
+
+def grid_setup(Umax, FWHM):
+    """(dU, NN) of initialize_gridding (3_vcg/tgefuncs.c:305-308)"""
+    dU = C.c_double(); NN = C.c_int64()
+    _chk(_lib.amb_grid_setup(float(Umax), float(FWHM), C.byref(dU), C.byref(NN)))
+    return dU.value, NN.value
+
+
+def read_vcg_inputs(path):
+    """(nc, Umax, FWHM, f) of the vcg inputparameters file"""
+    nc = C.c_int(); a = C.c_double(); b = C.c_double(); f = C.c_double()
+    _chk(_lib.amb_read_vcg_inputs(path.encode(), C.byref(nc), C.byref(a), C.byref(b), C.byref(f)))
+    return nc.value, a.value, b.value, f.value
+
+
+def _grid_cfg(NN, nc, ranpar, data, nu_chan0, dU, Umax, flag_key, n_collapse, chan1):
+    ranpar = np.ascontiguousarray(ranpar, np.float32)
+    data = np.ascontiguousarray(data, np.float32)
+    if ranpar.ndim != 2 or data.ndim != 4 or data.shape[3] != 3 or data.shape[0] != ranpar.shape[0]:
+        raise ValueError("ranpar [ngroup][npar], data [ngroup][nchan][nstokes][3]")
+    cfg = GridCfg(int(NN), int(nc), data.shape[1], data.shape[2], int(chan1), ranpar.shape[1], int(flag_key),
+                  int(n_collapse), float(nu_chan0), float(dU), float(Umax))
+    return cfg, ranpar, data
+
+
+def grid_cells(NN, ranpar, nu_chan0, dU, Umax):
+    """cell index of every group (-1 beyond Umax) and its sign flip"""
+    ranpar = np.ascontiguousarray(ranpar, np.float32)
+    cfg = GridCfg(int(NN), 1, 1, 1, 1, ranpar.shape[1], 1, 1, float(nu_chan0), float(dU), float(Umax))
+    cell = np.zeros(len(ranpar), np.int64); sign = np.zeros(len(ranpar), np.float32)
+    n = _lib.amb_grid_cells(C.byref(cfg), len(ranpar), _p(ranpar, _fp),
+                            cell.ctypes.data_as(C.POINTER(C.c_int64)), _p(sign, _fp))
+    if n < 0:
+        raise AmberError(int(n))
+    return cell, sign
 
 
 # ---------------------------------------------------------------- device context
@@ -582,6 +631,26 @@ class Context:
         self._c(_lib.amb_maps_cldnu(self._h, N, nc, float(Du), _p(gv, _dp), _p(samp, _dp), NB, float(umin),
                                     float(umax), _p(cl, _dp), _p(nd, _dp), _p(bu, _dp), _p(nm, _ip)))
         return cl, nd, bu, nm
+
+    def grid(self, NN, nc, ranpar, data, nu_chan0, dU, Umax, flag_key=1, n_collapse=1, chan1=1):
+        """vcg gridding on the GPU: (gv [NN(NN/2+1)][nc/n_collapse][2], samp, groups gridded)"""
+        cfg, ranpar, data = _grid_cfg(NN, nc, ranpar, data, nu_chan0, dU, Umax, flag_key, n_collapse, chan1)
+        cells = int(NN) * (int(NN) // 2 + 1)
+        ncout = int(nc) // int(n_collapse)
+        gv = np.empty((cells, ncout, 2)); samp = np.empty((cells, ncout))
+        n = C.c_int64()
+        self._c(_lib.amb_grid(self._h, C.byref(cfg), len(ranpar), _p(ranpar, _fp), _p(data, _fp), _p(gv, _dp),
+                              _p(samp, _dp), C.byref(n)))
+        return gv, samp, n.value
+
+    def grid_cube(self, NN, nc, ranpar, data, nu_chan0, dU, Umax, flag_key=1, n_collapse=1, chan1=1):
+        """vcg gridding kept resident as a cube: (cube id, groups gridded)"""
+        cfg, ranpar, data = _grid_cfg(NN, nc, ranpar, data, nu_chan0, dU, Umax, flag_key, n_collapse, chan1)
+        cid = C.c_int(); n = C.c_int64()
+        self._c(_lib.amb_grid_cube(self._h, C.byref(cfg), len(ranpar), _p(ranpar, _fp), _p(data, _fp),
+                                   C.byref(cid), C.byref(n)))
+        self._meta[cid.value] = dict(N=int(NN), nc=int(nc) // int(n_collapse), Du=float(dU), flag_key=int(flag_key))
+        return cid.value, n.value
 
     def batch_times(self):
         """completion time (ms from the call's start) of each triangle of the last batch"""

@@ -341,6 +341,64 @@ int amb_read_bsout(const char *path, int *kind, int *W, double *h0, double *h1, 
     return AMB_OK;
 }
 
+/* ---------------------------------------------------------------- gridder (host part) */
+int amb_read_vcg_inputs(const char *path, int *nc, double *Umax, double *FWHM, double *f)
+{   /* read_inputs, 3_vcg/tgefuncs.c:28-37 */
+    if (!path) return AMB_EARG;
+    FILE *fp = fopen(path, "r");
+    if (!fp) return AMB_EIO;
+    int n_;
+    double a, b, c;
+    int n = fscanf(fp, "%d%lf%lf%lf", &n_, &a, &b, &c);
+    fclose(fp);
+    if (n != 4) return AMB_EFORMAT;
+    if (nc) *nc = n_;
+    if (Umax) *Umax = a;
+    if (FWHM) *FWHM = b;
+    if (f) *f = c;
+    return AMB_OK;
+}
+
+int amb_grid_setup(double Umax, double FWHM, double *dU, int64_t *NN)
+{   /* initialize_gridding, tgefuncs.c:305-308 */
+    if (!dU || !NN || !(Umax > 0) || !(FWHM > 0)) return AMB_EARG;
+    const double theta_0 = 0.6 * FWHM * M_PI / (180. * 60.);
+    *dU = sqrt(log(2)) / (M_PI * theta_0);
+    const int Ng = (int)ceil(Umax / *dU);
+    *NN = (int64_t)2 * Ng;
+    return AMB_OK;
+}
+
+int64_t amb_grid_cells(const amb_grid_cfg *cfg, int64_t ngroup, const float *ranpar, int64_t *cell, float *signv)
+{   /* tgefuncs.c:386-391 and 448-451, float arithmetic as written; returns
+       the number of groups inside Umax, or AMB_EARG for a bad config or a
+       cell outside the grid */
+    if (!cfg || ngroup < 0 || (ngroup && (!ranpar || !cell)) || cfg->npar < 2 || cfg->NN < 2 || !(cfg->dU > 0))
+        return AMB_EARG;
+    const int64_t NN = cfg->NN, Nh = NN / 2 + 1;
+    int64_t in = 0;
+    for (int64_t g = 0; g < ngroup; ++g) {
+        const float *rp = ranpar + g * cfg->npar;
+        const float sv = (rp[1] < 0.) ? -1. : 1.;
+        const float u1 = cfg->nu_chan0 * sv * rp[0];
+        const float v1 = cfg->nu_chan0 * sv * rp[1];
+        const float uu = u1 * u1 + v1 * v1;
+        const double Uval = sqrt(uu);
+        if (signv) signv[g] = sv;
+        if (!(Uval <= cfg->Umax)) {
+            cell[g] = -1;
+            continue;
+        }
+        int ii = (int)round(u1 / cfg->dU);
+        ii = (ii < 0) ? (int)(NN + ii) : ii;
+        const int jj = (int)round(v1 / cfg->dU);
+        if (ii < 0 || ii >= NN || jj < 0 || jj >= Nh) return AMB_EARG;
+        cell[g] = (int64_t)ii * Nh + jj;
+        ++in;
+    }
+    return in;
+}
+
 int amb_read_params(const char *path, int64_t *NN, double *Du, double *nu0, double *dnu, double *chan0,
                     int64_t *nc, char *data_path, size_t cap)
 {   /* read_inputs, amber_funcs.c:159-172: seven fields, the path at most 511 chars */

@@ -332,6 +332,50 @@ int amb_maps_cldnu(amb_ctx *ctx, int64_t N, int64_t nc, double Du, const double 
                    int NB, double umin, double umax, double *cldnu, double *numdnu, double *binu,
                    int32_t *num);
 
+/* Upstream gridder vcg on the GPU (3_vcg/tgefuncs.c).  The reference reads
+ * a UVFITS random-groups file through CFITSIO (read_ranpar / read_data,
+ * read_fits_funcmfs.c:99-120); these entry points take what those calls
+ * return, for every group: pcount random parameters (u at 0, v at 1, in
+ * seconds) and nchan x nstokes x 3 floats (re, im, wt).
+ *   amb_read_vcg_inputs  read_inputs (tgefuncs.c:28-37): nc Umax FWHM f
+ *   amb_grid_setup       initialize_gridding (:290-376): dU from the beam
+ *                        FWHM (arcmin), NN = 2 ceil(Umax/dU)
+ *   amb_grid_cells       the cell of every group (-1 beyond Umax), with the
+ *                        reference's float arithmetic (:386-391, 448-451)
+ *   amb_grid             group_loop_gridding (flag_key 1, :377-509) or
+ *                        group_loop_gridding_without_flag (0, :511-626),
+ *                        then collapse_channels (:816-839) when
+ *                        n_collapse > 1: gv [NN(NN/2+1)][nc/n_collapse][2]
+ *                        and samp [..][nc/n_collapse] in the GV_data /
+ *                        sampling layout, bit-identical to the reference's
+ *                        sums (each cell adds its groups in group order)
+ *   amb_grid_cube        the same, kept resident in HBM as a cube (as if
+ *                        written and read back by amb_load_cube), so the
+ *                        bispectrum path needs no file round trip
+ * Only the RR stokes (index 0) reaches the reference's output files; the LL
+ * sums it also forms are never written and are not formed here.  A group
+ * whose cell falls outside the grid (possible only when NN or dU do not come
+ * from amb_grid_setup) is AMB_EARG. */
+typedef struct amb_grid_cfg {
+    int64_t NN;           /* grid side */
+    int64_t nc;           /* channels gridded, from chan1 */
+    int64_t nchan;        /* channels in the data (NAXIS4) */
+    int64_t nstokes;      /* stokes in the data (NAXIS3); 3 floats each */
+    int chan1;            /* first channel, 1-based (tgefuncs.c:19) */
+    int npar;             /* random parameters per group (PCOUNT) */
+    int flag_key;         /* 1: weight > 0 only; 0: every visibility */
+    int n_collapse;       /* 1: none; k: sums of k adjacent channels */
+    float nu_chan0;       /* CRVAL4, Hz (float as in read_fits_funcmfs.c:27) */
+    double dU, Umax;      /* wavelengths */
+} amb_grid_cfg;
+int amb_read_vcg_inputs(const char *path, int *nc, double *Umax, double *FWHM, double *f);
+int amb_grid_setup(double Umax, double FWHM, double *dU, int64_t *NN);
+int64_t amb_grid_cells(const amb_grid_cfg *cfg, int64_t ngroup, const float *ranpar, int64_t *cell, float *signv);
+int amb_grid(amb_ctx *ctx, const amb_grid_cfg *cfg, int64_t ngroup, const float *ranpar, const float *data,
+             double *gv, double *samp, int64_t *ngridded);
+int amb_grid_cube(amb_ctx *ctx, const amb_grid_cfg *cfg, int64_t ngroup, const float *ranpar, const float *data,
+                  int *cube_id, int64_t *ngridded);
+
 /* results/cldnu text (maps.c:213-222): NB*nc lines "b binu d cldnu numdnu num". */
 int amb_write_cldnu(const char *path, int NB, int64_t nc, const double *binu, const double *cldnu,
                     const double *numdnu, const int32_t *num);
@@ -367,6 +411,8 @@ typedef struct amb_stats {
     double ms_maps_total;   /* all MAPS kernels (normalise .. lag sums) */
     double flop_maps;       /* algorithmic 4 flops per (channel pair n1<=n2, binned cell) */
     int64_t n_maps_launch;
+    double ms_grid;         /* gridding kernels (sums and collapse) */
+    int64_t n_grid_groups;  /* groups gridded */
 } amb_stats;
 int amb_get_stats(amb_ctx *ctx, amb_stats *st);
 int amb_reset_stats(amb_ctx *ctx);

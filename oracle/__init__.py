@@ -180,3 +180,33 @@ def cldnu(N, nc, Du, gv, samp, NB=15, umin=1.0, umax=500.0):
     if rc != 0:
         raise RuntimeError("orc_cldnu failed")
     return cl, nd, bu, nm
+
+
+_lib.orc_grid_setup.argtypes = [C.c_double, C.c_double, _dp, C.POINTER(C.c_long)]
+_lib.orc_grid.restype = C.c_long
+_lib.orc_grid.argtypes = [C.c_long, C.c_long, C.c_long, C.c_long, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float,
+                          C.c_double, C.c_double, C.c_long, _fp, _fp, _dp, _dp]
+
+
+def grid_setup(Umax, FWHM):
+    """(dU, NN) of initialize_gridding"""
+    dU = C.c_double(); NN = C.c_long()
+    _lib.orc_grid_setup(Umax, FWHM, C.byref(dU), C.byref(NN))
+    return dU.value, NN.value
+
+
+def grid(NN, nc, ranpar, data, nu_chan0, dU, Umax, flag_key=1, n_collapse=1, chan1=1):
+    """vcg gridding: ranpar [ngroup][npar], data [ngroup][nchan][nstokes][3] float32;
+    returns (gv [NN(NN/2+1)][nc/n_collapse][2], samp [..][nc/n_collapse], groups gridded)"""
+    ranpar = np.ascontiguousarray(ranpar, np.float32)
+    data = np.ascontiguousarray(data, np.float32)
+    ngroup, npar = ranpar.shape
+    nchan, nstokes = data.shape[1], data.shape[2]
+    ncout = nc // n_collapse
+    cells = NN * (NN // 2 + 1)
+    gv = np.zeros((cells, ncout, 2)); samp = np.zeros((cells, ncout))
+    n = _lib.orc_grid(NN, nc, nchan, nstokes, chan1, npar, flag_key, n_collapse, nu_chan0, dU, Umax, ngroup,
+                      _p(ranpar, _fp), _p(data, _fp), _p(gv, _dp), _p(samp, _dp))
+    if n < 0:
+        raise RuntimeError("orc_grid failed (%d)" % n)
+    return gv, samp, n

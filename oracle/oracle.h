@@ -31,6 +31,12 @@
  *                     channel masks sum_n1 sum_x [S1>0][S2>0]|c|^2 / no_b(n1);
  *                     numdnu = nc - d; counts and mean |U| per bin by brute
  *                     enumeration of the cells
+ *   orc_grid_setup    the shipped inputparameters (Umax 602, FWHM 1480.73)
+ *                     worked by hand
+ *   orc_grid          hand-computed cells of chosen (u, v), the v < 0 flip
+ *                     equal to the conjugate group, constant visibilities
+ *                     summing to count x value, flagged weights, the
+ *                     Umax boundary, collapse sums
  * No function is left "parity unpinned".
  */
 #ifndef AMBER_ORACLE_H
@@ -125,6 +131,18 @@ int orc_maps_bin(long N, double Du, long i, long j, int NB, double umin, double 
  * of the last pair (nc-1, nc-1).  Returns 0, or -1 when out of memory. */
 int orc_cldnu(long N, long nc, double Du, const double *re_im, const double *samp, int NB, double umin,
               double umax, double *cldnu, double *numdnu, double *binu, int32_t *num);
+
+/* Upstream gridder vcg (3_vcg/tgefuncs.c:290-376, 377-626, 816-839).
+ * orc_grid_setup: dU and NN from Umax and the beam FWHM (arcmin).
+ * orc_grid: ranpar [ngroup][npar] (u at 0, v at 1, seconds), data
+ * [ngroup][nchan][nstokes][3] (re, im, wt); channels chan1 .. chan1+nc-1
+ * (1-based) of stokes 0 are gridded; gv [NN(NN/2+1)][nc/n_collapse][2],
+ * samp [NN(NN/2+1)][nc/n_collapse].  Returns the number of groups gridded,
+ * -1 out of memory, -2 a group outside the grid. */
+void orc_grid_setup(double Umax, double FWHM, double *dU, long *NN);
+long orc_grid(long NN, long nc, long nchan, long nstokes, int chan1, int npar, int flag_key, int n_collapse,
+              float nu_chan0, double dU, double Umax, long ngroup, const float *ranpar, const float *data,
+              double *gv, double *samp);
 
 #ifdef __cplusplus
 }

@@ -162,3 +162,22 @@ def test_context_without_gpu_is_an_error():
     with pytest.raises(A.AmberError) as e:
         A.Context()
     assert e.value.code == -4
+
+
+def test_grid_cells_and_setup_match_oracle():
+    """amb_grid_setup / amb_grid_cells (host side of the gridder) against the
+    oracle's grid and hand cells"""
+    import oracle as O
+    assert A.grid_setup(602.0, 1480.73) == O.grid_setup(602.0, 1480.73) == (1.0254384902642049, 1176)
+    NN, dU = 16, 2.0
+    rp = np.array([[3.1, 4.9], [-3.1, 4.9], [3.1, -4.9], [30.0, 1.0]], np.float32)
+    cell, sign = A.grid_cells(NN, rp, 1.0, dU, 20.0)
+    assert cell.tolist() == [2 * 9 + 2, 14 * 9 + 2, 14 * 9 + 2, -1] and sign.tolist() == [1, 1, -1, 1]
+    rng = np.random.default_rng(2)
+    dU, NN = O.grid_setup(50.0, 700.0)
+    rp = (rng.standard_normal((400, 3)) * 30 / 1.4e8).astype(np.float32)
+    data = np.ones((400, 1, 1, 3), np.float32)
+    cell, _ = A.grid_cells(NN, rp, np.float32(1.4e8), dU, 50.0)
+    _, samp, n = O.grid(NN, 1, rp, data, np.float32(1.4e8), dU, 50.0, flag_key=0)
+    assert n == (cell >= 0).sum()
+    assert np.array_equal(np.bincount(cell[cell >= 0], minlength=samp.shape[0]), samp[:, 0])
