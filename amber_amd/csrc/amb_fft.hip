@@ -254,33 +254,48 @@ hipError_t amb_launch_ring_c2r(hipStream_t s, int kind, const float2 *vis_t, con
 
 /* Upload: file layout [rows][Nh][nc]{re,im} doubles -> plane-major
  * vis_t[nc][Nh][N] complex64 and samp_t[nc][Nh][N] float, with
- * normalize_data (amber_funcs.c:1946-1978) applied in FP64 first. */
-__global__ void k_upload(const double *__restrict__ gv, const double *__restrict__ samp, int i0,
-                         int rows, int N, int Nh, int nc, float2 *__restrict__ vis_t,
-                         float *__restrict__ samp_t)
+ * normalize_data (amber_funcs.c:1946-1978) applied in FP64 first.  For each
+ * uv column j this is a transpose of the (row, channel) block: a workgroup
+ * reads a 64-row x 64-channel tile (whole 1 KB runs of one row's channels),
+ * normalises it into LDS, and writes it back as 64 channels x 64 rows
+ * (512-byte runs of one plane's rows), so both sides move whole lines. */
+__global__ __launch_bounds__(256) void k_upload(const double2 *__restrict__ gv, const double *__restrict__ samp,
+                                                int i0, int rows, int N, int Nh, int nc,
+                                                float2 *__restrict__ vis_t, float *__restrict__ samp_t)
 {
-    size_t tot = (size_t)rows * Nh * nc;
-    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < tot;
-         idx += (size_t)gridDim.x * blockDim.x) {
-        int c = (int)(idx % nc);
-        size_t cell = idx / nc;
-        int j = (int)(cell % Nh);
-        int r = (int)(cell / Nh);
-        double sv = samp[idx];
-        double re = gv[2 * idx], im = gv[2 * idx + 1];
-        if (sv > 0) { re /= sv; im /= sv; sv = 1.0; }
-        size_t dst = ((size_t)c * Nh + j) * N + (i0 + r);
-        vis_t[dst] = make_float2((float)re, (float)im);
-        samp_t[dst] = (float)sv;
+    __shared__ float2 tv[64][65];
+    __shared__ float ts[64][65];
+    const int c0 = blockIdx.x * 64, r0 = blockIdx.y * 64, j = blockIdx.z;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int rr = ty; rr < 64; rr += 4) {
+        const int r = r0 + rr, c = c0 + tx;
+        if (r < rows && c < nc) {
+            const size_t idx = ((size_t)r * Nh + j) * nc + c;
+            double sv = samp[idx];
+            const double2 g = gv[idx];
+            double re = g.x, im = g.y;
+            if (sv > 0) { re /= sv; im /= sv; sv = 1.0; }
+            tv[rr][tx] = make_float2((float)re, (float)im);
+            ts[rr][tx] = (float)sv;
+        }
+    }
+    __syncthreads();
+    for (int cc = ty; cc < 64; cc += 4) {
+        const int c = c0 + cc, r = r0 + tx;
+        if (r < rows && c < nc) {
+            const size_t dst = ((size_t)c * Nh + j) * N + (i0 + r);
+            vis_t[dst] = tv[tx][cc];
+            samp_t[dst] = ts[tx][cc];
+        }
     }
 }
 
 hipError_t amb_launch_upload(hipStream_t s, const double *gv, const double *samp, int i0, int rows,
                              int N, int Nh, int nc, float2 *vis_t, float *samp_t)
 {
-    size_t tot = (size_t)rows * Nh * nc;
-    int blocks = (int)((tot + 255) / 256);
-    if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL(k_upload, dim3(blocks), dim3(256), 0, s, gv, samp, i0, rows, N, Nh, nc, vis_t, samp_t);
+    if (rows < 1 || nc < 1 || Nh < 1 || Nh > 65535 || i0 < 0 || i0 + rows > N) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)((nc + 63) / 64), (unsigned)((rows + 63) / 64), (unsigned)Nh);
+    hipLaunchKernelGGL(k_upload, grid, dim3(256), 0, s, (const double2 *)gv, samp, i0, rows, N, Nh, nc, vis_t,
+                       samp_t);
     return hipGetLastError();
 }
