@@ -577,9 +577,11 @@ __device__ __forceinline__ void contract_item(const ContractArgs &a, int tile, i
  *     global -> LDS loads (global_load_lds_dwordx4: lane i's 16 bytes land
  *     at the instruction's LDS base + 16 i), one chunk ahead, so staging
  *     costs neither registers nor LDS store instructions;
- *   - rows stored without padding; a row's 16-byte slots are XOR-permuted by
- *     the low 3 bits of the row, which makes the fragment reads of 8
- *     consecutive rows hit 8 different bank groups;
+ *   - rows stored without padding; a row's 16-byte slots are XOR-permuted so
+ *     that the 16 lanes an LDS cycle serves for a 128-bit read hit 16
+ *     different 4-bank groups: A rows (16 slots, 256 bytes) by the low 4
+ *     bits of the row, B rows (8 slots, 128 bytes, so neighbouring rows sit
+ *     in opposite bank halves) by bits 1..3 of the row;
  *   - one barrier per chunk, and within a chunk the LDS reads and A
  *     formation of k-step s+1 are interleaved with the MFMAs of k-step s.
  * Rows and columns past nact and planes past nact read clamped (valid)
@@ -589,8 +591,8 @@ __device__ __forceinline__ void contract_item(const ContractArgs &a, int tile, i
 typedef __attribute__((address_space(3))) void lds_void_t;
 
 struct A16 V2Stage {
-    float a[TT * KC2];     /* D_r2 row r, float4 slot c at r*64 + 4*(c ^ (r & 7)) */
-    __bf16 bh[TT * KC2];   /* D_r3 hi col j, 16-byte slot c at j*64 + 8*(c ^ (j & 7)) */
+    float a[TT * KC2];     /* D_r2 row r, float4 slot c at r*64 + 4*(c ^ (r & 15)) */
+    __bf16 bh[TT * KC2];   /* D_r3 hi col j, 16-byte slot c at j*64 + 8*(c ^ ((j >> 1) & 7)) */
     __bf16 bl[TT * KC2];   /* D_r3 lo, same layout */
     float d1[QP * KC2];    /* D_r1 of the 4 planes, plain rows */
 };
@@ -622,7 +624,7 @@ __device__ __forceinline__ void v2_issue(V2Stage &s, const ContractArgs &a, long
     for (int u = 0; u < 4; ++u) {
         const int t = wv * 4 + u;                  /* 1 KB block of the A region */
         const int r = t * 4 + (lane >> 4), ph = lane & 15;
-        const int c = ph ^ (r & 7);
+        const int c = ph ^ (r & 15);
         const int gr = min(r0 + r, last);
         lds_load16(a.D2 + (size_t)gr * pitch + k0 + c * 4, &s.a[t * 256]);
     }
@@ -630,7 +632,7 @@ __device__ __forceinline__ void v2_issue(V2Stage &s, const ContractArgs &a, long
     for (int u = 0; u < 2; ++u) {
         const int t = wv * 2 + u;
         const int j = t * 8 + (lane >> 3), ph = lane & 7;
-        const int c = ph ^ (j & 7);
+        const int c = ph ^ ((j >> 1) & 7);
         const size_t off = (size_t)min(c0 + j, last) * pitch + k0 + c * 8;
         lds_load16(a.H3 + off, &s.bh[t * 512]);
         lds_load16(a.L3 + off, &s.bl[t * 512]);
@@ -652,14 +654,14 @@ __device__ __forceinline__ void v2_frag(const V2Stage &s, int st, int q, int rb,
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const int col = j * 32 + rl;
-        const int ph = (kc >> 3) ^ (col & 7);
+        const int ph = (kc >> 3) ^ ((col >> 1) & 7);
         f.bh[j] = *(const bf16x8 *)&s.bh[col * KC2 + ph * 8];
         f.bl[j] = *(const bf16x8 *)&s.bl[col * KC2 + ph * 8];
     }
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
         const int r = rb + i * 32 + rl;
-        const int p0 = (kc >> 2) ^ (r & 7);
+        const int p0 = (kc >> 2) ^ (r & 15);
         float x[8];
         *(float4 *)&x[0] = *(const float4 *)&s.a[r * KC2 + p0 * 4];
         *(float4 *)&x[4] = *(const float4 *)&s.a[r * KC2 + (p0 ^ 1) * 4];
@@ -700,7 +702,7 @@ __device__ __forceinline__ void v2_read_a(const V2Stage &s, int st, int q, int r
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
         const int row = rb + i * 32 + rl;
-        const int p0 = (kc >> 2) ^ (row & 7);
+        const int p0 = (kc >> 2) ^ (row & 15);
         *(float4 *)&r.x[i][0] = *(const float4 *)&s.a[row * KC2 + p0 * 4];
         *(float4 *)&r.x[i][4] = *(const float4 *)&s.a[row * KC2 + (p0 ^ 1) * 4];
     }
@@ -713,7 +715,7 @@ __device__ __forceinline__ void v2_read_b(const V2Stage &s, int st, int lane, F2
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const int col = j * 32 + rl;
-        const int ph = (kc >> 3) ^ (col & 7);
+        const int ph = (kc >> 3) ^ ((col >> 1) & 7);
         f.bh[j] = *(const bf16x8 *)&s.bh[col * KC2 + ph * 8];
         f.bl[j] = *(const bf16x8 *)&s.bl[col * KC2 + ph * 8];
     }
