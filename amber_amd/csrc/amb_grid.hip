@@ -96,8 +96,8 @@ static int grid_core(amb_ctx *c, const amb_grid_cfg *cfg, int64_t ngroup, const 
     const int64_t in = amb_grid_cells(cfg, ngroup, ranpar, cell.data(), sign.data());
     if (in < 0) return fail(c, AMB_EARG, "grid: a group falls outside the %lld x %lld grid", (long long)NN,
                             (long long)Nh);
-    /* counting sort of the gridded groups by cell, group order kept */
-    std::vector<int> cnt;
+    if (in > INT32_MAX) return fail(c, AMB_EARG, "grid: %lld groups exceed the cell tables", (long long)in);
+    /* the gridded groups sorted by cell, group order kept inside a cell */
     std::vector<long long> ccell;
     std::vector<int> cstart;
     std::vector<long long> gidx(std::max<int64_t>(in, 1));
@@ -119,7 +119,6 @@ static int grid_core(amb_ctx *c, const amb_grid_cfg *cfg, int64_t ngroup, const 
         }
         cstart.push_back((int)m);
     }
-    if (in > INT32_MAX) return fail(c, AMB_EARG, "grid: %lld groups exceed the cell tables", (long long)in);
     const int nonempty = (int)ccell.size();
     const int64_t gstride = cfg->nchan * cfg->nstokes * 3;
     const size_t data_bytes = sizeof(float) * (size_t)ngroup * gstride;
@@ -168,6 +167,8 @@ static int grid_core(amb_ctx *c, const amb_grid_cfg *cfg, int64_t ngroup, const 
         timer_stop(c);
         e = hipGetLastError();
     }
+    /* the host tables above are copied from this frame: finish before it ends */
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
         hipStreamSynchronize(st);
         timers_collect(c);

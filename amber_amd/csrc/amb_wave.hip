@@ -226,7 +226,9 @@ int run_wave(amb_ctx *c, Wave &w, const amb_tri_req *req)
     HIPCHK(c, hipMemcpyAsync(b.desc, h.data(), h.size(), hipMemcpyHostToDevice, c->stream));
     hipError_t e = launch_wave(c, c->stream, w, b, true);
     c->st.n_contract_launch++;
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    /* always drain: the descriptor copy reads h, which dies with this frame */
+    const hipError_t es = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = es;
     timers_collect(c);
     if (e != hipSuccess)
         return fail(c, AMB_EHIP, "grouped wave of %zu triangles: %s", w.jobs.size(), hipGetErrorString(e));
