@@ -91,7 +91,8 @@ class Stats(C.Structure):
                 ("evals", C.c_int64), ("flop_contract_exec", C.c_double),
                 ("ms_graph", C.c_double), ("n_graph_launch", C.c_int64),
                 ("ms_maps", C.c_double), ("ms_maps_total", C.c_double), ("flop_maps", C.c_double),
-                ("n_maps_launch", C.c_int64), ("ms_grid", C.c_double), ("n_grid_groups", C.c_int64)]
+                ("n_maps_launch", C.c_int64), ("ms_grid", C.c_double), ("n_grid_groups", C.c_int64),
+                ("ms_post", C.c_double), ("flop_post", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

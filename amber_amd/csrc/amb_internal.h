@@ -96,11 +96,6 @@ int amb_contract_grouped_blocks(int dev);
 /* n words set to v by a kernel (graph-capturable, unlike a memset) */
 hipError_t amb_launch_fill_u32(hipStream_t s, unsigned *p, long n, unsigned v);
 
-/* lag-space post-processing of one triangle (fold, normalise, cut, window,
- * inverse 2-D transform); X: M x N, Z: M x N, out: N x N scratch/results */
-hipError_t amb_launch_cyl(hipStream_t s, const float *bl, const void *den, int den_is_int, int W, int nc10, int ncut,
-                          float mabs_norm, const double *win, const double2 *tw, double scale, double *X, double2 *Z,
-                          double *out);
 
 /* calCR triple sums: one FP64 sum per triangle over P pixels. */
 hipError_t amb_launch_triple_sum(hipStream_t s, const float *II, long P, const int *tri, int ntri,

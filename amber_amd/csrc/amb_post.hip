@@ -10,16 +10,21 @@
  *             (cut_nan, :259-260), optional Blackman-Nuttall window
  *             (mfunc_winds.py:16-30, 94-101);
  *   transform fftshift(irfft2(ifftshift(X, axes=1), axes=(1,0), s=(N,N)))
- *             * rc^4 L^2, L = N dnu rc' (Bdnu2Bk_fft, :325-334), in FP64 by
- *             direct sums over a twiddle table: a complex inverse DFT along
- *             the columns, then the half-spectrum inverse along the rows.
+ *             * rc^4 L^2, L = N dnu rc' (Bdnu2Bk_fft, :325-334), in FP64 as
+ *             two direct-DFT matrix products on the FP64 matrix cores:
+ *             [Zr | Zi] = X [cos | sin] / N (the complex inverse DFT along the
+ *             columns, e^{+2 pi i j k / N}), then
+ *             y = [w cos | -w sin] [Zr ; Zi] / N (the half-spectrum inverse
+ *             along the rows, w_0 = 1, w_m = 2).
  * N = W - 2n is odd (W = 2 nc10 + 1), so the row transform has no Nyquist
- * term; like numpy's c2r it uses only the real part of the m = 0 row.
- * Cost is about N^3 FP64 multiply-adds per triangle (N = 1519: 3.5e9),
- * under a millisecond; the arrays are small enough that HBM is no concern.
+ * term; like numpy's c2r it uses only the real part of the m = 0 row.  The
+ * twiddle matrices are built once per call from one table of e^{2 pi i t/N}
+ * indexed by (j k) mod N.  Work: 8 M N^2 flop per triangle (N = 1519,
+ * M = 760: 1.4e10).
  */
 #include "amb_ctx.h"
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -55,73 +60,111 @@ __global__ void k_cyl_prep(const float *__restrict__ bl, const void *__restrict_
     X[(size_t)m * N + k] = v;
 }
 
-/* Z[m][k] = (1/N) sum_j X[m][j] e^{+2 pi i j k / N}; a block stages one row
- * and the twiddles in LDS */
-__global__ void k_cyl_cols(const double *__restrict__ X, const double2 *__restrict__ tw, int N, int M,
-                           double2 *__restrict__ Z)
+/* B1 = [cos | sin](2 pi (j k mod N) / N), N x 2N; A2 = [w cos | -w sin](2 pi
+ * (m t mod N) / N), N x 2M (row t, column m), w_0 = 1, w_m = 2 */
+__global__ void k_cyl_tables(const double2 *__restrict__ tw, int N, int M, double *__restrict__ B1,
+                             double *__restrict__ A2)
 {
-    extern __shared__ double sm[];
-    double *row = sm;
-    double2 *t = (double2 *)(sm + N);
-    const int m = blockIdx.y;
-    for (int i = threadIdx.x; i < N; i += blockDim.x) {
-        row[i] = X[(size_t)m * N + i];
-        t[i] = tw[i];
+    const int r = blockIdx.y, k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < N) {
+        const double2 t = tw[(int)(((long)r * k) % N)];
+        B1[(size_t)r * 2 * N + k] = t.x;
+        B1[(size_t)r * 2 * N + N + k] = t.y;
     }
-    __syncthreads();
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= N) return;
-    double re = 0.0, im = 0.0;
-    int idx = 0; /* j k mod N */
-    for (int j = 0; j < N; ++j) {
-        re = fma(row[j], t[idx].x, re);
-        im = fma(row[j], t[idx].y, im);
-        idx += k;
-        if (idx >= N) idx -= N;
+    if (k < M) {
+        const double2 t = tw[(int)(((long)k * r) % N)];
+        const double w = k == 0 ? 1.0 : 2.0;
+        A2[(size_t)r * 2 * M + k] = w * t.x;
+        A2[(size_t)r * 2 * M + M + k] = -(w * t.y);
     }
-    Z[(size_t)m * N + k] = make_double2(re / N, im / N);
 }
 
-/* y[tt][k] = (1/N)(Re Z[0][k] + 2 sum_{m>=1} Re(Z[m][k] e^{+2 pi i m tt / N})),
- * written fftshifted on both axes and scaled */
-__global__ void k_cyl_rows(const double2 *__restrict__ Z, const double2 *__restrict__ tw, int N, int M, double scale,
-                           double *__restrict__ out)
+/* C[M][N] = A[M][K] B[K][N], row-major with leading dimensions, FP64 on the
+ * matrix cores.  A 64x64 tile per workgroup, 32-deep k blocks through LDS,
+ * four waves of 2x2 v_mfma_f64_16x16x4f64 blocks: A[m][k] = lane m + 16 k,
+ * B[k][n] = lane n + 16 k, lane l's four results are rows 4 r + l / 16 of
+ * column l % 16.  Out-of-range operands load as zeros. */
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_dgemm(const double *__restrict__ A, int lda, const double *__restrict__ B,
+                                               int ldb, double *__restrict__ C, int ldc, int M, int N, int K)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x, tt = blockIdx.y;
-    if (k >= N) return;
-    double acc = Z[k].x;
-    double s2 = 0.0;
-    int idx = 0; /* m tt mod N */
-    for (int m = 1; m < M; ++m) {
-        idx += tt;
-        if (idx >= N) idx -= N;
-        const double2 z = Z[(size_t)m * N + k], w = tw[idx];
-        s2 = fma(z.x, w.x, s2);
-        s2 = fma(-z.y, w.y, s2);
+    __shared__ double As[32][65], Bs[32][65]; /* [k][m], [k][n] */
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+    const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
+    const int fm = lane & 15, fk = lane >> 4;
+    f64x4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = (f64x4){0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < K; k0 += 32) {
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int e = tid + 256 * q;
+            const int ar = e >> 5, ak = e & 31; /* A: 64 rows x 32 k, k fastest */
+            const int gm = m0 + ar, gk = k0 + ak;
+            As[ak][ar] = (gm < M && gk < K) ? A[(size_t)gm * lda + gk] : 0.0;
+            const int bk = e >> 6, bn = e & 63; /* B: 32 k x 64 columns */
+            const int gk2 = k0 + bk, gn = n0 + bn;
+            Bs[bk][bn] = (gk2 < K && gn < N) ? B[(size_t)gk2 * ldb + gn] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < 8; ++ks) {
+            const int row = 4 * ks + fk;
+            double fa[2], fb[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) fa[i] = As[row][wr + 16 * i + fm];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) fb[j] = Bs[row][wc + 16 * j + fm];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[i], fb[j], acc[i][j], 0, 0, 0);
+        }
     }
-    acc = (acc + 2.0 * s2) / N;
-    const int a = (tt + N / 2) % N, b = (k + N / 2) % N; /* fftshift: out[a] = y[a - N/2] */
-    out[(size_t)a * N + b] = acc * scale;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int gm = m0 + wr + 16 * i + 4 * r + fk;
+            if (gm >= M) continue;
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int gn = n0 + wc + 16 * j + fm;
+                if (gn < N) C[(size_t)gm * ldc + gn] = acc[i][j][r];
+            }
+        }
+}
+
+/* [Zr ; Zi] (2M x N) from C1 = X [cos | sin] (M x 2N), divided by N */
+__global__ void k_cyl_stack(const double *__restrict__ C1, int N, int M, double *__restrict__ Z2)
+{
+    const int m = blockIdx.y, k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    Z2[(size_t)m * N + k] = C1[(size_t)m * 2 * N + k] / N;
+    Z2[(size_t)(M + m) * N + k] = C1[(size_t)m * 2 * N + N + k] / N;
+}
+
+/* y = C2 / N, written fftshifted on both axes and scaled */
+__global__ void k_cyl_out(const double *__restrict__ C2, int N, double scale, double *__restrict__ out)
+{
+    const int t = blockIdx.y, k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    const int a = (t + N / 2) % N, b = (k + N / 2) % N; /* fftshift: out[a] = y[a - N/2] */
+    out[(size_t)a * N + b] = C2[(size_t)t * N + k] / N * scale;
 }
 
 } // namespace
 
-hipError_t amb_launch_cyl(hipStream_t s, const float *bl, const void *den, int den_is_int, int W, int nc10, int ncut,
-                          float mabs_norm, const double *win, const double2 *tw, double scale, double *X, double2 *Z,
-                          double *out)
+static hipError_t dgemm(hipStream_t s, const double *A, int lda, const double *B, int ldb, double *C, int ldc, int M,
+                        int N, int K)
 {
-    const int N = W - 2 * ncut, M = nc10 + 1 - ncut;
-    if (N < 1 || M < 1 || W != 2 * nc10 + 1) return hipErrorInvalidValue;
-    dim3 g1((N + 255) / 256, M);
-    if (den_is_int) hipLaunchKernelGGL(k_cyl_prep<true>, g1, dim3(256), 0, s, bl, den, W, nc10, ncut, mabs_norm, win, X);
-    else hipLaunchKernelGGL(k_cyl_prep<false>, g1, dim3(256), 0, s, bl, den, W, nc10, ncut, mabs_norm, win, X);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    const size_t lds = sizeof(double) * N + sizeof(double2) * N;
-    hipLaunchKernelGGL(k_cyl_cols, g1, dim3(256), lds, s, X, tw, N, M, Z);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    dim3 g2((N + 255) / 256, N);
-    hipLaunchKernelGGL(k_cyl_rows, g2, dim3(256), 0, s, Z, tw, N, M, scale, out);
+    hipLaunchKernelGGL(k_dgemm, dim3((N + 63) / 64, (M + 63) / 64), dim3(256), 0, s, A, lda, B, ldb, C, ldc, M, N, K);
     return hipGetLastError();
 }
 
@@ -137,12 +180,16 @@ extern "C" int amb_cyl_bispec(amb_ctx *c, int n, int W, const float *bl, const v
     const int nc10 = (W - 1) / 2, ncut = p->ncut;
     const int N = W - 2 * ncut, M = nc10 + 1 - ncut;
     if (ncut < 0 || M < 1) return fail(c, AMB_EARG, "cut %d leaves no lags of W=%d", ncut, W);
-    if (24L * N > 65536) return fail(c, AMB_EARG, "N=%d exceeds the transform's LDS row (2730)", N);
+    if (N > 32768) return fail(c, AMB_EARG, "N=%d exceeds the transform tables", N);
     if (!(p->mabs_norm != 0.0)) return fail(c, AMB_EARG, "mabs_norm is zero");
     const size_t WW = (size_t)W * W, MN = (size_t)M * N, NN = (size_t)N * N;
-    /* device scratch: bl, den, X, Z, out, window, twiddles */
-    const size_t bytes = sizeof(float) * WW * 2 + sizeof(double) * MN + sizeof(double2) * MN + sizeof(double) * NN +
-                         sizeof(double) * MN + sizeof(double2) * N + 8 * 256;
+    /* device scratch: bl, den, X, C1 = X [cos|sin], Z2 = [Zr;Zi], C2, out,
+       window, the two twiddle matrices, the twiddle table */
+    size_t bytes = 0;
+    auto sz = [&](size_t b) { bytes += (b + 255) / 256 * 256; };
+    sz(sizeof(float) * WW); sz(sizeof(float) * WW); sz(sizeof(double) * MN); sz(sizeof(double) * 2 * MN);
+    sz(sizeof(double) * 2 * MN); sz(sizeof(double) * NN); sz(sizeof(double) * NN); sz(sizeof(double) * MN);
+    sz(sizeof(double) * 2 * NN); sz(sizeof(double) * 2 * MN); sz(sizeof(double2) * N);
     void *d = nullptr;
     int rc = dev_alloc(c, &d, bytes, "post-processing scratch");
     if (rc) return rc;
@@ -151,9 +198,13 @@ extern "C" int amb_cyl_bispec(amb_ctx *c, int n, int W, const float *bl, const v
     float *d_bl = (float *)take(sizeof(float) * WW);
     void *d_den = take(sizeof(float) * WW);
     double *d_X = (double *)take(sizeof(double) * MN);
-    double2 *d_Z = (double2 *)take(sizeof(double2) * MN);
+    double *d_C1 = (double *)take(sizeof(double) * 2 * MN);
+    double *d_Z2 = (double *)take(sizeof(double) * 2 * MN);
+    double *d_C2 = (double *)take(sizeof(double) * NN);
     double *d_out = (double *)take(sizeof(double) * NN);
     double *d_win = (double *)take(sizeof(double) * MN);
+    double *d_B1 = (double *)take(sizeof(double) * 2 * NN);
+    double *d_A2 = (double *)take(sizeof(double) * 2 * MN);
     double2 *d_tw = (double2 *)take(sizeof(double2) * N);
     std::vector<double2> tw(N);
     for (int t = 0; t < N; ++t) tw[t] = make_double2(cos(2.0 * M_PI * t / N), sin(2.0 * M_PI * t / N));
@@ -171,21 +222,47 @@ extern "C" int amb_cyl_bispec(amb_ctx *c, int n, int W, const float *bl, const v
     }
     const double L = N * p->dnu * p->rc_prime;
     const double scale = pow(p->rc, 4) * L * L;
-    hipError_t e = hipMemcpyAsync(d_tw, tw.data(), sizeof(double2) * N, hipMemcpyHostToDevice, c->stream);
+    hipStream_t st = c->stream;
+    hipError_t e = hipMemcpyAsync(d_tw, tw.data(), sizeof(double2) * N, hipMemcpyHostToDevice, st);
     if (e == hipSuccess && p->window)
-        e = hipMemcpyAsync(d_win, win.data(), sizeof(double) * MN, hipMemcpyHostToDevice, c->stream);
+        e = hipMemcpyAsync(d_win, win.data(), sizeof(double) * MN, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_cyl_tables, dim3((std::max(N, M) + 255) / 256, N), dim3(256), 0, st, d_tw, N, M, d_B1,
+                           d_A2);
+        e = hipGetLastError();
+    }
+    const dim3 g1((N + 255) / 256, M), g2((N + 255) / 256, N);
     for (int t = 0; t < n && e == hipSuccess; ++t) {
-        e = hipMemcpyAsync(d_bl, bl + WW * t, sizeof(float) * WW, hipMemcpyHostToDevice, c->stream);
+        e = hipMemcpyAsync(d_bl, bl + WW * t, sizeof(float) * WW, hipMemcpyHostToDevice, st);
         if (e == hipSuccess)
             e = hipMemcpyAsync(d_den, (const char *)den + sizeof(float) * WW * t, sizeof(float) * WW,
-                               hipMemcpyHostToDevice, c->stream);
+                               hipMemcpyHostToDevice, st);
+        if (e != hipSuccess) break;
+        const double *win_p = p->window ? d_win : nullptr;
+        if (den_is_int)
+            hipLaunchKernelGGL(k_cyl_prep<true>, g1, dim3(256), 0, st, d_bl, d_den, W, nc10, ncut,
+                               (float)p->mabs_norm, win_p, d_X);
+        else
+            hipLaunchKernelGGL(k_cyl_prep<false>, g1, dim3(256), 0, st, d_bl, d_den, W, nc10, ncut,
+                               (float)p->mabs_norm, win_p, d_X);
+        timer_start(c, &c->st.ms_post);
+        e = dgemm(st, d_X, N, d_B1, 2 * N, d_C1, 2 * N, M, 2 * N, N);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_cyl_stack, g1, dim3(256), 0, st, d_C1, N, M, d_Z2);
+            e = dgemm(st, d_A2, 2 * M, d_Z2, N, d_C2, N, N, N, 2 * M);
+        }
+        timer_stop(c);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_cyl_out, g2, dim3(256), 0, st, d_C2, N, scale, d_out);
+            e = hipGetLastError();
+        }
         if (e == hipSuccess)
-            e = amb_launch_cyl(c->stream, d_bl, d_den, den_is_int, W, nc10, ncut, (float)p->mabs_norm,
-                               p->window ? d_win : nullptr, d_tw, scale, d_X, d_Z, d_out);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(out + NN * t, d_out, sizeof(double) * NN, hipMemcpyDeviceToHost, c->stream);
+            e = hipMemcpyAsync(out + NN * t, d_out, sizeof(double) * NN, hipMemcpyDeviceToHost, st);
     }
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    const hipError_t es = hipStreamSynchronize(st);
+    if (e == hipSuccess) e = es;
+    timers_collect(c);
+    c->st.flop_post += 8.0 * M * (double)N * N * n;
     dev_free(c, d, bytes);
     if (e != hipSuccess) return fail(c, AMB_EHIP, "lag-space post-processing: %s", hipGetErrorString(e));
     return AMB_OK;

@@ -383,16 +383,23 @@ def bench_post(A, args):
     cnt = rng.integers(1, 768, (n, W, W)).astype(np.int32)
     ctx = A.Context(device=0)
     ctx.cyl_bispec(bl[:2], cnt[:2])
+    ctx.reset_stats()
+    steps = max(args.steps, 1)
     t = time.time()
-    for _ in range(max(args.steps, 1)):
+    for _ in range(steps):
         ctx.cyl_bispec(bl, cnt)
-    dt = (time.time() - t) / max(args.steps, 1)
+    dt = (time.time() - t) / steps
+    st = ctx.stats()
     N = W - 2 * A.ANALYSIS["ncut"]
     M = nc10 + 1 - A.ANALYSIS["ncut"]
     fl = 8.0 * M * N * N * n
+    kt = st["flop_post"] / (st["ms_post"] * 1e-3) / 1e12
     line = {"metric": "lag-space post-processing (fold, normalise, window, lag->k_par transform)",
             "value": round(n / dt, 2), "unit": "triangles/s", "triangles": n, "W": W, "N": N,
-            "s_per_batch": round(dt, 4), "fp64_tflops": round(fl / dt / 1e12, 2), "fp64_peak_tflops": 78.6,
+            "s_per_batch": round(dt, 4), "fp64_tflops_wall": round(fl / dt / 1e12, 2),
+            "roofline": {"bound": "mfma", "kernel": "k_dgemm (two per triangle)", "achieved": round(kt, 2),
+                         "peak": 78.6, "unit": "TFLOP/s", "frac": round(kt / 78.6, 4)},
+            "ms_transform_per_triangle": round(st["ms_post"] / steps / n, 3),
             "note": "host->device copy of the W x W arrays and device->host copy of the N x N results included"}
     ctx.close()
     print(json.dumps(line), flush=True)

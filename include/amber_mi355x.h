@@ -413,6 +413,8 @@ typedef struct amb_stats {
     int64_t n_maps_launch;
     double ms_grid;         /* gridding kernels (sums and collapse) */
     int64_t n_grid_groups;  /* groups gridded */
+    double ms_post;         /* lag-space transform matrix products */
+    double flop_post;       /* their algorithmic 8 M N^2 flop per triangle */
 } amb_stats;
 int amb_get_stats(amb_ctx *ctx, amb_stats *st);
 int amb_reset_stats(amb_ctx *ctx);

@@ -56,3 +56,19 @@ def test_cyl_on_pipeline_results(ctx):
             assert np.all(np.isfinite(ref))
             assert _rel(got[t], ref) <= 1e-12
     ctx.release_cube(cube)
+
+
+@pytest.mark.parametrize("window", [False, True])
+def test_cyl_full_size(ctx, window):
+    """the metric workload's lag grid, W = 1535 (nc10 = 767): N = 1519 after
+    the 8-lag cut, two matrix products of K = 1519 and K = 1520 on the FP64
+    matrix cores, against numpy's FFT chain"""
+    rng = np.random.default_rng(21)
+    nc10 = 767
+    W = 2 * nc10 + 1
+    bl = rng.standard_normal((1, W, W)).astype(np.float32) * 1e3
+    den = rng.integers(1, 767, (1, W, W)).astype(np.int32)
+    print("case cyl full size window=%s" % window, flush=True)
+    got = ctx.cyl_bispec(bl, den, window=window)
+    assert got.shape == (1,) + (W - 2 * K["ncut"],) * 2
+    assert _rel(got[0], _ref(bl[0], den[0], nc10, window)) <= 1e-12
