@@ -432,17 +432,29 @@ int prepare_tri(amb_ctx *c, const PlaneSrc &src, int id, int r1, int r2, int r3,
     return AMB_OK;
 }
 
-void account_tri(amb_ctx *c, const TriJob &j, const int32_t *cnt)
-{
-    int64_t ev = 0;
-    double pairs = 0;
+void count_tri(const TriJob &j, const int32_t *cnt, int64_t *evals, int64_t *pairs)
+{   /* integer sums: exact, and free of a floating-point dependency chain */
+    int64_t ev = 0, pr = 0;
     for (size_t i = 0; i < j.cells; ++i) {
         ev += cnt[i] > 0;
-        pairs += cnt[i];
+        pr += cnt[i];
     }
-    c->st.evals += ev;
-    c->st.flop_contract += 2.0 * pairs * (double)j.P;
+    *evals = ev;
+    *pairs = pr;
+}
+
+void account_counts(amb_ctx *c, const TriJob &j, int64_t evals, int64_t pairs)
+{
+    c->st.evals += evals;
+    c->st.flop_contract += 2.0 * (double)pairs * (double)j.P;
     c->st.flop_contract_exec += j.flop_exec;
+}
+
+void account_tri(amb_ctx *c, const TriJob &j, const int32_t *cnt)
+{
+    int64_t ev, pr;
+    count_tri(j, cnt, &ev, &pr);
+    account_counts(c, j, ev, pr);
 }
 
 /* one triangle with its own launches; output pointers are host memory */

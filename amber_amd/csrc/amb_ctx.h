@@ -153,8 +153,11 @@ hipError_t launch_build(amb_ctx *c, hipStream_t s, const BuildOp &op, float2 *G)
 size_t build_scratch_bytes(amb_ctx *c, const BuildOp &op);
 /* resolve a triangle: validate, pin its planes, fill kernel arguments */
 int prepare_tri(amb_ctx *c, const PlaneSrc &src, int cube_id, int r1, int r2, int r3, int want_norm, TriJob &j);
-/* evals and algorithmic flops of one finished triangle from its counts */
+/* evals and algorithmic flops of one finished triangle from its counts;
+ * count_tri is thread-safe, account_counts adds to the context's stats */
 void account_tri(amb_ctx *c, const TriJob &j, const int32_t *cnt);
+void count_tri(const TriJob &j, const int32_t *cnt, int64_t *evals, int64_t *pairs);
+void account_counts(amb_ctx *c, const TriJob &j, int64_t evals, int64_t pairs);
 /* algorithmic HBM bytes of one c2r build (0 for derived kinds) */
 double build_fft_bytes(amb_ctx *c, const BuildOp &op);
 /* one triangle with its own launches, results in host memory */

@@ -187,7 +187,11 @@ hipError_t launch_wave(amb_ctx *c, hipStream_t s, const Wave &w, const WaveBufs 
 /* results of a finished wave from the host copy into the requests */
 int finish_wave(amb_ctx *c, const Wave &w, const char *host, const amb_tri_req *req)
 {
-    for (size_t i = 0; i < w.jobs.size(); ++i) {
+    /* the triangles' copies out of the pinned staging and their count sums
+       are independent: spread them over a few host threads */
+    const size_t nj = w.jobs.size();
+    std::vector<int64_t> ev(nj), pr(nj);
+    auto one = [&](size_t i) {
         const TriJob &j = w.jobs[i];
         const amb_tri_req &q = req[w.req[i]];
         const char *o = host + w.o_off[i];
@@ -196,8 +200,21 @@ int finish_wave(amb_ctx *c, const Wave &w, const char *host, const amb_tri_req *
         if (q.cnt) memcpy(q.cnt, cnt, sizeof(int32_t) * j.cells);
         if (j.want_norm && q.norm)
             memcpy(q.norm, o + al(j.cells * sizeof(float)) + al(j.cells * sizeof(int)), sizeof(float) * j.cells);
-        account_tri(c, j, cnt);
+        count_tri(j, cnt, &ev[i], &pr[i]);
+    };
+    const size_t nth = std::min<size_t>(nj, 8);
+    if (nth <= 1) {
+        for (size_t i = 0; i < nj; ++i) one(i);
+    } else {
+        std::vector<std::thread> th;
+        th.reserve(nth);
+        for (size_t t = 0; t < nth; ++t)
+            th.emplace_back([&, t] {
+                for (size_t i = t; i < nj; i += nth) one(i);
+            });
+        for (auto &x : th) x.join();
     }
+    for (size_t i = 0; i < nj; ++i) account_counts(c, w.jobs[i], ev[i], pr[i]);
     if (w.nnorm) {
         float m;
         memcpy(&m, host + w.o_total, sizeof m);
