@@ -582,8 +582,9 @@ __device__ __forceinline__ void contract_item(const ContractArgs &a, int tile, i
  *     different 4-bank groups: A rows (16 slots, 256 bytes) by the low 4
  *     bits of the row, B rows (8 slots, 128 bytes, so neighbouring rows sit
  *     in opposite bank halves) by bits 1..3 of the row;
- *   - one barrier per chunk, and within a chunk the LDS reads and A
- *     formation of k-step s+1 are interleaved with the MFMAs of k-step s.
+ *   - one barrier per chunk; the LDS reads and A formation of k-step s+1
+ *     are interleaved with the MFMAs of k-step s, across chunk boundaries
+ *     too (the barrier sits before a chunk's last k-step).
  * Rows and columns past nact and planes past nact read clamped (valid)
  * rows; their results fall outside every window and are never stored.
  * ------------------------------------------------------------------------- */
@@ -610,37 +611,57 @@ __device__ __forceinline__ void lds_load16(const void *g, void *l)
     __builtin_amdgcn_global_load_lds(g, (lds_void_t *)l, 16, 0, 0);
 }
 
-/* One chunk into stage s; wave w issues A rows 16w..16w+15 (4 loads), hi and
- * lo columns 16w..16w+15 (2 + 2 loads), wave 0 also the 4 D_r1 rows. */
-__device__ __forceinline__ void v2_issue(V2Stage &s, const ContractArgs &a, long k0, int r0, int c0, int g0,
-                                         int wv, int lane)
+/* A lane's byte offsets into the plane arrays for the loads of one work
+ * item, without the chunk's pixel offset.  Each array holds nact rows of
+ * pitch elements and the launchers run this form only when that is under
+ * 4 GiB (v2_fits), so the offsets are 32-bit: a load is the array base plus
+ * the chunk offset (scalar registers) plus this (one vector register), and
+ * a chunk needs no per-lane address arithmetic. */
+struct V2Off {
+    unsigned a[4], b[2], d;
+};
+
+__device__ __forceinline__ void v2_offsets(V2Off &o, const ContractArgs &a, int r0, int c0, int g0, int wv,
+                                           int lane)
 {
-    const long pitch = a.pitch;
+    const unsigned pitch = (unsigned)a.pitch;
     const int last = a.nact - 1;
-    /* opaque lane: the nine per-lane addresses are recomputed each chunk
-       instead of being hoisted out of the chunk loop into 18 registers */
-    asm volatile("" : "+v"(lane));
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
         const int t = wv * 4 + u;                  /* 1 KB block of the A region */
-        const int r = t * 4 + (lane >> 4), ph = lane & 15;
-        const int c = ph ^ (r & 15);
-        const int gr = min(r0 + r, last);
-        lds_load16(a.D2 + (size_t)gr * pitch + k0 + c * 4, &s.a[t * 256]);
+        const int r = t * 4 + (lane >> 4), c = (lane & 15) ^ (r & 15);
+        o.a[u] = ((unsigned)min(r0 + r, last) * pitch + (unsigned)c * 4u) * 4u;
     }
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
         const int t = wv * 2 + u;
-        const int j = t * 8 + (lane >> 3), ph = lane & 7;
-        const int c = ph ^ ((j >> 1) & 7);
-        const size_t off = (size_t)min(c0 + j, last) * pitch + k0 + c * 8;
-        lds_load16(a.H3 + off, &s.bh[t * 512]);
-        lds_load16(a.L3 + off, &s.bl[t * 512]);
+        const int j = t * 8 + (lane >> 3), c = (lane & 7) ^ ((j >> 1) & 7);
+        o.b[u] = ((unsigned)min(c0 + j, last) * pitch + (unsigned)c * 8u) * 2u;
     }
-    if (wv == 0) {
-        const int q = lane >> 4, c = lane & 15;
-        lds_load16(a.D1 + (size_t)min(g0 + q, last) * pitch + k0 + c * 4, &s.d1[0]);
+    o.d = ((unsigned)min(g0 + (lane >> 4), last) * pitch + (unsigned)(lane & 15) * 4u) * 4u;
+}
+
+/* One chunk into stage s; wave w issues A rows 16w..16w+15 (4 loads), hi and
+ * lo columns 16w..16w+15 (2 + 2 loads), wave 0 also the 4 D_r1 rows. */
+__device__ __forceinline__ void v2_issue(V2Stage &s, const ContractArgs &a, long k0, const V2Off &o, int wv)
+{
+    const char *pa = (const char *)a.D2 + k0 * 4, *pd = (const char *)a.D1 + k0 * 4;
+    const char *ph = (const char *)a.H3 + k0 * 2, *pl = (const char *)a.L3 + k0 * 2;
+    /* opaque scalar bases and offsets: otherwise the chunk loop's strength
+       reduction turns base + offset into 64-bit vector induction variables,
+       or the offsets are hoisted as 64-bit values; either way the loads
+       lose the scalar-base form and the registers spill */
+    asm volatile("" : "+s"(pa), "+s"(pd), "+s"(ph), "+s"(pl));
+    V2Off x = o;
+    asm volatile("" : "+v"(x.a[0]), "+v"(x.a[1]), "+v"(x.a[2]), "+v"(x.a[3]), "+v"(x.b[0]), "+v"(x.b[1]), "+v"(x.d));
+#pragma unroll
+    for (int u = 0; u < 4; ++u) lds_load16(pa + x.a[u], &s.a[(wv * 4 + u) * 256]);
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        lds_load16(ph + x.b[u], &s.bh[(wv * 2 + u) * 512]);
+        lds_load16(pl + x.b[u], &s.bl[(wv * 2 + u) * 512]);
     }
+    if (wv == 0) lds_load16(pd + x.d, &s.d1[0]);
 }
 
 /* fragments of k-step st (16 pixels) of a chunk */
@@ -750,7 +771,9 @@ __device__ __forceinline__ void v2_mfma_one(const F2 &f, f32x16 (&acc)[2][4], in
  * MFMAs to land), after MFMA 16 (the current lo halves are dead by then)
  * the 8 reads of the next B side.  sched_barrier keeps the compiler from
  * clustering the MFMAs again, so a wave's VALU and LDS work runs beside its
- * own MFMAs. */
+ * own MFMAs.  The empty asm consumes the split here: otherwise, where the
+ * next k-step lies beyond a branch (the segment flush), the split is sunk
+ * past it and runs with no MFMA beside it. */
 __device__ __forceinline__ void v2_step(const F2 &cur, f32x16 (&acc)[2][4], const V2Stage &s, int st_next, int q,
                                         int rb, int lane, F2 &nxt)
 {
@@ -764,6 +787,7 @@ __device__ __forceinline__ void v2_step(const F2 &cur, f32x16 (&acc)[2][4], cons
         if (m == 16) v2_read_b(s, st_next, lane, nxt);
         __builtin_amdgcn_sched_barrier(0);
     }
+    asm volatile("" : "+v"(hp[0]), "+v"(hp[1]), "+v"(lp[0]), "+v"(lp[1]));
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
         nxt.ah[i] = __builtin_bit_cast(bf16x8, hp[i]);
@@ -782,7 +806,8 @@ __device__ __forceinline__ void contract_item_v2(const ContractArgs &a, int tile
     if (r0 >= hi || r0 + TT <= lo || c0 >= hi || c0 + TT <= lo) return;
     if (a.sym12 && r0 + TT <= g0) return;
 
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    /* the wave index is uniform across the wave: branches on it are scalar */
+    const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int q = wv & 3, rb = (wv >> 2) * 64;
     const bool live = g0 + q < nact;
     f32x16 acc[2][4];
@@ -795,26 +820,53 @@ __device__ __forceinline__ void contract_item_v2(const ContractArgs &a, int tile
 
     const long nchunk = a.pitch / KC2;
     constexpr int SEG2 = SEGC * KC / KC2; /* the same 8192-pixel segments */
-    v2_issue(L.st[0], a, 0, r0, c0, g0, wv, lane);
-    for (long c = 0; c < nchunk; ++c) {
-        /* chunk c has landed for this wave; after the barrier, for every
-           wave, and every wave is done reading the other stage */
-        __builtin_amdgcn_s_waitcnt(0x0F70); /* vmcnt(0) */
-        __syncthreads();
-        if (c + 1 < nchunk) v2_issue(L.st[(c + 1) & 1], a, (c + 1) * KC2, r0, c0, g0, wv, lane);
-        const V2Stage &S = L.st[c & 1];
-        if (live) {
-            F2 f0, f1;
-            v2_frag(S, 0, q, rb, lane, f0);
-            v2_step(f0, acc, S, 1, q, rb, lane, f1);
-            v2_step(f1, acc, S, 2, q, rb, lane, f0);
-            v2_step(f0, acc, S, 3, q, rb, lane, f1);
-            v2_mfma(f1, acc);
-            if ((c + 1) % SEG2 == 0 && c + 1 < nchunk)
-                flush_acc<AMB_PREC_BF16X3, 2>(a, acc, c + 1 == SEG2, g0 + q, r0, c0, rb, lane, 1.f, L.rsc, L.csc);
+    /* The k-steps run as one stream across chunks: the MFMAs of a chunk's
+       last k-step carry the LDS reads and A formation of the next chunk's
+       first, so no k-step starts without operands.  The barrier therefore
+       sits before that last k-step: by then every wave has read the
+       chunk's LDS data (step 3 operands are in flight to registers) and
+       waited for its loads of the next chunk, which was issued one
+       barrier earlier; after it the stage just read takes chunk c + 2. */
+    V2Off o;
+    v2_offsets(o, a, r0, c0, g0, wv, lane);
+    v2_issue(L.st[0], a, 0, o, wv);
+    __builtin_amdgcn_s_waitcnt(0x0F70); /* vmcnt(0) */
+    __syncthreads();
+    if (nchunk > 1) v2_issue(L.st[1], a, KC2, o, wv);
+    if (!live) {
+        /* no plane for this wave: it loads its share and keeps the barriers */
+        for (long c = 0; c + 1 < nchunk; ++c) {
+            __builtin_amdgcn_s_waitcnt(0x0070); /* vmcnt(0) lgkmcnt(0) */
+            __syncthreads();
+            if (c + 2 < nchunk) v2_issue(L.st[c & 1], a, (c + 2) * KC2, o, wv);
         }
+        return;
     }
-    if (live) flush_acc<AMB_PREC_BF16X3, 2>(a, acc, nchunk <= SEG2, g0 + q, r0, c0, rb, lane, 1.f, L.rsc, L.csc);
+    /* Straight-line loop body with one carried fragment set (f0) and the
+       last chunk peeled: conditions on the wave or on the last chunk inside
+       the loop made the compiler spill the accumulators. */
+    F2 f0, f1;
+    v2_frag(L.st[0], 0, q, rb, lane, f0);
+    for (long c = 0; c + 1 < nchunk; ++c) {
+        const V2Stage &S = L.st[c & 1];
+        v2_step(f0, acc, S, 1, q, rb, lane, f1);
+        v2_step(f1, acc, S, 2, q, rb, lane, f0);
+        v2_step(f0, acc, S, 3, q, rb, lane, f1);
+        /* this wave's reads of stage c & 1 and loads of chunk c + 1 are
+           complete; after the barrier, every wave's are */
+        __builtin_amdgcn_s_waitcnt(0x0070); /* vmcnt(0) lgkmcnt(0) */
+        __syncthreads();
+        if (c + 2 < nchunk) v2_issue(L.st[c & 1], a, (c + 2) * KC2, o, wv);
+        v2_step(f1, acc, L.st[(c + 1) & 1], 0, q, rb, lane, f0);
+        if ((c + 1) % SEG2 == 0)
+            flush_acc<AMB_PREC_BF16X3, 2>(a, acc, c + 1 == SEG2, g0 + q, r0, c0, rb, lane, 1.f, L.rsc, L.csc);
+    }
+    const V2Stage &S = L.st[(nchunk - 1) & 1];
+    v2_step(f0, acc, S, 1, q, rb, lane, f1);
+    v2_step(f1, acc, S, 2, q, rb, lane, f0);
+    v2_step(f0, acc, S, 3, q, rb, lane, f1);
+    v2_mfma(f1, acc);
+    flush_acc<AMB_PREC_BF16X3, 2>(a, acc, nchunk <= SEG2, g0 + q, r0, c0, rb, lane, 1.f, L.rsc, L.csc);
 }
 
 __global__ __launch_bounds__(512, 1) void k_contract_v2(ContractArgs a, int ngroups, int per_xcd, int nwork)
@@ -976,9 +1028,16 @@ static void launch_grouped_var(int var, dim3 grid, hipStream_t s, const Contract
     else hipLaunchKernelGGL((k_contract_grouped<MODE, NORM, 2, 0>), grid, dim3(512), 0, s, d, b, n, ctr);
 }
 
+bool amb_contract_v2_fits(int nact, long pitch)
+{   /* the 32-bit byte offsets of V2Off: a plane array of nact rows of pitch
+       FP32 values must lie within 4 GiB */
+    return nact > 0 && pitch > 0 && (unsigned long long)nact * (unsigned long long)pitch * 4ull <= 0xFFFFFFFFull;
+}
+
 template <int MODE, int NORM>
 static void launch_var(int var, dim3 grid, hipStream_t s, const ContractArgs &a, int ngroups, int per_xcd, int nwork)
 {
+    if (var == 2 && !amb_contract_v2_fits(a.nact, a.pitch)) var = 0;
     if (var == 2 && MODE == AMB_PREC_BF16X3 && NORM == 0)
         hipLaunchKernelGGL(k_contract_v2, grid, dim3(512), 0, s, a, ngroups, per_xcd, nwork);
     else if (var == 1) hipLaunchKernelGGL((k_contract<MODE, NORM, 2, 1>), grid, dim3(512), 0, s, a, ngroups, per_xcd, nwork);

@@ -52,6 +52,9 @@ struct ContractArgs {
  * MFMA with the next block's A formation, 2: BF16x3 data form with direct
  * global->LDS loads and k-step pipelining; other forms run variant 0) */
 #define AMB_NVAR 3
+/* variant 2 addresses each plane array with 32-bit byte offsets: true when
+ * nact rows of pitch FP32 values fit in 4 GiB; otherwise run variant 0 */
+bool amb_contract_v2_fits(int nact, long pitch);
 hipError_t amb_launch_contract(hipStream_t s, int prec, int norm, int var, const ContractArgs &a);
 hipError_t amb_launch_split(hipStream_t s, const float *D, size_t n, __bf16 *H, __bf16 *L);
 /* FP8 preparation of n planes: sc[0..n) A-side scales, sc[n..2n) B-side

@@ -159,8 +159,11 @@ hipError_t launch_wave(amb_ctx *c, hipStream_t s, const Wave &w, const WaveBufs 
     const int prec = c->prec, nprec = prec == AMB_PREC_FP32 ? AMB_PREC_FP32 : AMB_PREC_BF16X3;
     unsigned long long *ctr = (unsigned long long *)(b.desc + w.off_ctr);
     unsigned *mg = (unsigned *)(b.desc + w.off_mg);
+    int var = c->kvar;
+    for (const TriJob &j : w.jobs)
+        if (var == 2 && !amb_contract_v2_fits(j.nact, j.pitch)) var = 0;
     if (timed) timer_start(c, &c->st.ms_contract);
-    hipError_t e = amb_launch_contract_grouped(s, prec, 0, c->kvar, (const ContractArgs *)(b.desc + w.off_da),
+    hipError_t e = amb_launch_contract_grouped(s, prec, 0, var, (const ContractArgs *)(b.desc + w.off_da),
                                                (const long *)(b.desc + w.off_b), n, w.items, ctr,
                                                c->grouped_blocks);
     if (timed) timer_stop(c);
