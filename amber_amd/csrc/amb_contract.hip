@@ -591,15 +591,12 @@ __device__ __forceinline__ void contract_item(const ContractArgs &a, int tile, i
 #define KC2 64
 typedef __attribute__((address_space(3))) void lds_void_t;
 
-struct A16 V2Stage {
-    float a[TT * KC2];     /* D_r2 row r, float4 slot c at r*64 + 4*(c ^ (r & 15)) */
-    __bf16 bh[TT * KC2];   /* D_r3 hi col j, 16-byte slot c at j*64 + 8*(c ^ ((j >> 1) & 7)) */
-    __bf16 bl[TT * KC2];   /* D_r3 lo, same layout */
-    float d1[QP * KC2];    /* D_r1 of the 4 planes, plain rows */
-};
-struct A16 V2Smem {
-    V2Stage st[2];
-    float rsc[TT], csc[TT]; /* unused by this form; flush_acc takes them */
+/* One LDS stage: views of separate __shared__ arrays (see the kernels) */
+struct V2Stage {
+    float *a;    /* D_r2 row r, float4 slot c at r*64 + 4*(c ^ (r & 15)) */
+    __bf16 *bh;  /* D_r3 hi col j, 16-byte slot c at j*64 + 8*(c ^ ((j >> 1) & 7)) */
+    __bf16 *bl;  /* D_r3 lo, same layout */
+    float *d1;   /* D_r1 of the 4 planes, plain rows */
 };
 
 struct F2 {
@@ -611,57 +608,44 @@ __device__ __forceinline__ void lds_load16(const void *g, void *l)
     __builtin_amdgcn_global_load_lds(g, (lds_void_t *)l, 16, 0, 0);
 }
 
-/* A lane's byte offsets into the plane arrays for the loads of one work
- * item, without the chunk's pixel offset.  Each array holds nact rows of
- * pitch elements and the launchers run this form only when that is under
- * 4 GiB (v2_fits), so the offsets are 32-bit: a load is the array base plus
- * the chunk offset (scalar registers) plus this (one vector register), and
- * a chunk needs no per-lane address arithmetic. */
-struct V2Off {
-    unsigned a[4], b[2], d;
-};
-
-__device__ __forceinline__ void v2_offsets(V2Off &o, const ContractArgs &a, int r0, int c0, int g0, int wv,
-                                           int lane)
+/* One chunk into stage s; wave w issues A rows 16w..16w+15 (4 loads), hi and
+ * lo columns 16w..16w+15 (2 + 2 loads), wave 0 also the 4 D_r1 rows.  Each
+ * array holds nact rows of pitch elements and the launchers run this form
+ * only when that is under 4 GiB (v2_fits), so a load is the array base plus
+ * the chunk offset (scalar registers) plus a 32-bit lane offset, formed
+ * with 24-bit multiplies (row < 2^24, pitch < 2^24). */
+__device__ __forceinline__ void v2_issue(const V2Stage &s, const ContractArgs &a, long k0, int r0, int c0, int g0,
+                                         int wv, int lane)
 {
+    const char *pa = (const char *)a.D2 + k0 * 4, *pd = (const char *)a.D1 + k0 * 4;
+    const char *ph = (const char *)a.H3 + k0 * 2, *pl = (const char *)a.L3 + k0 * 2;
+    /* opaque scalar bases and lane: otherwise the chunk loop's strength
+       reduction or hoisting keeps per-load 64-bit addresses in vector
+       registers across the loop, and they spill */
+    asm volatile("" : "+s"(pa), "+s"(pd), "+s"(ph), "+s"(pl));
+    asm volatile("" : "+v"(lane));
     const unsigned pitch = (unsigned)a.pitch;
     const int last = a.nact - 1;
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
         const int t = wv * 4 + u;                  /* 1 KB block of the A region */
         const int r = t * 4 + (lane >> 4), c = (lane & 15) ^ (r & 15);
-        o.a[u] = ((unsigned)min(r0 + r, last) * pitch + (unsigned)c * 4u) * 4u;
+        const unsigned off = (__umul24((unsigned)min(r0 + r, last), pitch) + (unsigned)c * 4u) * 4u;
+        lds_load16(pa + off, &s.a[t * 256]);
     }
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
         const int t = wv * 2 + u;
         const int j = t * 8 + (lane >> 3), c = (lane & 7) ^ ((j >> 1) & 7);
-        o.b[u] = ((unsigned)min(c0 + j, last) * pitch + (unsigned)c * 8u) * 2u;
+        const unsigned off = (__umul24((unsigned)min(c0 + j, last), pitch) + (unsigned)c * 8u) * 2u;
+        lds_load16(ph + off, &s.bh[t * 512]);
+        lds_load16(pl + off, &s.bl[t * 512]);
     }
-    o.d = ((unsigned)min(g0 + (lane >> 4), last) * pitch + (unsigned)(lane & 15) * 4u) * 4u;
-}
-
-/* One chunk into stage s; wave w issues A rows 16w..16w+15 (4 loads), hi and
- * lo columns 16w..16w+15 (2 + 2 loads), wave 0 also the 4 D_r1 rows. */
-__device__ __forceinline__ void v2_issue(V2Stage &s, const ContractArgs &a, long k0, const V2Off &o, int wv)
-{
-    const char *pa = (const char *)a.D2 + k0 * 4, *pd = (const char *)a.D1 + k0 * 4;
-    const char *ph = (const char *)a.H3 + k0 * 2, *pl = (const char *)a.L3 + k0 * 2;
-    /* opaque scalar bases and offsets: otherwise the chunk loop's strength
-       reduction turns base + offset into 64-bit vector induction variables,
-       or the offsets are hoisted as 64-bit values; either way the loads
-       lose the scalar-base form and the registers spill */
-    asm volatile("" : "+s"(pa), "+s"(pd), "+s"(ph), "+s"(pl));
-    V2Off x = o;
-    asm volatile("" : "+v"(x.a[0]), "+v"(x.a[1]), "+v"(x.a[2]), "+v"(x.a[3]), "+v"(x.b[0]), "+v"(x.b[1]), "+v"(x.d));
-#pragma unroll
-    for (int u = 0; u < 4; ++u) lds_load16(pa + x.a[u], &s.a[(wv * 4 + u) * 256]);
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-        lds_load16(ph + x.b[u], &s.bh[(wv * 2 + u) * 512]);
-        lds_load16(pl + x.b[u], &s.bl[(wv * 2 + u) * 512]);
+    if (wv == 0) {
+        const unsigned row = (unsigned)min(g0 + (lane >> 4), last);
+        const unsigned off = (__umul24(row, pitch) + (unsigned)(lane & 15) * 4u) * 4u;
+        lds_load16(pd + off, &s.d1[0]);
     }
-    if (wv == 0) lds_load16(pd + x.d, &s.d1[0]);
 }
 
 /* fragments of k-step st (16 pixels) of a chunk */
@@ -795,8 +779,48 @@ __device__ __forceinline__ void v2_step(const F2 &cur, f32x16 (&acc)[2][4], cons
     }
 }
 
+/* Chunk c of a work item, not its last: k-steps 1..3 from Sc, the barrier,
+ * chunk c + 2 into Sc, then k-step 3's MFMAs with k-step 0 of chunk c + 1
+ * from Sn. */
+__device__ __forceinline__ void v2_chunk(const ContractArgs &a, long c, long nchunk, const V2Stage &Sc,
+                                         const V2Stage &Sn, F2 &f0, F2 &f1, f32x16 (&acc)[2][4], int r0, int c0,
+                                         int g0, int wv, int q, int rb, int lane)
+{
+    constexpr int SEG2 = SEGC * KC / KC2; /* the same 8192-pixel segments */
+    v2_step(f0, acc, Sc, 1, q, rb, lane, f1);
+    v2_step(f1, acc, Sc, 2, q, rb, lane, f0);
+    v2_step(f0, acc, Sc, 3, q, rb, lane, f1);
+    /* this wave's reads of Sc and loads of chunk c + 1 are complete; after
+       the barrier, every wave's are */
+    __builtin_amdgcn_s_waitcnt(0x0070); /* vmcnt(0) lgkmcnt(0) */
+    __syncthreads();
+    if (c + 2 < nchunk) v2_issue(Sc, a, (c + 2) * KC2, r0, c0, g0, wv, lane);
+    v2_step(f1, acc, Sn, 0, q, rb, lane, f0);
+    if ((c + 1) % SEG2 == 0)
+        flush_acc<AMB_PREC_BF16X3, 2>(a, acc, c + 1 == SEG2, g0 + q, r0, c0, rb, lane, 1.f, nullptr, nullptr);
+}
+
+/* The last chunk from S, and the final flush. */
+__device__ __forceinline__ void v2_last(const ContractArgs &a, long nchunk, const V2Stage &S, F2 &f0, F2 &f1,
+                                        f32x16 (&acc)[2][4], int r0, int c0, int g0, int q, int rb, int lane)
+{
+    constexpr int SEG2 = SEGC * KC / KC2;
+    v2_step(f0, acc, S, 1, q, rb, lane, f1);
+    v2_step(f1, acc, S, 2, q, rb, lane, f0);
+    v2_step(f0, acc, S, 3, q, rb, lane, f1);
+    v2_mfma(f1, acc);
+    flush_acc<AMB_PREC_BF16X3, 2>(a, acc, nchunk <= SEG2, g0 + q, r0, c0, rb, lane, 1.f, nullptr, nullptr);
+}
+
+/* The two LDS stages are separate objects and every access names one of
+ * them (the chunk loop is unrolled by two): the compiler can then tell that
+ * the direct loads into one stage do not alias reads of the other and does
+ * not wait for the loads before those reads.  With one array indexed by
+ * chunk parity it waits for all loads before the first read after them, so
+ * a chunk's prefetch would land before any further MFMA issues. */
 template <int NORM>
-__device__ __forceinline__ void contract_item_v2(const ContractArgs &a, int tile, int grp, V2Smem &L)
+__device__ __forceinline__ void contract_item_v2(const ContractArgs &a, int tile, int grp, const V2Stage &S0,
+                                                 const V2Stage &S1)
 {
     const int nact = a.nact;
     const int nt3 = (nact + TT - 1) / TT;
@@ -819,26 +843,24 @@ __device__ __forceinline__ void contract_item_v2(const ContractArgs &a, int tile
             for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.f;
 
     const long nchunk = a.pitch / KC2;
-    constexpr int SEG2 = SEGC * KC / KC2; /* the same 8192-pixel segments */
     /* The k-steps run as one stream across chunks: the MFMAs of a chunk's
        last k-step carry the LDS reads and A formation of the next chunk's
        first, so no k-step starts without operands.  The barrier therefore
        sits before that last k-step: by then every wave has read the
        chunk's LDS data (step 3 operands are in flight to registers) and
        waited for its loads of the next chunk, which was issued one
-       barrier earlier; after it the stage just read takes chunk c + 2. */
-    V2Off o;
-    v2_offsets(o, a, r0, c0, g0, wv, lane);
-    v2_issue(L.st[0], a, 0, o, wv);
+       barrier earlier; after it the stage just read takes chunk c + 2.
+       Chunk c lives in stage c & 1. */
+    v2_issue(S0, a, 0, r0, c0, g0, wv, lane);
     __builtin_amdgcn_s_waitcnt(0x0F70); /* vmcnt(0) */
     __syncthreads();
-    if (nchunk > 1) v2_issue(L.st[1], a, KC2, o, wv);
+    if (nchunk > 1) v2_issue(S1, a, KC2, r0, c0, g0, wv, lane);
     if (!live) {
         /* no plane for this wave: it loads its share and keeps the barriers */
         for (long c = 0; c + 1 < nchunk; ++c) {
             __builtin_amdgcn_s_waitcnt(0x0070); /* vmcnt(0) lgkmcnt(0) */
             __syncthreads();
-            if (c + 2 < nchunk) v2_issue(L.st[c & 1], a, (c + 2) * KC2, o, wv);
+            if (c + 2 < nchunk) v2_issue((c & 1) ? S1 : S0, a, (c + 2) * KC2, r0, c0, g0, wv, lane);
         }
         return;
     }
@@ -846,44 +868,38 @@ __device__ __forceinline__ void contract_item_v2(const ContractArgs &a, int tile
        last chunk peeled: conditions on the wave or on the last chunk inside
        the loop made the compiler spill the accumulators. */
     F2 f0, f1;
-    v2_frag(L.st[0], 0, q, rb, lane, f0);
-    for (long c = 0; c + 1 < nchunk; ++c) {
-        const V2Stage &S = L.st[c & 1];
-        v2_step(f0, acc, S, 1, q, rb, lane, f1);
-        v2_step(f1, acc, S, 2, q, rb, lane, f0);
-        v2_step(f0, acc, S, 3, q, rb, lane, f1);
-        /* this wave's reads of stage c & 1 and loads of chunk c + 1 are
-           complete; after the barrier, every wave's are */
-        __builtin_amdgcn_s_waitcnt(0x0070); /* vmcnt(0) lgkmcnt(0) */
-        __syncthreads();
-        if (c + 2 < nchunk) v2_issue(L.st[c & 1], a, (c + 2) * KC2, o, wv);
-        v2_step(f1, acc, L.st[(c + 1) & 1], 0, q, rb, lane, f0);
-        if ((c + 1) % SEG2 == 0)
-            flush_acc<AMB_PREC_BF16X3, 2>(a, acc, c + 1 == SEG2, g0 + q, r0, c0, rb, lane, 1.f, L.rsc, L.csc);
+    v2_frag(S0, 0, q, rb, lane, f0);
+    long c = 0;
+    for (; c + 2 < nchunk; c += 2) {
+        v2_chunk(a, c, nchunk, S0, S1, f0, f1, acc, r0, c0, g0, wv, q, rb, lane);
+        v2_chunk(a, c + 1, nchunk, S1, S0, f0, f1, acc, r0, c0, g0, wv, q, rb, lane);
     }
-    const V2Stage &S = L.st[(nchunk - 1) & 1];
-    v2_step(f0, acc, S, 1, q, rb, lane, f1);
-    v2_step(f1, acc, S, 2, q, rb, lane, f0);
-    v2_step(f0, acc, S, 3, q, rb, lane, f1);
-    v2_mfma(f1, acc);
-    flush_acc<AMB_PREC_BF16X3, 2>(a, acc, nchunk <= SEG2, g0 + q, r0, c0, rb, lane, 1.f, L.rsc, L.csc);
+    if (c + 1 < nchunk) v2_chunk(a, c, nchunk, S0, S1, f0, f1, acc, r0, c0, g0, wv, q, rb, lane);
+    v2_last(a, nchunk, ((nchunk - 1) & 1) ? S1 : S0, f0, f1, acc, r0, c0, g0, q, rb, lane);
 }
+
+/* The two stages: A rows, hi then lo B rows, D_r1 rows, each its own array */
+#define V2_STAGES                                                                                                  \
+    __shared__ A16 float sA0[TT * KC2], sA1[TT * KC2];                                                             \
+    __shared__ A16 __bf16 sB0[2 * TT * KC2], sB1[2 * TT * KC2];                                                    \
+    __shared__ A16 float sD0[QP * KC2], sD1[QP * KC2];                                                             \
+    const V2Stage S0{sA0, sB0, sB0 + TT * KC2, sD0}, S1{sA1, sB1, sB1 + TT * KC2, sD1};
 
 __global__ __launch_bounds__(512, 1) void k_contract_v2(ContractArgs a, int ngroups, int per_xcd, int nwork)
 {
-    __shared__ V2Smem L;
+    V2_STAGES
     const int b = blockIdx.x;
     const int slot = b >> 3;
     const int w = (b & 7) * per_xcd + slot;
     if (slot >= per_xcd || w >= nwork) return;
-    contract_item_v2<0>(a, w / ngroups, w % ngroups, L);
+    contract_item_v2<0>(a, w / ngroups, w % ngroups, S0, S1);
 }
 
 __global__ __launch_bounds__(512, 1) void k_contract_grouped_v2(const ContractArgs *__restrict__ descs,
                                                                 const long *__restrict__ begin, int ntri,
                                                                 unsigned long long *counter)
 {
-    __shared__ V2Smem L;
+    V2_STAGES
     __shared__ long item;
     const long total = begin[ntri];
     for (;;) {
@@ -901,7 +917,7 @@ __global__ __launch_bounds__(512, 1) void k_contract_grouped_v2(const ContractAr
         const ContractArgs &a = descs[lo];
         const int ngroups = (a.nact + QP - 1) / QP;
         const long r = w - begin[lo];
-        contract_item_v2<0>(a, (int)(r / ngroups), (int)(r % ngroups), L);
+        contract_item_v2<0>(a, (int)(r / ngroups), (int)(r % ngroups), S0, S1);
         __syncthreads();
     }
 }
@@ -1029,9 +1045,11 @@ static void launch_grouped_var(int var, dim3 grid, hipStream_t s, const Contract
 }
 
 bool amb_contract_v2_fits(int nact, long pitch)
-{   /* the 32-bit byte offsets of V2Off: a plane array of nact rows of pitch
-       FP32 values must lie within 4 GiB */
-    return nact > 0 && pitch > 0 && (unsigned long long)nact * (unsigned long long)pitch * 4ull <= 0xFFFFFFFFull;
+{   /* the 32-bit lane offsets of v2_issue: a plane array of nact rows of
+       pitch FP32 values lies within 4 GiB, and rows and pitch fit the
+       24-bit multiplies */
+    return nact > 0 && pitch > 0 && pitch < (1l << 24) &&
+           (unsigned long long)nact * (unsigned long long)pitch * 4ull <= 0xFFFFFFFFull;
 }
 
 template <int MODE, int NORM>
