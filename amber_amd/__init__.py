@@ -37,7 +37,7 @@ EXPORTS = [
     "amb_graph_destroy", "amb_serve_start", "amb_submit", "amb_poll", "amb_wait", "amb_serve_stop",
     "amb_cyl_bispec", "amb_maps_cldnu", "amb_write_cldnu", "amb_read_params", "amb_write_bsw", "amb_read_bsw",
     "amb_read_bsout", "amb_crc32", "amb_read_vcg_inputs", "amb_grid_setup", "amb_grid_cells", "amb_grid",
-    "amb_grid_cube",
+    "amb_grid_cube", "amb_set_cert_strict",
 ]
 
 BATCH_PER_TRIANGLE, BATCH_GROUPED = 0, 1
@@ -92,7 +92,8 @@ class Stats(C.Structure):
                 ("ms_graph", C.c_double), ("n_graph_launch", C.c_int64),
                 ("ms_maps", C.c_double), ("ms_maps_total", C.c_double), ("flop_maps", C.c_double),
                 ("n_maps_launch", C.c_int64), ("ms_grid", C.c_double), ("n_grid_groups", C.c_int64),
-                ("ms_post", C.c_double), ("flop_post", C.c_double)]
+                ("ms_post", C.c_double), ("flop_post", C.c_double), ("cert_margin_max", C.c_double),
+                ("n_uncertified", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -158,6 +159,7 @@ _lib.amb_grid.argtypes = [C.c_void_p, C.POINTER(GridCfg), C.c_int64, _fp, _fp, _
 _lib.amb_grid_cube.argtypes = [C.c_void_p, C.POINTER(GridCfg), C.c_int64, _fp, _fp, _ip, C.POINTER(C.c_int64)]
 _lib.amb_get_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
 _lib.amb_reset_stats.argtypes = [C.c_void_p]
+_lib.amb_set_cert_strict.argtypes = [C.c_void_p, C.c_int]
 _lib.amb_uavg.argtypes = [_dp, _dp, C.c_int, _dp]
 _lib.amb_triangles.argtypes = [_dp, C.c_int, _ip, C.c_int]
 _lib.amb_annular_rings.argtypes = [C.c_int, _dp, _dp]
@@ -668,6 +670,12 @@ class Context:
 
     def reset_stats(self):
         self._c(_lib.amb_reset_stats(self._h))
+
+    def set_cert_strict(self, strict=True):
+        """v2 normalisation sums not certified exact (FP32 accumulation error
+        past 1/32 of their 1/8 grid): counted in stats() (default) or, when
+        strict, an AmberError"""
+        self._c(_lib.amb_set_cert_strict(self._h, int(bool(strict))))
 
 
 class Graph:

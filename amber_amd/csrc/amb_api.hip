@@ -450,6 +450,18 @@ void account_counts(amb_ctx *c, const TriJob &j, int64_t evals, int64_t pairs)
     c->st.flop_contract_exec += j.flop_exec;
 }
 
+int check_cert(amb_ctx *c, float m)
+{
+    if (!(m < AMB_CERT_RANGE)) /* also NaN */
+        return fail(c, AMB_ECERT, "normalisation sum beyond the int32 range of eighths");
+    if (m > c->st.cert_margin_max) c->st.cert_margin_max = m;
+    if (m > 1.0f / 32.0f) {
+        ++c->st.n_uncertified;
+        if (c->cert_strict) return fail(c, AMB_ECERT, "normalisation sums off their 1/8 grid by %g", (double)m);
+    }
+    return AMB_OK;
+}
+
 void account_tri(amb_ctx *c, const TriJob &j, const int32_t *cnt)
 {
     int64_t ev, pr;
@@ -532,8 +544,7 @@ int bispec_core(amb_ctx *c, int id, int r1, int r2, int r3, int want_norm, amb_r
         float m;
         memcpy(&m, &mbits, sizeof m);
         res->cert_margin = m;
-        if (m > 1.0f / 32.0f)
-            return fail(c, AMB_ECERT, "normalisation sums off their 1/8 grid by %g", (double)m);
+        return check_cert(c, m);
     }
     return AMB_OK;
 }
@@ -1002,6 +1013,14 @@ int amb_reset_stats(amb_ctx *c)
     if (!c) return AMB_EARG;
     Lock lk(c->mu);
     c->st = amb_stats{};
+    return AMB_OK;
+}
+
+int amb_set_cert_strict(amb_ctx *c, int strict)
+{
+    if (!c) return AMB_EARG;
+    Lock lk(c->mu);
+    c->cert_strict = strict ? 1 : 0;
     return AMB_OK;
 }
 

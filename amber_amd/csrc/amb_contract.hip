@@ -447,6 +447,42 @@ __device__ __forceinline__ void flush_acc(const ContractArgs &a, f32x16 (&acc)[R
     if (first) __builtin_amdgcn_s_waitcnt(0x0F70); /* vmcnt(0) */
 }
 
+/* Normalisation epilogue: the whole-pixel-range sum of reference plane i1
+ * (never segmented: it must be whole to be certified) rounded to the
+ * nearest 1/8 and stored as an integer count of eighths; the worst distance
+ * from that grid goes to the margin word. */
+template <int RB>
+__device__ __forceinline__ void norm_store(const ContractArgs &a, f32x16 (&acc)[RB][4], int i1, int r0, int c0,
+                                           int rb, int lane)
+{
+    const int nact = a.nact, lo = a.wlo[i1], hi = a.whi[i1];
+    const double invP = 1.0 / (double)a.P;
+    float worst = 0.f;
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int li = rb + i * 32 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+                const int lj = j * 32 + (lane & 31);
+                const int gi = r0 + li, gj = c0 + lj;
+                if (gi < lo || gi >= hi || gj < lo || gj >= hi) continue;
+                const size_t idx = ((size_t)i1 * nact + gi) * nact + gj;
+                double val = (double)acc[i][j][v] * invP;
+                double q8 = rint(val * 8.0);
+                if (fabs(q8) < 2147483647.0) {
+                    worst = fmaxf(worst, (float)fabs(val - q8 * 0.125));
+                    a.Cq[idx] = (int)q8;
+                } else { /* not representable: flagged, stored clamped */
+                    worst = AMB_CERT_RANGE;
+                    a.Cq[idx] = q8 > 0 ? 2147483647 : -2147483647;
+                }
+            }
+    /* non-negative floats order like their bit patterns */
+    if (worst > 0.f) atomicMax(a.margin, __float_as_uint(worst));
+}
+
 /* One work item: the (tile, group) product of one triangle. */
 template <int MODE, int NORM, int NWR, int PIPE>
 __device__ __forceinline__ void contract_item(const ContractArgs &a, int tile, int grp,
@@ -538,35 +574,8 @@ __device__ __forceinline__ void contract_item(const ContractArgs &a, int tile, i
     if (!live) return;
 
     const int i1 = g0 + q;
-    if constexpr (!NORM) {
-        flush_acc<MODE, RB>(a, acc, nchunk <= SEGC, i1, r0, c0, rb, lane, s1, L.rsc, L.csc);
-        return;
-    }
-    /* normalisation epilogue (the sum must be whole to be certified) */
-    lo = a.wlo[i1];
-    hi = a.whi[i1];
-    const double invP = 1.0 / (double)a.P;
-    float worst = 0.f;
-#pragma unroll
-    for (int i = 0; i < RB; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-#pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                const int li = rb + i * 32 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
-                const int lj = j * 32 + (lane & 31);
-                const int gi = r0 + li, gj = c0 + lj;
-                if (gi < lo || gi >= hi || gj < lo || gj >= hi) continue;
-                const size_t idx = ((size_t)i1 * nact + gi) * nact + gj;
-                double val = (double)acc[i][j][v] * invP;
-                double q8 = rint(val * 8.0);
-                worst = fmaxf(worst, (float)fabs(val - q8 * 0.125));
-                a.Cq[idx] = (int)q8;
-            }
-    if constexpr (NORM) {
-        /* non-negative floats order like their bit patterns */
-        if (worst > 0.f) atomicMax(a.margin, __float_as_uint(worst));
-    }
+    if constexpr (!NORM) flush_acc<MODE, RB>(a, acc, nchunk <= SEGC, i1, r0, c0, rb, lane, s1, L.rsc, L.csc);
+    else norm_store<RB>(a, acc, i1, r0, c0, rb, lane);
 }
 
 /* ---------------------------------------------------------------------------
@@ -782,6 +791,7 @@ __device__ __forceinline__ void v2_step(const F2 &cur, f32x16 (&acc)[2][4], cons
 /* Chunk c of a work item, not its last: k-steps 1..3 from Sc, the barrier,
  * chunk c + 2 into Sc, then k-step 3's MFMAs with k-step 0 of chunk c + 1
  * from Sn. */
+template <int NORM>
 __device__ __forceinline__ void v2_chunk(const ContractArgs &a, long c, long nchunk, const V2Stage &Sc,
                                          const V2Stage &Sn, F2 &f0, F2 &f1, f32x16 (&acc)[2][4], int r0, int c0,
                                          int g0, int wv, int q, int rb, int lane)
@@ -796,11 +806,12 @@ __device__ __forceinline__ void v2_chunk(const ContractArgs &a, long c, long nch
     __syncthreads();
     if (c + 2 < nchunk) v2_issue(Sc, a, (c + 2) * KC2, r0, c0, g0, wv, lane);
     v2_step(f1, acc, Sn, 0, q, rb, lane, f0);
-    if ((c + 1) % SEG2 == 0)
+    if (!NORM && (c + 1) % SEG2 == 0)
         flush_acc<AMB_PREC_BF16X3, 2>(a, acc, c + 1 == SEG2, g0 + q, r0, c0, rb, lane, 1.f, nullptr, nullptr);
 }
 
-/* The last chunk from S, and the final flush. */
+/* The last chunk from S, and the final flush (or the normalisation epilogue). */
+template <int NORM>
 __device__ __forceinline__ void v2_last(const ContractArgs &a, long nchunk, const V2Stage &S, F2 &f0, F2 &f1,
                                         f32x16 (&acc)[2][4], int r0, int c0, int g0, int q, int rb, int lane)
 {
@@ -809,7 +820,8 @@ __device__ __forceinline__ void v2_last(const ContractArgs &a, long nchunk, cons
     v2_step(f1, acc, S, 2, q, rb, lane, f0);
     v2_step(f0, acc, S, 3, q, rb, lane, f1);
     v2_mfma(f1, acc);
-    flush_acc<AMB_PREC_BF16X3, 2>(a, acc, nchunk <= SEG2, g0 + q, r0, c0, rb, lane, 1.f, nullptr, nullptr);
+    if constexpr (NORM) norm_store<2>(a, acc, g0 + q, r0, c0, rb, lane);
+    else flush_acc<AMB_PREC_BF16X3, 2>(a, acc, nchunk <= SEG2, g0 + q, r0, c0, rb, lane, 1.f, nullptr, nullptr);
 }
 
 /* The two LDS stages are separate objects and every access names one of
@@ -871,11 +883,11 @@ __device__ __forceinline__ void contract_item_v2(const ContractArgs &a, int tile
     v2_frag(S0, 0, q, rb, lane, f0);
     long c = 0;
     for (; c + 2 < nchunk; c += 2) {
-        v2_chunk(a, c, nchunk, S0, S1, f0, f1, acc, r0, c0, g0, wv, q, rb, lane);
-        v2_chunk(a, c + 1, nchunk, S1, S0, f0, f1, acc, r0, c0, g0, wv, q, rb, lane);
+        v2_chunk<NORM>(a, c, nchunk, S0, S1, f0, f1, acc, r0, c0, g0, wv, q, rb, lane);
+        v2_chunk<NORM>(a, c + 1, nchunk, S1, S0, f0, f1, acc, r0, c0, g0, wv, q, rb, lane);
     }
-    if (c + 1 < nchunk) v2_chunk(a, c, nchunk, S0, S1, f0, f1, acc, r0, c0, g0, wv, q, rb, lane);
-    v2_last(a, nchunk, ((nchunk - 1) & 1) ? S1 : S0, f0, f1, acc, r0, c0, g0, q, rb, lane);
+    if (c + 1 < nchunk) v2_chunk<NORM>(a, c, nchunk, S0, S1, f0, f1, acc, r0, c0, g0, wv, q, rb, lane);
+    v2_last<NORM>(a, nchunk, ((nchunk - 1) & 1) ? S1 : S0, f0, f1, acc, r0, c0, g0, q, rb, lane);
 }
 
 /* The two stages: A rows, hi then lo B rows, D_r1 rows, each its own array */
@@ -885,6 +897,7 @@ __device__ __forceinline__ void contract_item_v2(const ContractArgs &a, int tile
     __shared__ A16 float sD0[QP * KC2], sD1[QP * KC2];                                                             \
     const V2Stage S0{sA0, sB0, sB0 + TT * KC2, sD0}, S1{sA1, sB1, sB1 + TT * KC2, sD1};
 
+template <int NORM>
 __global__ __launch_bounds__(512, 1) void k_contract_v2(ContractArgs a, int ngroups, int per_xcd, int nwork)
 {
     V2_STAGES
@@ -892,9 +905,10 @@ __global__ __launch_bounds__(512, 1) void k_contract_v2(ContractArgs a, int ngro
     const int slot = b >> 3;
     const int w = (b & 7) * per_xcd + slot;
     if (slot >= per_xcd || w >= nwork) return;
-    contract_item_v2<0>(a, w / ngroups, w % ngroups, S0, S1);
+    contract_item_v2<NORM>(a, w / ngroups, w % ngroups, S0, S1);
 }
 
+template <int NORM>
 __global__ __launch_bounds__(512, 1) void k_contract_grouped_v2(const ContractArgs *__restrict__ descs,
                                                                 const long *__restrict__ begin, int ntri,
                                                                 unsigned long long *counter)
@@ -917,7 +931,7 @@ __global__ __launch_bounds__(512, 1) void k_contract_grouped_v2(const ContractAr
         const ContractArgs &a = descs[lo];
         const int ngroups = (a.nact + QP - 1) / QP;
         const long r = w - begin[lo];
-        contract_item_v2<0>(a, (int)(r / ngroups), (int)(r % ngroups), S0, S1);
+        contract_item_v2<NORM>(a, (int)(r / ngroups), (int)(r % ngroups), S0, S1);
         __syncthreads();
     }
 }
@@ -1038,8 +1052,8 @@ template <int MODE, int NORM>
 static void launch_grouped_var(int var, dim3 grid, hipStream_t s, const ContractArgs *d, const long *b, int n,
                                unsigned long long *ctr)
 {
-    if (var == 2 && MODE == AMB_PREC_BF16X3 && NORM == 0)
-        hipLaunchKernelGGL(k_contract_grouped_v2, grid, dim3(512), 0, s, d, b, n, ctr);
+    if (var == 2 && MODE == AMB_PREC_BF16X3)
+        hipLaunchKernelGGL(k_contract_grouped_v2<NORM>, grid, dim3(512), 0, s, d, b, n, ctr);
     else if (var == 1) hipLaunchKernelGGL((k_contract_grouped<MODE, NORM, 2, 1>), grid, dim3(512), 0, s, d, b, n, ctr);
     else hipLaunchKernelGGL((k_contract_grouped<MODE, NORM, 2, 0>), grid, dim3(512), 0, s, d, b, n, ctr);
 }
@@ -1056,8 +1070,8 @@ template <int MODE, int NORM>
 static void launch_var(int var, dim3 grid, hipStream_t s, const ContractArgs &a, int ngroups, int per_xcd, int nwork)
 {
     if (var == 2 && !amb_contract_v2_fits(a.nact, a.pitch)) var = 0;
-    if (var == 2 && MODE == AMB_PREC_BF16X3 && NORM == 0)
-        hipLaunchKernelGGL(k_contract_v2, grid, dim3(512), 0, s, a, ngroups, per_xcd, nwork);
+    if (var == 2 && MODE == AMB_PREC_BF16X3)
+        hipLaunchKernelGGL(k_contract_v2<NORM>, grid, dim3(512), 0, s, a, ngroups, per_xcd, nwork);
     else if (var == 1) hipLaunchKernelGGL((k_contract<MODE, NORM, 2, 1>), grid, dim3(512), 0, s, a, ngroups, per_xcd, nwork);
     else hipLaunchKernelGGL((k_contract<MODE, NORM, 2, 0>), grid, dim3(512), 0, s, a, ngroups, per_xcd, nwork);
 }

@@ -103,6 +103,7 @@ struct TriJob {
 struct amb_ctx {
     int device = 0, prec = AMB_PREC_BF16X3, verbose = 0, sync_debug = 0, kvar = 2, no_sym = 0;
     int batch_mode = AMB_BATCH_GROUPED;
+    int cert_strict = 0; /* uncertified normalisation sums: count (0) or fail (1) */
     uint64_t budget = 0, used = 0;
     hipStream_t stream = nullptr;
     std::recursive_mutex mu;
@@ -158,6 +159,10 @@ int prepare_tri(amb_ctx *c, const PlaneSrc &src, int cube_id, int r1, int r2, in
 void account_tri(amb_ctx *c, const TriJob &j, const int32_t *cnt);
 void count_tri(const TriJob &j, const int32_t *cnt, int64_t *evals, int64_t *pairs);
 void account_counts(amb_ctx *c, const TriJob &j, int64_t evals, int64_t pairs);
+/* the normalisation certificate of one call or wave (worst grid distance m):
+ * recorded in the stats; an error if out of the int32 range of eighths, or
+ * beyond 1/32 with cert_strict set */
+int check_cert(amb_ctx *c, float m);
 /* algorithmic HBM bytes of one c2r build (0 for derived kinds) */
 double build_fft_bytes(amb_ctx *c, const BuildOp &op);
 /* one triangle with its own launches, results in host memory */

@@ -49,9 +49,12 @@ struct ContractArgs {
     unsigned *margin;          /* norm: max |S/P - q| as float bits */
 };
 /* var: kernel variant (0: plain k-steps, 1: BF16x3 k-steps interleaving
- * MFMA with the next block's A formation, 2: BF16x3 data form with direct
- * global->LDS loads and k-step pipelining; other forms run variant 0) */
+ * MFMA with the next block's A formation, 2: BF16x3 data and normalisation
+ * forms with direct global->LDS loads and k-step pipelining; other forms
+ * run variant 0) */
 #define AMB_NVAR 3
+/* margin word value of a normalisation sum beyond the int32 range of eighths */
+#define AMB_CERT_RANGE 1e30f
 /* variant 2 addresses each plane array with 32-bit byte offsets: true when
  * nact rows of pitch FP32 values fit in 4 GiB; otherwise run variant 0 */
 bool amb_contract_v2_fits(int nact, long pitch);

@@ -62,6 +62,8 @@ def main():
                     help="development only: K triangles (the line is then not the metric workload)")
     ap.add_argument("--tri-pick", default="first", choices=["first", "largest"])
     ap.add_argument("--batch-mode", default="grouped", choices=["grouped", "per_triangle"])
+    ap.add_argument("--estimator", default="v1", choices=["v1", "v2"],
+                    help="v1: amber (data contraction); v2: amber_v2 (data plus the certified normalisation)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -112,7 +114,8 @@ def main():
     log("uploaded cube (%.2f GB of FP64 file data) in %.2f s" % ((gv.nbytes + samp.nbytes) / 1e9, upload_s))
 
     W = 2 * nc10 + 1
-    out = (np.zeros((len(tris), W, W), np.float32), np.zeros((len(tris), W, W), np.int32), None)
+    out = (np.zeros((len(tris), W, W), np.float32), np.zeros((len(tris), W, W), np.int32),
+           np.zeros((len(tris), W, W), np.float32) if args.estimator == "v2" else None)
 
     def step():
         # every step rebuilds its filtered planes: nothing carries over
@@ -155,7 +158,7 @@ def main():
             "issued_frac_of_instr_peak": round(st["flop_contract_exec"] / (ms_c * 1e-3) / 1e12 /
                                                peak, 4) if ms_c > 0 else None}
     stages = {"fft_ms": round(st["ms_fft"], 2), "contract_ms": round(st["ms_contract"], 2),
-              "reduce_ms": round(st["ms_reduce"], 2),
+              "norm_ms": round(st["ms_norm"], 2), "reduce_ms": round(st["ms_reduce"], 2),
               "fft_gbs": round(st["bytes_fft"] / (st["ms_fft"] * 1e-3) / 1e9, 1) if st["ms_fft"] > 0 else None}
 
     err = None
@@ -166,6 +169,8 @@ def main():
         cpu = cpu_baseline(A, cfg, gv, samp, uf, fk, lo, hi, tris, args.cpu_seconds)
 
     workload = cfg["desc"] + ", %d triangles" % len(tris)
+    if args.estimator == "v2":
+        workload += ", amber_v2 estimator (data and certified normalisation contractions)"
     if args.tri_limit:
         workload += " (DEVELOPMENT SUBSET, not the metric workload)"
     line = {
@@ -175,7 +180,8 @@ def main():
         "scaling": "weak", "vs_baseline": None,
         "dtype": {"bf16x3": "bf16", "fp32": "f32", "fp8": "fp8_e4m3"}[args.precision],
         "precision": args.precision, "data": "synthetic",
-        "config": {"workload": workload, "channels": nc, "grid": N, "bins": cfg["rings"][2],
+        "config": {"workload": workload, "estimator": args.estimator, "channels": nc, "grid": N,
+                   "bins": cfg["rings"][2],
                    "triangles": int(len(tris)), "nc10": nc10, "flagged": cfg["nflag"],
                    "evals_per_step": int(evals_per_step)},
         "roofline": roof, "stages": stages,
@@ -188,6 +194,11 @@ def main():
         "rel_err_vs_fp64": err, "cpu_baseline": cpu,
         "upload_s_pcie_incl": round(upload_s, 3),
     }
+    if args.estimator == "v2":
+        # waves whose normalisation sums were past 1/32 of their 1/8 grid
+        # (FP32 accumulation error, as in the reference's own loop)
+        line["norm_certificate"] = {"uncertified_waves": int(st["n_uncertified"]),
+                                    "worst_grid_distance": round(float(st["cert_margin_max"]), 4)}
     ctx.close()
     print(json.dumps(line), flush=True)
 

@@ -415,9 +415,25 @@ typedef struct amb_stats {
     int64_t n_grid_groups;  /* groups gridded */
     double ms_post;         /* lag-space transform matrix products */
     double flop_post;       /* their algorithmic 8 M N^2 flop per triangle */
+    double cert_margin_max; /* worst normalisation grid distance (see amb_set_cert_strict) */
+    int64_t n_uncertified;  /* triangle calls or waves whose normalisation was not certified */
 } amb_stats;
 int amb_get_stats(amb_ctx *ctx, amb_stats *st);
 int amb_reset_stats(amb_ctx *ctx);
+
+/* The v2 normalisation sum of a triplet is a whole number of eighths of
+ * Ng^2 (the sampling images are 0/1 masks, half-weighted on the kx = 0
+ * column).  The contraction accumulates in FP32 and rounds each sum to the
+ * nearest 1/8; the distance of the worst sum from that grid is its
+ * certificate.  Within 1/32 the rounded sums are exact.  Beyond it (large
+ * Ng: the sums reach ~1e5 Ng^2, past what FP32 resolves) the rounded sums
+ * carry FP32 accumulation error, as the reference's own FP32 loop
+ * (amber_v2.c:262-266) does.  strict = 0 (default): such calls succeed and
+ * are counted in the stats (fields n_uncertified, cert_margin_max), like the
+ * reference, which never fails here.  strict = 1: they fail with
+ * AMB_ECERT.  No counterpart in the reference.  A sum beyond the int32
+ * range of eighths fails in either mode. */
+int amb_set_cert_strict(amb_ctx *ctx, int strict);
 
 #ifdef __cplusplus
 }

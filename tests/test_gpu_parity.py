@@ -245,7 +245,8 @@ def test_ring_image_vs_oracle(ctx, N, nc, lo, hi, ng):
 def test_pipelined_variant_is_bit_identical(ctx, N, nc, nflag, nc10, rings, monkeypatch):
     """The BF16x3 kernel with direct global->LDS loads and pipelined k-steps
     (variant 2) runs the same arithmetic in the same order as variant 0: its
-    results, per triangle and in grouped waves, must match bit for bit."""
+    results, data and v2 normalisation, per triangle and in grouped waves,
+    must match bit for bit."""
     lo, hi = A.log_rings(*rings)
     gv, samp, uf = A.generate(N, nc, nflag=nflag, seed=61)
     fk = 1 if nflag else 0
@@ -265,8 +266,42 @@ def test_pipelined_variant_is_bit_identical(ctx, N, nc, nflag, nc10, rings, monk
         outs = []
         for c, cb in zip(ctxs, cubes):
             c.set_batch_mode(mode)
-            outs.append(c.bispec_batch(cb, tris))
+            outs.append(c.bispec_batch(cb, tris, want_norm=True))
         assert np.array_equal(outs[0][1], outs[1][1])
         assert np.array_equal(outs[0][0], outs[1][0]), mode
+        assert np.array_equal(outs[0][2], outs[1][2]), mode
     for c in ctxs:
         c.close()
+
+
+def test_uncertified_normalisation_counts_or_fails(ctx):
+    """At Ng = 134 the v2 normalisation sums reach ~1.4e5 Ng^2, past what the
+    FP32 accumulation resolves on their 1/8 grid (the FP64 oracle's sums sit
+    on it to 1e-9): by default the call succeeds, counts the uncertified
+    call in the stats and returns sums within FP32 accumulation error of the
+    exact ones, as the reference's FP32 loop would; in strict mode it fails
+    with AMB_ECERT."""
+    N, nc, nc10 = 256, 40, 12
+    lo, hi = A.log_rings(20, 60, 3)
+    cube, gv, samp, uf, mask = _setup(ctx, N, nc, 0, 61, lo, hi, nc10)
+    r1, r2, r3 = 1, 0, 0
+    ctx.set_precision("bf16x3")
+    print("case uncertified normalisation N=%d Ng=%d" % (N, A.ng(hi[r1])), flush=True)
+    bl64, cnt64, n64 = O.bispec(N, nc, 1.0, gv, samp, lo, hi, r1, r2, r3, mask, nc10, want_norm=True)
+    P = A.ng(hi[r1]) ** 2
+    assert np.max(np.abs(n64 / P * 8 - np.rint(n64 / P * 8))) < 1e-6   # the exact sums are on the grid
+    ctx.reset_stats()
+    res = ctx.bispec_ex(cube, r1, r2, r3, want_norm=True)
+    st = ctx.stats()
+    assert res["cert_margin"] > 1.0 / 32 and st["n_uncertified"] == 1
+    assert st["cert_margin_max"] == pytest.approx(res["cert_margin"])
+    assert np.array_equal(res["cnt"], cnt64)
+    assert _relerr(res["norm64"], n64) <= 1e-5
+    ctx.set_cert_strict(True)
+    try:
+        with pytest.raises(A.AmberError) as e:
+            ctx.bispec_ex(cube, r1, r2, r3, want_norm=True)
+        assert e.value.code == -8
+    finally:
+        ctx.set_cert_strict(False)
+    ctx.release_cube(cube)
