@@ -42,7 +42,7 @@ HBM_PEAK_GBS = 8000.0
 # pipelined BF16x3 form): FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 from two PMC
 # runs of this workload, the factors being what the same counters read for
 # a plain 1 GiB device copy.  Counters cannot be read inside a timed run.
-MEASURED_TRAFFIC = {("cfg3", "bf16x3", "grouped"): 2.842e11}
+MEASURED_TRAFFIC = {("cfg3", "bf16x3", "grouped"): 4.497e11}
 
 
 def log(msg):
