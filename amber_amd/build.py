@@ -47,7 +47,8 @@ def build(verbose=False):
         if verbose and out.strip():
             print(out)
         objs.append(o)
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-o", LIB] + objs + ["-lgomp", "-lm"])
+    # rocBLAS: the two plain FP64 matrix products of the lag-space post-processing
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-o", LIB] + objs + ["-lrocblas", "-lgomp", "-lm"])
     for name, src, extra in (("amber", "amber_main.c", []), ("amber_v2", "amber_main.c", ["-DAMB_V2"]),
                              ("amber_maps", "amber_maps.c", [])):
         _run(["gcc", "-O2", "-std=gnu11", "-Wall", "-I" + INC] + extra +

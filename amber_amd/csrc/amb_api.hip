@@ -614,6 +614,7 @@ void amb_destroy(amb_ctx *c)
     hipFree(c->G); hipFree(c->C); hipFree(c->Cq); hipFree(c->margin); hipFree(c->out); hipFree(c->wdesc);
     if (c->hstage) hipHostFree(c->hstage);
     for (auto e : c->evpool) hipEventDestroy(e);
+    post_release(c);
     hipStreamDestroy(c->stream);
     delete c;
 }

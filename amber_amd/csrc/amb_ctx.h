@@ -116,6 +116,7 @@ struct amb_ctx {
     std::map<int, float2 *> tw;
     std::map<int, FftPlan> plans;
     int grouped_blocks = 0; /* persistent grid of the grouped contraction */
+    void *blas = nullptr;   /* rocBLAS handle of the post-processing products (first use) */
     /* scratch (grow-only) */
     float2 *G = nullptr; size_t G_bytes = 0;
     float *C = nullptr; size_t C_bytes = 0;
@@ -163,6 +164,8 @@ void account_counts(amb_ctx *c, const TriJob &j, int64_t evals, int64_t pairs);
  * recorded in the stats; an error if out of the int32 range of eighths, or
  * beyond 1/32 with cert_strict set */
 int check_cert(amb_ctx *c, float m);
+/* releases the post-processing library handle (amb_destroy) */
+void post_release(amb_ctx *c);
 /* algorithmic HBM bytes of one c2r build (0 for derived kinds) */
 double build_fft_bytes(amb_ctx *c, const BuildOp &op);
 /* one triangle with its own launches, results in host memory */

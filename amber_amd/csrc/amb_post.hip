@@ -11,7 +11,8 @@
  *             (mfunc_winds.py:16-30, 94-101);
  *   transform fftshift(irfft2(ifftshift(X, axes=1), axes=(1,0), s=(N,N)))
  *             * rc^4 L^2, L = N dnu rc' (Bdnu2Bk_fft, :325-334), in FP64 as
- *             two direct-DFT matrix products on the FP64 matrix cores:
+ *             two direct-DFT matrix products (rocBLAS DGEMM on the FP64
+ *             matrix cores):
  *             [Zr | Zi] = X [cos | sin] / N (the complex inverse DFT along the
  *             columns, e^{+2 pi i j k / N}), then
  *             y = [w cos | -w sin] [Zr ; Zi] / N (the half-spectrum inverse
@@ -23,6 +24,8 @@
  * M = 760: 1.4e10).
  */
 #include "amb_ctx.h"
+
+#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cmath>
@@ -79,68 +82,6 @@ __global__ void k_cyl_tables(const double2 *__restrict__ tw, int N, int M, doubl
     }
 }
 
-/* C[M][N] = A[M][K] B[K][N], row-major with leading dimensions, FP64 on the
- * matrix cores.  A 64x64 tile per workgroup, 32-deep k blocks through LDS,
- * four waves of 2x2 v_mfma_f64_16x16x4f64 blocks: A[m][k] = lane m + 16 k,
- * B[k][n] = lane n + 16 k, lane l's four results are rows 4 r + l / 16 of
- * column l % 16.  Out-of-range operands load as zeros. */
-typedef double f64x4 __attribute__((ext_vector_type(4)));
-
-__global__ __launch_bounds__(256) void k_dgemm(const double *__restrict__ A, int lda, const double *__restrict__ B,
-                                               int ldb, double *__restrict__ C, int ldc, int M, int N, int K)
-{
-    __shared__ double As[32][65], Bs[32][65]; /* [k][m], [k][n] */
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
-    const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
-    const int fm = lane & 15, fk = lane >> 4;
-    f64x4 acc[2][2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = (f64x4){0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < K; k0 += 32) {
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int e = tid + 256 * q;
-            const int ar = e >> 5, ak = e & 31; /* A: 64 rows x 32 k, k fastest */
-            const int gm = m0 + ar, gk = k0 + ak;
-            As[ak][ar] = (gm < M && gk < K) ? A[(size_t)gm * lda + gk] : 0.0;
-            const int bk = e >> 6, bn = e & 63; /* B: 32 k x 64 columns */
-            const int gk2 = k0 + bk, gn = n0 + bn;
-            Bs[bk][bn] = (gk2 < K && gn < N) ? B[(size_t)gk2 * ldb + gn] : 0.0;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int ks = 0; ks < 8; ++ks) {
-            const int row = 4 * ks + fk;
-            double fa[2], fb[2];
-#pragma unroll
-            for (int i = 0; i < 2; ++i) fa[i] = As[row][wr + 16 * i + fm];
-#pragma unroll
-            for (int j = 0; j < 2; ++j) fb[j] = Bs[row][wc + 16 * j + fm];
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int j = 0; j < 2; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[i], fb[j], acc[i][j], 0, 0, 0);
-        }
-    }
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int gm = m0 + wr + 16 * i + 4 * r + fk;
-            if (gm >= M) continue;
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                const int gn = n0 + wc + 16 * j + fm;
-                if (gn < N) C[(size_t)gm * ldc + gn] = acc[i][j][r];
-            }
-        }
-}
-
 /* [Zr ; Zi] (2M x N) from C1 = X [cos | sin] (M x 2N), divided by N */
 __global__ void k_cyl_stack(const double *__restrict__ C1, int N, int M, double *__restrict__ Z2)
 {
@@ -161,12 +102,40 @@ __global__ void k_cyl_out(const double *__restrict__ C2, int N, double scale, do
 
 } // namespace
 
-static hipError_t dgemm(hipStream_t s, const double *A, int lda, const double *B, int ldb, double *C, int ldc, int M,
-                        int N, int K)
+namespace amb {
+
+/* C[M][N] = A[M][K] B[K][N], row-major with leading dimensions: two plain
+ * FP64 matrix products, on rocBLAS's gfx950 DGEMM (FP64 matrix cores).  In
+ * rocBLAS's column-major terms the product is C^T = B^T A^T, so the call
+ * passes B first with m = N and n = M.  The handle lives with the context
+ * and runs on its stream.  A rocBLAS failure is returned as its status
+ * name in *why. */
+static hipError_t dgemm(amb_ctx *c, hipStream_t s, const double *A, int lda, const double *B, int ldb, double *C,
+                        int ldc, int M, int N, int K, const char **why)
 {
-    hipLaunchKernelGGL(k_dgemm, dim3((N + 63) / 64, (M + 63) / 64), dim3(256), 0, s, A, lda, B, ldb, C, ldc, M, N, K);
-    return hipGetLastError();
+    rocblas_status r = rocblas_status_success;
+    if (!c->blas) {
+        rocblas_handle h = nullptr;
+        r = rocblas_create_handle(&h);
+        if (r == rocblas_status_success) c->blas = h;
+    }
+    if (r == rocblas_status_success) r = rocblas_set_stream((rocblas_handle)c->blas, s);
+    const double one = 1.0, zero = 0.0;
+    if (r == rocblas_status_success)
+        r = rocblas_dgemm((rocblas_handle)c->blas, rocblas_operation_none, rocblas_operation_none, N, M, K, &one, B,
+                          ldb, A, lda, &zero, C, ldc);
+    if (r == rocblas_status_success) return hipGetLastError();
+    *why = rocblas_status_to_string(r);
+    return hipErrorUnknown;
 }
+
+void post_release(amb_ctx *c)
+{
+    if (c->blas) rocblas_destroy_handle((rocblas_handle)c->blas);
+    c->blas = nullptr;
+}
+
+} // namespace amb
 
 using namespace amb;
 
@@ -223,6 +192,7 @@ extern "C" int amb_cyl_bispec(amb_ctx *c, int n, int W, const float *bl, const v
     const double L = N * p->dnu * p->rc_prime;
     const double scale = pow(p->rc, 4) * L * L;
     hipStream_t st = c->stream;
+    const char *why = nullptr; /* a rocBLAS status, if that is what failed */
     hipError_t e = hipMemcpyAsync(d_tw, tw.data(), sizeof(double2) * N, hipMemcpyHostToDevice, st);
     if (e == hipSuccess && p->window)
         e = hipMemcpyAsync(d_win, win.data(), sizeof(double) * MN, hipMemcpyHostToDevice, st);
@@ -246,10 +216,10 @@ extern "C" int amb_cyl_bispec(amb_ctx *c, int n, int W, const float *bl, const v
             hipLaunchKernelGGL(k_cyl_prep<false>, g1, dim3(256), 0, st, d_bl, d_den, W, nc10, ncut,
                                (float)p->mabs_norm, win_p, d_X);
         timer_start(c, &c->st.ms_post);
-        e = dgemm(st, d_X, N, d_B1, 2 * N, d_C1, 2 * N, M, 2 * N, N);
+        e = dgemm(c, st, d_X, N, d_B1, 2 * N, d_C1, 2 * N, M, 2 * N, N, &why);
         if (e == hipSuccess) {
             hipLaunchKernelGGL(k_cyl_stack, g1, dim3(256), 0, st, d_C1, N, M, d_Z2);
-            e = dgemm(st, d_A2, 2 * M, d_Z2, N, d_C2, N, N, N, 2 * M);
+            e = dgemm(c, st, d_A2, 2 * M, d_Z2, N, d_C2, N, N, N, 2 * M, &why);
         }
         timer_stop(c);
         if (e == hipSuccess) {
@@ -264,6 +234,7 @@ extern "C" int amb_cyl_bispec(amb_ctx *c, int n, int W, const float *bl, const v
     timers_collect(c);
     c->st.flop_post += 8.0 * M * (double)N * N * n;
     dev_free(c, d, bytes);
-    if (e != hipSuccess) return fail(c, AMB_EHIP, "lag-space post-processing: %s", hipGetErrorString(e));
+    if (e != hipSuccess)
+        return fail(c, AMB_EHIP, "lag-space post-processing: %s", why ? why : hipGetErrorString(e));
     return AMB_OK;
 }

@@ -63,7 +63,7 @@ def main():
     ap.add_argument("--tri-pick", default="first", choices=["first", "largest"])
     ap.add_argument("--batch-mode", default="grouped", choices=["grouped", "per_triangle"])
     ap.add_argument("--estimator", default="v1", choices=["v1", "v2"],
-                    help="v1: amber (data contraction); v2: amber_v2 (data plus the certified normalisation)")
+                    help="v1: amber (data contraction); v2: amber_v2 (data plus the normalisation contraction)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -408,7 +408,7 @@ def bench_post(A, args):
     line = {"metric": "lag-space post-processing (fold, normalise, window, lag->k_par transform)",
             "value": round(n / dt, 2), "unit": "triangles/s", "triangles": n, "W": W, "N": N,
             "s_per_batch": round(dt, 4), "fp64_tflops_wall": round(fl / dt / 1e12, 2),
-            "roofline": {"bound": "mfma", "kernel": "k_dgemm (two per triangle)", "achieved": round(kt, 2),
+            "roofline": {"bound": "mfma", "kernel": "rocBLAS DGEMM (two per triangle)", "achieved": round(kt, 2),
                          "peak": 78.6, "unit": "TFLOP/s", "frac": round(kt / 78.6, 4)},
             "ms_transform_per_triangle": round(st["ms_post"] / steps / n, 3),
             "note": "host->device copy of the W x W arrays and device->host copy of the N x N results included"}
