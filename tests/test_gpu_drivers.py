@@ -118,3 +118,24 @@ def test_hardened_wire_matches_legacy(tmp_path):
             assert h["ng"] == A.ng(hi[r1]) and h["precision"] == prec
             assert (h["h0"], h["h1"]) == (legacy[1], legacy[2])
             assert np.array_equal(bl, legacy[3]) and np.array_equal(den, legacy[4])
+
+
+def test_v2_certificate_modes(tmp_path):
+    """amber_v2 on a cube whose normalisation sums FP32 cannot certify
+    (N = 256, Ng up to 192): by default the run completes and says how many
+    waves were not certified; --cert strict makes it an error exit."""
+    N, nc, nflag, nc10 = 256, 40, 0, 12
+    lo, hi = A.log_rings(20, 60, 3)
+    _layout(tmp_path, N, nc, nflag, 61, lo, hi)
+    print("case v2 certificate modes N=%d nc=%d" % (N, nc), flush=True)
+    _run(tmp_path, "amber", "--nc10", str(nc10), "--uf-nc", str(nc))
+    for fn in ("rcount", "closed_rings"):
+        shutil.copy(tmp_path / "results/v1" / fn, tmp_path / "results/v2" / fn)
+    n, rd = A.read_rings(str(tmp_path / "results/v1"))
+    assert n > 0 and max(A.ng(hi[int(r1)]) for r1 in rd[:, 0]) >= 134
+    out = _run(tmp_path, "amber_v2", "--nc10", str(nc10), "--uf-nc", str(nc))
+    assert "not certified exact" in out
+    r = subprocess.run([os.path.join(BIN, "amber_v2"), "1", "--rings", "rings.txt", "--nc10", str(nc10), "--uf-nc",
+                        str(nc), "--cert", "strict"], cwd=tmp_path, stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, text=True, timeout=120)
+    assert r.returncode != 0 and "1/8 grid" in r.stdout, r.stdout[-2000:]
