@@ -19,7 +19,10 @@
  * are in flight while the current chunk is multiplied.  Workgroups are
  * numbered tile-major and dealt to the 8 XCDs in contiguous runs, so the
  * workgroups resident on one XCD share their (p2, p3) tile and stream the
- * same D_r2 / D_r3 chunks through that XCD's L2.
+ * same D_r2 / D_r3 chunks through that XCD's L2.  The BF16x3 data and
+ * sampling contractions run the pipelined form further down by default
+ * (64-pixel chunks loaded straight into LDS, k-steps streamed across chunk
+ * boundaries), with the same arithmetic in the same order.
  *
  * Precisions (the accumulate is FP32 in every case):
  *   FP32   v_mfma_f32_32x32x2_f32 on FP32 operands
