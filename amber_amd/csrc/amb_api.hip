@@ -403,8 +403,14 @@ int prepare_tri(amb_ctx *c, const PlaneSrc &src, int id, int r1, int r2, int r3,
     /* r1 == r2 symmetry needs every p3 of p1's window inside p2's window */
     const bool full_window = nact > 0 && cb->act.back() - cb->act.front() <= cb->nc10;
     j.sym12 = (r1 == r2 && full_window && !c->no_sym) ? 1 : 0;
+    /* r2 == r3: C[p1][p2][p3] and C[p1][p3][p2] are the same sum with the
+       roles of two planes exchanged (equal up to rounding), and the window
+       of p1 is the same for p2 and p3; with r1 == r2 as well only the exact
+       r1 == r2 symmetry is used */
+    j.sym23 = (r2 == r3 && r1 != r2 && !c->no_sym) ? 1 : 0;
     ContractArgs &a = j.a;
     a.P = j.P; a.pitch = j.pitch; a.nact = nact; a.wlo = cb->d_wlo; a.whi = cb->d_whi; a.sym12 = j.sym12;
+    a.sym23 = j.sym23;
     a.D1 = (const float *)D[0]; a.D2 = (const float *)D[1];
     if (prec == AMB_PREC_FP32) a.D3 = (const float *)D[2];
     else if (prec == AMB_PREC_BF16X3) {
@@ -426,8 +432,8 @@ int prepare_tri(amb_ctx *c, const PlaneSrc &src, int id, int r1, int r2, int r3,
     }
     ReduceArgs &ra = j.ra;
     ra.act = cb->d_act; ra.inv = cb->d_inv;
-    ra.nc = (int)cb->nc; ra.nact = nact; ra.nc10 = cb->nc10; ra.P = j.P; ra.sym12 = j.sym12;
-    j.flop_exec = (double)amb_contract_live_items(nact, cb->wlo.data(), cb->whi.data(), j.sym12) *
+    ra.nc = (int)cb->nc; ra.nact = nact; ra.nc10 = cb->nc10; ra.P = j.P; ra.sym12 = j.sym12; ra.sym23 = j.sym23;
+    j.flop_exec = (double)amb_contract_live_items(nact, cb->wlo.data(), cb->whi.data(), j.sym12, j.sym23) *
                   amb_contract_item_flops(prec, j.pitch);
     return AMB_OK;
 }

@@ -505,6 +505,7 @@ __device__ __forceinline__ void contract_item(const ContractArgs &a, int tile, i
        product of the same two floats), so rows below every reference plane
        of the group are read transposed and never computed */
     if (a.sym12 && r0 + TT <= g0) return;
+    if (a.sym23 && r0 >= c0 + TT) return;
 
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int q = wv & 3, rb = (wv >> 2) * (TT / NWR);
@@ -844,6 +845,7 @@ __device__ __forceinline__ void contract_item_v2(const ContractArgs &a, int tile
     int lo = a.wlo[g0], hi = a.whi[min(g0 + QP, nact) - 1];
     if (r0 >= hi || r0 + TT <= lo || c0 >= hi || c0 + TT <= lo) return;
     if (a.sym12 && r0 + TT <= g0) return;
+    if (a.sym23 && r0 >= c0 + TT) return;
 
     /* the wave index is uniform across the wave: branches on it are scalar */
     const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -1029,7 +1031,7 @@ long amb_contract_items(int nact)
     return nt * nt * ((nact + QP - 1) / QP);
 }
 
-long amb_contract_live_items(int nact, const int *wlo, const int *whi, int sym12)
+long amb_contract_live_items(int nact, const int *wlo, const int *whi, int sym12, int sym23)
 {   /* the skip tests at the top of contract_item */
     const int nt = (nact + TT - 1) / TT, ng = (nact + QP - 1) / QP;
     long n = 0;
@@ -1040,6 +1042,7 @@ long amb_contract_live_items(int nact, const int *wlo, const int *whi, int sym12
                 const int r0 = t2 * TT, c0 = t3 * TT;
                 if (r0 >= hi || r0 + TT <= lo || c0 >= hi || c0 + TT <= lo) continue;
                 if (sym12 && r0 + TT <= g0) continue;
+                if (sym23 && r0 >= c0 + TT) continue;
                 ++n;
             }
     }
@@ -1237,7 +1240,9 @@ __device__ __forceinline__ void reduce_cell(const ReduceArgs &a, long cell)
             const int j2 = ok[u] ? a.inv[p1 + d2] : -1, j3 = ok[u] ? a.inv[p1 + d3] : -1;
             ok[u] = ok[u] && j2 >= 0 && j3 >= 0;
             const bool sw = a.sym12 && j2 < i;
-            idx[u] = ok[u] ? ((size_t)(sw ? j2 : i) * na + (sw ? i : j2)) * na + j3 : 0;
+            const bool s23 = a.sym23 && j2 > j3; /* below the diagonal: its mirror */
+            const int k2 = s23 ? j3 : j2, k3 = s23 ? j2 : j3;
+            idx[u] = ok[u] ? ((size_t)(sw ? k2 : i) * na + (sw ? i : k2)) * na + k3 : 0;
         }
         float x[U];
         int qx[U];

@@ -90,7 +90,7 @@ struct Timer {
 struct TriJob {
     int cube = 0, r1 = 0, r2 = 0, r3 = 0, want_norm = 0;
     long ng = 0, P = 0, pitch = 0;
-    int nact = 0, W = 0, sym12 = 0;
+    int nact = 0, W = 0, sym12 = 0, sym23 = 0;
     size_t cells = 0;
     ContractArgs a{}; /* data contraction */
     ContractArgs b{}; /* normalisation contraction (want_norm) */

@@ -43,6 +43,8 @@ struct ContractArgs {
     int nact;
     int sym12;                 /* r1 == r2 and full windows: rows below the
                                   reference planes are never needed */
+    int sym23;                 /* r2 == r3 (r1 != r2): tiles below the
+                                  (p2, p3) diagonal are never needed */
     const int *wlo, *whi;      /* per-i1 window in compact channel indices */
     float *C;                  /* data: [nact][nact][nact] f32 */
     int *Cq;                   /* norm: [nact][nact][nact] int32 of 8*count */
@@ -71,6 +73,7 @@ struct ReduceArgs {
     const int *act, *inv;
     int nc, nact, nc10;
     int sym12;          /* read C[p2][p1][p3] when p2 < p1 */
+    int sym23;          /* read C[p1][p3][p2] when p2 > p3 */
     long P;
     float *bl;
     double *bl64;
@@ -84,7 +87,7 @@ hipError_t amb_launch_reduce(hipStream_t s, const ReduceArgs &a);
 long amb_contract_items(int nact); /* work items of one triangle */
 /* items the kernel computes (not skipped by its window and r1 == r2 tests),
  * from host copies of the per-plane windows */
-long amb_contract_live_items(int nact, const int *wlo, const int *whi, int sym12);
+long amb_contract_live_items(int nact, const int *wlo, const int *whi, int sym12, int sym23);
 /* MFMA flops one computed item issues at pixel pitch `pitch` */
 double amb_contract_item_flops(int prec, long pitch);
 bool amb_contract_args_ok(int prec, int norm, const ContractArgs &a);

@@ -305,3 +305,36 @@ def test_uncertified_normalisation_counts_or_fails(ctx):
     finally:
         ctx.set_cert_strict(False)
     ctx.release_cube(cube)
+
+
+@pytest.mark.parametrize("prec", ["bf16x3", "fp32"])
+def test_r2_eq_r3_symmetry(ctx, prec, monkeypatch):
+    """With r2 == r3 (r1 != r2) the kernel skips the (p2, p3) tiles below the
+    diagonal and the reduction reads the mirror element C[p1][p3][p2]: the
+    same sum with two planes' roles exchanged, so equal to the full
+    computation up to rounding; counts and the v2 normalisation stay exact,
+    and the lag array comes out exactly symmetric."""
+    N, nc, nc10 = 48, 300, 299
+    lo, hi = A.log_rings(2, 14, 4)
+    ctx.set_precision(prec)
+    cube, gv, samp, uf, mask = _setup(ctx, N, nc, 5, 57, lo, hi, nc10)
+    monkeypatch.setenv("AMB_NO_SYM", "1")
+    full = A.Context(device=0, precision=prec)
+    monkeypatch.delenv("AMB_NO_SYM")
+    cube2 = full.load_cube(N, nc, 1.0, 1, gv, samp)
+    full.set_channels(cube2, uf, nc10)
+    full.set_rings(lo, hi)
+    tris = [t for t in A.triangles(lo, hi) if t[1] == t[2] and t[0] != t[1]]
+    assert tris
+    print("case r2 == r3 symmetry prec=%s ntri=%d" % (prec, len(tris)), flush=True)
+    for (r1, r2, r3) in tris[-3:]:
+        a = ctx.bispec_ex(cube, int(r1), int(r2), int(r3), want_norm=True)
+        b = full.bispec_ex(cube2, int(r1), int(r2), int(r3), want_norm=True)
+        assert np.array_equal(a["cnt"], b["cnt"]) and np.array_equal(a["norm64"], b["norm64"])
+        assert np.array_equal(a["bl64"], a["bl64"].T)
+        assert _relerr(a["bl64"], b["bl64"]) <= TOL[prec]   # two roundings of the same sums
+        bl64, cnt64, _ = O.bispec(N, nc, 1.0, gv, samp, lo, hi, r1, r2, r3, mask, nc10)
+        assert _relerr(a["bl64"], bl64) <= TOL[prec]
+    full.close()
+    ctx.set_precision("bf16x3")
+    ctx.release_cube(cube)
