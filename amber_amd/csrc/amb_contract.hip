@@ -1213,6 +1213,9 @@ __device__ __forceinline__ void reduce_cell(const ReduceArgs &a, long cell)
     const int W = 2 * a.nc10 + 1;
     const int *__restrict__ first_ge = a.inv + a.nc; /* inv[nc] then first_ge[nc+1] */
     const int d2 = (int)(cell / W) - a.nc10, d3 = (int)(cell % W) - a.nc10;
+    /* r2 == r3: cell (d2, d3) with d2 > d3 sums the same elements in the same
+       order as (d3, d2), which writes both */
+    if (a.sym23 && d2 > d3) return;
     int plo = 0;
     if (-d2 > plo) plo = -d2;
     if (-d3 > plo) plo = -d3;
@@ -1239,10 +1242,11 @@ __device__ __forceinline__ void reduce_cell(const ReduceArgs &a, long cell)
             ok[u] = p1 < phi;
             const int j2 = ok[u] ? a.inv[p1 + d2] : -1, j3 = ok[u] ? a.inv[p1 + d3] : -1;
             ok[u] = ok[u] && j2 >= 0 && j3 >= 0;
+            /* r1 == r2: rows below i are read from their mirror; r2 == r3
+               needs none, d2 <= d3 here and compact indices ascend with
+               the channel, so j2 <= j3 */
             const bool sw = a.sym12 && j2 < i;
-            const bool s23 = a.sym23 && j2 > j3; /* below the diagonal: its mirror */
-            const int k2 = s23 ? j3 : j2, k3 = s23 ? j2 : j3;
-            idx[u] = ok[u] ? ((size_t)(sw ? k2 : i) * na + (sw ? i : k2)) * na + k3 : 0;
+            idx[u] = ok[u] ? ((size_t)(sw ? j2 : i) * na + (sw ? i : j2)) * na + j3 : 0;
         }
         float x[U];
         int qx[U];
@@ -1264,13 +1268,18 @@ __device__ __forceinline__ void reduce_cell(const ReduceArgs &a, long cell)
         if (past) break;
         i1 += U;
     }
-    if (a.bl) a.bl[cell] = sum + comp;
-    if (a.bl64) a.bl64[cell] = (double)sum + (double)comp;
-    if (a.cnt) a.cnt[cell] = n;
-    if (a.Cq) {
-        const double v = (double)a.P * ((double)q * 0.125);
-        if (a.norm) a.norm[cell] = (float)v;
-        if (a.norm64) a.norm64[cell] = v;
+    const long mirror = (long)(d3 + a.nc10) * W + (d2 + a.nc10);
+    const int nw = (a.sym23 && d2 != d3) ? 2 : 1;
+    for (int w = 0; w < nw; ++w) {
+        const long o = w ? mirror : cell;
+        if (a.bl) a.bl[o] = sum + comp;
+        if (a.bl64) a.bl64[o] = (double)sum + (double)comp;
+        if (a.cnt) a.cnt[o] = n;
+        if (a.Cq) {
+            const double v = (double)a.P * ((double)q * 0.125);
+            if (a.norm) a.norm[o] = (float)v;
+            if (a.norm64) a.norm64[o] = v;
+        }
     }
 }
 

@@ -73,7 +73,7 @@ struct ReduceArgs {
     const int *act, *inv;
     int nc, nact, nc10;
     int sym12;          /* read C[p2][p1][p3] when p2 < p1 */
-    int sym23;          /* read C[p1][p3][p2] when p2 > p3 */
+    int sym23;          /* r2 == r3: sum cells d2 <= d3, write their mirrors */
     long P;
     float *bl;
     double *bl64;
