@@ -591,6 +591,10 @@ int amb_create(const amb_opts *o, amb_ctx **out)
         c->kvar = (k >= 0 && k < AMB_NVAR) ? k : 0;
     }
     c->no_sym = getenv("AMB_NO_SYM") != nullptr; /* for A/B checks of the symmetry */
+    if (const char *v = getenv("AMB_WAVE_TRIS")) { /* smaller grouped waves, for tests */
+        const int k = atoi(v);
+        if (k > 0 && k < c->wave_max) c->wave_max = k;
+    }
     c->budget = (o && o->hbm_budget) ? o->hbm_budget : (uint64_t)(0.9 * (double)fr);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return AMB_EHIP; }
     void *m = nullptr;
@@ -618,7 +622,8 @@ void amb_destroy(amb_ctx *c)
     }
     for (auto &kv : c->tw) hipFree(kv.second);
     hipFree(c->G); hipFree(c->C); hipFree(c->Cq); hipFree(c->margin); hipFree(c->out); hipFree(c->wdesc);
-    if (c->hstage) hipHostFree(c->hstage);
+    for (void *h : c->hstage)
+        if (h) hipHostFree(h);
     for (auto e : c->evpool) hipEventDestroy(e);
     post_release(c);
     hipStreamDestroy(c->stream);

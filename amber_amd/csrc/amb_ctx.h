@@ -116,6 +116,7 @@ struct amb_ctx {
     std::map<int, float2 *> tw;
     std::map<int, FftPlan> plans;
     int grouped_blocks = 0; /* persistent grid of the grouped contraction */
+    int wave_max = 1024;    /* triangles per grouped wave */
     void *blas = nullptr;   /* rocBLAS handle of the post-processing products (first use) */
     /* scratch (grow-only) */
     float2 *G = nullptr; size_t G_bytes = 0;
@@ -124,7 +125,9 @@ struct amb_ctx {
     unsigned *margin = nullptr;
     void *out = nullptr; size_t out_bytes = 0;
     void *wdesc = nullptr; size_t wdesc_bytes = 0; /* wave descriptors */
-    void *hstage = nullptr; size_t hstage_bytes = 0; /* pinned host staging */
+    /* pinned host staging, two buffers: a grouped wave's results are copied
+       out of one while the next wave's land in the other */
+    void *hstage[2] = {nullptr, nullptr}; size_t hstage_bytes[2] = {0, 0};
     /* stats */
     amb_stats st{};
     std::vector<double> batch_ms; /* per-triangle completion time of the last batch */

@@ -11,6 +11,8 @@
  */
 #include "amb_ctx.h"
 
+#include <functional>
+
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -226,8 +228,10 @@ int finish_wave(amb_ctx *c, const Wave &w, const char *host, const amb_tri_req *
     return AMB_OK;
 }
 
-/* eager execution of one wave with the context's scratch */
-int run_wave(amb_ctx *c, Wave &w, const amb_tri_req *req)
+/* eager execution of one wave with the context's scratch, its results
+   staged in host buffer hb: `overlap` (the previous wave's copy-out, from
+   the other buffer) runs on the host while this wave's kernels run */
+int run_wave(amb_ctx *c, Wave &w, int hb, const std::function<int()> &overlap)
 {
     layout(w);
     void *C = c->C, *Cq = c->Cq;
@@ -239,20 +243,21 @@ int run_wave(amb_ctx *c, Wave &w, const amb_tri_req *req)
     }
     if (!rc) rc = grow(c, &c->out, &c->out_bytes, w.o_total, "lag arrays");
     if (!rc) rc = grow(c, &c->wdesc, &c->wdesc_bytes, w.d_total, "wave descriptors");
-    if (!rc) rc = grow_host(c, &c->hstage, &c->hstage_bytes, w.o_total + sizeof(unsigned));
+    if (!rc) rc = grow_host(c, &c->hstage[hb], &c->hstage_bytes[hb], w.o_total + sizeof(unsigned));
     if (rc) return rc;
-    WaveBufs b{c->C, c->Cq, (char *)c->out, (char *)c->wdesc, (char *)c->hstage};
+    WaveBufs b{c->C, c->Cq, (char *)c->out, (char *)c->wdesc, (char *)c->hstage[hb]};
     std::vector<char> h = descriptors(w, b);
     HIPCHK(c, hipMemcpyAsync(b.desc, h.data(), h.size(), hipMemcpyHostToDevice, c->stream));
     hipError_t e = launch_wave(c, c->stream, w, b, true);
     c->st.n_contract_launch++;
+    const int orc = overlap();
     /* always drain: the descriptor copy reads h, which dies with this frame */
     const hipError_t es = hipStreamSynchronize(c->stream);
     if (e == hipSuccess) e = es;
     timers_collect(c);
     if (e != hipSuccess)
         return fail(c, AMB_EHIP, "grouped wave of %zu triangles: %s", w.jobs.size(), hipGetErrorString(e));
-    return finish_wave(c, w, b.host, req);
+    return orc;
 }
 
 /* all requests as grouped waves; done_ms[i] gets request i's completion
@@ -268,18 +273,30 @@ int run_grouped(amb_ctx *c, const amb_tri_req *req, int n, double t0, double *do
     });
     const size_t c_cap = std::min<size_t>((size_t)16 << 30, c->budget / 4) / sizeof(float);
     const PlaneSrc src{&c->planes};
-    Wave w;
+    /* w: the wave being assembled; prev: a wave whose kernels finished and
+       whose results wait in host staging buffer hb ^ 1 */
+    Wave w, prev;
+    int hb = 0;
     size_t c_tot = 0, o_tot = 0;
-    auto flush = [&]() -> int {
-        if (w.jobs.empty()) return AMB_OK;
-        int rc = run_wave(c, w, req);
-        unpin_all(c);
+    auto finish_prev = [&]() -> int {
+        if (prev.jobs.empty()) return AMB_OK;
+        const int rc = finish_wave(c, prev, (const char *)c->hstage[hb ^ 1], req);
         const double t = now_ms() - t0;
         if (done_ms)
-            for (int r : w.req) done_ms[r] = t;
+            for (int r : prev.req) done_ms[r] = t;
+        prev = Wave{};
+        return rc;
+    };
+    auto flush = [&]() -> int {
+        if (w.jobs.empty()) return AMB_OK;
+        int rc = run_wave(c, w, hb, finish_prev);
+        unpin_all(c);
+        if (rc) return rc;
+        prev = std::move(w);
+        hb ^= 1;
         w = Wave{};
         c_tot = o_tot = 0;
-        return rc;
+        return AMB_OK;
     };
     for (int k = 0; k < n; ++k) {
         const amb_tri_req &q = req[order[k]];
@@ -288,7 +305,7 @@ int run_grouped(amb_ctx *c, const amb_tri_req *req, int n, double t0, double *do
         int rc = find_cube(c, q.cube_id, &cb);
         if (rc) { unpin_all(c); return rc; }
         const size_t nc3 = al((size_t)cb->nact() * cb->nact() * cb->nact() * sizeof(float)) / sizeof(float);
-        if (!w.jobs.empty() && (c_tot + nc3 > c_cap || (int)w.jobs.size() >= MAX_WAVE))
+        if (!w.jobs.empty() && (c_tot + nc3 > c_cap || (int)w.jobs.size() >= c->wave_max))
             if ((rc = flush())) return rc;
         TriJob j;
         rc = prepare_tri(c, src, q.cube_id, q.r1, q.r2, q.r3, q.norm != nullptr, j);
@@ -309,7 +326,8 @@ int run_grouped(amb_ctx *c, const amb_tri_req *req, int n, double t0, double *do
         c_tot += nc3;
         o_tot += ob;
     }
-    return flush();
+    const int rc = flush();
+    return rc ? rc : finish_prev();
 }
 
 } // namespace

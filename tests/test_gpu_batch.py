@@ -53,6 +53,30 @@ def test_grouped_equals_per_triangle(ctx, prec):
     ctx.release_cube(cube)
 
 
+def test_pipelined_waves_equal_per_triangle(monkeypatch):
+    """Three triangles per grouped wave: each wave's results are copied out
+    of one pinned staging buffer while the next wave runs into the other;
+    data, counts and v2 normalisation must equal the per-triangle path."""
+    monkeypatch.setenv("AMB_WAVE_TRIS", "3")
+    c = A.Context(device=0, precision="bf16x3")
+    monkeypatch.delenv("AMB_WAVE_TRIS")
+    try:
+        lo, hi = A.log_rings(2, 20, 6)
+        c.set_rings(lo, hi)
+        cube, gv, samp, uf, fk = _cube(c, 64, 24, 2, 5, 23)
+        tris = A.triangles(lo, hi)
+        print("case pipelined waves ntri=%d" % len(tris), flush=True)
+        c.set_batch_mode("per_triangle")
+        b1, c1, n1 = c.bispec_batch(cube, tris, want_norm=True)
+        c.set_batch_mode("grouped")
+        c.reset_stats()
+        b2, c2, n2 = c.bispec_batch(cube, tris, want_norm=True)
+        assert len(tris) > 6 and c.stats()["n_contract_launch"] == (len(tris) + 2) // 3
+        assert np.array_equal(b1, b2) and np.array_equal(c1, c2) and np.array_equal(n1, n2)
+    finally:
+        c.close()
+
+
 def test_multi_cube_wave_with_norm(ctx):
     """Two cubes of different shapes (and windows) in one grouped call, v2
     normalisation included; against single calls and, for a sample, the
