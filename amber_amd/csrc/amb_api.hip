@@ -621,7 +621,15 @@ void amb_destroy(amb_ctx *c)
         hipFree(cb.d_wlo); hipFree(cb.d_whi);
     }
     for (auto &kv : c->tw) hipFree(kv.second);
-    hipFree(c->G); hipFree(c->C); hipFree(c->Cq); hipFree(c->margin); hipFree(c->out); hipFree(c->wdesc);
+    if (c->cstream) {
+        hipStreamSynchronize(c->cstream);
+        hipStreamDestroy(c->cstream);
+        hipEventDestroy(c->ev_red);
+        hipEventDestroy(c->ev_cp[0]);
+        hipEventDestroy(c->ev_cp[1]);
+    }
+    hipFree(c->G); hipFree(c->C); hipFree(c->Cq); hipFree(c->margin); hipFree(c->out); hipFree(c->out2);
+    hipFree(c->wdesc);
     for (void *h : c->hstage)
         if (h) hipHostFree(h);
     for (auto e : c->evpool) hipEventDestroy(e);

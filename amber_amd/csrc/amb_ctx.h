@@ -124,6 +124,12 @@ struct amb_ctx {
     int *Cq = nullptr; size_t Cq_bytes = 0;
     unsigned *margin = nullptr;
     void *out = nullptr; size_t out_bytes = 0;
+    void *out2 = nullptr; size_t out2_bytes = 0; /* second result buffer of grouped waves */
+    /* grouped waves copy results to the host on their own stream (default
+       priority): ev_red marks a wave's reduction, ev_cp[k] the end of the
+       copy out of result buffer k */
+    hipStream_t cstream = nullptr;
+    hipEvent_t ev_red = nullptr, ev_cp[2] = {nullptr, nullptr};
     void *wdesc = nullptr; size_t wdesc_bytes = 0; /* wave descriptors */
     /* pinned host staging, two buffers: a grouped wave's results are copied
        out of one while the next wave's land in the other */

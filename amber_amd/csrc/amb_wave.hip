@@ -154,8 +154,9 @@ std::vector<char> descriptors(const Wave &w, const WaveBufs &b)
     return h;
 }
 
-/* the launches of one wave on stream s (descriptors already on device) */
-hipError_t launch_wave(amb_ctx *c, hipStream_t s, const Wave &w, const WaveBufs &b, bool timed)
+/* the launches of one wave on stream s (descriptors already on device);
+   copy: the lag arrays' copy to b.host on s too (else the caller makes it) */
+hipError_t launch_wave(amb_ctx *c, hipStream_t s, const Wave &w, const WaveBufs &b, bool timed, bool copy = true)
 {
     const int n = (int)w.jobs.size();
     const int prec = c->prec, nprec = prec == AMB_PREC_FP32 ? AMB_PREC_FP32 : AMB_PREC_BF16X3;
@@ -183,7 +184,7 @@ hipError_t launch_wave(amb_ctx *c, hipStream_t s, const Wave &w, const WaveBufs 
         e = amb_launch_reduce_grouped(s, (const ReduceArgs *)(b.desc + w.off_ra), (const long *)(b.desc + w.off_cb), n,
                                       w.cells);
     if (timed) timer_stop(c);
-    if (e == hipSuccess) e = hipMemcpyAsync(b.host, b.out, w.o_total, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && copy) e = hipMemcpyAsync(b.host, b.out, w.o_total, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess && w.nnorm)
         e = hipMemcpyAsync(b.host + w.o_total, mg, sizeof(unsigned), hipMemcpyDeviceToHost, s);
     return e;
@@ -228,9 +229,11 @@ int finish_wave(amb_ctx *c, const Wave &w, const char *host, const amb_tri_req *
     return AMB_OK;
 }
 
-/* eager execution of one wave with the context's scratch, its results
-   staged in host buffer hb: `overlap` (the previous wave's copy-out, from
-   the other buffer) runs on the host while this wave's kernels run */
+/* eager execution of one wave with the context's scratch, its results in
+   device buffer hb, copied to host staging buffer hb on the copy stream
+   (ev_cp[hb] marks the end): `overlap` (the previous wave's copy-out, from
+   the other buffers) runs on the host while this wave's kernels run. On
+   return the kernels are done; the copy may still run. */
 int run_wave(amb_ctx *c, Wave &w, int hb, const std::function<int()> &overlap)
 {
     layout(w);
@@ -241,28 +244,52 @@ int run_wave(amb_ctx *c, Wave &w, int hb, const std::function<int()> &overlap)
         rc = grow(c, &Cq, &c->Cq_bytes, sizeof(int) * w.c_total, "normalisation partials");
         c->Cq = (int *)Cq;
     }
-    if (!rc) rc = grow(c, &c->out, &c->out_bytes, w.o_total, "lag arrays");
+    void **outp = hb ? &c->out2 : &c->out;
+    /* the last copy out of buffer hb ended before its wave was finished */
+    if (!rc) rc = grow(c, outp, hb ? &c->out2_bytes : &c->out_bytes, w.o_total, "lag arrays");
+    if (!rc && !c->cstream) {
+        HIPCHK(c, hipStreamCreateWithFlags(&c->cstream, hipStreamNonBlocking));
+        HIPCHK(c, hipEventCreateWithFlags(&c->ev_red, hipEventDisableTiming));
+        HIPCHK(c, hipEventCreateWithFlags(&c->ev_cp[0], hipEventDisableTiming));
+        HIPCHK(c, hipEventCreateWithFlags(&c->ev_cp[1], hipEventDisableTiming));
+    }
     if (!rc) rc = grow(c, &c->wdesc, &c->wdesc_bytes, w.d_total, "wave descriptors");
     if (!rc) rc = grow_host(c, &c->hstage[hb], &c->hstage_bytes[hb], w.o_total + sizeof(unsigned));
     if (rc) return rc;
-    WaveBufs b{c->C, c->Cq, (char *)c->out, (char *)c->wdesc, (char *)c->hstage[hb]};
+    WaveBufs b{c->C, c->Cq, (char *)*outp, (char *)c->wdesc, (char *)c->hstage[hb]};
     std::vector<char> h = descriptors(w, b);
     HIPCHK(c, hipMemcpyAsync(b.desc, h.data(), h.size(), hipMemcpyHostToDevice, c->stream));
-    hipError_t e = launch_wave(c, c->stream, w, b, true);
+    hipError_t e = launch_wave(c, c->stream, w, b, true, false);
+    if (e == hipSuccess) e = hipEventRecord(c->ev_red, c->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(c->cstream, c->ev_red, 0);
+    if (e == hipSuccess) e = hipMemcpyAsync(b.host, b.out, w.o_total, hipMemcpyDeviceToHost, c->cstream);
+    if (e == hipSuccess) e = hipEventRecord(c->ev_cp[hb], c->cstream);
     c->st.n_contract_launch++;
     const int orc = overlap();
     /* always drain: the descriptor copy reads h, which dies with this frame */
     const hipError_t es = hipStreamSynchronize(c->stream);
     if (e == hipSuccess) e = es;
     timers_collect(c);
-    if (e != hipSuccess)
+    if (e != hipSuccess) {
+        hipStreamSynchronize(c->cstream);
         return fail(c, AMB_EHIP, "grouped wave of %zu triangles: %s", w.jobs.size(), hipGetErrorString(e));
+    }
     return orc;
 }
 
+int run_grouped_waves(amb_ctx *c, const amb_tri_req *req, int n, double t0, double *done_ms);
+
 /* all requests as grouped waves; done_ms[i] gets request i's completion
-   time relative to t0 (may be null) */
+   time relative to t0 (may be null). No result copy is left running on
+   return, also after an error. */
 int run_grouped(amb_ctx *c, const amb_tri_req *req, int n, double t0, double *done_ms)
+{
+    const int rc = run_grouped_waves(c, req, n, t0, done_ms);
+    if (c->cstream) hipStreamSynchronize(c->cstream);
+    return rc;
+}
+
+int run_grouped_waves(amb_ctx *c, const amb_tri_req *req, int n, double t0, double *done_ms)
 {
     hipSetDevice(c->device);
     std::vector<int> order(n);
@@ -274,12 +301,17 @@ int run_grouped(amb_ctx *c, const amb_tri_req *req, int n, double t0, double *do
     const size_t c_cap = std::min<size_t>((size_t)16 << 30, c->budget / 4) / sizeof(float);
     const PlaneSrc src{&c->planes};
     /* w: the wave being assembled; prev: a wave whose kernels finished and
-       whose results wait in host staging buffer hb ^ 1 */
+       whose results are copied (ev_cp[hb ^ 1]) into host staging buffer hb ^ 1 */
     Wave w, prev;
     int hb = 0;
     size_t c_tot = 0, o_tot = 0;
     auto finish_prev = [&]() -> int {
         if (prev.jobs.empty()) return AMB_OK;
+        const hipError_t e = hipEventSynchronize(c->ev_cp[hb ^ 1]);
+        if (e != hipSuccess) {
+            prev = Wave{};
+            return fail(c, AMB_EHIP, "copy of a grouped wave's results: %s", hipGetErrorString(e));
+        }
         const int rc = finish_wave(c, prev, (const char *)c->hstage[hb ^ 1], req);
         const double t = now_ms() - t0;
         if (done_ms)
