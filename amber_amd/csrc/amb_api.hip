@@ -144,6 +144,37 @@ int grow(amb_ctx *c, void **p, size_t *have, size_t need, const char *what)
 }
 
 /* grow-only pinned host staging */
+int copy_stream(amb_ctx *c)
+{
+    if (c->cstream) return AMB_OK;
+    HIPCHK(c, hipStreamCreateWithFlags(&c->cstream, hipStreamNonBlocking));
+    HIPCHK(c, hipEventCreateWithFlags(&c->ev_red, hipEventDisableTiming));
+    HIPCHK(c, hipEventCreateWithFlags(&c->ev_cp[0], hipEventDisableTiming));
+    HIPCHK(c, hipEventCreateWithFlags(&c->ev_cp[1], hipEventDisableTiming));
+    return AMB_OK;
+}
+
+void par_copy(const std::vector<std::tuple<void *, const void *, size_t>> &r)
+{
+    size_t tot = 0;
+    for (const auto &x : r) tot += std::get<2>(x);
+    const int nth = tot < ((size_t)4 << 20) ? 1 : 8;
+    auto part = [&](int t) {
+        for (const auto &x : r) {
+            const size_t b = std::get<2>(x), lo = b * t / nth, hi = b * (t + 1) / nth;
+            if (hi > lo) memcpy((char *)std::get<0>(x) + lo, (const char *)std::get<1>(x) + lo, hi - lo);
+        }
+    };
+    if (nth == 1) {
+        part(0);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int t = 1; t < nth; ++t) th.emplace_back(part, t);
+    part(0);
+    for (auto &x : th) x.join();
+}
+
 int grow_host(amb_ctx *c, void **p, size_t *have, size_t need)
 {
     if (*have >= need) return AMB_OK;

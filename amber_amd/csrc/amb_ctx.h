@@ -15,6 +15,7 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <tuple>
 #include <vector>
 
 struct AmbGraph;
@@ -132,7 +133,8 @@ struct amb_ctx {
     hipEvent_t ev_red = nullptr, ev_cp[2] = {nullptr, nullptr};
     void *wdesc = nullptr; size_t wdesc_bytes = 0; /* wave descriptors */
     /* pinned host staging, two buffers: a grouped wave's results are copied
-       out of one while the next wave's land in the other */
+       out of one while the next wave's land in the other; the lag-space
+       post-processing stages alternate triangles in them the same way */
     void *hstage[2] = {nullptr, nullptr}; size_t hstage_bytes[2] = {0, 0};
     /* stats */
     amb_stats st{};
@@ -153,6 +155,10 @@ int dev_alloc(amb_ctx *c, void **p, size_t bytes, const char *what);
 void dev_free(amb_ctx *c, void *p, size_t bytes);
 int grow(amb_ctx *c, void **p, size_t *have, size_t need, const char *what);
 int grow_host(amb_ctx *c, void **p, size_t *have, size_t need);
+/* creates the copy stream and its events on first use */
+int copy_stream(amb_ctx *c);
+/* memcpy of several (dst, src, bytes) ranges split over host threads */
+void par_copy(const std::vector<std::tuple<void *, const void *, size_t>> &r);
 int twiddles(amb_ctx *c, int n, float2 **out, FftPlan *pl);
 int find_cube(amb_ctx *c, int id, Cube **cb);
 long plane_pitch(long ng);

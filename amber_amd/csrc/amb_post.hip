@@ -157,7 +157,7 @@ extern "C" int amb_cyl_bispec(amb_ctx *c, int n, int W, const float *bl, const v
     size_t bytes = 0;
     auto sz = [&](size_t b) { bytes += (b + 255) / 256 * 256; };
     sz(sizeof(float) * WW); sz(sizeof(float) * WW); sz(sizeof(double) * MN); sz(sizeof(double) * 2 * MN);
-    sz(sizeof(double) * 2 * MN); sz(sizeof(double) * NN); sz(sizeof(double) * NN); sz(sizeof(double) * MN);
+    sz(sizeof(double) * 2 * MN); sz(sizeof(double) * NN); sz(sizeof(double) * 2 * NN); sz(sizeof(double) * MN);
     sz(sizeof(double) * 2 * NN); sz(sizeof(double) * 2 * MN); sz(sizeof(double2) * N);
     void *d = nullptr;
     int rc = dev_alloc(c, &d, bytes, "post-processing scratch");
@@ -170,7 +170,7 @@ extern "C" int amb_cyl_bispec(amb_ctx *c, int n, int W, const float *bl, const v
     double *d_C1 = (double *)take(sizeof(double) * 2 * MN);
     double *d_Z2 = (double *)take(sizeof(double) * 2 * MN);
     double *d_C2 = (double *)take(sizeof(double) * NN);
-    double *d_out = (double *)take(sizeof(double) * NN);
+    double *d_out = (double *)take(sizeof(double) * 2 * NN); /* two slots */
     double *d_win = (double *)take(sizeof(double) * MN);
     double *d_B1 = (double *)take(sizeof(double) * 2 * NN);
     double *d_A2 = (double *)take(sizeof(double) * 2 * MN);
@@ -201,12 +201,38 @@ extern "C" int amb_cyl_bispec(amb_ctx *c, int n, int W, const float *bl, const v
                            d_A2);
         e = hipGetLastError();
     }
+    /* Triangle t uses slot t & 1: a pinned host block [bl | den | result] and
+       a device result array. Its inputs are staged by host threads while
+       the GPU runs triangle t - 1; its result goes to the host on the copy
+       stream while the GPU runs triangle t + 1; slot reuse (t + 2) waits for
+       that copy (ev_cp), which also orders the reuse of the input block. */
+    const size_t in_b = sizeof(float) * WW, slot_b = 2 * in_b + sizeof(double) * NN;
+    for (int k = 0; k < 2 && !rc; ++k) rc = grow_host(c, &c->hstage[k], &c->hstage_bytes[k], slot_b);
+    if (!rc) rc = copy_stream(c);
+    if (rc) {
+        hipStreamSynchronize(st);
+        dev_free(c, d, bytes);
+        return rc;
+    }
+    auto drain = [&](int t) { /* result of triangle t from its slot to out */
+        char *h = (char *)c->hstage[t & 1];
+        par_copy({{out + NN * t, h + 2 * in_b, sizeof(double) * NN}});
+    };
     const dim3 g1((N + 255) / 256, M), g2((N + 255) / 256, N);
     for (int t = 0; t < n && e == hipSuccess; ++t) {
-        e = hipMemcpyAsync(d_bl, bl + WW * t, sizeof(float) * WW, hipMemcpyHostToDevice, st);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(d_den, (const char *)den + sizeof(float) * WW * t, sizeof(float) * WW,
-                               hipMemcpyHostToDevice, st);
+        const int sl = t & 1;
+        char *h = (char *)c->hstage[sl];
+        double *d_res = d_out + NN * sl;
+        std::vector<std::tuple<void *, const void *, size_t>> cp{
+            {h, bl + WW * t, in_b}, {h + in_b, (const char *)den + in_b * t, in_b}};
+        if (t >= 2) {
+            e = hipEventSynchronize(c->ev_cp[sl]);
+            if (e != hipSuccess) break;
+            cp.emplace_back(out + NN * (t - 2), h + 2 * in_b, sizeof(double) * NN);
+        }
+        par_copy(cp);
+        e = hipMemcpyAsync(d_bl, h, in_b, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(d_den, h + in_b, in_b, hipMemcpyHostToDevice, st);
         if (e != hipSuccess) break;
         const double *win_p = p->window ? d_win : nullptr;
         if (den_is_int)
@@ -223,14 +249,21 @@ extern "C" int amb_cyl_bispec(amb_ctx *c, int n, int W, const float *bl, const v
         }
         timer_stop(c);
         if (e == hipSuccess) {
-            hipLaunchKernelGGL(k_cyl_out, g2, dim3(256), 0, st, d_C2, N, scale, d_out);
+            hipLaunchKernelGGL(k_cyl_out, g2, dim3(256), 0, st, d_C2, N, scale, d_res);
             e = hipGetLastError();
         }
+        if (e == hipSuccess) e = hipEventRecord(c->ev_red, st);
+        if (e == hipSuccess) e = hipStreamWaitEvent(c->cstream, c->ev_red, 0);
         if (e == hipSuccess)
-            e = hipMemcpyAsync(out + NN * t, d_out, sizeof(double) * NN, hipMemcpyDeviceToHost, st);
+            e = hipMemcpyAsync(h + 2 * in_b, d_res, sizeof(double) * NN, hipMemcpyDeviceToHost, c->cstream);
+        if (e == hipSuccess) e = hipEventRecord(c->ev_cp[sl], c->cstream);
     }
-    const hipError_t es = hipStreamSynchronize(st);
+    hipError_t es = hipStreamSynchronize(st);
     if (e == hipSuccess) e = es;
+    es = hipStreamSynchronize(c->cstream);
+    if (e == hipSuccess) e = es;
+    if (e == hipSuccess)
+        for (int t = std::max(0, n - 2); t < n; ++t) drain(t);
     timers_collect(c);
     c->st.flop_post += 8.0 * M * (double)N * N * n;
     dev_free(c, d, bytes);

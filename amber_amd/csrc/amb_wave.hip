@@ -247,12 +247,7 @@ int run_wave(amb_ctx *c, Wave &w, int hb, const std::function<int()> &overlap)
     void **outp = hb ? &c->out2 : &c->out;
     /* the last copy out of buffer hb ended before its wave was finished */
     if (!rc) rc = grow(c, outp, hb ? &c->out2_bytes : &c->out_bytes, w.o_total, "lag arrays");
-    if (!rc && !c->cstream) {
-        HIPCHK(c, hipStreamCreateWithFlags(&c->cstream, hipStreamNonBlocking));
-        HIPCHK(c, hipEventCreateWithFlags(&c->ev_red, hipEventDisableTiming));
-        HIPCHK(c, hipEventCreateWithFlags(&c->ev_cp[0], hipEventDisableTiming));
-        HIPCHK(c, hipEventCreateWithFlags(&c->ev_cp[1], hipEventDisableTiming));
-    }
+    if (!rc) rc = copy_stream(c);
     if (!rc) rc = grow(c, &c->wdesc, &c->wdesc_bytes, w.d_total, "wave descriptors");
     if (!rc) rc = grow_host(c, &c->hstage[hb], &c->hstage_bytes[hb], w.o_total + sizeof(unsigned));
     if (rc) return rc;
