@@ -38,11 +38,20 @@ CONFIGS = {
 PEAK_TFLOPS = {"bf16x3": 2500.0, "fp32": 157.3, "fp8": 5000.0}
 PASSES = {"bf16x3": 3, "fp32": 1, "fp8": 1}
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per contraction launch of the default workload (grouped waves,
-# pipelined BF16x3 form): FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 from two PMC
-# runs of this workload, the factors being what the same counters read for
-# a plain 1 GiB device copy.  Counters cannot be read inside a timed run.
-MEASURED_TRAFFIC = {("cfg3", "bf16x3", "grouped"): 4.497e11}
+# HBM bytes of all contraction launches of one step of the default workload
+# (grouped waves, pipelined BF16x3 form): FETCH_SIZE x 2.00 + WRITE_SIZE x
+# 1.00 summed over the step's dispatches from two PMC runs of this workload,
+# the factors being what the same counters read for a plain 1 GiB device
+# copy.  Counters cannot be read inside a timed run; `traffic` is this total
+# over the step's launches.
+MEASURED_TRAFFIC_STEP = {("cfg3", "bf16x3", "grouped"): 4.690e13}
+
+
+def traffic_per_launch(args, st):
+    tot = MEASURED_TRAFFIC_STEP.get((args.config, args.precision, args.batch_mode))
+    if tot is None or args.tri_limit or os.environ.get("AMB_KVAR", "2") != "2":
+        return None
+    return round(tot / max(st["n_contract_launch"] / max(args.steps, 1), 1), -6)
 
 
 def log(msg):
@@ -147,8 +156,7 @@ def main():
     peak = PEAK_TFLOPS[args.precision]
     roof = {"bound": "mfma", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
             "frac": round(ach / peak, 4),
-            "traffic": (MEASURED_TRAFFIC.get((args.config, args.precision, args.batch_mode))
-                        if not args.tri_limit and os.environ.get("AMB_KVAR", "2") == "2" else None),
+            "traffic": traffic_per_launch(args, st),
             "kernel": "k_contract<%s>" % args.precision,
             "mfma_passes": PASSES[args.precision],
             "avg_launch_ms": round(ms_c / max(st["n_contract_launch"], 1), 3),
