@@ -921,12 +921,33 @@ __global__ __launch_bounds__(512, 1) void k_contract_grouped_v2(const ContractAr
     V2_STAGES
     __shared__ long item;
     const long total = begin[ntri];
+    /* XCD queues: runs of XRUN consecutive items (one tile's p1 groups, which
+       stream the same D2 / D3 chunks) are dealt round-robin to the 8 XCDs;
+       workgroup b sits on XCD b mod 8 and takes items from that XCD's
+       counter, then from the others once it runs dry.  Within a queue the
+       item index only grows, so a queue that yields an index past the end
+       is empty for good: each thread 0 makes at most 8 such failed takes. */
+    constexpr long XRUN = 32;
+    int xq = blockIdx.x & 7, left = 8;
     for (;;) {
-        if (threadIdx.x == 0) item = (long)atomicAdd(counter, 1ull);
+        if (threadIdx.x == 0) {
+            long it = -1;
+            while (left > 0) {
+                const unsigned long long j = atomicAdd(counter + xq, 1ull);
+                const long cand = ((long)(j / XRUN) * 8 + xq) * XRUN + (long)(j % XRUN);
+                if (cand < total) {
+                    it = cand;
+                    break;
+                }
+                xq = (xq + 1) & 7;
+                --left;
+            }
+            item = it;
+        }
         __syncthreads();
         const long w = item;
         __syncthreads();
-        if (w >= total) return;
+        if (w < 0) return;
         int lo = 0, hi = ntri - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
@@ -986,7 +1007,10 @@ __global__ __launch_bounds__(256 * NWR, 1) void k_contract_grouped(const Contrac
     }
 }
 
-__global__ void k_set_u64(unsigned long long *p, unsigned long long v) { *p = v; }
+__global__ void k_set_u64(unsigned long long *p, unsigned long long v, int n)
+{
+    if ((int)threadIdx.x < n) p[threadIdx.x] = v;
+}
 
 __global__ void k_fill_u32(unsigned *p, long n, unsigned v)
 {
@@ -1089,7 +1113,7 @@ hipError_t amb_launch_contract_grouped(hipStream_t s, int prec, int norm, int va
     if (ntri <= 0 || total <= 0) return hipSuccess;
     if (prec == AMB_PREC_FP8 && norm) return hipErrorInvalidValue;
     /* a kernel fill, not a memset, so the reset can live in a captured graph */
-    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, s, d_counter, 0ull);
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, s, d_counter, 0ull, AMB_GROUPED_COUNTERS);
     if ((long)nblocks > total) nblocks = (int)total;
     dim3 grid(nblocks);
     if (prec == AMB_PREC_FP8) launch_grouped_var<AMB_PREC_FP8, 0>(var, grid, s, d_descs, d_begin, ntri, d_counter);

@@ -97,7 +97,7 @@ void layout(Wave &w)
     w.off_b = o; o = al(o + sizeof(long) * (n + 1));
     w.off_nb = o; o = al(o + sizeof(long) * (w.nnorm + 1));
     w.off_cb = o; o = al(o + sizeof(long) * (n + 1));
-    w.off_ctr = o; o = al(o + sizeof(unsigned long long) * 2);
+    w.off_ctr = o; o = al(o + sizeof(unsigned long long) * 2 * AMB_GROUPED_COUNTERS);
     w.off_mg = o; o = al(o + sizeof(unsigned));
     w.d_total = o;
 }
@@ -179,7 +179,7 @@ hipError_t launch_wave(amb_ctx *c, hipStream_t s, const Wave &w, const WaveBufs 
         if (timed) timer_start(c, &c->st.ms_norm);
         if (e == hipSuccess)
             e = amb_launch_contract_grouped(s, nprec, 1, var, (const ContractArgs *)(b.desc + w.off_na),
-                                            (const long *)(b.desc + w.off_nb), w.nnorm, w.nitems, ctr + 1,
+                                            (const long *)(b.desc + w.off_nb), w.nnorm, w.nitems, ctr + AMB_GROUPED_COUNTERS,
                                             c->grouped_blocks);
         if (timed) timer_stop(c);
     }
