@@ -44,7 +44,7 @@ HBM_PEAK_GBS = 8000.0
 # the factors being what the same counters read for a plain 1 GiB device
 # copy.  Counters cannot be read inside a timed run; `traffic` is this total
 # over the step's launches.
-MEASURED_TRAFFIC_STEP = {("cfg3", "bf16x3", "grouped"): 4.690e13}
+MEASURED_TRAFFIC_STEP = {("cfg3", "bf16x3", "grouped"): 1.909e13}
 
 
 def traffic_per_launch(args, st):
