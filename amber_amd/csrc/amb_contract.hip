@@ -913,6 +913,26 @@ __global__ __launch_bounds__(512, 1) void k_contract_v2(ContractArgs a, int ngro
     contract_item_v2<NORM>(a, w / ngroups, w % ngroups, S0, S1);
 }
 
+/* XCD queues of the grouped contraction: runs of XRUN consecutive items (one
+ * tile's p1 groups, which stream the same D2 / D3 chunks) are dealt
+ * round-robin to the 8 XCDs; workgroup b sits on XCD b mod 8 and takes items
+ * from that XCD's counter (xq starts at b mod 8), then from the others once
+ * it runs dry.  Within a queue the item index only grows, so a queue that
+ * yields an index past the end is empty for good: each caller makes at most
+ * 8 such failed takes (left), then gets -1. */
+__device__ __forceinline__ long xcd_take(unsigned long long *counter, long total, int &xq, int &left)
+{
+    constexpr long XRUN = 32;
+    while (left > 0) {
+        const unsigned long long j = atomicAdd(counter + xq, 1ull);
+        const long cand = ((long)(j / XRUN) * 8 + xq) * XRUN + (long)(j % XRUN);
+        if (cand < total) return cand;
+        xq = (xq + 1) & 7;
+        --left;
+    }
+    return -1;
+}
+
 template <int NORM>
 __global__ __launch_bounds__(512, 1) void k_contract_grouped_v2(const ContractArgs *__restrict__ descs,
                                                                 const long *__restrict__ begin, int ntri,
@@ -921,29 +941,9 @@ __global__ __launch_bounds__(512, 1) void k_contract_grouped_v2(const ContractAr
     V2_STAGES
     __shared__ long item;
     const long total = begin[ntri];
-    /* XCD queues: runs of XRUN consecutive items (one tile's p1 groups, which
-       stream the same D2 / D3 chunks) are dealt round-robin to the 8 XCDs;
-       workgroup b sits on XCD b mod 8 and takes items from that XCD's
-       counter, then from the others once it runs dry.  Within a queue the
-       item index only grows, so a queue that yields an index past the end
-       is empty for good: each thread 0 makes at most 8 such failed takes. */
-    constexpr long XRUN = 32;
     int xq = blockIdx.x & 7, left = 8;
     for (;;) {
-        if (threadIdx.x == 0) {
-            long it = -1;
-            while (left > 0) {
-                const unsigned long long j = atomicAdd(counter + xq, 1ull);
-                const long cand = ((long)(j / XRUN) * 8 + xq) * XRUN + (long)(j % XRUN);
-                if (cand < total) {
-                    it = cand;
-                    break;
-                }
-                xq = (xq + 1) & 7;
-                --left;
-            }
-            item = it;
-        }
+        if (threadIdx.x == 0) item = xcd_take(counter, total, xq, left);
         __syncthreads();
         const long w = item;
         __syncthreads();
@@ -975,8 +975,8 @@ __global__ __launch_bounds__(256 * NWR, 1) void k_contract(ContractArgs a, int n
     contract_item<MODE, NORM, NWR, PIPE>(a, w / ngroups, w % ngroups, L);
 }
 
-/* Many triangles in one persistent launch: each workgroup takes the next
- * item from a global counter until the queue is empty.  Items are ordered
+/* Many triangles in one persistent launch: each workgroup takes items from
+ * the XCD queues (xcd_take) until they are empty.  Items are ordered
  * by the host (largest triangles first, tile-major within a triangle).
  * begin[t] is the first item of triangle t, begin[ntri] the total. */
 template <int MODE, int NORM, int NWR, int PIPE>
@@ -987,12 +987,13 @@ __global__ __launch_bounds__(256 * NWR, 1) void k_contract_grouped(const Contrac
     __shared__ Smem<MODE, nstages<PIPE>()> L;
     __shared__ long item;
     const long total = begin[ntri];
+    int xq = blockIdx.x & 7, left = 8;
     for (;;) {
-        if (threadIdx.x == 0) item = (long)atomicAdd(counter, 1ull);
+        if (threadIdx.x == 0) item = xcd_take(counter, total, xq, left);
         __syncthreads();
         const long w = item;
         __syncthreads();
-        if (w >= total) return;
+        if (w < 0) return;
         int lo = 0, hi = ntri - 1;
         while (lo < hi) { /* last t with begin[t] <= w */
             const int mid = (lo + hi + 1) >> 1;

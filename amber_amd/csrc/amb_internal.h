@@ -93,9 +93,8 @@ double amb_contract_item_flops(int prec, long pitch);
 bool amb_contract_args_ok(int prec, int norm, const ContractArgs &a);
 /* persistent contraction over ntri triangles: descs/begin in device memory,
  * begin[t] = first item of triangle t (largest first), begin[ntri] = total;
- * d_counter (AMB_GROUPED_COUNTERS words: the pipelined form keeps one queue
- * per XCD, the others use the first) is reset by a kernel inside the launch
- * sequence */
+ * d_counter (AMB_GROUPED_COUNTERS words, one work queue per XCD) is reset
+ * by a kernel inside the launch sequence */
 #define AMB_GROUPED_COUNTERS 8
 hipError_t amb_launch_contract_grouped(hipStream_t s, int prec, int norm, int var, const ContractArgs *d_descs,
                                        const long *d_begin, int ntri, long total, unsigned long long *d_counter,
