@@ -16,8 +16,7 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-# AMB_LIB names another build of the same library (A/B timing of builds)
-LIB_PATH = os.environ.get("AMB_LIB") or os.path.join(_HERE, "libamber_mi355x.so")
+LIB_PATH = os.path.join(_HERE, "libamber_mi355x.so")
 if not os.path.exists(LIB_PATH):
     raise ImportError("amber_amd: %s is missing; build it with `python -m amber_amd.build`" % LIB_PATH)
 _lib = C.CDLL(LIB_PATH)
@@ -93,7 +92,7 @@ class Stats(C.Structure):
                 ("ms_maps", C.c_double), ("ms_maps_total", C.c_double), ("flop_maps", C.c_double),
                 ("n_maps_launch", C.c_int64), ("ms_grid", C.c_double), ("n_grid_groups", C.c_int64),
                 ("ms_post", C.c_double), ("flop_post", C.c_double), ("cert_margin_max", C.c_double),
-                ("n_uncertified", C.c_int64)]
+                ("n_uncertified", C.c_int64), ("flop_contract_distinct", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

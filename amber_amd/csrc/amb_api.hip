@@ -469,21 +469,33 @@ int prepare_tri(amb_ctx *c, const PlaneSrc &src, int id, int r1, int r2, int r3,
     return AMB_OK;
 }
 
-void count_tri(const TriJob &j, const int32_t *cnt, int64_t *evals, int64_t *pairs)
+void count_tri(const TriJob &j, const int32_t *cnt, int64_t *evals, int64_t *pairs, int64_t *diag)
 {   /* integer sums: exact, and free of a floating-point dependency chain */
-    int64_t ev = 0, pr = 0;
+    int64_t ev = 0, pr = 0, dg = 0;
     for (size_t i = 0; i < j.cells; ++i) {
         ev += cnt[i] > 0;
         pr += cnt[i];
     }
+    /* triplets a symmetry maps onto themselves: p1 == p2 (lag row d2 = 0)
+       for r1 == r2, p2 == p3 (the d2 == d3 diagonal) for r2 == r3 */
+    const size_t W = (size_t)j.W, mid = W / 2;
+    if (j.sym12)
+        for (size_t k = 0; k < W; ++k) dg += cnt[mid * W + k];
+    else if (j.sym23)
+        for (size_t k = 0; k < W; ++k) dg += cnt[k * W + k];
     *evals = ev;
     *pairs = pr;
+    *diag = dg;
 }
 
-void account_counts(amb_ctx *c, const TriJob &j, int64_t evals, int64_t pairs)
+void account_counts(amb_ctx *c, const TriJob &j, int64_t evals, int64_t pairs, int64_t diag)
 {
     c->st.evals += evals;
     c->st.flop_contract += 2.0 * (double)pairs * (double)j.P;
+    /* distinct sums: under a symmetry the triplets pair up, except the
+       diag ones that map onto themselves, leaving (pairs + diag) / 2 */
+    const double distinct = (j.sym12 || j.sym23) ? 0.5 * (double)(pairs + diag) : (double)pairs;
+    c->st.flop_contract_distinct += 2.0 * distinct * (double)j.P;
     c->st.flop_contract_exec += j.flop_exec;
 }
 
@@ -501,9 +513,9 @@ int check_cert(amb_ctx *c, float m)
 
 void account_tri(amb_ctx *c, const TriJob &j, const int32_t *cnt)
 {
-    int64_t ev, pr;
-    count_tri(j, cnt, &ev, &pr);
-    account_counts(c, j, ev, pr);
+    int64_t ev, pr, dg;
+    count_tri(j, cnt, &ev, &pr, &dg);
+    account_counts(c, j, ev, pr, dg);
 }
 
 /* one triangle with its own launches; output pointers are host memory */

@@ -173,8 +173,8 @@ int prepare_tri(amb_ctx *c, const PlaneSrc &src, int cube_id, int r1, int r2, in
 /* evals and algorithmic flops of one finished triangle from its counts;
  * count_tri is thread-safe, account_counts adds to the context's stats */
 void account_tri(amb_ctx *c, const TriJob &j, const int32_t *cnt);
-void count_tri(const TriJob &j, const int32_t *cnt, int64_t *evals, int64_t *pairs);
-void account_counts(amb_ctx *c, const TriJob &j, int64_t evals, int64_t pairs);
+void count_tri(const TriJob &j, const int32_t *cnt, int64_t *evals, int64_t *pairs, int64_t *diag);
+void account_counts(amb_ctx *c, const TriJob &j, int64_t evals, int64_t pairs, int64_t diag);
 /* the normalisation certificate of one call or wave (worst grid distance m):
  * recorded in the stats; an error if out of the int32 range of eighths, or
  * beyond 1/32 with cert_strict set */

@@ -200,7 +200,7 @@ int finish_wave(amb_ctx *c, const Wave &w, const char *host, const amb_tri_req *
     /* the triangles' copies out of the pinned staging and their count sums
        are independent: spread them over a few host threads */
     const size_t nj = w.jobs.size();
-    std::vector<int64_t> ev(nj), pr(nj);
+    std::vector<int64_t> ev(nj), pr(nj), dg(nj);
     auto one = [&](size_t i) {
         const TriJob &j = w.jobs[i];
         const amb_tri_req &q = req[w.req[i]];
@@ -210,7 +210,7 @@ int finish_wave(amb_ctx *c, const Wave &w, const char *host, const amb_tri_req *
         if (q.cnt) memcpy(q.cnt, cnt, sizeof(int32_t) * j.cells);
         if (j.want_norm && q.norm)
             memcpy(q.norm, o + al(j.cells * sizeof(float)) + al(j.cells * sizeof(int)), sizeof(float) * j.cells);
-        count_tri(j, cnt, &ev[i], &pr[i]);
+        count_tri(j, cnt, &ev[i], &pr[i], &dg[i]);
     };
     const size_t nth = std::min<size_t>(nj, 8);
     if (nth <= 1) {
@@ -224,7 +224,7 @@ int finish_wave(amb_ctx *c, const Wave &w, const char *host, const amb_tri_req *
             });
         for (auto &x : th) x.join();
     }
-    for (size_t i = 0; i < nj; ++i) account_counts(c, w.jobs[i], ev[i], pr[i]);
+    for (size_t i = 0; i < nj; ++i) account_counts(c, w.jobs[i], ev[i], pr[i], dg[i]);
     if (w.nnorm) {
         float m;
         memcpy(&m, host + w.o_total, sizeof m);

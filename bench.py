@@ -1,7 +1,7 @@
 """Benchmark of the 4_fbe estimator hot path on one MI355X.
 
-A step is one full pass over every closed triangle of the workload: ring
-gather + c2r of every needed (ring, channel) plane, the per-p1 MFMA
+A step is one slice of the closed triangles of the workload: ring gather +
+c2r of every (ring, channel) plane the slice needs, the per-p1 MFMA
 contraction, the lag reduction with counts, and the copy of every
 triangle's W x W accumulator and count arrays to host memory.  The cube is
 resident in HBM before the timed region starts.
@@ -10,8 +10,12 @@ Default workload (the metric's configuration): 768 channels x 512^2 grid,
 24 log-spaced annular bins 1..200 cells, all 506 closed triangles, all lags
 (nc10 = 767), split-BF16 contraction.
 
-    python bench.py [--gpus 1] [--steps 1] [--warmup 1] [--config cfg3]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 2] [--config cfg3]
                     [--precision bf16x3|fp32|fp8] [--no-cpu-baseline]
+
+A pass over the workload is split into --steps slices of equal contraction
+cost (contiguous in the triangle list), so the timed steps together cover
+every triangle exactly once; warm-up steps are slices too.
 """
 import argparse
 import json
@@ -47,22 +51,54 @@ HBM_PEAK_GBS = 8000.0
 MEASURED_TRAFFIC_STEP = {("cfg3", "bf16x3", "grouped"): 1.909e13}
 
 
-def traffic_per_launch(args, st):
+def traffic_per_launch(args, st, nsteps):
     tot = MEASURED_TRAFFIC_STEP.get((args.config, args.precision, args.batch_mode))
     if tot is None or args.tri_limit or os.environ.get("AMB_KVAR", "2") != "2":
         return None
-    return round(tot / max(st["n_contract_launch"] / max(args.steps, 1), 1), -6)
+    # the measured total covers one pass over every triangle, as the timed steps do
+    return round(tot / max(st["n_contract_launch"], 1), -6)
 
 
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
+def tri_costs(A, tris, hi):
+    """Relative contraction cost of each triangle: Ng(r1)^2 pixels times the
+    triplets it computes (half of them where r1 == r2 or r2 == r3 lets the
+    kernel mirror the other half)."""
+    ng = np.array([A.ng(hi[t[0]]) for t in tris], np.float64)
+    sym = np.array([(t[0] == t[1]) or (t[1] == t[2]) for t in tris])
+    return ng * ng * np.where(sym, 0.5, 1.0)
+
+
+def cost_slices(costs, nslices):
+    """Contiguous slices of the triangle list (kept in its (r1, r2, r3) order,
+    so triangles that share a reference ring's planes mostly share a slice)
+    of near-equal total cost; every triangle lies in exactly one slice."""
+    n = len(costs)
+    nslices = max(1, min(int(nslices), n))
+    cum = np.concatenate([[0.0], np.cumsum(costs)])
+    bounds = [0]
+    for k in range(1, nslices):
+        target = cum[-1] * k / nslices
+        i = int(np.argmin(np.abs(cum - target)))  # cut before triangle i
+        i = min(max(i, bounds[-1] + 1), n - (nslices - k))
+        bounds.append(i)
+    bounds.append(n)
+    return [np.arange(bounds[k], bounds[k + 1]) for k in range(nslices)]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20,
+                    help="timed steps; the workload's triangles are split into this many slices of equal cost, "
+                         "so the timed steps together make exactly one pass over every triangle")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed steps, each one slice (heaviest first)")
+    ap.add_argument("--budget-s", type=float, default=540.0,
+                    help="wall-clock budget of the whole run: warm-up steps, the FP64 check and the CPU "
+                         "baseline are cut to fit it")
     ap.add_argument("--config", default="cfg3", choices=sorted(CONFIGS) + ["cfg4", "cfg5", "post", "maps", "grid"])
     ap.add_argument("--qps", type=float, default=0.0, help="cfg5: mean arrival rate (0: 60%% of the measured capacity)")
     ap.add_argument("--duration", type=float, default=30.0, help="cfg5: seconds of arrivals")
@@ -79,6 +115,7 @@ def main():
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (counter calibration)")
     args = ap.parse_args()
+    t_run = time.time()
     if args.copy_calib:
         copy_calib(args.copy_calib)
         return
@@ -123,45 +160,68 @@ def main():
     log("uploaded cube (%.2f GB of FP64 file data) in %.2f s" % ((gv.nbytes + samp.nbytes) / 1e9, upload_s))
 
     W = 2 * nc10 + 1
-    out = (np.zeros((len(tris), W, W), np.float32), np.zeros((len(tris), W, W), np.int32),
-           np.zeros((len(tris), W, W), np.float32) if args.estimator == "v2" else None)
+    costs = tri_costs(A, tris, hi)
+    slices = cost_slices(costs, max(args.steps, 1))
+    nmax = max(len(sl) for sl in slices)
+    out = (np.zeros((nmax, W, W), np.float32), np.zeros((nmax, W, W), np.int32),
+           np.zeros((nmax, W, W), np.float32) if args.estimator == "v2" else None)
+    log("%d triangles in %d slices of %d-%d triangles (cost spread %.3f-%.3f of a pass)"
+        % (len(tris), len(slices), min(len(x) for x in slices), nmax,
+           min(costs[x].sum() for x in slices) / costs.sum(), max(costs[x].sum() for x in slices) / costs.sum()))
 
-    def step():
-        # every step rebuilds its filtered planes: nothing carries over
+    def step(sl):
+        # every step rebuilds the filtered planes its slice needs: nothing carries over
         ctx.drop_planes(cube)
-        ctx.bispec_batch(cube, tris, out=out)
+        n = len(sl)
+        ctx.bispec_batch(cube, tris[sl], out=tuple(None if o is None else o[:n] for o in out))
 
+    # warm-up: the heaviest slices first, so every scratch buffer has its
+    # full size before the timed steps
+    heavy = sorted(range(len(slices)), key=lambda k: -costs[slices[k]].sum())
+    nwarm = 0
     for w in range(args.warmup):
+        sl = slices[heavy[w % len(slices)]]
+        if w > 0 and time.time() - t_run + 2 * costs[sl].sum() / costs.sum() * 60.0 > 0.25 * args.budget_s:
+            log("warm-up cut at %d steps to keep the run within its budget" % w)
+            break
         t0 = time.time()
-        step()
-        log("warmup step %d: %.2f s" % (w, time.time() - t0))
+        step(sl)
+        nwarm += 1
+        log("warmup step %d (%d triangles): %.2f s" % (w, len(sl), time.time() - t0))
     ctx.reset_stats()
     per_step = []
     lat = []
     t_start = time.time()
-    for s in range(args.steps):
+    for s_ in range(len(slices)):
         t0 = time.time()
-        step()
+        step(slices[s_])
         per_step.append(time.time() - t0)
         lat.extend(ctx.batch_times().tolist())
-        log("timed step %d: %.2f s" % (s, per_step[-1]))
+        log("timed step %d (%d triangles): %.2f s" % (s_, len(slices[s_]), per_step[-1]))
     elapsed = time.time() - t_start
+    nsteps = len(slices)
     st = ctx.stats()
-    evals_per_step = st["evals"] / max(args.steps, 1)
-    value = evals_per_step * args.steps / elapsed
+    value = st["evals"] / elapsed
 
-    # roofline of the dominant kernel (the contraction), from HIP events
+    # roofline of the dominant kernel (the contraction), from HIP events on
+    # the library's stream; work is counted once per distinct sum (a pair
+    # of triplets that r1 == r2 or r2 == r3 makes equal counts once)
     ms_c = st["ms_contract"]
-    ach = st["flop_contract"] / (ms_c * 1e-3) / 1e12 if ms_c > 0 else 0.0
+    ach = st["flop_contract_distinct"] / (ms_c * 1e-3) / 1e12 if ms_c > 0 else 0.0
     peak = PEAK_TFLOPS[args.precision]
     roof = {"bound": "mfma", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
             "frac": round(ach / peak, 4),
-            "traffic": traffic_per_launch(args, st),
+            "traffic": traffic_per_launch(args, st, nsteps),
             "kernel": "k_contract<%s>" % args.precision,
+            "work": "2 flop per (distinct triplet, pixel): 2*V*Ng^2 per triangle, halved (plus the self-mirrored "
+                    "triplets) where r1 == r2 or r2 == r3",
             "mfma_passes": PASSES[args.precision],
+            "frac_of_pass_peak": round(ach * PASSES[args.precision] / peak, 4),
             "avg_launch_ms": round(ms_c / max(st["n_contract_launch"], 1), 3),
-            # MFMA work actually issued (whole tiles, padded pitch, all passes;
-            # r1 == r2 tiles skipped) against the dense peak of the instruction
+            "launches": int(st["n_contract_launch"]),
+            "algorithmic_all_triplets_tflops": round(st["flop_contract"] / (ms_c * 1e-3) / 1e12, 2) if ms_c > 0 else None,
+            # MFMA work actually issued (whole tiles, padded pitch, all passes,
+            # skipped tiles excluded) against the dense peak of the instruction
             "issued_tflops": round(st["flop_contract_exec"] / (ms_c * 1e-3) / 1e12, 2) if ms_c > 0 else None,
             "issued_frac_of_instr_peak": round(st["flop_contract_exec"] / (ms_c * 1e-3) / 1e12 /
                                                peak, 4) if ms_c > 0 else None}
@@ -171,27 +231,32 @@ def main():
 
     err = None
     if not args.no_error:
-        err = fp64_error(A, ctx, cube, cfg, gv, samp, uf, fk, lo, hi, tris)
+        err = fp64_error(A, ctx, cube, cfg, gv, samp, uf, fk, lo, hi, tris,
+                         deadline=t_run + args.budget_s - args.cpu_seconds - 30.0)
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(A, cfg, gv, samp, uf, fk, lo, hi, tris, args.cpu_seconds)
+        cpu = cpu_baseline(A, cfg, gv, samp, uf, fk, lo, hi, tris,
+                           min(args.cpu_seconds, max(5.0, t_run + args.budget_s - time.time() - 20.0)))
 
     workload = cfg["desc"] + ", %d triangles" % len(tris)
     if args.estimator == "v2":
-        workload += ", amber_v2 estimator (data and certified normalisation contractions)"
+        workload += ", amber_v2 estimator (data and normalisation contractions)"
     if args.tri_limit:
         workload += " (DEVELOPMENT SUBSET, not the metric workload)"
     line = {
         "metric": "triangle-bin x lag-pair evals/sec at 768ch x 512^2; rel err vs FP64; p50/p99 latency",
-        "value": round(value, 1), "unit": "evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(1e3 * elapsed / max(args.steps, 1), 2), "higher_is_better": True,
+        "value": round(value, 1), "unit": "evals/s", "n_gpus": 1, "steps": nsteps, "warmup": nwarm,
+        "ms_per_step": round(1e3 * elapsed / nsteps, 2), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
         "dtype": {"bf16x3": "bf16", "fp32": "f32", "fp8": "fp8_e4m3"}[args.precision],
         "precision": args.precision, "data": "synthetic",
         "config": {"workload": workload, "estimator": args.estimator, "channels": nc, "grid": N,
                    "bins": cfg["rings"][2],
                    "triangles": int(len(tris)), "nc10": nc10, "flagged": cfg["nflag"],
-                   "evals_per_step": int(evals_per_step)},
+                   "evals_per_pass": int(st["evals"]),
+                   "step": "one slice of the triangle list (%d slices of equal contraction cost); the %d timed "
+                           "steps cover every triangle once" % (nsteps, nsteps)},
+        "pass_s": round(elapsed, 3),
         "roofline": roof, "stages": stages,
         "triangle_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                                 "p99": round(float(np.percentile(lat, 99)), 3),
@@ -201,10 +266,10 @@ def main():
                                         "under load is --config cfg5"},
         "rel_err_vs_fp64": err, "cpu_baseline": cpu,
         "upload_s_pcie_incl": round(upload_s, 3),
+        "run_s": round(time.time() - t_run, 1),
     }
     if args.estimator == "v2":
         # waves whose normalisation sums were past 1/32 of their 1/8 grid
-        # (FP32 accumulation error, as in the reference's own loop)
         line["norm_certificate"] = {"uncertified_waves": int(st["n_uncertified"]),
                                     "worst_grid_distance": round(float(st["cert_margin_max"]), 4)}
     ctx.close()
@@ -274,8 +339,8 @@ def bench_cfg4(A, args):
         return out, min(ts), st
 
     ctx.set_batch_mode("grouped")
-    ref, t_grp, st = timed("grouped", lambda: [drop_all(ctx, cubes), ctx.bispec_multi(reqs)][1],
-                           1 + max(args.steps, 1))
+    reps = 1 + min(max(args.steps, 1), 2)
+    ref, t_grp, st = timed("grouped", lambda: [drop_all(ctx, cubes), ctx.bispec_multi(reqs)][1], reps)
     evals = st["evals"]
     ctx.set_batch_mode("per_triangle")
     per, t_per, _ = timed("per_triangle", lambda: [drop_all(ctx, cubes), ctx.bispec_multi(reqs)][1], 1)
@@ -284,7 +349,7 @@ def bench_cfg4(A, args):
     t = time.time()
     g = ctx.graph(reqs)
     t_cap = time.time() - t
-    gout, t_graph, _ = timed("graph replay", g.run, 1 + max(args.steps, 1))
+    gout, t_graph, _ = timed("graph replay", g.run, reps)
     same_graph = all(np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) for a, b in zip(ref, gout))
     g.close()
     line = {"metric": "cfg4 mixed-batch evals/sec (200 uneven triangles, 9 cubes)", "unit": "evals/s",
@@ -536,8 +601,10 @@ def copy_calib(mib):
     print(json.dumps({"copy_calib_bytes": n, "dispatches": 3}), flush=True)
 
 
-def fp64_error(A, ctx, cube, cfg, gv, samp, uf, fk, lo, hi, tris):
-    """Relative error vs the FP64 oracle on sampled lag cells of two triangles."""
+def fp64_error(A, ctx, cube, cfg, gv, samp, uf, fk, lo, hi, tris, deadline=None):
+    """Relative error vs the FP64 oracle on sampled lag cells of four
+    triangles (Ng quantiles, the largest first); a triangle whose oracle run
+    would pass the deadline is left out and said so."""
     import oracle as O
     N, nc, nc10 = cfg["N"], cfg["nc"], cfg["nc10"]
     mask = O.chan_mask(nc, fk, uf)
@@ -546,10 +613,15 @@ def fp64_error(A, ctx, cube, cfg, gv, samp, uf, fk, lo, hi, tris):
     # three rings cannot close a triangle on the grid has B == 0 exactly and
     # is skipped (its relative error is undefined)
     order = np.argsort(ngs, kind="stable")
-    picks = sorted(set(int(order[min(len(order) - 1, int(f * len(order)))]) for f in (0.25, 0.5, 0.75, 1.0)))
+    picks = sorted(set(int(order[min(len(order) - 1, int(f * len(order)))]) for f in (0.25, 0.5, 0.75, 1.0)),
+                   key=lambda k: -ngs[k])
     rng = np.random.default_rng(7)
     out = []
+    last_s = 0.0
     for k in picks:
+        if deadline is not None and time.time() + 1.5 * last_s > deadline:
+            out.append({"triangle": [int(x) for x in tris[k]], "skipped": "run budget"})
+            continue
         r1, r2, r3 = map(int, tris[k])
         res = ctx.bispec_ex(cube, r1, r2, r3)
         W = 2 * nc10 + 1
@@ -576,6 +648,7 @@ def fp64_error(A, ctx, cube, cfg, gv, samp, uf, fk, lo, hi, tris):
                     "normwise": float(np.linalg.norm(got - b64) / np.linalg.norm(b64)),
                     "max_err_over_sum_abs_terms": float(np.max(np.abs(got - b64) / np.maximum(babs, 1e-300))),
                     "oracle_s": round(time.time() - t0, 1)})
+        last_s = time.time() - t0
         log("fp64 check triangle %s Ng=%d: normwise %.3g" % ((r1, r2, r3), res["ng"], out[-1]["normwise"]))
     return out
 

@@ -39,6 +39,10 @@ extern "C" {
 #define AMB_PREC_FP32 0   /* FP32 MFMA (v_mfma_f32_32x32x2_f32) */
 #define AMB_PREC_BF16X3 1 /* hi+lo BF16 split, 3 MFMA passes, FP32 accumulate */
 #define AMB_PREC_FP8 2    /* FP8 e4m3 with per-plane absmax scales */
+#define AMB_PREC_FP16X3 3 /* hi+lo FP16 split, 3 MFMA passes, FP32 accumulate; operands
+                             scaled by powers of two from each ring's plane-set maximum
+                             so that FP16's range holds them (22-bit operands, against
+                             16 bits for BF16x3) */
 
 const char *amb_strerror(int code);
 const char *amb_version(void);
@@ -417,6 +421,9 @@ typedef struct amb_stats {
     double flop_post;       /* their algorithmic 8 M N^2 flop per triangle */
     double cert_margin_max; /* worst normalisation grid distance (see amb_set_cert_strict) */
     int64_t n_uncertified;  /* triangle calls or waves whose normalisation was not certified */
+    double flop_contract_distinct; /* flop_contract over the distinct sums only: where
+                                      r1 == r2 (or r2 == r3) makes two triplets' sums the
+                                      same, the pair counts once */
 } amb_stats;
 int amb_get_stats(amb_ctx *ctx, amb_stats *st);
 int amb_reset_stats(amb_ctx *ctx);
