@@ -21,8 +21,8 @@ if not os.path.exists(LIB_PATH):
     raise ImportError("amber_amd: %s is missing; build it with `python -m amber_amd.build`" % LIB_PATH)
 _lib = C.CDLL(LIB_PATH)
 
-PREC_FP32, PREC_BF16X3, PREC_FP8 = 0, 1, 2
-_PREC = {"fp32": PREC_FP32, "bf16x3": PREC_BF16X3, "fp8": PREC_FP8}
+PREC_FP32, PREC_BF16X3, PREC_FP8, PREC_FP16X3 = 0, 1, 2, 3
+_PREC = {"fp32": PREC_FP32, "bf16x3": PREC_BF16X3, "fp8": PREC_FP8, "fp16x3": PREC_FP16X3}
 
 EXPORTS = [
     "amb_strerror", "amb_version", "amb_read_params_ess", "amb_read_gv_header", "amb_read_cube_files",

@@ -250,7 +250,8 @@ static size_t plane_bytes(int kind, int nact, size_t pitch)
 {
     if (kind == 4) return pitch * nact;
     if (kind == 5) return sizeof(float) * 2 * nact;
-    return sizeof(float) * pitch * nact; /* FP32, or hi + lo BF16 */
+    if (kind == 0) return sizeof(float) * pitch * nact + AMB_SET_SLOT_BYTES; /* + [max, E] */
+    return sizeof(float) * pitch * nact; /* FP32, or hi + lo BF16 / FP16 */
 }
 
 size_t build_scratch_bytes(amb_ctx *c, const BuildOp &op)
@@ -270,8 +271,10 @@ hipError_t launch_build(amb_ctx *c, hipStream_t s, const BuildOp &op, float2 *G)
     case 1: {
         const float2 *tw = c->tw[op.g.ng];
         const FftPlan &pl = c->plans[op.g.ng];
+        /* data sets are normalised in place, their [max, E] words after the planes */
+        unsigned *slot = op.kind == 0 ? (unsigned *)((float *)op.dst + (size_t)pitch * nact) : nullptr;
         return amb_launch_ring_c2r(s, op.kind, cb.vis_t, cb.samp_t, cb.d_act, nact, (int)cb.N, (int)(cb.N / 2 + 1),
-                                   pl, tw, op.g, G, (float *)op.dst, pitch);
+                                   pl, tw, op.g, G, (float *)op.dst, pitch, slot);
     }
     case 2:
     case 3: {
@@ -281,8 +284,12 @@ hipError_t launch_build(amb_ctx *c, hipStream_t s, const BuildOp &op, float2 *G)
     case 4:
         return amb_launch_fp8_quant(s, (const float *)op.src, pitch, nact, (const float *)op.sc + nact,
                                     (uint8_t *)op.dst);
-    default:
+    case 5:
         return amb_launch_fp8_scales(s, (const float *)op.src, pitch, nact, (float *)op.dst);
+    default: {
+        _Float16 *h = (_Float16 *)op.dst;
+        return amb_launch_split_f16(s, (const float *)op.src, (size_t)pitch * nact, h, h + (size_t)pitch * nact);
+    }
     }
 }
 
@@ -323,7 +330,7 @@ int get_planes(amb_ctx *c, const PlaneSrc &src, int cube_id, int kind, int r, in
     void *dep = nullptr, *sc = nullptr;
     if (kind == 2 || kind == 3) {
         if ((rc = get_planes(c, src, cube_id, kind - 2, r, ng, &dep))) return rc;
-    } else if (kind == 4 || kind == 5) {
+    } else if (kind >= 4) {
         if ((rc = get_planes(c, src, cube_id, 0, r, ng, &dep))) return rc;
         if (kind == 4 && (rc = get_planes(c, src, cube_id, 5, r, ng, &sc))) return rc;
     } else {
@@ -419,13 +426,16 @@ int prepare_tri(amb_ctx *c, const PlaneSrc &src, int id, int r1, int r2, int r3,
     const int nprec = prec == AMB_PREC_FP32 ? AMB_PREC_FP32 : AMB_PREC_BF16X3; /* normalisation form */
     /* D/S: FP32 images of r1, r2, r3; the third operand in the form the
        contraction reads (FP32, the hi/lo BF16 split, or scaled e4m3) */
-    void *D[3], *S[3] = {nullptr, nullptr, nullptr}, *SC[3] = {nullptr, nullptr, nullptr};
+    void *D[3], *S[3] = {nullptr, nullptr, nullptr}, *SC[3] = {nullptr, nullptr, nullptr}, *D0 = nullptr;
     const int rr[3] = {r1, r2, r3};
-    const int d3kind = prec == AMB_PREC_FP32 ? 0 : prec == AMB_PREC_BF16X3 ? 2 : 4;
+    const int d3kind = prec == AMB_PREC_FP32 ? 0 : prec == AMB_PREC_BF16X3 ? 2 : prec == AMB_PREC_FP16X3 ? 7 : 4;
     for (int k = 0; k < 3; ++k) {
         if ((rc = get_planes(c, src, id, k == 2 ? d3kind : 0, rr[k], (int)ng, &D[k]))) return rc;
         if (prec == AMB_PREC_FP8 && (rc = get_planes(c, src, id, 5, rr[k], (int)ng, &SC[k]))) return rc;
     }
+    /* the FP32 set of r3, for its exponent (built already as the source of
+       the derived third operand) */
+    if ((rc = get_planes(c, src, id, 0, r3, (int)ng, &D0))) return rc;
     if (want_norm)
         for (int k = 0; k < 3; ++k) {
             const int kind = (k == 2 && nprec != AMB_PREC_FP32) ? 3 : 1;
@@ -444,8 +454,8 @@ int prepare_tri(amb_ctx *c, const PlaneSrc &src, int id, int r1, int r2, int r3,
     a.sym23 = j.sym23;
     a.D1 = (const float *)D[0]; a.D2 = (const float *)D[1];
     if (prec == AMB_PREC_FP32) a.D3 = (const float *)D[2];
-    else if (prec == AMB_PREC_BF16X3) {
-        a.H3 = (const __bf16 *)D[2];
+    else if (prec == AMB_PREC_BF16X3 || prec == AMB_PREC_FP16X3) {
+        a.H3 = (const uint16_t *)D[2];
         a.L3 = a.H3 + (size_t)j.pitch * nact;
     } else {
         a.Q3 = (const uint8_t *)D[2];
@@ -453,13 +463,19 @@ int prepare_tri(amb_ctx *c, const PlaneSrc &src, int id, int r1, int r2, int r3,
         a.s2 = (const float *)SC[1];
         a.t3 = (const float *)SC[2] + nact;
     }
+    /* exponents of the normalised data sets: the second word after the planes */
+    const size_t setw = (size_t)j.pitch * nact;
+    a.ex1 = (const int *)((const float *)D[0] + setw) + 1;
+    a.ex2 = (const int *)((const float *)D[1] + setw) + 1;
+    a.ex3 = (const int *)((const float *)D0 + setw) + 1;
     if (want_norm) {
         ContractArgs &b = j.b;
         b = a;
         b.D1 = (const float *)S[0]; b.D2 = (const float *)S[1];
         b.D3 = nullptr; b.H3 = b.L3 = nullptr; b.Q3 = nullptr; b.s1 = b.s2 = b.t3 = nullptr;
+        b.ex1 = b.ex2 = b.ex3 = nullptr;
         if (nprec == AMB_PREC_FP32) b.D3 = (const float *)S[2];
-        else { b.H3 = (const __bf16 *)S[2]; b.L3 = b.H3 + (size_t)j.pitch * nact; }
+        else { b.H3 = (const uint16_t *)S[2]; b.L3 = b.H3 + (size_t)j.pitch * nact; }
     }
     ReduceArgs &ra = j.ra;
     ra.act = cb->d_act; ra.inv = cb->d_inv;
@@ -618,7 +634,7 @@ int amb_create(const amb_opts *o, amb_ctx **out)
         c->prec = o->precision;
         c->verbose = o->verbose;
     }
-    if (c->prec < AMB_PREC_FP32 || c->prec > AMB_PREC_FP8) { delete c; return AMB_EARG; }
+    if (c->prec < AMB_PREC_FP32 || c->prec > AMB_PREC_FP16X3) { delete c; return AMB_EARG; }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= c->device || c->device < 0) {
         (void)hipGetLastError();
@@ -687,7 +703,7 @@ int amb_set_precision(amb_ctx *c, int prec)
 {
     if (!c) return AMB_EARG;
     Lock g(c->mu);
-    if (prec < AMB_PREC_FP32 || prec > AMB_PREC_FP8) return fail(c, AMB_EARG, "precision %d", prec);
+    if (prec < AMB_PREC_FP32 || prec > AMB_PREC_FP16X3) return fail(c, AMB_EARG, "precision %d", prec);
     c->prec = prec;
     return AMB_OK;
 }
@@ -944,7 +960,7 @@ int amb_closed_rings(amb_ctx *c, int id, int plane, double *rdata, int cap, int 
     /* calCR's gather (amber_funcs.c:419-429) is the Dfg gather at Ng = N */
     for (int r = 0; r < nr && e == hipSuccess; ++r)
         e = amb_launch_ring_c2r(c->stream, 1, cb->vis_t, cb->samp_t, (const int *)dp, 1, N, N / 2 + 1, pl, tw,
-                                geo[r], c->G, (float *)II + (size_t)r * P, (long)P);
+                                geo[r], c->G, (float *)II + (size_t)r * P, (long)P, nullptr);
     timer_stop(c);
     if (e == hipSuccess) e = amb_launch_triple_sum(c->stream, (const float *)II, (long)P, (const int *)dtri, ntri, (double *)dnum);
     std::vector<double> num(ntri);
@@ -1044,9 +1060,17 @@ int amb_ring_image(amb_ctx *c, int id, int ring, int64_t ng, int plane, int kind
     unpin_all(c);
     if (rc) return rc;
     const float *src = (const float *)d + (size_t)ia * plane_pitch((int)ng);
+    int slot[2] = {0, 0};
     HIPCHK(c, hipMemcpyAsync(out, src, sizeof(float) * P, hipMemcpyDeviceToHost, c->stream));
+    if (kind == 0)
+        HIPCHK(c, hipMemcpyAsync(slot, (const float *)d + (size_t)plane_pitch((int)ng) * cb->nact(), sizeof slot,
+                                 hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     timers_collect(c);
+    if (kind == 0) { /* data sets are stored normalised: undo the power of two and the sign */
+        const float f = (ia & 1 ? -1.f : 1.f) * ldexpf(1.f, slot[1] - AMB_SET_EXP);
+        for (size_t i = 0; i < P; ++i) out[i] *= f;
+    }
     return AMB_OK;
 }
 

@@ -51,13 +51,25 @@
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 
 /* hi/lo split forms and their 16-bit element types */
 template <int MODE>
-constexpr bool is_split() { return MODE == AMB_PREC_BF16X3; }
+constexpr bool is_split() { return MODE == AMB_PREC_BF16X3 || MODE == AMB_PREC_FP16X3; }
 /* forms whose operands carry per-plane power-of-two scales */
 template <int MODE>
 constexpr bool is_scaled() { return MODE == AMB_PREC_FP8; }
+/* Unscale factor of a data triangle: the stored planes of r1, r2, r3 are
+ * (-1)^i D 2^(AMB_SET_EXP - E) (exactly), and the FP16x3 form's B operand
+ * carries another 2^AMB_F16_B_EXP, so accumulators times this power of two,
+ * and the sign of the element's three planes, are the sums of the planes
+ * as computed, bit for bit. */
+template <int MODE>
+__device__ __forceinline__ float data_unscale(const ContractArgs &a)
+{
+    const int e = *a.ex1 + *a.ex2 + *a.ex3 - 3 * AMB_SET_EXP - (MODE == AMB_PREC_FP16X3 ? AMB_F16_B_EXP : 0);
+    return ldexpf(1.f, e);
+}
 
 #define QP 4     /* reference planes per workgroup */
 #define TT 128   /* p2 rows = p3 columns per workgroup */
@@ -71,7 +83,7 @@ constexpr bool is_scaled() { return MODE == AMB_PREC_FP8; }
  * per lane).  NWR = 1 (4 waves of 4x4 blocks) needs all 256 AGPRs for
  * accumulators and spills once the segment flush is in the loop, so it
  * is not launched.
- * Variants (var, chosen per shape by the autotuner):
+ * Variants (var):
  *   0: PIPE = 0, a k-step forms all its fragments, then multiplies
  *   1: PIPE = 1, (BF16x3) a k-step forms the A fragment of row block i+1
  *      while the 12 MFMAs of block i issue; scheduling hints interleave
@@ -94,6 +106,14 @@ struct A16 Stage<AMB_PREC_BF16X3> {
     A16 float a[TT][KC + PADF];         /* D_r2 rows */
     A16 __bf16 bh[TT][KC + PADH];       /* D_r3 hi */
     A16 __bf16 bl[TT][KC + PADH];       /* D_r3 lo */
+    A16 float d1[QP][KC];               /* D_r1 rows of the 4 planes */
+};
+
+template <>
+struct A16 Stage<AMB_PREC_FP16X3> {
+    A16 float a[TT][KC + PADF];         /* D_r2 rows */
+    A16 _Float16 bh[TT][KC + PADH];     /* D_r3 hi (scaled) */
+    A16 _Float16 bl[TT][KC + PADH];     /* D_r3 lo (scaled) */
     A16 float d1[QP][KC];               /* D_r1 rows of the 4 planes */
 };
 
@@ -217,6 +237,10 @@ struct Frag<AMB_PREC_BF16X3, RB> {
     bf16x8 ah[RB], al[RB], bh[4], bl[4];
 };
 template <int RB>
+struct Frag<AMB_PREC_FP16X3, RB> {
+    f16x8 ah[RB], al[RB], bh[4], bl[4];
+};
+template <int RB>
 struct Frag<AMB_PREC_FP8, RB> {
     long aq[RB], bq[4];
 };
@@ -254,19 +278,48 @@ __device__ __forceinline__ void split8(const float (&x)[8], const float (&d)[8],
     lo = __builtin_bit_cast(bf16x8, lp);
 }
 
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+
+/* 8-element 16-bit operand vector of a split form */
+template <int MODE>
+struct HalfT;
+template <>
+struct HalfT<AMB_PREC_BF16X3> { typedef bf16x8 v8; };
+template <>
+struct HalfT<AMB_PREC_FP16X3> { typedef f16x8 v8; };
+
+/* x*d split into hi + lo FP16 for 8 elements, two at a time: the FP32
+ * product v rounds to nearest into hi (v_cvt_pk_f16_f32); v - hi is exact
+ * (v_fma_mix_f32 reads hi as FP16) and rounds to nearest into lo.  The
+ * stored planes are normalised to |x|, |d| <= 2^7, so |x*d| <= 2^14 lies
+ * inside FP16's range. */
+__device__ __forceinline__ void split8(const float (&x)[8], const float (&d)[8], f16x8 &hi, f16x8 &lo)
+{
+#pragma unroll
+    for (int e = 0; e < 8; e += 2) {
+        const f32x2 v = {x[e] * d[e], x[e + 1] * d[e + 1]};
+        const f16x2 h = __builtin_convertvector(v, f16x2);
+        const f32x2 r = {v[0] - (float)h[0], v[1] - (float)h[1]};
+        const f16x2 l = __builtin_convertvector(r, f16x2);
+        hi[e] = h[0]; hi[e + 1] = h[1];
+        lo[e] = l[0]; lo[e + 1] = l[1];
+    }
+}
+
 template <int MODE, int RB>
 __device__ __forceinline__ void load_frag(const Stage<MODE> &s, int step, int q, int rb, int lane, Frag<MODE, RB> &f)
 {
     const int rl = lane & 31, hl = lane >> 5;
     if constexpr (is_split<MODE>()) {
+        using V = typename HalfT<MODE>::v8;
         const int kc = step * 16 + hl * 8;
         float d1v[8];
         *(float4 *)&d1v[0] = *(const float4 *)&s.d1[q][kc];
         *(float4 *)&d1v[4] = *(const float4 *)&s.d1[q][kc + 4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            f.bh[j] = *(const bf16x8 *)&s.bh[j * 32 + rl][kc];
-            f.bl[j] = *(const bf16x8 *)&s.bl[j * 32 + rl][kc];
+            f.bh[j] = *(const V *)&s.bh[j * 32 + rl][kc];
+            f.bl[j] = *(const V *)&s.bl[j * 32 + rl][kc];
         }
 #pragma unroll
         for (int i = 0; i < RB; ++i) {
@@ -330,6 +383,23 @@ __device__ __forceinline__ void mfma_frag(const Frag<MODE, RB> &f, f32x16 (&acc)
 #pragma unroll
             for (int j = 0; j < 4; ++j)
                 acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.ah[i], f.bh[j], acc[i][j], 0, 0, 0);
+    } else if constexpr (MODE == AMB_PREC_FP16X3) {
+        /* the order of the BF16x3 passes, on the FP16 instruction */
+#pragma unroll
+        for (int i = 0; i < RB; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(f.al[i], f.bh[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < RB; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(f.ah[i], f.bl[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < RB; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(f.ah[i], f.bh[j], acc[i][j], 0, 0, 0);
     } else if constexpr (MODE == AMB_PREC_FP8) {
 #pragma unroll
         for (int i = 0; i < RB; ++i)
@@ -412,10 +482,12 @@ __device__ __forceinline__ void bf16_step_ilv(const Stage<AMB_PREC_BF16X3> &s, i
 }
 
 /* Adds (first: stores) the data-form accumulators of one wave into C and
- * clears them.  A 32x32 MFMA block leaves row (v/4)*8 + (l/32)*4 + v%4,
- * column l%32 in register v of lane l.  Flushing every SEGC chunks keeps
- * each register sum to 8192 pixels: FP32 rounding of a running sum grows
- * with its length, so the pixel sum becomes a sum of short sums. */
+ * clears them.  s1: the FP8 form's reference-plane scale.  A 32x32 MFMA block leaves row (v/4)*8 + (l/32)*4 + v%4,
+ * column l%32 in register v of lane l.  Flushing every segment keeps each
+ * register sum short: the rounding of a running sum grows with its length
+ * (for the 16-bit forms all the more: their MFMAs add each exact sum of 16
+ * products into the accumulator truncated toward zero, so the error of
+ * every add is biased), and the pixel sum becomes a sum of short sums. */
 template <int MODE, int RB>
 __device__ __forceinline__ void flush_acc(const ContractArgs &a, f32x16 (&acc)[RB][4], bool first, int i1,
                                           int r0, int c0, int rb, int lane, float s1, const float *rsc,
@@ -427,6 +499,10 @@ __device__ __forceinline__ void flush_acc(const ContractArgs &a, f32x16 (&acc)[R
     asm volatile("" : "+s"(nact));
     asm volatile("" : "+v"(lrow), "+v"(lcol));
     const int lo = a.wlo[i1], hi = a.whi[i1];
+    /* recomputed here (flushes are rare) rather than held in a register
+       across the chunk loop */
+    const float us = data_unscale<MODE>(a);
+    const float usl = ((i1 ^ lane) & 1) ? -us : us;
 #pragma unroll
     for (int i = 0; i < RB; ++i)
 #pragma unroll
@@ -440,6 +516,10 @@ __device__ __forceinline__ void flush_acc(const ContractArgs &a, f32x16 (&acc)[R
                 acc[i][j][v] = 0.f;
                 if (gi < lo || gi >= hi || gj < lo || gj >= hi) continue;
                 if constexpr (is_scaled<MODE>()) x = x * (1.f / (s1 * rsc[li])) * csc[lj];
+                /* the stored planes' signs (-1)^(i1 + gi + gj), with the scale:
+                   r0, c0 and the wave's row base are even, so gi is odd with
+                   v and gj with the lane */
+                x = x * ((v & 1) ? -usl : usl);
                 const size_t idx = ((size_t)i1 * nact + gi) * nact + gj;
                 /* one lane owns each element, so the adds land in a fixed
                    order; an add needs no register for the old value */
@@ -520,6 +600,7 @@ __device__ __forceinline__ void contract_item(const ContractArgs &a, int tile, i
         if (tid < QP) L.s1v[tid] = g0 + tid < nact ? a.s1[g0 + tid] : 1.f;
         __syncthreads();
     }
+
     f32x16 acc[RB][4];
 #pragma unroll
     for (int i = 0; i < RB; ++i)
@@ -606,14 +687,17 @@ typedef __attribute__((address_space(3))) void lds_void_t;
 
 /* One LDS stage: views of separate __shared__ arrays (see the kernels) */
 struct V2Stage {
-    float *a;    /* D_r2 row r, float4 slot c at r*64 + 4*(c ^ (r & 15)) */
-    __bf16 *bh;  /* D_r3 hi col j, 16-byte slot c at j*64 + 8*(c ^ ((j >> 1) & 7)) */
-    __bf16 *bl;  /* D_r3 lo, same layout */
-    float *d1;   /* D_r1 of the 4 planes, plain rows */
+    float *a;      /* D_r2 row r, float4 slot c at r*64 + 4*(c ^ (r & 15)) */
+    uint16_t *bh;  /* D_r3 hi col j, 16-byte slot c at j*64 + 8*(c ^ ((j >> 1) & 7)) */
+    uint16_t *bl;  /* D_r3 lo, same layout */
+    float *d1;     /* D_r1 of the 4 planes, plain rows */
 };
 
+/* FORM: AMB_PREC_BF16X3 or AMB_PREC_FP16X3 (16-bit elements of the split) */
+template <int FORM>
 struct F2 {
-    bf16x8 ah[2], al[2], bh[4], bl[4];
+    typedef typename HalfT<FORM>::v8 V;
+    V ah[2], al[2], bh[4], bl[4];
 };
 
 __device__ __forceinline__ void lds_load16(const void *g, void *l)
@@ -662,8 +746,10 @@ __device__ __forceinline__ void v2_issue(const V2Stage &s, const ContractArgs &a
 }
 
 /* fragments of k-step st (16 pixels) of a chunk */
-__device__ __forceinline__ void v2_frag(const V2Stage &s, int st, int q, int rb, int lane, F2 &f)
+template <int FORM>
+__device__ __forceinline__ void v2_frag(const V2Stage &s, int st, int q, int rb, int lane, F2<FORM> &f)
 {
+    typedef typename F2<FORM>::V V;
     const int rl = lane & 31, hl = lane >> 5;
     const int kc = st * 16 + hl * 8;
     float d1v[8];
@@ -673,8 +759,8 @@ __device__ __forceinline__ void v2_frag(const V2Stage &s, int st, int q, int rb,
     for (int j = 0; j < 4; ++j) {
         const int col = j * 32 + rl;
         const int ph = (kc >> 3) ^ ((col >> 1) & 7);
-        f.bh[j] = *(const bf16x8 *)&s.bh[col * KC2 + ph * 8];
-        f.bl[j] = *(const bf16x8 *)&s.bl[col * KC2 + ph * 8];
+        f.bh[j] = *(const V *)&s.bh[col * KC2 + ph * 8];
+        f.bl[j] = *(const V *)&s.bl[col * KC2 + ph * 8];
     }
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
@@ -687,21 +773,32 @@ __device__ __forceinline__ void v2_frag(const V2Stage &s, int st, int q, int rb,
     }
 }
 
+/* one 32x32x16 MFMA of the split form */
+__device__ __forceinline__ f32x16 mfma16(bf16x8 a, bf16x8 b, f32x16 c)
+{
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f32x16 mfma16(f16x8 a, f16x8 b, f32x16 c)
+{
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+}
+
 /* the three passes of one k-step, in the order of mfma_frag */
-__device__ __forceinline__ void v2_mfma(const F2 &f, f32x16 (&acc)[2][4])
+template <int FORM>
+__device__ __forceinline__ void v2_mfma(const F2<FORM> &f, f32x16 (&acc)[2][4])
 {
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.al[i], f.bh[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < 4; ++j) acc[i][j] = mfma16(f.al[i], f.bh[j], acc[i][j]);
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.ah[i], f.bl[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < 4; ++j) acc[i][j] = mfma16(f.ah[i], f.bl[j], acc[i][j]);
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.ah[i], f.bh[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < 4; ++j) acc[i][j] = mfma16(f.ah[i], f.bh[j], acc[i][j]);
 }
 
 /* A-side operands of a k-step as read from LDS: the D_r2 rows still to be
@@ -726,40 +823,53 @@ __device__ __forceinline__ void v2_read_a(const V2Stage &s, int st, int q, int r
     }
 }
 
-__device__ __forceinline__ void v2_read_b(const V2Stage &s, int st, int lane, F2 &f)
+template <int FORM>
+__device__ __forceinline__ void v2_read_b(const V2Stage &s, int st, int lane, F2<FORM> &f)
 {
+    typedef typename F2<FORM>::V V;
     const int rl = lane & 31, hl = lane >> 5;
     const int kc = st * 16 + hl * 8;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const int col = j * 32 + rl;
         const int ph = (kc >> 3) ^ ((col >> 1) & 7);
-        f.bh[j] = *(const bf16x8 *)&s.bh[col * KC2 + ph * 8];
-        f.bl[j] = *(const bf16x8 *)&s.bl[col * KC2 + ph * 8];
+        f.bh[j] = *(const V *)&s.bh[col * KC2 + ph * 8];
+        f.bl[j] = *(const V *)&s.bl[col * KC2 + ph * 8];
     }
 }
 
 /* pair e (0..7) of the hi/lo split of a k-step's A operand: row block e/4,
  * elements 2(e%4), 2(e%4)+1; the same arithmetic as split8 */
+template <int FORM>
 __device__ __forceinline__ void v2_split_pair(const R2 &r, int e, u32x4 (&hp)[2], u32x4 (&lp)[2])
 {
     const int i = e >> 2, k = (e & 3) * 2;
     const f32x2 v = {r.x[i][k] * r.d1v[k], r.x[i][k + 1] * r.d1v[k + 1]};
-    const unsigned hb = __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2));
-    const float h0 = __uint_as_float(hb << 16), h1 = __uint_as_float(hb & 0xffff0000u);
-    const f32x2 res = {fmaf(r.x[i][k], r.d1v[k], -h0), fmaf(r.x[i][k + 1], r.d1v[k + 1], -h1)};
-    hp[i][e & 3] = hb;
-    lp[i][e & 3] = __builtin_bit_cast(unsigned, __builtin_convertvector(res, bf16x2));
+    if constexpr (FORM == AMB_PREC_BF16X3) {
+        const unsigned hb = __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2));
+        const float h0 = __uint_as_float(hb << 16), h1 = __uint_as_float(hb & 0xffff0000u);
+        const f32x2 res = {fmaf(r.x[i][k], r.d1v[k], -h0), fmaf(r.x[i][k + 1], r.d1v[k + 1], -h1)};
+        hp[i][e & 3] = hb;
+        lp[i][e & 3] = __builtin_bit_cast(unsigned, __builtin_convertvector(res, bf16x2));
+    } else {
+        /* the residual of the rounded product (exact: v and hi are within a
+           factor 2 of each other) */
+        const f16x2 h = __builtin_convertvector(v, f16x2);
+        const f32x2 res = {v[0] - (float)h[0], v[1] - (float)h[1]};
+        hp[i][e & 3] = __builtin_bit_cast(unsigned, h);
+        lp[i][e & 3] = __builtin_bit_cast(unsigned, __builtin_convertvector(res, f16x2));
+    }
 }
 
 /* MFMA m (0..23) of a k-step in the order of mfma_frag: 8 lo.hi, 8 hi.lo,
  * 8 hi.hi, each over (row block, column block) */
-__device__ __forceinline__ void v2_mfma_one(const F2 &f, f32x16 (&acc)[2][4], int m)
+template <int FORM>
+__device__ __forceinline__ void v2_mfma_one(const F2<FORM> &f, f32x16 (&acc)[2][4], int m)
 {
     const int pass = m >> 3, i = (m >> 2) & 1, j = m & 3;
-    if (pass == 0) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.al[i], f.bh[j], acc[i][j], 0, 0, 0);
-    else if (pass == 1) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.ah[i], f.bl[j], acc[i][j], 0, 0, 0);
-    else acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.ah[i], f.bh[j], acc[i][j], 0, 0, 0);
+    if (pass == 0) acc[i][j] = mfma16(f.al[i], f.bh[j], acc[i][j]);
+    else if (pass == 1) acc[i][j] = mfma16(f.ah[i], f.bl[j], acc[i][j]);
+    else acc[i][j] = mfma16(f.ah[i], f.bh[j], acc[i][j]);
 }
 
 /* The 24 MFMAs of one k-step with the next k-step's work placed between
@@ -771,61 +881,64 @@ __device__ __forceinline__ void v2_mfma_one(const F2 &f, f32x16 (&acc)[2][4], in
  * own MFMAs.  The empty asm consumes the split here: otherwise, where the
  * next k-step lies beyond a branch (the segment flush), the split is sunk
  * past it and runs with no MFMA beside it. */
-__device__ __forceinline__ void v2_step(const F2 &cur, f32x16 (&acc)[2][4], const V2Stage &s, int st_next, int q,
-                                        int rb, int lane, F2 &nxt)
+template <int FORM>
+__device__ __forceinline__ void v2_step(const F2<FORM> &cur, f32x16 (&acc)[2][4], const V2Stage &s, int st_next,
+                                        int q, int rb, int lane, F2<FORM> &nxt)
 {
+    typedef typename F2<FORM>::V V;
     R2 r;
     u32x4 hp[2], lp[2];
 #pragma unroll
     for (int m = 0; m < 24; ++m) {
-        v2_mfma_one(cur, acc, m);
+        v2_mfma_one<FORM>(cur, acc, m);
         if (m == 0) v2_read_a(s, st_next, q, rb, lane, r);
-        if (m >= 8 && m < 16) v2_split_pair(r, m - 8, hp, lp);
-        if (m == 16) v2_read_b(s, st_next, lane, nxt);
+        if (m >= 8 && m < 16) v2_split_pair<FORM>(r, m - 8, hp, lp);
+        if (m == 16) v2_read_b<FORM>(s, st_next, lane, nxt);
         __builtin_amdgcn_sched_barrier(0);
     }
     asm volatile("" : "+v"(hp[0]), "+v"(hp[1]), "+v"(lp[0]), "+v"(lp[1]));
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
-        nxt.ah[i] = __builtin_bit_cast(bf16x8, hp[i]);
-        nxt.al[i] = __builtin_bit_cast(bf16x8, lp[i]);
+        nxt.ah[i] = __builtin_bit_cast(V, hp[i]);
+        nxt.al[i] = __builtin_bit_cast(V, lp[i]);
     }
 }
 
 /* Chunk c of a work item, not its last: k-steps 1..3 from Sc, the barrier,
  * chunk c + 2 into Sc, then k-step 3's MFMAs with k-step 0 of chunk c + 1
  * from Sn. */
-template <int NORM>
+template <int FORM, int NORM>
 __device__ __forceinline__ void v2_chunk(const ContractArgs &a, long c, long nchunk, const V2Stage &Sc,
-                                         const V2Stage &Sn, F2 &f0, F2 &f1, f32x16 (&acc)[2][4], int r0, int c0,
-                                         int g0, int wv, int q, int rb, int lane)
+                                         const V2Stage &Sn, F2<FORM> &f0, F2<FORM> &f1, f32x16 (&acc)[2][4], int r0,
+                                         int c0, int g0, int wv, int q, int rb, int lane)
 {
     constexpr int SEG2 = SEGC * KC / KC2; /* the same 8192-pixel segments */
-    v2_step(f0, acc, Sc, 1, q, rb, lane, f1);
-    v2_step(f1, acc, Sc, 2, q, rb, lane, f0);
-    v2_step(f0, acc, Sc, 3, q, rb, lane, f1);
+    v2_step<FORM>(f0, acc, Sc, 1, q, rb, lane, f1);
+    v2_step<FORM>(f1, acc, Sc, 2, q, rb, lane, f0);
+    v2_step<FORM>(f0, acc, Sc, 3, q, rb, lane, f1);
     /* this wave's reads of Sc and loads of chunk c + 1 are complete; after
        the barrier, every wave's are */
     __builtin_amdgcn_s_waitcnt(0x0070); /* vmcnt(0) lgkmcnt(0) */
     __syncthreads();
     if (c + 2 < nchunk) v2_issue(Sc, a, (c + 2) * KC2, r0, c0, g0, wv, lane);
-    v2_step(f1, acc, Sn, 0, q, rb, lane, f0);
+    v2_step<FORM>(f1, acc, Sn, 0, q, rb, lane, f0);
     if (!NORM && (c + 1) % SEG2 == 0)
-        flush_acc<AMB_PREC_BF16X3, 2>(a, acc, c + 1 == SEG2, g0 + q, r0, c0, rb, lane, 1.f, nullptr, nullptr);
+        flush_acc<FORM, 2>(a, acc, c + 1 == SEG2, g0 + q, r0, c0, rb, lane, 1.f, nullptr, nullptr);
 }
 
 /* The last chunk from S, and the final flush (or the normalisation epilogue). */
-template <int NORM>
-__device__ __forceinline__ void v2_last(const ContractArgs &a, long nchunk, const V2Stage &S, F2 &f0, F2 &f1,
-                                        f32x16 (&acc)[2][4], int r0, int c0, int g0, int q, int rb, int lane)
+template <int FORM, int NORM>
+__device__ __forceinline__ void v2_last(const ContractArgs &a, long nchunk, const V2Stage &S, F2<FORM> &f0,
+                                        F2<FORM> &f1, f32x16 (&acc)[2][4], int r0, int c0, int g0, int q, int rb,
+                                        int lane)
 {
     constexpr int SEG2 = SEGC * KC / KC2;
-    v2_step(f0, acc, S, 1, q, rb, lane, f1);
-    v2_step(f1, acc, S, 2, q, rb, lane, f0);
-    v2_step(f0, acc, S, 3, q, rb, lane, f1);
-    v2_mfma(f1, acc);
+    v2_step<FORM>(f0, acc, S, 1, q, rb, lane, f1);
+    v2_step<FORM>(f1, acc, S, 2, q, rb, lane, f0);
+    v2_step<FORM>(f0, acc, S, 3, q, rb, lane, f1);
+    v2_mfma<FORM>(f1, acc);
     if constexpr (NORM) norm_store<2>(a, acc, g0 + q, r0, c0, rb, lane);
-    else flush_acc<AMB_PREC_BF16X3, 2>(a, acc, nchunk <= SEG2, g0 + q, r0, c0, rb, lane, 1.f, nullptr, nullptr);
+    else flush_acc<FORM, 2>(a, acc, nchunk <= SEG2, g0 + q, r0, c0, rb, lane, 1.f, nullptr, nullptr);
 }
 
 /* The two LDS stages are separate objects and every access names one of
@@ -834,7 +947,7 @@ __device__ __forceinline__ void v2_last(const ContractArgs &a, long nchunk, cons
  * not wait for the loads before those reads.  With one array indexed by
  * chunk parity it waits for all loads before the first read after them, so
  * a chunk's prefetch would land before any further MFMA issues. */
-template <int NORM>
+template <int FORM, int NORM>
 __device__ __forceinline__ void contract_item_v2(const ContractArgs &a, int tile, int grp, const V2Stage &S0,
                                                  const V2Stage &S1)
 {
@@ -884,25 +997,25 @@ __device__ __forceinline__ void contract_item_v2(const ContractArgs &a, int tile
     /* Straight-line loop body with one carried fragment set (f0) and the
        last chunk peeled: conditions on the wave or on the last chunk inside
        the loop made the compiler spill the accumulators. */
-    F2 f0, f1;
-    v2_frag(S0, 0, q, rb, lane, f0);
+    F2<FORM> f0, f1;
+    v2_frag<FORM>(S0, 0, q, rb, lane, f0);
     long c = 0;
     for (; c + 2 < nchunk; c += 2) {
-        v2_chunk<NORM>(a, c, nchunk, S0, S1, f0, f1, acc, r0, c0, g0, wv, q, rb, lane);
-        v2_chunk<NORM>(a, c + 1, nchunk, S1, S0, f0, f1, acc, r0, c0, g0, wv, q, rb, lane);
+        v2_chunk<FORM, NORM>(a, c, nchunk, S0, S1, f0, f1, acc, r0, c0, g0, wv, q, rb, lane);
+        v2_chunk<FORM, NORM>(a, c + 1, nchunk, S1, S0, f0, f1, acc, r0, c0, g0, wv, q, rb, lane);
     }
-    if (c + 1 < nchunk) v2_chunk<NORM>(a, c, nchunk, S0, S1, f0, f1, acc, r0, c0, g0, wv, q, rb, lane);
-    v2_last<NORM>(a, nchunk, ((nchunk - 1) & 1) ? S1 : S0, f0, f1, acc, r0, c0, g0, q, rb, lane);
+    if (c + 1 < nchunk) v2_chunk<FORM, NORM>(a, c, nchunk, S0, S1, f0, f1, acc, r0, c0, g0, wv, q, rb, lane);
+    v2_last<FORM, NORM>(a, nchunk, ((nchunk - 1) & 1) ? S1 : S0, f0, f1, acc, r0, c0, g0, q, rb, lane);
 }
 
 /* The two stages: A rows, hi then lo B rows, D_r1 rows, each its own array */
 #define V2_STAGES                                                                                                  \
     __shared__ A16 float sA0[TT * KC2], sA1[TT * KC2];                                                             \
-    __shared__ A16 __bf16 sB0[2 * TT * KC2], sB1[2 * TT * KC2];                                                    \
+    __shared__ A16 uint16_t sB0[2 * TT * KC2], sB1[2 * TT * KC2];                                                  \
     __shared__ A16 float sD0[QP * KC2], sD1[QP * KC2];                                                             \
     const V2Stage S0{sA0, sB0, sB0 + TT * KC2, sD0}, S1{sA1, sB1, sB1 + TT * KC2, sD1};
 
-template <int NORM>
+template <int FORM, int NORM>
 __global__ __launch_bounds__(512, 1) void k_contract_v2(ContractArgs a, int ngroups, int per_xcd, int nwork)
 {
     V2_STAGES
@@ -910,7 +1023,7 @@ __global__ __launch_bounds__(512, 1) void k_contract_v2(ContractArgs a, int ngro
     const int slot = b >> 3;
     const int w = (b & 7) * per_xcd + slot;
     if (slot >= per_xcd || w >= nwork) return;
-    contract_item_v2<NORM>(a, w / ngroups, w % ngroups, S0, S1);
+    contract_item_v2<FORM, NORM>(a, w / ngroups, w % ngroups, S0, S1);
 }
 
 /* XCD queues of the grouped contraction: runs of XRUN consecutive items (one
@@ -933,7 +1046,7 @@ __device__ __forceinline__ long xcd_take(unsigned long long *counter, long total
     return -1;
 }
 
-template <int NORM>
+template <int FORM, int NORM>
 __global__ __launch_bounds__(512, 1) void k_contract_grouped_v2(const ContractArgs *__restrict__ descs,
                                                                 const long *__restrict__ begin, int ntri,
                                                                 unsigned long long *counter)
@@ -957,7 +1070,7 @@ __global__ __launch_bounds__(512, 1) void k_contract_grouped_v2(const ContractAr
         const ContractArgs &a = descs[lo];
         const int ngroups = (a.nact + QP - 1) / QP;
         const long r = w - begin[lo];
-        contract_item_v2<NORM>(a, (int)(r / ngroups), (int)(r % ngroups), S0, S1);
+        contract_item_v2<FORM, NORM>(a, (int)(r / ngroups), (int)(r % ngroups), S0, S1);
         __syncthreads();
     }
 }
@@ -1029,7 +1142,11 @@ hipError_t amb_launch_fill_u32(hipStream_t s, unsigned *p, long n, unsigned v)
 static bool args_ok(int prec, int norm, const ContractArgs &a)
 {
     if (a.pitch % KC != 0 || a.nact <= 0) return false;
-    if (prec == AMB_PREC_BF16X3 && (!a.H3 || !a.L3)) return false;
+    if ((prec == AMB_PREC_BF16X3 || prec == AMB_PREC_FP16X3) && (!a.H3 || !a.L3)) return false;
+    /* data forms read the exponents of the normalised plane sets; the FP16x3
+       form carries no normalisation epilogue */
+    if (!norm && (!a.ex1 || !a.ex2 || !a.ex3)) return false;
+    if (prec == AMB_PREC_FP16X3 && norm) return false;
     if (prec == AMB_PREC_FP32 && !a.D3) return false;
     if (prec == AMB_PREC_FP8 && !a.Q3) return false;
     /* scaled forms carry no normalisation epilogue */
@@ -1076,16 +1193,20 @@ long amb_contract_live_items(int nact, const int *wlo, const int *whi, int sym12
 
 double amb_contract_item_flops(int prec, long pitch)
 {
-    return 2.0 * QP * TT * TT * (double)pitch * (prec == AMB_PREC_BF16X3 ? 3.0 : 1.0);
+    return 2.0 * QP * TT * TT * (double)pitch * ((prec == AMB_PREC_BF16X3 || prec == AMB_PREC_FP16X3) ? 3.0 : 1.0);
 }
 
 template <int MODE, int NORM>
 static void launch_grouped_var(int var, dim3 grid, hipStream_t s, const ContractArgs *d, const long *b, int n,
                                unsigned long long *ctr)
 {
-    if (var == 2 && MODE == AMB_PREC_BF16X3)
-        hipLaunchKernelGGL(k_contract_grouped_v2<NORM>, grid, dim3(512), 0, s, d, b, n, ctr);
-    else if (var == 1) hipLaunchKernelGGL((k_contract_grouped<MODE, NORM, 2, 1>), grid, dim3(512), 0, s, d, b, n, ctr);
+    if constexpr (is_split<MODE>()) {
+        if (var == 2) {
+            hipLaunchKernelGGL((k_contract_grouped_v2<MODE, NORM>), grid, dim3(512), 0, s, d, b, n, ctr);
+            return;
+        }
+    }
+    if (var == 1) hipLaunchKernelGGL((k_contract_grouped<MODE, NORM, 2, 1>), grid, dim3(512), 0, s, d, b, n, ctr);
     else hipLaunchKernelGGL((k_contract_grouped<MODE, NORM, 2, 0>), grid, dim3(512), 0, s, d, b, n, ctr);
 }
 
@@ -1101,9 +1222,13 @@ template <int MODE, int NORM>
 static void launch_var(int var, dim3 grid, hipStream_t s, const ContractArgs &a, int ngroups, int per_xcd, int nwork)
 {
     if (var == 2 && !amb_contract_v2_fits(a.nact, a.pitch)) var = 0;
-    if (var == 2 && MODE == AMB_PREC_BF16X3)
-        hipLaunchKernelGGL(k_contract_v2<NORM>, grid, dim3(512), 0, s, a, ngroups, per_xcd, nwork);
-    else if (var == 1) hipLaunchKernelGGL((k_contract<MODE, NORM, 2, 1>), grid, dim3(512), 0, s, a, ngroups, per_xcd, nwork);
+    if constexpr (is_split<MODE>()) {
+        if (var == 2) {
+            hipLaunchKernelGGL((k_contract_v2<MODE, NORM>), grid, dim3(512), 0, s, a, ngroups, per_xcd, nwork);
+            return;
+        }
+    }
+    if (var == 1) hipLaunchKernelGGL((k_contract<MODE, NORM, 2, 1>), grid, dim3(512), 0, s, a, ngroups, per_xcd, nwork);
     else hipLaunchKernelGGL((k_contract<MODE, NORM, 2, 0>), grid, dim3(512), 0, s, a, ngroups, per_xcd, nwork);
 }
 
@@ -1112,12 +1237,14 @@ hipError_t amb_launch_contract_grouped(hipStream_t s, int prec, int norm, int va
                                        int nblocks)
 {
     if (ntri <= 0 || total <= 0) return hipSuccess;
-    if (prec == AMB_PREC_FP8 && norm) return hipErrorInvalidValue;
+    if ((prec == AMB_PREC_FP8 || prec == AMB_PREC_FP16X3) && norm) return hipErrorInvalidValue;
     /* a kernel fill, not a memset, so the reset can live in a captured graph */
     hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, s, d_counter, 0ull, AMB_GROUPED_COUNTERS);
     if ((long)nblocks > total) nblocks = (int)total;
     dim3 grid(nblocks);
     if (prec == AMB_PREC_FP8) launch_grouped_var<AMB_PREC_FP8, 0>(var, grid, s, d_descs, d_begin, ntri, d_counter);
+    else if (prec == AMB_PREC_FP16X3)
+        launch_grouped_var<AMB_PREC_FP16X3, 0>(var, grid, s, d_descs, d_begin, ntri, d_counter);
     else if (prec == AMB_PREC_FP32) {
         if (norm) launch_grouped_var<AMB_PREC_FP32, 1>(var, grid, s, d_descs, d_begin, ntri, d_counter);
         else launch_grouped_var<AMB_PREC_FP32, 0>(var, grid, s, d_descs, d_begin, ntri, d_counter);
@@ -1139,6 +1266,7 @@ hipError_t amb_launch_contract(hipStream_t s, int prec, int norm, int var, const
     const int per_xcd = (nwork + 7) / 8;
     dim3 grid(8 * per_xcd);
     if (prec == AMB_PREC_FP8) launch_var<AMB_PREC_FP8, 0>(var, grid, s, a, ngroups, per_xcd, nwork);
+    else if (prec == AMB_PREC_FP16X3) launch_var<AMB_PREC_FP16X3, 0>(var, grid, s, a, ngroups, per_xcd, nwork);
     else if (prec == AMB_PREC_FP32) {
         if (norm) launch_var<AMB_PREC_FP32, 1>(var, grid, s, a, ngroups, per_xcd, nwork);
         else launch_var<AMB_PREC_FP32, 0>(var, grid, s, a, ngroups, per_xcd, nwork);
@@ -1165,6 +1293,31 @@ hipError_t amb_launch_split(hipStream_t s, const float *D, size_t n, __bf16 *H, 
     int blocks = (int)std::min<size_t>((n + 255) / 256, 65536);
     if (blocks < 1) blocks = 1;
     hipLaunchKernelGGL(k_split, dim3(blocks), dim3(256), 0, s, D, n, H, L);
+    return hipGetLastError();
+}
+
+/* FP16 hi / lo split of the B operand: stored x times 2^AMB_F16_B_EXP
+ * (|x| <= 2^AMB_SET_EXP, so the product stays within 2^15) */
+__global__ void k_split_f16(const float *__restrict__ D, size_t n4, _Float16 *__restrict__ H, _Float16 *__restrict__ L)
+{
+    const float sb = (float)(1 << AMB_F16_B_EXP);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x) {
+        const float4 v = ((const float4 *)D)[i];
+        const f32x2 a = {v.x * sb, v.y * sb}, b = {v.z * sb, v.w * sb};
+        const f16x2 ha = __builtin_convertvector(a, f16x2), hb = __builtin_convertvector(b, f16x2);
+        const f32x2 ra = {a[0] - (float)ha[0], a[1] - (float)ha[1]}, rb = {b[0] - (float)hb[0], b[1] - (float)hb[1]};
+        const f16x2 la = __builtin_convertvector(ra, f16x2), lb = __builtin_convertvector(rb, f16x2);
+        ((uint2 *)H)[i] = make_uint2(__builtin_bit_cast(unsigned, ha), __builtin_bit_cast(unsigned, hb));
+        ((uint2 *)L)[i] = make_uint2(__builtin_bit_cast(unsigned, la), __builtin_bit_cast(unsigned, lb));
+    }
+}
+
+hipError_t amb_launch_split_f16(hipStream_t s, const float *D, size_t n, _Float16 *H, _Float16 *L)
+{
+    if (n % 4 != 0) return hipErrorInvalidValue;
+    const size_t n4 = n / 4;
+    const int blocks = (int)std::max<size_t>(1, std::min<size_t>((n4 + 255) / 256, 65536));
+    hipLaunchKernelGGL(k_split_f16, dim3(blocks), dim3(256), 0, s, D, n4, H, L);
     return hipGetLastError();
 }
 

@@ -45,7 +45,10 @@ struct Cube {
  * kind 0: data images (FP32), 1: sampling images (FP32),
  * 2 / 3: hi and lo BF16 halves of kind 0 / 1, stored hi planes then lo,
  * 4: e4m3 data images times their B-side scales,
- * 5: per-plane FP8 scales of kind 0 (A side [nact], then B side [nact]). */
+ * 5: per-plane FP8 scales of kind 0 (A side [nact], then B side [nact]),
+ * 7: hi and lo FP16 halves of kind 0 times 2^8, stored hi planes then lo.
+ * Kind 0 sets are stored normalised by a power of two (max |D| <= 2^7) and
+ * end in two words: max |D| as computed and its exponent E. */
 struct PlaneSet {
     int cube, kind, ng;
     double lo, hi;
@@ -61,7 +64,7 @@ struct BuildOp {
     int kind = 0, cube = 0;
     RingGeom g{};
     void *dst = nullptr;
-    const void *src = nullptr; /* kinds 2-5: the FP32 images */
+    const void *src = nullptr; /* kinds 2-7: the FP32 images */
     const void *sc = nullptr;  /* kind 4: the FP8 scales */
 };
 

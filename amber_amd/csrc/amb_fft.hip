@@ -23,6 +23,8 @@
 #include <math.h>
 #include <stdio.h>
 
+#include <algorithm>
+
 #define FFT_THREADS 256
 
 void amb_make_plan(int n, FftPlan *pl)
@@ -165,10 +167,12 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_cols(
     }
 }
 
-/* grid (ceil(n/(2 nb)), npl); each block nb transforms = 2 nb output rows */
+/* grid (ceil(n/(2 nb)), npl); each block nb transforms = 2 nb output rows.
+ * mx (may be null): max |value| over the whole set, as the bit pattern of
+ * a non-negative float (one atomic per workgroup). */
 __global__ __launch_bounds__(FFT_THREADS) void k_fft_rows(
     const float2 *__restrict__ G, int jcnt, FftPlan pl, const float2 *__restrict__ twg, int nb,
-    float *__restrict__ out, long pitch)
+    float *__restrict__ out, long pitch, unsigned *mx)
 {
     extern __shared__ float2 smem[];
     const int n = pl.n;
@@ -202,6 +206,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_rows(
     float *op = out + (size_t)pli * pitch;
     if (blockIdx.x == 0)
         for (long t = (long)n * n + threadIdx.x; t < pitch; t += blockDim.x) op[t] = 0.f;
+    float m = 0.f;
     for (int t = threadIdx.x; t < nb * n; t += blockDim.x) {
         int b = t / n, yv = t - b * n;
         int xa = x0 + 2 * b;
@@ -209,8 +214,51 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_rows(
             float2 z = res[t];
             op[(size_t)xa * n + yv] = z.x;
             op[(size_t)(xa + 1) * n + yv] = z.y;
+            m = fmaxf(m, fmaxf(fabsf(z.x), fabsf(z.y)));
         }
     }
+    if (mx) {
+        __shared__ float red[FFT_THREADS / 64];
+        for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int w = 1; w < FFT_THREADS / 64; ++w) m = fmaxf(m, red[w]);
+            atomicMax(mx, __float_as_uint(m));
+        }
+    }
+}
+
+/* Smallest E with m <= 2^E (0 for m == 0 or not finite), clamped to
+ * [-60, 60]. */
+__device__ __forceinline__ int set_exp(float m)
+{
+    if (!(m > 0.f) || !(m <= 3.0e38f)) return 0;
+    int e;
+    const float f = frexpf(m, &e); /* m = f 2^e, f in [0.5, 1) */
+    if (f == 0.5f) --e;
+    return e < -60 ? -60 : e > 60 ? 60 : e;
+}
+
+/* Normalises a data plane set in place: plane i (compact channel index)
+ * times (-1)^i 2^(7 - E), E = set_exp(max |x|) from slot[0], so every
+ * stored value lies in [-2^7, 2^7]; slot[1] gets E.  Both factors are
+ * exact and undone exactly by whoever reads the set.  The alternating sign
+ * makes the products of a lag cell's triplets (p1, p1 + d2, p1 + d3) carry
+ * (-1)^(p1 + d2 + d3): every other reference plane is accumulated negated,
+ * so a rounding bias of the matrix cores that does not follow the sign of
+ * its operands cancels in the sum over p1 instead of adding up. */
+__global__ void k_normalise_set(float *__restrict__ D, size_t n4, long pitch4, unsigned *slot)
+{
+    const int e = set_exp(__uint_as_float(slot[0]));
+    const float f = ldexpf(1.f, AMB_SET_EXP - e);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x) {
+        const float g = ((i / (size_t)pitch4) & 1) ? -f : f;
+        float4 v = ((float4 *)D)[i];
+        v.x *= g; v.y *= g; v.z *= g; v.w *= g;
+        ((float4 *)D)[i] = v;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) slot[1] = (unsigned)e;
 }
 
 __global__ void k_zero(float *p, size_t n)
@@ -230,11 +278,16 @@ static int lds_batch(int n)
 
 hipError_t amb_launch_ring_c2r(hipStream_t s, int kind, const float2 *vis_t, const float *samp_t,
                                const int *d_planes, int npl, int N, int Nh, const FftPlan &pl,
-                               const float2 *d_tw, RingGeom g, float2 *G, float *out, long pitch)
+                               const float2 *d_tw, RingGeom g, float2 *G, float *out, long pitch, unsigned *slot)
 {
     const int n = pl.n;
     if (npl <= 0) return hipSuccess;
-    if (pitch < (long)n * n) return hipErrorInvalidValue;
+    if (pitch < (long)n * n || pitch % 4 != 0) return hipErrorInvalidValue;
+    if (slot) {
+        /* max and exponent words start at zero: an empty set keeps E = 0 */
+        hipError_t e = amb_launch_fill_u32(s, slot, 2, 0u);
+        if (e != hipSuccess) return e;
+    }
     if (g.jcnt <= 0) {
         size_t tot = (size_t)npl * pitch;
         hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, s, out, tot);
@@ -248,7 +301,12 @@ hipError_t amb_launch_ring_c2r(hipStream_t s, int kind, const float2 *vis_t, con
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     dim3 gr((n + 2 * nb - 1) / (2 * nb), npl);
-    hipLaunchKernelGGL(k_fft_rows, gr, dim3(FFT_THREADS), lds, s, G, g.jcnt, pl, d_tw, nb, out, pitch);
+    hipLaunchKernelGGL(k_fft_rows, gr, dim3(FFT_THREADS), lds, s, G, g.jcnt, pl, d_tw, nb, out, pitch, slot);
+    e = hipGetLastError();
+    if (e != hipSuccess || !slot) return e;
+    const size_t n4 = (size_t)npl * pitch / 4;
+    const int blocks = (int)std::max<size_t>(1, std::min<size_t>((n4 + 255) / 256, 8192));
+    hipLaunchKernelGGL(k_normalise_set, dim3(blocks), dim3(256), 0, s, out, n4, pitch / 4, slot);
     return hipGetLastError();
 }
 

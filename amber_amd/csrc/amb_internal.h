@@ -20,12 +20,21 @@ struct RingGeom {
 };
 
 /* ---- FFT stage (amb_fft.hip) ---- */
+/* Data plane sets are stored normalised: D 2^(AMB_SET_EXP - E) with
+ * E = ceil(log2 max |D|) over the set, so |stored| <= 2^AMB_SET_EXP; two
+ * words after the set hold max |D| (float bits) and E.  Products of three
+ * stored planes carry 2^(3 AMB_SET_EXP - E1 - E2 - E3), removed exactly. */
+#define AMB_SET_EXP 7
+#define AMB_SET_SLOT_BYTES 256
 /* kind 0: data (complex vis), kind 1: sampling (0/1 real).
  * G scratch: [npl][jcnt][n] complex64.  out: npl planes of n*n floats at
- * `pitch` floats apart; the pitch - n*n tail of each plane is zeroed. */
+ * `pitch` floats apart; the pitch - n*n tail of each plane is zeroed.
+ * slot (data sets; null: stored as computed): the set's [max, E] words,
+ * the set normalised in place. */
 hipError_t amb_launch_ring_c2r(hipStream_t s, int kind, const float2 *vis_t, const float *samp_t,
                                const int *d_planes, int npl, int N, int Nh, const FftPlan &pl,
-                               const float2 *d_tw, RingGeom g, float2 *G, float *out, long pitch);
+                               const float2 *d_tw, RingGeom g, float2 *G, float *out, long pitch,
+                               unsigned *slot);
 void amb_make_plan(int n, FftPlan *pl);
 
 /* ---- cube upload (amb_fft.hip) ---- */
@@ -36,9 +45,12 @@ hipError_t amb_launch_upload(hipStream_t s, const double *gv, const double *samp
 struct ContractArgs {
     const float *D1, *D2;      /* [nact][pitch] FP32 */
     const float *D3;           /* FP32 form: [nact][pitch] */
-    const __bf16 *H3, *L3;     /* BF16x3 form: hi / lo split of D3 */
+    const uint16_t *H3, *L3;   /* split forms: hi / lo halves of D3 (BF16 bits,
+                                  or FP16 bits of the stored D3 times 2^8) */
     const uint8_t *Q3;         /* FP8 form: e4m3 D3 times t3 */
     const float *s1, *s2, *t3; /* FP8 form: per-plane scales of D1, D2, D3 */
+    const int *ex1, *ex2, *ex3; /* data forms: exponents E of the normalised
+                                   plane sets of r1, r2, r3 */
     long P, pitch;             /* pixels, plane pitch (multiple of 32) */
     int nact;
     int sym12;                 /* r1 == r2 and full windows: rows below the
@@ -62,6 +74,11 @@ struct ContractArgs {
 bool amb_contract_v2_fits(int nact, long pitch);
 hipError_t amb_launch_contract(hipStream_t s, int prec, int norm, int var, const ContractArgs &a);
 hipError_t amb_launch_split(hipStream_t s, const float *D, size_t n, __bf16 *H, __bf16 *L);
+/* FP16 hi / lo split of n stored (normalised) floats times 2^8, so
+ * |x 2^8| <= 2^15: H = fp16(x 2^8), L = fp16(x 2^8 - H), both rounded to
+ * nearest */
+#define AMB_F16_B_EXP 8
+hipError_t amb_launch_split_f16(hipStream_t s, const float *D, size_t n, _Float16 *H, _Float16 *L);
 /* FP8 preparation of n planes: sc[0..n) A-side scales, sc[n..2n) B-side
  * scales; quant writes the e4m3 planes D*t[i] */
 hipError_t amb_launch_fp8_scales(hipStream_t s, const float *D, long pitch, int n, float *sc);

@@ -39,8 +39,8 @@ CONFIGS = {
 
 # published dense MFMA peaks of the instruction each form issues; one
 # algorithmic BF16x3 flop costs three BF16 MFMA flops (mfma_passes)
-PEAK_TFLOPS = {"bf16x3": 2500.0, "fp32": 157.3, "fp8": 5000.0}
-PASSES = {"bf16x3": 3, "fp32": 1, "fp8": 1}
+PEAK_TFLOPS = {"bf16x3": 2500.0, "fp16x3": 2500.0, "fp32": 157.3, "fp8": 5000.0}
+PASSES = {"bf16x3": 3, "fp16x3": 3, "fp32": 1, "fp8": 1}
 HBM_PEAK_GBS = 8000.0
 # HBM bytes of all contraction launches of one step of the default workload
 # (grouped waves, pipelined BF16x3 form): FETCH_SIZE x 2.00 + WRITE_SIZE x
@@ -102,7 +102,7 @@ def main():
     ap.add_argument("--config", default="cfg3", choices=sorted(CONFIGS) + ["cfg4", "cfg5", "post", "maps", "grid"])
     ap.add_argument("--qps", type=float, default=0.0, help="cfg5: mean arrival rate (0: 60%% of the measured capacity)")
     ap.add_argument("--duration", type=float, default=30.0, help="cfg5: seconds of arrivals")
-    ap.add_argument("--precision", default="bf16x3", choices=["bf16x3", "fp32", "fp8"])
+    ap.add_argument("--precision", default="bf16x3", choices=["bf16x3", "fp16x3", "fp32", "fp8"])
     ap.add_argument("--tri-limit", type=int, default=0,
                     help="development only: K triangles (the line is then not the metric workload)")
     ap.add_argument("--tri-pick", default="first", choices=["first", "largest"])
@@ -248,7 +248,7 @@ def main():
         "value": round(value, 1), "unit": "evals/s", "n_gpus": 1, "steps": nsteps, "warmup": nwarm,
         "ms_per_step": round(1e3 * elapsed / nsteps, 2), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
-        "dtype": {"bf16x3": "bf16", "fp32": "f32", "fp8": "fp8_e4m3"}[args.precision],
+        "dtype": {"bf16x3": "bf16", "fp16x3": "fp16", "fp32": "f32", "fp8": "fp8_e4m3"}[args.precision],
         "precision": args.precision, "data": "synthetic",
         "config": {"workload": workload, "estimator": args.estimator, "channels": nc, "grid": N,
                    "bins": cfg["rings"][2],

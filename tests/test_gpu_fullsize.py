@@ -25,7 +25,7 @@ def big():
     return gv, samp, {}
 
 
-@pytest.mark.parametrize("prec", ["bf16x3", "fp32"])
+@pytest.mark.parametrize("prec", ["bf16x3", "fp16x3", "fp32"])
 def test_fullsize_counts_and_scaling(ctx, big, prec):
     gv, samp, cache = big
     lo, hi = A.log_rings(1, 200, 24)
@@ -66,3 +66,53 @@ def test_fullsize_counts_and_scaling(ctx, big, prec):
     ctx.release_cube(c1)
     ctx.release_cube(c2)
     ctx.set_precision("bf16x3")
+
+
+# normwise bar at the metric workload, every one of its 768 channels
+FULL_TOL = {"fp16x3": 1e-5, "fp32": 1e-5, "bf16x3": 1e-5}
+
+
+@pytest.fixture(scope="module")
+def metric_cells(big):
+    """FP64 oracle on 48 random lag cells (plus the zero lag) of the
+    metric workload's Ng = 320 and Ng = 612 triangles, all 768 channels."""
+    gv, samp, _ = big
+    lo, hi = A.log_rings(1, 200, 24)
+    tris = A.triangles(lo, hi)
+    ngs = np.array([A.ng(hi[t[0]]) for t in tris])
+    picks = [int(np.flatnonzero(ngs == 320)[0]), int(np.flatnonzero(ngs == 612)[0])]
+    rng = np.random.default_rng(11)
+    out = []
+    for k in picks:
+        t = tuple(int(x) for x in tris[k])
+        d2 = np.append(rng.integers(-NC10, NC10 + 1, 48), 0)
+        d3 = np.append(rng.integers(-NC10, NC10 + 1, 48), 0)
+        print("oracle cells of triangle %s Ng=%d" % (t, ngs[k]), flush=True)
+        b64, c64, _, ab = O.bispec_cells(N, NC, 1.0, gv, samp, lo, hi, *t, np.ones(NC, np.uint8), NC10, d2, d3,
+                                         want_abs=True)
+        out.append((t, int(ngs[k]), d2 + NC10, d3 + NC10, b64, c64))
+    return lo, hi, out
+
+
+@pytest.mark.parametrize("prec", ["fp16x3", "fp32", "bf16x3"])
+def test_metric_workload_error_vs_fp64(ctx, big, metric_cells, prec):
+    """The contraction's error against the FP64 oracle at the metric
+    workload (768 ch x 512^2, all lags), on the Ng = 320 and Ng = 612 bins
+    where cancellation is deepest: normwise over 49 cells per triangle."""
+    gv, samp, _ = big
+    lo, hi, cells = metric_cells
+    ctx.set_precision(prec)
+    ctx.set_rings(lo, hi)
+    c1 = ctx.load_cube(N, NC, 1.0, 0, gv, samp)
+    ctx.set_channels(c1, None, NC10)
+    try:
+        for t, ng, i2, i3, b64, c64 in cells:
+            res = ctx.bispec_ex(c1, *t)
+            got = res["bl64"][i2, i3]
+            assert np.array_equal(res["cnt"][i2, i3], c64)
+            e = float(np.linalg.norm(got - b64) / np.linalg.norm(b64))
+            print("metric workload %s triangle %s Ng=%d normwise %.3g" % (prec, t, ng, e), flush=True)
+            assert e <= FULL_TOL[prec], (prec, t, e)
+    finally:
+        ctx.release_cube(c1)
+        ctx.set_precision("bf16x3")
