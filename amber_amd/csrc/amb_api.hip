@@ -29,6 +29,7 @@ using Lock = std::lock_guard<std::recursive_mutex>;
 
 namespace amb {
 
+
 /* ------------------------------------------------------------------ helpers */
 int fail(amb_ctx *c, int code, const char *fmt, ...)
 {
@@ -614,6 +615,101 @@ int bispec_core(amb_ctx *c, int id, int r1, int r2, int r3, int want_norm, amb_r
     return AMB_OK;
 }
 
+/* ------------------------------------------------------------- calCR */
+/* Device scratch of one calCR call, 256-byte aligned pieces of one block:
+ * the FP64 ring images [nr][N*N], the column-pass scratch [jmax][N], the
+ * N twiddles, the plane id, the triangle list and the triangle sums. */
+struct CrLayout {
+    size_t images, G, tw, plane, tri, num, total;
+};
+
+static CrLayout cr_layout(int N, int nr, int jmax, int ntri)
+{
+    auto al = [](size_t x) { return (x + 255) / 256 * 256; };
+    CrLayout L;
+    size_t o = 0;
+    L.images = o; o += al(sizeof(double) * (size_t)nr * N * N);
+    L.G = o; o += al(sizeof(double2) * (size_t)std::max(jmax, 1) * N);
+    L.tw = o; o += al(sizeof(double2) * (size_t)N);
+    L.plane = o; o += al(sizeof(int));
+    L.tri = o; o += al(sizeof(int) * 3 * (size_t)std::max(ntri, 1));
+    L.num = o; o += al(sizeof(double) * (size_t)std::max(ntri, 1));
+    L.total = o;
+    return L;
+}
+
+/* calCR (amber_funcs.c:389-491): the ring images of one sampling plane on
+ * the full N x N grid in FP64, their triple sums per closed triangle, each
+ * Num = N^2 (count of closed sampled triangles) rounded to its 1/8 grid. */
+int closed_rings(amb_ctx *c, int id, int plane, double *rdata, int cap, int *rcount)
+{
+    hipSetDevice(c->device);
+    Cube *cb;
+    int rc = find_cube(c, id, &cb);
+    if (rc) return rc;
+    const int nr = (int)c->ulow.size();
+    if (nr == 0) return fail(c, AMB_ESTATE, "rings not set");
+    if (plane < 0 || plane >= cb->nc) return fail(c, AMB_EARG, "sampling plane %d outside [0,%lld)", plane, (long long)cb->nc);
+    const int N = (int)cb->N;
+    const size_t P = (size_t)N * N;
+    const int ntri = amb_triangles(c->uavg.data(), nr, nullptr, 0);
+    std::vector<int> tri(3 * (size_t)std::max(ntri, 1));
+    amb_triangles(c->uavg.data(), nr, tri.data(), ntri);
+    *rcount = ntri;
+    if (ntri == 0) return AMB_OK;
+    /* twiddles and two transform buffers of N complex doubles in 64 KiB of LDS */
+    if (48L * N > 65536) return fail(c, AMB_EARG, "grid %d exceeds the FP64 LDS transform size (1365)", N);
+    FftPlan pl;
+    amb_make_plan(N, &pl);
+    int jmax = 0;
+    std::vector<RingGeom> geo(nr);
+    for (int r = 0; r < nr; ++r) {
+        if ((rc = ring_geom(c, *cb, r, N, &geo[r]))) return rc;
+        jmax = std::max(jmax, geo[r].jcnt);
+    }
+    const CrLayout L = cr_layout(N, nr, jmax, ntri);
+    char *base = nullptr;
+    if ((rc = dev_alloc(c, (void **)&base, L.total, "calCR scratch"))) return rc;
+    double *II = (double *)(base + L.images);
+    double2 *G = (double2 *)(base + L.G), *tw = (double2 *)(base + L.tw);
+    int *dp = (int *)(base + L.plane), *dtri = (int *)(base + L.tri);
+    double *dnum = (double *)(base + L.num);
+    std::vector<double> num(ntri);
+    /* the plane id by a fill kernel, the triangle list by one copy */
+    hipError_t e = amb_launch_fill_u32(c->stream, (unsigned *)dp, 1, (unsigned)plane);
+    if (e == hipSuccess) e = hipMemcpyAsync(dtri, tri.data(), sizeof(int) * 3 * ntri, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = amb_launch_twiddles64(c->stream, tw, N);
+    timer_start(c, &c->st.ms_fft);
+    /* calCR's gather (amber_funcs.c:419-429) is the Dfg gather at Ng = N */
+    for (int r = 0; r < nr && e == hipSuccess; ++r)
+        e = amb_launch_ring_c2r64(c->stream, cb->samp_t, dp, N, N / 2 + 1, pl, tw, geo[r], G, II + (size_t)r * P);
+    timer_stop(c);
+    if (e == hipSuccess) e = amb_launch_triple_sum(c->stream, II, (long)P, dtri, ntri, dnum);
+    if (e == hipSuccess) e = hipMemcpyAsync(num.data(), dnum, sizeof(double) * ntri, hipMemcpyDeviceToHost, c->stream);
+    /* always drain: the copies read host vectors of this frame */
+    const hipError_t es = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = es;
+    timers_collect(c);
+    dev_free(c, base, L.total);
+    if (e != hipSuccess) return fail(c, AMB_EHIP, "calCR: %s", hipGetErrorString(e));
+    /* Num = N^2 * (count of closed sampled triangles), a multiple of N^2/8;
+       the FP64 sums sit within ~1e-9 of that grid */
+    const double NN2 = (double)N * (double)N;
+    for (int t = 0; t < ntri; ++t) {
+        double q = num[t] / NN2;
+        double q8 = std::rint(q * 8.0);
+        if (std::fabs(q - q8 * 0.125) > 1.0 / 32.0)
+            return fail(c, AMB_ECERT, "calCR triangle %d: Num/N^2=%.6f is not within 1/32 of a multiple of 1/8", t, q);
+        if (rdata && t < cap) {
+            rdata[4 * t] = tri[3 * t];
+            rdata[4 * t + 1] = tri[3 * t + 1];
+            rdata[4 * t + 2] = tri[3 * t + 2];
+            rdata[4 * t + 3] = q8 * 0.125 * NN2;
+        }
+    }
+    return AMB_OK;
+}
+
 } // namespace amb
 
 using namespace amb;
@@ -913,77 +1009,7 @@ int amb_closed_rings(amb_ctx *c, int id, int plane, double *rdata, int cap, int 
 {
     if (!c || !rcount) return AMB_EARG;
     Lock lk(c->mu);
-    hipSetDevice(c->device);
-    Cube *cb;
-    int rc = find_cube(c, id, &cb);
-    if (rc) return rc;
-    const int nr = (int)c->ulow.size();
-    if (nr == 0) return fail(c, AMB_ESTATE, "rings not set");
-    if (plane < 0 || plane >= cb->nc) return fail(c, AMB_EARG, "sampling plane %d outside [0,%lld)", plane, (long long)cb->nc);
-    const int N = (int)cb->N;
-    const size_t P = (size_t)N * N;
-    int ntri = amb_triangles(c->uavg.data(), nr, nullptr, 0);
-    std::vector<int> tri(3 * (size_t)std::max(ntri, 1));
-    amb_triangles(c->uavg.data(), nr, tri.data(), ntri);
-    *rcount = ntri;
-    if (ntri == 0) return AMB_OK;
-    if (24L * N > 65536) return fail(c, AMB_EARG, "grid %d exceeds the LDS transform size (2730)", N);
-    float2 *tw;
-    FftPlan pl;
-    if ((rc = twiddles(c, N, &tw, &pl))) return rc;
-    int jmax = 0;
-    std::vector<RingGeom> geo(nr);
-    for (int r = 0; r < nr; ++r) {
-        if ((rc = ring_geom(c, *cb, r, N, &geo[r]))) return rc;
-        jmax = std::max(jmax, geo[r].jcnt);
-    }
-    if ((rc = scratch(c, &c->G, &c->G_bytes, sizeof(float2) * (size_t)std::max(jmax, 1) * N, "fft scratch"))) return rc;
-    void *II = nullptr, *dp = nullptr, *dtri = nullptr, *dnum = nullptr;
-    if ((rc = dev_alloc(c, &II, sizeof(float) * P * nr, "calCR images"))) return rc;
-    if ((rc = dev_alloc(c, &dp, sizeof(int), "plane id")) ||
-        (rc = dev_alloc(c, &dtri, sizeof(int) * 3 * ntri, "triangle list")) ||
-        (rc = dev_alloc(c, &dnum, sizeof(double) * ntri, "calCR sums"))) {
-        dev_free(c, II, sizeof(float) * P * nr);
-        dev_free(c, dp, sizeof(int));
-        dev_free(c, dtri, sizeof(int) * 3 * ntri);
-        return rc;
-    }
-    auto cleanup = [&]() {
-        dev_free(c, II, sizeof(float) * P * nr);
-        dev_free(c, dp, sizeof(int));
-        dev_free(c, dtri, sizeof(int) * 3 * ntri);
-        dev_free(c, dnum, sizeof(double) * ntri);
-    };
-    hipError_t e = hipMemcpyAsync(dp, &plane, sizeof(int), hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(dtri, tri.data(), sizeof(int) * 3 * ntri, hipMemcpyHostToDevice, c->stream);
-    timer_start(c, &c->st.ms_fft);
-    /* calCR's gather (amber_funcs.c:419-429) is the Dfg gather at Ng = N */
-    for (int r = 0; r < nr && e == hipSuccess; ++r)
-        e = amb_launch_ring_c2r(c->stream, 1, cb->vis_t, cb->samp_t, (const int *)dp, 1, N, N / 2 + 1, pl, tw,
-                                geo[r], c->G, (float *)II + (size_t)r * P, (long)P, nullptr);
-    timer_stop(c);
-    if (e == hipSuccess) e = amb_launch_triple_sum(c->stream, (const float *)II, (long)P, (const int *)dtri, ntri, (double *)dnum);
-    std::vector<double> num(ntri);
-    if (e == hipSuccess) e = hipMemcpyAsync(num.data(), dnum, sizeof(double) * ntri, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    timers_collect(c);
-    cleanup();
-    if (e != hipSuccess) return fail(c, AMB_EHIP, "calCR: %s", hipGetErrorString(e));
-    /* Num = N^2 * (count of closed sampled triangles), a multiple of N^2/8 */
-    const double NN2 = (double)N * (double)N;
-    for (int t = 0; t < ntri; ++t) {
-        double q = num[t] / NN2;
-        double q8 = std::rint(q * 8.0);
-        if (std::fabs(q - q8 * 0.125) > 1.0 / 32.0)
-            return fail(c, AMB_ECERT, "calCR triangle %d: Num/N^2=%.6f is not within 1/32 of a multiple of 1/8", t, q);
-        if (rdata && t < cap) {
-            rdata[4 * t] = tri[3 * t];
-            rdata[4 * t + 1] = tri[3 * t + 1];
-            rdata[4 * t + 2] = tri[3 * t + 2];
-            rdata[4 * t + 3] = q8 * 0.125 * NN2;
-        }
-    }
-    return AMB_OK;
+    return closed_rings(c, id, plane, rdata, cap, rcount);
 }
 
 int amb_bispec_ex(amb_ctx *c, int id, int r1, int r2, int r3, int want_norm, amb_result *res)

@@ -1501,14 +1501,14 @@ hipError_t amb_launch_reduce(hipStream_t s, const ReduceArgs &a)
 
 /* calCR triple sums (amber_funcs.c:451-456): one block per triangle, FP64
  * accumulation in a fixed thread/tree order. */
-__global__ void k_triple_sum(const float *__restrict__ II, long P, const int *__restrict__ tri, double *out)
+__global__ void k_triple_sum(const double *__restrict__ II, long P, const int *__restrict__ tri, double *out)
 {
     __shared__ double red[256];
     const int t = blockIdx.x;
-    const float *a = II + (size_t)tri[3 * t] * P, *b = II + (size_t)tri[3 * t + 1] * P,
-                *c = II + (size_t)tri[3 * t + 2] * P;
+    const double *a = II + (size_t)tri[3 * t] * P, *b = II + (size_t)tri[3 * t + 1] * P,
+                 *c = II + (size_t)tri[3 * t + 2] * P;
     double s = 0.0;
-    for (long x = threadIdx.x; x < P; x += blockDim.x) s += (double)a[x] * (double)b[x] * (double)c[x];
+    for (long x = threadIdx.x; x < P; x += blockDim.x) s += a[x] * b[x] * c[x];
     red[threadIdx.x] = s;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
@@ -1518,7 +1518,7 @@ __global__ void k_triple_sum(const float *__restrict__ II, long P, const int *__
     if (threadIdx.x == 0) out[t] = red[0];
 }
 
-hipError_t amb_launch_triple_sum(hipStream_t s, const float *II, long P, const int *tri, int ntri, double *out)
+hipError_t amb_launch_triple_sum(hipStream_t s, const double *II, long P, const int *tri, int ntri, double *out)
 {
     if (ntri <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_triple_sum, dim3(ntri), dim3(256), 0, s, II, P, tri, out);

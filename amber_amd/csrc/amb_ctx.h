@@ -188,6 +188,8 @@ void post_release(amb_ctx *c);
 double build_fft_bytes(amb_ctx *c, const BuildOp &op);
 /* one triangle with its own launches, results in host memory */
 int bispec_core(amb_ctx *c, int id, int r1, int r2, int r3, int want_norm, amb_result *res);
+/* calCR (FP64 images and sums) */
+int closed_rings(amb_ctx *c, int id, int plane, double *rdata, int cap, int *rcount);
 
 } // namespace amb
 

@@ -39,17 +39,34 @@ void amb_make_plan(int n, FftPlan *pl)
         while (m % p == 0) { pl->rad[pl->nst++] = p; m /= p; }
 }
 
-__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+/* complex type of a real scalar type: FP32 for the filtered planes, FP64
+ * for the calCR images */
+template <typename R>
+struct CT;
+template <>
+struct CT<float> { typedef float2 c; };
+template <>
+struct CT<double> { typedef double2 c; };
+
+__device__ __forceinline__ float2 cmk(float x, float y) { return make_float2(x, y); }
+__device__ __forceinline__ double2 cmk(double x, double y) { return make_double2(x, y); }
+template <typename C>
+__device__ __forceinline__ C cmul(C a, C b)
 {
-    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    return cmk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
-__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+template <typename C>
+__device__ __forceinline__ C cadd(C a, C b) { return cmk(a.x + b.x, a.y + b.y); }
+template <typename C>
+__device__ __forceinline__ C csub(C a, C b) { return cmk(a.x - b.x, a.y - b.y); }
 
 /* nb transforms of length n in x[b*n + i]; inverse direction exp(+2 pi i/n).
  * tw[m] = exp(+2 pi i m / n).  Returns the buffer holding the result. */
-__device__ float2 *lds_fft_batch(float2 *x, float2 *y, const float2 *tw, const FftPlan &pl, int nb)
+template <typename T>
+__device__ typename CT<T>::c *lds_fft_batch(typename CT<T>::c *x, typename CT<T>::c *y,
+                                            const typename CT<T>::c *tw, const FftPlan &pl, int nb)
 {
+    typedef typename CT<T>::c float2; /* the complex type of this instance */
     const int n = pl.n;
     const int tid = threadIdx.x, nth = blockDim.x;
     int Ns = 1;
@@ -67,7 +84,7 @@ __device__ float2 *lds_fft_batch(float2 *x, float2 *y, const float2 *tw, const F
                 float2 v3 = cmul(xi[j + 3 * nR], tw[3 * k * tstep]);
                 float2 s02 = cadd(v0, v2), d02 = csub(v0, v2);
                 float2 s13 = cadd(v1, v3), d13 = csub(v1, v3);
-                float2 id13 = make_float2(-d13.y, d13.x); /* i*(v1-v3) */
+                float2 id13 = cmk(-d13.y, d13.x); /* i*(v1-v3) */
                 float2 *yo = y + b * n + (j / Ns) * Ns * 4 + k;
                 yo[0] = cadd(s02, s13);
                 yo[Ns] = cadd(d02, id13);
@@ -85,7 +102,7 @@ __device__ float2 *lds_fft_batch(float2 *x, float2 *y, const float2 *tw, const F
                 yo[Ns] = csub(v0, v1);
             }
         } else if (R == 3) {
-            const float h = 0.86602540378443864676f; /* sqrt(3)/2 */
+            const T h = (T)0.86602540378443864676; /* sqrt(3)/2 */
             for (int t = tid; t < nb * nR; t += nth) {
                 int b = t / nR, j = t - b * nR, k = j % Ns;
                 const float2 *xi = x + b * n;
@@ -93,8 +110,8 @@ __device__ float2 *lds_fft_batch(float2 *x, float2 *y, const float2 *tw, const F
                 float2 v1 = cmul(xi[j + nR], tw[k * tstep]);
                 float2 v2 = cmul(xi[j + 2 * nR], tw[2 * k * tstep]);
                 float2 sm = cadd(v1, v2), df = csub(v1, v2);
-                float2 m = make_float2(v0.x - 0.5f * sm.x, v0.y - 0.5f * sm.y);
-                float2 r = make_float2(-h * df.y, h * df.x); /* i*h*(v1-v2) */
+                float2 m = cmk(v0.x - (T)0.5 * sm.x, v0.y - (T)0.5 * sm.y);
+                float2 r = cmk(-h * df.y, h * df.x); /* i*h*(v1-v2) */
                 float2 *yo = y + b * n + (j / Ns) * Ns * 3 + k;
                 yo[0] = cadd(v0, sm);
                 yo[Ns] = cadd(m, r);
@@ -111,7 +128,7 @@ __device__ float2 *lds_fft_batch(float2 *x, float2 *y, const float2 *tw, const F
                 int j = g * Ns + k;
                 int e = k + q * Ns; /* exponent step in units of w_{NsR} */
                 const float2 *xi = x + b * n + j;
-                float2 acc = make_float2(0.f, 0.f);
+                float2 acc = cmk((T)0, (T)0);
                 int ex = 0;
                 for (int r = 0; r < R; ++r) {
                     acc = cadd(acc, cmul(xi[r * nR], tw[ex * tstep]));
@@ -129,14 +146,17 @@ __device__ float2 *lds_fft_batch(float2 *x, float2 *y, const float2 *tw, const F
 }
 
 /* grid (ceil(jcnt/nb), npl); dynamic LDS: tw[n] + 2 * nb * n complex */
+template <typename R>
 __global__ __launch_bounds__(FFT_THREADS) void k_fft_cols(
     int kind, const float2 *__restrict__ vis_t, const float *__restrict__ samp_t,
-    const int *__restrict__ planes, int N, int Nh, FftPlan pl, const float2 *__restrict__ twg,
-    double lo, double hi, int jcnt, int nb, float2 *__restrict__ G)
+    const int *__restrict__ planes, int N, int Nh, FftPlan pl, const typename CT<R>::c *__restrict__ twg,
+    double lo, double hi, int jcnt, int nb, typename CT<R>::c *__restrict__ G)
 {
-    extern __shared__ float2 smem[];
+    typedef typename CT<R>::c C;
+    extern __shared__ __align__(16) char smem_raw[];
+    C *smem = (C *)smem_raw;
     const int n = pl.n;
-    float2 *tw = smem, *bx = smem + n, *by = bx + nb * n;
+    C *tw = smem, *bx = smem + n, *by = bx + nb * n;
     const int pli = blockIdx.y;
     const int ch = planes[pli];
     const int j0 = blockIdx.x * nb;
@@ -146,20 +166,20 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_cols(
         int c = t / n, ing = t - c * n;
         int j = j0 + c;
         int ia = ing <= n / 2 ? ing : ing - n; /* ia in (-Ng/2, Ng/2] */
-        float2 v = make_float2(0.f, 0.f);
+        C v = cmk((R)0, (R)0);
         if (j < jcnt && j < jlim) {
             double uu = sqrt(1. * (double)(ia * ia + j * j));
             if (uu >= lo && uu < hi && uu > 0.0) {
                 int in = ia < 0 ? N + ia : ia;
                 size_t src = ((size_t)ch * Nh + j) * N + in;
-                if (kind == 0) v = vis_t[src];
-                else v = make_float2(samp_t[src], 0.f);
+                if (kind == 0) v = cmk((R)vis_t[src].x, (R)vis_t[src].y);
+                else v = cmk((R)samp_t[src], (R)0);
             }
         }
         bx[t] = v;
     }
     __syncthreads();
-    float2 *res = lds_fft_batch(bx, by, tw, pl, nb);
+    C *res = lds_fft_batch<R>(bx, by, tw, pl, nb);
     for (int t = threadIdx.x; t < nb * n; t += blockDim.x) {
         int c = t / n, x = t - c * n;
         int j = j0 + c;
@@ -170,11 +190,14 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_cols(
 /* grid (ceil(n/(2 nb)), npl); each block nb transforms = 2 nb output rows.
  * mx (may be null): max |value| over the whole set, as the bit pattern of
  * a non-negative float (one atomic per workgroup). */
+template <typename R>
 __global__ __launch_bounds__(FFT_THREADS) void k_fft_rows(
-    const float2 *__restrict__ G, int jcnt, FftPlan pl, const float2 *__restrict__ twg, int nb,
-    float *__restrict__ out, long pitch, unsigned *mx)
+    const typename CT<R>::c *__restrict__ G, int jcnt, FftPlan pl, const typename CT<R>::c *__restrict__ twg,
+    int nb, R *__restrict__ out, long pitch, unsigned *mx)
 {
-    extern __shared__ float2 smem[];
+    typedef typename CT<R>::c float2; /* the complex type of this instance */
+    extern __shared__ __align__(16) char smem_raw[];
+    float2 *smem = (float2 *)smem_raw;
     const int n = pl.n;
     float2 *tw = smem, *bx = smem + n, *by = bx + nb * n;
     const int pli = blockIdx.y;
@@ -185,27 +208,27 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_rows(
     for (int t = threadIdx.x; t < nb * n; t += blockDim.x) {
         int k = t / nb, b = t - k * nb;
         int xa = x0 + 2 * b, xb = xa + 1;
-        float2 z = make_float2(0.f, 0.f);
+        float2 z = cmk((R)0, (R)0);
         if (xb < n) {
             if (k == 0) {
                 float2 A = Gp[xa], B = Gp[xb];
-                z = make_float2(A.x, B.x);
+                z = cmk(A.x, B.x);
             } else if (k < jcnt) {
                 float2 A = Gp[(size_t)k * n + xa], B = Gp[(size_t)k * n + xb];
-                z = make_float2(A.x - B.y, A.y + B.x);
+                z = cmk(A.x - B.y, A.y + B.x);
             } else if (n - k < jcnt) {
                 int kk = n - k;
                 float2 A = Gp[(size_t)kk * n + xa], B = Gp[(size_t)kk * n + xb];
-                z = make_float2(A.x + B.y, B.x - A.y);
+                z = cmk(A.x + B.y, B.x - A.y);
             }
         }
         bx[b * n + k] = z;
     }
     __syncthreads();
-    float2 *res = lds_fft_batch(bx, by, tw, pl, nb);
-    float *op = out + (size_t)pli * pitch;
+    float2 *res = lds_fft_batch<R>(bx, by, tw, pl, nb);
+    R *op = out + (size_t)pli * pitch;
     if (blockIdx.x == 0)
-        for (long t = (long)n * n + threadIdx.x; t < pitch; t += blockDim.x) op[t] = 0.f;
+        for (long t = (long)n * n + threadIdx.x; t < pitch; t += blockDim.x) op[t] = (R)0;
     float m = 0.f;
     for (int t = threadIdx.x; t < nb * n; t += blockDim.x) {
         int b = t / n, yv = t - b * n;
@@ -214,7 +237,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_rows(
             float2 z = res[t];
             op[(size_t)xa * n + yv] = z.x;
             op[(size_t)(xa + 1) * n + yv] = z.y;
-            m = fmaxf(m, fmaxf(fabsf(z.x), fabsf(z.y)));
+            m = fmaxf(m, fmaxf(fabsf((float)z.x), fabsf((float)z.y)));
         }
     }
     if (mx) {
@@ -261,6 +284,12 @@ __global__ void k_normalise_set(float *__restrict__ D, size_t n4, long pitch4, u
     if (blockIdx.x == 0 && threadIdx.x == 0) slot[1] = (unsigned)e;
 }
 
+__global__ void k_zero64(double *p, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0.0;
+}
+
 __global__ void k_zero(float *p, size_t n)
 {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -268,9 +297,9 @@ __global__ void k_zero(float *p, size_t n)
     for (; i < n; i += st) p[i] = 0.f;
 }
 
-static int lds_batch(int n)
-{   /* transforms per block with tw + 2 buffers inside 64 KiB */
-    int nb = (65536 / (8 * n) - 1) / 2;
+static int lds_batch(int n, int cbytes = 8)
+{   /* transforms per block with tw + 2 buffers inside 64 KiB (cbytes per complex) */
+    int nb = (65536 / (cbytes * n) - 1) / 2;
     if (nb > 8) nb = 8;
     if (nb < 1) nb = 1;
     return nb;
@@ -296,17 +325,58 @@ hipError_t amb_launch_ring_c2r(hipStream_t s, int kind, const float2 *vis_t, con
     int nb = lds_batch(n);
     size_t lds = sizeof(float2) * (size_t)(n + 2 * nb * n);
     dim3 gc((g.jcnt + nb - 1) / nb, npl);
-    hipLaunchKernelGGL(k_fft_cols, gc, dim3(FFT_THREADS), lds, s, kind, vis_t, samp_t, d_planes, N, Nh,
+    hipLaunchKernelGGL(k_fft_cols<float>, gc, dim3(FFT_THREADS), lds, s, kind, vis_t, samp_t, d_planes, N, Nh,
                        pl, d_tw, g.lo, g.hi, g.jcnt, nb, G);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     dim3 gr((n + 2 * nb - 1) / (2 * nb), npl);
-    hipLaunchKernelGGL(k_fft_rows, gr, dim3(FFT_THREADS), lds, s, G, g.jcnt, pl, d_tw, nb, out, pitch, slot);
+    hipLaunchKernelGGL(k_fft_rows<float>, gr, dim3(FFT_THREADS), lds, s, G, g.jcnt, pl, d_tw, nb, out, pitch, slot);
     e = hipGetLastError();
     if (e != hipSuccess || !slot) return e;
     const size_t n4 = (size_t)npl * pitch / 4;
     const int blocks = (int)std::max<size_t>(1, std::min<size_t>((n4 + 255) / 256, 8192));
     hipLaunchKernelGGL(k_normalise_set, dim3(blocks), dim3(256), 0, s, out, n4, pitch / 4, slot);
+    return hipGetLastError();
+}
+
+/* FP64 c2r of one sampling plane's ring (the calCR images, whose triple
+ * sums must resolve a 1/8 grid of N^2): the same two passes in double. */
+hipError_t amb_launch_ring_c2r64(hipStream_t s, const float *samp_t, const int *d_plane, int N, int Nh,
+                                 const FftPlan &pl, const double2 *d_tw, RingGeom g, double2 *G, double *out)
+{
+    const int n = pl.n;
+    const long pitch = (long)n * n;
+    if (g.jcnt <= 0) {
+        hipLaunchKernelGGL(k_zero64, dim3(256), dim3(256), 0, s, out, (size_t)pitch);
+        return hipGetLastError();
+    }
+    const int nb = lds_batch(n, 16);
+    const size_t lds = sizeof(double2) * (size_t)(n + 2 * nb * n);
+    if (lds > 65536) return hipErrorInvalidValue;
+    dim3 gc((g.jcnt + nb - 1) / nb, 1);
+    hipLaunchKernelGGL(k_fft_cols<double>, gc, dim3(FFT_THREADS), lds, s, 1, nullptr, samp_t, d_plane, N, Nh, pl,
+                       d_tw, g.lo, g.hi, g.jcnt, nb, G);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    dim3 gr((n + 2 * nb - 1) / (2 * nb), 1);
+    hipLaunchKernelGGL(k_fft_rows<double>, gr, dim3(FFT_THREADS), lds, s, G, g.jcnt, pl, d_tw, nb, out, pitch,
+                       nullptr);
+    return hipGetLastError();
+}
+
+/* tw[m] = exp(+2 pi i m / n) in FP64 */
+__global__ void k_twiddles64(double2 *tw, int n)
+{
+    for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < n; m += gridDim.x * blockDim.x) {
+        double sn, cs;
+        sincospi(2.0 * (double)m / (double)n, &sn, &cs);
+        tw[m] = make_double2(cs, sn);
+    }
+}
+
+hipError_t amb_launch_twiddles64(hipStream_t s, double2 *tw, int n)
+{
+    hipLaunchKernelGGL(k_twiddles64, dim3((n + 255) / 256), dim3(256), 0, s, tw, n);
     return hipGetLastError();
 }
 

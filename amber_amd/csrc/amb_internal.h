@@ -36,6 +36,12 @@ hipError_t amb_launch_ring_c2r(hipStream_t s, int kind, const float2 *vis_t, con
                                const float2 *d_tw, RingGeom g, float2 *G, float *out, long pitch,
                                unsigned *slot);
 void amb_make_plan(int n, FftPlan *pl);
+/* FP64 c2r of one sampling plane's ring (d_plane: its channel id on the
+ * device) into out[n*n]; G: [jcnt][n] complex128 scratch; d_tw: n twiddles
+ * from amb_launch_twiddles64 */
+hipError_t amb_launch_ring_c2r64(hipStream_t s, const float *samp_t, const int *d_plane, int N, int Nh,
+                                 const FftPlan &pl, const double2 *d_tw, RingGeom g, double2 *G, double *out);
+hipError_t amb_launch_twiddles64(hipStream_t s, double2 *tw, int n);
 
 /* ---- cube upload (amb_fft.hip) ---- */
 hipError_t amb_launch_upload(hipStream_t s, const double *gv, const double *samp, int i0, int rows,
@@ -125,6 +131,6 @@ int amb_contract_grouped_blocks(int dev);
 hipError_t amb_launch_fill_u32(hipStream_t s, unsigned *p, long n, unsigned v);
 
 
-/* calCR triple sums: one FP64 sum per triangle over P pixels. */
-hipError_t amb_launch_triple_sum(hipStream_t s, const float *II, long P, const int *tri, int ntri,
+/* calCR triple sums: one FP64 sum per triangle over P pixels of FP64 images. */
+hipError_t amb_launch_triple_sum(hipStream_t s, const double *II, long P, const int *tri, int ntri,
                                  double *out);

@@ -190,10 +190,12 @@ int amb_set_channels(amb_ctx *ctx, int cube_id, const int *uf_chans, int uf_nc, 
 /* Ring edges in cells (replaces annular_rings, amber_funcs.c:2022-2094). */
 int amb_set_rings(amb_ctx *ctx, const double *ulow, const double *uhigh, int nr);
 
-/* calCR (amber_funcs.c:389-491) on the GPU: per ring the c2r of sampling
- * plane `plane` (5 in the reference) on the full grid, per closed triangle
- * the triple sum.  Num is certified to be an integer multiple of N^2 and
- * written exactly.  rdata gets rcount x {r1,r2,r3,Num}. */
+/* calCR (amber_funcs.c:389-491) on the GPU: per ring the FP64 c2r of
+ * sampling plane `plane` (5 in the reference) on the full grid, per closed
+ * triangle the FP64 triple sum, as the reference's FFTW double path.  Num
+ * is an integer multiple of N^2/8 (certified within 1/32 of it, else
+ * AMB_ECERT) and written exactly.  rdata gets rcount x {r1,r2,r3,Num}.
+ * N <= 1365 (two FP64 transform buffers in 64 KiB of LDS). */
 int amb_closed_rings(amb_ctx *ctx, int cube_id, int plane, double *rdata, int cap, int *rcount);
 
 /* One triangle, v1 (fill_s2d_float x3 + amber.c:226-304): bl[W*W] f32,

@@ -23,6 +23,7 @@ _lib.orc_dft_direct.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.c_int]
 _lib.orc_ng.restype = C.c_long
 _lib.orc_ng.argtypes = [C.c_double, C.c_double]
 _lib.orc_uavg.argtypes = [_dp, _dp, C.c_int, _dp]
+_lib.orc_normalize.argtypes = [C.c_long, C.c_long, _dp, _dp]
 _lib.orc_annular_rings.argtypes = [C.c_int, _dp, _dp]
 _lib.orc_triangles.argtypes = [_dp, C.c_int, _ip, C.c_int]
 _lib.orc_ring_c2r.argtypes = [C.c_long, C.c_long, C.c_long, C.c_double, C.c_double, _dp, _dp, C.c_long, _dp]
@@ -99,6 +100,16 @@ def calcr(N, nc, samp, lo, hi, plane=5):
     rd = np.zeros((max(n, 1), 4))
     got = _lib.orc_calcr(N, nc, _p(_f64(samp), _dp), plane, _p(lo, _dp), _p(hi, _dp), len(lo), _p(rd, _dp), n)
     return rd[:got]
+
+
+def normalize(gv, samp):
+    """normalize_data on copies of file-layout arrays: returns (gv, samp)."""
+    g = np.array(gv, np.float64, copy=True, order="C")
+    s = np.array(samp, np.float64, copy=True, order="C")
+    cells = s.shape[0]
+    nc = s.shape[1] if s.ndim > 1 else 1
+    _lib.orc_normalize(cells, nc, _p(g, _dp), _p(s, _dp))
+    return g, s
 
 
 def chan_mask(nc, flag_key, uf_chans):

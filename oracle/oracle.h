@@ -12,6 +12,12 @@
  *   orc_fft_c2c       direct O(n^2) DFT on small n and numpy.fft (pocketfft)
  *   orc_ring_c2r      numpy.fft.irfft2 * Ng^2 on the same masked half plane,
  *                     and a brute-force direct 2-D sum on tiny grids
+ *   orc_normalize     V = S c gives back c where S > 0 (S := 1), cells with
+ *                     S <= 0 untouched, idempotent
+ *   orc_bispec_cells  equal to orc_bispec cell for cell (same triplets, same
+ *                     products) on flagged cubes with lag windows
+ *   orc_uavg          (2/3) H at L = 0 and the area-weighted mean radius by
+ *                     quadrature
  *   orc_ng, orc_uavg,
  *   orc_triangles,
  *   orc_annular_rings closed forms and hand-computed cases (triangle counts of
@@ -53,6 +59,10 @@ void orc_fft_c2c(int n, double *re, double *im, int sign);
 
 /* Direct O(n^2) DFT used to validate orc_fft_c2c. */
 void orc_dft_direct(int n, const double *re, const double *im, double *ore, double *oim, int sign);
+
+/* normalize_data (amber_funcs.c:1946-1978) in place on file-layout arrays:
+ * where NTiG > 0, V /= NTiG and then NTiG := 1. */
+void orc_normalize(long cells, long nc, double *re_im, double *samp);
 
 /* Reduced grid side for a triangle whose largest ring has upper edge uhigh
  * (amber.c:193-197): Nmax=(long)((uhigh+3)/Du); Ng=3*Nmax; Ng+=2 if even else 3. */

@@ -85,6 +85,21 @@ void orc_dft_direct(int n, const double *re, const double *im, double *ore, doub
 /* Integer / FP64 rules                                                      */
 /* ------------------------------------------------------------------------ */
 
+/* normalize_data (4_fbe/amber_funcs.c:1946-1978): at every (cell, channel)
+ * whose count NTiG > 0 the gridded visibility is divided by NTiG (re and im,
+ * FP64), then NTiG := 1; cells with NTiG <= 0 are left as they are.  Arrays
+ * in the file layout: re_im[cells][nc][2], samp[cells][nc]. */
+void orc_normalize(long cells, long nc, double *re_im, double *samp)
+{
+    for (long i = 0; i < cells * nc; ++i)
+        if (samp[i] > 0) {
+            re_im[2 * i] /= samp[i];
+            re_im[2 * i + 1] /= samp[i];
+        }
+    for (long i = 0; i < cells * nc; ++i)
+        if (samp[i] > 0) samp[i] = 1;
+}
+
 long orc_ng(double uhigh, double du)
 {   /* amber.c:193-197 */
     double umax = uhigh + 3;

@@ -25,7 +25,7 @@ def big():
     return gv, samp, {}
 
 
-@pytest.mark.parametrize("prec", ["bf16x3", "fp16x3", "fp32"])
+@pytest.mark.parametrize("prec", ["bf16x3", "fp16x3", "fp32", "fp8"])
 def test_fullsize_counts_and_scaling(ctx, big, prec):
     gv, samp, cache = big
     lo, hi = A.log_rings(1, 200, 24)
@@ -62,7 +62,8 @@ def test_fullsize_counts_and_scaling(ctx, big, prec):
     got = res["bl64"][i2, i3]
     cell = np.max(np.abs(got - b64) / np.maximum(ab, 1e-300))
     print("fullsize Ng=%d prec=%s per-cell/|terms| %.3g" % (res["ng"], prec, cell), flush=True)
-    assert cell <= 2e-8
+    # FP8 (3 mantissa bits, a screening form): its own bar
+    assert cell <= (1e-4 if prec == "fp8" else 2e-8)
     ctx.release_cube(c1)
     ctx.release_cube(c2)
     ctx.set_precision("bf16x3")

@@ -132,6 +132,26 @@ def test_calcr_exact(ctx):
     ctx.release_cube(cube)
 
 
+@pytest.mark.parametrize("nc,rings,plane", [(40, (1, 200, 20), 5), (40, (1, 200, 24), 5)])
+def test_calcr_n512(ctx, nc, rings, plane):
+    """calCR on a 512^2 grid with the scaled and precision workloads' ring
+    sets (20 and 24 log bins 1-200; rings past N/3 are cut by the column
+    rule j < N/3): every triangle's Num equals the FP64 oracle's, snapped to
+    its 1/8 grid, bit for bit."""
+    N = 512
+    lo, hi = A.log_rings(*rings)
+    cube, gv, samp, uf, mask = _setup(ctx, N, nc, 0, 2001, lo, hi, nc - 1)
+    print("case calCR N=%d rings=%d" % (N, len(lo)), flush=True)
+    got = ctx.closed_rings(cube, plane=plane)
+    ref = O.calcr(N, nc, samp, lo, hi, plane=plane)
+    assert len(got) == len(A.triangles(lo, hi))
+    assert np.array_equal(got[:, :3], ref[:, :3])
+    q = ref[:, 3] / N ** 2 * 8
+    assert np.max(np.abs(q - np.rint(q))) < 1e-6     # the oracle's sums sit on the grid
+    assert np.array_equal(got[:, 3], np.rint(q) / 8 * N ** 2)
+    ctx.release_cube(cube)
+
+
 def test_deterministic_alone_and_in_batch(ctx):
     N, nc, nc10 = 32, 10, 9
     lo, hi = A.log_rings(2, 10, 4)
@@ -367,4 +387,97 @@ def test_power_of_two_scale_is_exact(ctx, prec):
         assert np.array_equal(b["bl64"], a["bl64"] * 2.0 ** -15), (prec, t)
     ctx.release_cube(c1)
     ctx.release_cube(c2)
+    ctx.set_precision("bf16x3")
+
+
+def test_counts_above_one_are_normalised(ctx):
+    """normalize_data on upload (amber_funcs.c:1946-1978): a cube whose
+    sampled cells carry counts 1..4 (V = S c, as vcg writes them) gives the
+    bispectrum, counts, v2 normalisation and calCR of the oracle run on the
+    normalised arrays."""
+    N, nc, nc10 = 64, 10, 9
+    lo, hi = A.log_rings(2, 18, 4)
+    ctx.set_precision("bf16x3")
+    gv, samp, uf = A.generate(N, nc, nflag=2, seed=83, count_max=4)
+    assert samp.max() == 4
+    gn, sn = O.normalize(gv, samp)
+    cube = ctx.load_cube(N, nc, 1.0, 1, gv, samp)
+    ctx.set_channels(cube, uf, nc10)
+    ctx.set_rings(lo, hi)
+    mask = O.chan_mask(nc, 1, uf)
+    print("case counts > 1 N=%d nc=%d" % (N, nc), flush=True)
+    for (r1, r2, r3) in A.triangles(lo, hi)[::2]:
+        res = ctx.bispec_ex(cube, int(r1), int(r2), int(r3), want_norm=True)
+        b64, c64, n64 = O.bispec(N, nc, 1.0, gn, sn, lo, hi, r1, r2, r3, mask, nc10, want_norm=True)
+        assert np.array_equal(res["cnt"], c64)
+        assert np.array_equal(res["norm64"], _exact_norm(n64, res["ng"] ** 2))
+        if np.linalg.norm(b64) > 0:
+            assert _relerr(res["bl64"], b64) <= TOL["bf16x3"]
+    got = ctx.closed_rings(cube, plane=5)
+    ref = O.calcr(N, nc, sn, lo, hi, plane=5)
+    assert np.array_equal(got[:, 3], np.rint(ref[:, 3] / N ** 2 * 8) / 8 * N ** 2)
+    ctx.release_cube(cube)
+
+
+@pytest.mark.parametrize("prec", ["bf16x3", "fp32"])
+def test_du_above_one(ctx, prec):
+    """Du = 1.0254 (the shipped grid's, mabs_funcs.py:30): only Ng changes
+    (amber.c:193-197, Ng = 3 floor((UHigh + 3) / Du) + 2 or 3), and rings
+    near Ng/3 alias on the reduced grid as in the reference (3 max|k| can
+    exceed Ng); bispectrum, counts and normalisation against the oracle."""
+    N, nc, nc10, Du = 128, 8, 7, 1.0254
+    lo, hi = A.log_rings(2, 40, 6)
+    ctx.set_precision(prec)
+    gv, samp, uf = A.generate(N, nc, nflag=0, seed=91)
+    cube = ctx.load_cube(N, nc, Du, 0, gv, samp)
+    ctx.set_channels(cube, None, nc10)
+    ctx.set_rings(lo, hi)
+    mask = O.chan_mask(nc, 0, uf)
+    tris = A.triangles(lo, hi)
+    assert any(A.ng(hi[t[0]], Du) != A.ng(hi[t[0]]) for t in tris)
+    print("case Du=%g prec=%s ntri=%d" % (Du, prec, len(tris)), flush=True)
+    for (r1, r2, r3) in tris[::3]:
+        res = ctx.bispec_ex(cube, int(r1), int(r2), int(r3), want_norm=True)
+        assert res["ng"] == O.ng(hi[r1], Du)
+        b64, c64, n64 = O.bispec(N, nc, Du, gv, samp, lo, hi, r1, r2, r3, mask, nc10, want_norm=True)
+        assert np.array_equal(res["cnt"], c64)
+        if res["cert_margin"] <= 1.0 / 32:
+            assert np.array_equal(res["norm64"], _exact_norm(n64, res["ng"] ** 2))
+        else:   # an uncertified call: FP32 accumulation error, counted in the stats
+            assert _relerr(res["norm64"], n64) <= 1e-5
+        if np.linalg.norm(b64) > 0:
+            assert _relerr(res["bl64"], b64) <= TOL[prec]
+    ctx.release_cube(cube)
+    ctx.set_precision("bf16x3")
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_cfg2_scaled(ctx, prec):
+    """The scaled workload: 384 ch x 512^2, 31 flagged channels, 20 log bins
+    1-200, all 383 lags.  The largest (Ng = 612) and a median triangle in
+    one grouped batch: counts on sampled cells and the zero lag against the
+    oracle exactly, sums normwise over 33 cells."""
+    N, nc, nc10 = 512, 384, 383
+    lo, hi = A.log_rings(1, 200, 20)
+    ctx.set_precision(prec)
+    cube, gv, samp, uf, mask = _setup(ctx, N, nc, 31, 2001, lo, hi, nc10)
+    tris = A.triangles(lo, hi)
+    assert len(tris) == 314
+    ngs = np.array([A.ng(hi[t[0]]) for t in tris])
+    pick = [int(np.argmax(ngs)), int(np.argsort(ngs)[len(ngs) // 2])]
+    print("case cfg2 prec=%s triangles=%s" % (prec, [tuple(map(int, tris[k])) for k in pick]), flush=True)
+    bl, cnt, _ = ctx.bispec_batch(cube, tris[pick])
+    rng = np.random.default_rng(21)
+    for n, k in enumerate(pick):
+        r1, r2, r3 = map(int, tris[k])
+        d2 = np.append(rng.integers(-nc10, nc10 + 1, 32), 0)
+        d3 = np.append(rng.integers(-nc10, nc10 + 1, 32), 0)
+        b64, c64, _ = O.bispec_cells(N, nc, 1.0, gv, samp, lo, hi, r1, r2, r3, mask, nc10, d2, d3)
+        got = bl[n][d2 + nc10, d3 + nc10].astype(np.float64)
+        assert np.array_equal(cnt[n][d2 + nc10, d3 + nc10], c64)
+        live = c64 > 0
+        e = _relerr(got[live], b64[live])
+        print("cfg2 %s triangle %s Ng=%d normwise %.3g" % (prec, (r1, r2, r3), ngs[k], e), flush=True)
+        assert e <= TOL[prec]
+    ctx.release_cube(cube)
     ctx.set_precision("bf16x3")

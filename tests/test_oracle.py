@@ -251,3 +251,61 @@ def test_refalgo_agrees_with_oracle():
     scale = np.sqrt(np.mean(bl64 ** 2))
     assert np.max(np.abs(bl - bl64)) <= 1e-4 * scale
     assert np.max(np.abs(numf - n64)) <= 1e-4 * max(1.0, np.max(np.abs(n64)))
+
+
+def test_normalize_data_pins():
+    """normalize_data (amber_funcs.c:1946-1978): V = S c with integer counts
+    S >= 1 gives back c exactly where S is a power of two and to one
+    rounding otherwise, with S := 1; cells with S <= 0 keep V and S; the
+    operation is idempotent."""
+    rng = np.random.default_rng(3)
+    cells, nc = 300, 7
+    c = rng.standard_normal((cells, nc, 2))
+    S = rng.integers(-1, 6, (cells, nc)).astype(np.float64)
+    gv = c * np.where(S > 0, S, 1.0)[..., None]
+    gv[S <= 0] = rng.standard_normal(((S <= 0).sum(), 2))   # untouched values
+    g, s = O.normalize(gv, S)
+    pos = S > 0
+    assert np.array_equal(s[pos], np.ones(pos.sum())) and np.array_equal(s[~pos], S[~pos])
+    assert np.array_equal(g[~pos], gv[~pos])
+    assert np.allclose(g[pos], c[pos], rtol=2 ** -52, atol=0)
+    p2 = pos & np.isin(S, [1, 2, 4])
+    assert np.array_equal(g[p2], c[p2])                     # exact division by powers of two
+    g2, s2 = O.normalize(g, s)
+    assert np.array_equal(g2, g) and np.array_equal(s2, s)
+
+
+def test_bispec_cells_equals_bispec():
+    """orc_bispec_cells (the checker of the large and full-size GPU tests)
+    against orc_bispec on every lag cell of a flagged cube with lag windows,
+    data and normalisation: the same triplets and products give the same
+    sums (to FP64 summation order) and identical counts."""
+    N, nc, nc10 = 32, 9, 5
+    lo, hi = A.log_rings(2, 10, 4)
+    gv, samp, uf = A.generate(N, nc, nflag=2, seed=19)
+    mask = O.chan_mask(nc, 1, uf)
+    W = 2 * nc10 + 1
+    d = np.arange(-nc10, nc10 + 1)
+    d2, d3 = [x.ravel() for x in np.meshgrid(d, d, indexing="ij")]
+    for (r1, r2, r3) in A.triangles(lo, hi)[::3]:
+        b, c, n = O.bispec(N, nc, 1.0, gv, samp, lo, hi, r1, r2, r3, mask, nc10, want_norm=True)
+        bc, cc, nn, ab = O.bispec_cells(N, nc, 1.0, gv, samp, lo, hi, r1, r2, r3, mask, nc10, d2, d3,
+                                        want_norm=True, want_abs=True)
+        assert np.array_equal(cc.reshape(W, W), c)
+        assert np.all(np.abs(bc.reshape(W, W) - b) <= 1e-12 * np.maximum(ab.reshape(W, W), 1e-300))
+        assert np.allclose(nn.reshape(W, W), n, rtol=1e-12, atol=1e-9)
+        assert np.all(ab >= np.abs(bc) * (1 - 1e-12))
+
+
+def test_uavg_closed_form_and_quadrature():
+    """Uavg = (2/3)(H^3 - L^3)/(H^2 - L^2) (amber_funcs.c:2087-2093) is the
+    area-weighted mean radius of the annulus [L, H): (2/3) H at L = 0, and
+    the midpoint-rule integral of r over the annulus's area elsewhere."""
+    lo = np.array([0.0, 1.0, 2.5, 150.0, 10.0])
+    hi = np.array([3.0, 2.0, 7.5, 200.0, 10.5])
+    u = O.uavg(lo, hi)
+    assert u[0] == pytest.approx(2.0, rel=1e-15)
+    for L, H, got in zip(lo, hi, u):
+        r = L + (np.arange(200000) + 0.5) * (H - L) / 200000
+        assert got == pytest.approx(np.sum(r * r) / np.sum(r), rel=1e-9)
+    assert np.array_equal(A.uavg(lo, hi), u)
