@@ -36,7 +36,7 @@ EXPORTS = [
     "amb_graph_destroy", "amb_serve_start", "amb_submit", "amb_poll", "amb_wait", "amb_serve_stop",
     "amb_cyl_bispec", "amb_maps_cldnu", "amb_write_cldnu", "amb_read_params", "amb_write_bsw", "amb_read_bsw",
     "amb_read_bsout", "amb_crc32", "amb_read_vcg_inputs", "amb_grid_setup", "amb_grid_cells", "amb_grid",
-    "amb_grid_cube", "amb_set_cert_strict",
+    "amb_grid_cube", "amb_set_cert_strict", "amb_closed_rings_noalloc",
 ]
 
 BATCH_PER_TRIANGLE, BATCH_GROUPED = 0, 1
@@ -114,6 +114,7 @@ _lib.amb_set_channels.argtypes = [C.c_void_p, C.c_int, _ip, C.c_int, C.c_int]
 _lib.amb_set_rings.argtypes = [C.c_void_p, _dp, _dp, C.c_int]
 _lib.amb_set_precision.argtypes = [C.c_void_p, C.c_int]
 _lib.amb_closed_rings.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, C.c_int, _ip]
+_lib.amb_closed_rings_noalloc.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, C.c_int, _ip]
 _lib.amb_bispec_v1.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, _fp, _ip, _dp, _dp]
 _lib.amb_bispec_v2.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _fp, _fp, _dp]
 _lib.amb_bispec_ex.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(Result)]
@@ -466,11 +467,14 @@ class Context:
         self._hi = np.ascontiguousarray(hi, np.float64)
         self._c(_lib.amb_set_rings(self._h, _p(self._lo, _dp), _p(self._hi, _dp), len(self._lo)))
 
-    def closed_rings(self, cube, plane=5):
+    def closed_rings(self, cube, plane=5, noalloc=False):
+        """calCR; noalloc=True runs the entry point that allocates no device
+        memory (amb_closed_rings_noalloc)."""
         n = len(triangles(self._lo, self._hi))
         rd = np.zeros((max(n, 1), 4))
         cnt = C.c_int()
-        self._c(_lib.amb_closed_rings(self._h, cube, plane, _p(rd, _dp), n, C.byref(cnt)))
+        fn = _lib.amb_closed_rings_noalloc if noalloc else _lib.amb_closed_rings
+        self._c(fn(self._h, cube, plane, _p(rd, _dp), n, C.byref(cnt)))
         return rd[: cnt.value]
 
     def _W(self, cube):

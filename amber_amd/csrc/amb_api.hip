@@ -27,8 +27,32 @@
 
 using Lock = std::lock_guard<std::recursive_mutex>;
 
+/* Scratch of the allocation-free entry points: one array of
+ * AMB_NOALLOC_SCRATCH_BYTES on every device that loads the library, held
+ * until the process ends.  Host code never uses the array's name as an
+ * address: its device address comes from hipGetSymbolAddress, once per
+ * device (noalloc_scratch). */
+static __device__ __align__(256) char scratch[AMB_NOALLOC_SCRATCH_BYTES];
+
 namespace amb {
 
+/* the device address of `scratch` on the current device `dev`, looked up
+ * once per device; null if the lookup failed */
+static void *noalloc_scratch(int dev)
+{
+    static std::mutex mu;
+    static std::map<int, void *> addr;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = addr.find(dev);
+    if (it != addr.end()) return it->second;
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) {
+        (void)hipGetLastError();
+        p = nullptr;
+    }
+    addr[dev] = p;
+    return p;
+}
 
 /* ------------------------------------------------------------------ helpers */
 int fail(amb_ctx *c, int code, const char *fmt, ...)
@@ -640,8 +664,10 @@ static CrLayout cr_layout(int N, int nr, int jmax, int ntri)
 
 /* calCR (amber_funcs.c:389-491): the ring images of one sampling plane on
  * the full N x N grid in FP64, their triple sums per closed triangle, each
- * Num = N^2 (count of closed sampled triangles) rounded to its 1/8 grid. */
-int closed_rings(amb_ctx *c, int id, int plane, double *rdata, int cap, int *rcount)
+ * Num = N^2 (count of closed sampled triangles) rounded to its 1/8 grid.
+ * noalloc: the scratch is the library's static device array (no device
+ * allocation at all); otherwise it is allocated and freed here. */
+int closed_rings(amb_ctx *c, int id, int plane, double *rdata, int cap, int *rcount, bool noalloc)
 {
     hipSetDevice(c->device);
     Cube *cb;
@@ -669,7 +695,16 @@ int closed_rings(amb_ctx *c, int id, int plane, double *rdata, int cap, int *rco
     }
     const CrLayout L = cr_layout(N, nr, jmax, ntri);
     char *base = nullptr;
-    if ((rc = dev_alloc(c, (void **)&base, L.total, "calCR scratch"))) return rc;
+    if (noalloc) {
+        if (!c->na_scratch) return fail(c, AMB_ESTATE, "no static scratch on device %d", c->device);
+        if (L.total > AMB_NOALLOC_SCRATCH_BYTES)
+            return fail(c, AMB_EBUDGET, "calCR of %d rings on a %d^2 grid needs %zu bytes of scratch; the "
+                        "allocation-free entry point holds %zu (amb_closed_rings has no such limit)",
+                        nr, N, L.total, (size_t)AMB_NOALLOC_SCRATCH_BYTES);
+        base = (char *)c->na_scratch;
+    } else if ((rc = dev_alloc(c, (void **)&base, L.total, "calCR scratch"))) {
+        return rc;
+    }
     double *II = (double *)(base + L.images);
     double2 *G = (double2 *)(base + L.G), *tw = (double2 *)(base + L.tw);
     int *dp = (int *)(base + L.plane), *dtri = (int *)(base + L.tri);
@@ -679,18 +714,22 @@ int closed_rings(amb_ctx *c, int id, int plane, double *rdata, int cap, int *rco
     hipError_t e = amb_launch_fill_u32(c->stream, (unsigned *)dp, 1, (unsigned)plane);
     if (e == hipSuccess) e = hipMemcpyAsync(dtri, tri.data(), sizeof(int) * 3 * ntri, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = amb_launch_twiddles64(c->stream, tw, N);
-    timer_start(c, &c->st.ms_fft);
+    /* the kernel timers' events come from a pool that may create events:
+       not on the allocation-free path */
+    if (!noalloc) timer_start(c, &c->st.ms_fft);
     /* calCR's gather (amber_funcs.c:419-429) is the Dfg gather at Ng = N */
     for (int r = 0; r < nr && e == hipSuccess; ++r)
         e = amb_launch_ring_c2r64(c->stream, cb->samp_t, dp, N, N / 2 + 1, pl, tw, geo[r], G, II + (size_t)r * P);
-    timer_stop(c);
+    if (!noalloc) timer_stop(c);
     if (e == hipSuccess) e = amb_launch_triple_sum(c->stream, II, (long)P, dtri, ntri, dnum);
     if (e == hipSuccess) e = hipMemcpyAsync(num.data(), dnum, sizeof(double) * ntri, hipMemcpyDeviceToHost, c->stream);
     /* always drain: the copies read host vectors of this frame */
     const hipError_t es = hipStreamSynchronize(c->stream);
     if (e == hipSuccess) e = es;
-    timers_collect(c);
-    dev_free(c, base, L.total);
+    if (!noalloc) {
+        timers_collect(c);
+        dev_free(c, base, L.total);
+    }
     if (e != hipSuccess) return fail(c, AMB_EHIP, "calCR: %s", hipGetErrorString(e));
     /* Num = N^2 * (count of closed sampled triangles), a multiple of N^2/8;
        the FP64 sums sit within ~1e-9 of that grid */
@@ -757,6 +796,7 @@ int amb_create(const amb_opts *o, amb_ctx **out)
     c->margin = (unsigned *)m;
     c->used = 256;
     c->grouped_blocks = amb_contract_grouped_blocks(c->device);
+    c->na_scratch = noalloc_scratch(c->device); /* with the context's device current */
     *out = c;
     return AMB_OK;
 }
@@ -1009,7 +1049,14 @@ int amb_closed_rings(amb_ctx *c, int id, int plane, double *rdata, int cap, int 
 {
     if (!c || !rcount) return AMB_EARG;
     Lock lk(c->mu);
-    return closed_rings(c, id, plane, rdata, cap, rcount);
+    return closed_rings(c, id, plane, rdata, cap, rcount, false);
+}
+
+int amb_closed_rings_noalloc(amb_ctx *c, int id, int plane, double *rdata, int cap, int *rcount)
+{
+    if (!c || !rcount) return AMB_EARG;
+    Lock lk(c->mu);
+    return closed_rings(c, id, plane, rdata, cap, rcount, true);
 }
 
 int amb_bispec_ex(amb_ctx *c, int id, int r1, int r2, int r3, int want_norm, amb_result *res)

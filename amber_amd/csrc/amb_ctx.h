@@ -122,6 +122,7 @@ struct amb_ctx {
     int grouped_blocks = 0; /* persistent grid of the grouped contraction */
     int wave_max = 1024;    /* triangles per grouped wave */
     void *blas = nullptr;   /* rocBLAS handle of the post-processing products (first use) */
+    void *na_scratch = nullptr; /* device address of the static scratch (allocation-free entry points) */
     /* scratch (grow-only) */
     float2 *G = nullptr; size_t G_bytes = 0;
     float *C = nullptr; size_t C_bytes = 0;
@@ -188,8 +189,8 @@ void post_release(amb_ctx *c);
 double build_fft_bytes(amb_ctx *c, const BuildOp &op);
 /* one triangle with its own launches, results in host memory */
 int bispec_core(amb_ctx *c, int id, int r1, int r2, int r3, int want_norm, amb_result *res);
-/* calCR (FP64 images and sums) */
-int closed_rings(amb_ctx *c, int id, int plane, double *rdata, int cap, int *rcount);
+/* calCR; noalloc: all device scratch from the static array, no allocation */
+int closed_rings(amb_ctx *c, int id, int plane, double *rdata, int cap, int *rcount, bool noalloc);
 
 } // namespace amb
 

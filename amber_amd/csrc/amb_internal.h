@@ -4,6 +4,8 @@
 #include <stdint.h>
 
 #define AMB_MAX_RADIX_STAGES 24
+/* bytes of the static device scratch of the allocation-free entry points */
+#define AMB_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
 
 /* Stockham plan for one transform length n: radices in the order applied. */
 struct FftPlan {

@@ -198,6 +198,17 @@ int amb_set_rings(amb_ctx *ctx, const double *ulow, const double *uhigh, int nr)
  * N <= 1365 (two FP64 transform buffers in 64 KiB of LDS). */
 int amb_closed_rings(amb_ctx *ctx, int cube_id, int plane, double *rdata, int cap, int *rcount);
 
+/* amb_closed_rings with no device allocation at all.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Its
+ * scratch (FP64 ring images, FFT scratch, twiddles, triangle sums) is a
+ * static device array of 64 MiB that every device loading the library
+ * holds until the process ends; a call that needs more (nr * N^2 * 8 bytes
+ * plus a little) fails with AMB_EBUDGET.  Results equal amb_closed_rings'
+ * bit for bit.  All calls on one device share the array: calls that
+ * overlap on one GPU (several contexts or host threads at once) race. */
+int amb_closed_rings_noalloc(amb_ctx *ctx, int cube_id, int plane, double *rdata, int cap, int *rcount);
+
 /* One triangle, v1 (fill_s2d_float x3 + amber.c:226-304): bl[W*W] f32,
  * cnt[W*W] i32; *ntri and *num as written by the reference (Num =
  * (Ng/N)^2 * rdata_num, Ntri = Num/(Ng*Ng)). */
