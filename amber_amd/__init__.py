@@ -92,7 +92,8 @@ class Stats(C.Structure):
                 ("ms_maps", C.c_double), ("ms_maps_total", C.c_double), ("flop_maps", C.c_double),
                 ("n_maps_launch", C.c_int64), ("ms_grid", C.c_double), ("n_grid_groups", C.c_int64),
                 ("ms_post", C.c_double), ("flop_post", C.c_double), ("cert_margin_max", C.c_double),
-                ("n_uncertified", C.c_int64), ("flop_contract_distinct", C.c_double)]
+                ("n_uncertified", C.c_int64), ("flop_contract_distinct", C.c_double),
+                ("n_norm_exact", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

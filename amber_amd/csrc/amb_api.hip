@@ -276,14 +276,31 @@ static size_t plane_bytes(int kind, int nact, size_t pitch)
     if (kind == 4) return pitch * nact;
     if (kind == 5) return sizeof(float) * 2 * nact;
     if (kind == 0) return sizeof(float) * pitch * nact + AMB_SET_SLOT_BYTES; /* + [max, E] */
+    if (kind == 8) return sizeof(double) * pitch * nact;
     return sizeof(float) * pitch * nact; /* FP32, or hi + lo BF16 / FP16 */
 }
 
 size_t build_scratch_bytes(amb_ctx *c, const BuildOp &op)
 {
-    if (op.kind > 1) return 0;
+    if (op.kind > 1 && op.kind != 8) return 0;
     const Cube &cb = c->cubes[op.cube];
-    return sizeof(float2) * (size_t)cb.nact() * (size_t)std::max(op.g.jcnt, 1) * op.g.ng;
+    const size_t cb8 = op.kind == 8 ? sizeof(double2) : sizeof(float2);
+    return cb8 * (size_t)cb.nact() * (size_t)std::max(op.g.jcnt, 1) * op.g.ng;
+}
+
+/* FP64 twiddles of length n (the exact normalisation's c2r), built on the
+ * device once per context */
+static int twiddles64(amb_ctx *c, int n, double2 **out)
+{
+    auto it = c->tw64.find(n);
+    if (it != c->tw64.end()) { *out = it->second; return AMB_OK; }
+    void *d = nullptr;
+    int rc = dev_alloc(c, &d, sizeof(double2) * n, "FP64 twiddle table");
+    if (rc) return rc;
+    HIPCHK(c, amb_launch_twiddles64(c->stream, (double2 *)d, n));
+    c->tw64[n] = (double2 *)d;
+    *out = (double2 *)d;
+    return AMB_OK;
 }
 
 hipError_t launch_build(amb_ctx *c, hipStream_t s, const BuildOp &op, float2 *G)
@@ -311,6 +328,10 @@ hipError_t launch_build(amb_ctx *c, hipStream_t s, const BuildOp &op, float2 *G)
                                     (uint8_t *)op.dst);
     case 5:
         return amb_launch_fp8_scales(s, (const float *)op.src, pitch, nact, (float *)op.dst);
+    case 8:
+        return amb_launch_ring_c2r64(s, cb.samp_t, cb.d_act, nact, (int)cb.N, (int)(cb.N / 2 + 1),
+                                     c->plans[op.g.ng], c->tw64[op.g.ng], op.g, (double2 *)G, (double *)op.dst,
+                                     pitch);
     default: {
         _Float16 *h = (_Float16 *)op.dst;
         return amb_launch_split_f16(s, (const float *)op.src, (size_t)pitch * nact, h, h + (size_t)pitch * nact);
@@ -355,6 +376,13 @@ int get_planes(amb_ctx *c, const PlaneSrc &src, int cube_id, int kind, int r, in
     void *dep = nullptr, *sc = nullptr;
     if (kind == 2 || kind == 3) {
         if ((rc = get_planes(c, src, cube_id, kind - 2, r, ng, &dep))) return rc;
+    } else if (kind == 8) {
+        /* two FP64 transform buffers and twiddles in 64 KiB of LDS */
+        if (48L * ng > 65536) return fail(c, AMB_EARG, "grid %d exceeds the FP64 LDS transform size (1365)", ng);
+        float2 *tw;
+        FftPlan pl;
+        double2 *tw64;
+        if ((rc = twiddles(c, ng, &tw, &pl)) || (rc = twiddles64(c, ng, &tw64))) return rc;
     } else if (kind >= 4) {
         if ((rc = get_planes(c, src, cube_id, 0, r, ng, &dep))) return rc;
         if (kind == 4 && (rc = get_planes(c, src, cube_id, 5, r, ng, &sc))) return rc;
@@ -383,7 +411,7 @@ int get_planes(amb_ctx *c, const PlaneSrc &src, int cube_id, int kind, int r, in
     if (src.record) {
         src.record->push_back(op);
     } else {
-        if (kind <= 1 && (rc = scratch(c, &c->G, &c->G_bytes, build_scratch_bytes(c, op), "fft scratch"))) {
+        if ((kind <= 1 || kind == 8) && (rc = scratch(c, &c->G, &c->G_bytes, build_scratch_bytes(c, op), "fft scratch"))) {
             dev_free(c, d, bytes);
             return rc;
         }
@@ -540,15 +568,87 @@ void account_counts(amb_ctx *c, const TriJob &j, int64_t evals, int64_t pairs, i
     c->st.flop_contract_exec += j.flop_exec;
 }
 
-int check_cert(amb_ctx *c, float m)
+int check_cert(amb_ctx *c, float m, bool *recompute)
 {
+    *recompute = false;
     if (!(m < AMB_CERT_RANGE)) /* also NaN */
         return fail(c, AMB_ECERT, "normalisation sum beyond the int32 range of eighths");
     if (m > c->st.cert_margin_max) c->st.cert_margin_max = m;
     if (m > 1.0f / 32.0f) {
         ++c->st.n_uncertified;
         if (c->cert_strict) return fail(c, AMB_ECERT, "normalisation sums off their 1/8 grid by %g", (double)m);
+        *recompute = true;
     }
+    return AMB_OK;
+}
+
+/* The exact v2 normalisation of one triangle: FP64 sampling images, FP64
+ * MFMA products, whole-range FP64 sums rounded to their 1/8 grid, reduced
+ * over p1 as the fast path does.  norm (f32) / norm64 (f64): host arrays
+ * of W*W, either may be null.  Uses its own scratch and margin word, so it
+ * may run while a grouped wave's buffers are still live. */
+int norm_exact(amb_ctx *c, int id, int r1, int r2, int r3, float *norm, double *norm64, float *margin)
+{
+    Cube *cb;
+    int rc = find_cube(c, id, &cb);
+    if (rc) return rc;
+    const long ng = amb_ng(c->uhigh[r1], cb->Du);
+    const long P = ng * ng, pitch = plane_pitch(ng);
+    const int nact = cb->nact();
+    const size_t W = (size_t)(2 * cb->nc10 + 1), cells = W * W, nC = (size_t)nact * nact * nact;
+    const PlaneSrc src{&c->planes};
+    void *S[3];
+    const int rr[3] = {r1, r2, r3};
+    for (int k = 0; k < 3; ++k)
+        if ((rc = get_planes(c, src, id, 8, rr[k], (int)ng, &S[k]))) { unpin_all(c); return rc; }
+    void *Cq = c->Cqx, *out = c->outx;
+    if ((rc = grow(c, &Cq, &c->Cqx_bytes, sizeof(int) * nC, "exact normalisation partials")) ||
+        (rc = grow(c, &out, &c->outx_bytes, cells * (sizeof(float) + sizeof(double)), "exact normalisation"))) {
+        c->Cqx = (int *)Cq; c->outx = out;
+        unpin_all(c);
+        return rc;
+    }
+    c->Cqx = (int *)Cq; c->outx = out;
+    double *d_n64 = (double *)out;
+    float *d_n = (float *)(d_n64 + cells);
+    const bool full_window = nact > 0 && cb->act.back() - cb->act.front() <= cb->nc10;
+    ContractArgs64 a{};
+    a.D1 = (const double *)S[0]; a.D2 = (const double *)S[1]; a.D3 = (const double *)S[2];
+    a.P = P; a.pitch = pitch; a.nact = nact;
+    a.sym12 = (r1 == r2 && full_window && !c->no_sym) ? 1 : 0;
+    a.sym23 = (r2 == r3 && r1 != r2 && !c->no_sym) ? 1 : 0;
+    a.wlo = cb->d_wlo; a.whi = cb->d_whi;
+    a.Cq = c->Cqx;
+    a.margin = c->margin + 1; /* the fast path uses word 0 */
+    ReduceArgs ra{};
+    ra.C = nullptr; ra.Cq = c->Cqx;
+    ra.act = cb->d_act; ra.inv = cb->d_inv;
+    ra.nc = (int)cb->nc; ra.nact = nact; ra.nc10 = cb->nc10; ra.P = P; ra.sym12 = a.sym12; ra.sym23 = a.sym23;
+    ra.norm = d_n; ra.norm64 = d_n64;
+    int cus = 256;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device);
+    unsigned mbits = 0;
+    hipError_t e = amb_launch_fill_u32(c->stream, a.margin, 1, 0u);
+    timer_start(c, &c->st.ms_norm);
+    if (e == hipSuccess) e = amb_launch_norm64(c->stream, a, 4 * cus);
+    timer_stop(c);
+    if (e == hipSuccess) e = amb_launch_reduce(c->stream, ra);
+    if (e == hipSuccess && norm64) e = hipMemcpyAsync(norm64, d_n64, sizeof(double) * cells, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && norm) e = hipMemcpyAsync(norm, d_n, sizeof(float) * cells, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&mbits, a.margin, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream);
+    const hipError_t es = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = es;
+    timers_collect(c);
+    unpin_all(c);
+    if (e != hipSuccess)
+        return fail(c, AMB_EHIP, "exact normalisation (%d,%d,%d) Ng=%ld: %s", r1, r2, r3, ng, hipGetErrorString(e));
+    float m;
+    memcpy(&m, &mbits, sizeof m);
+    if (margin) *margin = m;
+    ++c->st.n_norm_exact;
+    if (!(m <= 1.0f / 32.0f))
+        return fail(c, AMB_ECERT, "exact normalisation (%d,%d,%d): FP64 sums off their 1/8 grid by %g", r1, r2, r3,
+                    (double)m);
     return AMB_OK;
 }
 
@@ -634,7 +734,16 @@ int bispec_core(amb_ctx *c, int id, int r1, int r2, int r3, int want_norm, amb_r
         float m;
         memcpy(&m, &mbits, sizeof m);
         res->cert_margin = m;
-        return check_cert(c, m);
+        bool again = false;
+        if ((rc = check_cert(c, m, &again))) return rc;
+        /* sums the fast path could not resolve on their grid: recomputed
+           exactly in FP64 */
+        if (again) {
+            float mx = 0.f;
+            rc = norm_exact(c, id, r1, r2, r3, res->norm, res->norm64, &mx);
+            res->cert_margin = mx;
+            return rc;
+        }
     }
     return AMB_OK;
 }
@@ -719,7 +828,8 @@ int closed_rings(amb_ctx *c, int id, int plane, double *rdata, int cap, int *rco
     if (!noalloc) timer_start(c, &c->st.ms_fft);
     /* calCR's gather (amber_funcs.c:419-429) is the Dfg gather at Ng = N */
     for (int r = 0; r < nr && e == hipSuccess; ++r)
-        e = amb_launch_ring_c2r64(c->stream, cb->samp_t, dp, N, N / 2 + 1, pl, tw, geo[r], G, II + (size_t)r * P);
+        e = amb_launch_ring_c2r64(c->stream, cb->samp_t, dp, 1, N, N / 2 + 1, pl, tw, geo[r], G, II + (size_t)r * P,
+                                  (long)P);
     if (!noalloc) timer_stop(c);
     if (e == hipSuccess) e = amb_launch_triple_sum(c->stream, II, (long)P, dtri, ntri, dnum);
     if (e == hipSuccess) e = hipMemcpyAsync(num.data(), dnum, sizeof(double) * ntri, hipMemcpyDeviceToHost, c->stream);
@@ -816,6 +926,7 @@ void amb_destroy(amb_ctx *c)
         hipFree(cb.d_wlo); hipFree(cb.d_whi);
     }
     for (auto &kv : c->tw) hipFree(kv.second);
+    for (auto &kv : c->tw64) hipFree(kv.second);
     if (c->cstream) {
         hipStreamSynchronize(c->cstream);
         hipStreamDestroy(c->cstream);
@@ -824,6 +935,7 @@ void amb_destroy(amb_ctx *c)
         hipEventDestroy(c->ev_cp[1]);
     }
     hipFree(c->G); hipFree(c->C); hipFree(c->Cq); hipFree(c->margin); hipFree(c->out); hipFree(c->out2);
+    hipFree(c->Cqx); hipFree(c->outx);
     hipFree(c->wdesc);
     for (void *h : c->hstage)
         if (h) hipHostFree(h);

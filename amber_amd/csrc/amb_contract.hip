@@ -1524,3 +1524,139 @@ hipError_t amb_launch_triple_sum(hipStream_t s, const double *II, long P, const 
     hipLaunchKernelGGL(k_triple_sum, dim3(ntri), dim3(256), 0, s, II, P, tri, out);
     return hipGetLastError();
 }
+
+/* ---------------------------------------------------------------------------
+ * The exact v2 normalisation (amber_v2.c:262-268): for the triangles whose
+ * split or FP32 sampling sums are not within 1/32 of their 1/8 grid, the
+ * same products on FP64 images with FP64 MFMA and whole-range FP64 sums.
+ * An item is a 64 x 64 (p2, p3) tile and 4 reference planes; wave q holds
+ * plane g0 + q as 4 x 4 blocks of v_mfma_f64_16x16x4f64 (A[m][k] = lane
+ * m + 16 k, B[k][n] = lane n + 16 k, lane l's four results are rows
+ * 4 r + l / 16 of column l % 16).  Chunks of 16 pixels are double-buffered
+ * through LDS (rows padded by one double so a read's 16 rows hit distinct
+ * banks); the next chunk's loads are in flight during the current one's
+ * products.  Items are taken in a grid-stride loop.
+ * ------------------------------------------------------------------------- */
+#define T64 64
+#define KC64 16
+
+struct Pref64 {
+    double2 a[2], b[2]; /* D2 and D3: rows tid/4, pixels 4 (tid%4) .. +4 */
+    double d1;          /* D1: plane tid/16, pixel tid%16 (threads 0..63) */
+};
+
+__device__ __forceinline__ void pref64(Pref64 &r, const ContractArgs64 &a, long k0, int r0, int c0, int g0, int tid)
+{
+    const int row = tid >> 2, k = (tid & 3) * 4;
+    const int last = a.nact - 1;
+    /* rows past nact read a valid row; their results are never stored */
+    const double *pa = a.D2 + (size_t)min(r0 + row, last) * a.pitch + k0 + k;
+    const double *pb = a.D3 + (size_t)min(c0 + row, last) * a.pitch + k0 + k;
+    r.a[0] = *(const double2 *)pa; r.a[1] = *(const double2 *)(pa + 2);
+    r.b[0] = *(const double2 *)pb; r.b[1] = *(const double2 *)(pb + 2);
+    if (tid < QP * KC64) r.d1 = a.D1[(size_t)min(g0 + (tid >> 4), last) * a.pitch + k0 + (tid & 15)];
+}
+
+struct Smem64 {
+    double A[2][T64][KC64 + 1];
+    double B[2][T64][KC64 + 1];
+    double D[2][QP][KC64];
+};
+
+__device__ __forceinline__ void stage64(Smem64 &L, int st, const Pref64 &r, int tid)
+{
+    const int row = tid >> 2, k = (tid & 3) * 4;
+    L.A[st][row][k] = r.a[0].x; L.A[st][row][k + 1] = r.a[0].y;
+    L.A[st][row][k + 2] = r.a[1].x; L.A[st][row][k + 3] = r.a[1].y;
+    L.B[st][row][k] = r.b[0].x; L.B[st][row][k + 1] = r.b[0].y;
+    L.B[st][row][k + 2] = r.b[1].x; L.B[st][row][k + 3] = r.b[1].y;
+    if (tid < QP * KC64) L.D[st][tid >> 4][tid & 15] = r.d1;
+}
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_norm64(ContractArgs64 a, int ngroups, int nt, long nwork)
+{
+    __shared__ Smem64 L;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int q = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int fm = lane & 15, fk = lane >> 4;
+    const long nchunk = a.pitch / KC64;
+    const double invP = 1.0 / (double)a.P;
+    for (long w = blockIdx.x; w < nwork; w += gridDim.x) {
+        const int grp = (int)(w % ngroups), tile = (int)(w / ngroups);
+        const int t2 = tile / nt, t3 = tile - t2 * nt;
+        const int r0 = t2 * T64, c0 = t3 * T64, g0 = grp * QP;
+        const int lo = a.wlo[g0], hi = a.whi[min(g0 + QP, a.nact) - 1];
+        if (r0 >= hi || r0 + T64 <= lo || c0 >= hi || c0 + T64 <= lo) continue;
+        if (a.sym12 && r0 + T64 <= g0) continue;
+        if (a.sym23 && r0 >= c0 + T64) continue;
+        const bool live = g0 + q < a.nact;
+        f64x4 acc[4][4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = (f64x4){0.0, 0.0, 0.0, 0.0};
+        Pref64 pf;
+        pref64(pf, a, 0, r0, c0, g0, tid);
+        __syncthreads(); /* the previous item's reads of L are done */
+        stage64(L, 0, pf, tid);
+        __syncthreads();
+        for (long c = 0; c < nchunk; ++c) {
+            const int cur = (int)(c & 1);
+            if (c + 1 < nchunk) pref64(pf, a, (c + 1) * KC64, r0, c0, g0, tid);
+            if (live) {
+#pragma unroll
+                for (int s = 0; s < KC64 / 4; ++s) {
+                    const int k = 4 * s + fk;
+                    const double d1 = L.D[cur][q][k];
+                    double fa[4], fb[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) fa[i] = L.A[cur][16 * i + fm][k] * d1;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) fb[j] = L.B[cur][16 * j + fm][k];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[i], fb[j], acc[i][j], 0, 0, 0);
+                }
+            }
+            if (c + 1 < nchunk) stage64(L, cur ^ 1, pf, tid);
+            __syncthreads();
+        }
+        if (!live) continue;
+        const int i1 = g0 + q, wlo = a.wlo[i1], whi = a.whi[i1];
+        float worst = 0.f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int gi = r0 + 16 * i + 4 * r + fk, gj = c0 + 16 * j + fm;
+                    if (gi < wlo || gi >= whi || gj < wlo || gj >= whi) continue;
+                    const double val = acc[i][j][r] * invP;
+                    const double q8 = rint(val * 8.0);
+                    const size_t idx = ((size_t)i1 * a.nact + gi) * a.nact + gj;
+                    if (fabs(q8) < 2147483647.0) {
+                        worst = fmaxf(worst, (float)fabs(val - q8 * 0.125));
+                        a.Cq[idx] = (int)q8;
+                    } else {
+                        worst = AMB_CERT_RANGE;
+                        a.Cq[idx] = q8 > 0 ? 2147483647 : -2147483647;
+                    }
+                }
+        if (worst > 0.f) atomicMax(a.margin, __float_as_uint(worst));
+    }
+}
+
+hipError_t amb_launch_norm64(hipStream_t s, const ContractArgs64 &a, int nblocks)
+{
+    if (a.nact <= 0 || a.pitch % KC64 != 0 || !a.D1 || !a.D2 || !a.D3 || !a.Cq || !a.margin) return hipErrorInvalidValue;
+    const int nt = (a.nact + T64 - 1) / T64, ngroups = (a.nact + QP - 1) / QP;
+    const long nwork = (long)nt * nt * ngroups;
+    const int blocks = (int)std::max<long>(1, std::min<long>(nwork, nblocks));
+    hipLaunchKernelGGL(k_norm64, dim3(blocks), dim3(256), 0, s, a, ngroups, nt, nwork);
+    return hipGetLastError();
+}

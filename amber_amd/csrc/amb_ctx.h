@@ -46,7 +46,8 @@ struct Cube {
  * 2 / 3: hi and lo BF16 halves of kind 0 / 1, stored hi planes then lo,
  * 4: e4m3 data images times their B-side scales,
  * 5: per-plane FP8 scales of kind 0 (A side [nact], then B side [nact]),
- * 7: hi and lo FP16 halves of kind 0 times 2^8, stored hi planes then lo.
+ * 7: hi and lo FP16 halves of kind 0 times 2^8, stored hi planes then lo,
+ * 8: FP64 sampling images (the exact v2 normalisation).
  * Kind 0 sets are stored normalised by a power of two (max |D| <= 2^7) and
  * end in two words: max |D| as computed and its exponent E. */
 struct PlaneSet {
@@ -119,6 +120,7 @@ struct amb_ctx {
     uint64_t tick = 0;
     std::map<int, float2 *> tw;
     std::map<int, FftPlan> plans;
+    std::map<int, double2 *> tw64; /* FP64 twiddles (exact normalisation) */
     int grouped_blocks = 0; /* persistent grid of the grouped contraction */
     int wave_max = 1024;    /* triangles per grouped wave */
     void *blas = nullptr;   /* rocBLAS handle of the post-processing products (first use) */
@@ -127,6 +129,8 @@ struct amb_ctx {
     float2 *G = nullptr; size_t G_bytes = 0;
     float *C = nullptr; size_t C_bytes = 0;
     int *Cq = nullptr; size_t Cq_bytes = 0;
+    int *Cqx = nullptr; size_t Cqx_bytes = 0;   /* exact normalisation partials */
+    void *outx = nullptr; size_t outx_bytes = 0; /* exact normalisation results */
     unsigned *margin = nullptr;
     void *out = nullptr; size_t out_bytes = 0;
     void *out2 = nullptr; size_t out2_bytes = 0; /* second result buffer of grouped waves */
@@ -182,7 +186,9 @@ void account_counts(amb_ctx *c, const TriJob &j, int64_t evals, int64_t pairs, i
 /* the normalisation certificate of one call or wave (worst grid distance m):
  * recorded in the stats; an error if out of the int32 range of eighths, or
  * beyond 1/32 with cert_strict set */
-int check_cert(amb_ctx *c, float m);
+int check_cert(amb_ctx *c, float m, bool *recompute);
+/* the exact (FP64) v2 normalisation of one triangle into host arrays */
+int norm_exact(amb_ctx *c, int id, int r1, int r2, int r3, float *norm, double *norm64, float *margin);
 /* releases the post-processing library handle (amb_destroy) */
 void post_release(amb_ctx *c);
 /* algorithmic HBM bytes of one c2r build (0 for derived kinds) */

@@ -339,26 +339,29 @@ hipError_t amb_launch_ring_c2r(hipStream_t s, int kind, const float2 *vis_t, con
     return hipGetLastError();
 }
 
-/* FP64 c2r of one sampling plane's ring (the calCR images, whose triple
- * sums must resolve a 1/8 grid of N^2): the same two passes in double. */
-hipError_t amb_launch_ring_c2r64(hipStream_t s, const float *samp_t, const int *d_plane, int N, int Nh,
-                                 const FftPlan &pl, const double2 *d_tw, RingGeom g, double2 *G, double *out)
+/* FP64 c2r of sampling planes' ring (the calCR images, and the exact v2
+ * normalisation's, whose triple sums must resolve a 1/8 grid of Ng^2): the
+ * same two passes in double. */
+hipError_t amb_launch_ring_c2r64(hipStream_t s, const float *samp_t, const int *d_planes, int npl, int N, int Nh,
+                                 const FftPlan &pl, const double2 *d_tw, RingGeom g, double2 *G, double *out,
+                                 long pitch)
 {
     const int n = pl.n;
-    const long pitch = (long)n * n;
+    if (npl <= 0) return hipSuccess;
+    if (pitch < (long)n * n) return hipErrorInvalidValue;
     if (g.jcnt <= 0) {
-        hipLaunchKernelGGL(k_zero64, dim3(256), dim3(256), 0, s, out, (size_t)pitch);
+        hipLaunchKernelGGL(k_zero64, dim3(1024), dim3(256), 0, s, out, (size_t)npl * pitch);
         return hipGetLastError();
     }
     const int nb = lds_batch(n, 16);
     const size_t lds = sizeof(double2) * (size_t)(n + 2 * nb * n);
     if (lds > 65536) return hipErrorInvalidValue;
-    dim3 gc((g.jcnt + nb - 1) / nb, 1);
-    hipLaunchKernelGGL(k_fft_cols<double>, gc, dim3(FFT_THREADS), lds, s, 1, nullptr, samp_t, d_plane, N, Nh, pl,
+    dim3 gc((g.jcnt + nb - 1) / nb, npl);
+    hipLaunchKernelGGL(k_fft_cols<double>, gc, dim3(FFT_THREADS), lds, s, 1, nullptr, samp_t, d_planes, N, Nh, pl,
                        d_tw, g.lo, g.hi, g.jcnt, nb, G);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    dim3 gr((n + 2 * nb - 1) / (2 * nb), 1);
+    dim3 gr((n + 2 * nb - 1) / (2 * nb), npl);
     hipLaunchKernelGGL(k_fft_rows<double>, gr, dim3(FFT_THREADS), lds, s, G, g.jcnt, pl, d_tw, nb, out, pitch,
                        nullptr);
     return hipGetLastError();

@@ -38,11 +38,12 @@ hipError_t amb_launch_ring_c2r(hipStream_t s, int kind, const float2 *vis_t, con
                                const float2 *d_tw, RingGeom g, float2 *G, float *out, long pitch,
                                unsigned *slot);
 void amb_make_plan(int n, FftPlan *pl);
-/* FP64 c2r of one sampling plane's ring (d_plane: its channel id on the
- * device) into out[n*n]; G: [jcnt][n] complex128 scratch; d_tw: n twiddles
- * from amb_launch_twiddles64 */
-hipError_t amb_launch_ring_c2r64(hipStream_t s, const float *samp_t, const int *d_plane, int N, int Nh,
-                                 const FftPlan &pl, const double2 *d_tw, RingGeom g, double2 *G, double *out);
+/* FP64 c2r of npl sampling planes' ring (d_planes: their channel ids on
+ * the device) into out[npl][pitch] (tails zeroed); G: [npl][jcnt][n]
+ * complex128 scratch; d_tw: n twiddles from amb_launch_twiddles64 */
+hipError_t amb_launch_ring_c2r64(hipStream_t s, const float *samp_t, const int *d_planes, int npl, int N, int Nh,
+                                 const FftPlan &pl, const double2 *d_tw, RingGeom g, double2 *G, double *out,
+                                 long pitch);
 hipError_t amb_launch_twiddles64(hipStream_t s, double2 *tw, int n);
 
 /* ---- cube upload (amb_fft.hip) ---- */
@@ -91,6 +92,18 @@ hipError_t amb_launch_split_f16(hipStream_t s, const float *D, size_t n, _Float1
  * scales; quant writes the e4m3 planes D*t[i] */
 hipError_t amb_launch_fp8_scales(hipStream_t s, const float *D, long pitch, int n, float *sc);
 hipError_t amb_launch_fp8_quant(hipStream_t s, const float *D, long pitch, int n, const float *t, uint8_t *Q);
+
+/* The exact v2 normalisation: FP64 sampling images, FP64 MFMA products,
+ * whole-pixel-range FP64 sums rounded to their 1/8 grid (Cq, margin). */
+struct ContractArgs64 {
+    const double *D1, *D2, *D3; /* [nact][pitch] FP64 sampling images of r1, r2, r3 */
+    long P, pitch;
+    int nact, sym12, sym23;
+    const int *wlo, *whi;
+    int *Cq;                    /* [nact][nact][nact] int32 of 8 * count */
+    unsigned *margin;           /* max |S/P - q| as float bits */
+};
+hipError_t amb_launch_norm64(hipStream_t s, const ContractArgs64 &a, int nblocks);
 
 struct ReduceArgs {
     const float *C;     /* data partials or NULL */

@@ -47,7 +47,8 @@ struct Wave {
     size_t c_total = 0, o_total = 0;
     int nnorm = 0;
     /* descriptor block: data args, norm args, reduce args, item begins
-       (data, norm), cell begins, two counters, the margin word */
+       (data, norm), cell begins, two counters, one margin word per
+       normalised triangle */
     size_t off_da = 0, off_na = 0, off_ra = 0, off_b = 0, off_nb = 0, off_cb = 0, off_ctr = 0, off_mg = 0;
     size_t d_total = 0;
     long items = 0, nitems = 0, cells = 0;
@@ -98,7 +99,7 @@ void layout(Wave &w)
     w.off_nb = o; o = al(o + sizeof(long) * (w.nnorm + 1));
     w.off_cb = o; o = al(o + sizeof(long) * (n + 1));
     w.off_ctr = o; o = al(o + sizeof(unsigned long long) * 2 * AMB_GROUPED_COUNTERS);
-    w.off_mg = o; o = al(o + sizeof(unsigned));
+    w.off_mg = o; o = al(o + sizeof(unsigned) * std::max(w.nnorm, 1));
     w.d_total = o;
 }
 
@@ -136,7 +137,7 @@ std::vector<char> descriptors(const Wave &w, const WaveBufs &b)
             na[k] = j.b;
             na[k].C = nullptr;
             na[k].Cq = b.Cq + w.c_off[i];
-            na[k].margin = mg;
+            na[k].margin = mg + k; /* per triangle: an uncertified one is recomputed alone */
             nb[k] = nit;
             nit += amb_contract_items(j.nact);
             ++k;
@@ -175,7 +176,7 @@ hipError_t launch_wave(amb_ctx *c, hipStream_t s, const Wave &w, const WaveBufs 
                                                c->grouped_blocks);
     if (timed) timer_stop(c);
     if (e == hipSuccess && w.nnorm) {
-        e = amb_launch_fill_u32(s, mg, 1, 0u);
+        e = amb_launch_fill_u32(s, mg, w.nnorm, 0u);
         if (timed) timer_start(c, &c->st.ms_norm);
         if (e == hipSuccess)
             e = amb_launch_contract_grouped(s, nprec, 1, var, (const ContractArgs *)(b.desc + w.off_na),
@@ -190,12 +191,14 @@ hipError_t launch_wave(amb_ctx *c, hipStream_t s, const Wave &w, const WaveBufs 
     if (timed) timer_stop(c);
     if (e == hipSuccess && copy) e = hipMemcpyAsync(b.host, b.out, w.o_total, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess && w.nnorm)
-        e = hipMemcpyAsync(b.host + w.o_total, mg, sizeof(unsigned), hipMemcpyDeviceToHost, s);
+        e = hipMemcpyAsync(b.host + w.o_total, mg, sizeof(unsigned) * w.nnorm, hipMemcpyDeviceToHost, s);
     return e;
 }
 
-/* results of a finished wave from the host copy into the requests */
-int finish_wave(amb_ctx *c, const Wave &w, const char *host, const amb_tri_req *req)
+/* results of a finished wave from the host copy into the requests; the
+   requests whose normalisation sums missed their certificate are appended
+   to *redo (recomputed exactly once every wave is done) */
+int finish_wave(amb_ctx *c, const Wave &w, const char *host, const amb_tri_req *req, std::vector<int> *redo)
 {
     /* the triangles' copies out of the pinned staging and their count sums
        are independent: spread them over a few host threads */
@@ -225,10 +228,15 @@ int finish_wave(amb_ctx *c, const Wave &w, const char *host, const amb_tri_req *
         for (auto &x : th) x.join();
     }
     for (size_t i = 0; i < nj; ++i) account_counts(c, w.jobs[i], ev[i], pr[i], dg[i]);
-    if (w.nnorm) {
+    int k = 0;
+    for (size_t i = 0; i < nj; ++i) {
+        if (!w.jobs[i].want_norm) continue;
         float m;
-        memcpy(&m, host + w.o_total, sizeof m);
-        return check_cert(c, m);
+        memcpy(&m, host + w.o_total + sizeof(unsigned) * k++, sizeof m);
+        bool again = false;
+        const int rc = check_cert(c, m, &again);
+        if (rc) return rc;
+        if (again) redo->push_back(w.req[i]);
     }
     return AMB_OK;
 }
@@ -253,7 +261,7 @@ int run_wave(amb_ctx *c, Wave &w, int hb, const std::function<int()> &overlap)
     if (!rc) rc = grow(c, outp, hb ? &c->out2_bytes : &c->out_bytes, w.o_total, "lag arrays");
     if (!rc) rc = copy_stream(c);
     if (!rc) rc = grow(c, &c->wdesc, &c->wdesc_bytes, w.d_total, "wave descriptors");
-    if (!rc) rc = grow_host(c, &c->hstage[hb], &c->hstage_bytes[hb], w.o_total + sizeof(unsigned));
+    if (!rc) rc = grow_host(c, &c->hstage[hb], &c->hstage_bytes[hb], w.o_total + sizeof(unsigned) * std::max(w.nnorm, 1));
     if (rc) return rc;
     WaveBufs b{c->C, c->Cq, (char *)*outp, (char *)c->wdesc, (char *)c->hstage[hb]};
     std::vector<char> h = descriptors(w, b);
@@ -276,19 +284,35 @@ int run_wave(amb_ctx *c, Wave &w, int hb, const std::function<int()> &overlap)
     return orc;
 }
 
-int run_grouped_waves(amb_ctx *c, const amb_tri_req *req, int n, double t0, double *done_ms);
+int run_grouped_waves(amb_ctx *c, const amb_tri_req *req, int n, double t0, double *done_ms, std::vector<int> *redo);
+
+/* the exact normalisation of the requests whose fast sums missed their
+   certificate (after every wave: it has buffers of its own but shares the
+   stream) */
+int redo_norms(amb_ctx *c, const amb_tri_req *req, const std::vector<int> &redo, double t0, double *done_ms)
+{
+    for (int i : redo) {
+        const amb_tri_req &q = req[i];
+        int rc = norm_exact(c, q.cube_id, q.r1, q.r2, q.r3, q.norm, nullptr, nullptr);
+        if (rc) return rc;
+        if (done_ms) done_ms[i] = now_ms() - t0;
+    }
+    return AMB_OK;
+}
 
 /* all requests as grouped waves; done_ms[i] gets request i's completion
    time relative to t0 (may be null). No result copy is left running on
    return, also after an error. */
 int run_grouped(amb_ctx *c, const amb_tri_req *req, int n, double t0, double *done_ms)
 {
-    const int rc = run_grouped_waves(c, req, n, t0, done_ms);
+    std::vector<int> redo;
+    int rc = run_grouped_waves(c, req, n, t0, done_ms, &redo);
     if (c->cstream) hipStreamSynchronize(c->cstream);
+    if (!rc) rc = redo_norms(c, req, redo, t0, done_ms);
     return rc;
 }
 
-int run_grouped_waves(amb_ctx *c, const amb_tri_req *req, int n, double t0, double *done_ms)
+int run_grouped_waves(amb_ctx *c, const amb_tri_req *req, int n, double t0, double *done_ms, std::vector<int> *redo)
 {
     hipSetDevice(c->device);
     std::vector<int> order(n);
@@ -311,7 +335,7 @@ int run_grouped_waves(amb_ctx *c, const amb_tri_req *req, int n, double t0, doub
             prev = Wave{};
             return fail(c, AMB_EHIP, "copy of a grouped wave's results: %s", hipGetErrorString(e));
         }
-        const int rc = finish_wave(c, prev, (const char *)c->hstage[hb ^ 1], req);
+        const int rc = finish_wave(c, prev, (const char *)c->hstage[hb ^ 1], req, redo);
         const double t = now_ms() - t0;
         if (done_ms)
             for (int r : prev.req) done_ms[r] = t;
@@ -552,7 +576,7 @@ int amb_graph_create(amb_ctx *c, const amb_tri_req *req, int n, amb_graph **out)
         cmax = std::max(cmax, wv.c_total);
         g->w_out.push_back(otot);
         g->w_desc.push_back(dtot);
-        otot += al(wv.o_total + sizeof(unsigned));
+        otot += al(wv.o_total + sizeof(unsigned) * std::max(wv.nnorm, 1));
         dtot += al(wv.d_total);
         any_norm |= wv.nnorm > 0;
     }
@@ -621,10 +645,13 @@ int amb_graph_run(amb_ctx *c, amb_graph *gh)
     c->st.bytes_fft += g->fft_bytes;
     c->st.n_fft_launch += g->n_fft;
     int rc = AMB_OK;
+    std::vector<int> redo;
     for (size_t k = 0; k < g->waves.size(); ++k) {
-        int r = finish_wave(c, g->waves[k], (const char *)g->host + g->w_out[k], g->req.data());
+        int r = finish_wave(c, g->waves[k], (const char *)g->host + g->w_out[k], g->req.data(), &redo);
         if (r && !rc) rc = r;
     }
+    /* outside the graph: the exact normalisation of uncertified triangles */
+    if (!rc) rc = redo_norms(c, g->req.data(), redo, now_ms(), nullptr);
     return rc;
 }
 

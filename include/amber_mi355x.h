@@ -437,22 +437,25 @@ typedef struct amb_stats {
     double flop_contract_distinct; /* flop_contract over the distinct sums only: where
                                       r1 == r2 (or r2 == r3) makes two triplets' sums the
                                       same, the pair counts once */
+    int64_t n_norm_exact;   /* triangles whose v2 normalisation was recomputed in FP64 */
 } amb_stats;
 int amb_get_stats(amb_ctx *ctx, amb_stats *st);
 int amb_reset_stats(amb_ctx *ctx);
 
 /* The v2 normalisation sum of a triplet is a whole number of eighths of
  * Ng^2 (the sampling images are 0/1 masks, half-weighted on the kx = 0
- * column).  The contraction accumulates in FP32 and rounds each sum to the
- * nearest 1/8; the distance of the worst sum from that grid is its
- * certificate.  Within 1/32 the rounded sums are exact.  Beyond it (large
- * Ng: the sums reach ~1e5 Ng^2, past what FP32 resolves) the rounded sums
- * carry FP32 accumulation error, as the reference's own FP32 loop
- * (amber_v2.c:262-266) does.  strict = 0 (default): such calls succeed and
- * are counted in the stats (fields n_uncertified, cert_margin_max), like the
- * reference, which never fails here.  strict = 1: they fail with
- * AMB_ECERT.  No counterpart in the reference.  A sum beyond the int32
- * range of eighths fails in either mode. */
+ * column).  The fast contraction (BF16x3, or FP32 in the FP32 form) rounds
+ * each sum to the nearest 1/8; the distance of the triangle's worst sum
+ * from that grid is its certificate.  Within 1/32 the rounded sums are
+ * exact.  A triangle beyond it (large Ng: the sums reach ~1e5 Ng^2, past
+ * what FP32 accumulation resolves) is recomputed with FP64 sampling images
+ * and FP64 MFMA products, whose sums sit within ~1e-9 of the grid: the
+ * normalisation is exact in every case (stats: n_uncertified counts the
+ * fast path's misses, n_norm_exact the recomputations).  strict = 0
+ * (default): recompute.  strict = 1: fail with AMB_ECERT instead, for
+ * callers that must not pay the FP64 cost.  No counterpart in the
+ * reference, whose FP32 sums (amber_v2.c:262-266) are never exact at these
+ * sizes.  A sum beyond the int32 range of eighths fails in either mode. */
 int amb_set_cert_strict(amb_ctx *ctx, int strict);
 
 #ifdef __cplusplus

@@ -121,12 +121,13 @@ def test_hardened_wire_matches_legacy(tmp_path):
 
 
 def test_v2_certificate_modes(tmp_path):
-    """amber_v2 on a cube whose normalisation sums FP32 cannot certify
-    (N = 256, Ng up to 192): by default the run completes and says how many
-    waves were not certified; --cert strict makes it an error exit."""
+    """amber_v2 on a cube whose fast normalisation sums cannot be certified
+    (N = 256, Ng up to 192): by default the run recomputes them in FP64 and
+    writes exact normalisations (checked against the oracle's snapped
+    sums); --cert strict makes it an error exit."""
     N, nc, nflag, nc10 = 256, 40, 0, 12
     lo, hi = A.log_rings(20, 60, 3)
-    _layout(tmp_path, N, nc, nflag, 61, lo, hi)
+    gv, samp, _ = _layout(tmp_path, N, nc, nflag, 61, lo, hi)
     print("case v2 certificate modes N=%d nc=%d" % (N, nc), flush=True)
     _run(tmp_path, "amber", "--nc10", str(nc10), "--uf-nc", str(nc))
     for fn in ("rcount", "closed_rings"):
@@ -134,7 +135,15 @@ def test_v2_certificate_modes(tmp_path):
     n, rd = A.read_rings(str(tmp_path / "results/v1"))
     assert n > 0 and max(A.ng(hi[int(r1)]) for r1 in rd[:, 0]) >= 134
     out = _run(tmp_path, "amber_v2", "--nc10", str(nc10), "--uf-nc", str(nc))
-    assert "not certified exact" in out
+    assert "recomputed in FP64" in out, out[-2000:]
+    mask = np.ones(nc, np.uint8)
+    W = 2 * nc10 + 1
+    for r1, r2, r3 in rd[:, :3].astype(int)[-2:]:
+        raw = (tmp_path / "results/v2" / ("bsout_final2_ra%d_rb%d_rc%d_R1" % (r1, r2, r3))).read_bytes()
+        norm = np.frombuffer(raw, np.float32, W * W, 8 + 4 * W * W).reshape(W, W)
+        _, _, n64 = O.bispec(N, nc, 1.0, gv, samp, lo, hi, r1, r2, r3, mask, nc10, want_norm=True)
+        P = A.ng(hi[r1]) ** 2
+        assert np.array_equal(norm, (P * np.rint(n64 / P * 8) / 8).astype(np.float32)), (r1, r2, r3)
     r = subprocess.run([os.path.join(BIN, "amber_v2"), "1", "--rings", "rings.txt", "--nc10", str(nc10), "--uf-nc",
                         str(nc), "--cert", "strict"], cwd=tmp_path, stdout=subprocess.PIPE,
                        stderr=subprocess.STDOUT, text=True, timeout=120)

@@ -117,3 +117,34 @@ def test_metric_workload_error_vs_fp64(ctx, big, metric_cells, prec):
     finally:
         ctx.release_cube(c1)
         ctx.set_precision("bf16x3")
+
+
+def test_metric_workload_normalisation_is_exact(ctx, big):
+    """The amber_v2 normalisation at the metric workload (768 ch x 512^2,
+    Ng = 612): equal bit for bit to the FP64 oracle's sums snapped to their
+    1/8 grid of Ng^2, on 49 lag cells, whether the fast path certified the
+    triangle or it was recomputed in FP64."""
+    gv, samp, _ = big
+    lo, hi = A.log_rings(1, 200, 24)
+    t = (23, 20, 20)
+    ctx.set_precision("bf16x3")
+    ctx.set_rings(lo, hi)
+    c1 = ctx.load_cube(N, NC, 1.0, 0, gv, samp)
+    ctx.set_channels(c1, None, NC10)
+    rng = np.random.default_rng(13)
+    d2 = np.append(rng.integers(-NC10, NC10 + 1, 48), 0)
+    d3 = np.append(rng.integers(-NC10, NC10 + 1, 48), 0)
+    try:
+        ctx.reset_stats()
+        res = ctx.bispec_ex(c1, *t, want_norm=True)
+        st = ctx.stats()
+        print("metric workload normalisation Ng=%d fast margin %.3g recomputed %d" %
+              (res["ng"], st["cert_margin_max"], st["n_norm_exact"]), flush=True)
+        _, c64, n64 = O.bispec_cells(N, NC, 1.0, gv, samp, lo, hi, *t, np.ones(NC, np.uint8), NC10, d2, d3,
+                                     want_bl=False, want_norm=True)
+        P = res["ng"] ** 2
+        assert np.max(np.abs(n64 / P * 8 - np.rint(n64 / P * 8))) < 1e-6
+        assert np.array_equal(res["cnt"][d2 + NC10, d3 + NC10], c64)
+        assert np.array_equal(res["norm64"][d2 + NC10, d3 + NC10], P * np.rint(n64 / P * 8) / 8)
+    finally:
+        ctx.release_cube(c1)

@@ -298,29 +298,37 @@ def test_pipelined_variant_is_bit_identical(ctx, N, nc, nflag, nc10, rings, prec
         c.close()
 
 
-def test_uncertified_normalisation_counts_or_fails(ctx):
+@pytest.mark.parametrize("prec", ["bf16x3", "fp32"])
+def test_uncertified_normalisation_is_recomputed_exactly(ctx, prec):
     """At Ng = 134 the v2 normalisation sums reach ~1.4e5 Ng^2, past what the
-    FP32 accumulation resolves on their 1/8 grid (the FP64 oracle's sums sit
-    on it to 1e-9): by default the call succeeds, counts the uncertified
-    call in the stats and returns sums within FP32 accumulation error of the
-    exact ones, as the reference's FP32 loop would; in strict mode it fails
-    with AMB_ECERT."""
+    fast path's FP32 accumulation resolves on their 1/8 grid (the FP64
+    oracle's sums sit on it to 1e-9): the call counts the miss, recomputes
+    the triangle in FP64 and returns the exact sums bit for bit; in strict
+    mode it fails with AMB_ECERT instead."""
     N, nc, nc10 = 256, 40, 12
     lo, hi = A.log_rings(20, 60, 3)
     cube, gv, samp, uf, mask = _setup(ctx, N, nc, 0, 61, lo, hi, nc10)
     r1, r2, r3 = 1, 0, 0
-    ctx.set_precision("bf16x3")
-    print("case uncertified normalisation N=%d Ng=%d" % (N, A.ng(hi[r1])), flush=True)
+    ctx.set_precision(prec)
+    print("case exact normalisation %s N=%d Ng=%d" % (prec, N, A.ng(hi[r1])), flush=True)
     bl64, cnt64, n64 = O.bispec(N, nc, 1.0, gv, samp, lo, hi, r1, r2, r3, mask, nc10, want_norm=True)
     P = A.ng(hi[r1]) ** 2
     assert np.max(np.abs(n64 / P * 8 - np.rint(n64 / P * 8))) < 1e-6   # the exact sums are on the grid
     ctx.reset_stats()
     res = ctx.bispec_ex(cube, r1, r2, r3, want_norm=True)
     st = ctx.stats()
-    assert res["cert_margin"] > 1.0 / 32 and st["n_uncertified"] == 1
-    assert st["cert_margin_max"] == pytest.approx(res["cert_margin"])
+    assert st["n_uncertified"] == 1 and st["n_norm_exact"] == 1 and st["cert_margin_max"] > 1.0 / 32
+    assert res["cert_margin"] < 1e-6                                    # the FP64 sums' own distance
     assert np.array_equal(res["cnt"], cnt64)
-    assert _relerr(res["norm64"], n64) <= 1e-5
+    assert np.array_equal(res["norm64"], _exact_norm(n64, P))
+    assert np.array_equal(res["norm"], _exact_norm(n64, P).astype(np.float32))
+    # in a grouped batch with other triangles: the same exact sums
+    tris = np.array([[r1, r2, r3], [2, 2, 1], [2, 1, 1]], np.int32)
+    bl, cnt, norm = ctx.bispec_batch(cube, tris, want_norm=True)
+    assert np.array_equal(norm[0], res["norm"])
+    for k in (1, 2):
+        _, _, nk = O.bispec(N, nc, 1.0, gv, samp, lo, hi, *tris[k], mask, nc10, want_norm=True)
+        assert np.array_equal(norm[k], _exact_norm(nk, A.ng(hi[tris[k][0]]) ** 2).astype(np.float32))
     ctx.set_cert_strict(True)
     try:
         with pytest.raises(A.AmberError) as e:
@@ -328,6 +336,7 @@ def test_uncertified_normalisation_counts_or_fails(ctx):
         assert e.value.code == -8
     finally:
         ctx.set_cert_strict(False)
+    ctx.set_precision("bf16x3")
     ctx.release_cube(cube)
 
 
@@ -441,10 +450,7 @@ def test_du_above_one(ctx, prec):
         assert res["ng"] == O.ng(hi[r1], Du)
         b64, c64, n64 = O.bispec(N, nc, Du, gv, samp, lo, hi, r1, r2, r3, mask, nc10, want_norm=True)
         assert np.array_equal(res["cnt"], c64)
-        if res["cert_margin"] <= 1.0 / 32:
-            assert np.array_equal(res["norm64"], _exact_norm(n64, res["ng"] ** 2))
-        else:   # an uncertified call: FP32 accumulation error, counted in the stats
-            assert _relerr(res["norm64"], n64) <= 1e-5
+        assert np.array_equal(res["norm64"], _exact_norm(n64, res["ng"] ** 2))
         if np.linalg.norm(b64) > 0:
             assert _relerr(res["bl64"], b64) <= TOL[prec]
     ctx.release_cube(cube)
