@@ -36,7 +36,8 @@ EXPORTS = [
     "amb_graph_destroy", "amb_serve_start", "amb_submit", "amb_poll", "amb_wait", "amb_serve_stop",
     "amb_cyl_bispec", "amb_maps_cldnu", "amb_write_cldnu", "amb_read_params", "amb_write_bsw", "amb_read_bsw",
     "amb_read_bsout", "amb_crc32", "amb_read_vcg_inputs", "amb_grid_setup", "amb_grid_cells", "amb_grid",
-    "amb_grid_cube", "amb_set_cert_strict", "amb_closed_rings_noalloc",
+    "amb_grid_cube", "amb_set_cert_strict", "amb_closed_rings_noalloc", "amb_autotune", "amb_tune_save",
+    "amb_tune_load", "amb_tune_clear", "amb_tune_entries", "amb_tune_lookup", "amb_set_launch_config",
 ]
 
 BATCH_PER_TRIANGLE, BATCH_GROUPED = 0, 1
@@ -63,6 +64,16 @@ class Result(C.Structure):
 class TriReq(C.Structure):
     _fields_ = [("cube_id", C.c_int), ("r1", C.c_int), ("r2", C.c_int), ("r3", C.c_int),
                 ("bl", _fp), ("cnt", _ip), ("norm", _fp)]
+
+
+class TuneEntry(C.Structure):
+    _fields_ = [("prec", C.c_int), ("ng_log2", C.c_int), ("nact_log2", C.c_int), ("variant", C.c_int),
+                ("xrun_log2", C.c_int), ("wave_gib", C.c_int), ("grid_pct", C.c_int), ("tflops", C.c_double)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["prec"] = {v: k for k, v in _PREC.items()}.get(self.prec, self.prec)
+        return d
 
 
 class CylParams(C.Structure):
@@ -125,6 +136,13 @@ _lib.amb_ring_image.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int64, C.c_int
 _lib.amb_batch_times.argtypes = [C.c_void_p, _dp, C.c_int, _ip]
 _lib.amb_bispec_multi.argtypes = [C.c_void_p, C.POINTER(TriReq), C.c_int]
 _lib.amb_set_batch_mode.argtypes = [C.c_void_p, C.c_int]
+_lib.amb_autotune.argtypes = [C.c_void_p, C.POINTER(TriReq), C.c_int, C.c_double, C.c_char_p]
+_lib.amb_tune_save.argtypes = [C.c_void_p, C.c_char_p]
+_lib.amb_tune_load.argtypes = [C.c_void_p, C.c_char_p]
+_lib.amb_tune_clear.argtypes = [C.c_void_p]
+_lib.amb_tune_entries.argtypes = [C.c_void_p, C.POINTER(TuneEntry), C.c_int, _ip]
+_lib.amb_tune_lookup.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(TuneEntry)]
+_lib.amb_set_launch_config.argtypes = [C.c_void_p, C.POINTER(TuneEntry)]
 _lib.amb_update_cube.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
 _lib.amb_graph_create.argtypes = [C.c_void_p, C.POINTER(TriReq), C.c_int, C.POINTER(C.c_void_p)]
 _lib.amb_graph_run.argtypes = [C.c_void_p, C.c_void_p]
@@ -543,6 +561,45 @@ class Context:
         """"grouped" (one persistent launch per wave) or "per_triangle" """
         m = {"grouped": BATCH_GROUPED, "per_triangle": BATCH_PER_TRIANGLE}[mode]
         self._c(_lib.amb_set_batch_mode(self._h, m))
+
+    def autotune(self, reqs, max_seconds=60.0, path=None, want_cnt=True, want_norm=False):
+        """Tune the contraction's launch configuration on a sample of
+        triangles [(cube, r1, r2, r3), ...] (amb_autotune); returns the
+        installed table and the sample's results."""
+        arr, outs = self._reqs(reqs, want_cnt, want_norm)
+        self._c(_lib.amb_autotune(self._h, arr, len(reqs), float(max_seconds),
+                                  None if path is None else os.fsencode(path)))
+        return self.tune_table(), outs
+
+    def tune_table(self):
+        n = C.c_int()
+        self._c(_lib.amb_tune_entries(self._h, None, 0, C.byref(n)))
+        arr = (TuneEntry * max(n.value, 1))()
+        self._c(_lib.amb_tune_entries(self._h, arr, n.value, C.byref(n)))
+        return [arr[i].as_dict() for i in range(n.value)]
+
+    def tune_save(self, path):
+        self._c(_lib.amb_tune_save(self._h, os.fsencode(path)))
+
+    def tune_load(self, path):
+        self._c(_lib.amb_tune_load(self._h, os.fsencode(path)))
+
+    def tune_clear(self):
+        self._c(_lib.amb_tune_clear(self._h))
+
+    def tune_lookup(self, ng, nact):
+        e = TuneEntry()
+        self._c(_lib.amb_tune_lookup(self._h, int(ng), int(nact), C.byref(e)))
+        return e.as_dict()
+
+    def set_launch_config(self, variant=None, xrun_log2=5, wave_gib=16, grid_pct=100):
+        """One launch configuration for every shape (variant None: back to
+        the tuned table)."""
+        if variant is None:
+            self._c(_lib.amb_set_launch_config(self._h, None))
+            return
+        e = TuneEntry(0, 0, 0, int(variant), int(xrun_log2), int(wave_gib), int(grid_pct), 0.0)
+        self._c(_lib.amb_set_launch_config(self._h, C.byref(e)))
 
     def update_cube(self, cube, gv, samp):
         gv = np.ascontiguousarray(gv, np.float64)

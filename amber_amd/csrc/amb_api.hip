@@ -693,8 +693,9 @@ int bispec_core(amb_ctx *c, int id, int r1, int r2, int r3, int want_norm, amb_r
         e = (expr);                                                   \
         if (e == hipSuccess && c->sync_debug) e = hipStreamSynchronize(c->stream); \
     }
+    const amb_tune_entry tc = tune_cfg(c, j.ng, j.nact);
     timer_start(c, &c->st.ms_contract);
-    STEP("contract", amb_launch_contract(c->stream, prec, 0, c->kvar, j.a));
+    STEP("contract", amb_launch_contract(c->stream, prec, 0, tune_variant(tc, prec, j.nact, j.pitch), j.a));
     timer_stop(c);
     c->st.n_contract_launch++;
     if (want_norm) {
@@ -702,7 +703,7 @@ int bispec_core(amb_ctx *c, int id, int r1, int r2, int r3, int want_norm, amb_r
         j.b.margin = c->margin;
         STEP("margin reset", amb_launch_fill_u32(c->stream, c->margin, 1, 0u));
         timer_start(c, &c->st.ms_norm);
-        STEP("norm contract", amb_launch_contract(c->stream, nprec, 1, c->kvar, j.b));
+        STEP("norm contract", amb_launch_contract(c->stream, nprec, 1, tune_variant(tc, nprec, j.nact, j.pitch), j.b));
         timer_stop(c);
     }
     ReduceArgs ra = j.ra;
@@ -890,10 +891,6 @@ int amb_create(const amb_opts *o, amb_ctx **out)
     size_t fr = 0, tot = 0;
     if (hipMemGetInfo(&fr, &tot) != hipSuccess) { delete c; return AMB_EHIP; }
     c->sync_debug = getenv("AMB_SYNC_DEBUG") != nullptr;
-    if (const char *v = getenv("AMB_KVAR")) {
-        const int k = atoi(v);
-        c->kvar = (k >= 0 && k < AMB_NVAR) ? k : 0;
-    }
     c->no_sym = getenv("AMB_NO_SYM") != nullptr; /* for A/B checks of the symmetry */
     if (const char *v = getenv("AMB_WAVE_TRIS")) { /* smaller grouped waves, for tests */
         const int k = atoi(v);
@@ -906,6 +903,11 @@ int amb_create(const amb_opts *o, amb_ctx **out)
     c->margin = (unsigned *)m;
     c->used = 256;
     c->grouped_blocks = amb_contract_grouped_blocks(c->device);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) c->arch = prop.gcnArchName;
+    if (c->arch.empty()) c->arch = "unknown";
+    for (char &ch : c->arch)
+        if (ch == ' ') ch = '_';
     c->na_scratch = noalloc_scratch(c->device); /* with the context's device current */
     *out = c;
     return AMB_OK;

@@ -87,7 +87,12 @@ __device__ __forceinline__ float data_unscale(const ContractArgs &a)
  *   0: PIPE = 0, a k-step forms all its fragments, then multiplies
  *   1: PIPE = 1, (BF16x3) a k-step forms the A fragment of row block i+1
  *      while the 12 MFMAs of block i issue; scheduling hints interleave
- *      MFMA with the LDS reads and VALU work (other forms: as var 0) */
+ *      MFMA with the LDS reads and VALU work (other forms: as var 0)
+ *   2: the split forms' k_contract_v2 / k_contract_grouped_v2 below (direct
+ *      global->LDS loads, pipelined k-steps)
+ * All three sum the same products in the same order.  The launch
+ * configuration table of the context (amb_tune.hip) picks the variant per
+ * shape bucket. */
 template <int NWR>
 struct Geo {
     static constexpr int NT = 256 * NWR; /* threads */
@@ -1026,19 +1031,21 @@ __global__ __launch_bounds__(512, 1) void k_contract_v2(ContractArgs a, int ngro
     contract_item_v2<FORM, NORM>(a, w / ngroups, w % ngroups, S0, S1);
 }
 
-/* XCD queues of the grouped contraction: runs of XRUN consecutive items (one
+/* XCD queues of the grouped contraction: runs of 2^xl consecutive items (one
  * tile's p1 groups, which stream the same D2 / D3 chunks) are dealt
  * round-robin to the 8 XCDs; workgroup b sits on XCD b mod 8 and takes items
  * from that XCD's counter (xq starts at b mod 8), then from the others once
  * it runs dry.  Within a queue the item index only grows, so a queue that
  * yields an index past the end is empty for good: each caller makes at most
- * 8 such failed takes (left), then gets -1. */
-__device__ __forceinline__ long xcd_take(unsigned long long *counter, long total, int &xq, int &left)
+ * 8 such failed takes (left), then gets -1.  The run length is a tuning
+ * parameter (32 by default); the items each triangle computes do not
+ * depend on it. */
+__device__ __forceinline__ long xcd_take(unsigned long long *counter, long total, int &xq, int &left, int xl)
 {
-    constexpr long XRUN = 32;
+    const unsigned long long msk = (1ull << xl) - 1;
     while (left > 0) {
         const unsigned long long j = atomicAdd(counter + xq, 1ull);
-        const long cand = ((long)(j / XRUN) * 8 + xq) * XRUN + (long)(j % XRUN);
+        const long cand = ((long)(j >> xl) * 8 + xq) * (1l << xl) + (long)(j & msk);
         if (cand < total) return cand;
         xq = (xq + 1) & 7;
         --left;
@@ -1049,14 +1056,14 @@ __device__ __forceinline__ long xcd_take(unsigned long long *counter, long total
 template <int FORM, int NORM>
 __global__ __launch_bounds__(512, 1) void k_contract_grouped_v2(const ContractArgs *__restrict__ descs,
                                                                 const long *__restrict__ begin, int ntri,
-                                                                unsigned long long *counter)
+                                                                unsigned long long *counter, int xl)
 {
     V2_STAGES
     __shared__ long item;
     const long total = begin[ntri];
     int xq = blockIdx.x & 7, left = 8;
     for (;;) {
-        if (threadIdx.x == 0) item = xcd_take(counter, total, xq, left);
+        if (threadIdx.x == 0) item = xcd_take(counter, total, xq, left, xl);
         __syncthreads();
         const long w = item;
         __syncthreads();
@@ -1095,14 +1102,14 @@ __global__ __launch_bounds__(256 * NWR, 1) void k_contract(ContractArgs a, int n
 template <int MODE, int NORM, int NWR, int PIPE>
 __global__ __launch_bounds__(256 * NWR, 1) void k_contract_grouped(const ContractArgs *__restrict__ descs,
                                                                    const long *__restrict__ begin, int ntri,
-                                                                   unsigned long long *counter)
+                                                                   unsigned long long *counter, int xl)
 {
     __shared__ Smem<MODE, nstages<PIPE>()> L;
     __shared__ long item;
     const long total = begin[ntri];
     int xq = blockIdx.x & 7, left = 8;
     for (;;) {
-        if (threadIdx.x == 0) item = xcd_take(counter, total, xq, left);
+        if (threadIdx.x == 0) item = xcd_take(counter, total, xq, left, xl);
         __syncthreads();
         const long w = item;
         __syncthreads();
@@ -1198,16 +1205,16 @@ double amb_contract_item_flops(int prec, long pitch)
 
 template <int MODE, int NORM>
 static void launch_grouped_var(int var, dim3 grid, hipStream_t s, const ContractArgs *d, const long *b, int n,
-                               unsigned long long *ctr)
+                               unsigned long long *ctr, int xl)
 {
     if constexpr (is_split<MODE>()) {
         if (var == 2) {
-            hipLaunchKernelGGL((k_contract_grouped_v2<MODE, NORM>), grid, dim3(512), 0, s, d, b, n, ctr);
+            hipLaunchKernelGGL((k_contract_grouped_v2<MODE, NORM>), grid, dim3(512), 0, s, d, b, n, ctr, xl);
             return;
         }
     }
-    if (var == 1) hipLaunchKernelGGL((k_contract_grouped<MODE, NORM, 2, 1>), grid, dim3(512), 0, s, d, b, n, ctr);
-    else hipLaunchKernelGGL((k_contract_grouped<MODE, NORM, 2, 0>), grid, dim3(512), 0, s, d, b, n, ctr);
+    if (var == 1) hipLaunchKernelGGL((k_contract_grouped<MODE, NORM, 2, 1>), grid, dim3(512), 0, s, d, b, n, ctr, xl);
+    else hipLaunchKernelGGL((k_contract_grouped<MODE, NORM, 2, 0>), grid, dim3(512), 0, s, d, b, n, ctr, xl);
 }
 
 bool amb_contract_v2_fits(int nact, long pitch)
@@ -1234,23 +1241,24 @@ static void launch_var(int var, dim3 grid, hipStream_t s, const ContractArgs &a,
 
 hipError_t amb_launch_contract_grouped(hipStream_t s, int prec, int norm, int var, const ContractArgs *d_descs,
                                        const long *d_begin, int ntri, long total, unsigned long long *d_counter,
-                                       int nblocks)
+                                       int nblocks, int xrun_log2)
 {
+    if (xrun_log2 < 0 || xrun_log2 > 12) return hipErrorInvalidValue;
     if (ntri <= 0 || total <= 0) return hipSuccess;
     if ((prec == AMB_PREC_FP8 || prec == AMB_PREC_FP16X3) && norm) return hipErrorInvalidValue;
     /* a kernel fill, not a memset, so the reset can live in a captured graph */
     hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, s, d_counter, 0ull, AMB_GROUPED_COUNTERS);
     if ((long)nblocks > total) nblocks = (int)total;
     dim3 grid(nblocks);
-    if (prec == AMB_PREC_FP8) launch_grouped_var<AMB_PREC_FP8, 0>(var, grid, s, d_descs, d_begin, ntri, d_counter);
+    if (prec == AMB_PREC_FP8) launch_grouped_var<AMB_PREC_FP8, 0>(var, grid, s, d_descs, d_begin, ntri, d_counter, xrun_log2);
     else if (prec == AMB_PREC_FP16X3)
-        launch_grouped_var<AMB_PREC_FP16X3, 0>(var, grid, s, d_descs, d_begin, ntri, d_counter);
+        launch_grouped_var<AMB_PREC_FP16X3, 0>(var, grid, s, d_descs, d_begin, ntri, d_counter, xrun_log2);
     else if (prec == AMB_PREC_FP32) {
-        if (norm) launch_grouped_var<AMB_PREC_FP32, 1>(var, grid, s, d_descs, d_begin, ntri, d_counter);
-        else launch_grouped_var<AMB_PREC_FP32, 0>(var, grid, s, d_descs, d_begin, ntri, d_counter);
+        if (norm) launch_grouped_var<AMB_PREC_FP32, 1>(var, grid, s, d_descs, d_begin, ntri, d_counter, xrun_log2);
+        else launch_grouped_var<AMB_PREC_FP32, 0>(var, grid, s, d_descs, d_begin, ntri, d_counter, xrun_log2);
     } else {
-        if (norm) launch_grouped_var<AMB_PREC_BF16X3, 1>(var, grid, s, d_descs, d_begin, ntri, d_counter);
-        else launch_grouped_var<AMB_PREC_BF16X3, 0>(var, grid, s, d_descs, d_begin, ntri, d_counter);
+        if (norm) launch_grouped_var<AMB_PREC_BF16X3, 1>(var, grid, s, d_descs, d_begin, ntri, d_counter, xrun_log2);
+        else launch_grouped_var<AMB_PREC_BF16X3, 0>(var, grid, s, d_descs, d_begin, ntri, d_counter, xrun_log2);
     }
     return hipGetLastError();
 }

@@ -106,7 +106,7 @@ struct TriJob {
 } // namespace amb
 
 struct amb_ctx {
-    int device = 0, prec = AMB_PREC_BF16X3, verbose = 0, sync_debug = 0, kvar = 2, no_sym = 0;
+    int device = 0, prec = AMB_PREC_BF16X3, verbose = 0, sync_debug = 0, no_sym = 0;
     int batch_mode = AMB_BATCH_GROUPED;
     int cert_strict = 0; /* uncertified normalisation sums: count (0) or fail (1) */
     uint64_t budget = 0, used = 0;
@@ -121,7 +121,13 @@ struct amb_ctx {
     std::map<int, float2 *> tw;
     std::map<int, FftPlan> plans;
     std::map<int, double2 *> tw64; /* FP64 twiddles (exact normalisation) */
-    int grouped_blocks = 0; /* persistent grid of the grouped contraction */
+    int grouped_blocks = 0; /* resident workgroups of the grouped contraction */
+    /* launch configurations by (prec, ng_log2, nact_log2); `forced` (when
+       has_forced) overrides them for every shape */
+    std::map<std::tuple<int, int, int>, amb_tune_entry> tune;
+    amb_tune_entry forced{};
+    bool has_forced = false;
+    std::string arch; /* gcnArchName of the device */
     int wave_max = 1024;    /* triangles per grouped wave */
     void *blas = nullptr;   /* rocBLAS handle of the post-processing products (first use) */
     void *na_scratch = nullptr; /* device address of the static scratch (allocation-free entry points) */
@@ -195,6 +201,14 @@ void post_release(amb_ctx *c);
 double build_fft_bytes(amb_ctx *c, const BuildOp &op);
 /* one triangle with its own launches, results in host memory */
 int bispec_core(amb_ctx *c, int id, int r1, int r2, int r3, int want_norm, amb_result *res);
+/* the launch configuration for a triangle of grid ng over nact planes at
+ * the context's precision: the forced one, else the table's bucket, else
+ * the nearest tuned bucket of that precision, else the defaults */
+amb_tune_entry tune_cfg(amb_ctx *c, long ng, int nact);
+/* kernel variant a launch over (nact, pitch) runs under configuration t */
+int tune_variant(const amb_tune_entry &t, int prec, int nact, long pitch);
+/* grouped waves of the requests (results into their buffers); done_ms may be null */
+int run_grouped(amb_ctx *c, const amb_tri_req *req, int n, double t0, double *done_ms);
 /* calCR; noalloc: all device scratch from the static array, no allocation */
 int closed_rings(amb_ctx *c, int id, int plane, double *rdata, int cap, int *rcount, bool noalloc);
 

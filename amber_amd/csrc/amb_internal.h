@@ -136,7 +136,7 @@ bool amb_contract_args_ok(int prec, int norm, const ContractArgs &a);
 #define AMB_GROUPED_COUNTERS 8
 hipError_t amb_launch_contract_grouped(hipStream_t s, int prec, int norm, int var, const ContractArgs *d_descs,
                                        const long *d_begin, int ntri, long total, unsigned long long *d_counter,
-                                       int nblocks);
+                                       int nblocks, int xrun_log2);
 hipError_t amb_launch_reduce_grouped(hipStream_t s, const ReduceArgs *d_ra, const long *d_cbegin, int ntri,
                                      long total_cells);
 /* workgroups of the persistent grouped contraction that are resident at

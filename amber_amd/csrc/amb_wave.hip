@@ -25,10 +25,6 @@ namespace {
 constexpr size_t ALIGN = 256;
 constexpr size_t OUT_CAP = (size_t)1 << 30;  /* result bytes per wave */
 constexpr int MAX_WAVE = 1024;               /* triangles per wave */
-/* contraction partials per wave (at most a quarter of the HBM budget).
-   Three times this size gave 0.3% fewer wave tails but 1.5 times the
-   contraction's L2-miss traffic (more triangles' planes in flight at once) */
-constexpr size_t WAVE_C_BYTES = (size_t)16 << 30;
 
 size_t al(size_t x) { return (x + ALIGN - 1) / ALIGN * ALIGN; }
 
@@ -159,6 +155,32 @@ std::vector<char> descriptors(const Wave &w, const WaveBufs &b)
     return h;
 }
 
+/* contraction partials (floats) of a wave that starts with triangle j:
+   the tuned wave size of j's shape, at most a quarter of the HBM budget */
+size_t wave_c_cap(amb_ctx *c, const TriJob &j)
+{
+    const amb_tune_entry t = tune_cfg(c, j.ng, j.nact);
+    return std::min<size_t>((size_t)t.wave_gib << 30, c->budget / 4) / sizeof(float);
+}
+
+/* the launch configuration of a wave: its costliest triangle's (the
+   largest items x pitch), one kernel variant every triangle supports */
+amb_tune_entry wave_cfg(amb_ctx *c, const Wave &w, int prec, int *var, int *blocks)
+{
+    size_t k = 0;
+    double best = -1;
+    for (size_t i = 0; i < w.jobs.size(); ++i) {
+        const double cost = (double)amb_contract_items(w.jobs[i].nact) * (double)w.jobs[i].pitch;
+        if (cost > best) { best = cost; k = i; }
+    }
+    const amb_tune_entry t = tune_cfg(c, w.jobs[k].ng, w.jobs[k].nact);
+    *var = t.variant;
+    for (const TriJob &j : w.jobs) *var = std::min(*var, tune_variant(t, prec, j.nact, j.pitch));
+    /* a multiple of the 8 XCDs, so that every work queue has takers */
+    *blocks = std::max(8, (int)((long)c->grouped_blocks * t.grid_pct / 100) / 8 * 8);
+    return t;
+}
+
 /* the launches of one wave on stream s (descriptors already on device);
    copy: the lag arrays' copy to b.host on s too (else the caller makes it) */
 hipError_t launch_wave(amb_ctx *c, hipStream_t s, const Wave &w, const WaveBufs &b, bool timed, bool copy = true)
@@ -167,21 +189,21 @@ hipError_t launch_wave(amb_ctx *c, hipStream_t s, const Wave &w, const WaveBufs 
     const int prec = c->prec, nprec = prec == AMB_PREC_FP32 ? AMB_PREC_FP32 : AMB_PREC_BF16X3;
     unsigned long long *ctr = (unsigned long long *)(b.desc + w.off_ctr);
     unsigned *mg = (unsigned *)(b.desc + w.off_mg);
-    int var = c->kvar;
-    for (const TriJob &j : w.jobs)
-        if (var == 2 && !amb_contract_v2_fits(j.nact, j.pitch)) var = 0;
+    int var, nvar, blocks;
+    const amb_tune_entry t = wave_cfg(c, w, prec, &var, &blocks);
+    wave_cfg(c, w, nprec, &nvar, &blocks);
     if (timed) timer_start(c, &c->st.ms_contract);
     hipError_t e = amb_launch_contract_grouped(s, prec, 0, var, (const ContractArgs *)(b.desc + w.off_da),
-                                               (const long *)(b.desc + w.off_b), n, w.items, ctr,
-                                               c->grouped_blocks);
+                                               (const long *)(b.desc + w.off_b), n, w.items, ctr, blocks,
+                                               t.xrun_log2);
     if (timed) timer_stop(c);
     if (e == hipSuccess && w.nnorm) {
         e = amb_launch_fill_u32(s, mg, w.nnorm, 0u);
         if (timed) timer_start(c, &c->st.ms_norm);
         if (e == hipSuccess)
-            e = amb_launch_contract_grouped(s, nprec, 1, var, (const ContractArgs *)(b.desc + w.off_na),
+            e = amb_launch_contract_grouped(s, nprec, 1, nvar, (const ContractArgs *)(b.desc + w.off_na),
                                             (const long *)(b.desc + w.off_nb), w.nnorm, w.nitems, ctr + AMB_GROUPED_COUNTERS,
-                                            c->grouped_blocks);
+                                            blocks, t.xrun_log2);
         if (timed) timer_stop(c);
     }
     if (timed) timer_start(c, &c->st.ms_reduce);
@@ -303,7 +325,7 @@ int redo_norms(amb_ctx *c, const amb_tri_req *req, const std::vector<int> &redo,
 /* all requests as grouped waves; done_ms[i] gets request i's completion
    time relative to t0 (may be null). No result copy is left running on
    return, also after an error. */
-int run_grouped(amb_ctx *c, const amb_tri_req *req, int n, double t0, double *done_ms)
+int grouped_run(amb_ctx *c, const amb_tri_req *req, int n, double t0, double *done_ms)
 {
     std::vector<int> redo;
     int rc = run_grouped_waves(c, req, n, t0, done_ms, &redo);
@@ -321,7 +343,6 @@ int run_grouped_waves(amb_ctx *c, const amb_tri_req *req, int n, double t0, doub
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
         return req[x].cube_id != req[y].cube_id ? req[x].cube_id < req[y].cube_id : req[x].r1 < req[y].r1;
     });
-    const size_t c_cap = std::min<size_t>(WAVE_C_BYTES, c->budget / 4) / sizeof(float);
     const PlaneSrc src{&c->planes};
     /* w: the wave being assembled; prev: a wave whose kernels finished and
        whose results are copied (ev_cp[hb ^ 1]) into host staging buffer hb ^ 1 */
@@ -360,7 +381,7 @@ int run_grouped_waves(amb_ctx *c, const amb_tri_req *req, int n, double t0, doub
         int rc = find_cube(c, q.cube_id, &cb);
         if (rc) { unpin_all(c); return rc; }
         const size_t nc3 = al((size_t)cb->nact() * cb->nact() * cb->nact() * sizeof(float)) / sizeof(float);
-        if (!w.jobs.empty() && (c_tot + nc3 > c_cap || (int)w.jobs.size() >= c->wave_max))
+        if (!w.jobs.empty() && (c_tot + nc3 > wave_c_cap(c, w.jobs[0]) || (int)w.jobs.size() >= c->wave_max))
             if ((rc = flush())) return rc;
         TriJob j;
         rc = prepare_tri(c, src, q.cube_id, q.r1, q.r2, q.r3, q.norm != nullptr, j);
@@ -386,6 +407,11 @@ int run_grouped_waves(amb_ctx *c, const amb_tri_req *req, int n, double t0, doub
 }
 
 } // namespace
+
+int amb::run_grouped(amb_ctx *c, const amb_tri_req *req, int n, double t0, double *done_ms)
+{
+    return grouped_run(c, req, n, t0, done_ms);
+}
 
 /* ------------------------------------------------------------------ graphs */
 struct AmbGraph {
@@ -426,10 +452,11 @@ static void graph_free(amb_ctx *c, AmbGraph *g)
 
 /* Split a fixed batch into waves by C size, plane bytes and count, by
    resolving every triangle against a measuring arena. */
-static int plan_graph_waves(amb_ctx *c, const amb_tri_req *req, const std::vector<int> &order, size_t c_cap,
-                            size_t p_cap, std::vector<std::vector<int>> *plan, size_t *arena)
+static int plan_graph_waves(amb_ctx *c, const amb_tri_req *req, const std::vector<int> &order, size_t p_cap,
+                            std::vector<std::vector<int>> *plan, size_t *arena)
 {
     std::vector<int> cur;
+    size_t c_cap = 0; /* the tuned wave size of the wave's first triangle */
     std::vector<PlaneSet> list;
     std::vector<BuildOp> rec;
     Bump mb;
@@ -459,6 +486,7 @@ static int plan_graph_waves(amb_ctx *c, const amb_tri_req *req, const std::vecto
         if (b2.off > p_cap)
             return fail(c, AMB_EBUDGET, "triangle (%d,%d,%d): planes of %zu bytes exceed the graph arena cap %zu",
                         q.r1, q.r2, q.r3, b2.off, p_cap);
+        if (cur.empty()) c_cap = wave_c_cap(c, j);
         cur.push_back(i);
         list.swap(l2);
         rec.swap(r2);
@@ -544,11 +572,10 @@ int amb_graph_create(amb_ctx *c, const amb_tri_req *req, int n, amb_graph **out)
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
         return req[x].cube_id != req[y].cube_id ? req[x].cube_id < req[y].cube_id : req[x].r1 < req[y].r1;
     });
-    const size_t c_cap = std::min<size_t>(WAVE_C_BYTES, c->budget / 4) / sizeof(float);
     const size_t p_cap = std::min<size_t>((size_t)96 << 30, c->budget / 2);
     std::vector<std::vector<int>> plan;
     size_t arena = 0;
-    int rc = plan_graph_waves(c, req, order, c_cap, p_cap, &plan, &arena);
+    int rc = plan_graph_waves(c, req, order, p_cap, &plan, &arena);
     if (!rc) rc = grow(c, &g->arena, &g->arena_bytes, std::max<size_t>(arena, 256), "graph plane arena");
     /* resolve each wave against the real arena, recording its builds */
     for (size_t k = 0; k < plan.size() && !rc; ++k) {

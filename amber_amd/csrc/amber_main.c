@@ -43,7 +43,7 @@ static void usage(void)
             "Usage: %s <Rlzn> [--params FILE] [--rings FILE] [--nr N] [--nc10 N] [--uf-nc N]\n"
             "          [--rp-start N] [--rp-end N] [--precision fp32|bf16x3|fp16x3] [--device N]\n"
             "          [--calcr-plane N] [--res-path DIR] [--wire legacy|hardened|both]\n"
-            "          [--cert exact|strict]\n",
+            "          [--cert exact|strict] [--tune FILE]\n",
             PROG);
 }
 
@@ -88,6 +88,7 @@ int main(int argc, char **argv)
     int prec = AMB_PREC_BF16X3;
     int wire_legacy = 1, wire_hardened = 0; /* --wire: which result files to write */
     int cert_strict = 0; /* --cert: uncertified fast v2 normalisation sums recomputed exactly, or an error */
+    const char *tune_file = NULL; /* --tune: launch-configuration table, loaded, or tuned and saved */
     for (int i = 2; i < argc; ++i) {
         const char *a = argv[i];
         const char *v = i + 1 < argc ? argv[i + 1] : NULL;
@@ -112,6 +113,8 @@ int main(int argc, char **argv)
             else if (!strcmp(v, "hardened")) { wire_legacy = 0; wire_hardened = 1; }
             else if (!strcmp(v, "both")) { wire_legacy = 1; wire_hardened = 1; }
             else { usage(); return 1; }
+        } else if (!strcmp(a, "--tune")) {
+            tune_file = v;
         } else if (!strcmp(a, "--cert")) {
             if (!strcmp(v, "exact") || !strcmp(v, "count")) cert_strict = 0;
             else if (!strcmp(v, "strict")) cert_strict = 1;
@@ -199,6 +202,33 @@ int main(int argc, char **argv)
 #endif
     int *tri = malloc(sizeof(int) * 3 * blk);
     if (rp_end < 0 || rp_end > rcount) rp_end = rcount;
+    if (tune_file) {
+        FILE *tf = fopen(tune_file, "r");
+        if (tf) {
+            fclose(tf);
+            CHECK(amb_tune_load(ctx, tune_file));
+            printf("Launch configurations loaded from %s\n", tune_file);
+        } else if (rp_end > rp_start) {
+            /* tuned on up to 16 of this run's triangles, evenly spaced */
+            const int ns = rp_end - rp_start < 16 ? rp_end - rp_start : 16;
+            amb_tri_req *rq = calloc(ns, sizeof *rq);
+            float *sb = malloc(sizeof(float) * ncsize * ns);
+            if (!rq || !sb) { fprintf(stderr, "%s: out of host memory\n", PROG); return 1; }
+            for (int k = 0; k < ns; ++k) {
+                const int rp = rp_start + (int)((long)k * (rp_end - rp_start) / ns);
+                rq[k].cube_id = cube;
+                rq[k].r1 = (int)rdata[rp * 4];
+                rq[k].r2 = (int)rdata[rp * 4 + 1];
+                rq[k].r3 = (int)rdata[rp * 4 + 2];
+                rq[k].bl = sb + (size_t)k * ncsize;
+            }
+            const double t0 = now();
+            CHECK(amb_autotune(ctx, rq, ns, 120.0, tune_file));
+            printf("Launch configurations tuned on %d triangles in %.1f s, saved to %s\n", ns, now() - t0, tune_file);
+            free(rq);
+            free(sb);
+        }
+    }
     for (int rp0 = rp_start; rp0 < rp_end; rp0 += blk) {
         const int nb = rp_end - rp0 < blk ? rp_end - rp0 : blk;
         for (int k = 0; k < nb; ++k)

@@ -42,6 +42,9 @@ CONFIGS = {
 PEAK_TFLOPS = {"bf16x3": 2500.0, "fp16x3": 2500.0, "fp32": 157.3, "fp8": 5000.0}
 PASSES = {"bf16x3": 3, "fp16x3": 3, "fp32": 1, "fp8": 1}
 HBM_PEAK_GBS = 8000.0
+# launch-configuration table tuned on this workload on an MI355X (loaded
+# by default when present)
+TUNE_TABLE = os.path.join(ROOT, "amber_amd", "tune_gfx950.txt")
 # HBM bytes of all contraction launches of one step of the default workload
 # (grouped waves, pipelined BF16x3 form): FETCH_SIZE x 2.00 + WRITE_SIZE x
 # 1.00 summed over the step's dispatches from two PMC runs of this workload,
@@ -53,7 +56,7 @@ MEASURED_TRAFFIC_STEP = {("cfg3", "bf16x3", "grouped"): 1.909e13}
 
 def traffic_per_launch(args, st, nsteps):
     tot = MEASURED_TRAFFIC_STEP.get((args.config, args.precision, args.batch_mode))
-    if tot is None or args.tri_limit or os.environ.get("AMB_KVAR", "2") != "2":
+    if tot is None or args.tri_limit:
         return None
     # the measured total covers one pass over every triangle, as the timed steps do
     return round(tot / max(st["n_contract_launch"], 1), -6)
@@ -109,6 +112,11 @@ def main():
     ap.add_argument("--batch-mode", default="grouped", choices=["grouped", "per_triangle"])
     ap.add_argument("--estimator", default="v1", choices=["v1", "v2"],
                     help="v1: amber (data contraction); v2: amber_v2 (data plus the normalisation contraction)")
+    ap.add_argument("--tune", default=None, metavar="PATH",
+                    help="launch-configuration table: loaded when PATH exists, else tuned on a sample of the "
+                         "workload and saved there; 'none': the built-in defaults; default: the committed "
+                         "table when present, else the defaults")
+    ap.add_argument("--tune-seconds", type=float, default=150.0, help="bound of a tuning run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -161,6 +169,11 @@ def main():
 
     W = 2 * nc10 + 1
     costs = tri_costs(A, tris, hi)
+    # the sample a missing table is tuned on: up to 24 triangles evenly
+    # spread over the list ordered by cost, so every Ng bucket is present
+    by_cost = np.argsort(costs, kind="stable")
+    sample = [(cube, *map(int, tris[k])) for k in by_cost[np.linspace(0, len(tris) - 1, min(24, len(tris))).astype(int)]]
+    tuning = use_tuning(ctx, args, sample)
     slices = cost_slices(costs, max(args.steps, 1))
     nmax = max(len(sl) for sl in slices)
     out = (np.zeros((nmax, W, W), np.float32), np.zeros((nmax, W, W), np.int32),
@@ -265,6 +278,7 @@ def main():
                                         "(the triangles of one grouped wave complete together); request latency "
                                         "under load is --config cfg5"},
         "rel_err_vs_fp64": err, "cpu_baseline": cpu,
+        "launch_config": tuning,
         "upload_s_pcie_incl": round(upload_s, 3),
         "run_s": round(time.time() - t_run, 1),
     }
@@ -319,51 +333,108 @@ def mixed_workload(A, ctx, shapes, ntri, seed, max_r_frac=1.0 / 3.0):
 
 def bench_cfg4(A, args):
     """200 uneven triangles over 9 cubes {256,512,1024}^2 x {128,384,768}:
-    per-triangle launches vs grouped waves vs one captured graph."""
+    per-triangle launches of the plain form against grouped waves and one
+    captured graph, each under the built-in launch configuration and under
+    the table the autotuner makes from a sample of the mix."""
     shapes = [(N, nc) for N in (256, 512, 1024) for nc in (128, 384, 768)]
     ctx = A.Context(device=0, precision=args.precision)
     t0 = time.time()
     cubes, reqs, lo, hi, _ = mixed_workload(A, ctx, shapes, 200, 4001)
     log("cfg4: 9 cubes resident, 200 triangles, %d rings, setup %.1f s" % (len(lo), time.time() - t0))
-    res = {}
 
-    def timed(label, fn, reps):
-        ts = []
+    def timed(label, fn, reps, keys=("ms_contract", "ms_norm")):
+        ts, cs = [], []
         for _ in range(reps):
             ctx.reset_stats()
             t = time.time()
             out = fn()
             ts.append(time.time() - t)
-        st = ctx.stats()
-        log("cfg4 %s: %s s" % (label, ", ".join("%.2f" % x for x in ts)))
-        return out, min(ts), st
+            st = ctx.stats()
+            cs.append(sum(st[k] for k in keys))
+        log("cfg4 %s: %s s (device %s ms)" % (label, ", ".join("%.2f" % x for x in ts),
+                                              ", ".join("%.0f" % x for x in cs)))
+        return out, min(ts), st, min(cs)
 
-    ctx.set_batch_mode("grouped")
-    reps = 1 + min(max(args.steps, 1), 2)
-    ref, t_grp, st = timed("grouped", lambda: [drop_all(ctx, cubes), ctx.bispec_multi(reqs)][1], reps)
-    evals = st["evals"]
+    def same(a, b):
+        return all(np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1]) for x, y in zip(a, b))
+
+    def eager():
+        drop_all(ctx, cubes)
+        return ctx.bispec_multi(reqs)
+
+    def graph_run(label, reps):
+        t = time.time()
+        g = ctx.graph(reqs)
+        cap = time.time() - t
+        out, tg, _, cg = timed(label, g.run, reps, keys=("ms_graph",))
+        g.close()
+        return out, tg, cg, cap
+
+    reps = 2
+    ctx.tune_clear()
     ctx.set_batch_mode("per_triangle")
-    per, t_per, _ = timed("per_triangle", lambda: [drop_all(ctx, cubes), ctx.bispec_multi(reqs)][1], 1)
-    same_per = all(np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) for a, b in zip(ref, per))
+    per, t_per, st, c_per = timed("per-triangle launches", eager, 1)
+    evals = st["evals"]
     ctx.set_batch_mode("grouped")
+    fg, t_fg, _, c_fg = timed("grouped, fixed configuration", eager, reps)
+    fgr, t_fgr, c_fgr, cap_f = graph_run("graph, fixed configuration", reps)
+    # the autotuner on every fifth triangle of the mix
+    sample = reqs[::5]
     t = time.time()
-    g = ctx.graph(reqs)
-    t_cap = time.time() - t
-    gout, t_graph, _ = timed("graph replay", g.run, reps)
-    same_graph = all(np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) for a, b in zip(ref, gout))
-    g.close()
+    table, _ = ctx.autotune(sample, max_seconds=args.tune_seconds,
+                            path=None if args.tune in (None, "none") else args.tune)
+    t_tune = time.time() - t
+    log("cfg4: tuned %d buckets on %d triangles in %.1f s" % (len(table), len(sample), t_tune))
+    tg, t_tg, _, c_tg = timed("grouped, tuned", eager, reps)
+    tgr, t_tgr, c_tgr, cap_t = graph_run("graph, tuned", reps)
+
+    def rate(s_):
+        return {"s": round(s_, 3), "evals_per_s": round(evals / s_, 1)}
+
     line = {"metric": "cfg4 mixed-batch evals/sec (200 uneven triangles, 9 cubes)", "unit": "evals/s",
-            "precision": args.precision, "evals": int(evals),
-            "per_triangle": {"s": round(t_per, 3), "evals_per_s": round(evals / t_per, 1)},
-            "grouped": {"s": round(t_grp, 3), "evals_per_s": round(evals / t_grp, 1)},
-            "graph": {"s": round(t_graph, 3), "evals_per_s": round(evals / t_graph, 1),
-                      "capture_s": round(t_cap, 3)},
-            "grouped_speedup": round(t_per / t_grp, 3), "graph_speedup": round(t_per / t_graph, 3),
-            "bit_identical": {"per_triangle": same_per, "graph": same_graph},
-            "note": "each timed pass rebuilds every filtered plane (eager: planes dropped first; graph: "
-                    "the plane builds are nodes of the graph)"}
+            "value": round(evals / t_tgr, 1), "precision": args.precision, "evals": int(evals),
+            "per_triangle": dict(rate(t_per), contraction_ms=round(c_per, 1)),
+            "fixed": {"grouped": dict(rate(t_fg), contraction_ms=round(c_fg, 1)),
+                      "graph": dict(rate(t_fgr), device_ms=round(c_fgr, 1), capture_s=round(cap_f, 3))},
+            "tuned": {"grouped": dict(rate(t_tg), contraction_ms=round(c_tg, 1)),
+                      "graph": dict(rate(t_tgr), device_ms=round(c_tgr, 1), capture_s=round(cap_t, 3)),
+                      "tune_s": round(t_tune, 1), "sample_triangles": len(sample), "table": table},
+            "speedup_vs_per_triangle": {"fixed_grouped": round(t_per / t_fg, 3), "fixed_graph": round(t_per / t_fgr, 3),
+                                        "tuned_grouped": round(t_per / t_tg, 3), "tuned_graph": round(t_per / t_tgr, 3)},
+            "device_speedup_tuned_vs_fixed": {"grouped_contraction": round(c_fg / c_tg, 3),
+                                              "graph_replay": round(c_fgr / c_tgr, 3)},
+            "bit_identical_to_per_triangle": {"fixed_grouped": same(per, fg), "fixed_graph": same(per, fgr),
+                                              "tuned_grouped": same(per, tg), "tuned_graph": same(per, tgr)},
+            "note": "each timed pass rebuilds every filtered plane (eager: planes dropped first; graph: the plane "
+                    "builds are nodes of the graph); value is the tuned graph; contraction_ms: device time of the "
+                    "contraction kernels; device_ms of a graph: the whole replay (builds included)"}
     ctx.close()
     print(json.dumps(line), flush=True)
+
+
+def use_tuning(ctx, args, sample):
+    """Installs the launch-configuration table: --tune PATH is loaded when it
+    exists (a table of another device is reported and not used), else the
+    sample is tuned and saved there.  Returns what was done, for the
+    result line."""
+    path = args.tune
+    if path is None and os.path.exists(TUNE_TABLE):
+        path = TUNE_TABLE
+    if not path or path == "none":
+        return {"table": None, "note": "built-in defaults"}
+    if os.path.exists(path):
+        try:
+            ctx.tune_load(path)
+            return {"table": os.path.relpath(path, ROOT) if path.startswith(ROOT) else path,
+                    "entries": len(ctx.tune_table()), "tuned_now": False}
+        except Exception as e:  # a table of another device: the defaults
+            log("tuning table %s not used: %s" % (path, e))
+            return {"table": None, "note": "table not used: %s" % e}
+    t = time.time()
+    table, _ = ctx.autotune(sample, max_seconds=args.tune_seconds, path=path)
+    log("tuned %d buckets on %d triangles in %.1f s -> %s" % (len(table), len(sample), time.time() - t, path))
+    return {"table": path, "entries": len(table), "tuned_now": True, "tune_s": round(time.time() - t, 1),
+            "sample_triangles": len(sample)}
 
 
 def drop_all(ctx, cubes):

@@ -266,6 +266,49 @@ int amb_bispec_multi(amb_ctx *ctx, const amb_tri_req *req, int n);
 #define AMB_BATCH_GROUPED 1
 int amb_set_batch_mode(amb_ctx *ctx, int mode);
 
+/* Launch configuration of the contraction for one shape bucket, chosen by
+ * the autotuner.  A bucket is (precision, ceil(log2 Ng), ceil(log2 nact)).
+ * Every configuration computes each triangle with the same arithmetic in
+ * the same order, so results do not depend on the table: it only moves
+ * time.  The reference has no counterpart (its triangle loop,
+ * amber.c:187-326, has nothing to tune); this table replaces that loop's
+ * fixed schedule. */
+typedef struct amb_tune_entry {
+    int prec;      /* AMB_PREC_* */
+    int ng_log2;   /* bucket: ceil(log2 Ng) */
+    int nact_log2; /* bucket: ceil(log2 active channels) */
+    int variant;   /* contraction kernel: 0 plain k-steps, 1 interleaved
+                      A formation, 2 direct-to-LDS pipelined (split forms) */
+    int xrun_log2; /* work items per run of an XCD work queue, log2 (0..12) */
+    int wave_gib;  /* contraction partials per grouped wave, GiB (1..256) */
+    int grid_pct;  /* persistent grid, percent of the resident workgroups */
+    double tflops; /* issued MFMA TFLOP/s of the data contraction measured
+                      with this entry (0: not measured) */
+} amb_tune_entry;
+
+/* Times candidate configurations of each shape bucket present in req (a
+ * sample of the caller's workload; the requests' outputs receive their
+ * results) by coordinate descent over variant, queue run, wave size and
+ * grid, on resident planes, and installs the fastest per bucket.  Every
+ * candidate's results are compared with the first run's bit for bit (a
+ * difference is an AMB_EHIP error).  max_seconds bounds the search (the
+ * best so far is kept).  path (may be NULL): the table is also saved
+ * there.  Buckets not tuned fall back to the nearest tuned bucket of the
+ * same precision, then to the built-in defaults. */
+int amb_autotune(amb_ctx *ctx, const amb_tri_req *req, int n, double max_seconds, const char *path);
+/* text table: a header line naming the device architecture and CU count
+ * (a table from another device is an AMB_EFORMAT error), one entry a line */
+int amb_tune_save(amb_ctx *ctx, const char *path);
+int amb_tune_load(amb_ctx *ctx, const char *path);
+int amb_tune_clear(amb_ctx *ctx);
+/* installed entries, up to cap; *n gets the count */
+int amb_tune_entries(amb_ctx *ctx, amb_tune_entry *out, int cap, int *n);
+/* the configuration the context would use for (Ng, nact) at its precision */
+int amb_tune_lookup(amb_ctx *ctx, int ng, int nact, amb_tune_entry *out);
+/* one configuration for every shape, overriding the table (the prec and
+ * bucket fields are ignored); NULL returns to the table */
+int amb_set_launch_config(amb_ctx *ctx, const amb_tune_entry *cfg);
+
 /* Host wall time (ms) from the start of the last amb_bispec_batch /
  * amb_bispec_multi call to each triangle's results being in host memory,
  * in the order the triangles were given (a triangle of a grouped wave

@@ -148,3 +148,25 @@ def test_v2_certificate_modes(tmp_path):
                         str(nc), "--cert", "strict"], cwd=tmp_path, stdout=subprocess.PIPE,
                        stderr=subprocess.STDOUT, text=True, timeout=120)
     assert r.returncode != 0 and "1/8 grid" in r.stdout, r.stdout[-2000:]
+
+
+def test_tune_option_tunes_then_loads(tmp_path):
+    """--tune FILE: the first run tunes on a sample of its triangles and saves
+    the table, the second loads it; both write the same result files as a
+    run without a table (no launch configuration changes a result)."""
+    N, nc, nflag, nc10 = 64, 10, 2, 6
+    lo, hi = A.log_rings(2, 20, 5)
+    _layout(tmp_path, N, nc, nflag, 33, lo, hi)
+    print("case driver --tune N=%d nc=%d" % (N, nc), flush=True)
+    _run(tmp_path, "amber", "--nc10", str(nc10), "--uf-nc", str(nc - nflag))
+    n, rd = A.read_rings(str(tmp_path / "results/v1"))
+    names = ["bsout_final_ra%d_rb%d_rc%d_R1" % tuple(int(x) for x in r[:3]) for r in rd]
+    plain = {f: (tmp_path / "results/v1" / f).read_bytes() for f in names}
+    table = tmp_path / "tune.txt"
+    out = _run(tmp_path, "amber", "--nc10", str(nc10), "--uf-nc", str(nc - nflag), "--tune", str(table))
+    assert "tuned on" in out and table.exists()
+    assert table.read_text().startswith("amber_mi355x-tune 1 ")
+    assert all((tmp_path / "results/v1" / f).read_bytes() == plain[f] for f in names)
+    out = _run(tmp_path, "amber", "--nc10", str(nc10), "--uf-nc", str(nc - nflag), "--tune", str(table))
+    assert "loaded from" in out
+    assert all((tmp_path / "results/v1" / f).read_bytes() == plain[f] for f in names)

@@ -266,36 +266,39 @@ def test_ring_image_vs_oracle(ctx, N, nc, lo, hi, ng):
     (64, 200, 7, 199, (2, 20, 5)),    # partial 128-row tiles, flags, one segment
     (256, 40, 0, 12, (20, 60, 3)),    # lag windows, Ng up to 186: several 8192-pixel segments
 ])
-def test_pipelined_variant_is_bit_identical(ctx, N, nc, nflag, nc10, rings, prec, monkeypatch):
-    """The split-form kernels with direct global->LDS loads and pipelined
-    k-steps (variant 2) run the same arithmetic in the same order as
-    variant 0: their results, data and v2 normalisation, per triangle and
-    in grouped waves, must match bit for bit."""
+def test_launch_configurations_are_bit_identical(ctx, N, nc, nflag, nc10, rings, prec):
+    """Every launch configuration the autotuner may pick runs the same
+    arithmetic in the same order: the split-form kernels with direct
+    global->LDS loads and pipelined k-steps (variant 2), the interleaved
+    variant 1 and the plain variant 0, under any XCD-queue run length,
+    wave size and grid, give the same results, data and v2 normalisation,
+    per triangle and in grouped waves, bit for bit."""
     lo, hi = A.log_rings(*rings)
     gv, samp, uf = A.generate(N, nc, nflag=nflag, seed=61)
     fk = 1 if nflag else 0
-    ctxs, cubes = [], []
-    for var in ("0", "2"):
-        monkeypatch.setenv("AMB_KVAR", var)
-        c = A.Context(device=0, precision=prec)
-        cb = c.load_cube(N, nc, 1.0, fk, gv, samp)
-        c.set_channels(cb, uf if fk else None, nc10)
-        c.set_rings(lo, hi)
-        ctxs.append(c)
-        cubes.append(cb)
-    monkeypatch.delenv("AMB_KVAR")
+    ctx.set_precision(prec)
+    cb = ctx.load_cube(N, nc, 1.0, fk, gv, samp)
+    ctx.set_channels(cb, uf if fk else None, nc10)
+    ctx.set_rings(lo, hi)
     tris = A.triangles(lo, hi)
-    print("case pipelined variant %s N=%d nc=%d ntri=%d" % (prec, N, nc, len(tris)), flush=True)
-    for mode in ("per_triangle", "grouped"):
-        outs = []
-        for c, cb in zip(ctxs, cubes):
-            c.set_batch_mode(mode)
-            outs.append(c.bispec_batch(cb, tris, want_norm=True))
-        assert np.array_equal(outs[0][1], outs[1][1])
-        assert np.array_equal(outs[0][0], outs[1][0]), mode
-        assert np.array_equal(outs[0][2], outs[1][2]), mode
-    for c in ctxs:
-        c.close()
+    print("case launch configurations %s N=%d nc=%d ntri=%d" % (prec, N, nc, len(tris)), flush=True)
+    configs = [dict(variant=2), dict(variant=0), dict(variant=1, xrun_log2=3, grid_pct=50),
+               dict(variant=2, xrun_log2=8, wave_gib=1, grid_pct=75)]
+    try:
+        for mode in ("per_triangle", "grouped"):
+            ctx.set_batch_mode(mode)
+            outs = []
+            for cfg in configs:
+                ctx.set_launch_config(**cfg)
+                outs.append(ctx.bispec_batch(cb, tris, want_norm=True))
+            for cfg, o in zip(configs[1:], outs[1:]):
+                for k in range(3):
+                    assert np.array_equal(outs[0][k], o[k]), (mode, cfg, k)
+    finally:
+        ctx.set_launch_config(None)
+        ctx.set_batch_mode("grouped")
+        ctx.set_precision("bf16x3")
+        ctx.release_cube(cb)
 
 
 @pytest.mark.parametrize("prec", ["bf16x3", "fp32"])
