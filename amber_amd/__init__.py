@@ -33,7 +33,8 @@ EXPORTS = [
     "amb_closed_rings", "amb_bispec_v1", "amb_bispec_v2", "amb_bispec_ex", "amb_bispec_batch",
     "amb_get_stats", "amb_reset_stats", "amb_drop_planes", "amb_batch_times", "amb_ring_image",
     "amb_bispec_multi", "amb_set_batch_mode", "amb_update_cube", "amb_graph_create", "amb_graph_run",
-    "amb_graph_destroy", "amb_serve_start", "amb_submit", "amb_poll", "amb_wait", "amb_serve_stop",
+    "amb_graph_destroy", "amb_serve_start", "amb_submit", "amb_submit_cube", "amb_poll", "amb_wait",
+    "amb_serve_stop",
     "amb_cyl_bispec", "amb_maps_cldnu", "amb_write_cldnu", "amb_read_params", "amb_write_bsw", "amb_read_bsw",
     "amb_read_bsout", "amb_crc32", "amb_read_vcg_inputs", "amb_grid_setup", "amb_grid_cells", "amb_grid",
     "amb_grid_cube", "amb_set_cert_strict", "amb_closed_rings_noalloc", "amb_autotune", "amb_tune_save",
@@ -76,6 +77,12 @@ class TuneEntry(C.Structure):
         return d
 
 
+class CubeSrc(C.Structure):
+    _fields_ = [("N", C.c_int64), ("nc", C.c_int64), ("Du", C.c_double), ("flag_key", C.c_int32),
+                ("gv", C.POINTER(C.c_double)), ("samp", C.POINTER(C.c_double)), ("chans", C.POINTER(C.c_int)),
+                ("uf_nc", C.c_int), ("nc10", C.c_int)]
+
+
 class CylParams(C.Structure):
     _fields_ = [("ncut", C.c_int), ("mabs_norm", C.c_double), ("rc", C.c_double), ("rc_prime", C.c_double),
                 ("dnu", C.c_double), ("window", C.c_int)]
@@ -104,7 +111,8 @@ class Stats(C.Structure):
                 ("n_maps_launch", C.c_int64), ("ms_grid", C.c_double), ("n_grid_groups", C.c_int64),
                 ("ms_post", C.c_double), ("flop_post", C.c_double), ("cert_margin_max", C.c_double),
                 ("n_uncertified", C.c_int64), ("flop_contract_distinct", C.c_double),
-                ("n_norm_exact", C.c_int64)]
+                ("n_norm_exact", C.c_int64), ("n_evict_planes", C.c_int64), ("n_evict_cubes", C.c_int64),
+                ("n_cube_uploads", C.c_int64), ("n_serve_iters", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -150,6 +158,8 @@ _lib.amb_graph_destroy.argtypes = [C.c_void_p, C.c_void_p]
 _lib.amb_graph_destroy.restype = None
 _lib.amb_serve_start.argtypes = [C.c_void_p, C.c_int]
 _lib.amb_submit.argtypes = [C.c_void_p, C.POINTER(TriReq), C.c_int, C.POINTER(C.c_uint64)]
+_lib.amb_submit_cube.argtypes = [C.c_void_p, C.POINTER(CubeSrc), C.POINTER(TriReq), C.c_int, _ip,
+                                 C.POINTER(C.c_uint64)]
 _lib.amb_poll.argtypes = [C.c_void_p, C.c_uint64, _ip]
 _lib.amb_wait.argtypes = [C.c_void_p, C.c_uint64, _dp]
 _lib.amb_serve_stop.argtypes = [C.c_void_p]
@@ -606,13 +616,14 @@ class Context:
         samp = np.ascontiguousarray(samp, np.float64)
         self._c(_lib.amb_update_cube(self._h, cube, _p(gv, _dp), _p(samp, _dp)))
 
-    def _reqs(self, reqs, want_cnt=True, want_norm=False):
-        """reqs: (cube, r1, r2, r3) tuples -> TriReq array plus owned output arrays"""
+    def _reqs(self, reqs, want_cnt=True, want_norm=False, W0=None):
+        """reqs: (cube, r1, r2, r3) tuples -> TriReq array plus owned output
+        arrays (cube 0: the carried cube of a submission, of side W0)"""
         n = len(reqs)
         arr = (TriReq * max(n, 1))()
         outs = []
         for i, (cube, r1, r2, r3) in enumerate(reqs):
-            W = self._W(cube)
+            W = W0 if cube == 0 else self._W(cube)
             bl = np.zeros((W, W), np.float32)
             cnt = np.zeros((W, W), np.int32) if want_cnt else None
             norm = np.zeros((W, W), np.float32) if want_norm else None
@@ -631,9 +642,10 @@ class Context:
         """A captured, replayable batch (amb_graph_create)."""
         return Graph(self, reqs, want_cnt, want_norm)
 
-    def serve_start(self, max_wave=64):
+    def serve_start(self, max_active=256):
+        """Starts the serving queue; max_active bounds the triangles admitted at once."""
         self._tickets = {}
-        self._c(_lib.amb_serve_start(self._h, int(max_wave)))
+        self._c(_lib.amb_serve_start(self._h, int(max_active)))
 
     def submit(self, reqs, want_cnt=True, want_norm=False):
         """Queue a request of triangles; returns a ticket.  The outputs are
@@ -644,6 +656,25 @@ class Context:
         self._tickets[t.value] = (arr, outs)
         return t.value
 
+    def submit_cube(self, N, nc, Du, flag_key, gv, samp, reqs, uf=None, nc10=None, want_cnt=True, want_norm=False):
+        """Queue a request that carries its cube (amb_submit_cube): reqs are
+        (r1, r2, r3) on the carried cube or (cube, r1, r2, r3) on resident
+        ones (cube 0: the carried one).  Returns (ticket, cube id); the cube
+        stays resident under that id (evictable once idle)."""
+        gv = np.ascontiguousarray(gv, np.float64)
+        samp = np.ascontiguousarray(samp, np.float64)
+        nc10 = nc - 1 if nc10 is None else int(nc10)
+        ch = None if uf is None else np.ascontiguousarray(uf, np.int32)
+        src = CubeSrc(int(N), int(nc), float(Du), int(flag_key), _p(gv, _dp), _p(samp, _dp),
+                      _p(ch, _ip) if ch is not None else None, 0 if ch is None else len(ch), nc10)
+        reqs = [(0, *r) if len(r) == 3 else tuple(r) for r in reqs]
+        arr, outs = self._reqs(reqs, want_cnt, want_norm, W0=2 * nc10 + 1)
+        cid, t = C.c_int(), C.c_uint64()
+        self._c(_lib.amb_submit_cube(self._h, C.byref(src), arr, len(reqs), C.byref(cid), C.byref(t)))
+        self._meta[cid.value] = {"N": int(N), "nc": int(nc), "nc10": nc10}
+        self._tickets[t.value] = (arr, outs, gv, samp, ch, src)
+        return t.value, cid.value
+
     def poll(self, ticket):
         d = C.c_int()
         self._c(_lib.amb_poll(self._h, C.c_uint64(ticket), C.byref(d)))
@@ -653,7 +684,7 @@ class Context:
         """-> (outputs, latency_ms)"""
         lat = C.c_double()
         rc = _lib.amb_wait(self._h, C.c_uint64(ticket), C.byref(lat))
-        arr, outs = self._tickets.pop(ticket)
+        outs = self._tickets.pop(ticket)[1]
         self._c(rc)
         return outs, lat.value
 

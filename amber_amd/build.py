@@ -17,7 +17,7 @@ LIB = os.path.join(HERE, "libamber_mi355x.so")
 BIN = os.path.join(HERE, "bin")
 
 HIP_SOURCES = ["amb_fft.hip", "amb_contract.hip", "amb_api.hip", "amb_wave.hip", "amb_post.hip", "amb_maps.hip",
-               "amb_grid.hip", "amb_tune.hip"]
+               "amb_grid.hip", "amb_tune.hip", "amb_serve.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # packed-FP32 instructions are kept out of every HIP object
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", "-Wno-unused-value",

@@ -99,10 +99,16 @@ void timer_stop(amb_ctx *c)
         }
 }
 
-/* after a stream sync: fold recorded intervals into the stats */
+/* fold recorded intervals whose end has passed into the stats (after a
+   stream sync: all of them); intervals still running stay recorded */
 void timers_collect(amb_ctx *c)
 {
+    std::vector<Timer> keep;
     for (auto &t : c->timers) {
+        if (t.b && hipEventQuery(t.b) == hipErrorNotReady) {
+            keep.push_back(t);
+            continue;
+        }
         if (t.a && t.b) {
             float ms = 0.f;
             if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess && t.acc) *t.acc += ms;
@@ -110,7 +116,8 @@ void timers_collect(amb_ctx *c)
         if (t.a) c->evpool.push_back(t.a);
         if (t.b) c->evpool.push_back(t.b);
     }
-    c->timers.clear();
+    (void)hipGetLastError();
+    c->timers.swap(keep);
 }
 
 void dev_free(amb_ctx *c, void *p, size_t bytes)
@@ -126,10 +133,13 @@ static bool evict_one(amb_ctx *c)
 {
     int best = -1;
     for (size_t i = 0; i < c->planes.size(); ++i)
-        if (c->planes[i].pins == 0 && (best < 0 || c->planes[i].last < c->planes[best].last)) best = (int)i;
+        if (c->planes[i].pins == 0 && c->planes[i].spins == 0 &&
+            (best < 0 || c->planes[i].last < c->planes[best].last))
+            best = (int)i;
     if (best < 0) return false;
-    dev_free(c, c->planes[best].d, c->planes[best].bytes);
+    plane_free(c, c->planes[best].d, c->planes[best].bytes);
     c->planes.erase(c->planes.begin() + best);
+    c->st.n_evict_planes++;
     return true;
 }
 
@@ -142,8 +152,18 @@ int dev_alloc(amb_ctx *c, void **p, size_t bytes, const char *what)
             return fail(c, AMB_EBUDGET, "%s: %zu bytes exceed the HBM budget (%llu used of %llu)", what, bytes,
                         (unsigned long long)c->used, (unsigned long long)c->budget);
     hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess && c->ppool) { /* freed planes the pool still holds go back to the device */
+        (void)hipGetLastError();
+        hipStreamSynchronize(c->stream);
+        hipMemPoolTrimTo(c->ppool, 0);
+        e = hipMalloc(p, bytes);
+    }
     while (e != hipSuccess && evict_one(c)) {
         (void)hipGetLastError();
+        if (c->ppool) {
+            hipStreamSynchronize(c->stream);
+            hipMemPoolTrimTo(c->ppool, 0);
+        }
         e = hipMalloc(p, bytes);
     }
     if (e != hipSuccess) {
@@ -153,6 +173,40 @@ int dev_alloc(amb_ctx *c, void **p, size_t bytes, const char *what)
     }
     c->used += bytes;
     return AMB_OK;
+}
+
+int plane_alloc(amb_ctx *c, void **p, size_t bytes)
+{
+    if (!c->ppool) return dev_alloc(c, p, bytes, "filtered planes");
+    *p = nullptr;
+    if (bytes == 0) return AMB_OK;
+    while (c->used + bytes > c->budget)
+        if (!evict_one(c))
+            return fail(c, AMB_EBUDGET, "filtered planes: %zu bytes exceed the HBM budget (%llu used of %llu)", bytes,
+                        (unsigned long long)c->used, (unsigned long long)c->budget);
+    hipError_t e = hipMallocFromPoolAsync(p, bytes, c->ppool, c->stream);
+    if (e != hipSuccess) { /* the device is full outside the budget's view: wait, trim, retry */
+        (void)hipGetLastError();
+        hipStreamSynchronize(c->stream);
+        hipMemPoolTrimTo(c->ppool, 0);
+        e = hipMallocFromPoolAsync(p, bytes, c->ppool, c->stream);
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return fail(c, AMB_EBUDGET, "filtered planes: pool allocation of %zu bytes failed: %s", bytes,
+                    hipGetErrorString(e));
+    }
+    c->used += bytes;
+    return AMB_OK;
+}
+
+void plane_free(amb_ctx *c, void *p, size_t bytes)
+{
+    if (!p) return;
+    if (!c->ppool) return dev_free(c, p, bytes);
+    hipFreeAsync(p, c->stream); /* after every launch already on the stream that reads it */
+    c->used -= bytes;
 }
 
 /* grow-only device scratch */
@@ -366,6 +420,7 @@ int get_planes(amb_ctx *c, const PlaneSrc &src, int cube_id, int kind, int r, in
         if (p.cube == cube_id && p.kind == kind && p.ng == ng && p.lo == g.lo && p.hi == g.hi) {
             p.last = ++c->tick;
             p.pins++;
+            if (c->pin_log && src.list == &c->planes) c->pin_log->push_back(p.d);
             *out = p.d;
             return AMB_OK;
         }
@@ -404,7 +459,7 @@ int get_planes(amb_ctx *c, const PlaneSrc &src, int cube_id, int kind, int r, in
             return fail(c, AMB_EBUDGET, "plane arena: %zu + %zu bytes exceed %zu", src.bump->off, need, src.bump->cap);
         d = (void *)((uintptr_t)src.bump->base + src.bump->off); /* base 0: measuring only */
         src.bump->off += need;
-    } else if ((rc = dev_alloc(c, &d, bytes, "filtered planes"))) {
+    } else if ((rc = plane_alloc(c, &d, bytes))) {
         return rc;
     }
     op.dst = d;
@@ -412,7 +467,7 @@ int get_planes(amb_ctx *c, const PlaneSrc &src, int cube_id, int kind, int r, in
         src.record->push_back(op);
     } else {
         if ((kind <= 1 || kind == 8) && (rc = scratch(c, &c->G, &c->G_bytes, build_scratch_bytes(c, op), "fft scratch"))) {
-            dev_free(c, d, bytes);
+            plane_free(c, d, bytes);
             return rc;
         }
         if (kind <= 1) {
@@ -423,7 +478,7 @@ int get_planes(amb_ctx *c, const PlaneSrc &src, int cube_id, int kind, int r, in
         hipError_t e = launch_build(c, c->stream, op, c->G);
         if (kind <= 1) timer_stop(c);
         if (e != hipSuccess) {
-            dev_free(c, d, bytes);
+            plane_free(c, d, bytes);
             return fail(c, AMB_EHIP, "plane build (kind %d): %s", kind, hipGetErrorString(e));
         }
     }
@@ -431,6 +486,7 @@ int get_planes(amb_ctx *c, const PlaneSrc &src, int cube_id, int kind, int r, in
     p.cube = cube_id; p.kind = kind; p.ng = ng; p.lo = g.lo; p.hi = g.hi;
     p.d = d; p.bytes = bytes; p.last = ++c->tick; p.pins = 1;
     src.list->push_back(p);
+    if (c->pin_log && src.list == &c->planes) c->pin_log->push_back(d);
     *out = d;
     return AMB_OK;
 }
@@ -438,6 +494,29 @@ int get_planes(amb_ctx *c, const PlaneSrc &src, int cube_id, int kind, int r, in
 void unpin_all(amb_ctx *c)
 {
     for (auto &p : c->planes) p.pins = 0;
+}
+
+static PlaneSet *plane_of(amb_ctx *c, const void *d)
+{
+    for (auto &p : c->planes)
+        if (p.d == d) return &p;
+    return nullptr;
+}
+
+void pins_to_server(amb_ctx *c, const std::vector<void *> &log, bool spin)
+{
+    for (void *d : log)
+        if (PlaneSet *p = plane_of(c, d)) {
+            if (p->pins > 0) p->pins--;
+            if (spin) p->spins++;
+        }
+}
+
+void unspin(amb_ctx *c, const std::vector<void *> &log)
+{
+    for (void *d : log)
+        if (PlaneSet *p = plane_of(c, d))
+            if (p->spins > 0) p->spins--;
 }
 
 int find_cube(amb_ctx *c, int id, Cube **cb)
@@ -448,11 +527,12 @@ int find_cube(amb_ctx *c, int id, Cube **cb)
     return AMB_OK;
 }
 
+/* planes the serving queue holds stay (they are evicted once released) */
 static void drop_cube_planes(amb_ctx *c, int id)
 {
     for (size_t i = 0; i < c->planes.size();)
-        if (c->planes[i].cube == id) {
-            dev_free(c, c->planes[i].d, c->planes[i].bytes);
+        if (c->planes[i].cube == id && c->planes[i].spins == 0) {
+            plane_free(c, c->planes[i].d, c->planes[i].bytes);
             c->planes.erase(c->planes.begin() + i);
         } else ++i;
 }
@@ -903,6 +983,20 @@ int amb_create(const amb_opts *o, amb_ctx **out)
     c->margin = (unsigned *)m;
     c->used = 256;
     c->grouped_blocks = amb_contract_grouped_blocks(c->device);
+    {
+        hipMemPoolProps pp{};
+        pp.allocType = hipMemAllocationTypePinned;
+        pp.handleTypes = hipMemHandleTypeNone;
+        pp.location.type = hipMemLocationTypeDevice;
+        pp.location.id = c->device;
+        if (hipMemPoolCreate(&c->ppool, &pp) == hipSuccess) {
+            uint64_t keep = ~(uint64_t)0; /* freed planes stay in the pool for the next ones */
+            hipMemPoolSetAttribute(c->ppool, hipMemPoolAttrReleaseThreshold, &keep);
+        } else {
+            (void)hipGetLastError();
+            c->ppool = nullptr;
+        }
+    }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) c->arch = prop.gcnArchName;
     if (c->arch.empty()) c->arch = "unknown";
@@ -921,7 +1015,9 @@ void amb_destroy(amb_ctx *c)
     while (!c->graphs.empty()) amb_graph_destroy(c, (amb_graph *)c->graphs.back());
     hipStreamSynchronize(c->stream);
     timers_collect(c);
-    for (auto &p : c->planes) hipFree(p.d);
+    for (auto &p : c->planes) plane_free(c, p.d, p.bytes);
+    hipStreamSynchronize(c->stream);
+    if (c->ppool) hipMemPoolDestroy(c->ppool);
     for (auto &kv : c->cubes) {
         Cube &cb = kv.second;
         hipFree(cb.vis_t); hipFree(cb.samp_t); hipFree(cb.d_act); hipFree(cb.d_inv);
@@ -954,6 +1050,7 @@ int amb_set_precision(amb_ctx *c, int prec)
     if (!c) return AMB_EARG;
     Lock g(c->mu);
     if (prec < AMB_PREC_FP32 || prec > AMB_PREC_FP16X3) return fail(c, AMB_EARG, "precision %d", prec);
+    if (c->server && prec != c->prec) return fail(c, AMB_ESTATE, "the precision is fixed while serving");
     c->prec = prec;
     return AMB_OK;
 }
@@ -967,31 +1064,54 @@ int amb_set_batch_mode(amb_ctx *c, int mode)
     return AMB_OK;
 }
 
+} // extern "C"
+
+namespace amb {
+
 /* normalize_data (amber_funcs.c:1946-1978) fused into the transposing
-   upload; rows of cells are staged through a device buffer of <= ~256 MiB */
-static int upload_cube(amb_ctx *c, Cube &cb, const double *gv, const double *samp)
+   upload; rows of cells are staged through the device buffer stg */
+int upload_cube(amb_ctx *c, Cube &cb, const double *gv, const double *samp, hipStream_t s, void *stg, size_t stg_bytes)
 {
     const int64_t N = cb.N, nc = cb.nc, Nh = N / 2 + 1;
     const size_t row_doubles = (size_t)Nh * nc * 3;
-    int rows = (int)std::max<size_t>(1, (256u << 20) / (row_doubles * 8));
-    if (rows > N) rows = (int)N;
-    void *stg = nullptr;
-    int rc = dev_alloc(c, &stg, row_doubles * 8 * rows, "upload staging");
-    if (rc) return rc;
+    int rows = (int)std::min<size_t>((size_t)N, stg_bytes / (row_doubles * 8));
+    if (rows < 1) return fail(c, AMB_EARG, "upload staging of %zu bytes holds no row of %lld cells", stg_bytes, (long long)Nh);
     double *dg = (double *)stg, *ds = dg + (size_t)rows * Nh * nc * 2;
     hipError_t e = hipSuccess;
     for (int64_t i0 = 0; i0 < N && e == hipSuccess; i0 += rows) {
         int r = (int)std::min<int64_t>(rows, N - i0);
         size_t n1 = (size_t)r * Nh * nc;
-        e = hipMemcpyAsync(dg, gv + (size_t)i0 * Nh * nc * 2, n1 * 16, hipMemcpyHostToDevice, c->stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(ds, samp + (size_t)i0 * Nh * nc, n1 * 8, hipMemcpyHostToDevice, c->stream);
-        if (e == hipSuccess) e = amb_launch_upload(c->stream, dg, ds, (int)i0, r, (int)N, (int)Nh, (int)nc, cb.vis_t, cb.samp_t);
-        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        e = hipMemcpyAsync(dg, gv + (size_t)i0 * Nh * nc * 2, n1 * 16, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(ds, samp + (size_t)i0 * Nh * nc, n1 * 8, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = amb_launch_upload(s, dg, ds, (int)i0, r, (int)N, (int)Nh, (int)nc, cb.vis_t, cb.samp_t);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
     }
-    dev_free(c, stg, row_doubles * 8 * rows);
     if (e != hipSuccess) return fail(c, AMB_EHIP, "cube upload: %s", hipGetErrorString(e));
     return AMB_OK;
 }
+
+/* bytes of the staging buffer of an upload: <= ~256 MiB of whole rows */
+static size_t upload_staging_bytes(const Cube &cb)
+{
+    const size_t row = (size_t)(cb.N / 2 + 1) * cb.nc * 3 * 8;
+    const size_t rows = std::min<size_t>((size_t)cb.N, std::max<size_t>(1, (256u << 20) / row));
+    return row * rows;
+}
+
+static int upload_cube(amb_ctx *c, Cube &cb, const double *gv, const double *samp)
+{
+    const size_t sb = upload_staging_bytes(cb);
+    void *stg = nullptr;
+    int rc = dev_alloc(c, &stg, sb, "upload staging");
+    if (rc) return rc;
+    rc = upload_cube(c, cb, gv, samp, c->stream, stg, sb);
+    dev_free(c, stg, sb);
+    return rc;
+}
+
+} // namespace amb
+
+extern "C" {
 
 int amb_load_cube(amb_ctx *c, int64_t N, int64_t nc, double Du, int32_t flag_key, const double *gv,
                   const double *samp, int *cube_id)
@@ -1030,6 +1150,7 @@ int amb_update_cube(amb_ctx *c, int id, const double *gv, const double *samp)
     Cube *cb;
     int rc = find_cube(c, id, &cb);
     if (rc) return rc;
+    if (cb->serving) return fail(c, AMB_ESTATE, "cube %d has triangles in the serving queue", id);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     drop_cube_planes(c, id); /* filtered from the old values */
     return upload_cube(c, *cb, gv, samp);
@@ -1061,7 +1182,17 @@ int amb_release_cube(amb_ctx *c, int id)
     int rc = find_cube(c, id, &cb);
     if (rc) return rc;
     if (cb->graphs) return fail(c, AMB_ESTATE, "cube %d is read by %d captured graph(s)", id, cb->graphs);
+    if (cb->serving) return fail(c, AMB_ESTATE, "cube %d has triangles in the serving queue", id);
     hipStreamSynchronize(c->stream);
+    release_cube_buffers(c, id);
+    return AMB_OK;
+}
+
+} // extern "C"
+
+void amb::release_cube_buffers(amb_ctx *c, int id)
+{
+    Cube *cb = &c->cubes[id];
     drop_cube_planes(c, id);
     const size_t nv = (size_t)cb->N * (cb->N / 2 + 1) * cb->nc;
     dev_free(c, cb->vis_t, sizeof(float2) * nv);
@@ -1072,8 +1203,9 @@ int amb_release_cube(amb_ctx *c, int id)
     dev_free(c, cb->d_wlo, sizeof(int) * na);
     dev_free(c, cb->d_whi, sizeof(int) * na);
     c->cubes.erase(id);
-    return AMB_OK;
 }
+
+extern "C" {
 
 int amb_set_channels(amb_ctx *c, int id, const int *uf, int uf_nc, int nc10)
 {
@@ -1084,6 +1216,17 @@ int amb_set_channels(amb_ctx *c, int id, const int *uf, int uf_nc, int nc10)
     int rc = find_cube(c, id, &cb);
     if (rc) return rc;
     if (cb->graphs) return fail(c, AMB_ESTATE, "cube %d is read by %d captured graph(s)", id, cb->graphs);
+    if (cb->serving) return fail(c, AMB_ESTATE, "cube %d has triangles in the serving queue", id);
+    return set_channels_core(c, id, uf, uf_nc, nc10);
+}
+
+} // extern "C"
+
+int amb::set_channels_core(amb_ctx *c, int id, const int *uf, int uf_nc, int nc10)
+{
+    Cube *cb;
+    int rc = find_cube(c, id, &cb);
+    if (rc) return rc;
     const int nc = (int)cb->nc;
     if (nc10 < 0 || nc10 > nc - 1) return fail(c, AMB_EARG, "nc10=%d outside [0,%d]", nc10, nc - 1);
     std::vector<char> on(nc, 0);
@@ -1096,8 +1239,9 @@ int amb_set_channels(amb_ctx *c, int id, const int *uf, int uf_nc, int nc10)
     } else {
         std::fill(on.begin(), on.end(), 1);
     }
-    hipStreamSynchronize(c->stream);
-    /* planes are built per channel set: drop this cube's cached planes */
+    /* planes are built per channel set: drop this cube's cached planes (a
+       cube without channels has neither planes nor tables in use) */
+    if (cb->chans) hipStreamSynchronize(c->stream);
     drop_cube_planes(c, id);
     size_t na_old = cb->act.size();
     dev_free(c, cb->d_act, sizeof(int) * na_old);
@@ -1146,10 +1290,13 @@ int amb_set_channels(amb_ctx *c, int id, const int *uf, int uf_nc, int nc10)
     return AMB_OK;
 }
 
+extern "C" {
+
 int amb_set_rings(amb_ctx *c, const double *lo, const double *hi, int nr)
 {
     if (!c || !lo || !hi || nr <= 0) return AMB_EARG;
     Lock lk(c->mu);
+    if (c->server) return fail(c, AMB_ESTATE, "the rings are fixed while serving");
     for (int i = 0; i < nr; ++i)
         if (!(hi[i] > lo[i]) || !(lo[i] >= 0)) return fail(c, AMB_EARG, "ring %d edges [%g,%g)", i, lo[i], hi[i]);
     c->ulow.assign(lo, lo + nr);

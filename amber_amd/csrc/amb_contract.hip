@@ -1076,7 +1076,7 @@ __global__ __launch_bounds__(512, 1) void k_contract_grouped_v2(const ContractAr
         }
         const ContractArgs &a = descs[lo];
         const int ngroups = (a.nact + QP - 1) / QP;
-        const long r = w - begin[lo];
+        const long r = w - begin[lo] + a.item0;
         contract_item_v2<FORM, NORM>(a, (int)(r / ngroups), (int)(r % ngroups), S0, S1);
         __syncthreads();
     }
@@ -1122,7 +1122,7 @@ __global__ __launch_bounds__(256 * NWR, 1) void k_contract_grouped(const Contrac
         }
         const ContractArgs &a = descs[lo]; /* uniform: scalar loads */
         const int ngroups = (a.nact + QP - 1) / QP;
-        const long r = w - begin[lo];
+        const long r = w - begin[lo] + a.item0;
         contract_item<MODE, NORM, NWR, PIPE>(a, (int)(r / ngroups), (int)(r % ngroups), L);
         __syncthreads();
     }

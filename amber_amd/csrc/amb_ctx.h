@@ -33,6 +33,10 @@ struct Cube {
     bool chans = false;
     int nc10 = 0;
     int graphs = 0;            /* captured graphs that read this cube's buffers */
+    int serving = 0;           /* triangles of the serving queue admitted on this cube */
+    bool uploaded = false;     /* carried by a request: evictable once idle */
+    int holds = 0;             /* requests carrying it not yet admitted (not evictable) */
+    uint64_t last_use = 0;     /* tick of its last admitted triangle (LRU of uploaded cubes) */
     std::vector<int> act, inv; /* compact <-> channel */
     int *d_act = nullptr;      /* [nact] */
     int *d_inv = nullptr;      /* inv[nc] then first_ge[nc+1] */
@@ -56,7 +60,8 @@ struct PlaneSet {
     void *d = nullptr;
     size_t bytes = 0;
     uint64_t last = 0;
-    int pins = 0;
+    int pins = 0;  /* held by the call in progress (cleared by unpin_all) */
+    int spins = 0; /* held by triangles the serving queue has admitted */
 };
 
 /* What fills one plane set, recorded so it can be launched now or inside
@@ -128,6 +133,11 @@ struct amb_ctx {
     amb_tune_entry forced{};
     bool has_forced = false;
     std::string arch; /* gcnArchName of the device */
+    std::vector<void *> *pin_log = nullptr; /* when set, get_planes notes every cache plane it pins */
+    /* stream-ordered pool of the cached planes: a plane is freed on the
+       stream that reads it, so eviction never waits for the device (null:
+       plain allocations) */
+    hipMemPool_t ppool = nullptr;
     int wave_max = 1024;    /* triangles per grouped wave */
     void *blas = nullptr;   /* rocBLAS handle of the post-processing products (first use) */
     void *na_scratch = nullptr; /* device address of the static scratch (allocation-free entry points) */
@@ -167,6 +177,9 @@ void timer_stop(amb_ctx *c);
 void timers_collect(amb_ctx *c);
 int dev_alloc(amb_ctx *c, void **p, size_t bytes, const char *what);
 void dev_free(amb_ctx *c, void *p, size_t bytes);
+/* cached plane sets: from the stream-ordered pool on the context's stream */
+int plane_alloc(amb_ctx *c, void **p, size_t bytes);
+void plane_free(amb_ctx *c, void *p, size_t bytes);
 int grow(amb_ctx *c, void **p, size_t *have, size_t need, const char *what);
 int grow_host(amb_ctx *c, void **p, size_t *have, size_t need);
 /* creates the copy stream and its events on first use */
@@ -177,6 +190,17 @@ int twiddles(amb_ctx *c, int n, float2 **out, FftPlan *pl);
 int find_cube(amb_ctx *c, int id, Cube **cb);
 long plane_pitch(long ng);
 void unpin_all(amb_ctx *c);
+/* moves one call-pin of each logged plane to the serving queue (spin) or
+   drops it (spin = false); unspin releases serving pins */
+void pins_to_server(amb_ctx *c, const std::vector<void *> &log, bool spin);
+void unspin(amb_ctx *c, const std::vector<void *> &log);
+/* upload of host cube arrays into a cube's device buffers on stream s,
+   staged through stg (stg_bytes); s is synchronised */
+int upload_cube(amb_ctx *c, Cube &cb, const double *gv, const double *samp, hipStream_t s, void *stg, size_t stg_bytes);
+/* frees a cube's device buffers and its planes and forgets it */
+void release_cube_buffers(amb_ctx *c, int id);
+/* amb_set_channels without its lock and state checks */
+int set_channels_core(amb_ctx *c, int id, const int *uf, int uf_nc, int nc10);
 /* planes of (cube, kind, ring r) at grid ng from src, pinned on return */
 int get_planes(amb_ctx *c, const PlaneSrc &src, int cube_id, int kind, int r, int ng, void **out);
 /* launches one recorded build on stream s with FFT scratch G */

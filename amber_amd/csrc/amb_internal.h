@@ -70,6 +70,10 @@ struct ContractArgs {
     float *C;                  /* data: [nact][nact][nact] f32 */
     int *Cq;                   /* norm: [nact][nact][nact] int32 of 8*count */
     unsigned *margin;          /* norm: max |S/P - q| as float bits */
+    long item0;                /* grouped launches: the triangle's item that the
+                                  descriptor's first item is (a launch may run a
+                                  sub-range of a triangle's items; each item owns
+                                  its C block, so ranges run in any order) */
 };
 /* var: kernel variant (0: plain k-steps, 1: BF16x3 k-steps interleaving
  * MFMA with the next block's A formation, 2: BF16x3 data and normalisation

@@ -696,142 +696,3 @@ void amb_graph_destroy(amb_ctx *c, amb_graph *gh)
 }
 
 } // extern "C"
-
-/* ------------------------------------------------------------------ serving */
-struct AmbServer {
-    struct Ticket {
-        std::vector<amb_tri_req> req;
-        double t_submit = 0, t_done = 0;
-        int status = AMB_OK;
-        bool done = false;
-    };
-    std::thread th;
-    std::mutex m;
-    std::condition_variable cv_work, cv_done;
-    std::deque<uint64_t> queue;
-    std::map<uint64_t, Ticket> tickets;
-    uint64_t next = 1;
-    int max_wave = 64;
-    bool stop = false;
-};
-
-using Ticket = AmbServer::Ticket;
-
-static void serve_loop(amb_ctx *c, AmbServer *s)
-{
-    hipSetDevice(c->device);
-    for (;;) {
-        std::vector<uint64_t> take;
-        std::vector<amb_tri_req> req;
-        {
-            std::unique_lock<std::mutex> lk(s->m);
-            s->cv_work.wait(lk, [&] { return s->stop || !s->queue.empty(); });
-            if (s->queue.empty()) return; /* stopping and drained */
-            /* continuous batching: everything that has arrived, up to the wave size */
-            while (!s->queue.empty()) {
-                const Ticket &t = s->tickets[s->queue.front()];
-                if (!take.empty() && (int)(req.size() + t.req.size()) > s->max_wave) break;
-                req.insert(req.end(), t.req.begin(), t.req.end());
-                take.push_back(s->queue.front());
-                s->queue.pop_front();
-            }
-        }
-        int rc;
-        {
-            Lock lk(c->mu);
-            rc = run_grouped(c, req.data(), (int)req.size(), now_ms(), nullptr);
-        }
-        const double t = now_ms();
-        {
-            std::lock_guard<std::mutex> lk(s->m);
-            for (uint64_t id : take) {
-                Ticket &tk = s->tickets[id];
-                tk.status = rc;
-                tk.t_done = t;
-                tk.done = true;
-            }
-        }
-        s->cv_done.notify_all();
-    }
-}
-
-extern "C" {
-
-int amb_serve_start(amb_ctx *c, int max_wave)
-{
-    if (!c || max_wave <= 0) return AMB_EARG;
-    Lock lk(c->mu);
-    if (c->server) return fail(c, AMB_ESTATE, "serving already started");
-    if (c->batch_mode != AMB_BATCH_GROUPED) return fail(c, AMB_ESTATE, "serving runs grouped waves only");
-    AmbServer *s = new (std::nothrow) AmbServer();
-    if (!s) return AMB_ENOMEM;
-    s->max_wave = std::min(max_wave, MAX_WAVE);
-    c->server = s;
-    s->th = std::thread(serve_loop, c, s);
-    return AMB_OK;
-}
-
-int amb_submit(amb_ctx *c, const amb_tri_req *req, int n, amb_ticket *ticket)
-{
-    if (!c || !req || n <= 0 || !ticket) return AMB_EARG;
-    AmbServer *s = c->server;
-    if (!s) return AMB_ESTATE;
-    for (int i = 0; i < n; ++i)
-        if (!req[i].bl) return AMB_EARG;
-    std::lock_guard<std::mutex> lk(s->m);
-    if (s->stop) return AMB_ESTATE;
-    if ((int)s->queue.size() >= 65536) return AMB_EBUSY;
-    const uint64_t id = s->next++;
-    Ticket &t = s->tickets[id];
-    t.req.assign(req, req + n);
-    t.t_submit = now_ms();
-    s->queue.push_back(id);
-    *ticket = id;
-    s->cv_work.notify_one();
-    return AMB_OK;
-}
-
-int amb_poll(amb_ctx *c, amb_ticket ticket, int *done)
-{
-    if (!c || !done) return AMB_EARG;
-    AmbServer *s = c->server;
-    if (!s) return AMB_ESTATE;
-    std::lock_guard<std::mutex> lk(s->m);
-    auto it = s->tickets.find(ticket);
-    if (it == s->tickets.end()) return AMB_EARG;
-    *done = it->second.done ? 1 : 0;
-    return AMB_OK;
-}
-
-int amb_wait(amb_ctx *c, amb_ticket ticket, double *latency_ms)
-{
-    if (!c) return AMB_EARG;
-    AmbServer *s = c->server;
-    if (!s) return AMB_ESTATE;
-    std::unique_lock<std::mutex> lk(s->m);
-    auto it = s->tickets.find(ticket);
-    if (it == s->tickets.end()) return AMB_EARG;
-    s->cv_done.wait(lk, [&] { return it->second.done; });
-    const int rc = it->second.status;
-    if (latency_ms) *latency_ms = it->second.t_done - it->second.t_submit;
-    s->tickets.erase(it);
-    return rc;
-}
-
-int amb_serve_stop(amb_ctx *c)
-{
-    if (!c) return AMB_EARG;
-    AmbServer *s = c->server;
-    if (!s) return AMB_OK;
-    {
-        std::lock_guard<std::mutex> lk(s->m);
-        s->stop = true;
-    }
-    s->cv_work.notify_all();
-    if (s->th.joinable()) s->th.join(); /* drains the queue first */
-    c->server = nullptr;
-    delete s;
-    return AMB_OK;
-}
-
-} // extern "C"

@@ -104,7 +104,9 @@ def main():
                          "baseline are cut to fit it")
     ap.add_argument("--config", default="cfg3", choices=sorted(CONFIGS) + ["cfg4", "cfg5", "post", "maps", "grid"])
     ap.add_argument("--qps", type=float, default=0.0, help="cfg5: mean arrival rate (0: 60%% of the measured capacity)")
-    ap.add_argument("--duration", type=float, default=30.0, help="cfg5: seconds of arrivals")
+    ap.add_argument("--duration", type=float, default=90.0, help="cfg5: seconds of arrivals")
+    ap.add_argument("--probe-s", type=float, default=20.0, help="cfg5: seconds of the one-at-a-time capacity probe")
+    ap.add_argument("--upload-frac", type=float, default=0.1, help="cfg5: fraction of requests that carry a cube")
     ap.add_argument("--precision", default="bf16x3", choices=["bf16x3", "fp16x3", "fp32", "fp8"])
     ap.add_argument("--tri-limit", type=int, default=0,
                     help="development only: K triangles (the line is then not the metric workload)")
@@ -290,11 +292,12 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def mixed_workload(A, ctx, shapes, ntri, seed, max_r_frac=1.0 / 3.0):
+def mixed_workload(A, ctx, shapes, ntri, seed, max_r_frac=1.0 / 3.0, keep=None):
     """Resident cubes of the given (N, nc) shapes (no flags, all lags) and
     ntri random closed triangles over them: ring widths uniform in 1..16
     cells, radii below N/3, each triangle's three rings its own.  Returns
-    (cubes by shape, [(cube, r1, r2, r3)], ring lo, ring hi, triangles by cube)."""
+    (cubes by shape, [(cube, r1, r2, r3)], ring lo, ring hi, triangles by cube);
+    keep (a dict): receives each shape's host arrays (gv, samp)."""
     rng = np.random.default_rng(seed)
     cubes = {}
     for k, (N, nc) in enumerate(shapes):
@@ -302,6 +305,8 @@ def mixed_workload(A, ctx, shapes, ntri, seed, max_r_frac=1.0 / 3.0):
         cid = ctx.load_cube(N, nc, 1.0, 0, gv, samp)
         ctx.set_channels(cid, None, nc - 1)
         cubes[(N, nc)] = cid
+        if keep is not None:
+            keep[(N, nc)] = (gv, samp)
         del gv, samp
     lo, hi, reqs = [], [], []
     by_cube = {c: [] for c in cubes.values()}
@@ -443,34 +448,50 @@ def drop_all(ctx, cubes):
 
 
 def bench_cfg5(A, args):
-    """Serving: Poisson arrivals of requests of 1-64 triangles on resident
-    cubes, continuous batching into grouped waves, p50/p99 latency."""
+    """Serving: Poisson arrivals of requests, each of 1-64 triangle bins of
+    the cfg4 distribution on one cube, which is a resident cube of one of
+    the nine shapes {256,512,1024}^2 x {128,384,768} or (a fraction
+    --upload-frac of requests) a new cube of one of those shapes carried by
+    the request and uploaded by the serving queue; continuous batching at
+    work-item granularity; p50/p99 latency and throughput."""
     import threading
-    shapes = [(N, nc) for N in (256, 512) for nc in (128, 384)]
+    shapes = [(N, nc) for N in (256, 512, 1024) for nc in (128, 384, 768)]
     ctx = A.Context(device=0, precision=args.precision)
     t0 = time.time()
-    cubes, reqs, lo, hi, by_cube = mixed_workload(A, ctx, shapes, 400, 5001)
-    log("cfg5: 4 cubes resident, %d triangle bins, setup %.1f s" % (len(reqs), time.time() - t0))
+    host = {}
+    cubes, reqs, lo, hi, by_cube = mixed_workload(A, ctx, shapes, 600, 5001, keep=host)
+    log("cfg5: 9 cubes resident, %d triangle bins, setup %.1f s" % (len(reqs), time.time() - t0))
     rng = np.random.default_rng(5002)
+    shape_of = {cid: sh for sh, cid in cubes.items()}
     cube_ids = sorted(by_cube)
 
     def make_request():
         cid = cube_ids[rng.integers(len(cube_ids))]
         pool = by_cube[cid]
         k = int(rng.integers(1, 65))
-        pick = rng.choice(len(pool), size=min(k, len(pool)), replace=False)
-        return [pool[i] for i in pick]
+        pick = [pool[i] for i in rng.choice(len(pool), size=min(k, len(pool)), replace=False)]
+        if rng.random() < args.upload_frac:  # the same bins on a carried cube of this shape
+            return shape_of[cid], [t[1:] for t in pick]
+        return None, pick
 
-    # capacity probe: a closed loop of requests, one at a time
-    ctx.serve_start(max_wave=64)
+    def submit(r):
+        shape, tris = r
+        if shape is None:
+            return ctx.submit(tris)
+        gv, samp = host[shape]
+        return ctx.submit_cube(shape[0], shape[1], 1.0, 0, gv, samp, tris, nc10=shape[1] - 1)[0]
+
+    ctx.serve_start(max_active=256)
+    # capacity: a closed loop of requests, one at a time
     t = time.time()
     nprobe = 0
-    while time.time() - t < 5.0:
-        ctx.wait(ctx.submit(make_request()))
+    while time.time() - t < args.probe_s or nprobe < 3:
+        ctx.wait(submit(make_request()))
         nprobe += 1
     cap_rps = nprobe / (time.time() - t)
     qps = args.qps if args.qps > 0 else 0.6 * cap_rps
-    log("cfg5: offered load %.2f requests/s for %.0f s" % (qps, args.duration))
+    log("cfg5: capacity %.3f requests/s one at a time (%d requests); offered load %.3f requests/s for %.0f s"
+        % (cap_rps, nprobe, qps, args.duration))
     ctx.reset_stats()
     tickets = []
     lock = threading.Lock()
@@ -490,8 +511,10 @@ def bench_cfg5(A, args):
                 time.sleep(0.001)
                 continue
             outs, lat = ctx.wait(tk[0])
-            done.append((lat, tk[1]))
+            done.append((lat, tk[1], tk[2]))
             i += 1
+            if i % 5 == 0:
+                log("cfg5: %d requests done" % i)
 
     stop = [False]
     th = threading.Thread(target=waiter)
@@ -503,9 +526,9 @@ def bench_cfg5(A, args):
         if now < next_t:
             time.sleep(next_t - now)
         r = make_request()
-        tk = ctx.submit(r)
+        tk = submit(r)
         with lock:
-            tickets.append((tk, len(r)))
+            tickets.append((tk, len(r[1]), r[0] is not None))
         next_t += rng.exponential(1.0 / qps)
     stop[0] = True
     th.join()
@@ -513,16 +536,27 @@ def bench_cfg5(A, args):
     ctx.serve_stop()
     st = ctx.stats()
     lat = np.array([d[0] for d in done])
+    up = np.array([d[2] for d in done])
     ntri = sum(d[1] for d in done)
+
+    def pct(x):
+        return ({"p50": round(float(np.percentile(x, 50)), 1), "p99": round(float(np.percentile(x, 99)), 1),
+                 "max": round(float(np.max(x)), 1), "n": int(len(x))} if len(x) else None)
+
     line = {"metric": "cfg5 serving latency and throughput", "precision": args.precision,
-            "offered_qps": round(qps, 3), "capacity_rps_one_at_a_time": round(cap_rps, 3),
+            "offered_qps": round(qps, 4), "capacity_rps_one_at_a_time": round(cap_rps, 4),
             "requests": len(done), "triangles": int(ntri), "wall_s": round(wall, 2),
-            "throughput": {"requests_per_s": round(len(done) / wall, 3), "triangles_per_s": round(ntri / wall, 2),
+            "throughput": {"requests_per_s": round(len(done) / wall, 4), "triangles_per_s": round(ntri / wall, 2),
                            "evals_per_s": round(st["evals"] / wall, 1)},
-            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),
-                           "p99": round(float(np.percentile(lat, 99)), 2), "max": round(float(lat.max()), 2)},
-            "workload": "cubes {256,512}^2 x {128,384} ch resident, 400 triangle bins (widths 1-16, radii < N/3), "
-                        "requests of 1-64 bins of one cube, Poisson arrivals, waves of <= 64 triangles"}
+            "latency_ms": pct(lat), "latency_ms_resident_cube": pct(lat[~up]), "latency_ms_carried_cube": pct(lat[up]),
+            "cube_uploads": int(st["n_cube_uploads"]), "evicted_cubes": int(st["n_evict_cubes"]),
+            "evicted_plane_sets": int(st["n_evict_planes"]), "iterations": int(st["n_serve_iters"]),
+            "device_ms": {"contraction": round(st["ms_contract"] + st["ms_norm"], 1), "fft": round(st["ms_fft"], 1),
+                          "reduce": round(st["ms_reduce"], 1)},
+            "workload": "9 resident cubes {256,512,1024}^2 x {128,384,768} ch, 600 triangle bins (widths 1-16, "
+                        "radii < N/3); requests of 1-64 bins of one cube, %.0f%% of them carrying a new cube of that "
+                        "shape (up to 1024^2 x 768 complex64, uploaded by the queue); Poisson arrivals for %.0f s; "
+                        "HBM budget 0.9 of the free memory" % (100 * args.upload_frac, args.duration)}
     ctx.close()
     print(json.dumps(line), flush=True)
 

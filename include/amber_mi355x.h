@@ -331,20 +331,57 @@ int amb_graph_create(amb_ctx *ctx, const amb_tri_req *req, int n, amb_graph **ou
 int amb_graph_run(amb_ctx *ctx, amb_graph *g);
 void amb_graph_destroy(amb_ctx *ctx, amb_graph *g);
 
-/* Serving queue: a library thread takes submitted requests in arrival
- * order and runs them as grouped waves of at most max_wave triangles
- * (continuous batching: a wave starts as soon as the previous one ends,
- * with whatever has arrived).  Output pointers of a request must stay
- * valid until amb_wait returns for its ticket.  Other calls on the context
- * wait while a wave runs. */
+/* Serving queue (no counterpart in the reference, which runs one cube per
+ * process, amber.c:32-38): continuous batching at the granularity of
+ * contraction work items.  A scheduler thread admits requests' triangles
+ * as device memory allows and launches bounded contraction iterations,
+ * each taking a fair share of work items from every admitted request, so
+ * a request that arrives while others run starts within an iteration or
+ * two instead of waiting for them to end; a triangle is reduced and
+ * copied out as soon as its last items have run.  Triangles are admitted
+ * against the context's HBM budget: their planes come from the LRU plane
+ * cache (unpinned planes are evicted as needed, planes of admitted
+ * triangles are held until they finish), their partial sums and results
+ * from pools reserved at start (a quarter and a sixteenth of the budget,
+ * at most 16 GiB and 2 GiB).  max_active bounds the triangles admitted at
+ * once.  Results are bit-identical to amb_bispec_multi of the same
+ * triangles.  Output pointers of a request must stay valid until amb_wait
+ * returns for its ticket.  While serving, the context's precision and
+ * rings are fixed, and cubes with admitted triangles cannot be released,
+ * updated or re-channelled (AMB_ESTATE); other calls may run between
+ * iterations. */
 typedef uint64_t amb_ticket;
-int amb_serve_start(amb_ctx *ctx, int max_wave);
+int amb_serve_start(amb_ctx *ctx, int max_active);
+/* AMB_EBUSY: 4096 requests already wait for admission; AMB_EBUDGET: a
+ * triangle of the request can never fit the pools */
 int amb_submit(amb_ctx *ctx, const amb_tri_req *req, int n, amb_ticket *ticket);
+
+/* A cube carried by a request: uploaded by a thread of its own on a stream
+ * of its own (the copies overlap the contraction), then resident under the
+ * id returned at submission, which later requests may name (they wait for
+ * the upload).  Such a cube is evicted, least recently used first, once no
+ * admitted triangle reads it and the budget needs its memory; a request
+ * naming an evicted cube fails with AMB_ESTATE.  gv / samp are in the
+ * layout of amb_read_cube_files and must stay valid until the ticket is
+ * done; chans (uf_nc ids, flag_key = 1) likewise. */
+typedef struct amb_cube_src {
+    int64_t N, nc;
+    double Du;
+    int32_t flag_key;
+    const double *gv, *samp;
+    const int *chans;
+    int uf_nc, nc10;
+} amb_cube_src;
+/* req[i].cube_id of 0 names the carried cube; AMB_EBUDGET when the cube
+ * cannot fit the budget even with every evictable byte freed */
+int amb_submit_cube(amb_ctx *ctx, const amb_cube_src *cube, const amb_tri_req *req, int n, int *cube_id,
+                    amb_ticket *ticket);
 /* *done = 1 once every triangle of the ticket is in host memory */
 int amb_poll(amb_ctx *ctx, amb_ticket ticket, int *done);
 /* blocks until done; *latency_ms = submit to completion (may be NULL);
- * returns the status the ticket's wave ended with */
+ * returns the ticket's status (the first error of its triangles) */
 int amb_wait(amb_ctx *ctx, amb_ticket ticket, double *latency_ms);
+/* finishes every submitted request, then stops */
 int amb_serve_stop(amb_ctx *ctx);
 
 /* One ring-filtered real-space image of plane `plane`, as the triple
@@ -481,6 +518,10 @@ typedef struct amb_stats {
                                       r1 == r2 (or r2 == r3) makes two triplets' sums the
                                       same, the pair counts once */
     int64_t n_norm_exact;   /* triangles whose v2 normalisation was recomputed in FP64 */
+    int64_t n_evict_planes; /* filtered-plane sets evicted (LRU) for HBM budget */
+    int64_t n_evict_cubes;  /* cubes carried by requests evicted (LRU, idle) for HBM budget */
+    int64_t n_cube_uploads; /* cubes carried by requests uploaded by the serving queue */
+    int64_t n_serve_iters;  /* contraction iterations the serving queue launched */
 } amb_stats;
 int amb_get_stats(amb_ctx *ctx, amb_stats *st);
 int amb_reset_stats(amb_ctx *ctx);

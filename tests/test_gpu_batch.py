@@ -149,7 +149,7 @@ def test_serving_matches_alone(ctx):
         cube = ca if k % 2 else cb
         pick = rng.choice(len(tris), size=1 + k % 4, replace=False)
         requests.append([(cube,) + tuple(tris[i]) for i in pick])
-    ctx.serve_start(max_wave=8)
+    ctx.serve_start(max_active=8)
     try:
         tickets = [ctx.submit(r, want_norm=(k % 3 == 0)) for k, r in enumerate(requests)]
         results = [ctx.wait(t) for t in tickets]
