@@ -21,8 +21,8 @@ if not os.path.exists(LIB_PATH):
     raise ImportError("amber_amd: %s is missing; build it with `python -m amber_amd.build`" % LIB_PATH)
 _lib = C.CDLL(LIB_PATH)
 
-PREC_FP32, PREC_BF16X3, PREC_FP8, PREC_FP16X3 = 0, 1, 2, 3
-_PREC = {"fp32": PREC_FP32, "bf16x3": PREC_BF16X3, "fp8": PREC_FP8, "fp16x3": PREC_FP16X3}
+PREC_FP32, PREC_BF16X3, PREC_FP8, PREC_FP16X3, PREC_LAGFT = 0, 1, 2, 3, 4
+_PREC = {"fp32": PREC_FP32, "bf16x3": PREC_BF16X3, "fp8": PREC_FP8, "fp16x3": PREC_FP16X3, "lagft": PREC_LAGFT}
 
 EXPORTS = [
     "amb_strerror", "amb_version", "amb_read_params_ess", "amb_read_gv_header", "amb_read_cube_files",
@@ -112,7 +112,7 @@ class Stats(C.Structure):
                 ("ms_post", C.c_double), ("flop_post", C.c_double), ("cert_margin_max", C.c_double),
                 ("n_uncertified", C.c_int64), ("flop_contract_distinct", C.c_double),
                 ("n_norm_exact", C.c_int64), ("n_evict_planes", C.c_int64), ("n_evict_cubes", C.c_int64),
-                ("n_cube_uploads", C.c_int64), ("n_serve_iters", C.c_int64)]
+                ("n_cube_uploads", C.c_int64), ("n_serve_iters", C.c_int64), ("flop_lagft", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -17,7 +17,7 @@ LIB = os.path.join(HERE, "libamber_mi355x.so")
 BIN = os.path.join(HERE, "bin")
 
 HIP_SOURCES = ["amb_fft.hip", "amb_contract.hip", "amb_api.hip", "amb_wave.hip", "amb_post.hip", "amb_maps.hip",
-               "amb_grid.hip", "amb_tune.hip", "amb_serve.hip"]
+               "amb_grid.hip", "amb_tune.hip", "amb_serve.hip", "amb_lagft.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # packed-FP32 instructions are kept out of every HIP object
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", "-Wno-unused-value",
@@ -47,8 +47,7 @@ def build(verbose=False):
         if verbose and out.strip():
             print(out)
         objs.append(o)
-    # rocBLAS: the two plain FP64 matrix products of the lag-space post-processing
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-o", LIB] + objs + ["-lrocblas", "-lgomp", "-lm"])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-o", LIB] + objs + ["-lgomp", "-lm"])
     for name, src, extra in (("amber", "amber_main.c", []), ("amber_v2", "amber_main.c", ["-DAMB_V2"]),
                              ("amber_maps", "amber_maps.c", [])):
         _run(["gcc", "-O2", "-std=gnu11", "-Wall", "-I" + INC] + extra +

@@ -320,6 +320,15 @@ static int ring_geom(amb_ctx *c, const Cube &cb, int r, int ng, RingGeom *g)
     if (jc > ng / 3) jc = ng / 3;
     if (jc < 0) jc = 0;
     g->jcnt = (int)jc;
+    /* integer radius-squared bounds of the half-open ring, uu > 0 included:
+       sqrt is monotone, so each test is a threshold on s = ia^2 + j^2 */
+    auto first_ge = [](double r, long from) {
+        long s = std::max(from, (long)floor(r > 0 ? r * r : 0.0) - 4);
+        while (sqrt((double)s) < r) ++s;
+        return s;
+    };
+    g->s_lo = first_ge(g->lo, 1);
+    g->s_hi = std::max(first_ge(g->hi, 0), g->s_lo);
     return AMB_OK;
 }
 
@@ -344,16 +353,19 @@ size_t build_scratch_bytes(amb_ctx *c, const BuildOp &op)
 
 /* FP64 twiddles of length n (the exact normalisation's c2r), built on the
  * device once per context */
-static int twiddles64(amb_ctx *c, int n, double2 **out)
+int twiddles64(amb_ctx *c, int n, double2 **out)
 {
     auto it = c->tw64.find(n);
-    if (it != c->tw64.end()) { *out = it->second; return AMB_OK; }
+    if (it != c->tw64.end()) {
+        if (out) *out = it->second;
+        return AMB_OK;
+    }
     void *d = nullptr;
     int rc = dev_alloc(c, &d, sizeof(double2) * n, "FP64 twiddle table");
     if (rc) return rc;
     HIPCHK(c, amb_launch_twiddles64(c->stream, (double2 *)d, n));
     c->tw64[n] = (double2 *)d;
-    *out = (double2 *)d;
+    if (out) *out = (double2 *)d;
     return AMB_OK;
 }
 
@@ -382,6 +394,8 @@ hipError_t launch_build(amb_ctx *c, hipStream_t s, const BuildOp &op, float2 *G)
                                     (uint8_t *)op.dst);
     case 5:
         return amb_launch_fp8_scales(s, (const float *)op.src, pitch, nact, (float *)op.dst);
+    case 9:
+        return lag_launch_spectra(c, s, op);
     case 8:
         return amb_launch_ring_c2r64(s, cb.samp_t, cb.d_act, nact, (int)cb.N, (int)(cb.N / 2 + 1),
                                      c->plans[op.g.ng], c->tw64[op.g.ng], op.g, (double2 *)G, (double *)op.dst,
@@ -397,8 +411,10 @@ hipError_t launch_build(amb_ctx *c, hipStream_t s, const BuildOp &op, float2 *G)
    ring cells read */
 double build_fft_bytes(amb_ctx *c, const BuildOp &op)
 {
-    if (op.kind > 1) return 0.0;
     const Cube &cb = c->cubes[op.cube];
+    if (op.kind == 9) /* the FP32 set read, the half spectra written */
+        return (double)op.g.ng * op.g.ng * (4.0 * cb.nact() + 8.0 * (cb.lag_L / 2 + 1));
+    if (op.kind > 1) return 0.0;
     const RingGeom &g = op.g;
     double cells = 0;
     for (long ia = -(long)ceil(g.hi); ia <= (long)ceil(g.hi); ++ia)
@@ -438,6 +454,15 @@ int get_planes(amb_ctx *c, const PlaneSrc &src, int cube_id, int kind, int r, in
         FftPlan pl;
         double2 *tw64;
         if ((rc = twiddles(c, ng, &tw, &pl)) || (rc = twiddles64(c, ng, &tw64))) return rc;
+    } else if (kind == 9) {
+        /* channel spectra of the data set: its FP32 images and the channel
+           transform's twiddles */
+        if (src.record || src.bump) return fail(c, AMB_EARG, "lag-Fourier planes are not built inside graphs");
+        if ((rc = lag_setup(c, *cb))) return rc;
+        if ((rc = get_planes(c, src, cube_id, 0, r, ng, &dep))) return rc;
+        float2 *tw;
+        FftPlan pl;
+        if ((rc = twiddles(c, cb->lag_L, &tw, &pl))) return rc;
     } else if (kind >= 4) {
         if ((rc = get_planes(c, src, cube_id, 0, r, ng, &dep))) return rc;
         if (kind == 4 && (rc = get_planes(c, src, cube_id, 5, r, ng, &sc))) return rc;
@@ -451,7 +476,7 @@ int get_planes(amb_ctx *c, const PlaneSrc &src, int cube_id, int kind, int r, in
     }
     op.src = dep;
     op.sc = sc;
-    const size_t bytes = plane_bytes(kind, cb->nact(), plane_pitch(ng));
+    const size_t bytes = kind == 9 ? lag_spectra_bytes(*cb, ng) : plane_bytes(kind, cb->nact(), plane_pitch(ng));
     void *d = nullptr;
     if (src.bump) {
         const size_t need = (bytes + 255) / 256 * 256;
@@ -470,13 +495,13 @@ int get_planes(amb_ctx *c, const PlaneSrc &src, int cube_id, int kind, int r, in
             plane_free(c, d, bytes);
             return rc;
         }
-        if (kind <= 1) {
+        if (kind <= 1 || kind == 9) {
             timer_start(c, &c->st.ms_fft);
-            c->st.n_fft_launch += 2;
+            c->st.n_fft_launch += kind == 9 ? 1 : 2;
             c->st.bytes_fft += build_fft_bytes(c, op);
         }
         hipError_t e = launch_build(c, c->stream, op, c->G);
-        if (kind <= 1) timer_stop(c);
+        if (kind <= 1 || kind == 9) timer_stop(c);
         if (e != hipSuccess) {
             plane_free(c, d, bytes);
             return fail(c, AMB_EHIP, "plane build (kind %d): %s", kind, hipGetErrorString(e));
@@ -743,6 +768,32 @@ void account_tri(amb_ctx *c, const TriJob &j, const int32_t *cnt)
 int bispec_core(amb_ctx *c, int id, int r1, int r2, int r3, int want_norm, amb_result *res)
 {
     hipSetDevice(c->device);
+    if (c->prec == AMB_PREC_LAGFT) {
+        Cube *cb;
+        int rc = find_cube(c, id, &cb);
+        if (rc) return rc;
+        if (!cb->chans) return fail(c, AMB_ESTATE, "channels not set for cube %d", id);
+        if (r1 < 0 || r1 >= (int)c->uhigh.size()) return fail(c, AMB_EARG, "ring %d", r1);
+        const long ng = amb_ng(c->uhigh[r1], cb->Du);
+        res->ng = ng;
+        res->inv_ng2 = 1.0 / (ng * ng * 1.);
+        res->cert_margin = 0;
+        const size_t W = 2 * (size_t)cb->nc10 + 1;
+        std::vector<float> bl;
+        amb_tri_req q{};
+        q.cube_id = id; q.r1 = r1; q.r2 = r2; q.r3 = r3;
+        q.bl = res->bl;
+        if (!q.bl) { bl.resize(W * W); q.bl = bl.data(); }
+        q.cnt = res->cnt;
+        double *b64[1] = {res->bl64};
+        if ((rc = lagft_run(c, &q, 1, 0.0, nullptr, b64))) return rc;
+        if (want_norm) { /* this form's v2 normalisation: the exact FP64 one */
+            float m = 0.f;
+            rc = norm_exact(c, id, r1, r2, r3, res->norm, res->norm64, &m);
+            res->cert_margin = m;
+        }
+        return rc;
+    }
     TriJob j;
     const PlaneSrc src{&c->planes};
     int rc = prepare_tri(c, src, id, r1, r2, r3, want_norm, j);
@@ -960,7 +1011,7 @@ int amb_create(const amb_opts *o, amb_ctx **out)
         c->prec = o->precision;
         c->verbose = o->verbose;
     }
-    if (c->prec < AMB_PREC_FP32 || c->prec > AMB_PREC_FP16X3) { delete c; return AMB_EARG; }
+    if (c->prec < AMB_PREC_FP32 || c->prec > AMB_PREC_LAGFT) { delete c; return AMB_EARG; }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= c->device || c->device < 0) {
         (void)hipGetLastError();
@@ -1021,7 +1072,7 @@ void amb_destroy(amb_ctx *c)
     for (auto &kv : c->cubes) {
         Cube &cb = kv.second;
         hipFree(cb.vis_t); hipFree(cb.samp_t); hipFree(cb.d_act); hipFree(cb.d_inv);
-        hipFree(cb.d_wlo); hipFree(cb.d_whi);
+        hipFree(cb.d_wlo); hipFree(cb.d_whi); hipFree(cb.d_lagcnt);
     }
     for (auto &kv : c->tw) hipFree(kv.second);
     for (auto &kv : c->tw64) hipFree(kv.second);
@@ -1034,6 +1085,7 @@ void amb_destroy(amb_ctx *c)
     }
     hipFree(c->G); hipFree(c->C); hipFree(c->Cq); hipFree(c->margin); hipFree(c->out); hipFree(c->out2);
     hipFree(c->Cqx); hipFree(c->outx);
+    hipFree(c->lag_part); hipFree(c->lag_st); hipFree(c->lag_out);
     hipFree(c->wdesc);
     for (void *h : c->hstage)
         if (h) hipHostFree(h);
@@ -1049,7 +1101,7 @@ int amb_set_precision(amb_ctx *c, int prec)
 {
     if (!c) return AMB_EARG;
     Lock g(c->mu);
-    if (prec < AMB_PREC_FP32 || prec > AMB_PREC_FP16X3) return fail(c, AMB_EARG, "precision %d", prec);
+    if (prec < AMB_PREC_FP32 || prec > AMB_PREC_LAGFT) return fail(c, AMB_EARG, "precision %d", prec);
     if (c->server && prec != c->prec) return fail(c, AMB_ESTATE, "the precision is fixed while serving");
     c->prec = prec;
     return AMB_OK;
@@ -1202,6 +1254,7 @@ void amb::release_cube_buffers(amb_ctx *c, int id)
     dev_free(c, cb->d_inv, sizeof(int) * (2 * ncs + 1));
     dev_free(c, cb->d_wlo, sizeof(int) * na);
     dev_free(c, cb->d_whi, sizeof(int) * na);
+    lag_release(c, *cb);
     c->cubes.erase(id);
 }
 
@@ -1243,6 +1296,7 @@ int amb::set_channels_core(amb_ctx *c, int id, const int *uf, int uf_nc, int nc1
        cube without channels has neither planes nor tables in use) */
     if (cb->chans) hipStreamSynchronize(c->stream);
     drop_cube_planes(c, id);
+    lag_release(c, *cb);
     size_t na_old = cb->act.size();
     dev_free(c, cb->d_act, sizeof(int) * na_old);
     dev_free(c, cb->d_inv, sizeof(int) * (2 * (size_t)nc + 1));

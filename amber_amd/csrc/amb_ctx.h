@@ -42,6 +42,13 @@ struct Cube {
     int *d_inv = nullptr;      /* inv[nc] then first_ge[nc+1] */
     int *d_wlo = nullptr, *d_whi = nullptr;
     std::vector<int> wlo, whi; /* host copies of the per-plane windows */
+    /* lag-Fourier form: channel transform length L (0 until first use), row
+       pitch of the spectra (L/2 + 1 rounded up to 4), the channel set's lag
+       counts on the device and the host, their cells with cnt > 0 and sum */
+    int lag_L = 0, lag_lhp = 0;
+    int *d_lagcnt = nullptr;
+    std::vector<int32_t> h_lagcnt;
+    int64_t lag_evals = 0, lag_pairs = 0;
     int nact() const { return (int)act.size(); }
 };
 
@@ -51,7 +58,9 @@ struct Cube {
  * 4: e4m3 data images times their B-side scales,
  * 5: per-plane FP8 scales of kind 0 (A side [nact], then B side [nact]),
  * 7: hi and lo FP16 halves of kind 0 times 2^8, stored hi planes then lo,
- * 8: FP64 sampling images (the exact v2 normalisation).
+ * 8: FP64 sampling images (the exact v2 normalisation),
+ * 9: channel spectra of kind 0 (lag-Fourier form): [Ng^2][lag_lhp] complex
+ *    FP32, the DFT over channels of each pixel, then the [max, E] words.
  * Kind 0 sets are stored normalised by a power of two (max |D| <= 2^7) and
  * end in two words: max |D| as computed and its exponent E. */
 struct PlaneSet {
@@ -70,7 +79,7 @@ struct BuildOp {
     int kind = 0, cube = 0;
     RingGeom g{};
     void *dst = nullptr;
-    const void *src = nullptr; /* kinds 2-7: the FP32 images */
+    const void *src = nullptr; /* kinds 2-7, 9: the FP32 images */
     const void *sc = nullptr;  /* kind 4: the FP8 scales */
 };
 
@@ -139,7 +148,6 @@ struct amb_ctx {
        plain allocations) */
     hipMemPool_t ppool = nullptr;
     int wave_max = 1024;    /* triangles per grouped wave */
-    void *blas = nullptr;   /* rocBLAS handle of the post-processing products (first use) */
     void *na_scratch = nullptr; /* device address of the static scratch (allocation-free entry points) */
     /* scratch (grow-only) */
     float2 *G = nullptr; size_t G_bytes = 0;
@@ -156,6 +164,11 @@ struct amb_ctx {
     hipStream_t cstream = nullptr;
     hipEvent_t ev_red = nullptr, ev_cp[2] = {nullptr, nullptr};
     void *wdesc = nullptr; size_t wdesc_bytes = 0; /* wave descriptors */
+    /* lag-Fourier form: tile partials, the FP64 spectrum of one triangle,
+       lag arrays of a wave */
+    void *lag_part = nullptr; size_t lag_part_bytes = 0;
+    void *lag_st = nullptr; size_t lag_st_bytes = 0;
+    void *lag_out = nullptr; size_t lag_out_bytes = 0;
     /* pinned host staging, two buffers: a grouped wave's results are copied
        out of one while the next wave's land in the other; the lag-space
        post-processing stages alternate triangles in them the same way */
@@ -219,7 +232,7 @@ void account_counts(amb_ctx *c, const TriJob &j, int64_t evals, int64_t pairs, i
 int check_cert(amb_ctx *c, float m, bool *recompute);
 /* the exact (FP64) v2 normalisation of one triangle into host arrays */
 int norm_exact(amb_ctx *c, int id, int r1, int r2, int r3, float *norm, double *norm64, float *margin);
-/* releases the post-processing library handle (amb_destroy) */
+/* post-processing teardown at amb_destroy (nothing held today) */
 void post_release(amb_ctx *c);
 /* algorithmic HBM bytes of one c2r build (0 for derived kinds) */
 double build_fft_bytes(amb_ctx *c, const BuildOp &op);
@@ -233,6 +246,17 @@ amb_tune_entry tune_cfg(amb_ctx *c, long ng, int nact);
 int tune_variant(const amb_tune_entry &t, int prec, int nact, long pitch);
 /* grouped waves of the requests (results into their buffers); done_ms may be null */
 int run_grouped(amb_ctx *c, const amb_tri_req *req, int n, double t0, double *done_ms);
+/* FP64 twiddles of length n, built on the device once per context */
+int twiddles64(amb_ctx *c, int n, double2 **out);
+/* lag-Fourier form (amb_lagft.hip): transform length and counts of a
+ * cube's channel set (once), their release, the bytes of a spectra set at
+ * grid ng, its build from the kind 0 set op.src, and triangles in waves
+ * (bl64: per-request FP64 lag arrays, may be null) */
+int lag_setup(amb_ctx *c, Cube &cb);
+void lag_release(amb_ctx *c, Cube &cb);
+size_t lag_spectra_bytes(const Cube &cb, long ng);
+hipError_t lag_launch_spectra(amb_ctx *c, hipStream_t s, const BuildOp &op);
+int lagft_run(amb_ctx *c, const amb_tri_req *req, int n, double t0, double *done_ms, double *const *bl64);
 /* calCR; noalloc: all device scratch from the static array, no allocation */
 int closed_rings(amb_ctx *c, int id, int plane, double *rdata, int cap, int *rcount, bool noalloc);
 

@@ -4,22 +4,26 @@
  * (amber_funcs.c:967-1010, 1109-1131), in FP32.
  *
  * The c2r of an Ng x (Ng/2+1) half plane whose nonzeros are the cells of one
- * annulus runs in two LDS-staged Stockham passes:
+ * annulus runs in two passes, each transform owned by one 64-thread team of
+ * the LDS Stockham core (amb_fftcore.h: register codelets for 2, 3, 4 and
+ * the primes up to 47, a direct stage for larger primes, so every even Ng,
+ * 2*109, 2*181, 2*349 included, runs without padding):
  *   cols: for every column j < jcnt (the only columns a ring can touch,
  *         jcnt = min(Ng/3, ceil(UHigh))) gather the ring cells straight from
- *         the plane-major cube, inverse-DFT along the first axis in LDS and
- *         write G[plane][j][x] (x contiguous, coalesced).
+ *         the plane-major cube, inverse-DFT along the first axis and write
+ *         G[plane][j][x] (x contiguous, coalesced).
  *   rows: for every pair of output rows (xa, xb) build the Hermitian
  *         extension Z(k) = A(k) + i B(k), Z(Ng-k) = conj A(k) + i conj B(k),
  *         one complex inverse DFT gives row xa in Re and row xb in Im; the
  *         row is written once as Ng floats (coalesced).
  * The j=0 column enters the row pass through its real part, which is the
  * c2r of the Hermitian-symmetrised column (the FFTW/pocketfft behaviour).
- * Radices 2, 3, 4 have register butterflies; any other prime factor runs a
- * direct radix-p stage parallelised over its outputs, so every even Ng
- * (2*109, 2*181, 2*349, ...) is handled without padding.
+ * Ring membership is an integer test on ia^2 + j^2 against thresholds the
+ * host derives with the reference's double sqrt.  A plane's workgroups run
+ * as one contiguous range on one XCD, so its G rows stay in one L2.
  */
 #include "amb_internal.h"
+#include "amb_fftcore.h"
 #include <math.h>
 #include <stdio.h>
 
@@ -27,230 +31,197 @@
 
 #define FFT_THREADS 256
 
-void amb_make_plan(int n, FftPlan *pl)
-{
-    pl->n = n;
-    pl->nst = 0;
-    int m = n;
-    while (m % 4 == 0 && pl->nst < AMB_MAX_RADIX_STAGES) { pl->rad[pl->nst++] = 4; m /= 4; }
-    while (m % 2 == 0 && pl->nst < AMB_MAX_RADIX_STAGES) { pl->rad[pl->nst++] = 2; m /= 2; }
-    while (m % 3 == 0 && pl->nst < AMB_MAX_RADIX_STAGES) { pl->rad[pl->nst++] = 3; m /= 3; }
-    for (int p = 5; m > 1 && pl->nst < AMB_MAX_RADIX_STAGES; p += 2)
-        while (m % p == 0) { pl->rad[pl->nst++] = p; m /= p; }
-}
+void amb_make_plan(int n, FftPlan *pl) { fftc::make_plan(n, pl); }
 
-/* complex type of a real scalar type: FP32 for the filtered planes, FP64
- * for the calCR images */
-template <typename R>
-struct CT;
-template <>
-struct CT<float> { typedef float2 c; };
-template <>
-struct CT<double> { typedef double2 c; };
+namespace {
 
-__device__ __forceinline__ float2 cmk(float x, float y) { return make_float2(x, y); }
-__device__ __forceinline__ double2 cmk(double x, double y) { return make_double2(x, y); }
-template <typename C>
-__device__ __forceinline__ C cmul(C a, C b)
-{
-    return cmk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-template <typename C>
-__device__ __forceinline__ C cadd(C a, C b) { return cmk(a.x + b.x, a.y + b.y); }
-template <typename C>
-__device__ __forceinline__ C csub(C a, C b) { return cmk(a.x - b.x, a.y - b.y); }
-
-/* nb transforms of length n in x[b*n + i]; inverse direction exp(+2 pi i/n).
- * tw[m] = exp(+2 pi i m / n).  Returns the buffer holding the result. */
+using fftc::cx;
 template <typename T>
-__device__ typename CT<T>::c *lds_fft_batch(typename CT<T>::c *x, typename CT<T>::c *y,
-                                            const typename CT<T>::c *tw, const FftPlan &pl, int nb)
+using Cx = typename fftc::CT<T>::c;
+
+/* blocks b = x (mod 8) run on XCD x: XCD x takes the contiguous work range
+   [x per, (x+1) per), so the blocks of one plane share an L2 */
+__device__ __forceinline__ long xcd_item(long b, long nb)
 {
-    typedef typename CT<T>::c float2; /* the complex type of this instance */
-    const int n = pl.n;
-    const int tid = threadIdx.x, nth = blockDim.x;
-    int Ns = 1;
-    for (int s = 0; s < pl.nst; ++s) {
-        const int R = pl.rad[s];
-        const int nR = n / R;
-        const int tstep = n / (Ns * R);
-        if (R == 4) {
-            for (int t = tid; t < nb * nR; t += nth) {
-                int b = t / nR, j = t - b * nR, k = j % Ns;
-                const float2 *xi = x + b * n;
-                float2 v0 = xi[j];
-                float2 v1 = cmul(xi[j + nR], tw[k * tstep]);
-                float2 v2 = cmul(xi[j + 2 * nR], tw[2 * k * tstep]);
-                float2 v3 = cmul(xi[j + 3 * nR], tw[3 * k * tstep]);
-                float2 s02 = cadd(v0, v2), d02 = csub(v0, v2);
-                float2 s13 = cadd(v1, v3), d13 = csub(v1, v3);
-                float2 id13 = cmk(-d13.y, d13.x); /* i*(v1-v3) */
-                float2 *yo = y + b * n + (j / Ns) * Ns * 4 + k;
-                yo[0] = cadd(s02, s13);
-                yo[Ns] = cadd(d02, id13);
-                yo[2 * Ns] = csub(s02, s13);
-                yo[3 * Ns] = csub(d02, id13);
-            }
-        } else if (R == 2) {
-            for (int t = tid; t < nb * nR; t += nth) {
-                int b = t / nR, j = t - b * nR, k = j % Ns;
-                const float2 *xi = x + b * n;
-                float2 v0 = xi[j];
-                float2 v1 = cmul(xi[j + nR], tw[k * tstep]);
-                float2 *yo = y + b * n + (j / Ns) * Ns * 2 + k;
-                yo[0] = cadd(v0, v1);
-                yo[Ns] = csub(v0, v1);
-            }
-        } else if (R == 3) {
-            const T h = (T)0.86602540378443864676; /* sqrt(3)/2 */
-            for (int t = tid; t < nb * nR; t += nth) {
-                int b = t / nR, j = t - b * nR, k = j % Ns;
-                const float2 *xi = x + b * n;
-                float2 v0 = xi[j];
-                float2 v1 = cmul(xi[j + nR], tw[k * tstep]);
-                float2 v2 = cmul(xi[j + 2 * nR], tw[2 * k * tstep]);
-                float2 sm = cadd(v1, v2), df = csub(v1, v2);
-                float2 m = cmk(v0.x - (T)0.5 * sm.x, v0.y - (T)0.5 * sm.y);
-                float2 r = cmk(-h * df.y, h * df.x); /* i*h*(v1-v2) */
-                float2 *yo = y + b * n + (j / Ns) * Ns * 3 + k;
-                yo[0] = cadd(v0, sm);
-                yo[Ns] = cadd(m, r);
-                yo[2 * Ns] = csub(m, r);
-            }
-        } else {
-            /* direct radix-R stage, one thread per output:
-               out[(j/Ns)Ns R + k + q Ns] = sum_r x[j + r nR] w_{Ns R}^{r (k + q Ns)} */
-            const int NsR = Ns * R;
-            for (int t = tid; t < nb * n; t += nth) {
-                int b = t / n, o = t - b * n;
-                int g = o / NsR, rem = o - g * NsR;
-                int q = rem / Ns, k = rem - q * Ns;
-                int j = g * Ns + k;
-                int e = k + q * Ns; /* exponent step in units of w_{NsR} */
-                const float2 *xi = x + b * n + j;
-                float2 acc = cmk((T)0, (T)0);
-                int ex = 0;
-                for (int r = 0; r < R; ++r) {
-                    acc = cadd(acc, cmul(xi[r * nR], tw[ex * tstep]));
-                    ex += e;
-                    if (ex >= NsR) ex -= NsR;
-                }
-                y[b * n + o] = acc;
-            }
-        }
-        __syncthreads();
-        float2 *tmp = x; x = y; y = tmp;
-        Ns *= R;
-    }
-    return x;
+    const long per = nb / 8;
+    return (b % 8) * per + b / 8;
 }
 
-/* grid (ceil(jcnt/nb), npl); dynamic LDS: tw[n] + 2 * nb * n complex */
+/* largest codelet prime per scalar type (larger codelets spill: FP32 past 23,
+   FP64 past 19); larger prime factors take the direct stage */
 template <typename R>
-__global__ __launch_bounds__(FFT_THREADS) void k_fft_cols(
-    int kind, const float2 *__restrict__ vis_t, const float *__restrict__ samp_t,
-    const int *__restrict__ planes, int N, int Nh, FftPlan pl, const typename CT<R>::c *__restrict__ twg,
-    double lo, double hi, int jcnt, int nb, typename CT<R>::c *__restrict__ G)
+struct MaxP { static constexpr int v = 23; };
+template <>
+struct MaxP<double> { static constexpr int v = 19; };
+
+} // namespace
+
+/* Column pass: one 64-thread team per (plane, column j < jcnt).  The ring
+ * cells of column j (s_lo <= ia^2 + j^2 < s_hi, ia in (-Ng/2, Ng/2]) are
+ * gathered from the plane-major cube (unit stride in ia), inverse DFT along
+ * the first axis, written to G[plane][j][x].
+ * 1-D grid of nblk x nteam teams, a multiple of 8 blocks; dynamic LDS:
+ * [tw n if TWL] + nteam * 2 n complex. */
+template <typename R, bool TWL>
+__global__ __launch_bounds__(FFT_THREADS) void k_ring_cols(int kind, const float2 *__restrict__ vis_t,
+                                                           const float *__restrict__ samp_t,
+                                                           const int *__restrict__ planes, int npl, int N, int Nh,
+                                                           FftPlan pl, const Cx<R> *__restrict__ twg, long s_lo,
+                                                           long s_hi, int jcnt, int nbx, Cx<R> *__restrict__ G)
 {
-    typedef typename CT<R>::c C;
+    typedef Cx<R> C;
     extern __shared__ __align__(16) char smem_raw[];
-    C *smem = (C *)smem_raw;
+    C *sm = (C *)smem_raw;
     const int n = pl.n;
-    C *tw = smem, *bx = smem + n, *by = bx + nb * n;
-    const int pli = blockIdx.y;
-    const int ch = planes[pli];
-    const int j0 = blockIdx.x * nb;
-    for (int t = threadIdx.x; t < n; t += blockDim.x) tw[t] = twg[t];
-    const int jlim = n / 3; /* column cut j < Ng/3 (amber_funcs.c:582) */
-    for (int t = threadIdx.x; t < nb * n; t += blockDim.x) {
-        int c = t / n, ing = t - c * n;
-        int j = j0 + c;
-        int ia = ing <= n / 2 ? ing : ing - n; /* ia in (-Ng/2, Ng/2] */
-        C v = cmk((R)0, (R)0);
-        if (j < jcnt && j < jlim) {
-            double uu = sqrt(1. * (double)(ia * ia + j * j));
-            if (uu >= lo && uu < hi && uu > 0.0) {
-                int in = ia < 0 ? N + ia : ia;
-                size_t src = ((size_t)ch * Nh + j) * N + in;
-                if (kind == 0) v = cmk((R)vis_t[src].x, (R)vis_t[src].y);
-                else v = cmk((R)samp_t[src], (R)0);
-            }
+    const int nteam = blockDim.x >> 6, team = threadIdx.x >> 6, t = threadIdx.x & 63;
+    C *bx = sm + (TWL ? n : 0) + (size_t)team * 2 * n, *by = bx + n;
+    const C *tw = TWL ? sm : twg;
+    if (TWL)
+        for (int i = threadIdx.x; i < n; i += blockDim.x) sm[i] = twg[i];
+    const long item = xcd_item(blockIdx.x, gridDim.x);
+    const int pli = (int)(item / nbx);
+    const int j = (int)(item - (long)pli * nbx) * nteam + team;
+    const bool live = pli < npl && j < jcnt;
+    const int ch = live ? planes[pli] : 0;
+    for (int ing = t; ing < n; ing += 64) {
+        const int ia = ing <= n / 2 ? ing : ing - n;
+        const long s2 = (long)ia * ia + (long)j * j;
+        C v = cx((R)0, (R)0);
+        if (live && s2 >= s_lo && s2 < s_hi) {
+            const int in = ia < 0 ? N + ia : ia;
+            const size_t src = ((size_t)ch * Nh + j) * N + in;
+            if (kind == 0) v = cx((R)vis_t[src].x, (R)vis_t[src].y);
+            else v = cx((R)samp_t[src], (R)0);
         }
-        bx[t] = v;
+        bx[ing] = v;
     }
     __syncthreads();
-    C *res = lds_fft_batch<R>(bx, by, tw, pl, nb);
-    for (int t = threadIdx.x; t < nb * n; t += blockDim.x) {
-        int c = t / n, x = t - c * n;
-        int j = j0 + c;
-        if (j < jcnt) G[((size_t)pli * jcnt + j) * n + x] = res[t];
+    const C *res = fftc::transform<+1, MaxP<R>::v>(bx, by, tw, pl, t, 64);
+    if (live) {
+        C *g = G + ((size_t)pli * jcnt + j) * n;
+        for (int x = t; x < n; x += 64) g[x] = res[x];
     }
 }
 
-/* grid (ceil(n/(2 nb)), npl); each block nb transforms = 2 nb output rows.
- * mx (may be null): max |value| over the whole set, as the bit pattern of
- * a non-negative float (one atomic per workgroup). */
-template <typename R>
-__global__ __launch_bounds__(FFT_THREADS) void k_fft_rows(
-    const typename CT<R>::c *__restrict__ G, int jcnt, FftPlan pl, const typename CT<R>::c *__restrict__ twg,
-    int nb, R *__restrict__ out, long pitch, unsigned *mx)
+/* Row pass: one team per (plane, pair of output rows xa, xb).  The
+ * Hermitian extension of the two rows' half spectra as one complex sequence
+ * Z(k) = A(k) + i B(k), Z(Ng - k) = conj A(k) + i conj B(k); its inverse DFT
+ * holds row xa in the real part and row xb in the imaginary part (the j = 0
+ * column enters through its real part, the c2r of the Hermitian-symmetrised
+ * column).  mx (may be null): max |value| over the set, as the bits of a
+ * non-negative float (one atomic per workgroup). */
+template <typename R, bool TWL>
+__global__ __launch_bounds__(FFT_THREADS) void k_ring_rows(const Cx<R> *__restrict__ G, int jcnt, int npl, FftPlan pl,
+                                                           const Cx<R> *__restrict__ twg, int nbx,
+                                                           R *__restrict__ out, long pitch, unsigned *mx)
 {
-    typedef typename CT<R>::c float2; /* the complex type of this instance */
+    typedef Cx<R> C;
     extern __shared__ __align__(16) char smem_raw[];
-    float2 *smem = (float2 *)smem_raw;
+    C *sm = (C *)smem_raw;
     const int n = pl.n;
-    float2 *tw = smem, *bx = smem + n, *by = bx + nb * n;
-    const int pli = blockIdx.y;
-    const int x0 = blockIdx.x * 2 * nb;
-    for (int t = threadIdx.x; t < n; t += blockDim.x) tw[t] = twg[t];
-    const float2 *Gp = G + (size_t)pli * jcnt * n;
-    /* b fastest so a warp reads 2*nb consecutive x of one column */
-    for (int t = threadIdx.x; t < nb * n; t += blockDim.x) {
-        int k = t / nb, b = t - k * nb;
-        int xa = x0 + 2 * b, xb = xa + 1;
-        float2 z = cmk((R)0, (R)0);
-        if (xb < n) {
+    const int nteam = blockDim.x >> 6, team = threadIdx.x >> 6, t = threadIdx.x & 63;
+    C *bx = sm + (TWL ? n : 0) + (size_t)team * 2 * n, *by = bx + n;
+    const C *tw = TWL ? sm : twg;
+    if (TWL)
+        for (int i = threadIdx.x; i < n; i += blockDim.x) sm[i] = twg[i];
+    const long item = xcd_item(blockIdx.x, gridDim.x);
+    const int pli = (int)(item / nbx), bxi = (int)(item - (long)pli * nbx);
+    const int xa = 2 * (bxi * nteam + team), xb = xa + 1;
+    const bool live = pli < npl && xb < n;
+    const C *Gp = G + (size_t)(live ? pli : 0) * jcnt * n;
+    for (int k = t; k < n; k += 64) {
+        C z = cx((R)0, (R)0);
+        if (live) {
             if (k == 0) {
-                float2 A = Gp[xa], B = Gp[xb];
-                z = cmk(A.x, B.x);
+                z = cx(Gp[xa].x, Gp[xb].x);
             } else if (k < jcnt) {
-                float2 A = Gp[(size_t)k * n + xa], B = Gp[(size_t)k * n + xb];
-                z = cmk(A.x - B.y, A.y + B.x);
+                const C A = Gp[(size_t)k * n + xa], B = Gp[(size_t)k * n + xb];
+                z = cx(A.x - B.y, A.y + B.x);
             } else if (n - k < jcnt) {
-                int kk = n - k;
-                float2 A = Gp[(size_t)kk * n + xa], B = Gp[(size_t)kk * n + xb];
-                z = cmk(A.x + B.y, B.x - A.y);
+                const int kk = n - k;
+                const C A = Gp[(size_t)kk * n + xa], B = Gp[(size_t)kk * n + xb];
+                z = cx(A.x + B.y, B.x - A.y);
             }
         }
-        bx[b * n + k] = z;
+        bx[k] = z;
     }
     __syncthreads();
-    float2 *res = lds_fft_batch<R>(bx, by, tw, pl, nb);
-    R *op = out + (size_t)pli * pitch;
-    if (blockIdx.x == 0)
-        for (long t = (long)n * n + threadIdx.x; t < pitch; t += blockDim.x) op[t] = (R)0;
+    const C *res = fftc::transform<+1, MaxP<R>::v>(bx, by, tw, pl, t, 64);
     float m = 0.f;
-    for (int t = threadIdx.x; t < nb * n; t += blockDim.x) {
-        int b = t / n, yv = t - b * n;
-        int xa = x0 + 2 * b;
-        if (xa + 1 < n) {
-            float2 z = res[t];
-            op[(size_t)xa * n + yv] = z.x;
-            op[(size_t)(xa + 1) * n + yv] = z.y;
+    if (live) {
+        R *op = out + (size_t)pli * pitch;
+        if (bxi == 0 && team == 0)
+            for (long q = (long)n * n + t; q < pitch; q += 64) op[q] = (R)0;
+        for (int y = t; y < n; y += 64) {
+            const C z = res[y];
+            op[(size_t)xa * n + y] = z.x;
+            op[(size_t)xb * n + y] = z.y;
             m = fmaxf(m, fmaxf(fabsf((float)z.x), fabsf((float)z.y)));
         }
     }
     if (mx) {
         __shared__ float red[FFT_THREADS / 64];
         for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+        if (t == 0) red[team] = m;
         __syncthreads();
         if (threadIdx.x == 0) {
-            for (int w = 1; w < FFT_THREADS / 64; ++w) m = fmaxf(m, red[w]);
+            for (int w = 1; w < nteam; ++w) m = fmaxf(m, red[w]);
             atomicMax(mx, __float_as_uint(m));
         }
     }
 }
+
+namespace {
+
+/* teams per workgroup and whether the twiddles fit LDS beside them */
+struct FftLaunch {
+    int nteam = 0;
+    bool twl = true;
+    size_t lds = 0;
+};
+FftLaunch fft_launch(int n, size_t cbytes)
+{
+    FftLaunch f;
+    const size_t cap = 65536 / cbytes; /* complex values in 64 KiB */
+    f.nteam = (int)std::min<size_t>(4, cap > (size_t)n ? (cap - n) / (2 * (size_t)n) : 0);
+    if (f.nteam < 1) {
+        f.twl = false;
+        f.nteam = (int)std::min<size_t>(4, cap / (2 * (size_t)n));
+    }
+    f.lds = cbytes * ((f.twl ? n : 0) + 2 * (size_t)f.nteam * n);
+    return f;
+}
+
+template <typename R>
+hipError_t ring_passes(hipStream_t s, int kind, const float2 *vis_t, const float *samp_t, const int *d_planes, int npl,
+                       int N, int Nh, const FftPlan &pl, const Cx<R> *d_tw, const RingGeom &g, Cx<R> *G, R *out,
+                       long pitch, unsigned *mx)
+{
+    const int n = pl.n;
+    const FftLaunch f = fft_launch(n, sizeof(Cx<R>));
+    if (f.nteam < 1) return hipErrorInvalidValue;
+    const int nbc = (g.jcnt + f.nteam - 1) / f.nteam;
+    const long gc = ((long)npl * nbc + 7) / 8 * 8;
+    if (f.twl)
+        hipLaunchKernelGGL((k_ring_cols<R, true>), dim3((unsigned)gc), dim3(64 * f.nteam), f.lds, s, kind, vis_t, samp_t,
+                           d_planes, npl, N, Nh, pl, d_tw, g.s_lo, g.s_hi, g.jcnt, nbc, G);
+    else
+        hipLaunchKernelGGL((k_ring_cols<R, false>), dim3((unsigned)gc), dim3(64 * f.nteam), f.lds, s, kind, vis_t,
+                           samp_t, d_planes, npl, N, Nh, pl, d_tw, g.s_lo, g.s_hi, g.jcnt, nbc, G);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int nbr = (n / 2 + f.nteam - 1) / f.nteam;
+    const long gr = ((long)npl * nbr + 7) / 8 * 8;
+    if (f.twl)
+        hipLaunchKernelGGL((k_ring_rows<R, true>), dim3((unsigned)gr), dim3(64 * f.nteam), f.lds, s, G, g.jcnt, npl, pl,
+                           d_tw, nbr, out, pitch, mx);
+    else
+        hipLaunchKernelGGL((k_ring_rows<R, false>), dim3((unsigned)gr), dim3(64 * f.nteam), f.lds, s, G, g.jcnt, npl,
+                           pl, d_tw, nbr, out, pitch, mx);
+    return hipGetLastError();
+}
+
+} // namespace
 
 /* Smallest E with m <= 2^E (0 for m == 0 or not finite), clamped to
  * [-60, 60]. */
@@ -271,17 +242,18 @@ __device__ __forceinline__ int set_exp(float m)
  * (-1)^(p1 + d2 + d3): every other reference plane is accumulated negated,
  * so a rounding bias of the matrix cores that does not follow the sign of
  * its operands cancels in the sum over p1 instead of adding up. */
-__global__ void k_normalise_set(float *__restrict__ D, size_t n4, long pitch4, unsigned *slot)
+__global__ void k_normalise_set(float *__restrict__ D, long pitch4, unsigned *slot)
 {
     const int e = set_exp(__uint_as_float(slot[0]));
     const float f = ldexpf(1.f, AMB_SET_EXP - e);
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x) {
-        const float g = ((i / (size_t)pitch4) & 1) ? -f : f;
-        float4 v = ((float4 *)D)[i];
+    const float g = (blockIdx.y & 1) ? -f : f;
+    float4 *p = (float4 *)D + (size_t)blockIdx.y * pitch4;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < pitch4; i += (long)gridDim.x * blockDim.x) {
+        float4 v = p[i];
         v.x *= g; v.y *= g; v.z *= g; v.w *= g;
-        ((float4 *)D)[i] = v;
+        p[i] = v;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) slot[1] = (unsigned)e;
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) slot[1] = (unsigned)e;
 }
 
 __global__ void k_zero64(double *p, size_t n)
@@ -297,21 +269,13 @@ __global__ void k_zero(float *p, size_t n)
     for (; i < n; i += st) p[i] = 0.f;
 }
 
-static int lds_batch(int n, int cbytes = 8)
-{   /* transforms per block with tw + 2 buffers inside 64 KiB (cbytes per complex) */
-    int nb = (65536 / (cbytes * n) - 1) / 2;
-    if (nb > 8) nb = 8;
-    if (nb < 1) nb = 1;
-    return nb;
-}
-
 hipError_t amb_launch_ring_c2r(hipStream_t s, int kind, const float2 *vis_t, const float *samp_t,
                                const int *d_planes, int npl, int N, int Nh, const FftPlan &pl,
                                const float2 *d_tw, RingGeom g, float2 *G, float *out, long pitch, unsigned *slot)
 {
     const int n = pl.n;
     if (npl <= 0) return hipSuccess;
-    if (pitch < (long)n * n || pitch % 4 != 0) return hipErrorInvalidValue;
+    if (pitch < (long)n * n || pitch % 4 != 0 || (n & 1)) return hipErrorInvalidValue;
     if (slot) {
         /* max and exponent words start at zero: an empty set keeps E = 0 */
         hipError_t e = amb_launch_fill_u32(s, slot, 2, 0u);
@@ -322,20 +286,11 @@ hipError_t amb_launch_ring_c2r(hipStream_t s, int kind, const float2 *vis_t, con
         hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, s, out, tot);
         return hipGetLastError();
     }
-    int nb = lds_batch(n);
-    size_t lds = sizeof(float2) * (size_t)(n + 2 * nb * n);
-    dim3 gc((g.jcnt + nb - 1) / nb, npl);
-    hipLaunchKernelGGL(k_fft_cols<float>, gc, dim3(FFT_THREADS), lds, s, kind, vis_t, samp_t, d_planes, N, Nh,
-                       pl, d_tw, g.lo, g.hi, g.jcnt, nb, G);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    dim3 gr((n + 2 * nb - 1) / (2 * nb), npl);
-    hipLaunchKernelGGL(k_fft_rows<float>, gr, dim3(FFT_THREADS), lds, s, G, g.jcnt, pl, d_tw, nb, out, pitch, slot);
-    e = hipGetLastError();
+    hipError_t e = ring_passes<float>(s, kind, vis_t, samp_t, d_planes, npl, N, Nh, pl, d_tw, g, G, out, pitch, slot);
     if (e != hipSuccess || !slot) return e;
-    const size_t n4 = (size_t)npl * pitch / 4;
-    const int blocks = (int)std::max<size_t>(1, std::min<size_t>((n4 + 255) / 256, 8192));
-    hipLaunchKernelGGL(k_normalise_set, dim3(blocks), dim3(256), 0, s, out, n4, pitch / 4, slot);
+    const long p4 = pitch / 4;
+    const int bx = (int)std::min<long>((p4 + 255) / 256, 64);
+    hipLaunchKernelGGL(k_normalise_set, dim3((unsigned)bx, (unsigned)npl), dim3(256), 0, s, out, p4, slot);
     return hipGetLastError();
 }
 
@@ -348,23 +303,12 @@ hipError_t amb_launch_ring_c2r64(hipStream_t s, const float *samp_t, const int *
 {
     const int n = pl.n;
     if (npl <= 0) return hipSuccess;
-    if (pitch < (long)n * n) return hipErrorInvalidValue;
+    if (pitch < (long)n * n || (n & 1)) return hipErrorInvalidValue;
     if (g.jcnt <= 0) {
         hipLaunchKernelGGL(k_zero64, dim3(1024), dim3(256), 0, s, out, (size_t)npl * pitch);
         return hipGetLastError();
     }
-    const int nb = lds_batch(n, 16);
-    const size_t lds = sizeof(double2) * (size_t)(n + 2 * nb * n);
-    if (lds > 65536) return hipErrorInvalidValue;
-    dim3 gc((g.jcnt + nb - 1) / nb, npl);
-    hipLaunchKernelGGL(k_fft_cols<double>, gc, dim3(FFT_THREADS), lds, s, 1, nullptr, samp_t, d_planes, N, Nh, pl,
-                       d_tw, g.lo, g.hi, g.jcnt, nb, G);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    dim3 gr((n + 2 * nb - 1) / (2 * nb), npl);
-    hipLaunchKernelGGL(k_fft_rows<double>, gr, dim3(FFT_THREADS), lds, s, G, g.jcnt, pl, d_tw, nb, out, pitch,
-                       nullptr);
-    return hipGetLastError();
+    return ring_passes<double>(s, 1, nullptr, samp_t, d_planes, npl, N, Nh, pl, d_tw, g, G, out, pitch, nullptr);
 }
 
 /* tw[m] = exp(+2 pi i m / n) in FP64 */

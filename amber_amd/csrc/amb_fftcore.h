@@ -1,0 +1,244 @@
+/*
+ * Mixed-radix Stockham FFT in LDS for gfx950, shared by the ring c2r, the
+ * channel-axis transforms of the lag-Fourier form and the lag-space
+ * post-processing.
+ *
+ * A transform of length n is owned by a team of threads (one or more
+ * 64-wide wavefronts); a workgroup holds one or more teams, all running
+ * the same plan, so the stage loop ends with one workgroup barrier per
+ * stage.  A stage of radix R gives each thread butterflies j, j + T, ...
+ * (T threads per team), whose (group, offset) = (j / Ns, j % Ns) indices
+ * advance by a constant step: no integer division inside the loops.
+ * Butterflies are register codelets of compile-time size: 2, 3, 4 and the
+ * odd primes up to 47, the primes as a symmetric-pair DFT whose cos/sin
+ * coefficients are read from the transform's own twiddle table
+ * (tw[t n/R] = e^{2 pi i t/R}).  Any larger prime factor runs a direct stage
+ * (one output per thread and step).
+ *
+ * Direction DIR = +1 computes y_k = sum_m x_m e^{+2 pi i m k / n} (the
+ * unnormalised inverse, FFTW's c2r sign), DIR = -1 the forward transform.
+ * The twiddle table always holds e^{+2 pi i m / n}; the forward direction
+ * uses its conjugates.
+ */
+#pragma once
+#include "amb_internal.h"
+
+/* stages and butterflies also compile for the host, where a test runs a
+   team's threads one after another */
+#define FFTC_HD __host__ __device__ __forceinline__
+
+namespace fftc {
+
+/* plan of length n: radix-4 stages, a radix-2 stage, radix-3 stages, then
+   the other prime factors in ascending order */
+inline void make_plan(int n, FftPlan *pl)
+{
+    pl->n = n;
+    pl->nst = 0;
+    int m = n;
+    while (m % 4 == 0 && pl->nst < AMB_MAX_RADIX_STAGES) { pl->rad[pl->nst++] = 4; m /= 4; }
+    while (m % 2 == 0 && pl->nst < AMB_MAX_RADIX_STAGES) { pl->rad[pl->nst++] = 2; m /= 2; }
+    while (m % 3 == 0 && pl->nst < AMB_MAX_RADIX_STAGES) { pl->rad[pl->nst++] = 3; m /= 3; }
+    for (int p = 5; m > 1 && pl->nst < AMB_MAX_RADIX_STAGES; p += 2)
+        while (m % p == 0) { pl->rad[pl->nst++] = p; m /= p; }
+}
+
+template <typename T>
+struct CT;
+template <>
+struct CT<float> { typedef float2 c; };
+template <>
+struct CT<double> { typedef double2 c; };
+
+FFTC_HD float2 cx(float a, float b) { return make_float2(a, b); }
+FFTC_HD double2 cx(double a, double b) { return make_double2(a, b); }
+template <typename C>
+FFTC_HD C cadd(C a, C b) { return cx(a.x + b.x, a.y + b.y); }
+template <typename C>
+FFTC_HD C csub(C a, C b) { return cx(a.x - b.x, a.y - b.y); }
+template <typename C>
+FFTC_HD C cmul(C a, C b) { return cx(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+/* a times w (DIR > 0) or conj(w) (DIR < 0) */
+template <int DIR, typename C>
+FFTC_HD C twmul(C a, C w)
+{
+    if (DIR > 0) return cx(a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x);
+    return cx(a.x * w.x + a.y * w.y, a.y * w.x - a.x * w.y);
+}
+/* a times e^{DIR i pi / 2} */
+template <int DIR, typename C>
+FFTC_HD C rot90(C a)
+{
+    return DIR > 0 ? cx(-a.y, a.x) : cx(a.y, -a.x);
+}
+
+/* R-point DFT of the inputs v (already twiddled),
+ * y_q = sum_r v_r e^{DIR 2 pi i r q / R}, written to yo[q * ys].
+ * tw: the transform's table, stride rstep = n / R between the R-th roots. */
+template <int R, int DIR, typename C>
+struct Bfly {
+    /* odd prime R: symmetric pairs a_m = v_m + v_{R-m}, b_m = v_m - v_{R-m};
+       y_q = v_0 + sum a_m cos(2 pi m q / R) +- i sum b_m DIR sin(...); each
+       output pair is stored as soon as it is formed, so the outputs never
+       occupy registers beside the pairs */
+    FFTC_HD static void run(C *v, const C *tw, int rstep, C *yo, int ys)
+    {
+        typedef decltype(v[0].x) T;
+        constexpr int h = (R - 1) / 2;
+        T cs[h + 1], sn[h + 1];
+#pragma unroll
+        for (int m = 1; m <= h; ++m) {
+            const C w = tw[m * rstep];
+            cs[m] = w.x;
+            sn[m] = DIR > 0 ? w.y : -w.y;
+        }
+        C y0 = v[0];
+#pragma unroll
+        for (int m = 1; m <= h; ++m) {
+            const C a = cadd(v[m], v[R - m]), b = csub(v[m], v[R - m]);
+            v[m] = a;
+            v[R - m] = b;
+            y0 = cadd(y0, a);
+        }
+#pragma unroll
+        for (int q = 1; q <= h; ++q) {
+            C A = v[0], B = cx((T)0, (T)0);
+#pragma unroll
+            for (int m = 1; m <= h; ++m) {
+                const int t = (m * q) % R;
+                const T c = t <= h ? cs[t] : cs[R - t];
+                const T s = t <= h ? sn[t] : -sn[R - t];
+                A.x += v[m].x * c;
+                A.y += v[m].y * c;
+                B.x += v[R - m].x * s;
+                B.y += v[R - m].y * s;
+            }
+            /* i B = (-B.y, B.x) */
+            yo[q * ys] = cx(A.x - B.y, A.y + B.x);
+            yo[(R - q) * ys] = cx(A.x + B.y, A.y - B.x);
+        }
+        yo[0] = y0;
+    }
+};
+template <int DIR, typename C>
+struct Bfly<2, DIR, C> {
+    FFTC_HD static void run(C *v, const C *, int, C *yo, int ys)
+    {
+        yo[0] = cadd(v[0], v[1]);
+        yo[ys] = csub(v[0], v[1]);
+    }
+};
+template <int DIR, typename C>
+struct Bfly<3, DIR, C> {
+    FFTC_HD static void run(C *v, const C *, int, C *yo, int ys)
+    {
+        typedef decltype(v[0].x) T;
+        const T h = (T)0.86602540378443864676; /* sin(2 pi / 3) */
+        const C sm = cadd(v[1], v[2]), df = csub(v[1], v[2]);
+        const C m = cx(v[0].x - (T)0.5 * sm.x, v[0].y - (T)0.5 * sm.y);
+        const C r = rot90<DIR>(cx(h * df.x, h * df.y));
+        yo[0] = cadd(v[0], sm);
+        yo[ys] = cadd(m, r);
+        yo[2 * ys] = csub(m, r);
+    }
+};
+template <int DIR, typename C>
+struct Bfly<4, DIR, C> {
+    FFTC_HD static void run(C *v, const C *, int, C *yo, int ys)
+    {
+        const C s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+        const C s13 = cadd(v[1], v[3]), r13 = rot90<DIR>(csub(v[1], v[3]));
+        yo[0] = cadd(s02, s13);
+        yo[ys] = cadd(d02, r13);
+        yo[2 * ys] = csub(s02, s13);
+        yo[3 * ys] = csub(d02, r13);
+    }
+};
+
+/* One radix-R stage of a team's transform: x -> y (Stockham order). */
+template <int R, int DIR, typename C>
+FFTC_HD void stage(const C *__restrict__ x, C *__restrict__ y, const C *__restrict__ tw, int n,
+                                      int Ns, int t, int T)
+{
+    const int nR = n / R, tstep = n / (Ns * R);
+    int g = t / Ns, k = t - g * Ns;
+    const int dg = T / Ns, dk = T - dg * Ns;
+    for (int j = t; j < nR; j += T) {
+        C v[R];
+        v[0] = x[j];
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[r] = twmul<DIR>(x[j + r * nR], tw[r * k * tstep]);
+        Bfly<R, DIR, C>::run(v, tw, n / R, y + g * Ns * R + k, Ns);
+        k += dk;
+        g += dg;
+        if (k >= Ns) { k -= Ns; ++g; }
+    }
+}
+
+/* direct stage for a prime without a codelet: one output per step,
+   out[g Ns R + k + q Ns] = sum_r x[g Ns + k + r nR] w^{r (k + q Ns)},
+   w = e^{DIR 2 pi i / (Ns R)} */
+template <int DIR, typename C>
+__host__ __device__ void stage_direct(const C *__restrict__ x, C *__restrict__ y, const C *__restrict__ tw, int n, int R,
+                             int Ns, int t, int T)
+{
+    typedef decltype(x[0].x) Tr;
+    const int nR = n / R, NsR = Ns * R, tstep = n / NsR;
+    for (int o = t; o < n; o += T) {
+        const int g = o / NsR, rem = o - g * NsR;
+        const int q = rem / Ns, k = rem - q * Ns;
+        const int e = k + q * Ns;
+        const C *xi = x + g * Ns + k;
+        C acc = cx((Tr)0, (Tr)0);
+        int ex = 0;
+        for (int r = 0; r < R; ++r) {
+            acc = cadd(acc, twmul<DIR>(xi[r * nR], tw[ex * tstep]));
+            ex += e;
+            if (ex >= NsR) ex -= NsR;
+        }
+        y[o] = acc;
+    }
+}
+
+/* Full transform of a team's buffer x (scratch y of the same size); every
+ * thread of the workgroup calls this with the same plan (one barrier per
+ * stage).  t: thread index in the team, T: team size.  Returns the buffer
+ * holding the result (x or y). */
+template <int DIR, int MAXP = 47, typename C>
+__device__ C *transform(C *x, C *y, const C *tw, const FftPlan &pl, int t, int T)
+{
+    const int n = pl.n;
+    int Ns = 1;
+    for (int s = 0; s < pl.nst; ++s) {
+        const int R = pl.rad[s];
+        /* codelets above MAXP are not instantiated (register pressure of the
+           kernel): such factors take the direct stage */
+        switch (R > MAXP ? 0 : R) {
+        case 2: stage<2, DIR>(x, y, tw, n, Ns, t, T); break;
+        case 3: stage<3, DIR>(x, y, tw, n, Ns, t, T); break;
+        case 4: stage<4, DIR>(x, y, tw, n, Ns, t, T); break;
+        case 5: if constexpr (MAXP >= 5) stage<5, DIR>(x, y, tw, n, Ns, t, T); break;
+        case 7: if constexpr (MAXP >= 7) stage<7, DIR>(x, y, tw, n, Ns, t, T); break;
+        case 11: if constexpr (MAXP >= 11) stage<11, DIR>(x, y, tw, n, Ns, t, T); break;
+        case 13: if constexpr (MAXP >= 13) stage<13, DIR>(x, y, tw, n, Ns, t, T); break;
+        case 17: if constexpr (MAXP >= 17) stage<17, DIR>(x, y, tw, n, Ns, t, T); break;
+        case 19: if constexpr (MAXP >= 19) stage<19, DIR>(x, y, tw, n, Ns, t, T); break;
+        case 23: if constexpr (MAXP >= 23) stage<23, DIR>(x, y, tw, n, Ns, t, T); break;
+        case 29: if constexpr (MAXP >= 29) stage<29, DIR>(x, y, tw, n, Ns, t, T); break;
+        case 31: if constexpr (MAXP >= 31) stage<31, DIR>(x, y, tw, n, Ns, t, T); break;
+        case 37: if constexpr (MAXP >= 37) stage<37, DIR>(x, y, tw, n, Ns, t, T); break;
+        case 41: if constexpr (MAXP >= 41) stage<41, DIR>(x, y, tw, n, Ns, t, T); break;
+        case 43: if constexpr (MAXP >= 43) stage<43, DIR>(x, y, tw, n, Ns, t, T); break;
+        case 47: if constexpr (MAXP >= 47) stage<47, DIR>(x, y, tw, n, Ns, t, T); break;
+        default: stage_direct<DIR>(x, y, tw, n, R, Ns, t, T); break;
+        }
+        __syncthreads();
+        C *tmp = x;
+        x = y;
+        y = tmp;
+        Ns *= R;
+    }
+    return x;
+}
+
+} // namespace fftc

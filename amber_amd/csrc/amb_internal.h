@@ -19,6 +19,10 @@ struct RingGeom {
     double lo, hi;  /* [ULow, UHigh) in cells */
     int ng;         /* reduced grid side */
     int jcnt;       /* columns that can hold ring cells: min(ng/3, ceil(hi)) */
+    long s_lo, s_hi; /* a cell (ia, j) is in the ring iff s_lo <= ia^2 + j^2 < s_hi:
+                        the least s >= 1 with sqrt(s) >= ULow and the least s with
+                        sqrt(s) >= UHigh, sqrt in double as the reference's test
+                        (amber_funcs.c:588) evaluates it */
 };
 
 /* ---- FFT stage (amb_fft.hip) ---- */

@@ -10,22 +10,22 @@
  *             (cut_nan, :259-260), optional Blackman-Nuttall window
  *             (mfunc_winds.py:16-30, 94-101);
  *   transform fftshift(irfft2(ifftshift(X, axes=1), axes=(1,0), s=(N,N)))
- *             * rc^4 L^2, L = N dnu rc' (Bdnu2Bk_fft, :325-334), in FP64 as
- *             two direct-DFT matrix products (rocBLAS DGEMM on the FP64
- *             matrix cores):
- *             [Zr | Zi] = X [cos | sin] / N (the complex inverse DFT along the
- *             columns, e^{+2 pi i j k / N}), then
- *             y = [w cos | -w sin] [Zr ; Zi] / N (the half-spectrum inverse
- *             along the rows, w_0 = 1, w_m = 2).
- * N = W - 2n is odd (W = 2 nc10 + 1), so the row transform has no Nyquist
- * term; like numpy's c2r it uses only the real part of the m = 0 row.  The
- * twiddle matrices are built once per call from one table of e^{2 pi i t/N}
- * indexed by (j k) mod N.  Work: 8 M N^2 flop per triangle (N = 1519,
- * M = 760: 1.4e10).
+ *             * rc^4 L^2, L = N dnu rc' (Bdnu2Bk_fft, :325-334), in FP64 on
+ *             the LDS Stockham core (amb_fftcore.h):
+ *             Z[m][j] = sum_k X[m][k] e^{+2 pi i j k / N} (the complex inverse
+ *             DFT along the columns, two real rows per complex transform),
+ *             then y[t][j] = sum_m w_m Re(Z[m][j] e^{+2 pi i m t / N}) (the
+ *             half-spectrum inverse along the rows, w_0 = 1, w_m = 2, as a
+ *             length-N transform of the Hermitian extension, two columns per
+ *             complex transform), both divided by N.
+ * N = W - 2n is odd (W = 2 nc10 + 1, N = 2 M - 1), so the row transform has
+ * no Nyquist term; like numpy's c2r it uses only the real part of the
+ * m = 0 row.  At N = 1519 = 7^2 31 the radix-7 stages are register codelets
+ * and the radix-31 stage a direct one.  Nominal work 5 N log2 N flop per
+ * transform, ceil(M/2) + ceil(N/2) transforms per triangle.
  */
 #include "amb_ctx.h"
-
-#include <rocblas/rocblas.h>
+#include "amb_fftcore.h"
 
 #include <algorithm>
 #include <cmath>
@@ -63,77 +63,70 @@ __global__ void k_cyl_prep(const float *__restrict__ bl, const void *__restrict_
     X[(size_t)m * N + k] = v;
 }
 
-/* B1 = [cos | sin](2 pi (j k mod N) / N), N x 2N; A2 = [w cos | -w sin](2 pi
- * (m t mod N) / N), N x 2M (row t, column m), w_0 = 1, w_m = 2 */
-__global__ void k_cyl_tables(const double2 *__restrict__ tw, int N, int M, double *__restrict__ B1,
-                             double *__restrict__ A2)
+/* Z[m][j] = sum_k X[m][k] e^{+2 pi i j k / N} for rows m1 = 2 blockIdx.x
+   and m2 = m1 + 1 at once: z = X[m1] + i X[m2], T its inverse DFT, then
+   Z1(j) = (T(j) + conj T(N-j)) / 2, Z2(j) = (T(j) - conj T(N-j)) / 2i.
+   One 64-thread team; dynamic LDS 2 N complex double. */
+__global__ __launch_bounds__(64) void k_cyl_rows(const double *__restrict__ X, int M, FftPlan pl,
+                                                 const double2 *__restrict__ tw, double2 *__restrict__ Z)
 {
-    const int r = blockIdx.y, k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < N) {
-        const double2 t = tw[(int)(((long)r * k) % N)];
-        B1[(size_t)r * 2 * N + k] = t.x;
-        B1[(size_t)r * 2 * N + N + k] = t.y;
-    }
-    if (k < M) {
-        const double2 t = tw[(int)(((long)k * r) % N)];
-        const double w = k == 0 ? 1.0 : 2.0;
-        A2[(size_t)r * 2 * M + k] = w * t.x;
-        A2[(size_t)r * 2 * M + M + k] = -(w * t.y);
+    extern __shared__ __align__(16) double2 dsm[];
+    const int N = pl.n;
+    double2 *bx = dsm, *by = dsm + N;
+    const int m1 = 2 * blockIdx.x, m2 = m1 + 1;
+    const bool two = m2 < M;
+    for (int k = threadIdx.x; k < N; k += 64)
+        bx[k] = make_double2(X[(size_t)m1 * N + k], two ? X[(size_t)m2 * N + k] : 0.0);
+    __syncthreads();
+    const double2 *T = fftc::transform<+1, 19>(bx, by, tw, pl, threadIdx.x, 64);
+    for (int j = threadIdx.x; j < N; j += 64) {
+        const double2 a = T[j], b = T[j == 0 ? 0 : N - j];
+        Z[(size_t)m1 * N + j] = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
+        if (two) Z[(size_t)m2 * N + j] = make_double2(0.5 * (a.y + b.y), -0.5 * (a.x - b.x));
     }
 }
 
-/* [Zr ; Zi] (2M x N) from C1 = X [cos | sin] (M x 2N), divided by N */
-__global__ void k_cyl_stack(const double *__restrict__ C1, int N, int M, double *__restrict__ Z2)
+/* y[t][j] for columns ja = 2 blockIdx.x and jb = ja + 1: the Hermitian
+   extension Zh(m) = Z[m][j] (m < M; real part only at m = 0),
+   conj Z[N - m][j] (m >= M) of both columns as A + i B, one inverse DFT of
+   length N; written fftshifted on both axes, times scale / N^2 */
+__global__ __launch_bounds__(64) void k_cyl_cols(const double2 *__restrict__ Z, int M, FftPlan pl,
+                                                 const double2 *__restrict__ tw, double scale, double *__restrict__ out)
 {
-    const int m = blockIdx.y, k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= N) return;
-    Z2[(size_t)m * N + k] = C1[(size_t)m * 2 * N + k] / N;
-    Z2[(size_t)(M + m) * N + k] = C1[(size_t)m * 2 * N + N + k] / N;
-}
-
-/* y = C2 / N, written fftshifted on both axes and scaled */
-__global__ void k_cyl_out(const double *__restrict__ C2, int N, double scale, double *__restrict__ out)
-{
-    const int t = blockIdx.y, k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= N) return;
-    const int a = (t + N / 2) % N, b = (k + N / 2) % N; /* fftshift: out[a] = y[a - N/2] */
-    out[(size_t)a * N + b] = C2[(size_t)t * N + k] / N * scale;
+    extern __shared__ __align__(16) double2 dsm[];
+    const int N = pl.n;
+    double2 *bx = dsm, *by = dsm + N;
+    const int ja = 2 * blockIdx.x, jb = ja + 1 < N ? ja + 1 : ja;
+    for (int m = threadIdx.x; m < N; m += 64) {
+        double2 y;
+        if (m == 0) {
+            y = make_double2(Z[ja].x, Z[jb].x);
+        } else if (m < M) {
+            const double2 A = Z[(size_t)m * N + ja], B = Z[(size_t)m * N + jb];
+            y = make_double2(A.x - B.y, A.y + B.x);
+        } else {
+            const int mm = N - m;
+            const double2 A = Z[(size_t)mm * N + ja], B = Z[(size_t)mm * N + jb];
+            y = make_double2(A.x + B.y, B.x - A.y);
+        }
+        bx[m] = y;
+    }
+    __syncthreads();
+    const double2 *yv = fftc::transform<+1, 19>(bx, by, tw, pl, threadIdx.x, 64);
+    const double f = scale / ((double)N * (double)N);
+    const int ba = (ja + N / 2) % N, bb = (jb + N / 2) % N;
+    for (int t = threadIdx.x; t < N; t += 64) {
+        const size_t row = (size_t)((t + N / 2) % N) * N; /* fftshift: out[a] = y[a - N/2] */
+        out[row + ba] = yv[t].x * f;
+        if (jb != ja) out[row + bb] = yv[t].y * f;
+    }
 }
 
 } // namespace
 
 namespace amb {
 
-/* C[M][N] = A[M][K] B[K][N], row-major with leading dimensions: two plain
- * FP64 matrix products, on rocBLAS's gfx950 DGEMM (FP64 matrix cores).  In
- * rocBLAS's column-major terms the product is C^T = B^T A^T, so the call
- * passes B first with m = N and n = M.  The handle lives with the context
- * and runs on its stream.  A rocBLAS failure is returned as its status
- * name in *why. */
-static hipError_t dgemm(amb_ctx *c, hipStream_t s, const double *A, int lda, const double *B, int ldb, double *C,
-                        int ldc, int M, int N, int K, const char **why)
-{
-    rocblas_status r = rocblas_status_success;
-    if (!c->blas) {
-        rocblas_handle h = nullptr;
-        r = rocblas_create_handle(&h);
-        if (r == rocblas_status_success) c->blas = h;
-    }
-    if (r == rocblas_status_success) r = rocblas_set_stream((rocblas_handle)c->blas, s);
-    const double one = 1.0, zero = 0.0;
-    if (r == rocblas_status_success)
-        r = rocblas_dgemm((rocblas_handle)c->blas, rocblas_operation_none, rocblas_operation_none, N, M, K, &one, B,
-                          ldb, A, lda, &zero, C, ldc);
-    if (r == rocblas_status_success) return hipGetLastError();
-    *why = rocblas_status_to_string(r);
-    return hipErrorUnknown;
-}
-
-void post_release(amb_ctx *c)
-{
-    if (c->blas) rocblas_destroy_handle((rocblas_handle)c->blas);
-    c->blas = nullptr;
-}
+void post_release(amb_ctx *) {}
 
 } // namespace amb
 
@@ -149,34 +142,29 @@ extern "C" int amb_cyl_bispec(amb_ctx *c, int n, int W, const float *bl, const v
     const int nc10 = (W - 1) / 2, ncut = p->ncut;
     const int N = W - 2 * ncut, M = nc10 + 1 - ncut;
     if (ncut < 0 || M < 1) return fail(c, AMB_EARG, "cut %d leaves no lags of W=%d", ncut, W);
-    if (N > 32768) return fail(c, AMB_EARG, "N=%d exceeds the transform tables", N);
+    if (N > 2048) return fail(c, AMB_EARG, "N=%d exceeds the LDS transform size (2048)", N);
     if (!(p->mabs_norm != 0.0)) return fail(c, AMB_EARG, "mabs_norm is zero");
     const size_t WW = (size_t)W * W, MN = (size_t)M * N, NN = (size_t)N * N;
-    /* device scratch: bl, den, X, C1 = X [cos|sin], Z2 = [Zr;Zi], C2, out,
-       window, the two twiddle matrices, the twiddle table */
+    /* device scratch: bl, den, X, Z (complex, M x N), out (two slots), window */
     size_t bytes = 0;
     auto sz = [&](size_t b) { bytes += (b + 255) / 256 * 256; };
-    sz(sizeof(float) * WW); sz(sizeof(float) * WW); sz(sizeof(double) * MN); sz(sizeof(double) * 2 * MN);
-    sz(sizeof(double) * 2 * MN); sz(sizeof(double) * NN); sz(sizeof(double) * 2 * NN); sz(sizeof(double) * MN);
-    sz(sizeof(double) * 2 * NN); sz(sizeof(double) * 2 * MN); sz(sizeof(double2) * N);
-    void *d = nullptr;
-    int rc = dev_alloc(c, &d, bytes, "post-processing scratch");
+    sz(sizeof(float) * WW); sz(sizeof(float) * WW); sz(sizeof(double) * MN); sz(sizeof(double2) * MN);
+    sz(sizeof(double) * 2 * NN); sz(sizeof(double) * MN);
+    FftPlan pl;
+    fftc::make_plan(N, &pl);
+    double2 *d_tw = nullptr;
+    int rc = twiddles64(c, N, &d_tw);
     if (rc) return rc;
+    void *d = nullptr;
+    if ((rc = dev_alloc(c, &d, bytes, "post-processing scratch"))) return rc;
     char *q = (char *)d;
     auto take = [&](size_t b) { char *r = q; q += (b + 255) / 256 * 256; return r; };
     float *d_bl = (float *)take(sizeof(float) * WW);
     void *d_den = take(sizeof(float) * WW);
     double *d_X = (double *)take(sizeof(double) * MN);
-    double *d_C1 = (double *)take(sizeof(double) * 2 * MN);
-    double *d_Z2 = (double *)take(sizeof(double) * 2 * MN);
-    double *d_C2 = (double *)take(sizeof(double) * NN);
+    double2 *d_Z = (double2 *)take(sizeof(double2) * MN);
     double *d_out = (double *)take(sizeof(double) * 2 * NN); /* two slots */
     double *d_win = (double *)take(sizeof(double) * MN);
-    double *d_B1 = (double *)take(sizeof(double) * 2 * NN);
-    double *d_A2 = (double *)take(sizeof(double) * 2 * MN);
-    double2 *d_tw = (double2 *)take(sizeof(double2) * N);
-    std::vector<double2> tw(N);
-    for (int t = 0; t < N; ++t) tw[t] = make_double2(cos(2.0 * M_PI * t / N), sin(2.0 * M_PI * t / N));
     std::vector<double> win;
     if (p->window) { /* blackman_nuttall (mfunc_winds.py:16-30), folded half of the outer product */
         std::vector<double> w(N);
@@ -192,15 +180,8 @@ extern "C" int amb_cyl_bispec(amb_ctx *c, int n, int W, const float *bl, const v
     const double L = N * p->dnu * p->rc_prime;
     const double scale = pow(p->rc, 4) * L * L;
     hipStream_t st = c->stream;
-    const char *why = nullptr; /* a rocBLAS status, if that is what failed */
-    hipError_t e = hipMemcpyAsync(d_tw, tw.data(), sizeof(double2) * N, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess && p->window)
-        e = hipMemcpyAsync(d_win, win.data(), sizeof(double) * MN, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_cyl_tables, dim3((std::max(N, M) + 255) / 256, N), dim3(256), 0, st, d_tw, N, M, d_B1,
-                           d_A2);
-        e = hipGetLastError();
-    }
+    hipError_t e = hipSuccess;
+    if (p->window) e = hipMemcpyAsync(d_win, win.data(), sizeof(double) * MN, hipMemcpyHostToDevice, st);
     /* Triangle t uses slot t & 1: a pinned host block [bl | den | result] and
        a device result array. Its inputs are staged by host threads while
        the GPU runs triangle t - 1; its result goes to the host on the copy
@@ -218,7 +199,8 @@ extern "C" int amb_cyl_bispec(amb_ctx *c, int n, int W, const float *bl, const v
         char *h = (char *)c->hstage[t & 1];
         par_copy({{out + NN * t, h + 2 * in_b, sizeof(double) * NN}});
     };
-    const dim3 g1((N + 255) / 256, M), g2((N + 255) / 256, N);
+    const dim3 g1((N + 255) / 256, M);
+    const size_t lds = sizeof(double2) * 2 * (size_t)N;
     for (int t = 0; t < n && e == hipSuccess; ++t) {
         const int sl = t & 1;
         char *h = (char *)c->hstage[sl];
@@ -242,16 +224,11 @@ extern "C" int amb_cyl_bispec(amb_ctx *c, int n, int W, const float *bl, const v
             hipLaunchKernelGGL(k_cyl_prep<false>, g1, dim3(256), 0, st, d_bl, d_den, W, nc10, ncut,
                                (float)p->mabs_norm, win_p, d_X);
         timer_start(c, &c->st.ms_post);
-        e = dgemm(c, st, d_X, N, d_B1, 2 * N, d_C1, 2 * N, M, 2 * N, N, &why);
-        if (e == hipSuccess) {
-            hipLaunchKernelGGL(k_cyl_stack, g1, dim3(256), 0, st, d_C1, N, M, d_Z2);
-            e = dgemm(c, st, d_A2, 2 * M, d_Z2, N, d_C2, N, N, N, 2 * M, &why);
-        }
+        hipLaunchKernelGGL(k_cyl_rows, dim3((unsigned)((M + 1) / 2)), dim3(64), lds, st, d_X, M, pl, d_tw, d_Z);
+        hipLaunchKernelGGL(k_cyl_cols, dim3((unsigned)((N + 1) / 2)), dim3(64), lds, st, d_Z, M, pl, d_tw, scale,
+                           d_res);
+        e = hipGetLastError();
         timer_stop(c);
-        if (e == hipSuccess) {
-            hipLaunchKernelGGL(k_cyl_out, g2, dim3(256), 0, st, d_C2, N, scale, d_res);
-            e = hipGetLastError();
-        }
         if (e == hipSuccess) e = hipEventRecord(c->ev_red, st);
         if (e == hipSuccess) e = hipStreamWaitEvent(c->cstream, c->ev_red, 0);
         if (e == hipSuccess)
@@ -265,9 +242,9 @@ extern "C" int amb_cyl_bispec(amb_ctx *c, int n, int W, const float *bl, const v
     if (e == hipSuccess)
         for (int t = std::max(0, n - 2); t < n; ++t) drain(t);
     timers_collect(c);
-    c->st.flop_post += 8.0 * M * (double)N * N * n;
+    /* nominal 5 N log2 N flop per length-N transform */
+    c->st.flop_post += 5.0 * N * log2((double)N) * (double)((M + 1) / 2 + (N + 1) / 2) * n;
     dev_free(c, d, bytes);
-    if (e != hipSuccess)
-        return fail(c, AMB_EHIP, "lag-space post-processing: %s", why ? why : hipGetErrorString(e));
+    if (e != hipSuccess) return fail(c, AMB_EHIP, "lag-space post-processing: %s", hipGetErrorString(e));
     return AMB_OK;
 }

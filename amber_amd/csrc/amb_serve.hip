@@ -797,6 +797,8 @@ int amb_serve_start(amb_ctx *c, int max_active)
     Lock lk(c->mu);
     hipSetDevice(c->device);
     if (c->server) return fail(c, AMB_ESTATE, "serving already started");
+    if (c->prec == AMB_PREC_LAGFT)
+        return fail(c, AMB_ESTATE, "the serving queue batches contraction work items; the lag-Fourier form has none");
     Srv *s = new (std::nothrow) Srv();
     if (!s) return AMB_ENOMEM;
     s->c = c;

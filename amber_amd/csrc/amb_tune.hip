@@ -231,6 +231,8 @@ int amb_autotune(amb_ctx *c, const amb_tri_req *req, int n, double max_seconds, 
     if (!c || !req || n <= 0 || !(max_seconds > 0)) return AMB_EARG;
     Lock lk(c->mu);
     if (c->server) return fail(c, AMB_ESTATE, "tuning while the serving queue runs");
+    if (c->prec == AMB_PREC_LAGFT)
+        return fail(c, AMB_ESTATE, "the lag-Fourier form has no contraction launch configuration to tune");
     std::map<Key, Bucket> bk;
     for (int i = 0; i < n; ++i) {
         const amb_tri_req &q = req[i];

@@ -508,6 +508,7 @@ int amb_bispec_multi(amb_ctx *c, const amb_tri_req *req, int n)
     Lock lk(c->mu);
     const double t0 = now_ms();
     c->batch_ms.assign(n, 0.0);
+    if (c->prec == AMB_PREC_LAGFT) return lagft_run(c, req, n, t0, c->batch_ms.data(), nullptr);
     if (c->batch_mode == AMB_BATCH_PER_TRIANGLE) {
         std::vector<int> order(n);
         for (int i = 0; i < n; ++i) order[i] = i;
@@ -558,6 +559,8 @@ int amb_graph_create(amb_ctx *c, const amb_tri_req *req, int n, amb_graph **out)
     Lock lk(c->mu);
     hipSetDevice(c->device);
     *out = nullptr;
+    if (c->prec == AMB_PREC_LAGFT)
+        return fail(c, AMB_ESTATE, "the lag-Fourier form runs its own waves; it is not captured in graphs");
     AmbGraph *g = new (std::nothrow) AmbGraph();
     if (!g) return AMB_ENOMEM;
     g->req.assign(req, req + n);

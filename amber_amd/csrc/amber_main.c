@@ -41,7 +41,7 @@ static void usage(void)
 {
     fprintf(stderr,
             "Usage: %s <Rlzn> [--params FILE] [--rings FILE] [--nr N] [--nc10 N] [--uf-nc N]\n"
-            "          [--rp-start N] [--rp-end N] [--precision fp32|bf16x3|fp16x3] [--device N]\n"
+            "          [--rp-start N] [--rp-end N] [--precision fp32|bf16x3|fp16x3|lagft] [--device N]\n"
             "          [--calcr-plane N] [--res-path DIR] [--wire legacy|hardened|both]\n"
             "          [--cert exact|strict] [--tune FILE]\n",
             PROG);
@@ -107,6 +107,7 @@ int main(int argc, char **argv)
             if (!strcmp(v, "fp32")) prec = AMB_PREC_FP32;
             else if (!strcmp(v, "bf16x3")) prec = AMB_PREC_BF16X3;
             else if (!strcmp(v, "fp16x3")) prec = AMB_PREC_FP16X3;
+            else if (!strcmp(v, "lagft")) prec = AMB_PREC_LAGFT;
             else { usage(); return 1; }
         } else if (!strcmp(a, "--wire")) {
             if (!strcmp(v, "legacy")) { wire_legacy = 1; wire_hardened = 0; }

@@ -39,8 +39,12 @@ CONFIGS = {
 
 # published dense MFMA peaks of the instruction each form issues; one
 # algorithmic BF16x3 flop costs three BF16 MFMA flops (mfma_passes)
-PEAK_TFLOPS = {"bf16x3": 2500.0, "fp16x3": 2500.0, "fp32": 157.3, "fp8": 5000.0}
-PASSES = {"bf16x3": 3, "fp16x3": 3, "fp32": 1, "fp8": 1}
+PEAK_TFLOPS = {"bf16x3": 2500.0, "fp16x3": 2500.0, "fp32": 157.3, "fp8": 5000.0, "lagft": 157.3}
+PASSES = {"bf16x3": 3, "fp16x3": 3, "fp32": 1, "fp8": 1, "lagft": 1}
+# the lag-Fourier form's products run on the vector ALUs in FP32: the
+# published FP32 vector peak (157.3 TFLOP/s) counts packed FP32 pairs, which
+# this build does not issue; one FP32 FMA per lane and clock is half of it
+VALU_FP32_UNPACKED_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
 # launch-configuration table tuned on this workload on an MI355X (loaded
 # by default when present)
@@ -66,12 +70,14 @@ def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
-def tri_costs(A, tris, hi):
+def tri_costs(A, tris, hi, precision="bf16x3"):
     """Relative contraction cost of each triangle: Ng(r1)^2 pixels times the
     triplets it computes (half of them where r1 == r2 or r2 == r3 lets the
-    kernel mirror the other half)."""
+    kernel mirror the other half; the lag-Fourier form has no such halving)."""
     ng = np.array([A.ng(hi[t[0]]) for t in tris], np.float64)
     sym = np.array([(t[0] == t[1]) or (t[1] == t[2]) for t in tris])
+    if precision == "lagft":
+        return ng * ng
     return ng * ng * np.where(sym, 0.5, 1.0)
 
 
@@ -107,7 +113,7 @@ def main():
     ap.add_argument("--duration", type=float, default=90.0, help="cfg5: seconds of arrivals")
     ap.add_argument("--probe-s", type=float, default=20.0, help="cfg5: seconds of the one-at-a-time capacity probe")
     ap.add_argument("--upload-frac", type=float, default=0.1, help="cfg5: fraction of requests that carry a cube")
-    ap.add_argument("--precision", default="bf16x3", choices=["bf16x3", "fp16x3", "fp32", "fp8"])
+    ap.add_argument("--precision", default="bf16x3", choices=["bf16x3", "fp16x3", "fp32", "fp8", "lagft"])
     ap.add_argument("--tri-limit", type=int, default=0,
                     help="development only: K triangles (the line is then not the metric workload)")
     ap.add_argument("--tri-pick", default="first", choices=["first", "largest"])
@@ -170,7 +176,7 @@ def main():
     log("uploaded cube (%.2f GB of FP64 file data) in %.2f s" % ((gv.nbytes + samp.nbytes) / 1e9, upload_s))
 
     W = 2 * nc10 + 1
-    costs = tri_costs(A, tris, hi)
+    costs = tri_costs(A, tris, hi, args.precision)
     # the sample a missing table is tuned on: up to 24 triangles evenly
     # spread over the list ordered by cost, so every Ng bucket is present
     by_cost = np.argsort(costs, kind="stable")
@@ -222,14 +228,18 @@ def main():
     # the library's stream; work is counted once per distinct sum (a pair
     # of triplets that r1 == r2 or r2 == r3 makes equal counts once)
     ms_c = st["ms_contract"]
-    ach = st["flop_contract_distinct"] / (ms_c * 1e-3) / 1e12 if ms_c > 0 else 0.0
+    lagft = args.precision == "lagft"
+    work = st["flop_lagft"] if lagft else st["flop_contract_distinct"]
+    ach = work / (ms_c * 1e-3) / 1e12 if ms_c > 0 else 0.0
     peak = PEAK_TFLOPS[args.precision]
-    roof = {"bound": "mfma", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
+    roof = {"bound": "valu" if lagft else "mfma", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
             "frac": round(ach / peak, 4),
             "traffic": traffic_per_launch(args, st, nsteps),
-            "kernel": "k_contract<%s>" % args.precision,
-            "work": "2 flop per (distinct triplet, pixel): 2*V*Ng^2 per triangle, halved (plus the self-mirrored "
-                    "triplets) where r1 == r2 or r2 == r3",
+            "kernel": "k_lagft_tiles" if lagft else "k_contract<%s>" % args.precision,
+            "work": ("14 flop per (k2, k3, pixel) of the half spectrum k3 <= L/2 (one complex triple product "
+                     "conj(F) G H), L = channel transform length" if lagft else
+                     "2 flop per (distinct triplet, pixel): 2*V*Ng^2 per triangle, halved (plus the self-mirrored "
+                     "triplets) where r1 == r2 or r2 == r3"),
             "mfma_passes": PASSES[args.precision],
             "frac_of_pass_peak": round(ach * PASSES[args.precision] / peak, 4),
             "avg_launch_ms": round(ms_c / max(st["n_contract_launch"], 1), 3),
@@ -239,7 +249,11 @@ def main():
             # skipped tiles excluded) against the dense peak of the instruction
             "issued_tflops": round(st["flop_contract_exec"] / (ms_c * 1e-3) / 1e12, 2) if ms_c > 0 else None,
             "issued_frac_of_instr_peak": round(st["flop_contract_exec"] / (ms_c * 1e-3) / 1e12 /
-                                               peak, 4) if ms_c > 0 else None}
+                                               (VALU_FP32_UNPACKED_TFLOPS if lagft else peak), 4) if ms_c > 0 else None}
+    if lagft:
+        roof["frac_of_unpacked_fp32_valu"] = round(ach / VALU_FP32_UNPACKED_TFLOPS, 4)
+        roof["direct_sum_equivalent_tflops"] = round(st["flop_contract_distinct"] / (ms_c * 1e-3) / 1e12, 2) \
+            if ms_c > 0 else None
     stages = {"fft_ms": round(st["ms_fft"], 2), "contract_ms": round(st["ms_contract"], 2),
               "norm_ms": round(st["ms_norm"], 2), "reduce_ms": round(st["ms_reduce"], 2),
               "fft_gbs": round(st["bytes_fft"] / (st["ms_fft"] * 1e-3) / 1e9, 1) if st["ms_fft"] > 0 else None}
@@ -263,7 +277,7 @@ def main():
         "value": round(value, 1), "unit": "evals/s", "n_gpus": 1, "steps": nsteps, "warmup": nwarm,
         "ms_per_step": round(1e3 * elapsed / nsteps, 2), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
-        "dtype": {"bf16x3": "bf16", "fp16x3": "fp16", "fp32": "f32", "fp8": "fp8_e4m3"}[args.precision],
+        "dtype": {"bf16x3": "bf16", "fp16x3": "fp16", "fp32": "f32", "fp8": "fp8_e4m3", "lagft": "f32"}[args.precision],
         "precision": args.precision, "data": "synthetic",
         "config": {"workload": workload, "estimator": args.estimator, "channels": nc, "grid": N,
                    "bins": cfg["rings"][2],
@@ -563,8 +577,9 @@ def bench_cfg5(A, args):
 
 def bench_post(A, args):
     """Lag-space post-processing (get_cyl_BS) of cfg3-sized result arrays
-    (W = 1535, N = 1519 after the 8-lag cut): triangles/s and the FP64 rate
-    of the two direct-sum transforms (8 M N^2 flop per triangle)."""
+    (W = 1535, N = 1519 after the 8-lag cut): triangles/s, and the HBM rate
+    of the two FP64 FFT passes of the 2-D inverse transform (algorithmic
+    bytes: the M x N real input read and the N x N output written)."""
     nc10, n = 767, 32
     W = 2 * nc10 + 1
     rng = np.random.default_rng(6001)
@@ -581,14 +596,17 @@ def bench_post(A, args):
     st = ctx.stats()
     N = W - 2 * A.ANALYSIS["ncut"]
     M = nc10 + 1 - A.ANALYSIS["ncut"]
-    fl = 8.0 * M * N * N * n
-    kt = st["flop_post"] / (st["ms_post"] * 1e-3) / 1e12
+    byt = (8.0 * M * N + 8.0 * N * N) * n * steps
+    gbs = byt / (st["ms_post"] * 1e-3) / 1e9
     line = {"metric": "lag-space post-processing (fold, normalise, window, lag->k_par transform)",
             "value": round(n / dt, 2), "unit": "triangles/s", "triangles": n, "W": W, "N": N,
-            "s_per_batch": round(dt, 4), "fp64_tflops_wall": round(fl / dt / 1e12, 2),
-            "roofline": {"bound": "mfma", "kernel": "rocBLAS DGEMM (two per triangle)", "achieved": round(kt, 2),
-                         "peak": 78.6, "unit": "TFLOP/s", "frac": round(kt / 78.6, 4)},
-            "ms_transform_per_triangle": round(st["ms_post"] / steps / n, 3),
+            "s_per_batch": round(dt, 4),
+            "roofline": {"bound": "hbm", "kernel": "k_cyl_rows + k_cyl_cols (FP64 FFT passes)",
+                         "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(gbs / HBM_PEAK_GBS, 4),
+                         "work": "8 B per input lag cell (M x N) read + 8 B per output cell (N x N) written",
+                         "nominal_fft_gflops": round(st["flop_post"] / (st["ms_post"] * 1e-3) / 1e9, 1)},
+            "ms_transform_per_triangle": round(st["ms_post"] / steps / n, 4),
             "note": "host->device copy of the W x W arrays and device->host copy of the N x N results included"}
     ctx.close()
     print(json.dumps(line), flush=True)

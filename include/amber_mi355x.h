@@ -43,6 +43,14 @@ extern "C" {
                              scaled by powers of two from each ring's plane-set maximum
                              so that FP16's range holds them (22-bit operands, against
                              16 bits for BF16x3) */
+#define AMB_PREC_LAGFT 4  /* lag-Fourier form: the sum over reference planes as a
+                             triple correlation along the channel axis, evaluated in
+                             the DFT domain of the channel axis (FP32 products of
+                             channel spectra on the vector ALUs, FP64 segment sums and
+                             2-D inverse); L^2/2 complex products per pixel instead of
+                             nc^3 real ones.  Batches run as waves of their own; not
+                             available in captured graphs or the serving queue.  The
+                             v2 normalisation of this form is the exact FP64 one. */
 
 const char *amb_strerror(int code);
 const char *amb_version(void);
@@ -522,6 +530,9 @@ typedef struct amb_stats {
     int64_t n_evict_cubes;  /* cubes carried by requests evicted (LRU, idle) for HBM budget */
     int64_t n_cube_uploads; /* cubes carried by requests uploaded by the serving queue */
     int64_t n_serve_iters;  /* contraction iterations the serving queue launched */
+    double flop_lagft;      /* lag-Fourier form: 14 flops per (k2, k3, pixel) of the half
+                               spectrum k3 <= L/2, the algorithmic work of its products
+                               (ms_contract times them) */
 } amb_stats;
 int amb_get_stats(amb_ctx *ctx, amb_stats *st);
 int amb_reset_stats(amb_ctx *ctx);

@@ -1,0 +1,52 @@
+/*
+ * Host build of the LDS FFT core's stages (amber_amd/csrc/amb_fftcore.h):
+ * runs a team's T threads one after another per stage, the order a
+ * workgroup barrier separates on the device, so tests can compare every
+ * plan against numpy without a GPU.
+ */
+#include "amb_fftcore.h"
+
+#include <cmath>
+#include <vector>
+
+template <typename T, int DIR>
+static int run(int n, int TT, const double *in, double *out)
+{
+    typedef typename fftc::CT<T>::c C;
+    FftPlan pl;
+    fftc::make_plan(n, &pl);
+    std::vector<C> x(n), y(n), tw(n);
+    for (int m = 0; m < n; ++m) {
+        const double a = 2.0 * M_PI * (double)m / (double)n;
+        tw[m] = fftc::cx((T)cos(a), (T)sin(a));
+        x[m] = fftc::cx((T)in[2 * m], (T)in[2 * m + 1]);
+    }
+    C *px = x.data(), *py = y.data();
+    int Ns = 1;
+    for (int s = 0; s < pl.nst; ++s) {
+        const int R = pl.rad[s];
+        for (int t = 0; t < TT; ++t) {
+            switch (R) {
+#define CASE(r) case r: fftc::stage<r, DIR>(px, py, tw.data(), n, Ns, t, TT); break;
+            CASE(2) CASE(3) CASE(4) CASE(5) CASE(7) CASE(11) CASE(13) CASE(17) CASE(19) CASE(23) CASE(29)
+            CASE(31) CASE(37) CASE(41) CASE(43) CASE(47)
+#undef CASE
+            default: fftc::stage_direct<DIR>(px, py, tw.data(), n, R, Ns, t, TT); break;
+            }
+        }
+        C *tmp = px; px = py; py = tmp;
+        Ns *= R;
+    }
+    for (int m = 0; m < n; ++m) {
+        out[2 * m] = (double)px[m].x;
+        out[2 * m + 1] = (double)px[m].y;
+    }
+    return pl.nst;
+}
+
+extern "C" int fftc_host(int n, int dir, int dbl, int team, const double *in, double *out)
+{
+    if (n < 1 || team < 1) return -1;
+    if (dbl) return dir > 0 ? run<double, 1>(n, team, in, out) : run<double, -1>(n, team, in, out);
+    return dir > 0 ? run<float, 1>(n, team, in, out) : run<float, -1>(n, team, in, out);
+}

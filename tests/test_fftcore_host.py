@@ -1,0 +1,69 @@
+"""The LDS FFT core (amber_amd/csrc/amb_fftcore.h) on the host: every
+stage and butterfly of the device code, with a team's threads run one after
+another per stage, against numpy's FFT and, for small n, a direct DFT.
+
+Lengths cover each register codelet (2, 3, 4 and the odd primes up to 47),
+the direct stage of larger primes (109, 181, 349), the transform lengths the
+ring c2r meets at the configured grids (612, 362, 218, 1176, ...), the
+lag-Fourier length 1536 and the post-processing length 1519 = 7^2 31, with
+team sizes that do not divide the butterfly counts."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+SRC = os.path.join(HERE, "native", "fftcore_host.cpp")
+
+
+@pytest.fixture(scope="module")
+def lib(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("fftc") / "libfftcore_host.so")
+    subprocess.run(["/opt/rocm/bin/hipcc", "--offload-host-only", "-O1", "-fPIC", "-shared", "-std=c++17",
+                    "-I" + os.path.join(ROOT, "amber_amd", "csrc"), "-I" + os.path.join(ROOT, "include"), SRC,
+                    "-o", out], check=True, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    L = C.CDLL(out)
+    L.fftc_host.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+    return L
+
+
+def run(lib, x, direction, dbl, team):
+    n = len(x)
+    a = np.empty(2 * n)
+    a[0::2], a[1::2] = x.real, x.imag
+    o = np.zeros(2 * n)
+    assert lib.fftc_host(n, direction, dbl, team, a.ctypes.data, o.ctypes.data) > 0 or n == 1
+    return o[0::2] + 1j * o[1::2]
+
+
+LENGTHS = [2, 3, 4, 5, 7, 8, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 60, 86, 109, 128, 218, 258, 362,
+           396, 470, 492, 612, 698, 1176, 1519, 1536]
+
+
+@pytest.mark.parametrize("n", LENGTHS)
+@pytest.mark.parametrize("dbl", [1, 0])
+def test_against_numpy(lib, n, dbl):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    tol = 1e-13 if dbl else 2e-6
+    for team in (64, 256, 7):
+        inv = run(lib, x, +1, dbl, team)  # sum x_m e^{+2 pi i m k / n}
+        fwd = run(lib, x, -1, dbl, team)
+        ref_inv = np.fft.ifft(x) * n
+        ref_fwd = np.fft.fft(x)
+        assert np.linalg.norm(inv - ref_inv) <= tol * np.linalg.norm(ref_inv), (n, team)
+        assert np.linalg.norm(fwd - ref_fwd) <= tol * np.linalg.norm(ref_fwd), (n, team)
+
+
+@pytest.mark.parametrize("n", [5, 7, 17, 30, 47, 53])
+def test_against_direct_dft(lib, n):
+    """independent of numpy: the defining sum, evaluated in FP64"""
+    rng = np.random.default_rng(100 + n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    m = np.arange(n)
+    E = np.exp(2j * np.pi * np.outer(m, m) / n)
+    assert np.allclose(run(lib, x, +1, 1, 64), E @ x, rtol=0, atol=1e-12 * np.abs(x).sum())
+    assert np.allclose(run(lib, x, -1, 1, 64), np.conj(E) @ x, rtol=0, atol=1e-12 * np.abs(x).sum())

@@ -25,7 +25,7 @@ def big():
     return gv, samp, {}
 
 
-@pytest.mark.parametrize("prec", ["bf16x3", "fp16x3", "fp32", "fp8"])
+@pytest.mark.parametrize("prec", ["bf16x3", "fp16x3", "fp32", "fp8", "lagft"])
 def test_fullsize_counts_and_scaling(ctx, big, prec):
     gv, samp, cache = big
     lo, hi = A.log_rings(1, 200, 24)
@@ -70,7 +70,7 @@ def test_fullsize_counts_and_scaling(ctx, big, prec):
 
 
 # normwise bar at the metric workload, every one of its 768 channels
-FULL_TOL = {"fp16x3": 1e-5, "fp32": 1e-5, "bf16x3": 1e-5}
+FULL_TOL = {"fp16x3": 1e-5, "fp32": 1e-5, "bf16x3": 1e-5, "lagft": 1e-5}
 
 
 @pytest.fixture(scope="module")
@@ -95,7 +95,7 @@ def metric_cells(big):
     return lo, hi, out
 
 
-@pytest.mark.parametrize("prec", ["fp16x3", "fp32", "bf16x3"])
+@pytest.mark.parametrize("prec", ["fp16x3", "fp32", "bf16x3", "lagft"])
 def test_metric_workload_error_vs_fp64(ctx, big, metric_cells, prec):
     """The contraction's error against the FP64 oracle at the metric
     workload (768 ch x 512^2, all lags), on the Ng = 320 and Ng = 612 bins

@@ -1,0 +1,102 @@
+"""The lag-Fourier form (precision "lagft") against the FP64 oracle.
+
+Each lag cell of the reference's sum (amber.c:233-283) is computed here as
+the 2-D inverse DFT of S(k2, k3) = sum_x conj F(k2 + k3) G(k2) H(k3) over
+the channel spectra, so the tests pin what the direct sum fixes: every cell
+against the oracle on small cubes (flagged channels, partial lag windows,
+odd grids, r1 == r2 and r2 == r3 triangles), counts bit for bit, cells
+without triplets exactly zero, batches equal to single triangles bit for
+bit, the v2 normalisation exact, and the forms it does not serve refused."""
+import numpy as np
+import pytest
+
+import amber_amd as A
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    # N, nc, nflag, nc10, rings (lo, hi, n), seed
+    (64, 24, 3, 23, (2, 20, 6), 5),
+    (64, 24, 3, 9, (2, 20, 6), 6),
+    (96, 40, 0, 39, (2, 30, 5), 7),
+    (128, 64, 6, 63, (2, 60, 8), 1001),
+]
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_cells_vs_oracle(ctx, case):
+    N, nc, nflag, nc10, rings, seed = case
+    lo, hi = A.log_rings(*rings)
+    gv, samp, uf = A.generate(N, nc, nflag=nflag, seed=seed)
+    fk = 1 if nflag else 0
+    mask = O.chan_mask(nc, fk, uf)
+    tris = A.triangles(lo, hi)
+    pick = [0, len(tris) // 2, len(tris) - 1]
+    sym = [i for i, t in enumerate(tris) if t[0] == t[1] and t[1] != t[2]][:1] + \
+          [i for i, t in enumerate(tris) if t[1] == t[2] and t[0] != t[1]][:1]
+    ctx.set_precision("lagft")
+    try:
+        ctx.set_rings(lo, hi)
+        cube = ctx.load_cube(N, nc, 1.0, fk, gv, samp)
+        ctx.set_channels(cube, uf if fk else None, nc10)
+        for k in sorted(set(pick + sym)):
+            t = tuple(int(x) for x in tris[k])
+            print("case lagft N=%d nc=%d nc10=%d triangle %s" % (N, nc, nc10, t), flush=True)
+            res = ctx.bispec_ex(cube, *t)
+            b64, c64, _ = O.bispec(N, nc, 1.0, gv, samp, lo, hi, *t, mask, nc10)
+            assert np.array_equal(res["cnt"], c64)
+            assert np.all(res["bl"][c64 == 0] == 0.0) and np.all(res["bl64"][c64 == 0] == 0.0)
+            e = np.linalg.norm(res["bl64"] - b64) / np.linalg.norm(b64)
+            e32 = np.linalg.norm(res["bl"] - b64) / np.linalg.norm(b64)
+            print("  normwise %.3g (f32 out %.3g)" % (e, e32), flush=True)
+            assert e <= 2e-6 and e32 <= 2e-6, (t, e, e32)
+        ctx.release_cube(cube)
+    finally:
+        ctx.set_precision("bf16x3")
+
+
+def test_batch_equals_single_and_v2_norm_exact(ctx):
+    N, nc, nc10 = 64, 24, 23
+    lo, hi = A.log_rings(2, 20, 6)
+    gv, samp, uf = A.generate(N, nc, nflag=3, seed=8)
+    mask = O.chan_mask(nc, 1, uf)
+    tris = A.triangles(lo, hi)
+    ctx.set_precision("lagft")
+    try:
+        ctx.set_rings(lo, hi)
+        cube = ctx.load_cube(N, nc, 1.0, 1, gv, samp)
+        ctx.set_channels(cube, uf, nc10)
+        bl, cnt, norm = ctx.bispec_batch(cube, tris, want_norm=True)
+        for k in (0, len(tris) - 1):
+            t = tuple(int(x) for x in tris[k])
+            one = ctx.bispec_ex(cube, *t, want_norm=True)
+            assert np.array_equal(one["bl"], bl[k]) and np.array_equal(one["cnt"], cnt[k])
+            _, _, n64 = O.bispec(N, nc, 1.0, gv, samp, lo, hi, *t, mask, nc10, want_norm=True)
+            P = one["ng"] ** 2
+            snapped = P * np.rint(n64 / P * 8) / 8
+            assert np.array_equal(one["norm64"], snapped)
+            assert np.array_equal(norm[k], snapped.astype(np.float32))
+        ctx.release_cube(cube)
+    finally:
+        ctx.set_precision("bf16x3")
+
+
+def test_refused_in_graphs_and_serving(ctx):
+    N, nc = 32, 8
+    lo, hi = A.log_rings(2, 10, 4)
+    gv, samp, _ = A.generate(N, nc, seed=2)
+    ctx.set_precision("lagft")
+    try:
+        ctx.set_rings(lo, hi)
+        cube = ctx.load_cube(N, nc, 1.0, 0, gv, samp)
+        ctx.set_channels(cube, None, nc - 1)
+        t = tuple(int(x) for x in A.triangles(lo, hi)[0])
+        with pytest.raises(A.AmberError):
+            ctx.graph([(cube,) + t])
+        with pytest.raises(A.AmberError):
+            ctx.serve_start(4)
+        ctx.bispec_ex(cube, *t)  # the context stays usable
+        ctx.release_cube(cube)
+    finally:
+        ctx.set_precision("bf16x3")
