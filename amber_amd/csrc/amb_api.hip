@@ -395,6 +395,7 @@ hipError_t launch_build(amb_ctx *c, hipStream_t s, const BuildOp &op, float2 *G)
     case 5:
         return amb_launch_fp8_scales(s, (const float *)op.src, pitch, nact, (float *)op.dst);
     case 9:
+    case 10:
         return lag_launch_spectra(c, s, op);
     case 8:
         return amb_launch_ring_c2r64(s, cb.samp_t, cb.d_act, nact, (int)cb.N, (int)(cb.N / 2 + 1),
@@ -412,8 +413,8 @@ hipError_t launch_build(amb_ctx *c, hipStream_t s, const BuildOp &op, float2 *G)
 double build_fft_bytes(amb_ctx *c, const BuildOp &op)
 {
     const Cube &cb = c->cubes[op.cube];
-    if (op.kind == 9) /* the FP32 set read, the half spectra written */
-        return (double)op.g.ng * op.g.ng * (4.0 * cb.nact() + 8.0 * (cb.lag_L / 2 + 1));
+    if (op.kind >= 9) /* the image set read, the half spectra written */
+        return (double)op.g.ng * op.g.ng * (op.kind == 10 ? 2.0 : 1.0) * (4.0 * cb.nact() + 8.0 * (cb.lag_L / 2 + 1));
     if (op.kind > 1) return 0.0;
     const RingGeom &g = op.g;
     double cells = 0;
@@ -448,21 +449,22 @@ int get_planes(amb_ctx *c, const PlaneSrc &src, int cube_id, int kind, int r, in
     if (kind == 2 || kind == 3) {
         if ((rc = get_planes(c, src, cube_id, kind - 2, r, ng, &dep))) return rc;
     } else if (kind == 8) {
-        /* two FP64 transform buffers and twiddles in 64 KiB of LDS */
-        if (48L * ng > 65536) return fail(c, AMB_EARG, "grid %d exceeds the FP64 LDS transform size (1365)", ng);
+        /* two FP64 transform buffers in 64 KiB of LDS */
+        if (32L * ng > 65536) return fail(c, AMB_EARG, "grid %d exceeds the FP64 LDS transform size (2048)", ng);
         float2 *tw;
         FftPlan pl;
         double2 *tw64;
         if ((rc = twiddles(c, ng, &tw, &pl)) || (rc = twiddles64(c, ng, &tw64))) return rc;
-    } else if (kind == 9) {
-        /* channel spectra of the data set: its FP32 images and the channel
-           transform's twiddles */
+    } else if (kind == 9 || kind == 10) {
+        /* channel spectra of the data set (FP32 images) or of the sampling
+           set (FP64 images), with the channel transform's twiddles */
         if (src.record || src.bump) return fail(c, AMB_EARG, "lag-Fourier planes are not built inside graphs");
         if ((rc = lag_setup(c, *cb))) return rc;
-        if ((rc = get_planes(c, src, cube_id, 0, r, ng, &dep))) return rc;
+        if ((rc = get_planes(c, src, cube_id, kind == 9 ? 0 : 8, r, ng, &dep))) return rc;
         float2 *tw;
         FftPlan pl;
         if ((rc = twiddles(c, cb->lag_L, &tw, &pl))) return rc;
+        if (kind == 10 && (rc = twiddles64(c, cb->lag_L, nullptr))) return rc;
     } else if (kind >= 4) {
         if ((rc = get_planes(c, src, cube_id, 0, r, ng, &dep))) return rc;
         if (kind == 4 && (rc = get_planes(c, src, cube_id, 5, r, ng, &sc))) return rc;
@@ -476,7 +478,7 @@ int get_planes(amb_ctx *c, const PlaneSrc &src, int cube_id, int kind, int r, in
     }
     op.src = dep;
     op.sc = sc;
-    const size_t bytes = kind == 9 ? lag_spectra_bytes(*cb, ng) : plane_bytes(kind, cb->nact(), plane_pitch(ng));
+    const size_t bytes = kind >= 9 ? lag_spectra_bytes(*cb, ng, kind == 10) : plane_bytes(kind, cb->nact(), plane_pitch(ng));
     void *d = nullptr;
     if (src.bump) {
         const size_t need = (bytes + 255) / 256 * 256;
@@ -495,13 +497,13 @@ int get_planes(amb_ctx *c, const PlaneSrc &src, int cube_id, int kind, int r, in
             plane_free(c, d, bytes);
             return rc;
         }
-        if (kind <= 1 || kind == 9) {
+        if (kind <= 1 || kind >= 9) {
             timer_start(c, &c->st.ms_fft);
-            c->st.n_fft_launch += kind == 9 ? 1 : 2;
+            c->st.n_fft_launch += kind >= 9 ? 1 : 2;
             c->st.bytes_fft += build_fft_bytes(c, op);
         }
         hipError_t e = launch_build(c, c->stream, op, c->G);
-        if (kind <= 1 || kind == 9) timer_stop(c);
+        if (kind <= 1 || kind >= 9) timer_stop(c);
         if (e != hipSuccess) {
             plane_free(c, d, bytes);
             return fail(c, AMB_EHIP, "plane build (kind %d): %s", kind, hipGetErrorString(e));
@@ -785,13 +787,14 @@ int bispec_core(amb_ctx *c, int id, int r1, int r2, int r3, int want_norm, amb_r
         q.bl = res->bl;
         if (!q.bl) { bl.resize(W * W); q.bl = bl.data(); }
         q.cnt = res->cnt;
-        double *b64[1] = {res->bl64};
-        if ((rc = lagft_run(c, &q, 1, 0.0, nullptr, b64))) return rc;
-        if (want_norm) { /* this form's v2 normalisation: the exact FP64 one */
-            float m = 0.f;
-            rc = norm_exact(c, id, r1, r2, r3, res->norm, res->norm64, &m);
-            res->cert_margin = m;
+        std::vector<float> nb;
+        if (want_norm) {
+            q.norm = res->norm;
+            if (!q.norm) { nb.resize(W * W); q.norm = nb.data(); }
         }
+        double *b64[1] = {res->bl64}, *n64[1] = {want_norm ? res->norm64 : nullptr};
+        rc = lagft_run(c, &q, 1, 0.0, nullptr, b64, n64);
+        res->cert_margin = want_norm ? c->lag_margin : 0.f;
         return rc;
     }
     TriJob j;
@@ -924,8 +927,8 @@ int closed_rings(amb_ctx *c, int id, int plane, double *rdata, int cap, int *rco
     amb_triangles(c->uavg.data(), nr, tri.data(), ntri);
     *rcount = ntri;
     if (ntri == 0) return AMB_OK;
-    /* twiddles and two transform buffers of N complex doubles in 64 KiB of LDS */
-    if (48L * N > 65536) return fail(c, AMB_EARG, "grid %d exceeds the FP64 LDS transform size (1365)", N);
+    /* two transform buffers of N complex doubles in 64 KiB of LDS */
+    if (32L * N > 65536) return fail(c, AMB_EARG, "grid %d exceeds the FP64 LDS transform size (2048)", N);
     FftPlan pl;
     amb_make_plan(N, &pl);
     int jmax = 0;
@@ -1085,7 +1088,8 @@ void amb_destroy(amb_ctx *c)
     }
     hipFree(c->G); hipFree(c->C); hipFree(c->Cq); hipFree(c->margin); hipFree(c->out); hipFree(c->out2);
     hipFree(c->Cqx); hipFree(c->outx);
-    hipFree(c->lag_part); hipFree(c->lag_st); hipFree(c->lag_out);
+    hipFree(c->lag_part); hipFree(c->lag_st); hipFree(c->lag_out); hipFree(c->lag_out2);
+    for (auto &kv : c->lag_pairs_tab) hipFree(std::get<0>(kv.second));
     hipFree(c->wdesc);
     for (void *h : c->hstage)
         if (h) hipHostFree(h);

@@ -60,7 +60,8 @@ struct Cube {
  * 7: hi and lo FP16 halves of kind 0 times 2^8, stored hi planes then lo,
  * 8: FP64 sampling images (the exact v2 normalisation),
  * 9: channel spectra of kind 0 (lag-Fourier form): [Ng^2][lag_lhp] complex
- *    FP32, the DFT over channels of each pixel, then the [max, E] words.
+ *    FP32, the DFT over channels of each pixel, then the [max, E] words,
+ * 10: channel spectra of kind 8 (the lag-Fourier normalisation), complex FP64.
  * Kind 0 sets are stored normalised by a power of two (max |D| <= 2^7) and
  * end in two words: max |D| as computed and its exponent E. */
 struct PlaneSet {
@@ -79,7 +80,7 @@ struct BuildOp {
     int kind = 0, cube = 0;
     RingGeom g{};
     void *dst = nullptr;
-    const void *src = nullptr; /* kinds 2-7, 9: the FP32 images */
+    const void *src = nullptr; /* kinds 2-7, 9: the FP32 images; 10: the FP64 sampling images */
     const void *sc = nullptr;  /* kind 4: the FP8 scales */
 };
 
@@ -165,10 +166,15 @@ struct amb_ctx {
     hipEvent_t ev_red = nullptr, ev_cp[2] = {nullptr, nullptr};
     void *wdesc = nullptr; size_t wdesc_bytes = 0; /* wave descriptors */
     /* lag-Fourier form: tile partials, the FP64 spectrum of one triangle,
-       lag arrays of a wave */
+       lag arrays of a wave (two buffers, as the grouped waves) */
     void *lag_part = nullptr; size_t lag_part_bytes = 0;
     void *lag_st = nullptr; size_t lag_st_bytes = 0;
     void *lag_out = nullptr; size_t lag_out_bytes = 0;
+    void *lag_out2 = nullptr; size_t lag_out2_bytes = 0; /* second result buffer (copy overlap) */
+    float lag_margin = 0.f; /* worst normalisation grid distance of the last lag-Fourier call */
+    /* r1 == r2 tile tables of the lag-Fourier form by transform length:
+       device block [tiles (I, J)] [nt x nt tile index], tile count, bytes */
+    std::map<int, std::tuple<void *, int, size_t>> lag_pairs_tab;
     /* pinned host staging, two buffers: a grouped wave's results are copied
        out of one while the next wave's land in the other; the lag-space
        post-processing stages alternate triangles in them the same way */
@@ -254,9 +260,10 @@ int twiddles64(amb_ctx *c, int n, double2 **out);
  * (bl64: per-request FP64 lag arrays, may be null) */
 int lag_setup(amb_ctx *c, Cube &cb);
 void lag_release(amb_ctx *c, Cube &cb);
-size_t lag_spectra_bytes(const Cube &cb, long ng);
+size_t lag_spectra_bytes(const Cube &cb, long ng, bool fp64);
 hipError_t lag_launch_spectra(amb_ctx *c, hipStream_t s, const BuildOp &op);
-int lagft_run(amb_ctx *c, const amb_tri_req *req, int n, double t0, double *done_ms, double *const *bl64);
+int lagft_run(amb_ctx *c, const amb_tri_req *req, int n, double t0, double *done_ms, double *const *bl64,
+              double *const *norm64);
 /* calCR; noalloc: all device scratch from the static array, no allocation */
 int closed_rings(amb_ctx *c, int id, int plane, double *rdata, int cap, int *rcount, bool noalloc);
 

@@ -14,14 +14,18 @@
  * triple products per pixel (half the plane: B is real) instead of the
  * nc^3 real ones of the direct sum: 48x fewer at 768 channels.
  *
- * Stages (FP32 products, FP64 sums and transforms after them):
+ * Stages (data: FP32 products, FP64 sums and transforms after them; the v2
+ * normalisation runs the same stages in FP64 throughout, its sums rounded
+ * to their 1/8 grid of Ng^2 with a certificate):
  *   k_chan_spectra  per pixel pair, the real channel sequences packed as
  *                   one complex sequence, forward DFT of length L in LDS;
  *                   X[x][k], k = 0 .. L/2 (a plane set kind of its own);
  *   k_lagft_tiles   S over 128 x 128 (k2, k3) tiles, k3 < L/2, for one
  *                   segment of pixels per work item: FP32 accumulators in
  *                   registers, written as the item's partial;
- *   k_lagft_nyq     the k3 = L/2 column;
+ *                   the k3 = L/2 column rides along in the last k3 tile
+ *                   (which stages every conj F it needs), or is a column
+ *                   of the last, partial tile when TB does not divide L/2;
  *   k_lagft_reduce  the partials summed over segments in FP64, in segment
  *                   order (deterministic), transposed to S^T[k3][k2];
  *   k_lagft_cols    FP64 inverse DFT over k2 of every row k3;
@@ -44,11 +48,8 @@ using namespace amb;
 
 namespace {
 
-constexpr int LT = 128;             /* tile side in k2 and in k3 */
-constexpr int KPX = 4;              /* pixels per LDS chunk */
-constexpr int BLKB = 80;            /* LDS bytes per block of 8 complex (64 + 16 pad) */
-constexpr int PXB = 64 * BLKB;      /* one pixel: G 16 blocks, H 16, conj F 32 */
-constexpr int SEG = 2048;           /* pixels per work item */
+constexpr int BLKB = 80;            /* LDS bytes per unit: 64 B of values + 16 B pad */
+constexpr int SEG = 2048;           /* pixels per tile work item */
 constexpr int MAXL = 2048;          /* longest channel transform: 2 L FP64 complex in 64 KiB of LDS */
 constexpr size_t ALIGN = 256;
 constexpr size_t PART_CAP = (size_t)8 << 30; /* partial sums per wave */
@@ -63,18 +64,54 @@ double now_ms()
     return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
 }
 
-struct LagJob {
-    const float2 *F, *G, *H; /* channel spectra [P][lhp] of r1, r2, r3 */
-    const int *ex1, *ex2, *ex3; /* exponents E of the three sets */
-    float2 *part;            /* [nseg][ntile][LT*LT] tile partials */
-    float2 *nyq;             /* [nseg][L] partials of the k3 = L/2 column */
-    long P;
-    long item0, nitem0;      /* first tile item, first nyquist item in the launch */
-    int lhp, L, nseg, seg;
+/* Per scalar type: the complex type, each lane's block of BU k2 x BV k3
+ * accumulators (16 x 16 lanes: tiles of TA = 16 BU by TB = 16 BV), pixels
+ * per LDS chunk, waves per SIMD the kernel is built for, whether the k3 =
+ * L/2 column rides in the last tile (FOLD).  FP32: 8 x 8
+ * (128 x 128 tiles); FP64: 4 x 8 (64 x 128), whose 32 complex double
+ * accumulators take as many registers as the FP32 form's 64. */
+template <typename R>
+struct LagT;
+template <>
+struct LagT<float> {
+    typedef float2 C;
+    static constexpr int BU = 8, BV = 8, KPX = 4, WPS = 2;
+    static constexpr bool FOLD = true;
+};
+template <>
+struct LagT<double> {
+    typedef double2 C;
+    /* the k3 = L/2 column in a kernel of its own: folding it into the tiles
+       would take the registers of a second wave per SIMD */
+    static constexpr int BU = 4, BV = 8, KPX = 2, WPS = 2;
+    static constexpr bool FOLD = false;
 };
 
-__host__ __device__ inline int tiles3(int L) { return (L / 2 + LT - 1) / LT; }
-__host__ __device__ inline int tiles2(int L) { return (L + LT - 1) / LT; }
+template <typename R>
+struct LagJob {
+    typedef typename LagT<R>::C C;
+    const C *F, *G, *H;         /* channel spectra [P][lhp] of r1, r2, r3 */
+    const int *ex1, *ex2, *ex3; /* data sets: exponents E of the three sets (null: unscaled) */
+    C *part;                    /* [nseg][ntile][TA*TB] tile partials */
+    C *nyq;                     /* [nseg][L] partials of the k3 = L/2 column, when a
+                                   kernel of its own sums it (nyq_sep) */
+    const int2 *tlist;          /* mode 1: (I, J) of each tile */
+    long P;
+    long item0, nitem0;         /* first tile item, first nyquist item in the launch */
+    int lhp, L, nseg, seg;      /* tile segments and their pixels */
+    int mode;                   /* 0: (k2, k3) tiles; 1: r1 == r2, (i, j) tiles, i <= j */
+    int ntile;                  /* tiles per segment */
+    int nyq_sep;                /* the k3 = L/2 column is summed by k_lagft_nyq */
+};
+
+template <typename R>
+__host__ __device__ inline int tile_a() { return 16 * LagT<R>::BU; }
+template <typename R>
+__host__ __device__ inline int tile_b() { return 16 * LagT<R>::BV; }
+template <typename R>
+__host__ __device__ inline int tiles2(int L) { return (L + tile_a<R>() - 1) / tile_a<R>(); }
+template <typename R>
+__host__ __device__ inline int tiles3(int L) { return (L / 2 + tile_b<R>() - 1) / tile_b<R>(); }
 
 /* blocks b = x (mod 8) run on XCD x: XCD x takes items [x per, (x+1) per) */
 __device__ __forceinline__ long xcd_item(long b, long nb)
@@ -83,7 +120,8 @@ __device__ __forceinline__ long xcd_item(long b, long nb)
     return (b % 8) * per + b / 8;
 }
 
-__device__ __forceinline__ int find_job(const LagJob *__restrict__ jobs, int njob, long it, bool nyq)
+template <typename J>
+__device__ __forceinline__ int find_job(const J *__restrict__ jobs, int njob, long it, bool nyq = false)
 {
     int lo = 0, hi = njob - 1;
     while (lo < hi) {
@@ -96,30 +134,53 @@ __device__ __forceinline__ int find_job(const LagJob *__restrict__ jobs, int njo
 }
 
 /* X(k) of a pixel row for k in [0, L): stored for k <= L/2, else conj X(L - k) */
-__device__ __forceinline__ float2 spec_at(const float2 *__restrict__ X, long row, int k, int L)
+template <typename C>
+__device__ __forceinline__ C spec_at(const C *__restrict__ X, long row, int k, int L)
 {
     if (2 * k <= L) return X[row + k];
-    const float2 v = X[row + (L - k)];
-    return make_float2(v.x, -v.y);
+    C v = X[row + (L - k)];
+    v.y = -v.y;
+    return v;
+}
+
+/* the 64 value bytes of one LDS unit as complex values, in 16-byte reads */
+__device__ __forceinline__ void load_unit(const char *p, float2 *v)
+{
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const float4 w = *(const float4 *)(p + 16 * q);
+        v[2 * q] = make_float2(w.x, w.y);
+        v[2 * q + 1] = make_float2(w.z, w.w);
+    }
+}
+__device__ __forceinline__ void load_unit(const char *p, double2 *v)
+{
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = *(const double2 *)(p + 16 * q);
 }
 
 /* ---------------------------------------------------------------- spectra */
 /* grid: multiple of 8 blocks of nteam x 64 threads, each team ppt pixel
-   pairs; dynamic LDS: tw[L] + nteam * 2 * L complex.  Plane i of the
-   normalised set carries (-1)^i, removed here; channel ch = act0 + p sits
-   at position p of the sequence. */
-__global__ __launch_bounds__(256) void k_chan_spectra(const float *__restrict__ D, long pitch, const int *__restrict__ inv,
+   pairs; dynamic LDS: [tw L if TWL] + nteam * 2 * L complex.  SIGNED: plane
+   i of a normalised data set carries (-1)^i, removed here.  Channel
+   ch = act0 + p sits at position p of the sequence. */
+template <typename R, bool SIGNED, bool TWL>
+__global__ __launch_bounds__(256) void k_chan_spectra(const R *__restrict__ D, long pitch, const int *__restrict__ inv,
                                                       int nc, int act0, long P, FftPlan pl,
-                                                      const float2 *__restrict__ twg, int lhp, int ppt,
-                                                      float2 *__restrict__ X, const unsigned *__restrict__ slot_in,
-                                                      unsigned *__restrict__ slot_out)
+                                                      const typename LagT<R>::C *__restrict__ twg, int lhp, int ppt,
+                                                      typename LagT<R>::C *__restrict__ X,
+                                                      const unsigned *__restrict__ slot_in, unsigned *__restrict__ slot_out)
 {
-    extern __shared__ __align__(16) float2 lsm[];
+    typedef typename LagT<R>::C C;
+    extern __shared__ __align__(16) char lraw[];
+    C *lsm = (C *)lraw;
     const int L = pl.n, h = L / 2;
-    const int nteam = blockDim.x >> 6, team = threadIdx.x >> 6, t = threadIdx.x & 63;
-    float2 *tw = lsm, *bx = lsm + L + (size_t)team * 2 * L, *by = bx + L;
-    for (int i = threadIdx.x; i < L; i += blockDim.x) tw[i] = twg[i];
-    if (blockIdx.x == 0 && threadIdx.x < 2) slot_out[threadIdx.x] = slot_in[threadIdx.x];
+    const int team = threadIdx.x >> 6, t = threadIdx.x & 63, nteam = blockDim.x >> 6;
+    C *bx = lsm + (TWL ? L : 0) + (size_t)team * 2 * L, *by = bx + L;
+    const C *tw = TWL ? lsm : twg;
+    if (TWL)
+        for (int i = threadIdx.x; i < L; i += blockDim.x) lsm[i] = twg[i];
+    if (slot_out && blockIdx.x == 0 && threadIdx.x < 2) slot_out[threadIdx.x] = slot_in[threadIdx.x];
     const long npair = (P + 1) / 2;
     const long pair0 = (xcd_item(blockIdx.x, gridDim.x) * nteam + team) * ppt;
     __syncthreads();
@@ -128,29 +189,37 @@ __global__ __launch_bounds__(256) void k_chan_spectra(const float *__restrict__ 
         const bool live = pr < npair;
         const long x0 = 2 * pr;
         for (int p = t; p < L; p += 64) {
-            float2 v = make_float2(0.f, 0.f);
+            C v;
+            v.x = v.y = (R)0;
             const int ch = act0 + p;
             if (live && ch < nc) {
                 const int i = inv[ch];
                 if (i >= 0) {
-                    /* x0 even and pitch a multiple of 64: an aligned pair; for
-                       odd P the second value is the zeroed pitch tail */
-                    const float2 d = *(const float2 *)(D + (size_t)i * pitch + x0);
-                    v = (i & 1) ? make_float2(-d.x, -d.y) : d;
+                    /* x0 even and the pitch a multiple of 64: an aligned pair;
+                       for odd P the second value is the zeroed pitch tail */
+                    const R *d = D + (size_t)i * pitch + x0;
+                    v.x = d[0];
+                    v.y = d[1];
+                    if (SIGNED && (i & 1)) { v.x = -v.x; v.y = -v.y; }
                 }
             }
             bx[p] = v;
         }
         __syncthreads();
-        const float2 *Z = fftc::transform<-1, 5>(bx, by, tw, pl, t, 64);
+        const C *Z = fftc::transform<-1, 5>(bx, by, tw, pl, t, 64);
         if (live) {
-            float2 *xa = X + (size_t)x0 * lhp, *xb = xa + lhp;
+            C *xa = X + (size_t)x0 * lhp, *xb = xa + lhp;
             const bool two = x0 + 1 < P;
             for (int k = t; k <= h; k += 64) {
-                const float2 a = Z[k], b = Z[k == 0 ? 0 : L - k];
+                const C a = Z[k], b = Z[k == 0 ? 0 : L - k];
                 /* A = (Z(k) + conj Z(L-k)) / 2, B = (Z(k) - conj Z(L-k)) / 2i */
-                xa[k] = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y - b.y));
-                if (two) xb[k] = make_float2(0.5f * (a.y + b.y), -0.5f * (a.x - b.x));
+                C A, B;
+                A.x = (R)0.5 * (a.x + b.x);
+                A.y = (R)0.5 * (a.y - b.y);
+                B.x = (R)0.5 * (a.y + b.y);
+                B.y = (R)-0.5 * (a.x - b.x);
+                xa[k] = A;
+                if (two) xb[k] = B;
             }
         }
         __syncthreads(); /* the next pair overwrites both buffers */
@@ -158,79 +227,145 @@ __global__ __launch_bounds__(256) void k_chan_spectra(const float *__restrict__ 
 }
 
 /* ------------------------------------------------------------------ tiles */
-__global__ __launch_bounds__(256, 2) void k_lagft_tiles(const LagJob *__restrict__ jobs, int njob, long total)
+/* One work item: a TA x TB tile over one segment of pixels, as
+ * acc(p, q) += C(p + q) A(p) B(q) per pixel, the three operands staged in
+ * padded 80-byte LDS units (16 lanes reading 16 units at a stride of 20
+ * banks touch all 64 banks once); lane (li, lj) accumulates p = pa + BU li
+ * + u, q = pb + BV lj + v.
+ *   mode 0: p = k2, q = k3 (k3 < L/2): A = G(k2), B = H(k3), C = conj F(k2+k3);
+ *   mode 1 (r1 == r2, F = G): p = i = -k2, q = j = k2 + k3: A = conj F(i),
+ *           B = conj F(j), C = H(i + j), symmetric in (i, j), so only tiles
+ *           I <= J of the band (i + j) mod L in [0, L/2) are summed (the
+ *           job's tile list).
+ * FOLD (FP32): in mode 0 with TB | L/2 the last k3 tile also sums the k3 =
+ * L/2 column, S(k2, L/2) = conj F(k2 + L/2) G(k2) H(L/2): every conj F it
+ * needs is staged (at k2 - a + TB), and H(L/2) takes one more slot.  When
+ * TB does not divide L/2 the column is a column of the last, partial tile;
+ * otherwise (FP64, mode 1) k_lagft_nyq sums it. */
+template <typename R>
+__global__ __launch_bounds__(256, LagT<R>::WPS) void k_lagft_tiles(const LagJob<R> *__restrict__ jobs, int njob, long total)
 {
+    typedef typename LagT<R>::C C;
+    constexpr int BU = LagT<R>::BU, BV = LagT<R>::BV, KPX = LagT<R>::KPX;
+    constexpr bool FOLD = LagT<R>::FOLD;
+    constexpr int TA = 16 * BU, TB = 16 * BV;
+    constexpr int VPU = 64 / (int)sizeof(C);           /* values per LDS unit */
+    constexpr int UG = TA / VPU, UH = TB / VPU, UF = (TA + TB) / VPU;
+    constexpr int PXB = (UG + UH + UF + (FOLD ? 1 : 0)) * BLKB; /* one pixel (+ a unit for H(L/2)) */
+    constexpr int NSLOT = 2 * (TA + TB);               /* A, B, then TA + TB of C */
+    constexpr int SPT = (NSLOT + (FOLD ? 1 : 0) + 255) / 256; /* slots per thread */
+    constexpr int NF = BU + BV - 1, NFU = (NF + VPU - 1) / VPU;
     __shared__ __align__(16) char lds[2][KPX * PXB];
     const long it = xcd_item(blockIdx.x, gridDim.x);
     if (it >= total) return;
-    const int jb = find_job(jobs, njob, it, false);
-    const float2 *__restrict__ F = jobs[jb].F;
+    const int jb = find_job(jobs, njob, it);
     const long P = jobs[jb].P;
-    const int lhp = jobs[jb].lhp, L = jobs[jb].L, seg = jobs[jb].seg;
-    const int nt3 = tiles3(L), ntile = tiles2(L) * nt3;
+    const int lhp = jobs[jb].lhp, L = jobs[jb].L, seg = jobs[jb].seg, mode = jobs[jb].mode;
+    const int nt3 = tiles3<R>(L), ntile = jobs[jb].ntile;
     const long local = it - jobs[jb].item0;
     const int sg = (int)(local / ntile), tile = (int)(local - (long)sg * ntile);
-    const int a = (tile / nt3) * LT, b = (tile % nt3) * LT;
+    int a, b;
+    if (mode == 0) {
+        a = (tile / nt3) * TA;
+        b = (tile % nt3) * TB;
+    } else {
+        const int2 ij = jobs[jb].tlist[tile];
+        a = ij.x * TA;
+        b = ij.y * TB;
+    }
     const long xb = (long)sg * seg, xe = xb + seg < P ? xb + seg : P;
     const int tid = threadIdx.x;
+    const int half = L / 2;
+    const bool nyq_col = half % TB != 0;
+    const bool nyqx = FOLD && mode == 0 && !nyq_col && b + TB == half;
 
-    /* the two values this thread stages per pixel: slot tid (G for
-       tid < 128, else H) and slot 256 + tid (conj F) */
-    const float2 *__restrict__ src0;
-    int k0;
-    bool ok0, cj0;
-    if (tid < LT) {
-        const int k2 = a + tid;
-        ok0 = k2 < L;
-        cj0 = 2 * k2 > L;
-        k0 = !ok0 ? 0 : cj0 ? L - k2 : k2;
-        src0 = jobs[jb].G;
-    } else {
-        const int k3 = b + tid - LT;
-        ok0 = 2 * k3 < L;
-        cj0 = false;
-        k0 = ok0 ? k3 : 0;
-        src0 = jobs[jb].H;
+    /* the values this thread stages per pixel: the spectrum, its index
+       (stored half: k <= L/2, else the conjugate at L - k) and whether the
+       staged value is the conjugate of what is read */
+    const C *src[SPT];
+    int kk[SPT], off[SPT];
+    bool ok[SPT], cj[SPT];
+    auto at = [&](int u, const C *x, int k, bool conj_of_value) {
+        /* value x(k) (or its conjugate) for k in [0, L) */
+        const bool m = 2 * k > L;
+        src[u] = x;
+        kk[u] = m ? L - k : k;
+        cj[u] = m != conj_of_value;
+        ok[u] = true;
+    };
+#pragma unroll
+    for (int u = 0; u < SPT; ++u) {
+        const int sl = tid + 256 * u;
+        src[u] = jobs[jb].G;
+        kk[u] = 0;
+        ok[u] = cj[u] = false;
+        off[u] = -1;
+        int r, base;
+        if (sl < TA) {
+            const int p = a + sl;
+            if (p < L) at(u, mode == 0 ? jobs[jb].G : jobs[jb].F, p, mode == 1);
+            r = sl;
+            base = 0;
+        } else if (sl < TA + TB) {
+            const int q = b + sl - TA;
+            if (mode == 1) at(u, jobs[jb].F, q, true);
+            else if (2 * q < L || (nyq_col && q == half)) at(u, jobs[jb].H, q, false);
+            r = sl - TA;
+            base = UG;
+        } else if (sl < NSLOT) {
+            const int s1 = (a + b + sl - TA - TB) % L;
+            if (mode == 1) at(u, jobs[jb].H, s1, false);
+            else at(u, jobs[jb].F, s1, true);
+            r = sl - TA - TB;
+            base = UG + UH;
+        } else if (FOLD && sl == NSLOT && nyqx) {
+            at(u, jobs[jb].H, half, false); /* H(L/2), in the extra unit */
+            r = 0;
+            base = UG + UH + UF;
+        } else {
+            continue;
+        }
+        off[u] = (base + r / VPU) * BLKB + (r % VPU) * (int)sizeof(C);
     }
-    const int s1 = (a + b + tid) % L;
-    const bool m1 = 2 * s1 > L;          /* F(s) = conj X(L - s) */
-    const int k1 = m1 ? L - s1 : s1;     /* conj F(s): X(L - s), or conj X(s) */
-    const int off0 = (tid >> 3) * BLKB + (tid & 7) * 8;
-    const int off1 = (32 + (tid >> 3)) * BLKB + (tid & 7) * 8;
 
-    float2 r0[KPX], r1[KPX];
+    C rg[KPX][SPT];
     auto fetch = [&](long x0) {
 #pragma unroll
         for (int pp = 0; pp < KPX; ++pp) {
             const long x = x0 + pp;
-            float2 v0 = make_float2(0.f, 0.f), v1 = make_float2(0.f, 0.f);
-            if (x < xe) {
-                const size_t row = (size_t)x * lhp;
-                if (ok0) {
-                    v0 = src0[row + k0];
-                    if (cj0) v0.y = -v0.y;
+            const size_t row = (size_t)x * lhp;
+#pragma unroll
+            for (int u = 0; u < SPT; ++u) {
+                C v;
+                v.x = v.y = (R)0;
+                if (x < xe && ok[u]) {
+                    v = src[u][row + kk[u]];
+                    if (cj[u]) v.y = -v.y;
                 }
-                v1 = F[row + k1];
-                if (!m1) v1.y = -v1.y;
+                rg[pp][u] = v;
             }
-            r0[pp] = v0;
-            r1[pp] = v1;
         }
     };
     auto stash = [&](char *st) {
 #pragma unroll
-        for (int pp = 0; pp < KPX; ++pp) {
-            *(float2 *)(st + pp * PXB + off0) = r0[pp];
-            *(float2 *)(st + pp * PXB + off1) = r1[pp];
-        }
+        for (int pp = 0; pp < KPX; ++pp)
+#pragma unroll
+            for (int u = 0; u < SPT; ++u)
+                if (off[u] >= 0) *(C *)(st + pp * PXB + off[u]) = rg[pp][u];
     };
 
     const int li = tid >> 4, lj = tid & 15;
-    float acr[8][8], aci[8][8];
+    const int ug = (BU * li) / VPU, uh = UG + (BV * lj) / VPU, uf = UG + UH + (BU * li + BV * lj) / VPU;
+    R acr[BU][BV], aci[BU][BV];
 #pragma unroll
-    for (int u = 0; u < 8; ++u)
+    for (int u = 0; u < BU; ++u)
 #pragma unroll
-        for (int v = 0; v < 8; ++v) acr[u][v] = aci[u][v] = 0.f;
+        for (int v = 0; v < BV; ++v) acr[u][v] = aci[u][v] = (R)0;
+    /* k3 = L/2 column of k2 = a + tid (tid < TA, whole waves) */
+    const bool nx = FOLD && nyqx && tid < TA;
+    const int offg = (tid / VPU) * BLKB + (tid % VPU) * (int)sizeof(C);
+    const int offf = (UG + UH + (tid + TB) / VPU) * BLKB + ((tid + TB) % VPU) * (int)sizeof(C);
+    R anr = (R)0, ani = (R)0;
 
     const long nch = (xe - xb + KPX - 1) / KPX;
     fetch(xb);
@@ -242,102 +377,142 @@ __global__ __launch_bounds__(256, 2) void k_lagft_tiles(const LagJob *__restrict
 #pragma unroll 1
         for (int pp = 0; pp < KPX; ++pp) {
             const char *px = st + pp * PXB;
-            float gr[8], gi[8], hr[8], hi[8], fr[16], fi[16];
+            C g[(BU + VPU - 1) / VPU * VPU], hv[(BV + VPU - 1) / VPU * VPU], f[NFU * VPU];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const float4 g4 = *(const float4 *)(px + li * BLKB + 16 * q);
-                const float4 h4 = *(const float4 *)(px + (16 + lj) * BLKB + 16 * q);
-                const float4 f4 = *(const float4 *)(px + (32 + li + lj) * BLKB + 16 * q);
-                const float4 e4 = *(const float4 *)(px + (33 + li + lj) * BLKB + 16 * q);
-                gr[2 * q] = g4.x; gi[2 * q] = g4.y; gr[2 * q + 1] = g4.z; gi[2 * q + 1] = g4.w;
-                hr[2 * q] = h4.x; hi[2 * q] = h4.y; hr[2 * q + 1] = h4.z; hi[2 * q + 1] = h4.w;
-                fr[2 * q] = f4.x; fi[2 * q] = f4.y; fr[2 * q + 1] = f4.z; fi[2 * q + 1] = f4.w;
-                fr[8 + 2 * q] = e4.x; fi[8 + 2 * q] = e4.y; fr[9 + 2 * q] = e4.z; fi[9 + 2 * q] = e4.w;
-            }
+            for (int q = 0; q < (BU + VPU - 1) / VPU; ++q) load_unit(px + (ug + q) * BLKB, g + q * VPU);
 #pragma unroll
-            for (int u = 0; u < 8; ++u)
+            for (int q = 0; q < (BV + VPU - 1) / VPU; ++q) load_unit(px + (uh + q) * BLKB, hv + q * VPU);
 #pragma unroll
-                for (int v = 0; v < 8; ++v) {
+            for (int q = 0; q < NFU; ++q) load_unit(px + (uf + q) * BLKB, f + q * VPU);
+#pragma unroll
+            for (int u = 0; u < BU; ++u)
+#pragma unroll
+                for (int v = 0; v < BV; ++v) {
                     /* t = G H, acc += conj(F) t */
-                    const float tr = fmaf(gr[u], hr[v], -gi[u] * hi[v]);
-                    const float ti = fmaf(gr[u], hi[v], gi[u] * hr[v]);
-                    acr[u][v] = fmaf(fr[u + v], tr, fmaf(-fi[u + v], ti, acr[u][v]));
-                    aci[u][v] = fmaf(fr[u + v], ti, fmaf(fi[u + v], tr, aci[u][v]));
+                    const R tr = fma(g[u].x, hv[v].x, -g[u].y * hv[v].y);
+                    const R ti = fma(g[u].x, hv[v].y, g[u].y * hv[v].x);
+                    acr[u][v] = fma(f[u + v].x, tr, fma(-f[u + v].y, ti, acr[u][v]));
+                    aci[u][v] = fma(f[u + v].x, ti, fma(f[u + v].y, tr, aci[u][v]));
                 }
+            if (FOLD && nx) {
+                const C gg = *(const C *)(px + offg), ff = *(const C *)(px + offf);
+                const C hh = *(const C *)(px + (UG + UH + UF) * BLKB);
+                const R tr = fma(gg.x, hh.x, -gg.y * hh.y), ti = fma(gg.x, hh.y, gg.y * hh.x);
+                anr = fma(ff.x, tr, fma(-ff.y, ti, anr));
+                ani = fma(ff.x, ti, fma(ff.y, tr, ani));
+            }
         }
         if (ch + 1 < nch) stash(lds[(ch + 1) & 1]);
         __syncthreads();
     }
-    float2 *out = jobs[jb].part + ((size_t)sg * ntile + tile) * (LT * LT);
+    if (FOLD && nx && a + tid < L) {
+        C w;
+        w.x = anr;
+        w.y = ani;
+        jobs[jb].nyq[(size_t)sg * L + a + tid] = w;
+    }
+    C *out = jobs[jb].part + ((size_t)sg * ntile + tile) * (TA * TB);
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-        float4 *o = (float4 *)(out + (size_t)(8 * li + u) * LT + 8 * lj);
+    for (int u = 0; u < BU; ++u) {
+        C *o = out + (size_t)(BU * li + u) * TB + BV * lj;
 #pragma unroll
-        for (int v = 0; v < 8; v += 2) o[v / 2] = make_float4(acr[u][v], aci[u][v], acr[u][v + 1], aci[u][v + 1]);
+        for (int v = 0; v < BV; ++v) {
+            C w;
+            w.x = acr[u][v];
+            w.y = aci[u][v];
+            o[v] = w;
+        }
     }
 }
 
 /* --------------------------------------------------------------- nyquist */
-/* S(k2, L/2) over one segment per block; thread t owns k2 = t + 256 q */
-__global__ __launch_bounds__(256) void k_lagft_nyq(const LagJob *__restrict__ jobs, int njob, long total)
+/* S(k2, L/2) = sum_x conj F(k2 + L/2) G(k2) H(L/2) over one segment per
+   block, for jobs with nyq_sep; thread t owns k2 = t + 256 q */
+template <typename R>
+__global__ __launch_bounds__(256) void k_lagft_nyq(const LagJob<R> *__restrict__ jobs, int njob, long total)
 {
+    typedef typename LagT<R>::C C;
     const long it = xcd_item(blockIdx.x, gridDim.x);
     if (it >= total) return;
     const int jb = find_job(jobs, njob, it, true);
-    const float2 *__restrict__ F = jobs[jb].F, *__restrict__ G = jobs[jb].G, *__restrict__ H = jobs[jb].H;
+    const C *__restrict__ F = jobs[jb].F, *__restrict__ G = jobs[jb].G, *__restrict__ H = jobs[jb].H;
     const long P = jobs[jb].P;
     const int lhp = jobs[jb].lhp, L = jobs[jb].L, seg = jobs[jb].seg, h = L / 2;
     const int sg = (int)(it - jobs[jb].nitem0);
     const long xb = (long)sg * seg, xe = xb + seg < P ? xb + seg : P;
     constexpr int NQ = MAXL / 256;
-    float acr[NQ], aci[NQ];
+    R acr[NQ], aci[NQ];
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) acr[q] = aci[q] = 0.f;
+    for (int q = 0; q < NQ; ++q) acr[q] = aci[q] = (R)0;
     for (long x = xb; x < xe; ++x) {
         const size_t row = (size_t)x * lhp;
-        const float2 hv = H[row + h];
+        const C hv = H[row + h];
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
             const int k2 = threadIdx.x + 256 * q;
             if (256 * q < L && k2 < L) {
-                const float2 g = spec_at(G, row, k2, L);
+                const C g = spec_at(G, row, k2, L);
                 int s = k2 + h;
                 if (s >= L) s -= L;
-                const float2 f = spec_at(F, row, s, L);
-                const float tr = fmaf(g.x, hv.x, -g.y * hv.y), ti = fmaf(g.x, hv.y, g.y * hv.x);
-                acr[q] = fmaf(f.x, tr, fmaf(f.y, ti, acr[q]));
-                aci[q] = fmaf(f.x, ti, fmaf(-f.y, tr, aci[q]));
+                const C f = spec_at(F, row, s, L);
+                const R tr = fma(g.x, hv.x, -g.y * hv.y), ti = fma(g.x, hv.y, g.y * hv.x);
+                acr[q] = fma(f.x, tr, fma(f.y, ti, acr[q]));
+                aci[q] = fma(f.x, ti, fma(-f.y, tr, aci[q]));
             }
         }
     }
-    float2 *o = jobs[jb].nyq + (size_t)sg * L;
+    C *o = jobs[jb].nyq + (size_t)sg * L;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
         const int k2 = threadIdx.x + 256 * q;
-        if (256 * q < L && k2 < L) o[k2] = make_float2(acr[q], aci[q]);
+        if (256 * q < L && k2 < L) {
+            C w;
+            w.x = acr[q];
+            w.y = aci[q];
+            o[k2] = w;
+        }
     }
 }
 
 /* ----------------------------------------------------------------- reduce */
-/* St[k3][k2] (k3 < L/2) = sum over segments of the tile partials, FP64 in
-   segment order; 32 x 32 (k2, k3) blocks transposed through LDS */
-__global__ __launch_bounds__(256) void k_lagft_reduce(const float2 *__restrict__ part, int nseg, int L,
-                                                      double2 *__restrict__ St)
+/* St[k3][k2] = sum over segments of the tile partials, FP64 in segment
+   order, for k3 < L/2 (and k3 = L/2 when it is a column of the last,
+   partial tile); 32 x 32 (k2, k3) blocks transposed through LDS.
+   M1 (mode 1): the value sits at (i, j) = (-k2, k2 + k3) mod L, in listed
+   tile tmap[I][J] of (i, j) or, when I > J, of (j, i) (symmetric). */
+template <typename R, bool M1>
+__global__ __launch_bounds__(256) void k_lagft_reduce(const typename LagT<R>::C *__restrict__ part, int nseg, int ntile,
+                                                      int L, const int *__restrict__ tmap, double2 *__restrict__ St)
 {
+    typedef typename LagT<R>::C C;
+    constexpr int TA = 16 * LagT<R>::BU, TB = 16 * LagT<R>::BV;
     __shared__ double2 tr[32][33];
-    const int nt3 = tiles3(L);
-    const size_t tsz = (size_t)tiles2(L) * nt3 * LT * LT;
+    const int nt3 = tiles3<R>(L);
+    const size_t tsz = (size_t)ntile * TA * TB;
     const int X = blockIdx.x * 32, Y = blockIdx.y * 32;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-    const int k3 = Y + tx;
+    const int k3 = Y + tx, half = L / 2;
+    const bool col = !M1 && half % TB != 0; /* k3 = L/2 is a tile column */
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int k2 = X + ty + 8 * q;
         double sx = 0.0, sy = 0.0;
-        if (k2 < L && 2 * k3 < L) {
-            const size_t off = (size_t)((k2 / LT) * nt3 + k3 / LT) * (LT * LT) + (k2 % LT) * LT + (k3 % LT);
+        if (k2 < L && (k3 < half || (col && k3 == half))) {
+            size_t off;
+            if (M1) {
+                int i = k2 == 0 ? 0 : L - k2, j = k2 + k3;
+                if (j >= L) j -= L;
+                int I = i / TA, J = j / TB;
+                if (I > J) {
+                    const int t = i; i = j; j = t;
+                    const int T = I; I = J; J = T;
+                }
+                off = (size_t)tmap[I * (L / TA) + J] * (TA * TB) + (i - I * TA) * TB + (j - J * TB);
+            } else {
+                off = (size_t)((k2 / TA) * nt3 + k3 / TB) * (TA * TB) + (k2 % TA) * TB + (k3 % TB);
+            }
             for (int sg = 0; sg < nseg; ++sg) {
-                const float2 v = part[sg * tsz + off];
+                const C v = part[sg * tsz + off];
                 sx += (double)v.x;
                 sy += (double)v.y;
             }
@@ -348,19 +523,21 @@ __global__ __launch_bounds__(256) void k_lagft_reduce(const float2 *__restrict__
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int k3w = Y + ty + 8 * q, k2w = X + tx;
-        if (2 * k3w < L && k2w < L) St[(size_t)k3w * L + k2w] = tr[tx][ty + 8 * q];
+        if ((k3w < half || (col && k3w == half)) && k2w < L) St[(size_t)k3w * L + k2w] = tr[tx][ty + 8 * q];
     }
 }
 
-/* St[L/2][k2] = sum over segments of the nyquist partials (FP64, in order) */
-__global__ __launch_bounds__(256) void k_lagft_reduce_nyq(const float2 *__restrict__ nyq, int nseg, int L,
+/* St[L/2][k2] = sum over segments of the k3 = L/2 partials (FP64, in
+   order), when the last full tile summed them */
+template <typename R>
+__global__ __launch_bounds__(256) void k_lagft_reduce_nyq(const typename LagT<R>::C *__restrict__ nyq, int nseg, int L,
                                                           double2 *__restrict__ St)
 {
     const int k2 = blockIdx.x * 256 + threadIdx.x;
     if (k2 >= L) return;
     double sx = 0.0, sy = 0.0;
     for (int sg = 0; sg < nseg; ++sg) {
-        const float2 v = nyq[(size_t)sg * L + k2];
+        const typename LagT<R>::C v = nyq[(size_t)sg * L + k2];
         sx += (double)v.x;
         sy += (double)v.y;
     }
@@ -385,12 +562,18 @@ __global__ __launch_bounds__(64) void k_lagft_cols(double2 *__restrict__ St, Fft
 /* window rows wa = 2 blockIdx.x and wb = wa + 1 (d2 = w - nc10): the
    half spectra St[k3][d2], k3 <= L/2, extended Hermitian, as one complex
    sequence A + i B; its inverse DFT carries row wa in the real part and
-   row wb in the imaginary part (DC and Nyquist terms: real parts only) */
+   row wb in the imaginary part (DC and Nyquist terms: real parts only).
+   Data (snapP == 0): times 2^(E1+E2+E3-21)/L^2 (exponents null: 1/L^2),
+   into out32 and out64.  Normalisation (snapP = Ng^2): the sum divided by
+   L^2 is rounded to the nearest 1/8 of Ng^2 into out64 (and out32), and
+   the largest distance to that grid, in units of Ng^2, goes to margin
+   (float bits).  Cells without triplets are exact zeros. */
 __global__ __launch_bounds__(64) void k_lagft_rows(const double2 *__restrict__ St, FftPlan pl,
                                                    const double2 *__restrict__ tw, int nc10,
                                                    const int *__restrict__ cnt, const int *__restrict__ ex1,
                                                    const int *__restrict__ ex2, const int *__restrict__ ex3,
-                                                   float *__restrict__ bl, double *__restrict__ bl64)
+                                                   double snapP, float *__restrict__ out32,
+                                                   double *__restrict__ out64, unsigned *__restrict__ margin)
 {
     extern __shared__ __align__(16) double2 dsm[];
     const int L = pl.n, W = 2 * nc10 + 1;
@@ -413,19 +596,27 @@ __global__ __launch_bounds__(64) void k_lagft_rows(const double2 *__restrict__ S
     }
     __syncthreads();
     const double2 *res = fftc::transform<+1, 5>(bx, by, tw, pl, threadIdx.x, 64);
-    const double scale = ldexp(1.0, *ex1 + *ex2 + *ex3 - 3 * AMB_SET_EXP) / ((double)L * (double)L);
+    const double scale = (ex1 ? ldexp(1.0, *ex1 + *ex2 + *ex3 - 3 * AMB_SET_EXP) : 1.0) / ((double)L * (double)L);
+    float mg = 0.f;
+    auto put = [&](size_t cell, double v) {
+        if (!cnt[cell]) v = 0.0;
+        if (snapP > 0.0) {
+            const double q = rint(8.0 * v / snapP) / 8.0;
+            mg = fmaxf(mg, (float)fabs(v / snapP - q));
+            v = snapP * q;
+        }
+        out32[cell] = (float)v;
+        if (out64) out64[cell] = v;
+    };
     for (int w3 = threadIdx.x; w3 < W; w3 += 64) {
         const int idx = ((w3 - nc10) % L + L) % L;
         const double2 z = res[idx];
-        const size_t ca = (size_t)wa * W + w3, cb = (size_t)wb * W + w3;
-        const double va = cnt[ca] ? z.x * scale : 0.0;
-        bl[ca] = (float)va;
-        if (bl64) bl64[ca] = va;
-        if (wb != wa) {
-            const double vb = cnt[cb] ? z.y * scale : 0.0;
-            bl[cb] = (float)vb;
-            if (bl64) bl64[cb] = vb;
-        }
+        put((size_t)wa * W + w3, z.x * scale);
+        if (wb != wa) put((size_t)wb * W + w3, z.y * scale);
+    }
+    if (margin) {
+        for (int o = 32; o > 0; o >>= 1) mg = fmaxf(mg, __shfl_xor(mg, o));
+        if (threadIdx.x == 0) atomicMax(margin, __float_as_uint(mg));
     }
 }
 
@@ -516,35 +707,136 @@ void amb::lag_release(amb_ctx *c, Cube &cb)
     cb.lag_L = cb.lag_lhp = 0;
 }
 
-size_t amb::lag_spectra_bytes(const Cube &cb, long ng)
+size_t amb::lag_spectra_bytes(const Cube &cb, long ng, bool fp64)
 {
-    return sizeof(float2) * (size_t)ng * ng * cb.lag_lhp + AMB_SET_SLOT_BYTES;
+    return (fp64 ? sizeof(double2) : sizeof(float2)) * (size_t)ng * ng * cb.lag_lhp + AMB_SET_SLOT_BYTES;
 }
 
+/* kind 9: FP32 spectra of the normalised data set op.src (its sign undone,
+   its [max, E] words copied after the spectra); kind 10: FP64 spectra of
+   the FP64 sampling images op.src */
 hipError_t amb::lag_launch_spectra(amb_ctx *c, hipStream_t s, const BuildOp &op)
 {
     const Cube &cb = c->cubes[op.cube];
     const long ng = op.g.ng, P = ng * ng, pitch = plane_pitch(ng);
-    const int L = cb.lag_L;
-    auto itw = c->tw.find(L);
-    if (itw == c->tw.end()) return hipErrorInvalidValue;
+    const int L = cb.lag_L, ppt = 4;
+    const long npair = (P + 1) / 2;
     const FftPlan &pl = c->plans[L];
-    const int nteam = spectra_teams(L), ppt = 4;
-    const long npair = (P + 1) / 2, per_blk = (long)nteam * ppt;
-    const long nblk = ((npair + per_blk - 1) / per_blk + 7) / 8 * 8;
-    const size_t lds = sizeof(float2) * (size_t)L * (1 + 2 * nteam);
-    const size_t setw = (size_t)pitch * cb.nact();
-    const unsigned *slot_in = (const unsigned *)((const float *)op.src + setw);
-    unsigned *slot_out = (unsigned *)((float2 *)op.dst + (size_t)P * cb.lag_lhp);
-    hipLaunchKernelGGL(k_chan_spectra, dim3((unsigned)nblk), dim3(64 * nteam), lds, s, (const float *)op.src, pitch,
-                       cb.d_inv, (int)cb.nc, cb.act.front(), P, pl, itw->second, cb.lag_lhp, ppt, (float2 *)op.dst,
-                       slot_in, slot_out);
+    if (op.kind == 9) {
+        auto itw = c->tw.find(L);
+        if (itw == c->tw.end()) return hipErrorInvalidValue;
+        const int nteam = spectra_teams(L);
+        const long per_blk = (long)nteam * ppt;
+        const long nblk = ((npair + per_blk - 1) / per_blk + 7) / 8 * 8;
+        const size_t lds = sizeof(float2) * (size_t)L * (1 + 2 * nteam);
+        const size_t setw = (size_t)pitch * cb.nact();
+        const unsigned *slot_in = (const unsigned *)((const float *)op.src + setw);
+        unsigned *slot_out = (unsigned *)((float2 *)op.dst + (size_t)P * cb.lag_lhp);
+        hipLaunchKernelGGL((k_chan_spectra<float, true, true>), dim3((unsigned)nblk), dim3(64 * nteam), lds, s,
+                           (const float *)op.src, pitch, cb.d_inv, (int)cb.nc, cb.act.front(), P, pl, itw->second,
+                           cb.lag_lhp, ppt, (float2 *)op.dst, slot_in, slot_out);
+        return hipGetLastError();
+    }
+    auto itw = c->tw64.find(L);
+    if (itw == c->tw64.end()) return hipErrorInvalidValue;
+    const long nblk = ((npair + ppt - 1) / ppt + 7) / 8 * 8;
+    const size_t lds = sizeof(double2) * 2 * (size_t)L;
+    hipLaunchKernelGGL((k_chan_spectra<double, false, false>), dim3((unsigned)nblk), dim3(64), lds, s,
+                       (const double *)op.src, pitch, cb.d_inv, (int)cb.nc, cb.act.front(), P, pl, itw->second,
+                       cb.lag_lhp, ppt, (double2 *)op.dst, nullptr, nullptr);
     return hipGetLastError();
 }
 
+namespace {
+
+/* one triangle's stage of a wave: its data sums (FP32) or its
+   normalisation (FP64) */
+struct LJ {
+    int q;          /* request */
+    bool norm;      /* FP64 normalisation job */
+    Cube *cb;
+    long ng, P;
+    int nseg;
+    int mode, ntile, nyq_sep;
+    const int2 *tlist; /* mode 1 */
+    const int *tmap;   /* mode 1 */
+    void *X[3];     /* spectra of r1, r2, r3 */
+    size_t part, nyq, out, out64;
+};
+
+/* mode 1 tiles of a transform length L (128 | L): (I, J), I <= J, on the
+   tile diagonals m = (I + J) mod nt that hold cells with (i + j) mod L in
+   [0, L/2): m in [0, (64 nt - 1) / 128] and m = nt - 1 (a cell of tile
+   diagonal m has i + j in [128 m, 128 m + 254]) */
+std::vector<int2> pair_tiles(int L)
+{
+    const int nt = L / 128, mmax = (64 * nt - 1) / 128;
+    std::vector<int2> t;
+    for (int I = 0; I < nt; ++I)
+        for (int J = I; J < nt; ++J) {
+            const int m = (I + J) % nt;
+            if (m <= mmax || m == nt - 1) t.push_back(make_int2(I, J));
+        }
+    return t;
+}
+
+template <typename R>
+size_t job_part_bytes(const LJ &j, int L)
+{
+    const size_t cb = sizeof(typename LagT<R>::C);
+    return cb * (size_t)j.nseg * ((size_t)j.ntile * tile_a<R>() * tile_b<R>() + L);
+}
+
+template <typename R>
+void fill_job(LagJob<R> &g, const LJ &j, char *partbase, long *items, long *nitems)
+{
+    typedef typename LagT<R>::C C;
+    const int L = j.cb->lag_L;
+    g.F = (const C *)j.X[0];
+    g.G = (const C *)j.X[1];
+    g.H = (const C *)j.X[2];
+    g.ex1 = g.ex2 = g.ex3 = nullptr;
+    if (!j.norm) {
+        const int *ex[3];
+        for (int k = 0; k < 3; ++k) ex[k] = (const int *)((const C *)j.X[k] + (size_t)j.P * j.cb->lag_lhp) + 1;
+        g.ex1 = ex[0];
+        g.ex2 = ex[1];
+        g.ex3 = ex[2];
+    }
+    g.part = (C *)(partbase + j.part);
+    g.nyq = (C *)(partbase + j.nyq);
+    g.tlist = j.tlist;
+    g.P = j.P;
+    g.lhp = j.cb->lag_lhp;
+    g.L = L;
+    g.nseg = j.nseg;
+    g.seg = SEG;
+    g.mode = j.mode;
+    g.ntile = j.ntile;
+    g.nyq_sep = j.nyq_sep;
+    g.item0 = *items;
+    g.nitem0 = *nitems;
+    *items += (long)j.nseg * j.ntile;
+    if (j.nyq_sep) *nitems += j.nseg;
+}
+
+template <typename R>
+hipError_t launch_products(hipStream_t s, const LagJob<R> *d, int n, long items, long nitems)
+{
+    hipLaunchKernelGGL(k_lagft_tiles<R>, dim3((unsigned)((items + 7) / 8 * 8)), dim3(256), 0, s, d, n, items);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || nitems == 0) return e;
+    hipLaunchKernelGGL(k_lagft_nyq<R>, dim3((unsigned)((nitems + 7) / 8 * 8)), dim3(256), 0, s, d, n, nitems);
+    return hipGetLastError();
+}
+
+} // namespace
+
 /* triangles of the lag-Fourier form in waves; bl64[i] (may be null, or
-   hold nulls): FP64 lag arrays of request i */
-int amb::lagft_run(amb_ctx *c, const amb_tri_req *req, int n, double t0, double *done_ms, double *const *bl64)
+   hold nulls): FP64 lag arrays of request i; norm64[i] likewise for the
+   normalisation of requests that ask for one */
+int amb::lagft_run(amb_ctx *c, const amb_tri_req *req, int n, double t0, double *done_ms, double *const *bl64,
+                   double *const *norm64)
 {
     hipSetDevice(c->device);
     std::vector<int> order(n);
@@ -553,21 +845,49 @@ int amb::lagft_run(amb_ctx *c, const amb_tri_req *req, int n, double t0, double 
         return req[x].cube_id != req[y].cube_id ? req[x].cube_id < req[y].cube_id : req[x].r1 < req[y].r1;
     });
     const PlaneSrc src{&c->planes};
-    struct J {
-        int q;
-        Cube *cb;
-        long ng, P;
-        int nseg;
-        void *X[3];
-        size_t part, nyq, out, out64;
-    };
-    std::vector<J> w;
+    std::vector<LJ> w;
     size_t part_tot = 0, out_tot = 0;
-    std::vector<int> norm_redo;
+    c->lag_margin = 0.f;
+    /* a wave's lag arrays land in device buffer hb and are copied to pinned
+       staging hb on the copy stream (ev_cp[hb]); the previous wave's host
+       copies (prev, buffer hb ^ 1) run while this wave's kernels run */
+    int hb = 0;
+    std::vector<LJ> prev;
+    int rc0 = copy_stream(c);
+    if (rc0) return rc0;
+
+    auto finish_prev = [&]() -> int {
+        if (prev.empty()) return AMB_OK;
+        const hipError_t e = hipEventSynchronize(c->ev_cp[hb ^ 1]);
+        if (e != hipSuccess) {
+            prev.clear();
+            return fail(c, AMB_EHIP, "copy of a lag-Fourier wave's results: %s", hipGetErrorString(e));
+        }
+        const char *h = (const char *)c->hstage[hb ^ 1];
+        std::vector<std::tuple<void *, const void *, size_t>> cp;
+        for (auto &j : prev) {
+            const amb_tri_req &q = req[j.q];
+            const Cube &cb = *j.cb;
+            const size_t cells = cb.h_lagcnt.size();
+            if (j.norm) {
+                cp.emplace_back(q.norm, h + j.out, sizeof(float) * cells);
+                if (norm64 && norm64[j.q]) cp.emplace_back(norm64[j.q], h + j.out64, sizeof(double) * cells);
+                continue;
+            }
+            cp.emplace_back(q.bl, h + j.out, sizeof(float) * cells);
+            if (bl64 && bl64[j.q]) cp.emplace_back(bl64[j.q], h + j.out64, sizeof(double) * cells);
+            if (q.cnt) cp.emplace_back(q.cnt, cb.h_lagcnt.data(), sizeof(int32_t) * cells);
+        }
+        par_copy(cp);
+        const double tdone = now_ms() - t0;
+        if (done_ms)
+            for (auto &j : prev) done_ms[j.q] = tdone;
+        prev.clear();
+        return AMB_OK;
+    };
 
     auto flush = [&]() -> int {
         if (w.empty()) return AMB_OK;
-        /* buffers */
         size_t st_need = 0;
         for (auto &j : w) st_need = std::max(st_need, sizeof(double2) * (size_t)(j.cb->lag_L / 2 + 1) * j.cb->lag_L);
         void *p;
@@ -578,167 +898,239 @@ int amb::lagft_run(amb_ctx *c, const amb_tri_req *req, int n, double t0, double 
         p = c->lag_st;
         if (!rc) rc = grow(c, &p, &c->lag_st_bytes, st_need, "lag-Fourier spectrum");
         c->lag_st = p;
-        p = c->lag_out;
-        if (!rc) rc = grow(c, &p, &c->lag_out_bytes, out_tot, "lag arrays");
-        c->lag_out = p;
-        if (!rc) rc = grow(c, &c->wdesc, &c->wdesc_bytes, sizeof(LagJob) * w.size(), "wave descriptors");
-        if (!rc) rc = grow_host(c, &c->hstage[0], &c->hstage_bytes[0], out_tot);
+        /* buffer hb's last copy finished with finish_prev of the wave before */
+        void **outp = hb ? &c->lag_out2 : &c->lag_out;
+        p = *outp;
+        if (!rc) rc = grow(c, &p, hb ? &c->lag_out2_bytes : &c->lag_out_bytes, out_tot, "lag arrays");
+        *outp = p;
+        const size_t dbytes = (sizeof(LagJob<float>) + sizeof(LagJob<double>)) * w.size() + 256;
+        if (!rc) rc = grow(c, &c->wdesc, &c->wdesc_bytes, dbytes, "wave descriptors");
+        if (!rc) rc = grow_host(c, &c->hstage[hb], &c->hstage_bytes[hb], out_tot);
         if (rc) { unpin_all(c); w.clear(); return rc; }
-        std::vector<LagJob> d(w.size());
-        long items = 0, nitems = 0;
+        char *dout = (char *)*outp;
+        std::vector<LagJob<float>> d32;
+        std::vector<LagJob<double>> d64;
+        std::vector<int> slot(w.size()); /* index of each job in its descriptor array */
+        long it32 = 0, it64 = 0, nit32 = 0, nit64 = 0;
         for (size_t i = 0; i < w.size(); ++i) {
-            const J &j = w[i];
-            const int L = j.cb->lag_L;
-            LagJob &g = d[i];
-            g.F = (const float2 *)j.X[0];
-            g.G = (const float2 *)j.X[1];
-            g.H = (const float2 *)j.X[2];
-            for (int k = 0; k < 3; ++k) {
-                const int *ex = (const int *)((const float2 *)j.X[k] + (size_t)j.P * j.cb->lag_lhp) + 1;
-                (k == 0 ? g.ex1 : k == 1 ? g.ex2 : g.ex3) = ex;
+            if (w[i].norm) {
+                slot[i] = (int)d64.size();
+                d64.emplace_back();
+                fill_job(d64.back(), w[i], (char *)c->lag_part, &it64, &nit64);
+            } else {
+                slot[i] = (int)d32.size();
+                d32.emplace_back();
+                fill_job(d32.back(), w[i], (char *)c->lag_part, &it32, &nit32);
             }
-            g.part = (float2 *)((char *)c->lag_part + j.part);
-            g.nyq = (float2 *)((char *)c->lag_part + j.nyq);
-            g.P = j.P;
-            g.lhp = j.cb->lag_lhp;
-            g.L = L;
-            g.nseg = j.nseg;
-            g.seg = SEG;
-            g.item0 = items;
-            g.nitem0 = nitems;
-            items += (long)j.nseg * tiles2(L) * tiles3(L);
-            nitems += j.nseg;
         }
-        hipError_t e = hipMemcpyAsync(c->wdesc, d.data(), sizeof(LagJob) * d.size(), hipMemcpyHostToDevice, c->stream);
-        const LagJob *dj = (const LagJob *)c->wdesc;
-        timer_start(c, &c->st.ms_contract);
-        if (e == hipSuccess) {
-            hipLaunchKernelGGL(k_lagft_tiles, dim3((unsigned)((items + 7) / 8 * 8)), dim3(256), 0, c->stream, dj,
-                               (int)d.size(), items);
-            e = hipGetLastError();
+        char *dd = (char *)c->wdesc;
+        LagJob<float> *g32 = (LagJob<float> *)dd;
+        LagJob<double> *g64 = (LagJob<double> *)(dd + (sizeof(LagJob<float>) * d32.size() + 255) / 256 * 256);
+        hipError_t e = hipSuccess;
+        if (!d32.empty()) e = hipMemcpyAsync(g32, d32.data(), sizeof(LagJob<float>) * d32.size(), hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess && !d64.empty())
+            e = hipMemcpyAsync(g64, d64.data(), sizeof(LagJob<double>) * d64.size(), hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess && !d32.empty()) {
+            timer_start(c, &c->st.ms_contract);
+            e = launch_products<float>(c->stream, g32, (int)d32.size(), it32, nit32);
+            timer_stop(c);
+            c->st.n_contract_launch++;
         }
-        if (e == hipSuccess) {
-            hipLaunchKernelGGL(k_lagft_nyq, dim3((unsigned)((nitems + 7) / 8 * 8)), dim3(256), 0, c->stream, dj,
-                               (int)d.size(), nitems);
-            e = hipGetLastError();
+        const bool any_norm = !d64.empty();
+        if (e == hipSuccess && any_norm) {
+            e = amb_launch_fill_u32(c->stream, c->margin + 2, 1, 0u);
+            timer_start(c, &c->st.ms_norm);
+            if (e == hipSuccess) e = launch_products<double>(c->stream, g64, (int)d64.size(), it64, nit64);
+            timer_stop(c);
         }
-        timer_stop(c);
-        c->st.n_contract_launch++;
         timer_start(c, &c->st.ms_reduce);
         for (size_t i = 0; i < w.size() && e == hipSuccess; ++i) {
-            const J &j = w[i];
+            const LJ &j = w[i];
             const Cube &cb = *j.cb;
             const int L = cb.lag_L;
             double2 *St = (double2 *)c->lag_st;
-            hipLaunchKernelGGL(k_lagft_reduce, dim3((unsigned)((L + 31) / 32), (unsigned)((L / 2 + 31) / 32)), dim3(256),
-                               0, c->stream, d[i].part, j.nseg, L, St);
-            hipLaunchKernelGGL(k_lagft_reduce_nyq, dim3((unsigned)((L + 255) / 256)), dim3(256), 0, c->stream, d[i].nyq,
-                               j.nseg, L, St);
+            const dim3 gr((unsigned)((L + 31) / 32), (unsigned)((L / 2 + 1 + 31) / 32));
+            if (j.norm) {
+                const LagJob<double> &g = d64[slot[i]];
+                hipLaunchKernelGGL((k_lagft_reduce<double, false>), gr, dim3(256), 0, c->stream, g.part, j.nseg,
+                                   j.ntile, L, nullptr, St);
+                if (j.nyq_sep)
+                    hipLaunchKernelGGL(k_lagft_reduce_nyq<double>, dim3((unsigned)((L + 255) / 256)), dim3(256), 0,
+                                       c->stream, g.nyq, j.nseg, L, St);
+            } else {
+                const LagJob<float> &g = d32[slot[i]];
+                if (j.mode == 1)
+                    hipLaunchKernelGGL((k_lagft_reduce<float, true>), gr, dim3(256), 0, c->stream, g.part, j.nseg,
+                                       j.ntile, L, j.tmap, St);
+                else
+                    hipLaunchKernelGGL((k_lagft_reduce<float, false>), gr, dim3(256), 0, c->stream, g.part, j.nseg,
+                                       j.ntile, L, nullptr, St);
+                /* the column rode in the last tile (mode 0, TB | L/2), was a
+                   tile column (mode 0 otherwise), or had its own kernel */
+                const bool folded = j.mode == 0 && (L / 2) % tile_b<float>() == 0;
+                if (j.nyq_sep || folded)
+                    hipLaunchKernelGGL(k_lagft_reduce_nyq<float>, dim3((unsigned)((L + 255) / 256)), dim3(256), 0,
+                                       c->stream, g.nyq, j.nseg, L, St);
+            }
             const FftPlan &pl = c->plans[L];
             const double2 *tw = c->tw64[L];
             const size_t lds = sizeof(double2) * 2 * (size_t)L;
             hipLaunchKernelGGL(k_lagft_cols, dim3((unsigned)(L / 2 + 1)), dim3(64), lds, c->stream, St, pl, tw);
             const int W = 2 * cb.nc10 + 1;
-            float *o = (float *)((char *)c->lag_out + j.out);
-            double *o64 = j.out64 != (size_t)-1 ? (double *)((char *)c->lag_out + j.out64) : nullptr;
+            float *o = (float *)(dout + j.out);
+            double *o64 = j.out64 != (size_t)-1 ? (double *)(dout + j.out64) : nullptr;
+            const int *e1 = j.norm ? nullptr : d32[slot[i]].ex1, *e2 = j.norm ? nullptr : d32[slot[i]].ex2,
+                      *e3 = j.norm ? nullptr : d32[slot[i]].ex3;
             hipLaunchKernelGGL(k_lagft_rows, dim3((unsigned)((W + 1) / 2)), dim3(64), lds, c->stream, St, pl, tw, cb.nc10,
-                               cb.d_lagcnt, d[i].ex1, d[i].ex2, d[i].ex3, o, o64);
+                               cb.d_lagcnt, e1, e2, e3, j.norm ? (double)j.P : 0.0, o, o64,
+                               j.norm ? c->margin + 2 : nullptr);
             e = hipGetLastError();
         }
         timer_stop(c);
-        if (e == hipSuccess) e = hipMemcpyAsync(c->hstage[0], c->lag_out, out_tot, hipMemcpyDeviceToHost, c->stream);
+        unsigned mbits = 0;
+        if (e == hipSuccess) e = hipEventRecord(c->ev_red, c->stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(c->cstream, c->ev_red, 0);
+        if (e == hipSuccess) e = hipMemcpyAsync(c->hstage[hb], dout, out_tot, hipMemcpyDeviceToHost, c->cstream);
+        if (e == hipSuccess) e = hipEventRecord(c->ev_cp[hb], c->cstream);
+        if (e == hipSuccess && any_norm)
+            e = hipMemcpyAsync(&mbits, c->margin + 2, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream);
+        /* the previous wave's host copies while this wave runs */
+        const int prc = finish_prev();
         const hipError_t es = hipStreamSynchronize(c->stream);
         if (e == hipSuccess) e = es;
         timers_collect(c);
         unpin_all(c);
         if (e != hipSuccess) {
+            hipStreamSynchronize(c->cstream);
             w.clear();
-            return fail(c, AMB_EHIP, "lag-Fourier wave of %zu triangles: %s", d.size(), hipGetErrorString(e));
+            return fail(c, AMB_EHIP, "lag-Fourier wave of %zu jobs: %s", slot.size(), hipGetErrorString(e));
         }
-        const double tdone = now_ms() - t0;
+        if (prc) { hipStreamSynchronize(c->cstream); w.clear(); return prc; }
+        if (any_norm) {
+            float m;
+            memcpy(&m, &mbits, sizeof m);
+            if (m > c->st.cert_margin_max) c->st.cert_margin_max = m;
+            if (m > c->lag_margin) c->lag_margin = m;
+            if (!(m <= 1.0f / 32.0f)) {
+                hipStreamSynchronize(c->cstream);
+                w.clear();
+                return fail(c, AMB_ECERT, "lag-Fourier normalisation: FP64 sums off their 1/8 grid by %g", (double)m);
+            }
+        }
         for (auto &j : w) {
-            const amb_tri_req &q = req[j.q];
             const Cube &cb = *j.cb;
-            const size_t cells = cb.h_lagcnt.size();
-            memcpy(q.bl, (const char *)c->hstage[0] + j.out, sizeof(float) * cells);
-            if (bl64 && bl64[j.q]) memcpy(bl64[j.q], (const char *)c->hstage[0] + j.out64, sizeof(double) * cells);
-            if (q.cnt) memcpy(q.cnt, cb.h_lagcnt.data(), sizeof(int32_t) * cells);
+            if (j.norm) {
+                ++c->st.n_norm_exact;
+                continue;
+            }
             const int L = cb.lag_L;
             c->st.evals += cb.lag_evals;
             c->st.flop_contract += 2.0 * (double)cb.lag_pairs * (double)j.P;
             c->st.flop_contract_distinct += 2.0 * (double)cb.lag_pairs * (double)j.P;
-            /* 14 flops per (k2, k3, pixel) of the half plane k3 <= L/2 */
-            c->st.flop_lagft += 14.0 * (double)L * (double)(L / 2 + 1) * (double)j.P;
-            c->st.flop_contract_exec += 14.0 * (double)tiles2(L) * LT * ((double)tiles3(L) * LT + 1) * (double)j.P;
-            if (done_ms) done_ms[j.q] = tdone;
-            if (q.norm) norm_redo.push_back(j.q);
+            /* 14 flops per distinct (k2, k3, pixel) of the half spectrum
+               k3 <= L/2: half of them where r1 == r2 pairs equal sums */
+            c->st.flop_lagft += 14.0 * (double)L * (double)(L / 2 + 1) * (double)j.P * (j.mode == 1 ? 0.5 : 1.0);
+            /* issued: whole tiles, plus the k3 = L/2 column */
+            c->st.flop_contract_exec += 14.0 * ((double)j.ntile * tile_a<float>() * tile_b<float>() + L) * (double)j.P;
         }
+        prev = std::move(w);
         w.clear();
+        hb ^= 1;
         part_tot = out_tot = 0;
         return AMB_OK;
     };
 
-    for (int k = 0; k < n; ++k) {
-        const int qi = order[k];
-        const amb_tri_req &q = req[qi];
-        if (!q.bl) return fail(c, AMB_EARG, "request %d has no bl buffer", qi);
-        Cube *cb;
-        int rc = find_cube(c, q.cube_id, &cb);
-        if (rc) { unpin_all(c); return rc; }
-        if (!cb->chans) { unpin_all(c); return fail(c, AMB_ESTATE, "channels not set for cube %d", q.cube_id); }
-        const int nr = (int)c->ulow.size();
-        if (nr == 0) { unpin_all(c); return fail(c, AMB_ESTATE, "rings not set"); }
-        if (q.r1 < 0 || q.r1 >= nr || q.r2 < 0 || q.r2 >= nr || q.r3 < 0 || q.r3 >= nr) {
-            unpin_all(c);
-            return fail(c, AMB_EARG, "triangle (%d,%d,%d) outside [0,%d)", q.r1, q.r2, q.r3, nr);
-        }
-        if ((rc = lag_setup(c, *cb))) { unpin_all(c); return rc; }
-        const long ng = amb_ng(c->uhigh[q.r1], cb->Du);
-        if (ng <= 0 || ng > 2730) { unpin_all(c); return fail(c, AMB_EARG, "Ng=%ld out of range", ng); }
-        const int L = cb->lag_L;
-        {   /* FP64 twiddles and plan of L for the 2-D inverse */
-            float2 *tw;
-            FftPlan pl;
-            if ((rc = twiddles(c, L, &tw, &pl)) || (rc = twiddles64(c, L, nullptr))) { unpin_all(c); return rc; }
-        }
-        J j{};
-        j.q = qi;
-        j.cb = cb;
-        j.ng = ng;
-        j.P = ng * ng;
-        j.nseg = (int)((j.P + SEG - 1) / SEG);
-        const size_t pb = sizeof(float2) * (size_t)j.nseg * ((size_t)tiles2(L) * tiles3(L) * LT * LT + L);
-        const size_t cells = cb->h_lagcnt.size();
-        const bool want64 = bl64 && bl64[qi];
-        const size_t ob = al(sizeof(float) * cells) + (want64 ? al(sizeof(double) * cells) : 0);
-        if (!w.empty() && (part_tot + al(pb) > PART_CAP || out_tot + ob > OUT_CAP || (int)w.size() >= MAX_JOBS))
-            if ((rc = flush())) return rc;
-        const int rr[3] = {q.r1, q.r2, q.r3};
-        for (int t = 0; t < 3 && !rc; ++t) {
-            rc = get_planes(c, src, q.cube_id, 9, rr[t], (int)ng, &j.X[t]);
-            if (rc == AMB_EBUDGET && !w.empty()) {
-                /* the wave's planes hold the memory: run it, then resolve again */
-                if ((rc = flush())) return rc;
-                t = -1;
-                rc = AMB_OK;
+    auto run_all = [&]() -> int {
+        for (int k = 0; k < n; ++k) {
+            const int qi = order[k];
+            const amb_tri_req &q = req[qi];
+            if (!q.bl) return fail(c, AMB_EARG, "request %d has no bl buffer", qi);
+            Cube *cb;
+            int rc = find_cube(c, q.cube_id, &cb);
+            if (rc) { unpin_all(c); return rc; }
+            if (!cb->chans) { unpin_all(c); return fail(c, AMB_ESTATE, "channels not set for cube %d", q.cube_id); }
+            const int nr = (int)c->ulow.size();
+            if (nr == 0) { unpin_all(c); return fail(c, AMB_ESTATE, "rings not set"); }
+            if (q.r1 < 0 || q.r1 >= nr || q.r2 < 0 || q.r2 >= nr || q.r3 < 0 || q.r3 >= nr) {
+                unpin_all(c);
+                return fail(c, AMB_EARG, "triangle (%d,%d,%d) outside [0,%d)", q.r1, q.r2, q.r3, nr);
+            }
+            if ((rc = lag_setup(c, *cb))) { unpin_all(c); return rc; }
+            const long ng = amb_ng(c->uhigh[q.r1], cb->Du);
+            if (ng <= 0 || ng > 2730) { unpin_all(c); return fail(c, AMB_EARG, "Ng=%ld out of range", ng); }
+            const int L = cb->lag_L;
+            {   /* twiddles and plan of L: FP32 for the spectra, FP64 for the 2-D inverse */
+                float2 *tw;
+                FftPlan pl;
+                if ((rc = twiddles(c, L, &tw, &pl)) || (rc = twiddles64(c, L, nullptr))) { unpin_all(c); return rc; }
+            }
+            const size_t cells = cb->h_lagcnt.size();
+            const int rr[3] = {q.r1, q.r2, q.r3};
+            for (int pass = 0; pass < (q.norm ? 2 : 1); ++pass) {
+                const bool nm = pass == 1;
+                LJ j{};
+                j.q = qi;
+                j.norm = nm;
+                j.cb = cb;
+                j.ng = ng;
+                j.P = ng * ng;
+                j.nseg = (int)((j.P + SEG - 1) / SEG);
+                j.mode = 0;
+                j.ntile = nm ? tiles2<double>(L) * tiles3<double>(L) : tiles2<float>(L) * tiles3<float>(L);
+                if (!nm && q.r1 == q.r2 && !c->no_sym && L % 128 == 0) {
+                    /* r1 == r2: the (i, j) tiles, when fewer */
+                    auto it = c->lag_pairs_tab.find(L);
+                    if (it == c->lag_pairs_tab.end()) {
+                        const std::vector<int2> tl = pair_tiles(L);
+                        const int nt = L / 128;
+                        std::vector<int> tm((size_t)nt * nt, -1);
+                        for (size_t t = 0; t < tl.size(); ++t) tm[(size_t)tl[t].x * nt + tl[t].y] = (int)t;
+                        void *d = nullptr;
+                        const size_t tb = sizeof(int2) * tl.size(), mb = sizeof(int) * tm.size();
+                        if ((rc = dev_alloc(c, &d, tb + mb, "lag-Fourier tile tables"))) { unpin_all(c); return rc; }
+                        HIPCHK(c, hipMemcpy(d, tl.data(), tb, hipMemcpyHostToDevice));
+                        HIPCHK(c, hipMemcpy((char *)d + tb, tm.data(), mb, hipMemcpyHostToDevice));
+                        it = c->lag_pairs_tab.emplace(L, std::make_tuple(d, (int)tl.size(), tb + mb)).first;
+                    }
+                    const int n1 = std::get<1>(it->second);
+                    if (n1 < j.ntile) {
+                        j.mode = 1;
+                        j.ntile = n1;
+                        j.tlist = (const int2 *)std::get<0>(it->second);
+                        j.tmap = (const int *)((const char *)std::get<0>(it->second) + sizeof(int2) * n1);
+                    }
+                }
+                /* the k3 = L/2 column in k_lagft_nyq: mode 1, and FP64 where it is not a tile column */
+                j.nyq_sep = (j.mode == 1 || (nm && (L / 2) % tile_b<double>() == 0)) ? 1 : 0;
+                const size_t pb = nm ? job_part_bytes<double>(j, L) : job_part_bytes<float>(j, L);
+                const bool want64 = nm ? (norm64 && norm64[qi]) : (bl64 && bl64[qi]);
+                const size_t ob = al(sizeof(float) * cells) + (want64 ? al(sizeof(double) * cells) : 0);
+                if (!w.empty() && (part_tot + al(pb) > PART_CAP || out_tot + ob > OUT_CAP || (int)w.size() >= MAX_JOBS))
+                    if ((rc = flush())) return rc;
+                for (int t = 0; t < 3 && !rc; ++t) {
+                    rc = get_planes(c, src, q.cube_id, nm ? 10 : 9, rr[t], (int)ng, &j.X[t]);
+                    if (rc == AMB_EBUDGET && !w.empty()) {
+                        /* the wave's planes hold the memory: run it, then resolve again */
+                        if ((rc = flush())) return rc;
+                        t = -1;
+                        rc = AMB_OK;
+                    }
+                }
+                if (rc) { unpin_all(c); return rc; }
+                j.part = part_tot;
+                j.nyq = part_tot + (nm ? job_part_bytes<double>(j, L) - sizeof(double2) * (size_t)j.nseg * L
+                                       : job_part_bytes<float>(j, L) - sizeof(float2) * (size_t)j.nseg * L);
+                part_tot += al(pb);
+                j.out = out_tot;
+                j.out64 = want64 ? out_tot + al(sizeof(float) * cells) : (size_t)-1;
+                out_tot += ob;
+                w.push_back(j);
             }
         }
-        if (rc) { unpin_all(c); return rc; }
-        j.part = part_tot;
-        j.nyq = part_tot + sizeof(float2) * (size_t)j.nseg * tiles2(L) * tiles3(L) * LT * LT;
-        part_tot += al(pb);
-        j.out = out_tot;
-        j.out64 = want64 ? out_tot + al(sizeof(float) * cells) : (size_t)-1;
-        out_tot += ob;
-        w.push_back(j);
-    }
-    int rc = flush();
-    if (rc) return rc;
-    /* v2 normalisation of this form: the exact FP64 path */
-    for (int i : norm_redo) {
-        const amb_tri_req &q = req[i];
-        if ((rc = norm_exact(c, q.cube_id, q.r1, q.r2, q.r3, q.norm, nullptr, nullptr))) return rc;
-        if (done_ms) done_ms[i] = now_ms() - t0;
-    }
-    return AMB_OK;
+        int rc = flush();
+        return rc ? rc : finish_prev();
+    };
+    const int rc = run_all();
+    hipStreamSynchronize(c->cstream); /* no copy left running, also after an error */
+    return rc;
 }

@@ -508,7 +508,7 @@ int amb_bispec_multi(amb_ctx *c, const amb_tri_req *req, int n)
     Lock lk(c->mu);
     const double t0 = now_ms();
     c->batch_ms.assign(n, 0.0);
-    if (c->prec == AMB_PREC_LAGFT) return lagft_run(c, req, n, t0, c->batch_ms.data(), nullptr);
+    if (c->prec == AMB_PREC_LAGFT) return lagft_run(c, req, n, t0, c->batch_ms.data(), nullptr, nullptr);
     if (c->batch_mode == AMB_BATCH_PER_TRIANGLE) {
         std::vector<int> order(n);
         for (int i = 0; i < n; ++i) order[i] = i;

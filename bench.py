@@ -1,17 +1,22 @@
 """Benchmark of the 4_fbe estimator hot path on one MI355X.
 
 A step is one slice of the closed triangles of the workload: ring gather +
-c2r of every (ring, channel) plane the slice needs, the per-p1 MFMA
-contraction, the lag reduction with counts, and the copy of every
-triangle's W x W accumulator and count arrays to host memory.  The cube is
+c2r of every (ring, channel) plane the slice needs, the sums over reference
+planes (channel spectra and their products in the lag-Fourier form, or the
+per-p1 MFMA contraction and lag reduction), the counts, and the copy of
+every triangle's W x W accumulator and count arrays to host memory.  The cube is
 resident in HBM before the timed region starts.
 
 Default workload (the metric's configuration): 768 channels x 512^2 grid,
 24 log-spaced annular bins 1..200 cells, all 506 closed triangles, all lags
-(nc10 = 767), split-BF16 contraction.
+(nc10 = 767), the lag-Fourier form (the sum over reference planes as a
+channel-axis triple correlation in the DFT domain, FP32 products, FP64
+sums).  The split-BF16 MFMA contraction is timed over the same slices
+afterwards and reported beside it (--compare).
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 2] [--config cfg3]
-                    [--precision bf16x3|fp32|fp8] [--no-cpu-baseline]
+                    [--precision lagft|bf16x3|fp16x3|fp32|fp8]
+                    [--compare bf16x3|...|none] [--no-cpu-baseline]
 
 A pass over the workload is split into --steps slices of equal contraction
 cost (contiguous in the triangle list), so the timed steps together cover
@@ -73,11 +78,12 @@ def log(msg):
 def tri_costs(A, tris, hi, precision="bf16x3"):
     """Relative contraction cost of each triangle: Ng(r1)^2 pixels times the
     triplets it computes (half of them where r1 == r2 or r2 == r3 lets the
-    kernel mirror the other half; the lag-Fourier form has no such halving)."""
+    kernel mirror the other half); in the lag-Fourier form Ng(r1)^2 times
+    its tiles."""
     ng = np.array([A.ng(hi[t[0]]) for t in tris], np.float64)
     sym = np.array([(t[0] == t[1]) or (t[1] == t[2]) for t in tris])
-    if precision == "lagft":
-        return ng * ng
+    if precision == "lagft":  # r1 == r2: the symmetric (i, j) tiles, about 0.63 of the (k2, k3) ones
+        return ng * ng * np.where(np.array([t[0] == t[1] for t in tris]), 0.65, 1.0)
     return ng * ng * np.where(sym, 0.5, 1.0)
 
 
@@ -113,7 +119,11 @@ def main():
     ap.add_argument("--duration", type=float, default=90.0, help="cfg5: seconds of arrivals")
     ap.add_argument("--probe-s", type=float, default=20.0, help="cfg5: seconds of the one-at-a-time capacity probe")
     ap.add_argument("--upload-frac", type=float, default=0.1, help="cfg5: fraction of requests that carry a cube")
-    ap.add_argument("--precision", default="bf16x3", choices=["bf16x3", "fp16x3", "fp32", "fp8", "lagft"])
+    ap.add_argument("--precision", default="lagft", choices=["bf16x3", "fp16x3", "fp32", "fp8", "lagft"],
+                    help="lagft (default): the lag-Fourier form; bf16x3 / fp16x3 / fp32 / fp8: the MFMA contraction")
+    ap.add_argument("--compare", default="bf16x3", choices=["none", "bf16x3", "fp16x3", "fp32", "fp8", "lagft"],
+                    help="cfg1-3: a second form timed over the same slices after the main line's pass and reported "
+                         "beside it (none: skip)")
     ap.add_argument("--tri-limit", type=int, default=0,
                     help="development only: K triangles (the line is then not the metric workload)")
     ap.add_argument("--tri-pick", default="first", choices=["first", "largest"])
@@ -196,6 +206,19 @@ def main():
         n = len(sl)
         ctx.bispec_batch(cube, tris[sl], out=tuple(None if o is None else o[:n] for o in out))
 
+    def timed_pass(slc):
+        """warm-up on the heaviest slice, then every slice once; returns
+        (seconds per slice, stats, per-triangle latencies)"""
+        step(slc[int(np.argmax([costs[x].sum() for x in slc]))])
+        ctx.reset_stats()
+        secs, lt = [], []
+        for sl in slc:
+            t0 = time.time()
+            step(sl)
+            secs.append(time.time() - t0)
+            lt.extend(ctx.batch_times().tolist())
+        return secs, ctx.stats(), lt
+
     # warm-up: the heaviest slices first, so every scratch buffer has its
     # full size before the timed steps
     heavy = sorted(range(len(slices)), key=lambda k: -costs[slices[k]].sum())
@@ -259,9 +282,45 @@ def main():
               "fft_gbs": round(st["bytes_fft"] / (st["ms_fft"] * 1e-3) / 1e9, 1) if st["ms_fft"] > 0 else None}
 
     err = None
+    ocache = {}
     if not args.no_error:
         err = fp64_error(A, ctx, cube, cfg, gv, samp, uf, fk, lo, hi, tris,
-                         deadline=t_run + args.budget_s - args.cpu_seconds - 30.0)
+                         deadline=t_run + args.budget_s - args.cpu_seconds - 30.0, cache=ocache)
+    # the second form over the same slices, when the run budget allows one
+    # more pass of it (estimated from its flop rate on this workload)
+    compare = None
+    if args.compare not in ("none", args.precision) and not args.tri_limit:
+        est = {"bf16x3": 4.6, "fp16x3": 4.9, "fp8": 2.5, "fp32": 16.0, "lagft": 0.25}[args.compare] * elapsed \
+            if args.precision == "lagft" else elapsed
+        if time.time() - t_run + 1.2 * est + args.cpu_seconds + 40.0 > args.budget_s:
+            compare = {"precision": args.compare, "skipped": "run budget (estimated %.0f s)" % est}
+        else:
+            ctx.set_precision(args.compare)
+            secs, st2, _ = timed_pass(slices)
+            e2 = sum(secs)
+            ms2 = st2["ms_contract"]
+            lag2 = args.compare == "lagft"
+            w2 = st2["flop_lagft"] if lag2 else st2["flop_contract_distinct"]
+            a2 = w2 / (ms2 * 1e-3) / 1e12 if ms2 > 0 else 0.0
+            compare = {"precision": args.compare, "value": round(st2["evals"] / e2, 1), "unit": "evals/s",
+                       "pass_s": round(e2, 3), "ms_per_step": round(1e3 * e2 / len(slices), 2),
+                       "roofline": {"bound": "valu" if lag2 else "mfma", "achieved": round(a2, 2),
+                                    "peak": PEAK_TFLOPS[args.compare], "unit": "TFLOP/s",
+                                    "frac": round(a2 / PEAK_TFLOPS[args.compare], 4),
+                                    "kernel": "k_lagft_tiles" if lag2 else "k_contract<%s>" % args.compare,
+                                    "issued_tflops": round(st2["flop_contract_exec"] / (ms2 * 1e-3) / 1e12, 2)
+                                    if ms2 > 0 else None,
+                                    "avg_launch_ms": round(ms2 / max(st2["n_contract_launch"], 1), 3)},
+                       "stages": {"fft_ms": round(st2["ms_fft"], 2), "contract_ms": round(st2["ms_contract"], 2),
+                                  "reduce_ms": round(st2["ms_reduce"], 2)},
+                       "speedup_of_main_form": round(e2 / elapsed, 3)}
+            if not args.no_error:
+                compare["rel_err_vs_fp64"] = fp64_error(A, ctx, cube, cfg, gv, samp, uf, fk, lo, hi, tris,
+                                                        deadline=t_run + args.budget_s - args.cpu_seconds - 30.0,
+                                                        cache=ocache)
+            ctx.set_precision(args.precision)
+            log("compare %s: %.3g evals/s, pass %.1f s (main form %.1f s)" % (args.compare, compare["value"], e2,
+                                                                            elapsed))
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(A, cfg, gv, samp, uf, fk, lo, hi, tris,
@@ -293,7 +352,7 @@ def main():
                                 "note": "time from the start of a step to each triangle's results in host memory "
                                         "(the triangles of one grouped wave complete together); request latency "
                                         "under load is --config cfg5"},
-        "rel_err_vs_fp64": err, "cpu_baseline": cpu,
+        "rel_err_vs_fp64": err, "cpu_baseline": cpu, "compare_form": compare,
         "launch_config": tuning,
         "upload_s_pcie_incl": round(upload_s, 3),
         "run_s": round(time.time() - t_run, 1),
@@ -724,10 +783,12 @@ def copy_calib(mib):
     print(json.dumps({"copy_calib_bytes": n, "dispatches": 3}), flush=True)
 
 
-def fp64_error(A, ctx, cube, cfg, gv, samp, uf, fk, lo, hi, tris, deadline=None):
+def fp64_error(A, ctx, cube, cfg, gv, samp, uf, fk, lo, hi, tris, deadline=None, cache=None):
     """Relative error vs the FP64 oracle on sampled lag cells of four
     triangles (Ng quantiles, the largest first); a triangle whose oracle run
-    would pass the deadline is left out and said so."""
+    would pass the deadline is left out and said so.  cache (a dict): the
+    oracle's cells by triangle, filled on the first call and reused by later
+    ones (another precision form on the same cube)."""
     import oracle as O
     N, nc, nc10 = cfg["N"], cfg["nc"], cfg["nc10"]
     mask = O.chan_mask(nc, fk, uf)
@@ -747,15 +808,19 @@ def fp64_error(A, ctx, cube, cfg, gv, samp, uf, fk, lo, hi, tris, deadline=None)
             continue
         r1, r2, r3 = map(int, tris[k])
         res = ctx.bispec_ex(cube, r1, r2, r3)
-        W = 2 * nc10 + 1
-        cells = np.argwhere(res["cnt"] > 0)
-        sel = cells[rng.choice(len(cells), size=min(48, len(cells)), replace=False)]
-        sel = np.vstack([sel, [[nc10, nc10]]])
-        d2 = sel[:, 0] - nc10
-        d3 = sel[:, 1] - nc10
         t0 = time.time()
-        b64, c64, _, babs = O.bispec_cells(N, nc, 1.0, gv, samp, lo, hi, r1, r2, r3, mask, nc10, d2, d3,
-                                           nthreads=min(16, os.cpu_count() or 1), want_abs=True)
+        if cache is not None and (r1, r2, r3) in cache:
+            sel, b64, c64, babs = cache[(r1, r2, r3)]
+        else:
+            cells = np.argwhere(res["cnt"] > 0)
+            sel = cells[rng.choice(len(cells), size=min(48, len(cells)), replace=False)]
+            sel = np.vstack([sel, [[nc10, nc10]]])
+            d2 = sel[:, 0] - nc10
+            d3 = sel[:, 1] - nc10
+            b64, c64, _, babs = O.bispec_cells(N, nc, 1.0, gv, samp, lo, hi, r1, r2, r3, mask, nc10, d2, d3,
+                                               nthreads=min(16, os.cpu_count() or 1), want_abs=True)
+            if cache is not None:
+                cache[(r1, r2, r3)] = (sel, b64, c64, babs)
         got = res["bl64"][sel[:, 0], sel[:, 1]]
         if np.linalg.norm(b64) <= 1e-12 * np.linalg.norm(babs):
             log("triangle %s: B == 0 up to FP64 rounding (rings cannot close), skipped" % ((r1, r2, r3),))

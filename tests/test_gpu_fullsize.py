@@ -119,15 +119,17 @@ def test_metric_workload_error_vs_fp64(ctx, big, metric_cells, prec):
         ctx.set_precision("bf16x3")
 
 
-def test_metric_workload_normalisation_is_exact(ctx, big):
+@pytest.mark.parametrize("prec", ["bf16x3", "lagft"])
+def test_metric_workload_normalisation_is_exact(ctx, big, prec):
     """The amber_v2 normalisation at the metric workload (768 ch x 512^2,
     Ng = 612): equal bit for bit to the FP64 oracle's sums snapped to their
     1/8 grid of Ng^2, on 49 lag cells, whether the fast path certified the
-    triangle or it was recomputed in FP64."""
+    triangle or it was recomputed in FP64 (split form), or computed by the
+    FP64 lag-Fourier stages (lagft, certificate margin far inside 1/32)."""
     gv, samp, _ = big
     lo, hi = A.log_rings(1, 200, 24)
     t = (23, 20, 20)
-    ctx.set_precision("bf16x3")
+    ctx.set_precision(prec)
     ctx.set_rings(lo, hi)
     c1 = ctx.load_cube(N, NC, 1.0, 0, gv, samp)
     ctx.set_channels(c1, None, NC10)
@@ -146,5 +148,8 @@ def test_metric_workload_normalisation_is_exact(ctx, big):
         assert np.max(np.abs(n64 / P * 8 - np.rint(n64 / P * 8))) < 1e-6
         assert np.array_equal(res["cnt"][d2 + NC10, d3 + NC10], c64)
         assert np.array_equal(res["norm64"][d2 + NC10, d3 + NC10], P * np.rint(n64 / P * 8) / 8)
+        if prec == "lagft":
+            assert res["cert_margin"] < 1e-3
     finally:
         ctx.release_cube(c1)
+        ctx.set_precision("bf16x3")

@@ -100,3 +100,49 @@ def test_refused_in_graphs_and_serving(ctx):
         ctx.release_cube(cube)
     finally:
         ctx.set_precision("bf16x3")
+
+
+def test_r1_eq_r2_pair_tiles(ctx):
+    """r1 == r2 triangles at a transform length of 512 (256 channels, all
+    lags) run on the symmetric (i, j) tiles: sampled cells against the FP64
+    oracle, every cell against the (k2, k3) tiling of a context with the
+    symmetries switched off (AMB_NO_SYM), counts equal."""
+    import os
+    N, nc, nc10 = 64, 256, 255
+    lo, hi = A.log_rings(2, 20, 5)
+    gv, samp, _ = A.generate(N, nc, nflag=0, seed=12)
+    mask = np.ones(nc, np.uint8)
+    tris = [tuple(int(x) for x in t) for t in A.triangles(lo, hi) if t[0] == t[1]]
+    assert tris
+    os.environ["AMB_NO_SYM"] = "1"
+    try:
+        plain = A.Context(device=0, precision="lagft")
+    finally:
+        del os.environ["AMB_NO_SYM"]
+    ctx.set_precision("lagft")
+    try:
+        rng = np.random.default_rng(3)
+        for c in (ctx, plain):
+            c.set_rings(lo, hi)
+        cube = ctx.load_cube(N, nc, 1.0, 0, gv, samp)
+        ctx.set_channels(cube, None, nc10)
+        cube2 = plain.load_cube(N, nc, 1.0, 0, gv, samp)
+        plain.set_channels(cube2, None, nc10)
+        for t in tris[-3:]:
+            print("case lagft pair tiles triangle %s" % (t,), flush=True)
+            a = ctx.bispec_ex(cube, *t)
+            b = plain.bispec_ex(cube2, *t)
+            assert np.array_equal(a["cnt"], b["cnt"])
+            e = np.linalg.norm(a["bl64"] - b["bl64"]) / np.linalg.norm(b["bl64"])
+            d2 = np.append(rng.integers(-nc10, nc10 + 1, 24), 0)
+            d3 = np.append(rng.integers(-nc10, nc10 + 1, 24), 0)
+            b64, c64, _ = O.bispec_cells(N, nc, 1.0, gv, samp, lo, hi, *t, mask, nc10, d2, d3)
+            got = a["bl64"][d2 + nc10, d3 + nc10]
+            eo = np.linalg.norm(got - b64) / np.linalg.norm(b64)
+            print("  vs (k2,k3) tiles %.3g, vs oracle %.3g" % (e, eo), flush=True)
+            assert np.array_equal(a["cnt"][d2 + nc10, d3 + nc10], c64)
+            assert e <= 2e-6 and eo <= 2e-6, (t, e, eo)
+        ctx.release_cube(cube)
+    finally:
+        plain.close()
+        ctx.set_precision("bf16x3")
