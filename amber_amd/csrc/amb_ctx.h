@@ -264,6 +264,27 @@ size_t lag_spectra_bytes(const Cube &cb, long ng, bool fp64);
 hipError_t lag_launch_spectra(amb_ctx *c, hipStream_t s, const BuildOp &op);
 int lagft_run(amb_ctx *c, const amb_tri_req *req, int n, double t0, double *done_ms, double *const *bl64,
               double *const *norm64);
+/* one triangle of the lag-Fourier form as the serving queue runs it: its
+ * tiling, spectra sets (pinned when resolved) and work items */
+struct LagTri {
+    int cube = 0, r1 = 0, r2 = 0, r3 = 0;
+    long ng = 0, P = 0;
+    int L = 0, nseg = 0, mode = 0, ntile = 0, nyq_sep = 0;
+    void *X[3] = {nullptr, nullptr, nullptr};
+    const int2 *tlist = nullptr;
+    const int *tmap = nullptr;
+    size_t cells = 0;
+};
+int lag_prepare(amb_ctx *c, int cube, int r1, int r2, int r3, LagTri *t);
+size_t lag_part_bytes(const LagTri &t);
+long lag_items(const LagTri &t);
+size_t lag_desc_bytes();
+size_t lag_st_cap(); /* the spectrum buffer of any transform length */
+void lag_fill_range(amb_ctx *c, const LagTri &t, void *part, long i0, long item0, void *h);
+hipError_t lag_launch_tiles(hipStream_t s, const void *d, int n, long total);
+hipError_t lag_launch_finish(amb_ctx *c, hipStream_t s, const LagTri &t, const void *d_whole, void *part, void *St,
+                             float *bl);
+void lag_account(amb_ctx *c, const LagTri &t, int32_t *cnt);
 /* calCR; noalloc: all device scratch from the static array, no allocation */
 int closed_rings(amb_ctx *c, int id, int plane, double *rdata, int cap, int *rcount, bool noalloc);
 

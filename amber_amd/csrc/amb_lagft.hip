@@ -98,6 +98,9 @@ struct LagJob {
     const int2 *tlist;          /* mode 1: (I, J) of each tile */
     long P;
     long item0, nitem0;         /* first tile item, first nyquist item in the launch */
+    long ibase;                 /* the triangle's item that the launch's item0 is (a
+                                   launch may cover a sub-range of a triangle's items:
+                                   each item owns its partial block) */
     int lhp, L, nseg, seg;      /* tile segments and their pixels */
     int mode;                   /* 0: (k2, k3) tiles; 1: r1 == r2, (i, j) tiles, i <= j */
     int ntile;                  /* tiles per segment */
@@ -262,7 +265,7 @@ __global__ __launch_bounds__(256, LagT<R>::WPS) void k_lagft_tiles(const LagJob<
     const long P = jobs[jb].P;
     const int lhp = jobs[jb].lhp, L = jobs[jb].L, seg = jobs[jb].seg, mode = jobs[jb].mode;
     const int nt3 = tiles3<R>(L), ntile = jobs[jb].ntile;
-    const long local = it - jobs[jb].item0;
+    const long local = it - jobs[jb].item0 + jobs[jb].ibase;
     const int sg = (int)(local / ntile), tile = (int)(local - (long)sg * ntile);
     int a, b;
     if (mode == 0) {
@@ -816,6 +819,7 @@ void fill_job(LagJob<R> &g, const LJ &j, char *partbase, long *items, long *nite
     g.nyq_sep = j.nyq_sep;
     g.item0 = *items;
     g.nitem0 = *nitems;
+    g.ibase = 0;
     *items += (long)j.nseg * j.ntile;
     if (j.nyq_sep) *nitems += j.nseg;
 }
@@ -828,6 +832,44 @@ hipError_t launch_products(hipStream_t s, const LagJob<R> *d, int n, long items,
     if (e != hipSuccess || nitems == 0) return e;
     hipLaunchKernelGGL(k_lagft_nyq<R>, dim3((unsigned)((nitems + 7) / 8 * 8)), dim3(256), 0, s, d, n, nitems);
     return hipGetLastError();
+}
+
+/* tiling of a triangle: (k2, k3) tiles, or for r1 == r2 FP32 jobs the
+   symmetric (i, j) tiles where they are fewer (tables built once per L);
+   whether k_lagft_nyq sums the k3 = L/2 column (pair tiles, and FP64 where
+   the column is not a tile column) */
+int choose_tiles(amb_ctx *c, int L, bool fp64, bool r12, int *mode, int *ntile, int *nyq_sep, const int2 **tlist,
+                 const int **tmap)
+{
+    *mode = 0;
+    *ntile = fp64 ? tiles2<double>(L) * tiles3<double>(L) : tiles2<float>(L) * tiles3<float>(L);
+    *tlist = nullptr;
+    *tmap = nullptr;
+    if (!fp64 && r12 && !c->no_sym && L % 128 == 0) {
+        auto it = c->lag_pairs_tab.find(L);
+        if (it == c->lag_pairs_tab.end()) {
+            const std::vector<int2> tl = pair_tiles(L);
+            const int nt = L / 128;
+            std::vector<int> tm((size_t)nt * nt, -1);
+            for (size_t t = 0; t < tl.size(); ++t) tm[(size_t)tl[t].x * nt + tl[t].y] = (int)t;
+            void *d = nullptr;
+            const size_t tb = sizeof(int2) * tl.size(), mb = sizeof(int) * tm.size();
+            const int rc = dev_alloc(c, &d, tb + mb, "lag-Fourier tile tables");
+            if (rc) return rc;
+            HIPCHK(c, hipMemcpy(d, tl.data(), tb, hipMemcpyHostToDevice));
+            HIPCHK(c, hipMemcpy((char *)d + tb, tm.data(), mb, hipMemcpyHostToDevice));
+            it = c->lag_pairs_tab.emplace(L, std::make_tuple(d, (int)tl.size(), tb + mb)).first;
+        }
+        const int n1 = std::get<1>(it->second);
+        if (n1 < *ntile) {
+            *mode = 1;
+            *ntile = n1;
+            *tlist = (const int2 *)std::get<0>(it->second);
+            *tmap = (const int *)((const char *)std::get<0>(it->second) + sizeof(int2) * n1);
+        }
+    }
+    *nyq_sep = (*mode == 1 || (fp64 && (L / 2) % tile_b<double>() == 0)) ? 1 : 0;
+    return AMB_OK;
 }
 
 } // namespace
@@ -1075,33 +1117,10 @@ int amb::lagft_run(amb_ctx *c, const amb_tri_req *req, int n, double t0, double 
                 j.ng = ng;
                 j.P = ng * ng;
                 j.nseg = (int)((j.P + SEG - 1) / SEG);
-                j.mode = 0;
-                j.ntile = nm ? tiles2<double>(L) * tiles3<double>(L) : tiles2<float>(L) * tiles3<float>(L);
-                if (!nm && q.r1 == q.r2 && !c->no_sym && L % 128 == 0) {
-                    /* r1 == r2: the (i, j) tiles, when fewer */
-                    auto it = c->lag_pairs_tab.find(L);
-                    if (it == c->lag_pairs_tab.end()) {
-                        const std::vector<int2> tl = pair_tiles(L);
-                        const int nt = L / 128;
-                        std::vector<int> tm((size_t)nt * nt, -1);
-                        for (size_t t = 0; t < tl.size(); ++t) tm[(size_t)tl[t].x * nt + tl[t].y] = (int)t;
-                        void *d = nullptr;
-                        const size_t tb = sizeof(int2) * tl.size(), mb = sizeof(int) * tm.size();
-                        if ((rc = dev_alloc(c, &d, tb + mb, "lag-Fourier tile tables"))) { unpin_all(c); return rc; }
-                        HIPCHK(c, hipMemcpy(d, tl.data(), tb, hipMemcpyHostToDevice));
-                        HIPCHK(c, hipMemcpy((char *)d + tb, tm.data(), mb, hipMemcpyHostToDevice));
-                        it = c->lag_pairs_tab.emplace(L, std::make_tuple(d, (int)tl.size(), tb + mb)).first;
-                    }
-                    const int n1 = std::get<1>(it->second);
-                    if (n1 < j.ntile) {
-                        j.mode = 1;
-                        j.ntile = n1;
-                        j.tlist = (const int2 *)std::get<0>(it->second);
-                        j.tmap = (const int *)((const char *)std::get<0>(it->second) + sizeof(int2) * n1);
-                    }
+                if ((rc = choose_tiles(c, L, nm, q.r1 == q.r2, &j.mode, &j.ntile, &j.nyq_sep, &j.tlist, &j.tmap))) {
+                    unpin_all(c);
+                    return rc;
                 }
-                /* the k3 = L/2 column in k_lagft_nyq: mode 1, and FP64 where it is not a tile column */
-                j.nyq_sep = (j.mode == 1 || (nm && (L / 2) % tile_b<double>() == 0)) ? 1 : 0;
                 const size_t pb = nm ? job_part_bytes<double>(j, L) : job_part_bytes<float>(j, L);
                 const bool want64 = nm ? (norm64 && norm64[qi]) : (bl64 && bl64[qi]);
                 const size_t ob = al(sizeof(float) * cells) + (want64 ? al(sizeof(double) * cells) : 0);
@@ -1133,4 +1152,134 @@ int amb::lagft_run(amb_ctx *c, const amb_tri_req *req, int n, double t0, double 
     const int rc = run_all();
     hipStreamSynchronize(c->cstream); /* no copy left running, also after an error */
     return rc;
+}
+
+/* ------------------------------------------------ the serving queue's view */
+
+int amb::lag_prepare(amb_ctx *c, int cube, int r1, int r2, int r3, LagTri *t)
+{
+    Cube *cb;
+    int rc = find_cube(c, cube, &cb);
+    if (rc) return rc;
+    if (!cb->chans) return fail(c, AMB_ESTATE, "channels not set for cube %d", cube);
+    const int nr = (int)c->ulow.size();
+    if (nr == 0) return fail(c, AMB_ESTATE, "rings not set");
+    if (r1 < 0 || r1 >= nr || r2 < 0 || r2 >= nr || r3 < 0 || r3 >= nr)
+        return fail(c, AMB_EARG, "triangle (%d,%d,%d) outside [0,%d)", r1, r2, r3, nr);
+    if ((rc = lag_setup(c, *cb))) return rc;
+    const long ng = amb_ng(c->uhigh[r1], cb->Du);
+    if (ng <= 0 || ng > 2730) return fail(c, AMB_EARG, "Ng=%ld out of range", ng);
+    const int L = cb->lag_L;
+    float2 *tw;
+    FftPlan pl;
+    if ((rc = twiddles(c, L, &tw, &pl)) || (rc = twiddles64(c, L, nullptr))) return rc;
+    *t = LagTri{};
+    t->cube = cube; t->r1 = r1; t->r2 = r2; t->r3 = r3;
+    t->ng = ng;
+    t->P = ng * ng;
+    t->L = L;
+    t->nseg = (int)((t->P + SEG - 1) / SEG);
+    t->cells = cb->h_lagcnt.size();
+    if ((rc = choose_tiles(c, L, false, r1 == r2, &t->mode, &t->ntile, &t->nyq_sep, &t->tlist, &t->tmap))) return rc;
+    const PlaneSrc src{&c->planes};
+    const int rr[3] = {r1, r2, r3};
+    for (int k = 0; k < 3; ++k)
+        if ((rc = get_planes(c, src, cube, 9, rr[k], (int)ng, &t->X[k]))) return rc;
+    return AMB_OK;
+}
+
+size_t amb::lag_part_bytes(const LagTri &t)
+{
+    LJ j{};
+    j.nseg = t.nseg;
+    j.ntile = t.ntile;
+    return job_part_bytes<float>(j, t.L);
+}
+
+long amb::lag_items(const LagTri &t) { return (long)t.nseg * t.ntile; }
+
+size_t amb::lag_desc_bytes() { return sizeof(LagJob<float>); }
+
+size_t amb::lag_st_cap() { return sizeof(double2) * (size_t)(MAXL / 2 + 1) * MAXL; }
+
+/* the descriptor of items [i0, i0 + n) of t, numbered from item0 in its launch */
+void amb::lag_fill_range(amb_ctx *c, const LagTri &t, void *part, long i0, long item0, void *h)
+{
+    LJ j{};
+    j.norm = false;
+    j.cb = &c->cubes[t.cube];
+    j.ng = t.ng;
+    j.P = t.P;
+    j.nseg = t.nseg;
+    j.mode = t.mode;
+    j.ntile = t.ntile;
+    j.nyq_sep = t.nyq_sep;
+    j.tlist = t.tlist;
+    j.tmap = t.tmap;
+    for (int k = 0; k < 3; ++k) j.X[k] = t.X[k];
+    j.part = 0;
+    j.nyq = sizeof(float2) * (size_t)j.nseg * j.ntile * tile_a<float>() * tile_b<float>();
+    LagJob<float> g;
+    long items = item0, nitems = 0;
+    fill_job(g, j, (char *)part, &items, &nitems);
+    g.ibase = i0;
+    memcpy(h, &g, sizeof g);
+}
+
+hipError_t amb::lag_launch_tiles(hipStream_t s, const void *d, int n, long total)
+{
+    if (n <= 0 || total <= 0) return hipSuccess;
+    return launch_products<float>(s, (const LagJob<float> *)d, n, total, 0);
+}
+
+/* after the last of t's items: the k3 = L/2 column where it has a kernel of
+   its own (d_whole: t's descriptor with item0 = ibase = 0), the segment sums,
+   the 2-D inverse and the lag array bl (f32, W x W) */
+hipError_t amb::lag_launch_finish(amb_ctx *c, hipStream_t s, const LagTri &t, const void *d_whole, void *part,
+                                  void *St, float *bl)
+{
+    const Cube &cb = c->cubes[t.cube];
+    const int L = t.L;
+    const LagJob<float> *g = (const LagJob<float> *)d_whole;
+    const float2 *pt = (const float2 *)part;
+    const float2 *nyq = pt + (size_t)t.nseg * t.ntile * tile_a<float>() * tile_b<float>();
+    if (t.nyq_sep) {
+        hipLaunchKernelGGL(k_lagft_nyq<float>, dim3((unsigned)((t.nseg + 7) / 8 * 8)), dim3(256), 0, s, g, 1,
+                           (long)t.nseg);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    double2 *st = (double2 *)St;
+    const dim3 gr((unsigned)((L + 31) / 32), (unsigned)((L / 2 + 1 + 31) / 32));
+    if (t.mode == 1)
+        hipLaunchKernelGGL((k_lagft_reduce<float, true>), gr, dim3(256), 0, s, pt, t.nseg, t.ntile, L, t.tmap, st);
+    else
+        hipLaunchKernelGGL((k_lagft_reduce<float, false>), gr, dim3(256), 0, s, pt, t.nseg, t.ntile, L, nullptr, st);
+    const bool folded = t.mode == 0 && (L / 2) % tile_b<float>() == 0;
+    if (t.nyq_sep || folded)
+        hipLaunchKernelGGL(k_lagft_reduce_nyq<float>, dim3((unsigned)((L + 255) / 256)), dim3(256), 0, s, nyq, t.nseg, L,
+                           st);
+    const FftPlan &pl = c->plans[L];
+    const double2 *tw = c->tw64[L];
+    const size_t lds = sizeof(double2) * 2 * (size_t)L;
+    hipLaunchKernelGGL(k_lagft_cols, dim3((unsigned)(L / 2 + 1)), dim3(64), lds, s, st, pl, tw);
+    const int W = 2 * cb.nc10 + 1;
+    const int *ex[3];
+    for (int k = 0; k < 3; ++k) ex[k] = (const int *)((const float2 *)t.X[k] + (size_t)t.P * cb.lag_lhp) + 1;
+    hipLaunchKernelGGL(k_lagft_rows, dim3((unsigned)((W + 1) / 2)), dim3(64), lds, s, st, pl, tw, cb.nc10,
+                       cb.d_lagcnt, ex[0], ex[1], ex[2], 0.0, bl, nullptr, nullptr);
+    return hipGetLastError();
+}
+
+/* counts, evals and work of one finished triangle */
+void amb::lag_account(amb_ctx *c, const LagTri &t, int32_t *cnt)
+{
+    const Cube &cb = c->cubes[t.cube];
+    if (cnt) memcpy(cnt, cb.h_lagcnt.data(), sizeof(int32_t) * cb.h_lagcnt.size());
+    const int L = t.L;
+    c->st.evals += cb.lag_evals;
+    c->st.flop_contract += 2.0 * (double)cb.lag_pairs * (double)t.P;
+    c->st.flop_contract_distinct += 2.0 * (double)cb.lag_pairs * (double)t.P;
+    c->st.flop_lagft += 14.0 * (double)L * (double)(L / 2 + 1) * (double)t.P * (t.mode == 1 ? 0.5 : 1.0);
+    c->st.flop_contract_exec += 14.0 * ((double)t.ntile * tile_a<float>() * tile_b<float>() + L) * (double)t.P;
 }

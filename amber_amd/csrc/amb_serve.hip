@@ -15,6 +15,10 @@
  * request that arrives mid-run is admitted at once (memory permitting) and
  * gets its share of the next iteration.
  *
+ * In the lag-Fourier precision the work items are the (tile, pixel
+ * segment) items of the triple-product kernel; a triangle whose last items
+ * ran gets its segment sums, 2-D inverse and lag array in the iteration.
+ *
  * Memory: planes come from the context's LRU cache and are held by
  * serving pins while their triangles are admitted; partial sums and
  * results come from two pools reserved at start; cubes carried by requests
@@ -43,6 +47,9 @@ constexpr size_t UPLOAD_STAGING = (size_t)256 << 20; /* at most; a sixteenth of 
 /* work of one iteration per resident workgroup, in items x pixels: two
    items of a 512^2 grid (about 40 ms of the split form) */
 constexpr double ITER_PX_PER_BLOCK = 2.0 * 512 * 512;
+/* a lag-Fourier item (one 128 x 128 tile over 2048 pixels, about 1.3 ms of
+   a workgroup) in the same units: about 32 items per workgroup and iteration */
+constexpr double LAG_ITEM_UNITS = 8.0 * 2048;
 
 size_t al(size_t x) { return (x + ALIGN - 1) / ALIGN * ALIGN; }
 
@@ -126,11 +133,14 @@ struct AmbServer {
     struct Job {
         uint64_t tk = 0;
         int idx = 0;
-        TriJob j;
+        TriJob j;                 /* contraction; for lag-Fourier jobs only cube, rings, cells, want_norm */
+        bool lag = false;
+        LagTri lt;                /* lag-Fourier jobs */
         std::vector<void *> pins;
-        size_t c_off = 0, q_off = 0, o_off = 0, c_bytes = 0, o_bytes = 0;
+        size_t c_off = 0, q_off = 0, o_off = 0, c_bytes = 0, c_total = 0, o_bytes = 0;
         OutLayout ol{};
         long items = 0, next = 0, nitems = 0, nnext = 0;
+        double px = 0;            /* work of one item, pixel units */
     };
     struct Iter {
         int slot = 0;
@@ -149,6 +159,7 @@ struct AmbServer {
     bool broken = false;           /* a launch failed: every later triangle fails */
     std::atomic<int> n_jobs{0};    /* admitted triangles not finished */
     int max_active = 256;
+    bool lag = false;              /* the lag-Fourier form (fixed while serving) */
     /* the scheduler's own state */
     std::map<int, Job> jobs;
     int next_job = 1;
@@ -164,6 +175,8 @@ struct AmbServer {
     hipStream_t ustream = nullptr;
     void *ustg = nullptr;
     size_t ustg_bytes = 0;
+    void *lag_st = nullptr;        /* the lag-Fourier 2-D spectrum, shared by finishing triangles in stream order */
+    size_t lag_st_bytes = 0;
 };
 
 using Srv = AmbServer;
@@ -262,7 +275,8 @@ void admit(Srv *s)
             Srv::Job jb;
             std::vector<void *> log;
             c->pin_log = &log;
-            int rc = prepare_tri(c, PlaneSrc{&c->planes}, q.cube_id, q.r1, q.r2, q.r3, q.norm != nullptr, jb.j);
+            int rc = s->lag ? lag_prepare(c, q.cube_id, q.r1, q.r2, q.r3, &jb.lt)
+                            : prepare_tri(c, PlaneSrc{&c->planes}, q.cube_id, q.r1, q.r2, q.r3, q.norm != nullptr, jb.j);
             c->pin_log = nullptr;
             if (rc == AMB_EBUDGET && !s->jobs.empty()) { /* finishing triangles free their planes */
                 pins_to_server(c, log, false);
@@ -287,15 +301,36 @@ void admit(Srv *s)
             }
             pins_to_server(c, log, true);
             jb.pins.swap(log);
-            const TriJob &j = jb.j;
-            const size_t n3 = (size_t)j.nact * j.nact * j.nact;
-            jb.c_bytes = al(n3 * sizeof(float));
-            jb.ol = out_layout(j.cells, j.want_norm);
-            jb.o_bytes = jb.ol.total;
-            bool got_c = s->cpool.take(jb.c_bytes * (j.want_norm ? 2 : 1), &jb.c_off);
+            TriJob &j = jb.j;
+            if (s->lag) {
+                jb.lag = true;
+                j.cube = q.cube_id;
+                j.r1 = q.r1;
+                j.r2 = q.r2;
+                j.r3 = q.r3;
+                j.cells = jb.lt.cells;
+                j.want_norm = q.norm != nullptr; /* the exact FP64 path when the triangle completes */
+                jb.c_bytes = jb.c_total = al(lag_part_bytes(jb.lt));
+                jb.o_bytes = al(jb.lt.cells * sizeof(float));
+            } else {
+                const size_t n3 = (size_t)j.nact * j.nact * j.nact;
+                jb.c_bytes = al(n3 * sizeof(float));
+                jb.c_total = jb.c_bytes * (j.want_norm ? 2 : 1);
+                jb.ol = out_layout(j.cells, j.want_norm);
+                jb.o_bytes = jb.ol.total;
+            }
+            if (jb.c_total > s->cpool.cap) {
+                unspin(c, jb.pins);
+                std::lock_guard<std::mutex> g(s->m);
+                ++t->admitted;
+                tri_done(s, id, fail(c, AMB_EBUDGET, "triangle needs %zu bytes of partial sums, the serving pool holds %zu",
+                                     jb.c_total, s->cpool.cap));
+                continue;
+            }
+            bool got_c = s->cpool.take(jb.c_total, &jb.c_off);
             bool got_o = got_c && s->opool.take(jb.o_bytes, &jb.o_off);
             if (!got_o) {
-                if (got_c) s->cpool.give(jb.c_off, jb.c_bytes * (j.want_norm ? 2 : 1));
+                if (got_c) s->cpool.give(jb.c_off, jb.c_total);
                 unspin(c, jb.pins);
                 if (!s->jobs.empty()) { blocked = true; break; }
                 /* submission checked the pools: this cannot happen with nothing admitted */
@@ -305,12 +340,19 @@ void admit(Srv *s)
                 continue;
             }
             jb.q_off = jb.c_off + jb.c_bytes;
-            jb.items = amb_contract_items(j.nact);
-            jb.nitems = j.want_norm ? jb.items : 0;
+            if (jb.lag) {
+                jb.items = lag_items(jb.lt);
+                jb.nitems = 0;
+                jb.px = LAG_ITEM_UNITS;
+            } else {
+                jb.items = amb_contract_items(j.nact);
+                jb.nitems = j.want_norm ? jb.items : 0;
+                jb.px = (double)j.pitch;
+            }
             jb.tk = id;
             jb.idx = (int)t->admitted;
             /* the margin word collects atomicMax over the triangle's items */
-            if (j.want_norm &&
+            if (!jb.lag && j.want_norm &&
                 amb_launch_fill_u32(c->stream, (unsigned *)(s->obuf + jb.o_off + jb.ol.margin), 1, 0u) != hipSuccess)
                 s->broken = true;
             Cube &cb = c->cubes[q.cube_id];
@@ -382,7 +424,7 @@ bool launch_iter(Srv *s)
     auto grant = [&](Srv::Job &jb, int jid, double budget) -> double {
         auto &tk = take[jid];
         const long d_left = jb.items - jb.next - tk.first, n_left = jb.nitems - jb.nnext - tk.second;
-        const double px = (double)jb.j.pitch;
+        const double px = jb.px;
         long want = (long)std::max(1.0, std::floor(budget / px));
         const long d = std::min(want, d_left);
         tk.first += d;
@@ -403,19 +445,21 @@ bool launch_iter(Srv *s)
         if (spent >= quota) break;
         spent += grant(s->jobs[jid], jid, quota - spent);
     }
-    std::vector<Range> rd, rn;
+    std::vector<Range> rd, rn, rl;
     std::vector<int> finishing;
     for (int jid : s->active) {
         auto it = take.find(jid);
         if (it == take.end()) continue;
         Srv::Job &jb = s->jobs[jid];
-        if (it->second.first > 0) rd.push_back({jid, false, jb.next, it->second.first, jb.j.pitch});
+        if (it->second.first > 0) (jb.lag ? rl : rd).push_back({jid, false, jb.next, it->second.first, jb.j.pitch});
         if (it->second.second > 0) rn.push_back({jid, true, jb.nnext, it->second.second, jb.j.pitch});
         jb.next += it->second.first;
         jb.nnext += it->second.second;
         if (jb.next == jb.items && jb.nnext == jb.nitems) finishing.push_back(jid);
     }
-    if (rd.empty() && rn.empty() && finishing.empty()) return false;
+    if (rd.empty() && rn.empty() && rl.empty() && finishing.empty()) return false;
+    std::vector<int> fin_c, fin_l; /* finishing contraction and lag-Fourier jobs */
+    for (int jid : finishing) (s->jobs[jid].lag ? fin_l : fin_c).push_back(jid);
     /* largest items first: the tail of the launch is made of small ones */
     auto by_pitch = [](const Range &a, const Range &b) { return a.pitch > b.pitch; };
     std::stable_sort(rd.begin(), rd.end(), by_pitch);
@@ -427,14 +471,18 @@ bool launch_iter(Srv *s)
     size_t o = 0;
     const size_t off_da = o; o = al(o + sizeof(ContractArgs) * rd.size());
     const size_t off_na = o; o = al(o + sizeof(ContractArgs) * rn.size());
-    const size_t off_ra = o; o = al(o + sizeof(ReduceArgs) * finishing.size());
+    const size_t off_ra = o; o = al(o + sizeof(ReduceArgs) * fin_c.size());
     const size_t off_b = o; o = al(o + sizeof(long) * (rd.size() + 1));
     const size_t off_nb = o; o = al(o + sizeof(long) * (rn.size() + 1));
-    const size_t off_cb = o; o = al(o + sizeof(long) * (finishing.size() + 1));
+    const size_t off_cb = o; o = al(o + sizeof(long) * (fin_c.size() + 1));
     const size_t off_ctr = o; o = al(o + sizeof(unsigned long long) * 2 * AMB_GROUPED_COUNTERS);
+    const size_t ld = lag_desc_bytes();
+    const size_t off_la = o; o = al(o + ld * rl.size());
+    const size_t off_lw = o; o = al(o + ld * fin_l.size());
     if (o > s->slot_bytes) { /* cannot happen below max_active triangles; split the work instead */
         for (const Range &r : rd) s->jobs[r.jid].next -= r.n;
         for (const Range &r : rn) s->jobs[r.jid].nnext -= r.n;
+        for (const Range &r : rl) s->jobs[r.jid].next -= r.n;
         return false;
     }
     ContractArgs *da = (ContractArgs *)(h + off_da), *na = (ContractArgs *)(h + off_na);
@@ -464,8 +512,8 @@ bool launch_iter(Srv *s)
         nit += rn[k].n;
     }
     nb[rn.size()] = nit;
-    for (size_t k = 0; k < finishing.size(); ++k) {
-        const Srv::Job &jb = s->jobs[finishing[k]];
+    for (size_t k = 0; k < fin_c.size(); ++k) {
+        const Srv::Job &jb = s->jobs[fin_c[k]];
         char *out = s->obuf + jb.o_off;
         ra[k] = jb.j.ra;
         ra[k].C = (const float *)(s->cbuf + jb.c_off);
@@ -478,14 +526,31 @@ bool launch_iter(Srv *s)
         cb[k] = cl;
         cl += (long)jb.j.cells;
     }
-    cb[finishing.size()] = cl;
+    cb[fin_c.size()] = cl;
+    /* lag-Fourier item ranges (each descriptor names the launch's first item
+       of its range and the triangle's item that is), and one whole-triangle
+       descriptor per finishing triangle for its k3 = L/2 column */
+    long lit = 0;
+    for (size_t k = 0; k < rl.size(); ++k) {
+        const Srv::Job &jb = s->jobs[rl[k].jid];
+        lag_fill_range(c, jb.lt, s->cbuf + jb.c_off, rl[k].i0, lit, h + off_la + k * ld);
+        lit += rl[k].n;
+    }
+    for (size_t k = 0; k < fin_l.size(); ++k) {
+        const Srv::Job &jb = s->jobs[fin_l[k]];
+        lag_fill_range(c, jb.lt, s->cbuf + jb.c_off, 0, 0, h + off_lw + k * ld);
+    }
     /* launch configuration of the costliest triangle, a variant all support */
     const int prec = c->prec, nprec = prec == AMB_PREC_FP32 ? AMB_PREC_FP32 : AMB_PREC_BF16X3;
-    const amb_tune_entry t = tune_cfg(c, big->j.ng, big->j.nact);
-    int var = t.variant, nvar = t.variant;
-    for (const Range &r : rd) var = std::min(var, tune_variant(t, prec, s->jobs[r.jid].j.nact, r.pitch));
-    for (const Range &r : rn) nvar = std::min(nvar, tune_variant(t, nprec, s->jobs[r.jid].j.nact, r.pitch));
-    const int blocks = std::max(8, (int)((long)c->grouped_blocks * t.grid_pct / 100) / 8 * 8);
+    amb_tune_entry t{};
+    int var = 0, nvar = 0, blocks = 8;
+    if (big) {
+        t = tune_cfg(c, big->j.ng, big->j.nact);
+        var = nvar = t.variant;
+        for (const Range &r : rd) var = std::min(var, tune_variant(t, prec, s->jobs[r.jid].j.nact, r.pitch));
+        for (const Range &r : rn) nvar = std::min(nvar, tune_variant(t, nprec, s->jobs[r.jid].j.nact, r.pitch));
+        blocks = std::max(8, (int)((long)c->grouped_blocks * t.grid_pct / 100) / 8 * 8);
+    }
     unsigned long long *ctr = (unsigned long long *)(d + off_ctr);
     hipError_t e = hipMemcpyAsync(d, h, o, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess && !rd.empty()) {
@@ -501,10 +566,24 @@ bool launch_iter(Srv *s)
                                         blocks, t.xrun_log2);
         timer_stop(c);
     }
-    if (e == hipSuccess && !finishing.empty()) {
+    if (e == hipSuccess && !fin_c.empty()) {
         timer_start(c, &c->st.ms_reduce);
         e = amb_launch_reduce_grouped(c->stream, (const ReduceArgs *)(d + off_ra), (const long *)(d + off_cb),
-                                      (int)finishing.size(), cl);
+                                      (int)fin_c.size(), cl);
+        timer_stop(c);
+    }
+    if (e == hipSuccess && !rl.empty()) {
+        timer_start(c, &c->st.ms_contract);
+        e = lag_launch_tiles(c->stream, d + off_la, (int)rl.size(), lit);
+        timer_stop(c);
+    }
+    if (e == hipSuccess && !fin_l.empty()) {
+        timer_start(c, &c->st.ms_reduce);
+        for (size_t k = 0; k < fin_l.size() && e == hipSuccess; ++k) {
+            const Srv::Job &jb = s->jobs[fin_l[k]];
+            e = lag_launch_finish(c, c->stream, jb.lt, d + off_lw + k * ld, s->cbuf + jb.c_off, s->lag_st,
+                                  (float *)(s->obuf + jb.o_off));
+        }
         timer_stop(c);
     }
     for (int jid : finishing) {
@@ -561,7 +640,11 @@ void complete_oldest(Srv *s)
                 std::lock_guard<std::mutex> g(s->m);
                 q = &s->tickets[jb.tk].req[jb.idx];
             }
-            if (!rc) {
+            if (!rc && jb.lag) {
+                memcpy(q->bl, s->ohost + jb.o_off, sizeof(float) * j.cells);
+                lag_account(c, jb.lt, q->cnt);
+                if (q->norm) rc = norm_exact(c, j.cube, j.r1, j.r2, j.r3, q->norm, nullptr, nullptr);
+            } else if (!rc) {
                 const char *o = s->ohost + jb.o_off;
                 const int32_t *cnt = (const int32_t *)(o + jb.ol.cnt);
                 memcpy(q->bl, o, sizeof(float) * j.cells);
@@ -581,7 +664,7 @@ void complete_oldest(Srv *s)
             unspin(c, jb.pins);
             auto cit = c->cubes.find(j.cube);
             if (cit != c->cubes.end() && cit->second.serving > 0) cit->second.serving--;
-            s->cpool.give(jb.c_off, jb.c_bytes * (j.want_norm ? 2 : 1));
+            s->cpool.give(jb.c_off, jb.c_total);
             s->opool.give(jb.o_off, jb.o_bytes);
             done.emplace_back(jb.tk, rc);
             s->jobs.erase(jid);
@@ -702,6 +785,7 @@ void server_free(amb_ctx *c, Srv *s)
     dev_free(c, s->cbuf, s->cpool.cap);
     dev_free(c, s->obuf, s->opool.cap);
     dev_free(c, s->ustg, s->ustg_bytes);
+    dev_free(c, s->lag_st, s->lag_st_bytes);
     for (int k = 0; k < NSLOT; ++k) {
         dev_free(c, s->dslot[k], s->slot_bytes);
         if (s->hslot[k]) hipHostFree(s->hslot[k]);
@@ -730,8 +814,9 @@ int check_request(amb_ctx *c, Srv *s, const amb_tri_req *req, int n, const amb_c
             W = 2 * it->second.nc10 + 1;
         }
         const size_t n3 = (size_t)nact * nact * nact;
-        const size_t cb = al(n3 * sizeof(float)) * (req[i].norm ? 2 : 1);
-        const size_t ob = out_layout((size_t)W * W, req[i].norm != nullptr).total;
+        /* lag-Fourier partial sums depend on the grid and are checked at admission */
+        const size_t cb = s->lag ? 0 : al(n3 * sizeof(float)) * (req[i].norm ? 2 : 1);
+        const size_t ob = s->lag ? al((size_t)W * W * sizeof(float)) : out_layout((size_t)W * W, req[i].norm != nullptr).total;
         if (cb > s->cpool.cap || ob > s->opool.cap)
             return fail(c, AMB_EBUDGET, "triangle %d needs %zu + %zu bytes, the serving pools hold %zu + %zu", i, cb,
                         ob, s->cpool.cap, s->opool.cap);
@@ -797,15 +882,15 @@ int amb_serve_start(amb_ctx *c, int max_active)
     Lock lk(c->mu);
     hipSetDevice(c->device);
     if (c->server) return fail(c, AMB_ESTATE, "serving already started");
-    if (c->prec == AMB_PREC_LAGFT)
-        return fail(c, AMB_ESTATE, "the serving queue batches contraction work items; the lag-Fourier form has none");
     Srv *s = new (std::nothrow) Srv();
     if (!s) return AMB_ENOMEM;
     s->c = c;
     s->max_active = std::min(max_active, 4096);
+    s->lag = c->prec == AMB_PREC_LAGFT;
     const size_t ccap = al(std::min<size_t>((size_t)16 << 30, c->budget / 4));
     const size_t ocap = al(std::min<size_t>((size_t)2 << 30, c->budget / 16));
-    s->slot_bytes = al((sizeof(ContractArgs) * 2 + sizeof(ReduceArgs) + 3 * sizeof(long)) * (size_t)(s->max_active + 1) +
+    s->slot_bytes = al((sizeof(ContractArgs) * 2 + sizeof(ReduceArgs) + 3 * sizeof(long) + 2 * lag_desc_bytes()) *
+                           (size_t)(s->max_active + 1) +
                        16 * ALIGN);
     void *p = nullptr;
     int rc = dev_alloc(c, &p, ccap, "serving partial-sum pool");
@@ -813,6 +898,10 @@ int amb_serve_start(amb_ctx *c, int max_active)
     if (!rc && !(rc = dev_alloc(c, &p, ocap, "serving result pool"))) { s->obuf = (char *)p; s->opool.init(ocap); }
     const size_t ucap = al(std::min<size_t>(UPLOAD_STAGING, c->budget / 16));
     if (!rc && !(rc = dev_alloc(c, &p, ucap, "upload staging"))) { s->ustg = p; s->ustg_bytes = ucap; }
+    if (!rc && s->lag && !(rc = dev_alloc(c, &p, lag_st_cap(), "lag-Fourier spectrum"))) {
+        s->lag_st = p;
+        s->lag_st_bytes = lag_st_cap();
+    }
     for (int k = 0; k < NSLOT && !rc; ++k) {
         if ((rc = dev_alloc(c, &p, s->slot_bytes, "serving descriptors"))) break;
         s->dslot[k] = (char *)p;

@@ -39,12 +39,13 @@ def _run(tmp, prog, *args):
     return r.stdout
 
 
-def test_amber_v1_and_v2_drivers(tmp_path):
+@pytest.mark.parametrize("prec", ["bf16x3", "lagft"])
+def test_amber_v1_and_v2_drivers(tmp_path, prec):
     N, nc, nflag, nc10 = 64, 10, 2, 6
     lo, hi = A.log_rings(2, 20, 5)
     gv, samp, uf = _layout(tmp_path, N, nc, nflag, 31, lo, hi)
-    print("case drivers N=%d nc=%d" % (N, nc), flush=True)
-    _run(tmp_path, "amber", "--nc10", str(nc10), "--uf-nc", str(len(uf)))
+    print("case drivers N=%d nc=%d precision %s" % (N, nc, prec), flush=True)
+    _run(tmp_path, "amber", "--nc10", str(nc10), "--uf-nc", str(len(uf)), "--precision", prec)
     # closed_rings / rcount exactly as calCR would write them (exact integers * N^2)
     n, rd = A.read_rings(str(tmp_path / "results/v1"))
     ref = O.calcr(N, nc, samp, lo, hi, plane=5)
@@ -70,7 +71,7 @@ def test_amber_v1_and_v2_drivers(tmp_path):
     # amber_v2 reads results/v2/{rcount,closed_rings}, which it never writes
     for fn in ("rcount", "closed_rings"):
         shutil.copy(tmp_path / "results/v1" / fn, tmp_path / "results/v2" / fn)
-    _run(tmp_path, "amber_v2", "--nc10", str(nc10), "--uf-nc", str(len(uf)))
+    _run(tmp_path, "amber_v2", "--nc10", str(nc10), "--uf-nc", str(len(uf)), "--precision", prec)
     for (r1, r2, r3, num) in rd[::4]:
         r1, r2, r3 = int(r1), int(r2), int(r3)
         f = tmp_path / "results/v2" / ("bsout_final2_ra%d_rb%d_rc%d_R1" % (r1, r2, r3))

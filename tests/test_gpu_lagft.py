@@ -6,7 +6,8 @@ the channel spectra, so the tests pin what the direct sum fixes: every cell
 against the oracle on small cubes (flagged channels, partial lag windows,
 odd grids, r1 == r2 and r2 == r3 triangles), counts bit for bit, cells
 without triplets exactly zero, batches equal to single triangles bit for
-bit, the v2 normalisation exact, and the forms it does not serve refused."""
+bit, the v2 normalisation exact, the serving queue equal to the batch
+path, and graph capture (which it does not support) refused."""
 import numpy as np
 import pytest
 
@@ -82,7 +83,7 @@ def test_batch_equals_single_and_v2_norm_exact(ctx):
         ctx.set_precision("bf16x3")
 
 
-def test_refused_in_graphs_and_serving(ctx):
+def test_refused_in_graphs(ctx):
     N, nc = 32, 8
     lo, hi = A.log_rings(2, 10, 4)
     gv, samp, _ = A.generate(N, nc, seed=2)
@@ -94,10 +95,55 @@ def test_refused_in_graphs_and_serving(ctx):
         t = tuple(int(x) for x in A.triangles(lo, hi)[0])
         with pytest.raises(A.AmberError):
             ctx.graph([(cube,) + t])
-        with pytest.raises(A.AmberError):
-            ctx.serve_start(4)
         ctx.bispec_ex(cube, *t)  # the context stays usable
         ctx.release_cube(cube)
+    finally:
+        ctx.set_precision("bf16x3")
+
+
+def test_serving_equals_batch(ctx):
+    """The serving queue in the lag-Fourier form: 24 one-triangle requests on
+    a 512^2 x 256 cube (transform length 512, r1 == r2 triangles on pair
+    tiles) and a small flagged cube, submitted at once so that fair shares
+    split the larger triangles' tile items over several iterations; lag
+    arrays and counts equal the batch path bit for bit, and the v2
+    normalisation (the exact FP64 path) equals the batch path's."""
+    lo, hi = A.log_rings(4, 200, 6)
+    tris = [tuple(int(x) for x in t) for t in A.triangles(lo, hi)]
+    gv, samp, _ = A.generate(512, 256, nflag=0, seed=31)
+    gs, ss, us = A.generate(64, 24, nflag=3, seed=32)
+    ctx.set_precision("lagft")
+    try:
+        ctx.set_rings(lo, hi)
+        big = ctx.load_cube(512, 256, 1.0, 0, gv, samp)
+        ctx.set_channels(big, None, 255)
+        small = ctx.load_cube(64, 24, 1.0, 1, gs, ss)
+        ctx.set_channels(small, us, 23)
+        rng = np.random.default_rng(5)
+        pick = [tris[i] for i in rng.choice(len(tris), size=16, replace=False)]
+        pick += [t for t in tris if t[0] == t[1]][-4:]
+        reqs = [(big,) + t for t in pick]
+        # the small cube's N/2 = 32 bounds its rings
+        reqs += [(small,) + t for t in tris if max(hi[list(t)]) <= 32][:4]
+        reqs = [reqs[i] for i in rng.permutation(len(reqs))]
+        print("case lagft serving: %d requests" % len(reqs), flush=True)
+        ctx.serve_start(max_active=64)
+        try:
+            tickets = [ctx.submit([r], want_norm=(k % 5 == 2)) for k, r in enumerate(reqs)]
+            got = [ctx.wait(t)[0][0] for t in tickets]
+            st = ctx.stats()
+        finally:
+            ctx.serve_stop()
+        print("iterations %d" % st["n_serve_iters"], flush=True)
+        assert st["n_serve_iters"] >= 2
+        for k, r in enumerate(reqs):
+            want = ctx.bispec_multi([r], want_norm=(k % 5 == 2))[0]
+            assert np.array_equal(got[k][0], want[0]), (k, r)
+            assert np.array_equal(got[k][1], want[1]), (k, r)
+            if k % 5 == 2:
+                assert np.array_equal(got[k][2], want[2]), (k, r)
+        ctx.release_cube(big)
+        ctx.release_cube(small)
     finally:
         ctx.set_precision("bf16x3")
 
