@@ -48,9 +48,9 @@ extern "C" {
                              the DFT domain of the channel axis (FP32 products of
                              channel spectra on the vector ALUs, FP64 segment sums and
                              2-D inverse); L^2/2 complex products per pixel instead of
-                             nc^3 real ones.  Batches run as waves of their own; not
-                             available in captured graphs or the serving queue.  The
-                             v2 normalisation of this form is the exact FP64 one. */
+                             nc^3 real ones.  Batches run as waves of their own; the
+                             serving queue runs it; not available in captured graphs.
+                             The v2 normalisation of this form is the exact FP64 one. */
 
 const char *amb_strerror(int code);
 const char *amb_version(void);
@@ -341,7 +341,9 @@ void amb_graph_destroy(amb_ctx *ctx, amb_graph *g);
 
 /* Serving queue (no counterpart in the reference, which runs one cube per
  * process, amber.c:32-38): continuous batching at the granularity of
- * contraction work items.  A scheduler thread admits requests' triangles
+ * work items (the contraction's, or in AMB_PREC_LAGFT the tile kernel's
+ * (tile, pixel segment) items, a triangle's v2 normalisation then coming
+ * from the exact FP64 path when it completes).  A scheduler thread admits requests' triangles
  * as device memory allows and launches bounded contraction iterations,
  * each taking a fair share of work items from every admitted request, so
  * a request that arrives while others run starts within an iteration or
