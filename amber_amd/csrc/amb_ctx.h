@@ -47,6 +47,7 @@ struct Cube {
        counts on the device and the host, their cells with cnt > 0 and sum */
     int lag_L = 0, lag_lhp = 0;
     int *d_lagcnt = nullptr;
+    int *d_lagpos = nullptr; /* digit-reversed sequence position of each active plane (in-place channel DFT) */
     std::vector<int32_t> h_lagcnt;
     int64_t lag_evals = 0, lag_pairs = 0;
     int nact() const { return (int)act.size(); }

@@ -103,13 +103,17 @@ __global__ __launch_bounds__(FFT_THREADS) void k_ring_cols(int kind, const float
     }
 }
 
-/* Row pass: one team per (plane, pair of output rows xa, xb).  The
- * Hermitian extension of the two rows' half spectra as one complex sequence
+/* Row pass: one team per (plane, pair of output rows xa, xb), a workgroup
+ * holding the 2 nteam consecutive rows x0 .. x0 + 2 nteam - 1.  The
+ * Hermitian extension of two rows' half spectra as one complex sequence
  * Z(k) = A(k) + i B(k), Z(Ng - k) = conj A(k) + i conj B(k); its inverse DFT
  * holds row xa in the real part and row xb in the imaginary part (the j = 0
  * column enters through its real part, the c2r of the Hermitian-symmetrised
- * column).  mx (may be null): max |value| over the set, as the bits of a
- * non-negative float (one atomic per workgroup). */
+ * column).  The workgroup reads G[k][x0 ..] for 32 columns k per pass, each
+ * lane one (k, x) value (2 nteam x 8 contiguous bytes per k), and the lane
+ * of row xa takes row xb's value from its neighbour lane.  mx (may be
+ * null): max |value| over the set, as the bits of a non-negative float
+ * (one atomic per workgroup). */
 template <typename R, bool TWL>
 __global__ __launch_bounds__(FFT_THREADS) void k_ring_rows(const Cx<R> *__restrict__ G, int jcnt, int npl, FftPlan pl,
                                                            const Cx<R> *__restrict__ twg, int nbx,
@@ -126,24 +130,32 @@ __global__ __launch_bounds__(FFT_THREADS) void k_ring_rows(const Cx<R> *__restri
         for (int i = threadIdx.x; i < n; i += blockDim.x) sm[i] = twg[i];
     const long item = xcd_item(blockIdx.x, gridDim.x);
     const int pli = (int)(item / nbx), bxi = (int)(item - (long)pli * nbx);
-    const int xa = 2 * (bxi * nteam + team), xb = xa + 1;
+    const int W = 2 * nteam, x0 = W * bxi;
+    const int xa = x0 + 2 * team, xb = xa + 1;
     const bool live = pli < npl && xb < n;
-    const C *Gp = G + (size_t)(live ? pli : 0) * jcnt * n;
-    for (int k = t; k < n; k += 64) {
-        C z = cx((R)0, (R)0);
-        if (live) {
-            if (k == 0) {
-                z = cx(Gp[xa].x, Gp[xb].x);
-            } else if (k < jcnt) {
-                const C A = Gp[(size_t)k * n + xa], B = Gp[(size_t)k * n + xb];
-                z = cx(A.x - B.y, A.y + B.x);
-            } else if (n - k < jcnt) {
-                const int kk = n - k;
-                const C A = Gp[(size_t)kk * n + xa], B = Gp[(size_t)kk * n + xb];
-                z = cx(A.x + B.y, B.x - A.y);
+    /* the columns no ring cell reaches: jcnt <= n / 3 */
+    for (int k = jcnt + t; k <= n - jcnt; k += 64) bx[k] = cx((R)0, (R)0);
+    if (pli < npl) {
+        const C *Gp = G + (size_t)pli * jcnt * n;
+        const int xx = threadIdx.x % W, kq = threadIdx.x / W; /* 32 columns per pass */
+        for (int k0 = 0; k0 < jcnt; k0 += 32) {
+            const int k = k0 + kq;
+            C v = cx((R)0, (R)0);
+            if (k < jcnt && x0 + xx < n) v = Gp[(size_t)k * n + x0 + xx];
+            /* W is even: lanes 2m, 2m + 1 hold rows xa, xb of one pair at one k */
+            C o;
+            o.x = __shfl_xor(v.x, 1);
+            o.y = __shfl_xor(v.y, 1);
+            if (k < jcnt && !(xx & 1)) {
+                C *bt = sm + (TWL ? n : 0) + (size_t)(xx >> 1) * 2 * n;
+                if (k == 0) {
+                    bt[0] = cx(v.x, o.x);
+                } else {
+                    bt[k] = cx(v.x - o.y, v.y + o.x);
+                    bt[n - k] = cx(v.x + o.y, o.x - v.y);
+                }
             }
         }
-        bx[k] = z;
     }
     __syncthreads();
     const C *res = fftc::transform<+1, MaxP<R>::v>(bx, by, tw, pl, t, 64);

@@ -241,4 +241,66 @@ __device__ C *transform(C *x, C *y, const C *tw, const FftPlan &pl, int t, int T
     return x;
 }
 
+/* ------------------------------------------------------------- in place */
+/* Position of input element c in the digit-reversed order that the
+ * in-place transform below takes: with the plan's radices R_0 .. R_{S-1},
+ * c = d_{S-1} + R_{S-1} (d_{S-2} + R_{S-2} (... + R_1 d_0)) is stored at
+ * p = d_0 + R_0 (d_1 + R_1 (... + R_{S-2} d_{S-1})). */
+inline int digit_reversed(int c, const FftPlan &pl)
+{
+    int d[AMB_MAX_RADIX_STAGES];
+    for (int s = pl.nst - 1; s >= 0; --s) {
+        d[s] = c % pl.rad[s];
+        c /= pl.rad[s];
+    }
+    int p = 0;
+    for (int s = pl.nst - 1; s >= 0; --s) p = p * pl.rad[s] + d[s];
+    return p;
+}
+
+/* One in-place decimation-in-time stage of radix R: sub-transforms of
+ * length Ns are combined R at a time; butterfly (g, k) reads positions
+ * g Ns R + k + r Ns, twiddles input r by w^{r k n / (Ns R)} and writes its
+ * R outputs back to the same positions (they are in registers first). */
+template <int R, int DIR, typename C>
+FFTC_HD void stage_ip(C *__restrict__ x, const C *__restrict__ tw, int n, int Ns, int t, int T)
+{
+    const int nR = n / R, tstep = n / (Ns * R);
+    int g = t / Ns, k = t - g * Ns;
+    const int dg = T / Ns, dk = T - dg * Ns;
+    for (int j = t; j < nR; j += T) {
+        C *b = x + g * Ns * R + k;
+        C v[R];
+        v[0] = b[0];
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[r] = twmul<DIR>(b[r * Ns], tw[r * k * tstep]);
+        Bfly<R, DIR, C>::run(v, tw, n / R, b, Ns);
+        k += dk;
+        g += dg;
+        if (k >= Ns) { k -= Ns; ++g; }
+    }
+}
+
+/* In-place transform of a team's buffer x (input in digit-reversed order,
+ * output in natural order) for plans of radices 2, 3, 4 and 5 only (the
+ * {2,3,5}-smooth lengths of the channel axis); one workgroup barrier per
+ * stage, as transform(). */
+template <int DIR, typename C>
+__device__ void transform_ip(C *x, const C *tw, const FftPlan &pl, int t, int T)
+{
+    const int n = pl.n;
+    int Ns = 1;
+    for (int s = 0; s < pl.nst; ++s) {
+        const int R = pl.rad[s];
+        switch (R) {
+        case 2: stage_ip<2, DIR>(x, tw, n, Ns, t, T); break;
+        case 3: stage_ip<3, DIR>(x, tw, n, Ns, t, T); break;
+        case 4: stage_ip<4, DIR>(x, tw, n, Ns, t, T); break;
+        default: stage_ip<5, DIR>(x, tw, n, Ns, t, T); break;
+        }
+        __syncthreads();
+        Ns *= R;
+    }
+}
+
 } // namespace fftc

@@ -163,10 +163,13 @@ __device__ __forceinline__ void load_unit(const char *p, double2 *v)
 }
 
 /* ---------------------------------------------------------------- spectra */
-/* grid: multiple of 8 blocks of nteam x 64 threads, each team ppt pixel
-   pairs; dynamic LDS: [tw L if TWL] + nteam * 2 * L complex.  SIGNED: plane
-   i of a normalised data set carries (-1)^i, removed here.  Channel
-   ch = act0 + p sits at position p of the sequence. */
+/* grid: multiple of 8 blocks of nteam x 64 threads, each team ppt
+   consecutive pixel pairs; dynamic LDS: [tw L if TWL] + nteam * 2 * L
+   complex.  SIGNED: plane i of a normalised data set carries (-1)^i,
+   removed here.  Channel ch = act0 + p sits at position p of the sequence.
+   The next pair's plane values are loaded into registers before the
+   current pair's transform, so their latency hides behind it; a
+   workgroup's pairs cover whole 128-byte lines of every plane. */
 template <typename R, bool SIGNED, bool TWL>
 __global__ __launch_bounds__(256) void k_chan_spectra(const R *__restrict__ D, long pitch, const int *__restrict__ inv,
                                                       int nc, int act0, long P, FftPlan pl,
@@ -175,6 +178,7 @@ __global__ __launch_bounds__(256) void k_chan_spectra(const R *__restrict__ D, l
                                                       const unsigned *__restrict__ slot_in, unsigned *__restrict__ slot_out)
 {
     typedef typename LagT<R>::C C;
+    constexpr int NPF = MAXL / 64; /* sequence positions per thread */
     extern __shared__ __align__(16) char lraw[];
     C *lsm = (C *)lraw;
     const int L = pl.n, h = L / 2;
@@ -186,17 +190,21 @@ __global__ __launch_bounds__(256) void k_chan_spectra(const R *__restrict__ D, l
     if (slot_out && blockIdx.x == 0 && threadIdx.x < 2) slot_out[threadIdx.x] = slot_in[threadIdx.x];
     const long npair = (P + 1) / 2;
     const long pair0 = (xcd_item(blockIdx.x, gridDim.x) * nteam + team) * ppt;
-    __syncthreads();
-    for (int q = 0; q < ppt; ++q) {
-        const long pr = pair0 + q;
+    const int pmax = min(L, nc - act0); /* positions that can hold a channel */
+    /* FP64 (the normalisation's spectra) loads straight into LDS: its
+       prefetch registers would double the kernel's VGPRs */
+    constexpr bool PF = sizeof(R) == 4;
+    C pf[PF ? NPF : 1];
+    auto fetch = [&](long pr) {
         const bool live = pr < npair;
         const long x0 = 2 * pr;
-        for (int p = t; p < L; p += 64) {
+#pragma unroll
+        for (int u = 0; u < (PF ? NPF : (L + 63) / 64); ++u) {
+            const int p = t + 64 * u;
             C v;
             v.x = v.y = (R)0;
-            const int ch = act0 + p;
-            if (live && ch < nc) {
-                const int i = inv[ch];
+            if (live && p < pmax) {
+                const int i = inv[act0 + p];
                 if (i >= 0) {
                     /* x0 even and the pitch a multiple of 64: an aligned pair;
                        for odd P the second value is the zeroed pitch tail */
@@ -206,7 +214,25 @@ __global__ __launch_bounds__(256) void k_chan_spectra(const R *__restrict__ D, l
                     if (SIGNED && (i & 1)) { v.x = -v.x; v.y = -v.y; }
                 }
             }
-            bx[p] = v;
+            if constexpr (PF) pf[u] = v;
+            else if (p < L) bx[p] = v;
+        }
+    };
+    if (PF) fetch(pair0);
+    __syncthreads();
+    for (int q = 0; q < ppt; ++q) {
+        const long pr = pair0 + q;
+        const bool live = pr < npair;
+        const long x0 = 2 * pr;
+        if constexpr (PF) {
+#pragma unroll
+            for (int u = 0; u < NPF; ++u) {
+                const int p = t + 64 * u;
+                if (p < L) bx[p] = pf[u];
+            }
+            if (q + 1 < ppt) fetch(pr + 1);
+        } else {
+            fetch(pr);
         }
         __syncthreads();
         const C *Z = fftc::transform<-1, 5>(bx, by, tw, pl, t, 64);
@@ -226,6 +252,76 @@ __global__ __launch_bounds__(256) void k_chan_spectra(const R *__restrict__ D, l
             }
         }
         __syncthreads(); /* the next pair overwrites both buffers */
+    }
+}
+
+/* Channel spectra of a block of 32 pixels (16 pairs) per workgroup, FP32
+ * sets of at most SPB_PLANES planes.  The block's values are loaded once
+ * with whole-line reads (a wave reads 128 contiguous bytes of 4 planes per
+ * instruction) into registers: thread tid holds pixel pair tid % 16 of
+ * planes tid / 16 + 16 m.  Four rounds then each transform 4 pairs, one
+ * per 64-thread team: the holders of a round's pairs write their values at
+ * the digit-reversed positions pos[i] of a zeroed team buffer, and the
+ * in-place transform leaves the spectrum in natural order.  Dynamic LDS:
+ * tw L + 4 L complex + nact positions. */
+constexpr int SPB_PLANES = 768;
+__global__ __launch_bounds__(256, 2) void k_chan_spectra_blk(const float *__restrict__ D, long pitch,
+                                                             const int *__restrict__ posg, int nact, long P,
+                                                             FftPlan pl, const float2 *__restrict__ twg, int lhp,
+                                                             float2 *__restrict__ X,
+                                                             const unsigned *__restrict__ slot_in,
+                                                             unsigned *__restrict__ slot_out)
+{
+    constexpr int NQ = SPB_PLANES / 16;
+    extern __shared__ __align__(16) char lraw[];
+    const int L = pl.n, h = L / 2;
+    float2 *tw = (float2 *)lraw, *buf = tw + L;
+    int *pos = (int *)(buf + 4 * L);
+    const int tid = threadIdx.x, team = tid >> 6, t = tid & 63;
+    for (int i = tid; i < L; i += 256) tw[i] = twg[i];
+    for (int i = tid; i < nact; i += 256) pos[i] = posg[i];
+    if (slot_out && blockIdx.x == 0 && tid < 2) slot_out[tid] = slot_in[tid];
+    const long x0 = 32 * xcd_item(blockIdx.x, gridDim.x);
+    if (x0 >= P) return; /* whole workgroups: no barrier is skipped by part of one */
+    const int j = tid & 15, i0 = tid >> 4;
+    /* x0 + 2 j + 1 < x0 + 32 <= pitch (the pitch is a multiple of 64, its tail zero) */
+    float2 v[NQ];
+#pragma unroll
+    for (int m = 0; m < NQ; ++m) {
+        const int i = i0 + 16 * m;
+        float2 w = make_float2(0.f, 0.f);
+        if (i < nact) {
+            w = *(const float2 *)(D + (size_t)i * pitch + x0 + 2 * j);
+            if (i & 1) { w.x = -w.x; w.y = -w.y; } /* the set's (-1)^i */
+        }
+        v[m] = w;
+    }
+    float2 *bx = buf + (size_t)team * L;
+    for (int r = 0; r < 4; ++r) {
+        for (int p = t; p < L; p += 64) bx[p] = make_float2(0.f, 0.f);
+        __syncthreads();
+        if ((j >> 2) == r) {
+            float2 *bt = buf + (size_t)(j & 3) * L;
+#pragma unroll
+            for (int m = 0; m < NQ; ++m) {
+                const int i = i0 + 16 * m;
+                if (i < nact) bt[pos[i]] = v[m];
+            }
+        }
+        __syncthreads();
+        fftc::transform_ip<-1>(bx, tw, pl, t, 64);
+        const long xa = x0 + 2 * (4 * r + team);
+        if (xa < P) {
+            float2 *oa = X + (size_t)xa * lhp, *ob = oa + lhp;
+            const bool two = xa + 1 < P;
+            for (int k = t; k <= h; k += 64) {
+                const float2 a = bx[k], b = bx[k == 0 ? 0 : L - k];
+                /* A = (Z(k) + conj Z(L-k)) / 2, B = (Z(k) - conj Z(L-k)) / 2i */
+                oa[k] = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y - b.y));
+                if (two) ob[k] = make_float2(0.5f * (a.y + b.y), -0.5f * (a.x - b.x));
+            }
+        }
+        __syncthreads(); /* the next round zeroes the buffers */
     }
 }
 
@@ -693,6 +789,23 @@ int amb::lag_setup(amb_ctx *c, Cube &cb)
         ev += h[i] > 0;
         pr += h[i];
     }
+    /* digit-reversed sequence positions of the active planes */
+    FftPlan pl;
+    fftc::make_plan(L, &pl);
+    std::vector<int> hp(na);
+    for (int i = 0; i < na; ++i) hp[i] = fftc::digit_reversed(cb.act[i] - cb.act.front(), pl);
+    void *dp = nullptr;
+    if ((rc = dev_alloc(c, &dp, sizeof(int) * na, "lag positions"))) {
+        dev_free(c, d, sizeof(int) * cells);
+        return rc;
+    }
+    e = hipMemcpy(dp, hp.data(), sizeof(int) * na, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        dev_free(c, d, sizeof(int) * cells);
+        dev_free(c, dp, sizeof(int) * na);
+        return fail(c, AMB_EHIP, "lag positions: %s", hipGetErrorString(e));
+    }
+    cb.d_lagpos = (int *)dp;
     cb.lag_L = L;
     cb.lag_lhp = (L / 2 + 1 + 3) / 4 * 4;
     cb.d_lagcnt = (int *)d;
@@ -705,7 +818,9 @@ int amb::lag_setup(amb_ctx *c, Cube &cb)
 void amb::lag_release(amb_ctx *c, Cube &cb)
 {
     if (cb.d_lagcnt) dev_free(c, cb.d_lagcnt, sizeof(int) * cb.h_lagcnt.size());
+    if (cb.d_lagpos) dev_free(c, cb.d_lagpos, sizeof(int) * cb.act.size());
     cb.d_lagcnt = nullptr;
+    cb.d_lagpos = nullptr;
     cb.h_lagcnt.clear();
     cb.lag_L = cb.lag_lhp = 0;
 }
@@ -722,19 +837,33 @@ hipError_t amb::lag_launch_spectra(amb_ctx *c, hipStream_t s, const BuildOp &op)
 {
     const Cube &cb = c->cubes[op.cube];
     const long ng = op.g.ng, P = ng * ng, pitch = plane_pitch(ng);
-    const int L = cb.lag_L, ppt = 4;
+    const int L = cb.lag_L;
     const long npair = (P + 1) / 2;
+    /* pairs per team: up to 64 pixels of every plane per workgroup (FP32),
+       fewer while that leaves under 2048 workgroups; FP64: 4 */
+    const int nt9 = spectra_teams(L);
+    const int ppt = op.kind == 9 ? (int)std::max(1L, std::min<long>(32 / nt9, npair / (2048L * nt9))) : 4;
     const FftPlan &pl = c->plans[L];
     if (op.kind == 9) {
         auto itw = c->tw.find(L);
         if (itw == c->tw.end()) return hipErrorInvalidValue;
+        const size_t setw = (size_t)pitch * cb.nact();
+        const unsigned *slot_in = (const unsigned *)((const float *)op.src + setw);
+        unsigned *slot_out = (unsigned *)((float2 *)op.dst + (size_t)P * cb.lag_lhp);
+        bool smooth5 = true;
+        for (int s2 = 0; s2 < pl.nst; ++s2) smooth5 = smooth5 && pl.rad[s2] <= 5;
+        const size_t lds_blk = sizeof(float2) * 5 * (size_t)L + sizeof(int) * (size_t)cb.nact();
+        if (cb.nact() <= SPB_PLANES && smooth5 && lds_blk <= 65536) {
+            const long nblk = ((P + 31) / 32 + 7) / 8 * 8;
+            hipLaunchKernelGGL(k_chan_spectra_blk, dim3((unsigned)nblk), dim3(256), lds_blk, s, (const float *)op.src,
+                               pitch, (const int *)cb.d_lagpos, cb.nact(), P, pl, itw->second, cb.lag_lhp,
+                               (float2 *)op.dst, slot_in, slot_out);
+            return hipGetLastError();
+        }
         const int nteam = spectra_teams(L);
         const long per_blk = (long)nteam * ppt;
         const long nblk = ((npair + per_blk - 1) / per_blk + 7) / 8 * 8;
         const size_t lds = sizeof(float2) * (size_t)L * (1 + 2 * nteam);
-        const size_t setw = (size_t)pitch * cb.nact();
-        const unsigned *slot_in = (const unsigned *)((const float *)op.src + setw);
-        unsigned *slot_out = (unsigned *)((float2 *)op.dst + (size_t)P * cb.lag_lhp);
         hipLaunchKernelGGL((k_chan_spectra<float, true, true>), dim3((unsigned)nblk), dim3(64 * nteam), lds, s,
                            (const float *)op.src, pitch, cb.d_inv, (int)cb.nc, cb.act.front(), P, pl, itw->second,
                            cb.lag_lhp, ppt, (float2 *)op.dst, slot_in, slot_out);

@@ -50,3 +50,45 @@ extern "C" int fftc_host(int n, int dir, int dbl, int team, const double *in, do
     if (dbl) return dir > 0 ? run<double, 1>(n, team, in, out) : run<double, -1>(n, team, in, out);
     return dir > 0 ? run<float, 1>(n, team, in, out) : run<float, -1>(n, team, in, out);
 }
+
+/* the in-place form: input placed at its digit-reversed positions */
+template <typename T, int DIR>
+static int run_ip(int n, int TT, const double *in, double *out)
+{
+    typedef typename fftc::CT<T>::c C;
+    FftPlan pl;
+    fftc::make_plan(n, &pl);
+    for (int s = 0; s < pl.nst; ++s)
+        if (pl.rad[s] > 5) return -2;
+    std::vector<C> x(n), tw(n);
+    for (int m = 0; m < n; ++m) {
+        const double a = 2.0 * M_PI * (double)m / (double)n;
+        tw[m] = fftc::cx((T)cos(a), (T)sin(a));
+        x[fftc::digit_reversed(m, pl)] = fftc::cx((T)in[2 * m], (T)in[2 * m + 1]);
+    }
+    int Ns = 1;
+    for (int s = 0; s < pl.nst; ++s) {
+        const int R = pl.rad[s];
+        for (int t = 0; t < TT; ++t) {
+            switch (R) {
+            case 2: fftc::stage_ip<2, DIR>(x.data(), tw.data(), n, Ns, t, TT); break;
+            case 3: fftc::stage_ip<3, DIR>(x.data(), tw.data(), n, Ns, t, TT); break;
+            case 4: fftc::stage_ip<4, DIR>(x.data(), tw.data(), n, Ns, t, TT); break;
+            default: fftc::stage_ip<5, DIR>(x.data(), tw.data(), n, Ns, t, TT); break;
+            }
+        }
+        Ns *= R;
+    }
+    for (int m = 0; m < n; ++m) {
+        out[2 * m] = (double)x[m].x;
+        out[2 * m + 1] = (double)x[m].y;
+    }
+    return pl.nst;
+}
+
+extern "C" int fftc_host_ip(int n, int dir, int dbl, int team, const double *in, double *out)
+{
+    if (n < 1 || team < 1) return -1;
+    if (dbl) return dir > 0 ? run_ip<double, 1>(n, team, in, out) : run_ip<double, -1>(n, team, in, out);
+    return dir > 0 ? run_ip<float, 1>(n, team, in, out) : run_ip<float, -1>(n, team, in, out);
+}

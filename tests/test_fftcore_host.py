@@ -6,7 +6,8 @@ Lengths cover each register codelet (2, 3, 4 and the odd primes up to 47),
 the direct stage of larger primes (109, 181, 349), the transform lengths the
 ring c2r meets at the configured grids (612, 362, 218, 1176, ...), the
 lag-Fourier length 1536 and the post-processing length 1519 = 7^2 31, with
-team sizes that do not divide the butterfly counts."""
+team sizes that do not divide the butterfly counts; and the in-place form
+(digit-reversed input) of the {2,3,5}-smooth channel-axis lengths."""
 import ctypes as C
 import os
 import subprocess
@@ -27,15 +28,17 @@ def lib(tmp_path_factory):
                     "-o", out], check=True, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
     L = C.CDLL(out)
     L.fftc_host.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+    L.fftc_host_ip.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
     return L
 
 
-def run(lib, x, direction, dbl, team):
+def run(lib, x, direction, dbl, team, inplace=False):
     n = len(x)
     a = np.empty(2 * n)
     a[0::2], a[1::2] = x.real, x.imag
     o = np.zeros(2 * n)
-    assert lib.fftc_host(n, direction, dbl, team, a.ctypes.data, o.ctypes.data) > 0 or n == 1
+    f = lib.fftc_host_ip if inplace else lib.fftc_host
+    assert f(n, direction, dbl, team, a.ctypes.data, o.ctypes.data) > 0 or n == 1
     return o[0::2] + 1j * o[1::2]
 
 
@@ -67,3 +70,21 @@ def test_against_direct_dft(lib, n):
     E = np.exp(2j * np.pi * np.outer(m, m) / n)
     assert np.allclose(run(lib, x, +1, 1, 64), E @ x, rtol=0, atol=1e-12 * np.abs(x).sum())
     assert np.allclose(run(lib, x, -1, 1, 64), np.conj(E) @ x, rtol=0, atol=1e-12 * np.abs(x).sum())
+
+
+# the in-place form (digit-reversed input) on {2,3,5}-smooth lengths: the
+# channel-axis transforms of the lag-Fourier spectra
+IP_LENGTHS = [2, 4, 6, 10, 12, 30, 48, 60, 128, 250, 384, 768, 1000, 1152, 1536, 1600, 2048]
+
+
+@pytest.mark.parametrize("n", IP_LENGTHS)
+@pytest.mark.parametrize("dbl", [1, 0])
+def test_in_place_against_numpy(lib, n, dbl):
+    rng = np.random.default_rng(7 * n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    tol = 1e-13 if dbl else 2e-6
+    for team in (64, 7):
+        fwd = run(lib, x, -1, dbl, team, inplace=True)
+        inv = run(lib, x, +1, dbl, team, inplace=True)
+        assert np.linalg.norm(fwd - np.fft.fft(x)) <= tol * np.linalg.norm(np.fft.fft(x)), (n, team)
+        assert np.linalg.norm(inv - np.fft.ifft(x) * n) <= tol * np.linalg.norm(np.fft.ifft(x) * n), (n, team)
