@@ -174,8 +174,10 @@ struct amb_ctx {
     void *lag_out2 = nullptr; size_t lag_out2_bytes = 0; /* second result buffer (copy overlap) */
     float lag_margin = 0.f; /* worst normalisation grid distance of the last lag-Fourier call */
     /* r1 == r2 tile tables of the lag-Fourier form by transform length:
-       device block [tiles (I, J)] [nt x nt tile index], tile count, bytes */
-    std::map<int, std::tuple<void *, int, size_t>> lag_pairs_tab;
+       device block [tiles (I, J)] [nt x nt tile index] [k2 list of the
+       k3 = L/2 column that no listed tile holds], tile count, bytes, list
+       length (-1: the list was too long, every k2 is summed apart) */
+    std::map<int, std::tuple<void *, int, size_t, int>> lag_pairs_tab;
     /* pinned host staging, two buffers: a grouped wave's results are copied
        out of one while the next wave's land in the other; the lag-space
        post-processing stages alternate triangles in them the same way */
@@ -274,6 +276,8 @@ struct LagTri {
     void *X[3] = {nullptr, nullptr, nullptr};
     const int2 *tlist = nullptr;
     const int *tmap = nullptr;
+    const int *nyql = nullptr;
+    int nnyq = 0;
     size_t cells = 0;
 };
 int lag_prepare(amb_ctx *c, int cube, int r1, int r2, int r3, LagTri *t);

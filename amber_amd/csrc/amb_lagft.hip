@@ -96,6 +96,9 @@ struct LagJob {
     C *nyq;                     /* [nseg][L] partials of the k3 = L/2 column, when a
                                    kernel of its own sums it (nyq_sep) */
     const int2 *tlist;          /* mode 1: (I, J) of each tile */
+    const int *nyql;            /* mode 1: the k2 of the k3 = L/2 column that no listed
+                                   tile holds (null: every k2) */
+    int nnyq;
     long P;
     long item0, nitem0;         /* first tile item, first nyquist item in the launch */
     long ibase;                 /* the triangle's item that the launch's item0 is (a
@@ -106,6 +109,9 @@ struct LagJob {
     int ntile;                  /* tiles per segment */
     int nyq_sep;                /* the k3 = L/2 column is summed by k_lagft_nyq */
 };
+
+/* most k2 of the k3 = L/2 column a pair-tile job leaves to k_lagft_nyq */
+constexpr int NYQ_LIST_MAX = 32;
 
 template <typename R>
 __host__ __device__ inline int tile_a() { return 16 * LagT<R>::BU; }
@@ -526,7 +532,10 @@ __global__ __launch_bounds__(256, LagT<R>::WPS) void k_lagft_tiles(const LagJob<
 
 /* --------------------------------------------------------------- nyquist */
 /* S(k2, L/2) = sum_x conj F(k2 + L/2) G(k2) H(L/2) over one segment per
-   block, for jobs with nyq_sep; thread t owns k2 = t + 256 q */
+   block, for jobs with nyq_sep.  Every k2: thread t owns k2 = t + 256 q.
+   A k2 list (pair-tile jobs, whose listed tiles hold the rest of the
+   column): thread t sums pixels xb + t + 256 m for every listed k2, then a
+   fixed-order tree over the block. */
 template <typename R>
 __global__ __launch_bounds__(256) void k_lagft_nyq(const LagJob<R> *__restrict__ jobs, int njob, long total)
 {
@@ -539,6 +548,55 @@ __global__ __launch_bounds__(256) void k_lagft_nyq(const LagJob<R> *__restrict__
     const int lhp = jobs[jb].lhp, L = jobs[jb].L, seg = jobs[jb].seg, h = L / 2;
     const int sg = (int)(it - jobs[jb].nitem0);
     const long xb = (long)sg * seg, xe = xb + seg < P ? xb + seg : P;
+    if constexpr (sizeof(R) == 4) if (jobs[jb].nyql) {
+        const int *__restrict__ kl = jobs[jb].nyql;
+        const int nk = jobs[jb].nnyq;
+        R lr[NYQ_LIST_MAX], li[NYQ_LIST_MAX];
+#pragma unroll
+        for (int q = 0; q < NYQ_LIST_MAX; ++q) lr[q] = li[q] = (R)0;
+        for (long x = xb + threadIdx.x; x < xe; x += 256) {
+            const size_t row = (size_t)x * lhp;
+            const C hv = H[row + h];
+#pragma unroll
+            for (int q = 0; q < NYQ_LIST_MAX; ++q) {
+                if (q < nk) {
+                    const int k2 = kl[q];
+                    const C g = spec_at(G, row, k2, L);
+                    int s2 = k2 + h;
+                    if (s2 >= L) s2 -= L;
+                    const C f = spec_at(F, row, s2, L);
+                    const R tr = fma(g.x, hv.x, -g.y * hv.y), ti = fma(g.x, hv.y, g.y * hv.x);
+                    lr[q] = fma(f.x, tr, fma(f.y, ti, lr[q]));
+                    li[q] = fma(f.x, ti, fma(-f.y, tr, li[q]));
+                }
+            }
+        }
+        __shared__ R red[2][256];
+        for (int q = 0; q < nk; ++q) {
+            R vr = (R)0, vi = (R)0;
+#pragma unroll
+            for (int u = 0; u < NYQ_LIST_MAX; ++u)
+                if (u == q) { vr = lr[u]; vi = li[u]; }
+            red[0][threadIdx.x] = vr;
+            red[1][threadIdx.x] = vi;
+            __syncthreads();
+            for (int w = 128; w > 0; w >>= 1) {
+                if ((int)threadIdx.x < w) {
+                    red[0][threadIdx.x] += red[0][threadIdx.x + w];
+                    red[1][threadIdx.x] += red[1][threadIdx.x + w];
+                }
+                __syncthreads();
+            }
+            if (threadIdx.x == 0) {
+                C w;
+                w.x = red[0][0];
+                w.y = red[1][0];
+                jobs[jb].nyq[(size_t)sg * L + kl[q]] = w;
+            }
+            __syncthreads();
+        }
+        return;
+    }
     constexpr int NQ = MAXL / 256;
     R acr[NQ], aci[NQ];
 #pragma unroll
@@ -591,13 +649,16 @@ __global__ __launch_bounds__(256) void k_lagft_reduce(const typename LagT<R>::C 
     const int X = blockIdx.x * 32, Y = blockIdx.y * 32;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
     const int k3 = Y + tx, half = L / 2;
-    const bool col = !M1 && half % TB != 0; /* k3 = L/2 is a tile column */
+    /* k3 = L/2 is a tile column (mode 0, TB does not divide L/2), or in mode
+       1 a diagonal of the listed tiles (k_lagft_reduce_nyq then writes the
+       k2 that no listed tile holds) */
+    const bool col = M1 || half % TB != 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int k2 = X + ty + 8 * q;
         double sx = 0.0, sy = 0.0;
         if (k2 < L && (k3 < half || (col && k3 == half))) {
-            size_t off;
+            long off = 0;
             if (M1) {
                 int i = k2 == 0 ? 0 : L - k2, j = k2 + k3;
                 if (j >= L) j -= L;
@@ -606,11 +667,12 @@ __global__ __launch_bounds__(256) void k_lagft_reduce(const typename LagT<R>::C 
                     const int t = i; i = j; j = t;
                     const int T = I; I = J; J = T;
                 }
-                off = (size_t)tmap[I * (L / TA) + J] * (TA * TB) + (i - I * TA) * TB + (j - J * TB);
+                const int tl = tmap[I * (L / TA) + J];
+                off = tl < 0 ? -1 : (long)tl * (TA * TB) + (i - I * TA) * TB + (j - J * TB);
             } else {
-                off = (size_t)((k2 / TA) * nt3 + k3 / TB) * (TA * TB) + (k2 % TA) * TB + (k3 % TB);
+                off = (long)((k2 / TA) * nt3 + k3 / TB) * (TA * TB) + (k2 % TA) * TB + (k3 % TB);
             }
-            for (int sg = 0; sg < nseg; ++sg) {
+            for (int sg = 0; sg < nseg && off >= 0; ++sg) {
                 const C v = part[sg * tsz + off];
                 sx += (double)v.x;
                 sy += (double)v.y;
@@ -627,13 +689,14 @@ __global__ __launch_bounds__(256) void k_lagft_reduce(const typename LagT<R>::C 
 }
 
 /* St[L/2][k2] = sum over segments of the k3 = L/2 partials (FP64, in
-   order), when the last full tile summed them */
+   order), for every k2 or the nk listed ones (kl non-null) */
 template <typename R>
 __global__ __launch_bounds__(256) void k_lagft_reduce_nyq(const typename LagT<R>::C *__restrict__ nyq, int nseg, int L,
-                                                          double2 *__restrict__ St)
+                                                          const int *__restrict__ kl, int nk, double2 *__restrict__ St)
 {
-    const int k2 = blockIdx.x * 256 + threadIdx.x;
-    if (k2 >= L) return;
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    if (kl ? idx >= nk : idx >= L) return;
+    const int k2 = kl ? kl[idx] : idx;
     double sx = 0.0, sy = 0.0;
     for (int sg = 0; sg < nseg; ++sg) {
         const typename LagT<R>::C v = nyq[(size_t)sg * L + k2];
@@ -892,6 +955,8 @@ struct LJ {
     int mode, ntile, nyq_sep;
     const int2 *tlist; /* mode 1 */
     const int *tmap;   /* mode 1 */
+    const int *nyql;   /* mode 1: k2 of the k3 = L/2 column left to k_lagft_nyq (null: all) */
+    int nnyq;
     void *X[3];     /* spectra of r1, r2, r3 */
     size_t part, nyq, out, out64;
 };
@@ -938,6 +1003,8 @@ void fill_job(LagJob<R> &g, const LJ &j, char *partbase, long *items, long *nite
     g.part = (C *)(partbase + j.part);
     g.nyq = (C *)(partbase + j.nyq);
     g.tlist = j.tlist;
+    g.nyql = j.nyql;
+    g.nnyq = j.nnyq;
     g.P = j.P;
     g.lhp = j.cb->lag_lhp;
     g.L = L;
@@ -968,36 +1035,56 @@ hipError_t launch_products(hipStream_t s, const LagJob<R> *d, int n, long items,
    whether k_lagft_nyq sums the k3 = L/2 column (pair tiles, and FP64 where
    the column is not a tile column) */
 int choose_tiles(amb_ctx *c, int L, bool fp64, bool r12, int *mode, int *ntile, int *nyq_sep, const int2 **tlist,
-                 const int **tmap)
+                 const int **tmap, const int **nyql, int *nnyq)
 {
     *mode = 0;
     *ntile = fp64 ? tiles2<double>(L) * tiles3<double>(L) : tiles2<float>(L) * tiles3<float>(L);
     *tlist = nullptr;
     *tmap = nullptr;
+    *nyql = nullptr;
+    *nnyq = 0;
     if (!fp64 && r12 && !c->no_sym && L % 128 == 0) {
         auto it = c->lag_pairs_tab.find(L);
         if (it == c->lag_pairs_tab.end()) {
+            /* tiles, the tile map, and the k2 whose k3 = L/2 cell, (i, j) =
+               (-k2, k2 + L/2), lies in no listed tile */
             const std::vector<int2> tl = pair_tiles(L);
             const int nt = L / 128;
-            std::vector<int> tm((size_t)nt * nt, -1);
+            std::vector<int> tm((size_t)nt * nt, -1), miss;
             for (size_t t = 0; t < tl.size(); ++t) tm[(size_t)tl[t].x * nt + tl[t].y] = (int)t;
+            for (int k2 = 0; k2 < L; ++k2) {
+                const int i = (L - k2) % L, j = (k2 + L / 2) % L;
+                const int I = std::min(i, j) / 128, J = std::max(i, j) / 128;
+                if (tm[(size_t)I * nt + J] < 0) miss.push_back(k2);
+            }
+            const int nm = (int)miss.size() <= NYQ_LIST_MAX ? (int)miss.size() : -1; /* -1: every k2 */
             void *d = nullptr;
             const size_t tb = sizeof(int2) * tl.size(), mb = sizeof(int) * tm.size();
-            const int rc = dev_alloc(c, &d, tb + mb, "lag-Fourier tile tables");
+            const size_t lb = sizeof(int) * std::max(nm, 0);
+            const int rc = dev_alloc(c, &d, tb + mb + lb, "lag-Fourier tile tables");
             if (rc) return rc;
             HIPCHK(c, hipMemcpy(d, tl.data(), tb, hipMemcpyHostToDevice));
             HIPCHK(c, hipMemcpy((char *)d + tb, tm.data(), mb, hipMemcpyHostToDevice));
-            it = c->lag_pairs_tab.emplace(L, std::make_tuple(d, (int)tl.size(), tb + mb)).first;
+            if (nm > 0) HIPCHK(c, hipMemcpy((char *)d + tb + mb, miss.data(), lb, hipMemcpyHostToDevice));
+            it = c->lag_pairs_tab.emplace(L, std::make_tuple(d, (int)tl.size(), tb + mb + lb, nm)).first;
         }
         const int n1 = std::get<1>(it->second);
         if (n1 < *ntile) {
+            const int nt = L / 128;
             *mode = 1;
             *ntile = n1;
             *tlist = (const int2 *)std::get<0>(it->second);
             *tmap = (const int *)((const char *)std::get<0>(it->second) + sizeof(int2) * n1);
+            const int nm = std::get<3>(it->second);
+            if (nm >= 0) {
+                *nyql = *tmap + (size_t)nt * nt;
+                *nnyq = nm;
+            }
         }
     }
-    *nyq_sep = (*mode == 1 || (fp64 && (L / 2) % tile_b<double>() == 0)) ? 1 : 0;
+    /* pair tiles whose listed tiles hold the whole k3 = L/2 column need no
+       kernel for it */
+    *nyq_sep = ((*mode == 1 && !(*nyql && *nnyq == 0)) || (fp64 && (L / 2) % tile_b<double>() == 0)) ? 1 : 0;
     return AMB_OK;
 }
 
@@ -1127,7 +1214,7 @@ int amb::lagft_run(amb_ctx *c, const amb_tri_req *req, int n, double t0, double 
                                    j.ntile, L, nullptr, St);
                 if (j.nyq_sep)
                     hipLaunchKernelGGL(k_lagft_reduce_nyq<double>, dim3((unsigned)((L + 255) / 256)), dim3(256), 0,
-                                       c->stream, g.nyq, j.nseg, L, St);
+                                       c->stream, g.nyq, j.nseg, L, nullptr, 0, St);
             } else {
                 const LagJob<float> &g = d32[slot[i]];
                 if (j.mode == 1)
@@ -1141,7 +1228,7 @@ int amb::lagft_run(amb_ctx *c, const amb_tri_req *req, int n, double t0, double 
                 const bool folded = j.mode == 0 && (L / 2) % tile_b<float>() == 0;
                 if (j.nyq_sep || folded)
                     hipLaunchKernelGGL(k_lagft_reduce_nyq<float>, dim3((unsigned)((L + 255) / 256)), dim3(256), 0,
-                                       c->stream, g.nyq, j.nseg, L, St);
+                                       c->stream, g.nyq, j.nseg, L, j.nyql, j.nnyq, St);
             }
             const FftPlan &pl = c->plans[L];
             const double2 *tw = c->tw64[L];
@@ -1201,8 +1288,10 @@ int amb::lagft_run(amb_ctx *c, const amb_tri_req *req, int n, double t0, double 
             /* 14 flops per distinct (k2, k3, pixel) of the half spectrum
                k3 <= L/2: half of them where r1 == r2 pairs equal sums */
             c->st.flop_lagft += 14.0 * (double)L * (double)(L / 2 + 1) * (double)j.P * (j.mode == 1 ? 0.5 : 1.0);
-            /* issued: whole tiles, plus the k3 = L/2 column */
-            c->st.flop_contract_exec += 14.0 * ((double)j.ntile * tile_a<float>() * tile_b<float>() + L) * (double)j.P;
+            /* issued: whole tiles, plus the k3 = L/2 column (or its k2 left
+               to k_lagft_nyq by pair tiles) */
+            c->st.flop_contract_exec +=
+                14.0 * ((double)j.ntile * tile_a<float>() * tile_b<float>() + (j.nyql ? j.nnyq : L)) * (double)j.P;
         }
         prev = std::move(w);
         w.clear();
@@ -1246,7 +1335,8 @@ int amb::lagft_run(amb_ctx *c, const amb_tri_req *req, int n, double t0, double 
                 j.ng = ng;
                 j.P = ng * ng;
                 j.nseg = (int)((j.P + SEG - 1) / SEG);
-                if ((rc = choose_tiles(c, L, nm, q.r1 == q.r2, &j.mode, &j.ntile, &j.nyq_sep, &j.tlist, &j.tmap))) {
+                if ((rc = choose_tiles(c, L, nm, q.r1 == q.r2, &j.mode, &j.ntile, &j.nyq_sep, &j.tlist, &j.tmap,
+                                       &j.nyql, &j.nnyq))) {
                     unpin_all(c);
                     return rc;
                 }
@@ -1309,7 +1399,9 @@ int amb::lag_prepare(amb_ctx *c, int cube, int r1, int r2, int r3, LagTri *t)
     t->L = L;
     t->nseg = (int)((t->P + SEG - 1) / SEG);
     t->cells = cb->h_lagcnt.size();
-    if ((rc = choose_tiles(c, L, false, r1 == r2, &t->mode, &t->ntile, &t->nyq_sep, &t->tlist, &t->tmap))) return rc;
+    if ((rc = choose_tiles(c, L, false, r1 == r2, &t->mode, &t->ntile, &t->nyq_sep, &t->tlist, &t->tmap, &t->nyql,
+                           &t->nnyq)))
+        return rc;
     const PlaneSrc src{&c->planes};
     const int rr[3] = {r1, r2, r3};
     for (int k = 0; k < 3; ++k)
@@ -1345,6 +1437,8 @@ void amb::lag_fill_range(amb_ctx *c, const LagTri &t, void *part, long i0, long 
     j.nyq_sep = t.nyq_sep;
     j.tlist = t.tlist;
     j.tmap = t.tmap;
+    j.nyql = t.nyql;
+    j.nnyq = t.nnyq;
     for (int k = 0; k < 3; ++k) j.X[k] = t.X[k];
     j.part = 0;
     j.nyq = sizeof(float2) * (size_t)j.nseg * j.ntile * tile_a<float>() * tile_b<float>();
@@ -1387,7 +1481,7 @@ hipError_t amb::lag_launch_finish(amb_ctx *c, hipStream_t s, const LagTri &t, co
     const bool folded = t.mode == 0 && (L / 2) % tile_b<float>() == 0;
     if (t.nyq_sep || folded)
         hipLaunchKernelGGL(k_lagft_reduce_nyq<float>, dim3((unsigned)((L + 255) / 256)), dim3(256), 0, s, nyq, t.nseg, L,
-                           st);
+                           t.nyql, t.nnyq, st);
     const FftPlan &pl = c->plans[L];
     const double2 *tw = c->tw64[L];
     const size_t lds = sizeof(double2) * 2 * (size_t)L;
@@ -1410,5 +1504,6 @@ void amb::lag_account(amb_ctx *c, const LagTri &t, int32_t *cnt)
     c->st.flop_contract += 2.0 * (double)cb.lag_pairs * (double)t.P;
     c->st.flop_contract_distinct += 2.0 * (double)cb.lag_pairs * (double)t.P;
     c->st.flop_lagft += 14.0 * (double)L * (double)(L / 2 + 1) * (double)t.P * (t.mode == 1 ? 0.5 : 1.0);
-    c->st.flop_contract_exec += 14.0 * ((double)t.ntile * tile_a<float>() * tile_b<float>() + L) * (double)t.P;
+    c->st.flop_contract_exec +=
+        14.0 * ((double)t.ntile * tile_a<float>() * tile_b<float>() + (t.nyql ? t.nnyq : L)) * (double)t.P;
 }
