@@ -13,7 +13,8 @@
  * odd primes up to 47, the primes as a symmetric-pair DFT whose cos/sin
  * coefficients are read from the transform's own twiddle table
  * (tw[t n/R] = e^{2 pi i t/R}).  Any larger prime factor runs a direct stage
- * (one output per thread and step).
+ * (inputs twiddled in place, then two outputs per thread and step from
+ * symmetric pairs).
  *
  * Direction DIR = +1 computes y_k = sum_m x_m e^{+2 pi i m k / n} (the
  * unnormalised inverse, FFTW's c2r sign), DIR = -1 the forward transform.
@@ -175,28 +176,71 @@ FFTC_HD void stage(const C *__restrict__ x, C *__restrict__ y, const C *__restri
     }
 }
 
-/* direct stage for a prime without a codelet: one output per step,
-   out[g Ns R + k + q Ns] = sum_r x[g Ns + k + r nR] w^{r (k + q Ns)},
-   w = e^{DIR 2 pi i / (Ns R)} */
+/* Stage of a prime R without a codelet, in two steps with a workgroup
+ * barrier between them.  First the inputs r >= 1 of every butterfly are
+ * twiddled in place, x[j + r nR] *= w^{r k n/(Ns R)}; then each step of a
+ * thread forms outputs q and R - q of one butterfly from the symmetric
+ * pairs a_m = v_m + v_{R-m}, b_m = v_m - v_{R-m}:
+ *   y_q = v_0 + sum a_m cos(2 pi m q/R) + i sum b_m DIR sin(2 pi m q/R),
+ *   y_{R-q} = the same with -i,
+ * cos/sin read from the transform's table, and q = 0 sums all inputs. */
 template <int DIR, typename C>
-__host__ __device__ void stage_direct(const C *__restrict__ x, C *__restrict__ y, const C *__restrict__ tw, int n, int R,
-                             int Ns, int t, int T)
+FFTC_HD void stage_direct_tw(C *__restrict__ x, const C *__restrict__ tw, int n, int R, int Ns, int t, int T)
+{
+    /* element u of the n - nR inputs r >= 1: j = u mod nR, r = 1 + u / nR,
+       k = j mod Ns (Ns divides nR), advanced by constant steps */
+    const int nR = n / R, tstep = n / (Ns * R), tot = n - nR;
+    const int dj = T % nR, dr = T / nR, dk = dj % Ns;
+    int j = t % nR, r = 1 + t / nR, k = j % Ns;
+    for (int u = t; u < tot; u += T) {
+        x[j + r * nR] = twmul<DIR>(x[j + r * nR], tw[r * k * tstep]);
+        j += dj;
+        r += dr;
+        k += dk;
+        if (k >= Ns) k -= Ns;
+        if (j >= nR) { j -= nR; ++r; }
+    }
+}
+template <int DIR, typename C>
+FFTC_HD void stage_direct_dft(const C *__restrict__ x, C *__restrict__ y, const C *__restrict__ tw, int n, int R,
+                              int Ns, int t, int T)
 {
     typedef decltype(x[0].x) Tr;
-    const int nR = n / R, NsR = Ns * R, tstep = n / NsR;
-    for (int o = t; o < n; o += T) {
-        const int g = o / NsR, rem = o - g * NsR;
-        const int q = rem / Ns, k = rem - q * Ns;
-        const int e = k + q * Ns;
-        const C *xi = x + g * Ns + k;
-        C acc = cx((Tr)0, (Tr)0);
-        int ex = 0;
-        for (int r = 0; r < R; ++r) {
-            acc = cadd(acc, twmul<DIR>(xi[r * nR], tw[ex * tstep]));
-            ex += e;
-            if (ex >= NsR) ex -= NsR;
+    /* item u of nR (h + 1): butterfly j = u mod nR = g Ns + k, output pair q = u / nR */
+    const int nR = n / R, h = (R - 1) / 2, rstep = n / R, tot = nR * (h + 1);
+    const int dj = T % nR, dq = T / nR, dgj = dj / Ns, dkj = dj - dgj * Ns, gw = nR / Ns;
+    int j = t % nR, q = t / nR, g = j / Ns, k = j - g * Ns;
+    for (int u = t; u < tot; u += T) {
+        const C *xi = x + j;
+        C *yo = y + g * Ns * R + k;
+        if (q == 0) {
+            C sum = xi[0];
+            for (int r = 1; r < R; ++r) sum = cadd(sum, xi[r * nR]);
+            yo[0] = sum;
+        } else {
+            C A = xi[0], B = cx((Tr)0, (Tr)0);
+            int tq = 0;
+#pragma unroll 4
+            for (int m = 1; m <= h; ++m) {
+                tq += q;
+                if (tq >= R) tq -= R;
+                const C w = tw[tq * rstep];
+                const Tr c = w.x, sn = DIR > 0 ? w.y : -w.y;
+                const C vm = xi[m * nR], vr = xi[(R - m) * nR];
+                A.x += (vm.x + vr.x) * c;
+                A.y += (vm.y + vr.y) * c;
+                B.x += (vm.x - vr.x) * sn;
+                B.y += (vm.y - vr.y) * sn;
+            }
+            yo[q * Ns] = cx(A.x - B.y, A.y + B.x);
+            yo[(R - q) * Ns] = cx(A.x + B.y, A.y - B.x);
         }
-        y[o] = acc;
+        j += dj;
+        q += dq;
+        k += dkj;
+        g += dgj;
+        if (k >= Ns) { k -= Ns; ++g; }
+        if (j >= nR) { j -= nR; ++q; g -= gw; }
     }
 }
 
@@ -230,7 +274,11 @@ __device__ C *transform(C *x, C *y, const C *tw, const FftPlan &pl, int t, int T
         case 41: if constexpr (MAXP >= 41) stage<41, DIR>(x, y, tw, n, Ns, t, T); break;
         case 43: if constexpr (MAXP >= 43) stage<43, DIR>(x, y, tw, n, Ns, t, T); break;
         case 47: if constexpr (MAXP >= 47) stage<47, DIR>(x, y, tw, n, Ns, t, T); break;
-        default: stage_direct<DIR>(x, y, tw, n, R, Ns, t, T); break;
+        default:
+            stage_direct_tw<DIR>(x, tw, n, R, Ns, t, T);
+            __syncthreads();
+            stage_direct_dft<DIR>(x, y, tw, n, R, Ns, t, T);
+            break;
         }
         __syncthreads();
         C *tmp = x;

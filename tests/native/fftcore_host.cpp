@@ -10,7 +10,7 @@
 #include <vector>
 
 template <typename T, int DIR>
-static int run(int n, int TT, const double *in, double *out)
+static int run(int n, int TT, int maxp, const double *in, double *out)
 {
     typedef typename fftc::CT<T>::c C;
     FftPlan pl;
@@ -25,13 +25,20 @@ static int run(int n, int TT, const double *in, double *out)
     int Ns = 1;
     for (int s = 0; s < pl.nst; ++s) {
         const int R = pl.rad[s];
-        for (int t = 0; t < TT; ++t) {
+        /* primes above maxp take the direct stage (its two steps are
+           separated by the barrier every thread reaches) */
+        const bool direct = R > maxp || R > 47;
+        if (direct) {
+            for (int t = 0; t < TT; ++t) fftc::stage_direct_tw<DIR>(px, tw.data(), n, R, Ns, t, TT);
+            for (int t = 0; t < TT; ++t) fftc::stage_direct_dft<DIR>(px, py, tw.data(), n, R, Ns, t, TT);
+        }
+        for (int t = 0; t < TT && !direct; ++t) {
             switch (R) {
 #define CASE(r) case r: fftc::stage<r, DIR>(px, py, tw.data(), n, Ns, t, TT); break;
             CASE(2) CASE(3) CASE(4) CASE(5) CASE(7) CASE(11) CASE(13) CASE(17) CASE(19) CASE(23) CASE(29)
             CASE(31) CASE(37) CASE(41) CASE(43) CASE(47)
 #undef CASE
-            default: fftc::stage_direct<DIR>(px, py, tw.data(), n, R, Ns, t, TT); break;
+            default: break;
             }
         }
         C *tmp = px; px = py; py = tmp;
@@ -44,11 +51,12 @@ static int run(int n, int TT, const double *in, double *out)
     return pl.nst;
 }
 
-extern "C" int fftc_host(int n, int dir, int dbl, int team, const double *in, double *out)
+/* maxp: largest prime run as a codelet (the device kernels' MAXP) */
+extern "C" int fftc_host(int n, int dir, int dbl, int team, int maxp, const double *in, double *out)
 {
     if (n < 1 || team < 1) return -1;
-    if (dbl) return dir > 0 ? run<double, 1>(n, team, in, out) : run<double, -1>(n, team, in, out);
-    return dir > 0 ? run<float, 1>(n, team, in, out) : run<float, -1>(n, team, in, out);
+    if (dbl) return dir > 0 ? run<double, 1>(n, team, maxp, in, out) : run<double, -1>(n, team, maxp, in, out);
+    return dir > 0 ? run<float, 1>(n, team, maxp, in, out) : run<float, -1>(n, team, maxp, in, out);
 }
 
 /* the in-place form: input placed at its digit-reversed positions */

@@ -27,18 +27,21 @@ def lib(tmp_path_factory):
                     "-I" + os.path.join(ROOT, "amber_amd", "csrc"), "-I" + os.path.join(ROOT, "include"), SRC,
                     "-o", out], check=True, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
     L = C.CDLL(out)
-    L.fftc_host.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+    L.fftc_host.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
     L.fftc_host_ip.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
     return L
 
 
-def run(lib, x, direction, dbl, team, inplace=False):
+def run(lib, x, direction, dbl, team, inplace=False, maxp=47):
     n = len(x)
     a = np.empty(2 * n)
     a[0::2], a[1::2] = x.real, x.imag
     o = np.zeros(2 * n)
-    f = lib.fftc_host_ip if inplace else lib.fftc_host
-    assert f(n, direction, dbl, team, a.ctypes.data, o.ctypes.data) > 0 or n == 1
+    if inplace:
+        rc = lib.fftc_host_ip(n, direction, dbl, team, a.ctypes.data, o.ctypes.data)
+    else:
+        rc = lib.fftc_host(n, direction, dbl, team, maxp, a.ctypes.data, o.ctypes.data)
+    assert rc > 0 or n == 1
     return o[0::2] + 1j * o[1::2]
 
 
@@ -88,3 +91,19 @@ def test_in_place_against_numpy(lib, n, dbl):
         inv = run(lib, x, +1, dbl, team, inplace=True)
         assert np.linalg.norm(fwd - np.fft.fft(x)) <= tol * np.linalg.norm(np.fft.fft(x)), (n, team)
         assert np.linalg.norm(inv - np.fft.ifft(x) * n) <= tol * np.linalg.norm(np.fft.ifft(x) * n), (n, team)
+
+
+# the direct stage of primes above a kernel's codelet limit (the ring c2r
+# kernels stop at 23 in FP32, 19 in FP64): grids 492 = 4 3 41, 258 = 2 3 43,
+# 62, 74, the post-processing length 1519 = 7^2 31, and larger primes
+@pytest.mark.parametrize("n", [29, 62, 74, 258, 492, 1519, 109 * 2, 349, 1000 - 1])
+@pytest.mark.parametrize("maxp", [23, 19, 5])
+def test_direct_stage_against_numpy(lib, n, maxp):
+    rng = np.random.default_rng(11 * n + maxp)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    for dbl, tol in ((1, 1e-13), (0, 3e-6)):
+        for team in (64, 7):
+            fwd = run(lib, x, -1, dbl, team, maxp=maxp)
+            inv = run(lib, x, +1, dbl, team, maxp=maxp)
+            assert np.linalg.norm(fwd - np.fft.fft(x)) <= tol * np.linalg.norm(np.fft.fft(x)), (n, team, dbl)
+            assert np.linalg.norm(inv - np.fft.ifft(x) * n) <= tol * np.linalg.norm(np.fft.ifft(x) * n), (n, team, dbl)
