@@ -338,14 +338,14 @@ static size_t plane_bytes(int kind, int nact, size_t pitch)
 {
     if (kind == 4) return pitch * nact;
     if (kind == 5) return sizeof(float) * 2 * nact;
-    if (kind == 0) return sizeof(float) * pitch * nact + AMB_SET_SLOT_BYTES; /* + [max, E] */
+    if (kind == 0 || kind == 11) return sizeof(float) * pitch * nact + AMB_SET_SLOT_BYTES; /* + [max, E] */
     if (kind == 8) return sizeof(double) * pitch * nact;
     return sizeof(float) * pitch * nact; /* FP32, or hi + lo BF16 / FP16 */
 }
 
 size_t build_scratch_bytes(amb_ctx *c, const BuildOp &op)
 {
-    if (op.kind > 1 && op.kind != 8) return 0;
+    if (op.kind > 1 && op.kind != 8 && op.kind != 11) return 0;
     const Cube &cb = c->cubes[op.cube];
     const size_t cb8 = op.kind == 8 ? sizeof(double2) : sizeof(float2);
     return cb8 * (size_t)cb.nact() * (size_t)std::max(op.g.jcnt, 1) * op.g.ng;
@@ -376,13 +376,15 @@ hipError_t launch_build(amb_ctx *c, hipStream_t s, const BuildOp &op, float2 *G)
     const long pitch = plane_pitch(op.g.ng);
     switch (op.kind) {
     case 0:
-    case 1: {
+    case 1:
+    case 11: {
         const float2 *tw = c->tw[op.g.ng];
         const FftPlan &pl = c->plans[op.g.ng];
-        /* data sets are normalised in place, their [max, E] words after the planes */
-        unsigned *slot = op.kind == 0 ? (unsigned *)((float *)op.dst + (size_t)pitch * nact) : nullptr;
-        return amb_launch_ring_c2r(s, op.kind, cb.vis_t, cb.samp_t, cb.d_act, nact, (int)cb.N, (int)(cb.N / 2 + 1),
-                                   pl, tw, op.g, G, (float *)op.dst, pitch, slot);
+        /* data sets are normalised in place (kind 11: left as computed), their
+           [max, E] words after the planes */
+        unsigned *slot = op.kind != 1 ? (unsigned *)((float *)op.dst + (size_t)pitch * nact) : nullptr;
+        return amb_launch_ring_c2r(s, op.kind == 11 ? 0 : op.kind, cb.vis_t, cb.samp_t, cb.d_act, nact, (int)cb.N,
+                                   (int)(cb.N / 2 + 1), pl, tw, op.g, G, (float *)op.dst, pitch, slot, op.kind == 11);
     }
     case 2:
     case 3: {
@@ -413,9 +415,9 @@ hipError_t launch_build(amb_ctx *c, hipStream_t s, const BuildOp &op, float2 *G)
 double build_fft_bytes(amb_ctx *c, const BuildOp &op)
 {
     const Cube &cb = c->cubes[op.cube];
-    if (op.kind >= 9) /* the image set read, the half spectra written */
+    if (op.kind == 9 || op.kind == 10) /* the image set read, the half spectra written */
         return (double)op.g.ng * op.g.ng * (op.kind == 10 ? 2.0 : 1.0) * (4.0 * cb.nact() + 8.0 * (cb.lag_L / 2 + 1));
-    if (op.kind > 1) return 0.0;
+    if (op.kind > 1 && op.kind != 11) return 0.0;
     const RingGeom &g = op.g;
     double cells = 0;
     for (long ia = -(long)ceil(g.hi); ia <= (long)ceil(g.hi); ++ia)
@@ -423,7 +425,7 @@ double build_fft_bytes(amb_ctx *c, const BuildOp &op)
             double uu = sqrt((double)(ia * ia + j * j));
             if (uu >= g.lo && uu < g.hi && uu > 0) cells += 1;
         }
-    return (double)cb.nact() * (4.0 * (double)g.ng * g.ng + (op.kind == 0 ? 8.0 : 4.0) * cells);
+    return (double)cb.nact() * (4.0 * (double)g.ng * g.ng + (op.kind != 1 ? 8.0 : 4.0) * cells);
 }
 
 int get_planes(amb_ctx *c, const PlaneSrc &src, int cube_id, int kind, int r, int ng, void **out)
@@ -460,12 +462,12 @@ int get_planes(amb_ctx *c, const PlaneSrc &src, int cube_id, int kind, int r, in
            set (FP64 images), with the channel transform's twiddles */
         if (src.record || src.bump) return fail(c, AMB_EARG, "lag-Fourier planes are not built inside graphs");
         if ((rc = lag_setup(c, *cb))) return rc;
-        if ((rc = get_planes(c, src, cube_id, kind == 9 ? 0 : 8, r, ng, &dep))) return rc;
+        if ((rc = get_planes(c, src, cube_id, kind == 9 ? 11 : 8, r, ng, &dep))) return rc;
         float2 *tw;
         FftPlan pl;
         if ((rc = twiddles(c, cb->lag_L, &tw, &pl))) return rc;
         if (kind == 10 && (rc = twiddles64(c, cb->lag_L, nullptr))) return rc;
-    } else if (kind >= 4) {
+    } else if (kind >= 4 && kind != 11) {
         if ((rc = get_planes(c, src, cube_id, 0, r, ng, &dep))) return rc;
         if (kind == 4 && (rc = get_planes(c, src, cube_id, 5, r, ng, &sc))) return rc;
     } else {
@@ -478,7 +480,8 @@ int get_planes(amb_ctx *c, const PlaneSrc &src, int cube_id, int kind, int r, in
     }
     op.src = dep;
     op.sc = sc;
-    const size_t bytes = kind >= 9 ? lag_spectra_bytes(*cb, ng, kind == 10) : plane_bytes(kind, cb->nact(), plane_pitch(ng));
+    const size_t bytes = kind == 9 || kind == 10 ? lag_spectra_bytes(*cb, ng, kind == 10)
+                                                 : plane_bytes(kind, cb->nact(), plane_pitch(ng));
     void *d = nullptr;
     if (src.bump) {
         const size_t need = (bytes + 255) / 256 * 256;
@@ -493,17 +496,18 @@ int get_planes(amb_ctx *c, const PlaneSrc &src, int cube_id, int kind, int r, in
     if (src.record) {
         src.record->push_back(op);
     } else {
-        if ((kind <= 1 || kind == 8) && (rc = scratch(c, &c->G, &c->G_bytes, build_scratch_bytes(c, op), "fft scratch"))) {
+        const bool c2r = kind <= 1 || kind == 11, spectra = kind == 9 || kind == 10;
+        if ((c2r || kind == 8) && (rc = scratch(c, &c->G, &c->G_bytes, build_scratch_bytes(c, op), "fft scratch"))) {
             plane_free(c, d, bytes);
             return rc;
         }
-        if (kind <= 1 || kind >= 9) {
+        if (c2r || spectra) {
             timer_start(c, &c->st.ms_fft);
-            c->st.n_fft_launch += kind >= 9 ? 1 : 2;
+            c->st.n_fft_launch += spectra ? 1 : 2;
             c->st.bytes_fft += build_fft_bytes(c, op);
         }
         hipError_t e = launch_build(c, c->stream, op, c->G);
-        if (kind <= 1 || kind >= 9) timer_stop(c);
+        if (c2r || spectra) timer_stop(c);
         if (e != hipSuccess) {
             plane_free(c, d, bytes);
             return fail(c, AMB_EHIP, "plane build (kind %d): %s", kind, hipGetErrorString(e));

@@ -60,9 +60,12 @@ struct Cube {
  * 5: per-plane FP8 scales of kind 0 (A side [nact], then B side [nact]),
  * 7: hi and lo FP16 halves of kind 0 times 2^8, stored hi planes then lo,
  * 8: FP64 sampling images (the exact v2 normalisation),
- * 9: channel spectra of kind 0 (lag-Fourier form): [Ng^2][lag_lhp] complex
- *    FP32, the DFT over channels of each pixel, then the [max, E] words,
- * 10: channel spectra of kind 8 (the lag-Fourier normalisation), complex FP64.
+ * 9: channel spectra of kind 11 (lag-Fourier form): [Ng^2][lag_lhp] complex
+ *    FP32, the DFT over channels of each pixel of the set times 2^(7 - E),
+ *    then the [max, E] words,
+ * 10: channel spectra of kind 8 (the lag-Fourier normalisation), complex FP64,
+ * 11: data images (FP32) as computed, not normalised, then the max word
+ *    (the source of kind 9, which scales while it reads).
  * Kind 0 sets are stored normalised by a power of two (max |D| <= 2^7) and
  * end in two words: max |D| as computed and its exponent E. */
 struct PlaneSet {

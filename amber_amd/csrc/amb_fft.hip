@@ -235,19 +235,8 @@ hipError_t ring_passes(hipStream_t s, int kind, const float2 *vis_t, const float
 
 } // namespace
 
-/* Smallest E with m <= 2^E (0 for m == 0 or not finite), clamped to
- * [-60, 60]. */
-__device__ __forceinline__ int set_exp(float m)
-{
-    if (!(m > 0.f) || !(m <= 3.0e38f)) return 0;
-    int e;
-    const float f = frexpf(m, &e); /* m = f 2^e, f in [0.5, 1) */
-    if (f == 0.5f) --e;
-    return e < -60 ? -60 : e > 60 ? 60 : e;
-}
-
 /* Normalises a data plane set in place: plane i (compact channel index)
- * times (-1)^i 2^(7 - E), E = set_exp(max |x|) from slot[0], so every
+ * times (-1)^i 2^(7 - E), E = amb_set_exp(max |x|) from slot[0], so every
  * stored value lies in [-2^7, 2^7]; slot[1] gets E.  Both factors are
  * exact and undone exactly by whoever reads the set.  The alternating sign
  * makes the products of a lag cell's triplets (p1, p1 + d2, p1 + d3) carry
@@ -256,7 +245,7 @@ __device__ __forceinline__ int set_exp(float m)
  * its operands cancels in the sum over p1 instead of adding up. */
 __global__ void k_normalise_set(float *__restrict__ D, long pitch4, unsigned *slot)
 {
-    const int e = set_exp(__uint_as_float(slot[0]));
+    const int e = amb_set_exp(__uint_as_float(slot[0]));
     const float f = ldexpf(1.f, AMB_SET_EXP - e);
     const float g = (blockIdx.y & 1) ? -f : f;
     float4 *p = (float4 *)D + (size_t)blockIdx.y * pitch4;
@@ -283,7 +272,8 @@ __global__ void k_zero(float *p, size_t n)
 
 hipError_t amb_launch_ring_c2r(hipStream_t s, int kind, const float2 *vis_t, const float *samp_t,
                                const int *d_planes, int npl, int N, int Nh, const FftPlan &pl,
-                               const float2 *d_tw, RingGeom g, float2 *G, float *out, long pitch, unsigned *slot)
+                               const float2 *d_tw, RingGeom g, float2 *G, float *out, long pitch, unsigned *slot,
+                               bool raw)
 {
     const int n = pl.n;
     if (npl <= 0) return hipSuccess;
@@ -299,7 +289,7 @@ hipError_t amb_launch_ring_c2r(hipStream_t s, int kind, const float2 *vis_t, con
         return hipGetLastError();
     }
     hipError_t e = ring_passes<float>(s, kind, vis_t, samp_t, d_planes, npl, N, Nh, pl, d_tw, g, G, out, pitch, slot);
-    if (e != hipSuccess || !slot) return e;
+    if (e != hipSuccess || !slot || raw) return e;
     const long p4 = pitch / 4;
     const int bx = (int)std::min<long>((p4 + 255) / 256, 64);
     hipLaunchKernelGGL(k_normalise_set, dim3((unsigned)bx, (unsigned)npl), dim3(256), 0, s, out, p4, slot);

@@ -32,15 +32,26 @@ struct RingGeom {
  * stored planes carry 2^(3 AMB_SET_EXP - E1 - E2 - E3), removed exactly. */
 #define AMB_SET_EXP 7
 #define AMB_SET_SLOT_BYTES 256
+/* Smallest E with m <= 2^E (0 for m == 0 or not finite), clamped to
+ * [-60, 60]: the exponent of a set whose max |D| is m. */
+__device__ __forceinline__ int amb_set_exp(float m)
+{
+    if (!(m > 0.f) || !(m <= 3.0e38f)) return 0;
+    int e;
+    const float f = frexpf(m, &e); /* m = f 2^e, f in [0.5, 1) */
+    if (f == 0.5f) --e;
+    return e < -60 ? -60 : e > 60 ? 60 : e;
+}
 /* kind 0: data (complex vis), kind 1: sampling (0/1 real).
  * G scratch: [npl][jcnt][n] complex64.  out: npl planes of n*n floats at
  * `pitch` floats apart; the pitch - n*n tail of each plane is zeroed.
- * slot (data sets; null: stored as computed): the set's [max, E] words,
- * the set normalised in place. */
+ * slot (data sets; null: stored as computed): the set's max word (and E
+ * after it), the set normalised in place unless raw (then the max word
+ * only, the planes as computed). */
 hipError_t amb_launch_ring_c2r(hipStream_t s, int kind, const float2 *vis_t, const float *samp_t,
                                const int *d_planes, int npl, int N, int Nh, const FftPlan &pl,
                                const float2 *d_tw, RingGeom g, float2 *G, float *out, long pitch,
-                               unsigned *slot);
+                               unsigned *slot, bool raw = false);
 void amb_make_plan(int n, FftPlan *pl);
 /* FP64 c2r of npl sampling planes' ring (d_planes: their channel ids on
  * the device) into out[npl][pitch] (tails zeroed); G: [npl][jcnt][n]

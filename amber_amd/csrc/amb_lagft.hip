@@ -171,12 +171,13 @@ __device__ __forceinline__ void load_unit(const char *p, double2 *v)
 /* ---------------------------------------------------------------- spectra */
 /* grid: multiple of 8 blocks of nteam x 64 threads, each team ppt
    consecutive pixel pairs; dynamic LDS: [tw L if TWL] + nteam * 2 * L
-   complex.  SIGNED: plane i of a normalised data set carries (-1)^i,
-   removed here.  Channel ch = act0 + p sits at position p of the sequence.
+   complex.  SCALED: a data set as computed (kind 11), read times 2^(7 - E),
+   E = amb_set_exp of its max word, which goes with E after the spectra.
+   Channel ch = act0 + p sits at position p of the sequence.
    The next pair's plane values are loaded into registers before the
    current pair's transform, so their latency hides behind it; a
    workgroup's pairs cover whole 128-byte lines of every plane. */
-template <typename R, bool SIGNED, bool TWL>
+template <typename R, bool SCALED, bool TWL>
 __global__ __launch_bounds__(256) void k_chan_spectra(const R *__restrict__ D, long pitch, const int *__restrict__ inv,
                                                       int nc, int act0, long P, FftPlan pl,
                                                       const typename LagT<R>::C *__restrict__ twg, int lhp, int ppt,
@@ -193,7 +194,12 @@ __global__ __launch_bounds__(256) void k_chan_spectra(const R *__restrict__ D, l
     const C *tw = TWL ? lsm : twg;
     if (TWL)
         for (int i = threadIdx.x; i < L; i += blockDim.x) lsm[i] = twg[i];
-    if (slot_out && blockIdx.x == 0 && threadIdx.x < 2) slot_out[threadIdx.x] = slot_in[threadIdx.x];
+    const int E = SCALED ? amb_set_exp(__uint_as_float(slot_in[0])) : 0;
+    const R sc = SCALED ? (R)ldexpf(1.f, AMB_SET_EXP - E) : (R)1;
+    if (SCALED && blockIdx.x == 0 && threadIdx.x == 0) {
+        slot_out[0] = slot_in[0];
+        slot_out[1] = (unsigned)E;
+    }
     const long npair = (P + 1) / 2;
     const long pair0 = (xcd_item(blockIdx.x, gridDim.x) * nteam + team) * ppt;
     const int pmax = min(L, nc - act0); /* positions that can hold a channel */
@@ -217,7 +223,7 @@ __global__ __launch_bounds__(256) void k_chan_spectra(const R *__restrict__ D, l
                     const R *d = D + (size_t)i * pitch + x0;
                     v.x = d[0];
                     v.y = d[1];
-                    if (SIGNED && (i & 1)) { v.x = -v.x; v.y = -v.y; }
+                    if (SCALED) { v.x *= sc; v.y *= sc; } /* a power of two: exact */
                 }
             }
             if constexpr (PF) pf[u] = v;
@@ -269,7 +275,9 @@ __global__ __launch_bounds__(256) void k_chan_spectra(const R *__restrict__ D, l
  * per 64-thread team: the holders of a round's pairs write their values at
  * the digit-reversed positions pos[i] of a zeroed team buffer, and the
  * in-place transform leaves the spectrum in natural order.  Dynamic LDS:
- * tw L + 4 L complex + nact positions. */
+ * tw L + 4 L complex + nact positions.  The source is a data set as
+ * computed (kind 11), read times 2^(7 - E), E = amb_set_exp of its max
+ * word; the max word and E follow the spectra. */
 constexpr int SPB_PLANES = 768;
 __global__ __launch_bounds__(256, 2) void k_chan_spectra_blk(const float *__restrict__ D, long pitch,
                                                              const int *__restrict__ posg, int nact, long P,
@@ -286,7 +294,12 @@ __global__ __launch_bounds__(256, 2) void k_chan_spectra_blk(const float *__rest
     const int tid = threadIdx.x, team = tid >> 6, t = tid & 63;
     for (int i = tid; i < L; i += 256) tw[i] = twg[i];
     for (int i = tid; i < nact; i += 256) pos[i] = posg[i];
-    if (slot_out && blockIdx.x == 0 && tid < 2) slot_out[tid] = slot_in[tid];
+    const int E = amb_set_exp(__uint_as_float(slot_in[0]));
+    const float sc = ldexpf(1.f, AMB_SET_EXP - E);
+    if (blockIdx.x == 0 && tid == 0) {
+        slot_out[0] = slot_in[0];
+        slot_out[1] = (unsigned)E;
+    }
     const long x0 = 32 * xcd_item(blockIdx.x, gridDim.x);
     if (x0 >= P) return; /* whole workgroups: no barrier is skipped by part of one */
     const int j = tid & 15, i0 = tid >> 4;
@@ -298,7 +311,8 @@ __global__ __launch_bounds__(256, 2) void k_chan_spectra_blk(const float *__rest
         float2 w = make_float2(0.f, 0.f);
         if (i < nact) {
             w = *(const float2 *)(D + (size_t)i * pitch + x0 + 2 * j);
-            if (i & 1) { w.x = -w.x; w.y = -w.y; } /* the set's (-1)^i */
+            w.x *= sc; /* a power of two: exact */
+            w.y *= sc;
         }
         v[m] = w;
     }
@@ -893,9 +907,9 @@ size_t amb::lag_spectra_bytes(const Cube &cb, long ng, bool fp64)
     return (fp64 ? sizeof(double2) : sizeof(float2)) * (size_t)ng * ng * cb.lag_lhp + AMB_SET_SLOT_BYTES;
 }
 
-/* kind 9: FP32 spectra of the normalised data set op.src (its sign undone,
-   its [max, E] words copied after the spectra); kind 10: FP64 spectra of
-   the FP64 sampling images op.src */
+/* kind 9: FP32 spectra of the data set op.src (kind 11) times 2^(7 - E),
+   its [max, E] words after the spectra; kind 10: FP64 spectra of the FP64
+   sampling images op.src */
 hipError_t amb::lag_launch_spectra(amb_ctx *c, hipStream_t s, const BuildOp &op)
 {
     const Cube &cb = c->cubes[op.cube];
