@@ -237,7 +237,7 @@ void par_copy(const std::vector<std::tuple<void *, const void *, size_t>> &r)
 {
     size_t tot = 0;
     for (const auto &x : r) tot += std::get<2>(x);
-    const int nth = tot < ((size_t)4 << 20) ? 1 : 8;
+    const int nth = tot < ((size_t)4 << 20) ? 1 : (int)std::min<size_t>(16, std::max<size_t>(2, tot >> 22));
     auto part = [&](int t) {
         for (const auto &x : r) {
             const size_t b = std::get<2>(x), lo = b * t / nth, hi = b * (t + 1) / nth;

@@ -53,7 +53,9 @@ constexpr int SEG = 2048;           /* pixels per tile work item */
 constexpr int MAXL = 2048;          /* longest channel transform: 2 L FP64 complex in 64 KiB of LDS */
 constexpr size_t ALIGN = 256;
 constexpr size_t PART_CAP = (size_t)8 << 30; /* partial sums per wave */
-constexpr size_t OUT_CAP = (size_t)1 << 30;  /* result bytes per wave */
+/* result bytes per wave: a wave's copies to host memory (PCIe, then into
+   the caller's arrays) must fit behind the next wave's device work */
+constexpr size_t OUT_CAP = (size_t)384 << 20;
 constexpr int MAX_JOBS = 512;
 
 size_t al(size_t x) { return (x + ALIGN - 1) / ALIGN * ALIGN; }
@@ -1314,11 +1316,63 @@ int amb::lagft_run(amb_ctx *c, const amb_tri_req *req, int n, double t0, double 
         return AMB_OK;
     };
 
+    /* Wave cuts: each wave's results go to host memory (PCIe, then into the
+       caller's arrays) while the next wave computes, and the last wave's
+       copies end the call.  From the end backwards: the last wave is the
+       last triangle, and each earlier wave takes as many triangles as the
+       following wave's estimated device time covers with its copies
+       (device time ~ 14 flop per (k2, k3, pixel) at 60 TFLOP/s, halved for
+       pair tiles, FP64 normalisation 8x; copies ~ 15 GB/s). */
+    std::vector<char> cut(n + 1, 0);
+    if (n >= 2) {
+        std::vector<double> tdev(n), tcp(n);
+        for (int k = 0; k < n; ++k) {
+            const amb_tri_req &q = req[order[k]];
+            auto it = c->cubes.find(q.cube_id);
+            const int nr = (int)c->ulow.size();
+            if (it == c->cubes.end() || q.r1 < 0 || q.r1 >= nr || !it->second.chans || lag_setup(c, it->second)) {
+                tdev[k] = tcp[k] = 0;
+                continue;
+            }
+            const Cube &cb = it->second;
+            const double ng = (double)amb_ng(c->uhigh[q.r1], cb.Du), L = cb.lag_L;
+            const double cells = (double)cb.h_lagcnt.size();
+            tdev[k] = 14.0 * L * (L / 2 + 1) * ng * ng * (q.r1 == q.r2 ? 0.55 : 1.0) * (q.norm ? 9.0 : 1.0) / 60e12;
+            const double bytes = cells * (4.0 + (q.cnt ? 4.0 : 0.0) + (q.norm ? 4.0 : 0.0) +
+                                          (bl64 && bl64[order[k]] ? 8.0 : 0.0) + (norm64 && norm64[order[k]] ? 8.0 : 0.0));
+            tcp[k] = bytes / 15e9;
+        }
+        /* each wave at least 1.5x the one after it (where copies outweigh
+           device time more waves would not hide them) */
+        int end = n - 1; /* first triangle of the wave sized last; the last wave is [n-1] */
+        int after = 1;   /* its size */
+        cut[n - 1] = 1;
+        double cover = tdev[n - 1];
+        while (end > 0) {
+            int start = end - 1;
+            double cp = tcp[start], dv = tdev[start];
+            const int minw = (3 * after + 1) / 2;
+            while (start > 0 && (cp + tcp[start - 1] <= cover || end - start < minw)) {
+                --start;
+                cp += tcp[start];
+                dv += tdev[start];
+            }
+            cut[start] = 1;
+            cover = dv;
+            after = end - start;
+            end = start;
+        }
+    }
+
     auto run_all = [&]() -> int {
         for (int k = 0; k < n; ++k) {
             const int qi = order[k];
             const amb_tri_req &q = req[qi];
             if (!q.bl) return fail(c, AMB_EARG, "request %d has no bl buffer", qi);
+            if (cut[k] && !w.empty()) {
+                int rc = flush();
+                if (rc) return rc;
+            }
             Cube *cb;
             int rc = find_cube(c, q.cube_id, &cb);
             if (rc) { unpin_all(c); return rc; }

@@ -198,6 +198,9 @@ def main():
     nmax = max(len(sl) for sl in slices)
     out = (np.zeros((nmax, W, W), np.float32), np.zeros((nmax, W, W), np.int32),
            np.zeros((nmax, W, W), np.float32) if args.estimator == "v2" else None)
+    for o in out:  # the caller's result buffers, reused by every step: their pages are mapped up front
+        if o is not None:
+            o.fill(0)
     log("%d triangles in %d slices of %d-%d triangles (cost spread %.3f-%.3f of a pass)"
         % (len(tris), len(slices), min(len(x) for x in slices), nmax,
            min(costs[x].sum() for x in slices) / costs.sum(), max(costs[x].sum() for x in slices) / costs.sum()))
