@@ -181,6 +181,9 @@ struct amb_ctx {
        k3 = L/2 column that no listed tile holds], tile count, bytes, list
        length (-1: the list was too long, every k2 is summed apart) */
     std::map<int, std::tuple<void *, int, size_t, int>> lag_pairs_tab;
+    /* r2 == r3 tile tables by transform length: [tiles (a, b)] [nt2 x nt3
+       tile index], tile count, bytes */
+    std::map<int, std::tuple<void *, int, size_t>> lag_r23_tab;
     /* pinned host staging, two buffers: a grouped wave's results are copied
        out of one while the next wave's land in the other; the lag-space
        post-processing stages alternate triangles in them the same way */

@@ -107,7 +107,9 @@ struct LagJob {
                                    launch may cover a sub-range of a triangle's items:
                                    each item owns its partial block) */
     int lhp, L, nseg, seg;      /* tile segments and their pixels */
-    int mode;                   /* 0: (k2, k3) tiles; 1: r1 == r2, (i, j) tiles, i <= j */
+    int mode;                   /* 0: (k2, k3) tiles; 1: r1 == r2, (i, j) tiles, i <= j;
+                                   2: r2 == r3, listed (k2, k3) tiles (tiles below the
+                                   diagonal of [0, L/2)^2 dropped: S is symmetric there) */
     int ntile;                  /* tiles per segment */
     int nyq_sep;                /* the k3 = L/2 column is summed by k_lagft_nyq */
 };
@@ -357,7 +359,10 @@ __global__ __launch_bounds__(256, 2) void k_chan_spectra_blk(const float *__rest
  *   mode 1 (r1 == r2, F = G): p = i = -k2, q = j = k2 + k3: A = conj F(i),
  *           B = conj F(j), C = H(i + j), symmetric in (i, j), so only tiles
  *           I <= J of the band (i + j) mod L in [0, L/2) are summed (the
- *           job's tile list).
+ *           job's tile list);
+ *   mode 2 (r2 == r3, G = H): as mode 0 over the job's tile list, which
+ *           leaves out the tiles of [0, L/2)^2 below its diagonal (S is
+ *           symmetric in (k2, k3) there).
  * FOLD (FP32): in mode 0 with TB | L/2 the last k3 tile also sums the k3 =
  * L/2 column, S(k2, L/2) = conj F(k2 + L/2) G(k2) H(L/2): every conj F it
  * needs is staged (at k2 - a + TB), and H(L/2) takes one more slot.  When
@@ -398,7 +403,7 @@ __global__ __launch_bounds__(256, LagT<R>::WPS) void k_lagft_tiles(const LagJob<
     const int tid = threadIdx.x;
     const int half = L / 2;
     const bool nyq_col = half % TB != 0;
-    const bool nyqx = FOLD && mode == 0 && !nyq_col && b + TB == half;
+    const bool nyqx = FOLD && mode != 1 && !nyq_col && b + TB == half;
 
     /* the values this thread stages per pixel: the spectrum, its index
        (stored half: k <= L/2, else the conjugate at L - k) and whether the
@@ -424,7 +429,7 @@ __global__ __launch_bounds__(256, LagT<R>::WPS) void k_lagft_tiles(const LagJob<
         int r, base;
         if (sl < TA) {
             const int p = a + sl;
-            if (p < L) at(u, mode == 0 ? jobs[jb].G : jobs[jb].F, p, mode == 1);
+            if (p < L) at(u, mode == 1 ? jobs[jb].F : jobs[jb].G, p, mode == 1);
             r = sl;
             base = 0;
         } else if (sl < TA + TB) {
@@ -651,9 +656,11 @@ __global__ __launch_bounds__(256) void k_lagft_nyq(const LagJob<R> *__restrict__
 /* St[k3][k2] = sum over segments of the tile partials, FP64 in segment
    order, for k3 < L/2 (and k3 = L/2 when it is a column of the last,
    partial tile); 32 x 32 (k2, k3) blocks transposed through LDS.
-   M1 (mode 1): the value sits at (i, j) = (-k2, k2 + k3) mod L, in listed
-   tile tmap[I][J] of (i, j) or, when I > J, of (j, i) (symmetric). */
-template <typename R, bool M1>
+   MODE 1: the value sits at (i, j) = (-k2, k2 + k3) mod L, in listed
+   tile tmap[I][J] of (i, j) or, when I > J, of (j, i) (symmetric).
+   MODE 2: listed tile tmap[a][b] of (k2, k3), or of (k3, k2) for a tile of
+   [0, L/2)^2 below its diagonal (TA = TB). */
+template <typename R, int MODE>
 __global__ __launch_bounds__(256) void k_lagft_reduce(const typename LagT<R>::C *__restrict__ part, int nseg, int ntile,
                                                       int L, const int *__restrict__ tmap, double2 *__restrict__ St)
 {
@@ -668,14 +675,24 @@ __global__ __launch_bounds__(256) void k_lagft_reduce(const typename LagT<R>::C 
     /* k3 = L/2 is a tile column (mode 0, TB does not divide L/2), or in mode
        1 a diagonal of the listed tiles (k_lagft_reduce_nyq then writes the
        k2 that no listed tile holds) */
-    const bool col = M1 || half % TB != 0;
+    const bool col = MODE == 1 || half % TB != 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int k2 = X + ty + 8 * q;
         double sx = 0.0, sy = 0.0;
         if (k2 < L && (k3 < half || (col && k3 == half))) {
             long off = 0;
-            if (M1) {
+            if (MODE == 2) {
+                int p2 = k2, p3 = k3, a = k2 / TA, b = k3 / TB;
+                if (a > b && (a + 1) * TA <= half) {
+                    p2 = k3;
+                    p3 = k2;
+                    a = p2 / TA;
+                    b = p3 / TB;
+                }
+                const int tl = tmap[a * nt3 + b];
+                off = tl < 0 ? -1 : (long)tl * (TA * TB) + (p2 - a * TA) * TB + (p3 - b * TB);
+            } else if (MODE == 1) {
                 int i = k2 == 0 ? 0 : L - k2, j = k2 + k3;
                 if (j >= L) j -= L;
                 int I = i / TA, J = j / TB;
@@ -1046,12 +1063,21 @@ hipError_t launch_products(hipStream_t s, const LagJob<R> *d, int n, long items,
     return hipGetLastError();
 }
 
-/* tiling of a triangle: (k2, k3) tiles, or for r1 == r2 FP32 jobs the
-   symmetric (i, j) tiles where they are fewer (tables built once per L);
-   whether k_lagft_nyq sums the k3 = L/2 column (pair tiles, and FP64 where
-   the column is not a tile column) */
-int choose_tiles(amb_ctx *c, int L, bool fp64, bool r12, int *mode, int *ntile, int *nyq_sep, const int2 **tlist,
-                 const int **tmap, const int **nyql, int *nnyq)
+/* share of the half spectrum's (k2, k3) whose sums are distinct under the
+   triangle's symmetries */
+double lag_distinct_r(int r1, int r2, int r3)
+{
+    return r1 == r2 && r2 == r3 ? 1.0 / 3.0 : (r1 == r2 || r2 == r3) ? 0.5 : 1.0;
+}
+double lag_distinct(const amb_tri_req &q) { return lag_distinct_r(q.r1, q.r2, q.r3); }
+
+/* tiling of a triangle: (k2, k3) tiles, or for FP32 jobs the symmetric
+   (i, j) tiles of r1 == r2 or the listed (k2, k3) tiles of r2 == r3 where
+   they are fewer (tables built once per L); whether k_lagft_nyq sums the
+   k3 = L/2 column (pair tiles, and FP64 where the column is not a tile
+   column) */
+int choose_tiles(amb_ctx *c, int L, bool fp64, bool r12, bool r23, int *mode, int *ntile, int *nyq_sep,
+                 const int2 **tlist, const int **tmap, const int **nyql, int *nnyq)
 {
     *mode = 0;
     *ntile = fp64 ? tiles2<double>(L) * tiles3<double>(L) : tiles2<float>(L) * tiles3<float>(L);
@@ -1096,6 +1122,35 @@ int choose_tiles(amb_ctx *c, int L, bool fp64, bool r12, int *mode, int *ntile, 
                 *nyql = *tmap + (size_t)nt * nt;
                 *nnyq = nm;
             }
+        }
+    }
+    if (!fp64 && !r12 && r23 && !c->no_sym) {
+        /* (k2, k3) tiles without those of [0, L/2)^2 below its diagonal */
+        auto it = c->lag_r23_tab.find(L);
+        if (it == c->lag_r23_tab.end()) {
+            const int n2 = tiles2<float>(L), n3 = tiles3<float>(L), T = tile_a<float>();
+            std::vector<int2> tl;
+            std::vector<int> tm((size_t)n2 * n3, -1);
+            for (int a = 0; a < n2; ++a)
+                for (int b = 0; b < n3; ++b)
+                    if (!(a > b && (a + 1) * T <= L / 2)) {
+                        tm[(size_t)a * n3 + b] = (int)tl.size();
+                        tl.push_back(make_int2(a, b));
+                    }
+            void *d = nullptr;
+            const size_t tb = sizeof(int2) * tl.size(), mb = sizeof(int) * tm.size();
+            const int rc = dev_alloc(c, &d, tb + mb, "lag-Fourier tile tables");
+            if (rc) return rc;
+            HIPCHK(c, hipMemcpy(d, tl.data(), tb, hipMemcpyHostToDevice));
+            HIPCHK(c, hipMemcpy((char *)d + tb, tm.data(), mb, hipMemcpyHostToDevice));
+            it = c->lag_r23_tab.emplace(L, std::make_tuple(d, (int)tl.size(), tb + mb)).first;
+        }
+        const int n1 = std::get<1>(it->second);
+        if (n1 < *ntile) {
+            *mode = 2;
+            *ntile = n1;
+            *tlist = (const int2 *)std::get<0>(it->second);
+            *tmap = (const int *)((const char *)std::get<0>(it->second) + sizeof(int2) * n1);
         }
     }
     /* pair tiles whose listed tiles hold the whole k3 = L/2 column need no
@@ -1226,7 +1281,7 @@ int amb::lagft_run(amb_ctx *c, const amb_tri_req *req, int n, double t0, double 
             const dim3 gr((unsigned)((L + 31) / 32), (unsigned)((L / 2 + 1 + 31) / 32));
             if (j.norm) {
                 const LagJob<double> &g = d64[slot[i]];
-                hipLaunchKernelGGL((k_lagft_reduce<double, false>), gr, dim3(256), 0, c->stream, g.part, j.nseg,
+                hipLaunchKernelGGL((k_lagft_reduce<double, 0>), gr, dim3(256), 0, c->stream, g.part, j.nseg,
                                    j.ntile, L, nullptr, St);
                 if (j.nyq_sep)
                     hipLaunchKernelGGL(k_lagft_reduce_nyq<double>, dim3((unsigned)((L + 255) / 256)), dim3(256), 0,
@@ -1234,14 +1289,18 @@ int amb::lagft_run(amb_ctx *c, const amb_tri_req *req, int n, double t0, double 
             } else {
                 const LagJob<float> &g = d32[slot[i]];
                 if (j.mode == 1)
-                    hipLaunchKernelGGL((k_lagft_reduce<float, true>), gr, dim3(256), 0, c->stream, g.part, j.nseg,
+                    hipLaunchKernelGGL((k_lagft_reduce<float, 1>), gr, dim3(256), 0, c->stream, g.part, j.nseg,
+                                       j.ntile, L, j.tmap, St);
+                else if (j.mode == 2)
+                    hipLaunchKernelGGL((k_lagft_reduce<float, 2>), gr, dim3(256), 0, c->stream, g.part, j.nseg,
                                        j.ntile, L, j.tmap, St);
                 else
-                    hipLaunchKernelGGL((k_lagft_reduce<float, false>), gr, dim3(256), 0, c->stream, g.part, j.nseg,
+                    hipLaunchKernelGGL((k_lagft_reduce<float, 0>), gr, dim3(256), 0, c->stream, g.part, j.nseg,
                                        j.ntile, L, nullptr, St);
-                /* the column rode in the last tile (mode 0, TB | L/2), was a
-                   tile column (mode 0 otherwise), or had its own kernel */
-                const bool folded = j.mode == 0 && (L / 2) % tile_b<float>() == 0;
+                /* the column rode in the last tile (modes 0 and 2, TB | L/2),
+                   was a tile column (TB does not divide L/2), or had its own
+                   kernel */
+                const bool folded = j.mode != 1 && (L / 2) % tile_b<float>() == 0;
                 if (j.nyq_sep || folded)
                     hipLaunchKernelGGL(k_lagft_reduce_nyq<float>, dim3((unsigned)((L + 255) / 256)), dim3(256), 0,
                                        c->stream, g.nyq, j.nseg, L, j.nyql, j.nnyq, St);
@@ -1302,8 +1361,9 @@ int amb::lagft_run(amb_ctx *c, const amb_tri_req *req, int n, double t0, double 
             c->st.flop_contract += 2.0 * (double)cb.lag_pairs * (double)j.P;
             c->st.flop_contract_distinct += 2.0 * (double)cb.lag_pairs * (double)j.P;
             /* 14 flops per distinct (k2, k3, pixel) of the half spectrum
-               k3 <= L/2: half of them where r1 == r2 pairs equal sums */
-            c->st.flop_lagft += 14.0 * (double)L * (double)(L / 2 + 1) * (double)j.P * (j.mode == 1 ? 0.5 : 1.0);
+               k3 <= L/2: half of them where r1 == r2 or r2 == r3 pair equal
+               sums, a third where r1 == r2 == r3 */
+            c->st.flop_lagft += 14.0 * (double)L * (double)(L / 2 + 1) * (double)j.P * lag_distinct(req[j.q]);
             /* issued: whole tiles, plus the k3 = L/2 column (or its k2 left
                to k_lagft_nyq by pair tiles) */
             c->st.flop_contract_exec +=
@@ -1403,8 +1463,8 @@ int amb::lagft_run(amb_ctx *c, const amb_tri_req *req, int n, double t0, double 
                 j.ng = ng;
                 j.P = ng * ng;
                 j.nseg = (int)((j.P + SEG - 1) / SEG);
-                if ((rc = choose_tiles(c, L, nm, q.r1 == q.r2, &j.mode, &j.ntile, &j.nyq_sep, &j.tlist, &j.tmap,
-                                       &j.nyql, &j.nnyq))) {
+                if ((rc = choose_tiles(c, L, nm, q.r1 == q.r2, q.r2 == q.r3, &j.mode, &j.ntile, &j.nyq_sep, &j.tlist,
+                                       &j.tmap, &j.nyql, &j.nnyq))) {
                     unpin_all(c);
                     return rc;
                 }
@@ -1467,8 +1527,8 @@ int amb::lag_prepare(amb_ctx *c, int cube, int r1, int r2, int r3, LagTri *t)
     t->L = L;
     t->nseg = (int)((t->P + SEG - 1) / SEG);
     t->cells = cb->h_lagcnt.size();
-    if ((rc = choose_tiles(c, L, false, r1 == r2, &t->mode, &t->ntile, &t->nyq_sep, &t->tlist, &t->tmap, &t->nyql,
-                           &t->nnyq)))
+    if ((rc = choose_tiles(c, L, false, r1 == r2, r2 == r3, &t->mode, &t->ntile, &t->nyq_sep, &t->tlist, &t->tmap,
+                           &t->nyql, &t->nnyq)))
         return rc;
     const PlaneSrc src{&c->planes};
     const int rr[3] = {r1, r2, r3};
@@ -1543,10 +1603,12 @@ hipError_t amb::lag_launch_finish(amb_ctx *c, hipStream_t s, const LagTri &t, co
     double2 *st = (double2 *)St;
     const dim3 gr((unsigned)((L + 31) / 32), (unsigned)((L / 2 + 1 + 31) / 32));
     if (t.mode == 1)
-        hipLaunchKernelGGL((k_lagft_reduce<float, true>), gr, dim3(256), 0, s, pt, t.nseg, t.ntile, L, t.tmap, st);
+        hipLaunchKernelGGL((k_lagft_reduce<float, 1>), gr, dim3(256), 0, s, pt, t.nseg, t.ntile, L, t.tmap, st);
+    else if (t.mode == 2)
+        hipLaunchKernelGGL((k_lagft_reduce<float, 2>), gr, dim3(256), 0, s, pt, t.nseg, t.ntile, L, t.tmap, st);
     else
-        hipLaunchKernelGGL((k_lagft_reduce<float, false>), gr, dim3(256), 0, s, pt, t.nseg, t.ntile, L, nullptr, st);
-    const bool folded = t.mode == 0 && (L / 2) % tile_b<float>() == 0;
+        hipLaunchKernelGGL((k_lagft_reduce<float, 0>), gr, dim3(256), 0, s, pt, t.nseg, t.ntile, L, nullptr, st);
+    const bool folded = t.mode != 1 && (L / 2) % tile_b<float>() == 0;
     if (t.nyq_sep || folded)
         hipLaunchKernelGGL(k_lagft_reduce_nyq<float>, dim3((unsigned)((L + 255) / 256)), dim3(256), 0, s, nyq, t.nseg, L,
                            t.nyql, t.nnyq, st);
@@ -1571,7 +1633,7 @@ void amb::lag_account(amb_ctx *c, const LagTri &t, int32_t *cnt)
     c->st.evals += cb.lag_evals;
     c->st.flop_contract += 2.0 * (double)cb.lag_pairs * (double)t.P;
     c->st.flop_contract_distinct += 2.0 * (double)cb.lag_pairs * (double)t.P;
-    c->st.flop_lagft += 14.0 * (double)L * (double)(L / 2 + 1) * (double)t.P * (t.mode == 1 ? 0.5 : 1.0);
+    c->st.flop_lagft += 14.0 * (double)L * (double)(L / 2 + 1) * (double)t.P * lag_distinct_r(t.r1, t.r2, t.r3);
     c->st.flop_contract_exec +=
         14.0 * ((double)t.ntile * tile_a<float>() * tile_b<float>() + (t.nyql ? t.nnyq : L)) * (double)t.P;
 }

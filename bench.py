@@ -84,8 +84,11 @@ def tri_costs(A, tris, hi, precision="bf16x3"):
     its tiles."""
     ng = np.array([A.ng(hi[t[0]]) for t in tris], np.float64)
     sym = np.array([(t[0] == t[1]) or (t[1] == t[2]) for t in tris])
-    if precision == "lagft":  # r1 == r2: the symmetric (i, j) tiles, about 0.63 of the (k2, k3) ones
-        return ng * ng * np.where(np.array([t[0] == t[1] for t in tris]), 0.65, 1.0)
+    if precision == "lagft":  # r1 == r2: the symmetric (i, j) tiles, about 0.63 of the (k2, k3) ones;
+        # r2 == r3: the (k2, k3) tiles without those below the diagonal of [0, L/2)^2, about 0.79
+        r12 = np.array([t[0] == t[1] for t in tris])
+        r23 = np.array([t[1] == t[2] for t in tris])
+        return ng * ng * np.where(r12, 0.65, np.where(r23, 0.8, 1.0))
     return ng * ng * np.where(sym, 0.5, 1.0)
 
 
@@ -264,8 +267,9 @@ def main():
             "frac": round(ach / peak, 4),
             "traffic": traffic_per_launch(args, st, nsteps),
             "kernel": "k_lagft_tiles" if lagft else "k_contract<%s>" % args.precision,
-            "work": ("14 flop per (k2, k3, pixel) of the half spectrum k3 <= L/2 (one complex triple product "
-                     "conj(F) G H), L = channel transform length" if lagft else
+            "work": ("14 flop per distinct (k2, k3, pixel) of the half spectrum k3 <= L/2 (one complex triple "
+                     "product conj(F) G H), L = channel transform length; half of the half spectrum where r1 == r2 "
+                     "or r2 == r3 makes sums pairwise equal, a third where r1 == r2 == r3" if lagft else
                      "2 flop per (distinct triplet, pixel): 2*V*Ng^2 per triangle, halved (plus the self-mirrored "
                      "triplets) where r1 == r2 or r2 == r3"),
             "mfma_passes": PASSES[args.precision],

@@ -6,8 +6,9 @@ the channel spectra, so the tests pin what the direct sum fixes: every cell
 against the oracle on small cubes (flagged channels, partial lag windows,
 odd grids, r1 == r2 and r2 == r3 triangles), counts bit for bit, cells
 without triplets exactly zero, batches equal to single triangles bit for
-bit, the v2 normalisation exact, the serving queue equal to the batch
-path, and graph capture (which it does not support) refused."""
+bit, the v2 normalisation exact, the symmetric tilings (r1 == r2,
+r2 == r3) equal to the full one, the serving queue equal to the batch path,
+and graph capture (which it does not support) refused."""
 import numpy as np
 import pytest
 
@@ -148,17 +149,23 @@ def test_serving_equals_batch(ctx):
         ctx.set_precision("bf16x3")
 
 
-def test_r1_eq_r2_pair_tiles(ctx):
-    """r1 == r2 triangles at a transform length of 512 (256 channels, all
-    lags) run on the symmetric (i, j) tiles: sampled cells against the FP64
-    oracle, every cell against the (k2, k3) tiling of a context with the
-    symmetries switched off (AMB_NO_SYM), counts equal."""
+@pytest.mark.parametrize("sym", ["r1==r2", "r2==r3"])
+def test_symmetric_tilings(ctx, sym):
+    """Triangles at a transform length of 512 (256 channels, all lags) on
+    the symmetric tilings: r1 == r2 on the (i, j) pair tiles, r2 == r3 on the
+    (k2, k3) tiles without those below the diagonal of [0, L/2)^2.  Sampled
+    cells against the FP64 oracle, every cell against the full (k2, k3)
+    tiling of a context with the symmetries switched off (AMB_NO_SYM),
+    counts equal."""
     import os
     N, nc, nc10 = 64, 256, 255
     lo, hi = A.log_rings(2, 20, 5)
     gv, samp, _ = A.generate(N, nc, nflag=0, seed=12)
     mask = np.ones(nc, np.uint8)
-    tris = [tuple(int(x) for x in t) for t in A.triangles(lo, hi) if t[0] == t[1]]
+    if sym == "r1==r2":
+        tris = [tuple(int(x) for x in t) for t in A.triangles(lo, hi) if t[0] == t[1]]
+    else:
+        tris = [tuple(int(x) for x in t) for t in A.triangles(lo, hi) if t[1] == t[2] and t[0] != t[1]]
     assert tris
     os.environ["AMB_NO_SYM"] = "1"
     try:
@@ -175,7 +182,7 @@ def test_r1_eq_r2_pair_tiles(ctx):
         cube2 = plain.load_cube(N, nc, 1.0, 0, gv, samp)
         plain.set_channels(cube2, None, nc10)
         for t in tris[-3:]:
-            print("case lagft pair tiles triangle %s" % (t,), flush=True)
+            print("case lagft %s tiles triangle %s" % (sym, t), flush=True)
             a = ctx.bispec_ex(cube, *t)
             b = plain.bispec_ex(cube2, *t)
             assert np.array_equal(a["cnt"], b["cnt"])
