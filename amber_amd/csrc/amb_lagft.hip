@@ -279,10 +279,11 @@ __global__ __launch_bounds__(256) void k_chan_spectra(const R *__restrict__ D, l
  * per 64-thread team: the holders of a round's pairs write their values at
  * the digit-reversed positions pos[i] of a zeroed team buffer, and the
  * in-place transform leaves the spectrum in natural order.  Dynamic LDS:
- * tw L + 4 L complex + nact positions.  The source is a data set as
+ * [tw L if TWL] + 4 L complex + nact positions.  The source is a data set as
  * computed (kind 11), read times 2^(7 - E), E = amb_set_exp of its max
  * word; the max word and E follow the spectra. */
 constexpr int SPB_PLANES = 768;
+template <bool TWL>
 __global__ __launch_bounds__(256, 2) void k_chan_spectra_blk(const float *__restrict__ D, long pitch,
                                                              const int *__restrict__ posg, int nact, long P,
                                                              FftPlan pl, const float2 *__restrict__ twg, int lhp,
@@ -293,10 +294,14 @@ __global__ __launch_bounds__(256, 2) void k_chan_spectra_blk(const float *__rest
     constexpr int NQ = SPB_PLANES / 16;
     extern __shared__ __align__(16) char lraw[];
     const int L = pl.n, h = L / 2;
-    float2 *tw = (float2 *)lraw, *buf = tw + L;
+    /* TWL: the twiddles in LDS; otherwise read through the cache, and
+       three workgroups fit a CU instead of two */
+    float2 *twl = (float2 *)lraw, *buf = twl + (TWL ? L : 0);
+    const float2 *tw = TWL ? twl : twg;
     int *pos = (int *)(buf + 4 * L);
     const int tid = threadIdx.x, team = tid >> 6, t = tid & 63;
-    for (int i = tid; i < L; i += 256) tw[i] = twg[i];
+    if (TWL)
+        for (int i = tid; i < L; i += 256) twl[i] = twg[i];
     for (int i = tid; i < nact; i += 256) pos[i] = posg[i];
     const int E = amb_set_exp(__uint_as_float(slot_in[0]));
     const float sc = ldexpf(1.f, AMB_SET_EXP - E);
@@ -948,12 +953,12 @@ hipError_t amb::lag_launch_spectra(amb_ctx *c, hipStream_t s, const BuildOp &op)
         unsigned *slot_out = (unsigned *)((float2 *)op.dst + (size_t)P * cb.lag_lhp);
         bool smooth5 = true;
         for (int s2 = 0; s2 < pl.nst; ++s2) smooth5 = smooth5 && pl.rad[s2] <= 5;
-        const size_t lds_blk = sizeof(float2) * 5 * (size_t)L + sizeof(int) * (size_t)cb.nact();
+        const size_t lds_blk = sizeof(float2) * 4 * (size_t)L + sizeof(int) * (size_t)cb.nact();
         if (cb.nact() <= SPB_PLANES && smooth5 && lds_blk <= 65536) {
             const long nblk = ((P + 31) / 32 + 7) / 8 * 8;
-            hipLaunchKernelGGL(k_chan_spectra_blk, dim3((unsigned)nblk), dim3(256), lds_blk, s, (const float *)op.src,
-                               pitch, (const int *)cb.d_lagpos, cb.nact(), P, pl, itw->second, cb.lag_lhp,
-                               (float2 *)op.dst, slot_in, slot_out);
+            hipLaunchKernelGGL(k_chan_spectra_blk<false>, dim3((unsigned)nblk), dim3(256), lds_blk, s,
+                               (const float *)op.src, pitch, (const int *)cb.d_lagpos, cb.nact(), P, pl, itw->second,
+                               cb.lag_lhp, (float2 *)op.dst, slot_in, slot_out);
             return hipGetLastError();
         }
         const int nteam = spectra_teams(L);
