@@ -62,7 +62,7 @@ TUNE_TABLE = os.path.join(ROOT, "amber_amd", "tune_gfx950.txt")
 # over the step's launches.
 MEASURED_TRAFFIC_STEP = {("cfg3", "bf16x3", "grouped"): 1.909e13,
                          # lag-Fourier form: k_lagft_tiles + k_lagft_nyq, FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00
-                         ("cfg3", "lagft", "grouped"): 2.407e12}
+                         ("cfg3", "lagft", "grouped"): 2.172e12}
 
 
 def traffic_per_launch(args, st, nsteps):
