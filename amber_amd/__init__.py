@@ -112,7 +112,8 @@ class Stats(C.Structure):
                 ("ms_post", C.c_double), ("flop_post", C.c_double), ("cert_margin_max", C.c_double),
                 ("n_uncertified", C.c_int64), ("flop_contract_distinct", C.c_double),
                 ("n_norm_exact", C.c_int64), ("n_evict_planes", C.c_int64), ("n_evict_cubes", C.c_int64),
-                ("n_cube_uploads", C.c_int64), ("n_serve_iters", C.c_int64), ("flop_lagft", C.c_double)]
+                ("n_cube_uploads", C.c_int64), ("n_serve_iters", C.c_int64), ("flop_lagft", C.c_double),
+                ("plane_pool_bytes", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -143,27 +143,67 @@ static bool evict_one(amb_ctx *c)
     return true;
 }
 
+namespace {
+
+constexpr size_t ARENA_CHUNK = (size_t)4 << 30; /* default chunk of the plane arena */
+constexpr size_t ARENA_ALIGN = 4096;
+
+size_t arena_round(size_t b) { return (b + ARENA_ALIGN - 1) / ARENA_ALIGN * ARENA_ALIGN; }
+
+/* device memory the context holds: live allocations other than planes, and
+   every arena chunk */
+uint64_t held_bytes(amb_ctx *c) { return c->used - c->plane_live + c->arena_held; }
+
+/* returns the arena's empty chunks to the device (once the stream has
+   drained: their last readers may still run) */
+bool arena_release_empty(amb_ctx *c)
+{
+    bool any = false;
+    for (size_t i = 0; i < c->arena.size();) {
+        if (c->arena[i].live == 0) {
+            if (!any) hipStreamSynchronize(c->stream);
+            hipFree(c->arena[i].base);
+            c->arena_held -= c->arena[i].size;
+            c->arena.erase(c->arena.begin() + i);
+            any = true;
+        } else {
+            ++i;
+        }
+    }
+    return any;
+}
+
+/* first fit over the chunks' free ranges */
+bool arena_take(amb_ctx *c, size_t bytes, void **p)
+{
+    for (auto &k : c->arena)
+        for (auto it = k.fr.begin(); it != k.fr.end(); ++it)
+            if (it->second >= bytes) {
+                const size_t off = it->first, rest = it->second - bytes;
+                k.fr.erase(it);
+                if (rest) k.fr[off + bytes] = rest;
+                k.live += bytes;
+                *p = k.base + off;
+                return true;
+            }
+    return false;
+}
+
+} // namespace
+
 int dev_alloc(amb_ctx *c, void **p, size_t bytes, const char *what)
 {
     *p = nullptr;
     if (bytes == 0) return AMB_OK;
-    while (c->used + bytes > c->budget)
-        if (!evict_one(c))
-            return fail(c, AMB_EBUDGET, "%s: %zu bytes exceed the HBM budget (%llu used of %llu)", what, bytes,
-                        (unsigned long long)c->used, (unsigned long long)c->budget);
+    while (held_bytes(c) + bytes > c->budget)
+        if (!arena_release_empty(c) && !evict_one(c))
+            return fail(c, AMB_EBUDGET, "%s: %zu bytes exceed the HBM budget (%llu live, %llu held of %llu)", what,
+                        bytes, (unsigned long long)c->used, (unsigned long long)held_bytes(c),
+                        (unsigned long long)c->budget);
     hipError_t e = hipMalloc(p, bytes);
-    if (e != hipSuccess && c->ppool) { /* freed planes the pool still holds go back to the device */
+    while (e != hipSuccess) { /* the device is full outside the budget's view: give back what the arena holds */
         (void)hipGetLastError();
-        hipStreamSynchronize(c->stream);
-        hipMemPoolTrimTo(c->ppool, 0);
-        e = hipMalloc(p, bytes);
-    }
-    while (e != hipSuccess && evict_one(c)) {
-        (void)hipGetLastError();
-        if (c->ppool) {
-            hipStreamSynchronize(c->stream);
-            hipMemPoolTrimTo(c->ppool, 0);
-        }
+        if (!arena_release_empty(c) && !evict_one(c)) break;
         e = hipMalloc(p, bytes);
     }
     if (e != hipSuccess) {
@@ -177,36 +217,66 @@ int dev_alloc(amb_ctx *c, void **p, size_t bytes, const char *what)
 
 int plane_alloc(amb_ctx *c, void **p, size_t bytes)
 {
-    if (!c->ppool) return dev_alloc(c, p, bytes, "filtered planes");
     *p = nullptr;
     if (bytes == 0) return AMB_OK;
+    bytes = arena_round(bytes);
     while (c->used + bytes > c->budget)
         if (!evict_one(c))
             return fail(c, AMB_EBUDGET, "filtered planes: %zu bytes exceed the HBM budget (%llu used of %llu)", bytes,
                         (unsigned long long)c->used, (unsigned long long)c->budget);
-    hipError_t e = hipMallocFromPoolAsync(p, bytes, c->ppool, c->stream);
-    if (e != hipSuccess) { /* the device is full outside the budget's view: wait, trim, retry */
-        (void)hipGetLastError();
-        hipStreamSynchronize(c->stream);
-        hipMemPoolTrimTo(c->ppool, 0);
-        e = hipMallocFromPoolAsync(p, bytes, c->ppool, c->stream);
-    }
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
-        *p = nullptr;
-        return fail(c, AMB_EBUDGET, "filtered planes: pool allocation of %zu bytes failed: %s", bytes,
-                    hipGetErrorString(e));
+    while (!arena_take(c, bytes, p)) {
+        /* a new chunk (the default size, else the request alone) within the
+           budget; else empty chunks go back and planes are evicted */
+        bool grown = false;
+        for (size_t cs : {std::max(ARENA_CHUNK, bytes), bytes}) {
+            if (held_bytes(c) + cs > c->budget) continue;
+            void *q = nullptr;
+            if (hipMalloc(&q, cs) != hipSuccess) {
+                (void)hipGetLastError();
+                continue;
+            }
+            amb_ctx::ArenaChunk k;
+            k.base = (char *)q;
+            k.size = cs;
+            k.fr[0] = cs;
+            c->arena.push_back(std::move(k));
+            c->arena_held += cs;
+            grown = true;
+            break;
+        }
+        if (grown || arena_release_empty(c) || evict_one(c)) continue;
+        return fail(c, AMB_EBUDGET, "filtered planes: %zu bytes fit neither the budget nor the device (%llu held of %llu)",
+                    bytes, (unsigned long long)held_bytes(c), (unsigned long long)c->budget);
     }
     c->used += bytes;
+    c->plane_live += bytes;
     return AMB_OK;
 }
 
 void plane_free(amb_ctx *c, void *p, size_t bytes)
 {
     if (!p) return;
-    if (!c->ppool) return dev_free(c, p, bytes);
-    hipFreeAsync(p, c->stream); /* after every launch already on the stream that reads it */
+    bytes = arena_round(bytes);
+    for (auto &k : c->arena)
+        if ((char *)p >= k.base && (char *)p < k.base + k.size) {
+            auto it = k.fr.emplace((size_t)((char *)p - k.base), bytes).first;
+            auto nx = std::next(it);
+            if (nx != k.fr.end() && it->first + it->second == nx->first) {
+                it->second += nx->second;
+                k.fr.erase(nx);
+            }
+            if (it != k.fr.begin()) {
+                auto pv = std::prev(it);
+                if (pv->first + pv->second == it->first) {
+                    pv->second += it->second;
+                    k.fr.erase(it);
+                }
+            }
+            k.live -= bytes;
+            break;
+        }
     c->used -= bytes;
+    c->plane_live -= bytes;
 }
 
 /* grow-only device scratch */
@@ -257,6 +327,9 @@ void par_copy(const std::vector<std::tuple<void *, const void *, size_t>> &r)
 int grow_host(amb_ctx *c, void **p, size_t *have, size_t need)
 {
     if (*have >= need) return AMB_OK;
+    /* pinning is slow: grow by half again at least, so a run of growing
+       waves re-pins a few times only */
+    need = std::max(need, *have + *have / 2);
     if (*p) hipHostFree(*p);
     *p = nullptr;
     *have = 0;
@@ -1041,20 +1114,6 @@ int amb_create(const amb_opts *o, amb_ctx **out)
     c->margin = (unsigned *)m;
     c->used = 256;
     c->grouped_blocks = amb_contract_grouped_blocks(c->device);
-    {
-        hipMemPoolProps pp{};
-        pp.allocType = hipMemAllocationTypePinned;
-        pp.handleTypes = hipMemHandleTypeNone;
-        pp.location.type = hipMemLocationTypeDevice;
-        pp.location.id = c->device;
-        if (hipMemPoolCreate(&c->ppool, &pp) == hipSuccess) {
-            uint64_t keep = ~(uint64_t)0; /* freed planes stay in the pool for the next ones */
-            hipMemPoolSetAttribute(c->ppool, hipMemPoolAttrReleaseThreshold, &keep);
-        } else {
-            (void)hipGetLastError();
-            c->ppool = nullptr;
-        }
-    }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) c->arch = prop.gcnArchName;
     if (c->arch.empty()) c->arch = "unknown";
@@ -1075,11 +1134,11 @@ void amb_destroy(amb_ctx *c)
     timers_collect(c);
     for (auto &p : c->planes) plane_free(c, p.d, p.bytes);
     hipStreamSynchronize(c->stream);
-    if (c->ppool) hipMemPoolDestroy(c->ppool);
+    for (auto &k : c->arena) hipFree(k.base);
     for (auto &kv : c->cubes) {
         Cube &cb = kv.second;
         hipFree(cb.vis_t); hipFree(cb.samp_t); hipFree(cb.d_act); hipFree(cb.d_inv);
-        hipFree(cb.d_wlo); hipFree(cb.d_whi); hipFree(cb.d_lagcnt);
+        hipFree(cb.d_wlo); hipFree(cb.d_whi); hipFree(cb.d_lagcnt); hipFree(cb.d_lagpos);
     }
     for (auto &kv : c->tw) hipFree(kv.second);
     for (auto &kv : c->tw64) hipFree(kv.second);
@@ -1489,6 +1548,7 @@ int amb_get_stats(amb_ctx *c, amb_stats *st)
     Lock lk(c->mu);
     *st = c->st;
     st->ms_total = st->ms_fft + st->ms_contract + st->ms_reduce + st->ms_norm;
+    st->plane_pool_bytes = (int64_t)c->arena_held;
     return AMB_OK;
 }
 

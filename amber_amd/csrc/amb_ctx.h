@@ -148,10 +148,19 @@ struct amb_ctx {
     bool has_forced = false;
     std::string arch; /* gcnArchName of the device */
     std::vector<void *> *pin_log = nullptr; /* when set, get_planes notes every cache plane it pins */
-    /* stream-ordered pool of the cached planes: a plane is freed on the
-       stream that reads it, so eviction never waits for the device (null:
-       plain allocations) */
-    hipMemPool_t ppool = nullptr;
+    /* arena of the cached planes: device chunks allocated once and kept
+       (until the device or the budget needs them back), planes placed
+       first-fit in their free ranges.  Every reader of a plane runs on the
+       library stream, so a freed range is reused in stream order and
+       eviction never waits for the device.  plane_live: bytes of live
+       planes (counted in used); arena_held: bytes of all chunks. */
+    struct ArenaChunk {
+        char *base = nullptr;
+        size_t size = 0, live = 0;
+        std::map<size_t, size_t> fr; /* offset -> length of free ranges */
+    };
+    std::vector<ArenaChunk> arena;
+    uint64_t arena_held = 0, plane_live = 0;
     int wave_max = 1024;    /* triangles per grouped wave */
     void *na_scratch = nullptr; /* device address of the static scratch (allocation-free entry points) */
     /* scratch (grow-only) */

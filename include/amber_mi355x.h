@@ -532,9 +532,11 @@ typedef struct amb_stats {
     int64_t n_evict_cubes;  /* cubes carried by requests evicted (LRU, idle) for HBM budget */
     int64_t n_cube_uploads; /* cubes carried by requests uploaded by the serving queue */
     int64_t n_serve_iters;  /* contraction iterations the serving queue launched */
-    double flop_lagft;      /* lag-Fourier form: 14 flops per (k2, k3, pixel) of the half
-                               spectrum k3 <= L/2, the algorithmic work of its products
+    double flop_lagft;      /* lag-Fourier form: 14 flops per distinct (k2, k3, pixel) of the
+                               half spectrum k3 <= L/2, the algorithmic work of its products
                                (ms_contract times them) */
+    int64_t plane_pool_bytes; /* device memory the plane pool holds (in use or free for
+                                 reuse), when read */
 } amb_stats;
 int amb_get_stats(amb_ctx *ctx, amb_stats *st);
 int amb_reset_stats(amb_ctx *ctx);
