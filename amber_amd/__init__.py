@@ -37,7 +37,7 @@ EXPORTS = [
     "amb_serve_stop",
     "amb_cyl_bispec", "amb_maps_cldnu", "amb_write_cldnu", "amb_read_params", "amb_write_bsw", "amb_read_bsw",
     "amb_read_bsout", "amb_crc32", "amb_read_vcg_inputs", "amb_grid_setup", "amb_grid_cells", "amb_grid",
-    "amb_grid_cube", "amb_set_cert_strict", "amb_closed_rings_noalloc", "amb_autotune", "amb_tune_save",
+    "amb_grid_cube", "amb_set_cert_strict", "amb_set_grid", "amb_compute_grid", "amb_ring_reach", "amb_closed_rings_noalloc", "amb_autotune", "amb_tune_save",
     "amb_tune_load", "amb_tune_clear", "amb_tune_entries", "amb_tune_lookup", "amb_set_launch_config",
 ]
 
@@ -190,6 +190,11 @@ _lib.amb_grid_cube.argtypes = [C.c_void_p, C.POINTER(GridCfg), C.c_int64, _fp, _
 _lib.amb_get_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
 _lib.amb_reset_stats.argtypes = [C.c_void_p]
 _lib.amb_set_cert_strict.argtypes = [C.c_void_p, C.c_int]
+_lib.amb_set_grid.argtypes = [C.c_void_p, C.c_int]
+_lib.amb_compute_grid.restype = C.c_long
+_lib.amb_compute_grid.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double]
+_lib.amb_ring_reach.restype = C.c_long
+_lib.amb_ring_reach.argtypes = [C.c_double]
 _lib.amb_uavg.argtypes = [_dp, _dp, C.c_int, _dp]
 _lib.amb_triangles.argtypes = [_dp, C.c_int, _ip, C.c_int]
 _lib.amb_annular_rings.argtypes = [C.c_int, _dp, _dp]
@@ -232,6 +237,18 @@ def version():
 def ng(uhigh, du=1.0):
     """Reduced grid side (amber.c:193-197)."""
     return int(_lib.amb_ng(float(uhigh), float(du)))
+
+
+def ring_reach(uhigh):
+    """Largest |component| of a cell of a ring below uhigh (amb_ring_reach)."""
+    return int(_lib.amb_ring_reach(float(uhigh)))
+
+
+def compute_grid(uhigh1, uhigh2, uhigh3, du=1.0):
+    """The grid the lag-Fourier form computes a triangle's sums on
+    (amb_compute_grid): the smallest even 7-smooth alias-free grid of its
+    three rings, or ng(uhigh1)."""
+    return int(_lib.amb_compute_grid(float(uhigh1), float(uhigh2), float(uhigh3), float(du)))
 
 
 def annular_rings(nr=22):
@@ -769,6 +786,11 @@ class Context:
         past 1/32 of their 1/8 grid): counted in stats() (default) or, when
         strict, an AmberError"""
         self._c(_lib.amb_set_cert_strict(self._h, int(bool(strict))))
+
+    def set_grid(self, reduced=True):
+        """lag-Fourier sums on each triangle's smallest alias-free grid
+        (compute_grid, default) or on the reference's Ng(r1)"""
+        self._c(_lib.amb_set_grid(self._h, int(bool(reduced))))
 
 
 class Graph:

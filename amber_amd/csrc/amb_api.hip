@@ -377,7 +377,7 @@ int twiddles(amb_ctx *c, int n, float2 **out, FftPlan *pl)
     return AMB_OK;
 }
 
-static int ring_geom(amb_ctx *c, const Cube &cb, int r, int ng, RingGeom *g)
+static int ring_geom(amb_ctx *c, const Cube &cb, int r, int ng, RingGeom *g, int ngref)
 {
     if (r < 0 || r >= (int)c->ulow.size()) return fail(c, AMB_EARG, "ring %d outside [0,%zu)", r, c->ulow.size());
     g->lo = c->ulow[r];
@@ -390,7 +390,7 @@ static int ring_geom(amb_ctx *c, const Cube &cb, int r, int ng, RingGeom *g)
     if (g->hi > (double)(cb.N / 2))
         return fail(c, AMB_EARG, "ring %d UHigh=%g exceeds N/2=%lld", r, g->hi, (long long)(cb.N / 2));
     long jc = (long)ceil(g->hi);
-    if (jc > ng / 3) jc = ng / 3;
+    if (jc > ngref / 3) jc = ngref / 3;
     if (jc < 0) jc = 0;
     g->jcnt = (int)jc;
     /* integer radius-squared bounds of the half-open ring, uu > 0 included:
@@ -501,15 +501,18 @@ double build_fft_bytes(amb_ctx *c, const BuildOp &op)
     return (double)cb.nact() * (4.0 * (double)g.ng * g.ng + (op.kind != 1 ? 8.0 : 4.0) * cells);
 }
 
-int get_planes(amb_ctx *c, const PlaneSrc &src, int cube_id, int kind, int r, int ng, void **out)
+int get_planes(amb_ctx *c, const PlaneSrc &src, int cube_id, int kind, int r, int ng, void **out, int ngref)
 {
     Cube *cb;
     int rc = find_cube(c, cube_id, &cb);
     if (rc) return rc;
+    if (ngref <= 0) ngref = ng;
     RingGeom g;
-    if ((rc = ring_geom(c, *cb, r, ng, &g))) return rc;
+    if ((rc = ring_geom(c, *cb, r, ng, &g, ngref))) return rc;
+    /* columns j < jcnt of the half spectrum j <= ng/2 */
+    if (g.jcnt > ng / 2 + 1) return fail(c, AMB_EARG, "ring %d needs %d columns on a grid of %d", r, g.jcnt, ng);
     for (auto &p : *src.list)
-        if (p.cube == cube_id && p.kind == kind && p.ng == ng && p.lo == g.lo && p.hi == g.hi) {
+        if (p.cube == cube_id && p.kind == kind && p.ng == ng && p.jcnt == g.jcnt && p.lo == g.lo && p.hi == g.hi) {
             p.last = ++c->tick;
             p.pins++;
             if (c->pin_log && src.list == &c->planes) c->pin_log->push_back(p.d);
@@ -522,7 +525,7 @@ int get_planes(amb_ctx *c, const PlaneSrc &src, int cube_id, int kind, int r, in
     op.g = g;
     void *dep = nullptr, *sc = nullptr;
     if (kind == 2 || kind == 3) {
-        if ((rc = get_planes(c, src, cube_id, kind - 2, r, ng, &dep))) return rc;
+        if ((rc = get_planes(c, src, cube_id, kind - 2, r, ng, &dep, ngref))) return rc;
     } else if (kind == 8) {
         /* two FP64 transform buffers in 64 KiB of LDS */
         if (32L * ng > 65536) return fail(c, AMB_EARG, "grid %d exceeds the FP64 LDS transform size (2048)", ng);
@@ -535,14 +538,14 @@ int get_planes(amb_ctx *c, const PlaneSrc &src, int cube_id, int kind, int r, in
            set (FP64 images), with the channel transform's twiddles */
         if (src.record || src.bump) return fail(c, AMB_EARG, "lag-Fourier planes are not built inside graphs");
         if ((rc = lag_setup(c, *cb))) return rc;
-        if ((rc = get_planes(c, src, cube_id, kind == 9 ? 11 : 8, r, ng, &dep))) return rc;
+        if ((rc = get_planes(c, src, cube_id, kind == 9 ? 11 : 8, r, ng, &dep, ngref))) return rc;
         float2 *tw;
         FftPlan pl;
         if ((rc = twiddles(c, cb->lag_L, &tw, &pl))) return rc;
         if (kind == 10 && (rc = twiddles64(c, cb->lag_L, nullptr))) return rc;
     } else if (kind >= 4 && kind != 11) {
-        if ((rc = get_planes(c, src, cube_id, 0, r, ng, &dep))) return rc;
-        if (kind == 4 && (rc = get_planes(c, src, cube_id, 5, r, ng, &sc))) return rc;
+        if ((rc = get_planes(c, src, cube_id, 0, r, ng, &dep, ngref))) return rc;
+        if (kind == 4 && (rc = get_planes(c, src, cube_id, 5, r, ng, &sc, ngref))) return rc;
     } else {
         /* one transform of ng complex values plus twiddles and a second
            buffer must fit a workgroup's 64 KiB of LDS */
@@ -587,7 +590,7 @@ int get_planes(amb_ctx *c, const PlaneSrc &src, int cube_id, int kind, int r, in
         }
     }
     PlaneSet p;
-    p.cube = cube_id; p.kind = kind; p.ng = ng; p.lo = g.lo; p.hi = g.hi;
+    p.cube = cube_id; p.kind = kind; p.ng = ng; p.jcnt = g.jcnt; p.lo = g.lo; p.hi = g.hi;
     p.d = d; p.bytes = bytes; p.last = ++c->tick; p.pins = 1;
     src.list->push_back(p);
     if (c->pin_log && src.list == &c->planes) c->pin_log->push_back(d);
@@ -1011,7 +1014,7 @@ int closed_rings(amb_ctx *c, int id, int plane, double *rdata, int cap, int *rco
     int jmax = 0;
     std::vector<RingGeom> geo(nr);
     for (int r = 0; r < nr; ++r) {
-        if ((rc = ring_geom(c, *cb, r, N, &geo[r]))) return rc;
+        if ((rc = ring_geom(c, *cb, r, N, &geo[r], N))) return rc;
         jmax = std::max(jmax, geo[r].jcnt);
     }
     const CrLayout L = cr_layout(N, nr, jmax, ntri);
@@ -1565,6 +1568,14 @@ int amb_set_cert_strict(amb_ctx *c, int strict)
     if (!c) return AMB_EARG;
     Lock lk(c->mu);
     c->cert_strict = strict ? 1 : 0;
+    return AMB_OK;
+}
+
+int amb_set_grid(amb_ctx *c, int reduced)
+{
+    if (!c) return AMB_EARG;
+    Lock lk(c->mu);
+    c->grid_reduced = reduced ? 1 : 0;
     return AMB_OK;
 }
 

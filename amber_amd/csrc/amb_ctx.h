@@ -69,7 +69,7 @@ struct Cube {
  * Kind 0 sets are stored normalised by a power of two (max |D| <= 2^7) and
  * end in two words: max |D| as computed and its exponent E. */
 struct PlaneSet {
-    int cube, kind, ng;
+    int cube, kind, ng, jcnt;
     double lo, hi;
     void *d = nullptr;
     size_t bytes = 0;
@@ -128,6 +128,7 @@ struct amb_ctx {
     int device = 0, prec = AMB_PREC_BF16X3, verbose = 0, sync_debug = 0, no_sym = 0;
     int batch_mode = AMB_BATCH_GROUPED;
     int cert_strict = 0; /* uncertified normalisation sums: count (0) or fail (1) */
+    int grid_reduced = 1; /* lag-Fourier sums on amb_compute_grid (1) or on Ng(r1) (0) */
     uint64_t budget = 0, used = 0;
     hipStream_t stream = nullptr;
     std::recursive_mutex mu;
@@ -239,7 +240,9 @@ void release_cube_buffers(amb_ctx *c, int id);
 /* amb_set_channels without its lock and state checks */
 int set_channels_core(amb_ctx *c, int id, const int *uf, int uf_nc, int nc10);
 /* planes of (cube, kind, ring r) at grid ng from src, pinned on return */
-int get_planes(amb_ctx *c, const PlaneSrc &src, int cube_id, int kind, int r, int ng, void **out);
+/* ngref (0: ng): the reference grid whose column cut j < ngref/3 selects
+   the ring cells (amber_funcs.c:582); the set is computed on ng */
+int get_planes(amb_ctx *c, const PlaneSrc &src, int cube_id, int kind, int r, int ng, void **out, int ngref = 0);
 /* launches one recorded build on stream s with FFT scratch G */
 hipError_t launch_build(amb_ctx *c, hipStream_t s, const BuildOp &op, float2 *G);
 size_t build_scratch_bytes(amb_ctx *c, const BuildOp &op);
@@ -286,7 +289,8 @@ int lagft_run(amb_ctx *c, const amb_tri_req *req, int n, double t0, double *done
  * tiling, spectra sets (pinned when resolved) and work items */
 struct LagTri {
     int cube = 0, r1 = 0, r2 = 0, r3 = 0;
-    long ng = 0, P = 0;
+    long ng = 0, P = 0; /* Ng(r1) of the reference; pixels of the compute grid */
+    long ngc = 0;       /* the compute grid (amb_compute_grid, or ng) */
     int L = 0, nseg = 0, mode = 0, ntile = 0, nyq_sep = 0;
     void *X[3] = {nullptr, nullptr, nullptr};
     const int2 *tlist = nullptr;

@@ -446,6 +446,47 @@ long amb_ng(double uhigh, double du)
     return (ng % 2 == 0) ? ng + 2 : ng + 3;
 }
 
+long amb_ring_reach(double uhigh)
+{   /* a ring cell (ia, j) has ia^2 + j^2 < s_hi, s_hi the least s with
+       sqrt(s) >= UHigh (sqrt in double, as the ring test); so |ia|, |j| are
+       at most isqrt(s_hi - 1) */
+    if (!(uhigh > 0)) return 0;
+    long s = (long)floor(uhigh * uhigh) - 4;
+    if (s < 0) s = 0;
+    while (sqrt((double)s) < uhigh) ++s;
+    if (s == 0) return 0;
+    long m = (long)floor(sqrt((double)(s - 1)));
+    while (m * m > s - 1) --m;
+    while ((m + 1) * (m + 1) <= s - 1) ++m;
+    return m;
+}
+
+static int smooth7(long n)
+{
+    static const long pr[4] = {2, 3, 5, 7};
+    for (int i = 0; i < 4; ++i)
+        while (n % pr[i] == 0) n /= pr[i];
+    return n == 1;
+}
+
+long amb_compute_grid(double uhigh1, double uhigh2, double uhigh3, double du)
+{
+    const long ng = amb_ng(uhigh1, du);
+    const long m1 = amb_ring_reach(uhigh1), m2 = amb_ring_reach(uhigh2), m3 = amb_ring_reach(uhigh3);
+    long mx = m1 > m2 ? m1 : m2;
+    if (m3 > mx) mx = m3;
+    /* n > m1 + m2 + m3: a sum of three ring frequencies is a multiple of n
+       only when it is zero; n >= 2 mx + 2: every ring row and column is a
+       distinct index of the half spectrum, clear of its Nyquist row and
+       column */
+    long need = m1 + m2 + m3 + 1;
+    if (2 * mx + 2 > need) need = 2 * mx + 2;
+    if (need >= ng) return ng; /* the reference grid aliases, or nothing smaller holds the rings */
+    long n = need + (need & 1);
+    while (!smooth7(n)) n += 2;
+    return n < ng ? n : ng;
+}
+
 void amb_uavg(const double *ulow, const double *uhigh, int nr, double *uavg)
 {   /* amber_funcs.c:2065: (2/3)(H^3-L^3)/(H^2-L^2) */
     for (int i = 0; i < nr; ++i) {

@@ -763,17 +763,20 @@ __global__ __launch_bounds__(64) void k_lagft_cols(double2 *__restrict__ St, Fft
    half spectra St[k3][d2], k3 <= L/2, extended Hermitian, as one complex
    sequence A + i B; its inverse DFT carries row wa in the real part and
    row wb in the imaginary part (DC and Nyquist terms: real parts only).
-   Data (snapP == 0): times 2^(E1+E2+E3-21)/L^2 (exponents null: 1/L^2),
-   into out32 and out64.  Normalisation (snapP = Ng^2): the sum divided by
-   L^2 is rounded to the nearest 1/8 of Ng^2 into out64 (and out32), and
-   the largest distance to that grid, in units of Ng^2, goes to margin
-   (float bits).  Cells without triplets are exact zeros. */
+   Data (snapP == 0): times gscale 2^(E1+E2+E3-21)/L^2 (exponents null:
+   gscale/L^2), into out32 and out64 (normalisation: gscale = 1); gscale = (Ng/n)^2 carries a sum over
+   the n^2 pixels of a compute grid to the reference's Ng^2.  Normalisation
+   (snapP = n^2, outP = Ng^2): the sum divided by L^2 is rounded to the
+   nearest 1/8 of n^2, and that many eighths of Ng^2 go to out64 (and
+   out32); the largest distance to the grid, in units of n^2, goes to
+   margin (float bits).  Cells without triplets are exact zeros. */
 __global__ __launch_bounds__(64) void k_lagft_rows(const double2 *__restrict__ St, FftPlan pl,
                                                    const double2 *__restrict__ tw, int nc10,
                                                    const int *__restrict__ cnt, const int *__restrict__ ex1,
                                                    const int *__restrict__ ex2, const int *__restrict__ ex3,
-                                                   double snapP, float *__restrict__ out32,
-                                                   double *__restrict__ out64, unsigned *__restrict__ margin)
+                                                   double snapP, double outP, double gscale,
+                                                   float *__restrict__ out32, double *__restrict__ out64,
+                                                   unsigned *__restrict__ margin)
 {
     extern __shared__ __align__(16) double2 dsm[];
     const int L = pl.n, W = 2 * nc10 + 1;
@@ -796,14 +799,14 @@ __global__ __launch_bounds__(64) void k_lagft_rows(const double2 *__restrict__ S
     }
     __syncthreads();
     const double2 *res = fftc::transform<+1, 5>(bx, by, tw, pl, threadIdx.x, 64);
-    const double scale = (ex1 ? ldexp(1.0, *ex1 + *ex2 + *ex3 - 3 * AMB_SET_EXP) : 1.0) / ((double)L * (double)L);
+    const double scale = (ex1 ? ldexp(gscale, *ex1 + *ex2 + *ex3 - 3 * AMB_SET_EXP) : gscale) / ((double)L * (double)L);
     float mg = 0.f;
     auto put = [&](size_t cell, double v) {
         if (!cnt[cell]) v = 0.0;
         if (snapP > 0.0) {
             const double q = rint(8.0 * v / snapP) / 8.0;
             mg = fmaxf(mg, (float)fabs(v / snapP - q));
-            v = snapP * q;
+            v = outP * q;
         }
         out32[cell] = (float)v;
         if (out64) out64[cell] = v;
@@ -988,7 +991,8 @@ struct LJ {
     int q;          /* request */
     bool norm;      /* FP64 normalisation job */
     Cube *cb;
-    long ng, P;
+    long ng, P;     /* Ng(r1); pixels of the compute grid */
+    long ngc, Pref; /* the compute grid; Ng(r1)^2 */
     int nseg;
     int mode, ntile, nyq_sep;
     const int2 *tlist; /* mode 1 */
@@ -1075,6 +1079,15 @@ double lag_distinct_r(int r1, int r2, int r3)
     return r1 == r2 && r2 == r3 ? 1.0 / 3.0 : (r1 == r2 || r2 == r3) ? 0.5 : 1.0;
 }
 double lag_distinct(const amb_tri_req &q) { return lag_distinct_r(q.r1, q.r2, q.r3); }
+
+/* the grid a triangle's sums run on: the smallest alias-free grid of its
+   three rings (amb_compute_grid), or Ng(r1) when reduced grids are off */
+long compute_grid(const amb_ctx *c, const Cube &cb, int r1, int r2, int r3)
+{
+    const long ng = amb_ng(c->uhigh[r1], cb.Du);
+    if (!c->grid_reduced) return ng;
+    return amb_compute_grid(c->uhigh[r1], c->uhigh[r2], c->uhigh[r3], cb.Du);
+}
 
 /* tiling of a triangle: (k2, k3) tiles, or for FP32 jobs the symmetric
    (i, j) tiles of r1 == r2 or the listed (k2, k3) tiles of r2 == r3 where
@@ -1320,7 +1333,8 @@ int amb::lagft_run(amb_ctx *c, const amb_tri_req *req, int n, double t0, double 
             const int *e1 = j.norm ? nullptr : d32[slot[i]].ex1, *e2 = j.norm ? nullptr : d32[slot[i]].ex2,
                       *e3 = j.norm ? nullptr : d32[slot[i]].ex3;
             hipLaunchKernelGGL(k_lagft_rows, dim3((unsigned)((W + 1) / 2)), dim3(64), lds, c->stream, St, pl, tw, cb.nc10,
-                               cb.d_lagcnt, e1, e2, e3, j.norm ? (double)j.P : 0.0, o, o64,
+                               cb.d_lagcnt, e1, e2, e3, j.norm ? (double)j.P : 0.0, (double)j.Pref,
+                               j.norm ? 1.0 : (double)j.Pref / (double)j.P, o, o64,
                                j.norm ? c->margin + 2 : nullptr);
             e = hipGetLastError();
         }
@@ -1363,10 +1377,11 @@ int amb::lagft_run(amb_ctx *c, const amb_tri_req *req, int n, double t0, double 
             }
             const int L = cb.lag_L;
             c->st.evals += cb.lag_evals;
-            c->st.flop_contract += 2.0 * (double)cb.lag_pairs * (double)j.P;
-            c->st.flop_contract_distinct += 2.0 * (double)cb.lag_pairs * (double)j.P;
-            /* 14 flops per distinct (k2, k3, pixel) of the half spectrum
-               k3 <= L/2: half of them where r1 == r2 or r2 == r3 pair equal
+            /* the direct sum's work on the reference grid */
+            c->st.flop_contract += 2.0 * (double)cb.lag_pairs * (double)j.Pref;
+            c->st.flop_contract_distinct += 2.0 * (double)cb.lag_pairs * (double)j.Pref;
+            /* 14 flops per distinct (k2, k3, pixel of the compute grid) of the
+               half spectrum k3 <= L/2: half of them where r1 == r2 or r2 == r3 pair equal
                sums, a third where r1 == r2 == r3 */
             c->st.flop_lagft += 14.0 * (double)L * (double)(L / 2 + 1) * (double)j.P * lag_distinct(req[j.q]);
             /* issued: whole tiles, plus the k3 = L/2 column (or its k2 left
@@ -1400,7 +1415,7 @@ int amb::lagft_run(amb_ctx *c, const amb_tri_req *req, int n, double t0, double 
                 continue;
             }
             const Cube &cb = it->second;
-            const double ng = (double)amb_ng(c->uhigh[q.r1], cb.Du), L = cb.lag_L;
+            const double ng = (double)compute_grid(c, cb, q.r1, q.r2, q.r3), L = cb.lag_L;
             const double cells = (double)cb.h_lagcnt.size();
             tdev[k] = 14.0 * L * (L / 2 + 1) * ng * ng * (q.r1 == q.r2 ? 0.55 : 1.0) * (q.norm ? 9.0 : 1.0) / 60e12;
             const double bytes = cells * (4.0 + (q.cnt ? 4.0 : 0.0) + (q.norm ? 4.0 : 0.0) +
@@ -1449,7 +1464,7 @@ int amb::lagft_run(amb_ctx *c, const amb_tri_req *req, int n, double t0, double 
                 return fail(c, AMB_EARG, "triangle (%d,%d,%d) outside [0,%d)", q.r1, q.r2, q.r3, nr);
             }
             if ((rc = lag_setup(c, *cb))) { unpin_all(c); return rc; }
-            const long ng = amb_ng(c->uhigh[q.r1], cb->Du);
+            const long ng = amb_ng(c->uhigh[q.r1], cb->Du), ngc = compute_grid(c, *cb, q.r1, q.r2, q.r3);
             if (ng <= 0 || ng > 2730) { unpin_all(c); return fail(c, AMB_EARG, "Ng=%ld out of range", ng); }
             const int L = cb->lag_L;
             {   /* twiddles and plan of L: FP32 for the spectra, FP64 for the 2-D inverse */
@@ -1466,7 +1481,9 @@ int amb::lagft_run(amb_ctx *c, const amb_tri_req *req, int n, double t0, double 
                 j.norm = nm;
                 j.cb = cb;
                 j.ng = ng;
-                j.P = ng * ng;
+                j.ngc = ngc;
+                j.P = ngc * ngc;
+                j.Pref = ng * ng;
                 j.nseg = (int)((j.P + SEG - 1) / SEG);
                 if ((rc = choose_tiles(c, L, nm, q.r1 == q.r2, q.r2 == q.r3, &j.mode, &j.ntile, &j.nyq_sep, &j.tlist,
                                        &j.tmap, &j.nyql, &j.nnyq))) {
@@ -1479,7 +1496,7 @@ int amb::lagft_run(amb_ctx *c, const amb_tri_req *req, int n, double t0, double 
                 if (!w.empty() && (part_tot + al(pb) > PART_CAP || out_tot + ob > OUT_CAP || (int)w.size() >= MAX_JOBS))
                     if ((rc = flush())) return rc;
                 for (int t = 0; t < 3 && !rc; ++t) {
-                    rc = get_planes(c, src, q.cube_id, nm ? 10 : 9, rr[t], (int)ng, &j.X[t]);
+                    rc = get_planes(c, src, q.cube_id, nm ? 10 : 9, rr[t], (int)ngc, &j.X[t], (int)ng);
                     if (rc == AMB_EBUDGET && !w.empty()) {
                         /* the wave's planes hold the memory: run it, then resolve again */
                         if ((rc = flush())) return rc;
@@ -1519,7 +1536,7 @@ int amb::lag_prepare(amb_ctx *c, int cube, int r1, int r2, int r3, LagTri *t)
     if (r1 < 0 || r1 >= nr || r2 < 0 || r2 >= nr || r3 < 0 || r3 >= nr)
         return fail(c, AMB_EARG, "triangle (%d,%d,%d) outside [0,%d)", r1, r2, r3, nr);
     if ((rc = lag_setup(c, *cb))) return rc;
-    const long ng = amb_ng(c->uhigh[r1], cb->Du);
+    const long ng = amb_ng(c->uhigh[r1], cb->Du), ngc = compute_grid(c, *cb, r1, r2, r3);
     if (ng <= 0 || ng > 2730) return fail(c, AMB_EARG, "Ng=%ld out of range", ng);
     const int L = cb->lag_L;
     float2 *tw;
@@ -1528,7 +1545,8 @@ int amb::lag_prepare(amb_ctx *c, int cube, int r1, int r2, int r3, LagTri *t)
     *t = LagTri{};
     t->cube = cube; t->r1 = r1; t->r2 = r2; t->r3 = r3;
     t->ng = ng;
-    t->P = ng * ng;
+    t->ngc = ngc;
+    t->P = ngc * ngc;
     t->L = L;
     t->nseg = (int)((t->P + SEG - 1) / SEG);
     t->cells = cb->h_lagcnt.size();
@@ -1538,7 +1556,7 @@ int amb::lag_prepare(amb_ctx *c, int cube, int r1, int r2, int r3, LagTri *t)
     const PlaneSrc src{&c->planes};
     const int rr[3] = {r1, r2, r3};
     for (int k = 0; k < 3; ++k)
-        if ((rc = get_planes(c, src, cube, 9, rr[k], (int)ng, &t->X[k]))) return rc;
+        if ((rc = get_planes(c, src, cube, 9, rr[k], (int)ngc, &t->X[k], (int)ng))) return rc;
     return AMB_OK;
 }
 
@@ -1563,7 +1581,9 @@ void amb::lag_fill_range(amb_ctx *c, const LagTri &t, void *part, long i0, long 
     j.norm = false;
     j.cb = &c->cubes[t.cube];
     j.ng = t.ng;
+    j.ngc = t.ngc;
     j.P = t.P;
+    j.Pref = t.ng * t.ng;
     j.nseg = t.nseg;
     j.mode = t.mode;
     j.ntile = t.ntile;
@@ -1625,7 +1645,8 @@ hipError_t amb::lag_launch_finish(amb_ctx *c, hipStream_t s, const LagTri &t, co
     const int *ex[3];
     for (int k = 0; k < 3; ++k) ex[k] = (const int *)((const float2 *)t.X[k] + (size_t)t.P * cb.lag_lhp) + 1;
     hipLaunchKernelGGL(k_lagft_rows, dim3((unsigned)((W + 1) / 2)), dim3(64), lds, s, st, pl, tw, cb.nc10,
-                       cb.d_lagcnt, ex[0], ex[1], ex[2], 0.0, bl, nullptr, nullptr);
+                       cb.d_lagcnt, ex[0], ex[1], ex[2], 0.0, (double)(t.ng * t.ng),
+                       (double)(t.ng * t.ng) / (double)t.P, bl, nullptr, nullptr);
     return hipGetLastError();
 }
 
@@ -1636,8 +1657,8 @@ void amb::lag_account(amb_ctx *c, const LagTri &t, int32_t *cnt)
     if (cnt) memcpy(cnt, cb.h_lagcnt.data(), sizeof(int32_t) * cb.h_lagcnt.size());
     const int L = t.L;
     c->st.evals += cb.lag_evals;
-    c->st.flop_contract += 2.0 * (double)cb.lag_pairs * (double)t.P;
-    c->st.flop_contract_distinct += 2.0 * (double)cb.lag_pairs * (double)t.P;
+    c->st.flop_contract += 2.0 * (double)cb.lag_pairs * (double)(t.ng * t.ng);
+    c->st.flop_contract_distinct += 2.0 * (double)cb.lag_pairs * (double)(t.ng * t.ng);
     c->st.flop_lagft += 14.0 * (double)L * (double)(L / 2 + 1) * (double)t.P * lag_distinct_r(t.r1, t.r2, t.r3);
     c->st.flop_contract_exec +=
         14.0 * ((double)t.ntile * tile_a<float>() * tile_b<float>() + (t.nyql ? t.nnyq : L)) * (double)t.P;

@@ -77,12 +77,21 @@ def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
-def tri_costs(A, tris, hi, precision="bf16x3"):
-    """Relative contraction cost of each triangle: Ng(r1)^2 pixels times the
+def tri_grids(A, tris, hi, precision="bf16x3", ref_grid=False):
+    """The grid each triangle is summed on: its compute grid in the
+    lag-Fourier form (the smallest alias-free grid of its rings), Ng(r1)
+    otherwise"""
+    if precision == "lagft" and not ref_grid:
+        return np.array([A.compute_grid(hi[t[0]], hi[t[1]], hi[t[2]]) for t in tris], np.float64)
+    return np.array([A.ng(hi[t[0]]) for t in tris], np.float64)
+
+
+def tri_costs(A, tris, hi, precision="bf16x3", ref_grid=False):
+    """Relative contraction cost of each triangle: its grid's pixels times the
     triplets it computes (half of them where r1 == r2 or r2 == r3 lets the
-    kernel mirror the other half); in the lag-Fourier form Ng(r1)^2 times
+    kernel mirror the other half); in the lag-Fourier form the pixels times
     its tiles."""
-    ng = np.array([A.ng(hi[t[0]]) for t in tris], np.float64)
+    ng = tri_grids(A, tris, hi, precision, ref_grid)
     sym = np.array([(t[0] == t[1]) or (t[1] == t[2]) for t in tris])
     if precision == "lagft":  # r1 == r2: the symmetric (i, j) tiles, about 0.63 of the (k2, k3) ones;
         # r2 == r3: the (k2, k3) tiles without those below the diagonal of [0, L/2)^2, about 0.79
@@ -133,6 +142,8 @@ def main():
     ap.add_argument("--tri-limit", type=int, default=0,
                     help="development only: K triangles (the line is then not the metric workload)")
     ap.add_argument("--tri-pick", default="first", choices=["first", "largest"])
+    ap.add_argument("--ref-grid", action="store_true",
+                    help="lag-Fourier form on every triangle's reference grid Ng(r1) instead of its compute grid")
     ap.add_argument("--batch-mode", default="grouped", choices=["grouped", "per_triangle"])
     ap.add_argument("--estimator", default="v1", choices=["v1", "v2"],
                     help="v1: amber (data contraction); v2: amber_v2 (data plus the normalisation contraction)")
@@ -192,7 +203,10 @@ def main():
     log("uploaded cube (%.2f GB of FP64 file data) in %.2f s" % ((gv.nbytes + samp.nbytes) / 1e9, upload_s))
 
     W = 2 * nc10 + 1
-    costs = tri_costs(A, tris, hi, args.precision)
+    if args.ref_grid:
+        ctx.set_grid(False)
+    costs = tri_costs(A, tris, hi, args.precision, args.ref_grid)
+    grids = tri_grids(A, tris, hi, args.precision, args.ref_grid)
     # the sample a missing table is tuned on: up to 24 triangles evenly
     # spread over the list ordered by cost, so every Ng bucket is present
     by_cost = np.argsort(costs, kind="stable")
@@ -216,8 +230,8 @@ def main():
         ctx.bispec_batch(cube, tris[sl], out=tuple(None if o is None else o[:n] for o in out))
 
     def footprint(sl):
-        """plane sets a slice builds: Ng(r1)^2 per distinct (ring, Ng)"""
-        sets = {(int(r), int(A.ng(hi[t[0]]))) for t in tris[sl] for r in t}
+        """plane sets a slice builds: n^2 per distinct (ring, grid n)"""
+        sets = {(int(r), int(grids[k])) for k in sl for r in tris[k]}
         return sum(g * g for _, g in sets)
 
     def warm_order(slc):
@@ -289,7 +303,8 @@ def main():
             "traffic": traffic_per_launch(args, st, nsteps),
             "kernel": "k_lagft_tiles" if lagft else "k_contract<%s>" % args.precision,
             "work": ("14 flop per distinct (k2, k3, pixel) of the half spectrum k3 <= L/2 (one complex triple "
-                     "product conj(F) G H), L = channel transform length; half of the half spectrum where r1 == r2 "
+                     "product conj(F) G H), L = channel transform length, pixels of the grid the triangle is "
+                     "summed on; half of the half spectrum where r1 == r2 "
                      "or r2 == r3 makes sums pairwise equal, a third where r1 == r2 == r3" if lagft else
                      "2 flop per (distinct triplet, pixel): 2*V*Ng^2 per triangle, halved (plus the self-mirrored "
                      "triplets) where r1 == r2 or r2 == r3"),
@@ -372,6 +387,8 @@ def main():
                    "bins": cfg["rings"][2],
                    "triangles": int(len(tris)), "nc10": nc10, "flagged": cfg["nflag"],
                    "evals_per_pass": int(st["evals"]),
+                   "sum_grid": ("per triangle the smallest alias-free grid of its three rings, sums scaled by "
+                                "(Ng/n)^2" if lagft and not args.ref_grid else "Ng(r1) of the reference"),
                    "step": "one slice of the triangle list (%d slices of equal contraction cost); the %d timed "
                            "steps cover every triangle once" % (nsteps, nsteps)},
         "pass_s": round(elapsed, 3),

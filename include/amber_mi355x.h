@@ -141,6 +141,18 @@ uint32_t amb_crc32(const void *p, size_t n);
  * hard-coded edges, amber_funcs.c:2022-2053.  amb_uavg: :2065.
  * amb_triangles: :443-449 (returns the count; fills up to cap triples). */
 long amb_ng(double uhigh, double du);
+/* The grid a triangle's sums are computed on (no counterpart in the
+ * reference, which computes every triangle on amb_ng(UHigh[r1])): the
+ * smallest even 7-smooth n with n > m1 + m2 + m3 and n >= 2 max(m) + 2,
+ * m = amb_ring_reach of each ring's UHigh (the largest |component| of a
+ * ring cell), or Ng itself when that is not smaller.  On such a grid no sum
+ * of three ring frequencies wraps, and neither does one on Ng when n < Ng
+ * (n < Ng implies m1 + m2 + m3 < Ng), so the triple sum over n^2 pixels is
+ * n^2 times the closed-triangle sum over the ring cells and the reference's
+ * sum over Ng^2 pixels is (Ng/n)^2 times the sum over n^2 (amber.c:193-266
+ * sums D_r1 D_r2 D_r3 over Ng(r1)^2 pixels). */
+long amb_ring_reach(double uhigh);
+long amb_compute_grid(double uhigh1, double uhigh2, double uhigh3, double du);
 void amb_annular_rings(int nr, double *ulow, double *uhigh);
 void amb_uavg(const double *ulow, const double *uhigh, int nr, double *uavg);
 int amb_triangles(const double *uavg, int nr, int *tri, int cap);
@@ -556,6 +568,15 @@ int amb_reset_stats(amb_ctx *ctx);
  * reference, whose FP32 sums (amber_v2.c:262-266) are never exact at these
  * sizes.  A sum beyond the int32 range of eighths fails in either mode. */
 int amb_set_cert_strict(amb_ctx *ctx, int strict);
+
+/* Grids of the lag-Fourier form.  reduced = 1 (default): each triangle's
+ * images, channel spectra and products are computed on amb_compute_grid of
+ * its rings and the sums scaled by (Ng/n)^2 in FP64 (the normalisation
+ * rounded to its 1/8 grid of n^2 first, so it stays exact); the lag arrays
+ * are the reference's sums over Ng(r1)^2 pixels (amber.c:193-266) to
+ * rounding.  reduced = 0: every triangle on Ng(r1), as the reference.  The
+ * MFMA forms always use Ng(r1).  No counterpart in the reference. */
+int amb_set_grid(amb_ctx *ctx, int reduced);
 
 #ifdef __cplusplus
 }

@@ -199,3 +199,58 @@ def test_symmetric_tilings(ctx, sym):
     finally:
         plain.close()
         ctx.set_precision("bf16x3")
+
+
+def test_compute_grids(ctx):
+    """Triangles summed on their compute grids (amb_compute_grid) against the
+    reference grid Ng(r1) and the FP64 oracle (which sums over Ng^2 pixels),
+    at Du = 1.0254 with rings to 200 cells on a 512^2 cube: triangles with
+    r1 on the last ring take the column cut j < Ng/3 = 198 of the reference
+    (amber_funcs.c:582) on a smaller grid, and those whose three rings wrap
+    on Ng (3 max|k| >= Ng, amber.c:193-197) stay on Ng.  Counts exact, v2
+    normalisation bit for bit on both grids, sums normwise against the
+    oracle and against each other."""
+    N, nc, nc10, Du = 512, 12, 11, 1.0254
+    lo, hi = A.log_rings(1, 200, 10)
+    gv, samp, uf = A.generate(N, nc, nflag=0, seed=41)
+    mask = O.chan_mask(nc, 0, uf)
+    tris = [tuple(int(x) for x in t) for t in A.triangles(lo, hi)]
+    last = len(lo) - 1
+    pick = [t for t in tris if t[0] == last]
+    kinds = [(A.compute_grid(hi[a], hi[b], hi[c], Du), A.ng(hi[a], Du)) for a, b, c in pick]
+    assert any(n < g for n, g in kinds) and any(n == g for n, g in kinds)
+    assert A.ng(hi[last], Du) // 3 < int(np.ceil(hi[last]))  # the cut applies
+    ctx.set_precision("lagft")
+    try:
+        ctx.set_rings(lo, hi)
+        cube = ctx.load_cube(N, nc, Du, 0, gv, samp)
+        ctx.set_channels(cube, None, nc10)
+        sel = pick[::max(1, len(pick) // 6)] + [pick[-1]]
+        for t in sel:
+            n, g = A.compute_grid(hi[t[0]], hi[t[1]], hi[t[2]], Du), A.ng(hi[t[0]], Du)
+            print("case lagft Du=%g triangle %s grid %d (Ng %d)" % (Du, t, n, g), flush=True)
+            ctx.set_grid(True)
+            red = ctx.bispec_ex(cube, *t, want_norm=True)
+            ctx.set_grid(False)
+            ref = ctx.bispec_ex(cube, *t, want_norm=True)
+            ctx.set_grid(True)
+            b64, c64, n64 = O.bispec(N, nc, Du, gv, samp, lo, hi, *t, mask, nc10, want_norm=True)
+            P = g * g
+            snapped = P * np.rint(n64 / P * 8) / 8
+            for res in (red, ref):
+                assert res["ng"] == g
+                assert np.array_equal(res["cnt"], c64)
+                assert np.array_equal(res["norm64"], snapped)
+            e_red = np.linalg.norm(red["bl64"] - b64) / np.linalg.norm(b64)
+            e_ref = np.linalg.norm(ref["bl64"] - b64) / np.linalg.norm(b64)
+            e_rr = np.linalg.norm(red["bl64"] - ref["bl64"]) / np.linalg.norm(b64)
+            print("  normwise: compute grid %.3g, Ng %.3g, between %.3g" % (e_red, e_ref, e_rr), flush=True)
+            # fewer pixels carry the same per-pixel FP32 rounding: the error
+            # grows about as Ng/n, against the 1e-5 bar
+            assert e_red <= 5e-6 and e_ref <= 2e-6 and e_rr <= 6e-6, (t, e_red, e_ref, e_rr)
+            if n == g:
+                assert np.array_equal(red["bl64"], ref["bl64"])
+        ctx.release_cube(cube)
+    finally:
+        ctx.set_grid(True)
+        ctx.set_precision("bf16x3")
