@@ -1106,6 +1106,7 @@ int amb_create(const amb_opts *o, amb_ctx **out)
     if (hipMemGetInfo(&fr, &tot) != hipSuccess) { delete c; return AMB_EHIP; }
     c->sync_debug = getenv("AMB_SYNC_DEBUG") != nullptr;
     c->no_sym = getenv("AMB_NO_SYM") != nullptr; /* for A/B checks of the symmetry */
+    c->no_shear = getenv("AMB_NO_SHEAR") != nullptr; /* r1 == r2 on the (i, j) pair tiles instead */
     if (const char *v = getenv("AMB_WAVE_TRIS")) { /* smaller grouped waves, for tests */
         const int k = atoi(v);
         if (k > 0 && k < c->wave_max) c->wave_max = k;

@@ -125,7 +125,7 @@ struct TriJob {
 } // namespace amb
 
 struct amb_ctx {
-    int device = 0, prec = AMB_PREC_BF16X3, verbose = 0, sync_debug = 0, no_sym = 0;
+    int device = 0, prec = AMB_PREC_BF16X3, verbose = 0, sync_debug = 0, no_sym = 0, no_shear = 0;
     int batch_mode = AMB_BATCH_GROUPED;
     int cert_strict = 0; /* uncertified normalisation sums: count (0) or fail (1) */
     int grid_reduced = 1; /* lag-Fourier sums on amb_compute_grid (1) or on Ng(r1) (0) */

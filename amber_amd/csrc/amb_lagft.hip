@@ -373,6 +373,210 @@ __global__ __launch_bounds__(256, 2) void k_chan_spectra_blk(const float *__rest
  * needs is staged (at k2 - a + TB), and H(L/2) takes one more slot.  When
  * TB does not divide L/2 the column is a column of the last, partial tile;
  * otherwise (FP64, mode 1) k_lagft_nyq sums it. */
+/* ---------------------------------------------------------- sheared tiles */
+/* Mode 3 (FP32, r1 == r2, F = G, 256 | L).  S(k2, k3) = S(-k2 - k3, k3):
+ * for a row k3 the map k2 -> -k2 - k3 (mod L) is a reflection, and in the
+ * coordinate t = k2 + floor(k3/2) it is t -> -1 - t (k3 odd) or t -> -t
+ * (k3 even, fixed points t = 0 and t = L/2).  So t in [0, L/2) holds every
+ * distinct sum of a row once, plus t = L/2 for even k3: the half plane's
+ * distinct sums are the rectangle (t, k3) in [0, L/2)^2, the fixed-point
+ * column t = L/2 (even k3 < L/2) and the row k3 = L/2 (t in [0, L/2]).
+ * A 128 x 128 tile (t in [a, a+128), k3 in [b, b+128)) stages per pixel
+ *   G(k2) for k2 = a - b/2 - 64 + r, r in [0, 193)      (units 0-24)
+ *   conj F(s) for s = a + b/2 + r,   r in [0, 193)      (units 25-49)
+ *   H(k3) for k3 = b + r,            r in [0, 129)      (units 50-66)
+ * Lane (li, lj) = (tid % 16, tid / 16) owns t = a + 8 li + u, k3 = b + 8 lj
+ * + v: G at r = 64 + 8 li - 4 lj + u - floor(v/2) (a 12-value window from
+ * 8 li - 4 lj + 60), conj F at r = 8 li + 4 lj + u + ceil(v/2) (12 values
+ * from 8 li + 4 lj), H at 8 lj + v.  Lanes that an LDS cycle serves
+ * together differ in li, so their windows start 8 values (one 80-byte
+ * unit, 20 banks) apart: conflict-free.  Per pixel that is 6 + 6 + 4
+ * 16-byte reads for 64 products, as the (k2, k3) tiles.
+ * The edge sums ride in the tiles that stage their operands, one per
+ * thread, into the job's [segment][L] column block at index k2:
+ *   k3 = L/2 (tiles with b = L/2 - 128): thread t < 128, t = a + tid,
+ *     k2 = t - L/4: G at r = tid, conj F at tid + 64, H(L/2) at 128;
+ *   t = L/2, even k3 (tiles with a = L/2 - 128): thread 128 + m, k3 = b +
+ *     2m, k2 = L/2 - k3/2: G at 192 - m, conj F at 128 + m, H at 2m;
+ *   (t, k3) = (L/2, L/2) (both): thread 192, k2 = L/4: G at 128, conj F at
+ *     192, H at 128. */
+constexpr int SH_UG = 25, SH_UF = 25, SH_UH = 17;
+constexpr int SH_PXB = (SH_UG + SH_UF + SH_UH) * BLKB;
+constexpr int SH_NG = 193, SH_NF = 193, SH_NH = 129;
+
+__device__ __forceinline__ int sh_off(int unit0, int r) { return (unit0 + (r >> 3)) * BLKB + (r & 7) * 8; }
+
+template <int KPX, int STRIDE>
+__device__ __forceinline__ void shear_tile(const LagJob<float> &J, int sg, int tile, char *lds0, char *lds1)
+{
+    typedef float2 C;
+    constexpr int SPT = (SH_NG + SH_NF + SH_NH + 255) / 256;
+    const long P = J.P;
+    const int lhp = J.lhp, L = J.L, seg = J.seg, half = L / 2;
+    const int nta = L / 256;
+    const int a = (tile / nta) * 128, b = (tile % nta) * 128;
+    const long xb = (long)sg * seg, xe = xb + seg < P ? xb + seg : P;
+    const int tid = threadIdx.x;
+
+    const C *src[SPT];
+    int kk[SPT], off[SPT];
+    bool ok[SPT], cj[SPT];
+#pragma unroll
+    for (int u = 0; u < SPT; ++u) {
+        const int sl = tid + 256 * u;
+        src[u] = J.G;
+        kk[u] = 0;
+        ok[u] = cj[u] = false;
+        off[u] = -1;
+        int k;
+        bool conj_of_value;
+        const C *x;
+        if (sl < SH_NG) {
+            k = a - b / 2 - 64 + sl;
+            x = J.G;
+            conj_of_value = false;
+            off[u] = sh_off(0, sl);
+        } else if (sl < SH_NG + SH_NF) {
+            k = a + b / 2 + (sl - SH_NG);
+            x = J.F;
+            conj_of_value = true;
+            off[u] = sh_off(SH_UG, sl - SH_NG);
+        } else if (sl < SH_NG + SH_NF + SH_NH) {
+            k = b + (sl - SH_NG - SH_NF); /* <= L/2 */
+            x = J.H;
+            conj_of_value = false;
+            off[u] = sh_off(SH_UG + SH_UF, sl - SH_NG - SH_NF);
+        } else {
+            continue;
+        }
+        k %= L;
+        if (k < 0) k += L;
+        const bool m = 2 * k > L;
+        src[u] = x;
+        kk[u] = m ? L - k : k;
+        cj[u] = m != conj_of_value;
+        ok[u] = true;
+    }
+
+    C rg[KPX][SPT];
+    auto fetch = [&](long x0) {
+#pragma unroll
+        for (int pp = 0; pp < KPX; ++pp) {
+            const long x = x0 + pp;
+            const size_t row = (size_t)x * lhp;
+#pragma unroll
+            for (int u = 0; u < SPT; ++u) {
+                C v = make_float2(0.f, 0.f);
+                if (x < xe && ok[u]) {
+                    v = src[u][row + kk[u]];
+                    if (cj[u]) v.y = -v.y;
+                }
+                rg[pp][u] = v;
+            }
+        }
+    };
+    auto stash = [&](char *st) {
+#pragma unroll
+        for (int pp = 0; pp < KPX; ++pp)
+#pragma unroll
+            for (int u = 0; u < SPT; ++u)
+                if (off[u] >= 0) *(C *)(st + pp * STRIDE + off[u]) = rg[pp][u];
+    };
+
+    const int li = tid & 15, lj = tid >> 4;
+    /* window starts (multiples of 4 values): reads 0, 1 (and 4, 5) sit at
+       +0, +16 (+80, +96) bytes of the window's first unit; reads 2, 3 at
+       +32, +48, or in the next unit at +80, +96 when the window starts
+       mid-unit */
+    const int gw = 8 * li - 4 * lj + 60, fw = 8 * li + 4 * lj;
+    const int gbase = sh_off(0, gw), fbase = sh_off(SH_UG, fw);
+    const int gmid = (gw & 7) ? 16 : 0, fmid = (fw & 7) ? 16 : 0;
+    const int hbase = (SH_UG + SH_UF + lj) * BLKB;
+
+    float acr[8][8], aci[8][8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+#pragma unroll
+        for (int v = 0; v < 8; ++v) acr[u][v] = aci[u][v] = 0.f;
+
+    /* the edge sum of this thread, if any */
+    int eg = -1, ef = 0, eh = 0, ek2 = 0;
+    if (b == half - 128 && tid < 128) {
+        eg = sh_off(0, tid);
+        ef = sh_off(SH_UG, tid + 64);
+        eh = sh_off(SH_UG + SH_UF, 128);
+        ek2 = (a + tid - L / 4 + L) % L;
+    } else if (a == half - 128 && tid >= 128 && tid < 192) {
+        const int m = tid - 128;
+        eg = sh_off(0, 192 - m);
+        ef = sh_off(SH_UG, 128 + m);
+        eh = sh_off(SH_UG + SH_UF, 2 * m);
+        ek2 = half - b / 2 - m;
+    } else if (a == half - 128 && b == half - 128 && tid == 192) {
+        eg = sh_off(0, 128);
+        ef = sh_off(SH_UG, 192);
+        eh = sh_off(SH_UG + SH_UF, 128);
+        ek2 = L / 4;
+    }
+    const bool ex = eg >= 0;
+    float enr = 0.f, eni = 0.f;
+
+    const long nch = (xe - xb + KPX - 1) / KPX;
+    fetch(xb);
+    stash(lds0);
+    __syncthreads();
+    for (long ch = 0; ch < nch; ++ch) {
+        if (ch + 1 < nch) fetch(xb + (ch + 1) * KPX);
+        const char *st = (ch & 1) ? lds1 : lds0;
+#pragma unroll 1
+        for (int pp = 0; pp < KPX; ++pp) {
+            const char *px = st + pp * STRIDE;
+            C g[12], f[12], hv[8];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                const int o = (q < 2 ? 16 * q : q < 4 ? 16 * q + gmid : 16 * q + 16);
+                const float4 w = *(const float4 *)(px + gbase + o);
+                g[2 * q] = make_float2(w.x, w.y);
+                g[2 * q + 1] = make_float2(w.z, w.w);
+            }
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                const int o = (q < 2 ? 16 * q : q < 4 ? 16 * q + fmid : 16 * q + 16);
+                const float4 w = *(const float4 *)(px + fbase + o);
+                f[2 * q] = make_float2(w.x, w.y);
+                f[2 * q + 1] = make_float2(w.z, w.w);
+            }
+            load_unit(px + hbase, hv);
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+#pragma unroll
+                for (int v = 0; v < 8; ++v) {
+                    const C gg = g[4 + u - (v >> 1)], ff = f[u + ((v + 1) >> 1)];
+                    const float tr = fmaf(gg.x, hv[v].x, -gg.y * hv[v].y);
+                    const float ti = fmaf(gg.x, hv[v].y, gg.y * hv[v].x);
+                    acr[u][v] = fmaf(ff.x, tr, fmaf(-ff.y, ti, acr[u][v]));
+                    aci[u][v] = fmaf(ff.x, ti, fmaf(ff.y, tr, aci[u][v]));
+                }
+            if (ex) {
+                const C gg = *(const C *)(px + eg), ff = *(const C *)(px + ef), hh = *(const C *)(px + eh);
+                const float tr = fmaf(gg.x, hh.x, -gg.y * hh.y), ti = fmaf(gg.x, hh.y, gg.y * hh.x);
+                enr = fmaf(ff.x, tr, fmaf(-ff.y, ti, enr));
+                eni = fmaf(ff.x, ti, fmaf(ff.y, tr, eni));
+            }
+        }
+        if (ch + 1 < nch) stash((ch & 1) ? lds0 : lds1);
+        __syncthreads();
+    }
+    if (ex) J.nyq[(size_t)sg * L + ek2] = make_float2(enr, eni);
+    C *out = J.part + ((size_t)sg * J.ntile + tile) * (128 * 128);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        C *o = out + (size_t)(8 * li + u) * 128 + 8 * lj;
+#pragma unroll
+        for (int v = 0; v < 8; ++v) o[v] = make_float2(acr[u][v], aci[u][v]);
+    }
+}
+
 template <typename R>
 __global__ __launch_bounds__(256, LagT<R>::WPS) void k_lagft_tiles(const LagJob<R> *__restrict__ jobs, int njob, long total)
 {
@@ -386,7 +590,8 @@ __global__ __launch_bounds__(256, LagT<R>::WPS) void k_lagft_tiles(const LagJob<
     constexpr int NSLOT = 2 * (TA + TB);               /* A, B, then TA + TB of C */
     constexpr int SPT = (NSLOT + (FOLD ? 1 : 0) + 255) / 256; /* slots per thread */
     constexpr int NF = BU + BV - 1, NFU = (NF + VPU - 1) / VPU;
-    __shared__ __align__(16) char lds[2][KPX * PXB];
+    constexpr int PXBM = sizeof(R) == 4 && SH_PXB > PXB ? SH_PXB : PXB;
+    __shared__ __align__(16) char lds[2][KPX * PXBM];
     const long it = xcd_item(blockIdx.x, gridDim.x);
     if (it >= total) return;
     const int jb = find_job(jobs, njob, it);
@@ -395,6 +600,12 @@ __global__ __launch_bounds__(256, LagT<R>::WPS) void k_lagft_tiles(const LagJob<
     const int nt3 = tiles3<R>(L), ntile = jobs[jb].ntile;
     const long local = it - jobs[jb].item0 + jobs[jb].ibase;
     const int sg = (int)(local / ntile), tile = (int)(local - (long)sg * ntile);
+    if constexpr (sizeof(R) == 4) {
+        if (mode == 3) {
+            shear_tile<KPX, SH_PXB>(jobs[jb], sg, tile, lds[0], lds[1]);
+            return;
+        }
+    }
     int a, b;
     if (mode == 0) {
         a = (tile / nt3) * TA;
@@ -664,7 +875,10 @@ __global__ __launch_bounds__(256) void k_lagft_nyq(const LagJob<R> *__restrict__
    MODE 1: the value sits at (i, j) = (-k2, k2 + k3) mod L, in listed
    tile tmap[I][J] of (i, j) or, when I > J, of (j, i) (symmetric).
    MODE 2: listed tile tmap[a][b] of (k2, k3), or of (k3, k2) for a tile of
-   [0, L/2)^2 below its diagonal (TA = TB). */
+   [0, L/2)^2 below its diagonal (TA = TB).
+   MODE 3: the sheared tiles' (t, k3), t = k2 + floor(k3/2), or its mirror
+   (-1 - t for odd k3, -t for even) when t is past L/2; the fixed points t =
+   L/2 and the row k3 = L/2 from the column block after the partials. */
 template <typename R, int MODE>
 __global__ __launch_bounds__(256) void k_lagft_reduce(const typename LagT<R>::C *__restrict__ part, int nseg, int ntile,
                                                       int L, const int *__restrict__ tmap, double2 *__restrict__ St)
@@ -680,14 +894,27 @@ __global__ __launch_bounds__(256) void k_lagft_reduce(const typename LagT<R>::C 
     /* k3 = L/2 is a tile column (mode 0, TB does not divide L/2), or in mode
        1 a diagonal of the listed tiles (k_lagft_reduce_nyq then writes the
        k2 that no listed tile holds) */
-    const bool col = MODE == 1 || half % TB != 0;
+    const bool col = MODE == 1 || MODE == 3 || half % TB != 0;
+    const C *nyq = part + (size_t)nseg * tsz;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int k2 = X + ty + 8 * q;
         double sx = 0.0, sy = 0.0;
         if (k2 < L && (k3 < half || (col && k3 == half))) {
             long off = 0;
-            if (MODE == 2) {
+            int ni = -1;
+            if (MODE == 3) {
+                const int fl = k3 >> 1;
+                int t = (k2 + fl) % L;
+                off = -1;
+                if (k3 == half) {
+                    ni = t <= half ? k2 : (3 * (L / 4) - t + L) % L;
+                } else {
+                    if (t > half || (t == half && (k3 & 1))) t = (k3 & 1) ? L - 1 - t : L - t;
+                    if (t == half) ni = half - fl;
+                    else off = (long)((t / TA) * (L / 256) + k3 / TB) * (TA * TB) + (t % TA) * TB + (k3 % TB);
+                }
+            } else if (MODE == 2) {
                 int p2 = k2, p3 = k3, a = k2 / TA, b = k3 / TB;
                 if (a > b && (a + 1) * TA <= half) {
                     p2 = k3;
@@ -712,6 +939,11 @@ __global__ __launch_bounds__(256) void k_lagft_reduce(const typename LagT<R>::C 
             }
             for (int sg = 0; sg < nseg && off >= 0; ++sg) {
                 const C v = part[sg * tsz + off];
+                sx += (double)v.x;
+                sy += (double)v.y;
+            }
+            for (int sg = 0; sg < nseg && ni >= 0; ++sg) {
+                const C v = nyq[(size_t)sg * L + ni];
                 sx += (double)v.x;
                 sy += (double)v.y;
             }
@@ -1080,6 +1312,15 @@ double lag_distinct_r(int r1, int r2, int r3)
 }
 double lag_distinct(const amb_tri_req &q) { return lag_distinct_r(q.r1, q.r2, q.r3); }
 
+/* sums outside whole tiles: the k3 = L/2 column (or the k2 of it that a
+   pair-tile job leaves to k_lagft_nyq); sheared tiles: the row k3 = L/2
+   (t <= L/2) and the fixed points t = L/2 of the even rows */
+double edge_cells(int mode, int L, const int *nyql, int nnyq)
+{
+    if (mode == 3) return (double)(L / 2 + 1 + L / 4);
+    return (double)(nyql ? nnyq : L);
+}
+
 /* the grid a triangle's sums run on: the smallest alias-free grid of its
    three rings (amb_compute_grid), or Ng(r1) when reduced grids are off */
 long compute_grid(const amb_ctx *c, const Cube &cb, int r1, int r2, int r3)
@@ -1103,6 +1344,13 @@ int choose_tiles(amb_ctx *c, int L, bool fp64, bool r12, bool r23, int *mode, in
     *tmap = nullptr;
     *nyql = nullptr;
     *nnyq = 0;
+    if (!fp64 && r12 && !c->no_sym && !c->no_shear && L % 256 == 0) {
+        /* sheared (t, k3) tiles: exactly the distinct sums, edges folded */
+        *mode = 3;
+        *ntile = (L / 256) * (L / 256);
+        *nyq_sep = 0;
+        return AMB_OK;
+    }
     if (!fp64 && r12 && !c->no_sym && L % 128 == 0) {
         auto it = c->lag_pairs_tab.find(L);
         if (it == c->lag_pairs_tab.end()) {
@@ -1312,13 +1560,16 @@ int amb::lagft_run(amb_ctx *c, const amb_tri_req *req, int n, double t0, double 
                 else if (j.mode == 2)
                     hipLaunchKernelGGL((k_lagft_reduce<float, 2>), gr, dim3(256), 0, c->stream, g.part, j.nseg,
                                        j.ntile, L, j.tmap, St);
+                else if (j.mode == 3)
+                    hipLaunchKernelGGL((k_lagft_reduce<float, 3>), gr, dim3(256), 0, c->stream, g.part, j.nseg,
+                                       j.ntile, L, nullptr, St);
                 else
                     hipLaunchKernelGGL((k_lagft_reduce<float, 0>), gr, dim3(256), 0, c->stream, g.part, j.nseg,
                                        j.ntile, L, nullptr, St);
                 /* the column rode in the last tile (modes 0 and 2, TB | L/2),
                    was a tile column (TB does not divide L/2), or had its own
                    kernel */
-                const bool folded = j.mode != 1 && (L / 2) % tile_b<float>() == 0;
+                const bool folded = (j.mode == 0 || j.mode == 2) && (L / 2) % tile_b<float>() == 0;
                 if (j.nyq_sep || folded)
                     hipLaunchKernelGGL(k_lagft_reduce_nyq<float>, dim3((unsigned)((L + 255) / 256)), dim3(256), 0,
                                        c->stream, g.nyq, j.nseg, L, j.nyql, j.nnyq, St);
@@ -1387,7 +1638,8 @@ int amb::lagft_run(amb_ctx *c, const amb_tri_req *req, int n, double t0, double 
             /* issued: whole tiles, plus the k3 = L/2 column (or its k2 left
                to k_lagft_nyq by pair tiles) */
             c->st.flop_contract_exec +=
-                14.0 * ((double)j.ntile * tile_a<float>() * tile_b<float>() + (j.nyql ? j.nnyq : L)) * (double)j.P;
+                14.0 * ((double)j.ntile * tile_a<float>() * tile_b<float>() + edge_cells(j.mode, L, j.nyql, j.nnyq)) *
+                (double)j.P;
         }
         prev = std::move(w);
         w.clear();
@@ -1631,9 +1883,11 @@ hipError_t amb::lag_launch_finish(amb_ctx *c, hipStream_t s, const LagTri &t, co
         hipLaunchKernelGGL((k_lagft_reduce<float, 1>), gr, dim3(256), 0, s, pt, t.nseg, t.ntile, L, t.tmap, st);
     else if (t.mode == 2)
         hipLaunchKernelGGL((k_lagft_reduce<float, 2>), gr, dim3(256), 0, s, pt, t.nseg, t.ntile, L, t.tmap, st);
+    else if (t.mode == 3)
+        hipLaunchKernelGGL((k_lagft_reduce<float, 3>), gr, dim3(256), 0, s, pt, t.nseg, t.ntile, L, nullptr, st);
     else
         hipLaunchKernelGGL((k_lagft_reduce<float, 0>), gr, dim3(256), 0, s, pt, t.nseg, t.ntile, L, nullptr, st);
-    const bool folded = t.mode != 1 && (L / 2) % tile_b<float>() == 0;
+    const bool folded = (t.mode == 0 || t.mode == 2) && (L / 2) % tile_b<float>() == 0;
     if (t.nyq_sep || folded)
         hipLaunchKernelGGL(k_lagft_reduce_nyq<float>, dim3((unsigned)((L + 255) / 256)), dim3(256), 0, s, nyq, t.nseg, L,
                            t.nyql, t.nnyq, st);
@@ -1661,5 +1915,6 @@ void amb::lag_account(amb_ctx *c, const LagTri &t, int32_t *cnt)
     c->st.flop_contract_distinct += 2.0 * (double)cb.lag_pairs * (double)(t.ng * t.ng);
     c->st.flop_lagft += 14.0 * (double)L * (double)(L / 2 + 1) * (double)t.P * lag_distinct_r(t.r1, t.r2, t.r3);
     c->st.flop_contract_exec +=
-        14.0 * ((double)t.ntile * tile_a<float>() * tile_b<float>() + (t.nyql ? t.nnyq : L)) * (double)t.P;
+        14.0 * ((double)t.ntile * tile_a<float>() * tile_b<float>() + edge_cells(t.mode, L, t.nyql, t.nnyq)) *
+        (double)t.P;
 }

@@ -149,20 +149,20 @@ def test_serving_equals_batch(ctx):
         ctx.set_precision("bf16x3")
 
 
-@pytest.mark.parametrize("sym", ["r1==r2", "r2==r3"])
+@pytest.mark.parametrize("sym", ["r1==r2", "r1==r2 pair tiles", "r2==r3"])
 def test_symmetric_tilings(ctx, sym):
     """Triangles at a transform length of 512 (256 channels, all lags) on
-    the symmetric tilings: r1 == r2 on the (i, j) pair tiles, r2 == r3 on the
-    (k2, k3) tiles without those below the diagonal of [0, L/2)^2.  Sampled
-    cells against the FP64 oracle, every cell against the full (k2, k3)
-    tiling of a context with the symmetries switched off (AMB_NO_SYM),
-    counts equal."""
+    the symmetric tilings: r1 == r2 on the sheared (t, k3) tiles (default)
+    or on the (i, j) pair tiles (AMB_NO_SHEAR), r2 == r3 on the (k2, k3)
+    tiles without those below the diagonal of [0, L/2)^2.  Sampled cells
+    against the FP64 oracle, every cell against the full (k2, k3) tiling of
+    a context with the symmetries switched off (AMB_NO_SYM), counts equal."""
     import os
     N, nc, nc10 = 64, 256, 255
     lo, hi = A.log_rings(2, 20, 5)
     gv, samp, _ = A.generate(N, nc, nflag=0, seed=12)
     mask = np.ones(nc, np.uint8)
-    if sym == "r1==r2":
+    if sym.startswith("r1==r2"):
         tris = [tuple(int(x) for x in t) for t in A.triangles(lo, hi) if t[0] == t[1]]
     else:
         tris = [tuple(int(x) for x in t) for t in A.triangles(lo, hi) if t[1] == t[2] and t[0] != t[1]]
@@ -172,6 +172,13 @@ def test_symmetric_tilings(ctx, sym):
         plain = A.Context(device=0, precision="lagft")
     finally:
         del os.environ["AMB_NO_SYM"]
+    own = None
+    if sym == "r1==r2 pair tiles":
+        os.environ["AMB_NO_SHEAR"] = "1"
+        try:
+            own = ctx = A.Context(device=0, precision="lagft")
+        finally:
+            del os.environ["AMB_NO_SHEAR"]
     ctx.set_precision("lagft")
     try:
         rng = np.random.default_rng(3)
@@ -198,7 +205,10 @@ def test_symmetric_tilings(ctx, sym):
         ctx.release_cube(cube)
     finally:
         plain.close()
-        ctx.set_precision("bf16x3")
+        if own is not None:
+            own.close()
+        else:
+            ctx.set_precision("bf16x3")
 
 
 def test_compute_grids(ctx):
