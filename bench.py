@@ -61,13 +61,14 @@ TUNE_TABLE = os.path.join(ROOT, "amber_amd", "tune_gfx950.txt")
 # copy.  Counters cannot be read inside a timed run; `traffic` is this total
 # over the step's launches.
 MEASURED_TRAFFIC_STEP = {("cfg3", "bf16x3", "grouped"): 1.909e13,
-                         # lag-Fourier form: k_lagft_tiles + k_lagft_nyq, FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00
-                         ("cfg3", "lagft", "grouped"): 2.172e12}
+                         # lag-Fourier form (compute grids, sheared r1 == r2 tiles): k_lagft_tiles,
+                         # FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00
+                         ("cfg3", "lagft", "grouped"): 1.165e12}
 
 
 def traffic_per_launch(args, st, nsteps):
     tot = MEASURED_TRAFFIC_STEP.get((args.config, args.precision, args.batch_mode))
-    if tot is None or args.tri_limit:
+    if tot is None or args.tri_limit or args.ref_grid:
         return None
     # the measured total covers one pass over every triangle, as the timed steps do
     return round(tot / max(st["n_contract_launch"], 1), -6)
