@@ -302,6 +302,8 @@ struct LagTri {
 int lag_prepare(amb_ctx *c, int cube, int r1, int r2, int r3, LagTri *t);
 size_t lag_part_bytes(const LagTri &t);
 long lag_items(const LagTri &t);
+/* pixels of one lag-Fourier work item (a tile over one pixel segment) */
+long lag_item_px();
 size_t lag_desc_bytes();
 size_t lag_st_cap(); /* the spectrum buffer of any transform length */
 void lag_fill_range(amb_ctx *c, const LagTri &t, void *part, long i0, long item0, void *h);

@@ -47,9 +47,10 @@ constexpr size_t UPLOAD_STAGING = (size_t)256 << 20; /* at most; a sixteenth of 
 /* work of one iteration per resident workgroup, in items x pixels: two
    items of a 512^2 grid (about 40 ms of the split form) */
 constexpr double ITER_PX_PER_BLOCK = 2.0 * 512 * 512;
-/* a lag-Fourier item (one 128 x 128 tile over 2048 pixels, about 1.3 ms of
-   a workgroup) in the same units: about 32 items per workgroup and iteration */
-constexpr double LAG_ITEM_UNITS = 8.0 * 2048;
+/* a lag-Fourier item (one 128 x 128 tile over lag_item_px() pixels) in the
+   same units: eight pixel units per item pixel (about 32 items of 2048
+   pixels, or 64 of 1024, per workgroup and iteration) */
+constexpr double LAG_UNITS_PER_PX = 8.0;
 
 size_t al(size_t x) { return (x + ALIGN - 1) / ALIGN * ALIGN; }
 
@@ -343,7 +344,7 @@ void admit(Srv *s)
             if (jb.lag) {
                 jb.items = lag_items(jb.lt);
                 jb.nitems = 0;
-                jb.px = LAG_ITEM_UNITS;
+                jb.px = LAG_UNITS_PER_PX * (double)lag_item_px();
             } else {
                 jb.items = amb_contract_items(j.nact);
                 jb.nitems = j.want_norm ? jb.items : 0;
