@@ -509,8 +509,9 @@ int get_planes(amb_ctx *c, const PlaneSrc &src, int cube_id, int kind, int r, in
     if (ngref <= 0) ngref = ng;
     RingGeom g;
     if ((rc = ring_geom(c, *cb, r, ng, &g, ngref))) return rc;
-    /* columns j < jcnt of the half spectrum j <= ng/2 */
-    if (g.jcnt > ng / 2 + 1) return fail(c, AMB_EARG, "ring %d needs %d columns on a grid of %d", r, g.jcnt, ng);
+    /* columns j < jcnt <= ng/2: the row pass fills columns k and ng - k of
+       its Hermitian extension from column k < jcnt, which must not meet */
+    if (g.jcnt > ng / 2) return fail(c, AMB_EARG, "ring %d needs %d columns on a grid of %d", r, g.jcnt, ng);
     for (auto &p : *src.list)
         if (p.cube == cube_id && p.kind == kind && p.ng == ng && p.jcnt == g.jcnt && p.lo == g.lo && p.hi == g.hi) {
             p.last = ++c->tick;

@@ -133,7 +133,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_ring_rows(const Cx<R> *__restri
     const int W = 2 * nteam, x0 = W * bxi;
     const int xa = x0 + 2 * team, xb = xa + 1;
     const bool live = pli < npl && xb < n;
-    /* the columns no ring cell reaches: jcnt <= n / 3 */
+    /* the columns no ring cell reaches: jcnt <= n / 2 */
     for (int k = jcnt + t; k <= n - jcnt; k += 64) bx[k] = cx((R)0, (R)0);
     if (pli < npl) {
         const C *Gp = G + (size_t)pli * jcnt * n;

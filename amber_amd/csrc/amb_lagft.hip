@@ -111,7 +111,9 @@ struct LagJob {
     int lhp, L, nseg, seg;      /* tile segments and their pixels */
     int mode;                   /* 0: (k2, k3) tiles; 1: r1 == r2, (i, j) tiles, i <= j;
                                    2: r2 == r3, listed (k2, k3) tiles (tiles below the
-                                   diagonal of [0, L/2)^2 dropped: S is symmetric there) */
+                                   diagonal of [0, L/2)^2 dropped: S is symmetric there);
+                                   3: r1 == r2, sheared (t, k3) tiles; 4: r2 == r3, mode 2's
+                                   tiles without those past the anti-diagonal for k2 >= L/2 */
     int ntile;                  /* tiles per segment */
     int nyq_sep;                /* the k3 = L/2 column is summed by k_lagft_nyq */
 };
@@ -622,6 +624,10 @@ __global__ __launch_bounds__(256, LagT<R>::WPS) void k_lagft_tiles(const LagJob<
     const int half = L / 2;
     const bool nyq_col = half % TB != 0;
     const bool nyqx = FOLD && mode != 1 && !nyq_col && b + TB == half;
+    /* mode 4: the tile (I - 1, J) left of an unlisted corner (I, J), I + J =
+       L / 128, also sums the corner's one needed cell S(a + TA, b), on the
+       anti-diagonal k2 + k3 = L (G(a + TA) takes one more staging slot) */
+    const bool cornerx = FOLD && mode == 4 && a >= half && a / TA + b / TB == 2 * nt3 - 1 && b >= TB;
 
     /* the values this thread stages per pixel: the spectrum, its index
        (stored half: k <= L/2, else the conjugate at L - k) and whether the
@@ -666,6 +672,10 @@ __global__ __launch_bounds__(256, LagT<R>::WPS) void k_lagft_tiles(const LagJob<
             at(u, jobs[jb].H, half, false); /* H(L/2), in the extra unit */
             r = 0;
             base = UG + UH + UF;
+        } else if (FOLD && sl == NSLOT + 1 && cornerx) {
+            at(u, jobs[jb].G, a + TA, false); /* G(a + TA), beside it */
+            r = 1;
+            base = UG + UH + UF;
         } else {
             continue;
         }
@@ -705,10 +715,15 @@ __global__ __launch_bounds__(256, LagT<R>::WPS) void k_lagft_tiles(const LagJob<
     for (int u = 0; u < BU; ++u)
 #pragma unroll
         for (int v = 0; v < BV; ++v) acr[u][v] = aci[u][v] = (R)0;
-    /* k3 = L/2 column of k2 = a + tid (tid < TA, whole waves) */
+    /* k3 = L/2 column of k2 = a + tid (tid < TA, whole waves); thread TA
+       the mode 4 corner cell: G(a + TA), H(b), conj F(a + b + TA) */
     const bool nx = FOLD && nyqx && tid < TA;
-    const int offg = (tid / VPU) * BLKB + (tid % VPU) * (int)sizeof(C);
-    const int offf = (UG + UH + (tid + TB) / VPU) * BLKB + ((tid + TB) % VPU) * (int)sizeof(C);
+    const bool cx4 = FOLD && cornerx && tid == TA;
+    const int offg = cx4 ? (UG + UH + UF) * BLKB + (int)sizeof(C) : (tid / VPU) * BLKB + (tid % VPU) * (int)sizeof(C);
+    const int offf = cx4 ? (UG + UH + TA / VPU) * BLKB + (TA % VPU) * (int)sizeof(C)
+                         : (UG + UH + (tid + TB) / VPU) * BLKB + ((tid + TB) % VPU) * (int)sizeof(C);
+    const int offh = cx4 ? UG * BLKB : (UG + UH + UF) * BLKB;
+    const bool ex = nx || cx4;
     R anr = (R)0, ani = (R)0;
 
     const long nch = (xe - xb + KPX - 1) / KPX;
@@ -738,9 +753,9 @@ __global__ __launch_bounds__(256, LagT<R>::WPS) void k_lagft_tiles(const LagJob<
                     acr[u][v] = fma(f[u + v].x, tr, fma(-f[u + v].y, ti, acr[u][v]));
                     aci[u][v] = fma(f[u + v].x, ti, fma(f[u + v].y, tr, aci[u][v]));
                 }
-            if (FOLD && nx) {
+            if (FOLD && ex) {
                 const C gg = *(const C *)(px + offg), ff = *(const C *)(px + offf);
-                const C hh = *(const C *)(px + (UG + UH + UF) * BLKB);
+                const C hh = *(const C *)(px + offh);
                 const R tr = fma(gg.x, hh.x, -gg.y * hh.y), ti = fma(gg.x, hh.y, gg.y * hh.x);
                 anr = fma(ff.x, tr, fma(-ff.y, ti, anr));
                 ani = fma(ff.x, ti, fma(ff.y, tr, ani));
@@ -749,11 +764,11 @@ __global__ __launch_bounds__(256, LagT<R>::WPS) void k_lagft_tiles(const LagJob<
         if (ch + 1 < nch) stash(lds[(ch + 1) & 1]);
         __syncthreads();
     }
-    if (FOLD && nx && a + tid < L) {
+    if (FOLD && ex && (cx4 || a + tid < L)) {
         C w;
         w.x = anr;
         w.y = ani;
-        jobs[jb].nyq[(size_t)sg * L + a + tid] = w;
+        jobs[jb].nyq[(size_t)sg * L + (cx4 ? a + TA : a + tid)] = w;
     }
     C *out = jobs[jb].part + ((size_t)sg * ntile + tile) * (TA * TB);
 #pragma unroll
@@ -878,6 +893,10 @@ __global__ __launch_bounds__(256) void k_lagft_nyq(const LagJob<R> *__restrict__
    tile tmap[I][J] of (i, j) or, when I > J, of (j, i) (symmetric).
    MODE 2: listed tile tmap[a][b] of (k2, k3), or of (k3, k2) for a tile of
    [0, L/2)^2 below its diagonal (TA = TB).
+   MODE 4: as MODE 2 on [0, L/2)^2; for k2 >= L/2 the cells past the
+   anti-diagonal (k2 + k3 > L) from their conjugate mirror (L - k3, L - k2),
+   the anti-diagonal corners of unlisted tiles and the k3 = L/2 column from
+   the column block.
    MODE 3: the sheared tiles' (t, k3), t = k2 + floor(k3/2), or its mirror
    (-1 - t for odd k3, -t for even) when t is past L/2; the fixed points t =
    L/2 and the row k3 = L/2 from the column block after the partials. */
@@ -896,7 +915,7 @@ __global__ __launch_bounds__(256) void k_lagft_reduce(const typename LagT<R>::C 
     /* k3 = L/2 is a tile column (mode 0, TB does not divide L/2), or in mode
        1 a diagonal of the listed tiles (k_lagft_reduce_nyq then writes the
        k2 that no listed tile holds) */
-    const bool col = MODE == 1 || MODE == 3 || half % TB != 0;
+    const bool col = MODE == 1 || MODE == 3 || MODE == 4 || half % TB != 0;
     const C *nyq = part + (size_t)nseg * tsz;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -905,7 +924,30 @@ __global__ __launch_bounds__(256) void k_lagft_reduce(const typename LagT<R>::C 
         if (k2 < L && (k3 < half || (col && k3 == half))) {
             long off = 0;
             int ni = -1;
-            if (MODE == 3) {
+            bool cj = false;
+            if (MODE == 4) {
+                int p2 = k2, p3 = k3;
+                if (p2 >= half && p2 + p3 > L) {
+                    const int q2 = L - p3;
+                    p3 = L - p2;
+                    p2 = q2;
+                    cj = true;
+                }
+                off = -1;
+                if (p3 == half) {
+                    ni = p2;
+                } else {
+                    int a = p2 / TA, b = p3 / TB;
+                    if (p2 < half && a > b) {
+                        const int t = p2; p2 = p3; p3 = t;
+                        a = p2 / TA;
+                        b = p3 / TB;
+                    }
+                    const int tl = tmap[a * nt3 + b];
+                    if (tl < 0) ni = p2; /* a corner: p2 + p3 = L */
+                    else off = (long)tl * (TA * TB) + (p2 - a * TA) * TB + (p3 - b * TB);
+                }
+            } else if (MODE == 3) {
                 const int fl = k3 >> 1;
                 int t = (k2 + fl) % L;
                 off = -1;
@@ -949,6 +991,7 @@ __global__ __launch_bounds__(256) void k_lagft_reduce(const typename LagT<R>::C 
                 sx += (double)v.x;
                 sy += (double)v.y;
             }
+            if (cj) sy = -sy;
         }
         tr[ty + 8 * q][tx] = make_double2(sx, sy);
     }
@@ -1320,6 +1363,7 @@ double lag_distinct(const amb_tri_req &q) { return lag_distinct_r(q.r1, q.r2, q.
 double edge_cells(int mode, int L, const int *nyql, int nnyq)
 {
     if (mode == 3) return (double)(L / 2 + 1 + L / 4);
+    if (mode == 4) return (double)((L / 256) * 128 + 128 + (L / 256 - 1)); /* the folded column of the listed last-k3 tiles, corners */
     return (double)(nyql ? nnyq : L);
 }
 
@@ -1391,6 +1435,36 @@ int choose_tiles(amb_ctx *c, int L, bool fp64, bool r12, bool r23, int *mode, in
                 *nnyq = nm;
             }
         }
+    }
+    if (!fp64 && !r12 && r23 && !c->no_sym && L % 256 == 0) {
+        /* (k2, k3) tiles: on [0, L/2)^2 those on and above its diagonal, for
+           k2 >= L/2 those with a cell on or before the anti-diagonal k2 + k3
+           = L other than a corner (I + J < L/128); tables keyed by -L */
+        auto it = c->lag_r23_tab.find(-L);
+        if (it == c->lag_r23_tab.end()) {
+            const int n2 = tiles2<float>(L), n3 = tiles3<float>(L);
+            std::vector<int2> tl;
+            std::vector<int> tm((size_t)n2 * n3, -1);
+            for (int a = 0; a < n2; ++a)
+                for (int b = 0; b < n3; ++b)
+                    if (a < n3 ? a <= b : a + b <= 2 * n3 - 1) {
+                        tm[(size_t)a * n3 + b] = (int)tl.size();
+                        tl.push_back(make_int2(a, b));
+                    }
+            void *d = nullptr;
+            const size_t tb = sizeof(int2) * tl.size(), mb = sizeof(int) * tm.size();
+            const int rc = dev_alloc(c, &d, tb + mb, "lag-Fourier tile tables");
+            if (rc) return rc;
+            HIPCHK(c, hipMemcpy(d, tl.data(), tb, hipMemcpyHostToDevice));
+            HIPCHK(c, hipMemcpy((char *)d + tb, tm.data(), mb, hipMemcpyHostToDevice));
+            it = c->lag_r23_tab.emplace(-L, std::make_tuple(d, (int)tl.size(), tb + mb)).first;
+        }
+        *mode = 4;
+        *ntile = std::get<1>(it->second);
+        *tlist = (const int2 *)std::get<0>(it->second);
+        *tmap = (const int *)((const char *)std::get<0>(it->second) + sizeof(int2) * *ntile);
+        *nyq_sep = 0;
+        return AMB_OK;
     }
     if (!fp64 && !r12 && r23 && !c->no_sym) {
         /* (k2, k3) tiles without those of [0, L/2)^2 below its diagonal */
@@ -1565,6 +1639,9 @@ int amb::lagft_run(amb_ctx *c, const amb_tri_req *req, int n, double t0, double 
                 else if (j.mode == 3)
                     hipLaunchKernelGGL((k_lagft_reduce<float, 3>), gr, dim3(256), 0, c->stream, g.part, j.nseg,
                                        j.ntile, L, nullptr, St);
+                else if (j.mode == 4)
+                    hipLaunchKernelGGL((k_lagft_reduce<float, 4>), gr, dim3(256), 0, c->stream, g.part, j.nseg,
+                                       j.ntile, L, j.tmap, St);
                 else
                     hipLaunchKernelGGL((k_lagft_reduce<float, 0>), gr, dim3(256), 0, c->stream, g.part, j.nseg,
                                        j.ntile, L, nullptr, St);
@@ -1889,6 +1966,8 @@ hipError_t amb::lag_launch_finish(amb_ctx *c, hipStream_t s, const LagTri &t, co
         hipLaunchKernelGGL((k_lagft_reduce<float, 2>), gr, dim3(256), 0, s, pt, t.nseg, t.ntile, L, t.tmap, st);
     else if (t.mode == 3)
         hipLaunchKernelGGL((k_lagft_reduce<float, 3>), gr, dim3(256), 0, s, pt, t.nseg, t.ntile, L, nullptr, st);
+    else if (t.mode == 4)
+        hipLaunchKernelGGL((k_lagft_reduce<float, 4>), gr, dim3(256), 0, s, pt, t.nseg, t.ntile, L, t.tmap, st);
     else
         hipLaunchKernelGGL((k_lagft_reduce<float, 0>), gr, dim3(256), 0, s, pt, t.nseg, t.ntile, L, nullptr, st);
     const bool folded = (t.mode == 0 || t.mode == 2) && (L / 2) % tile_b<float>() == 0;

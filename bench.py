@@ -189,6 +189,12 @@ def main():
             tris = tris[np.sort(np.argsort(-ngs, kind="stable")[: args.tri_limit])]
         else:
             tris = tris[: args.tri_limit]
+    if args.precision == "lagft" and not args.ref_grid:
+        # triangles grouped by compute grid (then in (r1, r2, r3) order), so
+        # the contiguous slices share more (ring, grid) sets; every step
+        # still builds all the sets it reads
+        g = tri_grids(A, tris, hi, args.precision)
+        tris = tris[np.lexsort((tris[:, 2], tris[:, 1], tris[:, 0], g))]
     t0 = time.time()
     gv, samp, uf = A.generate(N, nc, nflag=cfg["nflag"], seed=cfg["seed"])
     fk = 1 if cfg["nflag"] else 0
@@ -390,8 +396,9 @@ def main():
                    "evals_per_pass": int(st["evals"]),
                    "sum_grid": ("per triangle the smallest alias-free grid of its three rings, sums scaled by "
                                 "(Ng/n)^2" if lagft and not args.ref_grid else "Ng(r1) of the reference"),
-                   "step": "one slice of the triangle list (%d slices of equal contraction cost); the %d timed "
-                           "steps cover every triangle once" % (nsteps, nsteps)},
+                   "step": "one slice of the triangle list%s (%d slices of equal contraction cost); the %d timed "
+                           "steps cover every triangle once and each rebuilds every plane set it reads"
+                           % (" grouped by compute grid" if lagft and not args.ref_grid else "", nsteps, nsteps)},
         "pass_s": round(elapsed, 3),
         "roofline": roof, "stages": stages,
         "triangle_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
@@ -831,6 +838,12 @@ def copy_calib(mib):
     print(json.dumps({"copy_calib_bytes": n, "dispatches": 3}), flush=True)
 
 
+def canonical(tris):
+    """the triangle list in the reference's (r1, r2, r3) order, whatever
+    order the timed slices use"""
+    return tris[np.lexsort((tris[:, 2], tris[:, 1], tris[:, 0]))]
+
+
 def fp64_error(A, ctx, cube, cfg, gv, samp, uf, fk, lo, hi, tris, deadline=None, cache=None):
     """Relative error vs the FP64 oracle on sampled lag cells of four
     triangles (Ng quantiles, the largest first); a triangle whose oracle run
@@ -840,6 +853,7 @@ def fp64_error(A, ctx, cube, cfg, gv, samp, uf, fk, lo, hi, tris, deadline=None,
     import oracle as O
     N, nc, nc10 = cfg["N"], cfg["nc"], cfg["nc10"]
     mask = O.chan_mask(nc, fk, uf)
+    tris = canonical(tris)
     ngs = np.array([A.ng(hi[t[0]]) for t in tris])
     # triangles at the 25/50/75 % quantiles of Ng and the largest; a bin whose
     # three rings cannot close a triangle on the grid has B == 0 exactly and
@@ -893,6 +907,7 @@ def cpu_baseline(A, cfg, gv, samp, uf, fk, lo, hi, tris, budget_s):
     """Reference-algorithm CPU path (FP64 c2r + FP32 triplet loop, OpenMP),
     timed on a bounded sample and extrapolated linearly in V*Ng^2."""
     import oracle as O
+    tris = canonical(tris)
     N, nc, nc10 = cfg["N"], cfg["nc"], cfg["nc10"]
     mask = O.chan_mask(nc, fk, uf)
     threads = min(16, os.cpu_count() or 1)
