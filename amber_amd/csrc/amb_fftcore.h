@@ -9,7 +9,7 @@
  * stage.  A stage of radix R gives each thread butterflies j, j + T, ...
  * (T threads per team), whose (group, offset) = (j / Ns, j % Ns) indices
  * advance by a constant step: no integer division inside the loops.
- * Butterflies are register codelets of compile-time size: 2, 3, 4 and the
+ * Butterflies are register codelets of compile-time size: 2, 3, 4, 8 and the
  * odd primes up to 47, the primes as a symmetric-pair DFT whose cos/sin
  * coefficients are read from the transform's own twiddle table
  * (tw[t n/R] = e^{2 pi i t/R}).  Any larger prime factor runs a direct stage
@@ -30,13 +30,14 @@
 
 namespace fftc {
 
-/* plan of length n: radix-4 stages, a radix-2 stage, radix-3 stages, then
-   the other prime factors in ascending order */
+/* plan of length n: radix-8 stages, a radix-4 stage, a radix-2 stage,
+   radix-3 stages, then the other prime factors in ascending order */
 inline void make_plan(int n, FftPlan *pl)
 {
     pl->n = n;
     pl->nst = 0;
     int m = n;
+    while (m % 8 == 0 && pl->nst < AMB_MAX_RADIX_STAGES) { pl->rad[pl->nst++] = 8; m /= 8; }
     while (m % 4 == 0 && pl->nst < AMB_MAX_RADIX_STAGES) { pl->rad[pl->nst++] = 4; m /= 4; }
     while (m % 2 == 0 && pl->nst < AMB_MAX_RADIX_STAGES) { pl->rad[pl->nst++] = 2; m /= 2; }
     while (m % 3 == 0 && pl->nst < AMB_MAX_RADIX_STAGES) { pl->rad[pl->nst++] = 3; m /= 3; }
@@ -156,6 +157,35 @@ struct Bfly<4, DIR, C> {
     }
 };
 
+template <int DIR, typename C>
+struct Bfly<8, DIR, C> {
+    /* two 4-point DFTs of the even and odd inputs, combined with w^q,
+       w = e^{DIR 2 pi i / 8}: y_q = E_q + w^q O_q, y_{q+4} = E_q - w^q O_q */
+    FFTC_HD static void run(C *v, const C *, int, C *yo, int ys)
+    {
+        typedef decltype(v[0].x) T;
+        const T c = (T)0.70710678118654752440; /* cos(pi / 4) */
+        const C es02 = cadd(v[0], v[4]), ed02 = csub(v[0], v[4]);
+        const C es13 = cadd(v[2], v[6]), er13 = rot90<DIR>(csub(v[2], v[6]));
+        const C os02 = cadd(v[1], v[5]), od02 = csub(v[1], v[5]);
+        const C os13 = cadd(v[3], v[7]), or13 = rot90<DIR>(csub(v[3], v[7]));
+        const C e0 = cadd(es02, es13), e1 = cadd(ed02, er13), e2 = csub(es02, es13), e3 = csub(ed02, er13);
+        const C o0 = cadd(os02, os13), o1 = cadd(od02, or13), o2 = csub(os02, os13), o3 = csub(od02, or13);
+        const C r1 = rot90<DIR>(o1), r3 = rot90<DIR>(o3);
+        const C w1 = cx(c * (o1.x + r1.x), c * (o1.y + r1.y));  /* w o1 */
+        const C w2 = rot90<DIR>(o2);                            /* w^2 o2 */
+        const C w3 = cx(c * (r3.x - o3.x), c * (r3.y - o3.y));  /* w^3 o3 */
+        yo[0] = cadd(e0, o0);
+        yo[ys] = cadd(e1, w1);
+        yo[2 * ys] = cadd(e2, w2);
+        yo[3 * ys] = cadd(e3, w3);
+        yo[4 * ys] = csub(e0, o0);
+        yo[5 * ys] = csub(e1, w1);
+        yo[6 * ys] = csub(e2, w2);
+        yo[7 * ys] = csub(e3, w3);
+    }
+};
+
 /* One radix-R stage of a team's transform: x -> y (Stockham order). */
 template <int R, int DIR, typename C>
 FFTC_HD void stage(const C *__restrict__ x, C *__restrict__ y, const C *__restrict__ tw, int n,
@@ -251,16 +281,19 @@ FFTC_HD void stage_direct_dft(const C *__restrict__ x, C *__restrict__ y, const 
 template <int DIR, int MAXP = 47, typename C>
 __device__ C *transform(C *x, C *y, const C *tw, const FftPlan &pl, int t, int T)
 {
+    static_assert(MAXP >= 5, "radices 2, 3, 4, 5 and 8 are codelets in every kernel");
     const int n = pl.n;
     int Ns = 1;
     for (int s = 0; s < pl.nst; ++s) {
         const int R = pl.rad[s];
-        /* codelets above MAXP are not instantiated (register pressure of the
-           kernel): such factors take the direct stage */
-        switch (R > MAXP ? 0 : R) {
+        /* prime codelets above MAXP are not instantiated (register pressure
+           of the kernel): such factors take the direct stage, which is for
+           odd primes only; radix 8 is always a codelet */
+        switch (R > MAXP && R != 8 ? 0 : R) {
         case 2: stage<2, DIR>(x, y, tw, n, Ns, t, T); break;
         case 3: stage<3, DIR>(x, y, tw, n, Ns, t, T); break;
         case 4: stage<4, DIR>(x, y, tw, n, Ns, t, T); break;
+        case 8: stage<8, DIR>(x, y, tw, n, Ns, t, T); break;
         case 5: if constexpr (MAXP >= 5) stage<5, DIR>(x, y, tw, n, Ns, t, T); break;
         case 7: if constexpr (MAXP >= 7) stage<7, DIR>(x, y, tw, n, Ns, t, T); break;
         case 11: if constexpr (MAXP >= 11) stage<11, DIR>(x, y, tw, n, Ns, t, T); break;
@@ -330,7 +363,7 @@ FFTC_HD void stage_ip(C *__restrict__ x, const C *__restrict__ tw, int n, int Ns
 }
 
 /* In-place transform of a team's buffer x (input in digit-reversed order,
- * output in natural order) for plans of radices 2, 3, 4 and 5 only (the
+ * output in natural order) for plans of radices 2, 3, 4, 5 and 8 only (the
  * {2,3,5}-smooth lengths of the channel axis); one workgroup barrier per
  * stage, as transform(). */
 template <int DIR, typename C>
@@ -344,6 +377,7 @@ __device__ void transform_ip(C *x, const C *tw, const FftPlan &pl, int t, int T)
         case 2: stage_ip<2, DIR>(x, tw, n, Ns, t, T); break;
         case 3: stage_ip<3, DIR>(x, tw, n, Ns, t, T); break;
         case 4: stage_ip<4, DIR>(x, tw, n, Ns, t, T); break;
+        case 8: stage_ip<8, DIR>(x, tw, n, Ns, t, T); break;
         default: stage_ip<5, DIR>(x, tw, n, Ns, t, T); break;
         }
         __syncthreads();

@@ -1231,8 +1231,9 @@ hipError_t amb::lag_launch_spectra(amb_ctx *c, hipStream_t s, const BuildOp &op)
         const size_t setw = (size_t)pitch * cb.nact();
         const unsigned *slot_in = (const unsigned *)((const float *)op.src + setw);
         unsigned *slot_out = (unsigned *)((float2 *)op.dst + (size_t)P * cb.lag_lhp);
+        /* the in-place transform's radices: 2, 3, 4, 5 and 8 */
         bool smooth5 = true;
-        for (int s2 = 0; s2 < pl.nst; ++s2) smooth5 = smooth5 && pl.rad[s2] <= 5;
+        for (int s2 = 0; s2 < pl.nst; ++s2) smooth5 = smooth5 && (pl.rad[s2] <= 5 || pl.rad[s2] == 8);
         const size_t lds_blk = sizeof(float2) * 4 * (size_t)L + sizeof(int) * (size_t)cb.nact();
         if (cb.nact() <= SPB_PLANES && smooth5 && lds_blk <= 65536) {
             const long nblk = ((P + 31) / 32 + 7) / 8 * 8;

@@ -27,7 +27,7 @@ static int run(int n, int TT, int maxp, const double *in, double *out)
         const int R = pl.rad[s];
         /* primes above maxp take the direct stage (its two steps are
            separated by the barrier every thread reaches) */
-        const bool direct = R > maxp || R > 47;
+        const bool direct = (R > maxp && R != 8) || R > 47;
         if (direct) {
             for (int t = 0; t < TT; ++t) fftc::stage_direct_tw<DIR>(px, tw.data(), n, R, Ns, t, TT);
             for (int t = 0; t < TT; ++t) fftc::stage_direct_dft<DIR>(px, py, tw.data(), n, R, Ns, t, TT);
@@ -35,7 +35,7 @@ static int run(int n, int TT, int maxp, const double *in, double *out)
         for (int t = 0; t < TT && !direct; ++t) {
             switch (R) {
 #define CASE(r) case r: fftc::stage<r, DIR>(px, py, tw.data(), n, Ns, t, TT); break;
-            CASE(2) CASE(3) CASE(4) CASE(5) CASE(7) CASE(11) CASE(13) CASE(17) CASE(19) CASE(23) CASE(29)
+            CASE(2) CASE(3) CASE(4) CASE(5) CASE(7) CASE(8) CASE(11) CASE(13) CASE(17) CASE(19) CASE(23) CASE(29)
             CASE(31) CASE(37) CASE(41) CASE(43) CASE(47)
 #undef CASE
             default: break;
@@ -67,7 +67,7 @@ static int run_ip(int n, int TT, const double *in, double *out)
     FftPlan pl;
     fftc::make_plan(n, &pl);
     for (int s = 0; s < pl.nst; ++s)
-        if (pl.rad[s] > 5) return -2;
+        if (pl.rad[s] > 5 && pl.rad[s] != 8) return -2;
     std::vector<C> x(n), tw(n);
     for (int m = 0; m < n; ++m) {
         const double a = 2.0 * M_PI * (double)m / (double)n;
@@ -82,6 +82,7 @@ static int run_ip(int n, int TT, const double *in, double *out)
             case 2: fftc::stage_ip<2, DIR>(x.data(), tw.data(), n, Ns, t, TT); break;
             case 3: fftc::stage_ip<3, DIR>(x.data(), tw.data(), n, Ns, t, TT); break;
             case 4: fftc::stage_ip<4, DIR>(x.data(), tw.data(), n, Ns, t, TT); break;
+            case 8: fftc::stage_ip<8, DIR>(x.data(), tw.data(), n, Ns, t, TT); break;
             default: fftc::stage_ip<5, DIR>(x.data(), tw.data(), n, Ns, t, TT); break;
             }
         }

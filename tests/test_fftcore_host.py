@@ -2,7 +2,7 @@
 stage and butterfly of the device code, with a team's threads run one after
 another per stage, against numpy's FFT and, for small n, a direct DFT.
 
-Lengths cover each register codelet (2, 3, 4 and the odd primes up to 47),
+Lengths cover each register codelet (2, 3, 4, 8 and the odd primes up to 47),
 the direct stage of larger primes (109, 181, 349), the transform lengths the
 ring c2r meets at the configured grids (612, 362, 218, 1176, ...), the
 lag-Fourier length 1536 and the post-processing length 1519 = 7^2 31, with
@@ -45,8 +45,8 @@ def run(lib, x, direction, dbl, team, inplace=False, maxp=47):
     return o[0::2] + 1j * o[1::2]
 
 
-LENGTHS = [2, 3, 4, 5, 7, 8, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 60, 86, 109, 128, 218, 258, 362,
-           396, 470, 492, 612, 698, 1176, 1519, 1536]
+LENGTHS = [2, 3, 4, 5, 7, 8, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 60, 64, 86, 109, 128, 218, 258, 362,
+           396, 448, 470, 492, 512, 612, 698, 1176, 1519, 1536, 2048]
 
 
 @pytest.mark.parametrize("n", LENGTHS)
@@ -94,9 +94,11 @@ def test_in_place_against_numpy(lib, n, dbl):
 
 
 # the direct stage of primes above a kernel's codelet limit (the ring c2r
-# kernels stop at 23 in FP32, 19 in FP64): grids 492 = 4 3 41, 258 = 2 3 43,
-# 62, 74, the post-processing length 1519 = 7^2 31, and larger primes
-@pytest.mark.parametrize("n", [29, 62, 74, 258, 492, 1519, 109 * 2, 349, 1000 - 1])
+# kernels stop at 23 in FP32, 19 in FP64, the lag-Fourier 2-D inverse at 5):
+# grids 492 = 4 3 41, 258 = 2 3 43, 62, 74, the post-processing length
+# 1519 = 7^2 31, larger primes, and lengths with radix-8 stages (always a
+# codelet, whatever the limit: 16, 1536 = 8^3 3, 328 = 8 41)
+@pytest.mark.parametrize("n", [16, 29, 62, 74, 258, 492, 1519, 1536, 109 * 2, 349, 8 * 41, 1000 - 1])
 @pytest.mark.parametrize("maxp", [23, 19, 5])
 def test_direct_stage_against_numpy(lib, n, maxp):
     rng = np.random.default_rng(11 * n + maxp)
