@@ -61,9 +61,9 @@ TUNE_TABLE = os.path.join(ROOT, "amber_amd", "tune_gfx950.txt")
 # copy.  Counters cannot be read inside a timed run; `traffic` is this total
 # over the step's launches.
 MEASURED_TRAFFIC_STEP = {("cfg3", "bf16x3", "grouped"): 1.909e13,
-                         # lag-Fourier form (compute grids, sheared r1 == r2 tiles): k_lagft_tiles,
-                         # FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00
-                         ("cfg3", "lagft", "grouped"): 1.165e12}
+                         # lag-Fourier form (compute grids, symmetric tilings, 1024-pixel items):
+                         # k_lagft_tiles, FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00
+                         ("cfg3", "lagft", "grouped"): 1.481e12}
 
 
 def traffic_per_launch(args, st, nsteps):
