@@ -274,10 +274,26 @@ FFTC_HD void stage_direct_dft(const C *__restrict__ x, C *__restrict__ y, const 
     }
 }
 
+/* Between the stages of a team's transform.  A team is one 64-wide
+ * wavefront (T = 64 in every kernel) and its stages touch only its own
+ * buffers, so the wavefront's own LDS order suffices: the fences keep the
+ * compiler from moving the team's LDS accesses across the point, and a
+ * wavefront's LDS operations complete in issue order.  The workgroup
+ * barriers stay where teams exchange data (gathers, scatters) and around
+ * the twiddle table a workgroup loads. */
+__device__ __forceinline__ void team_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 /* Full transform of a team's buffer x (scratch y of the same size); every
- * thread of the workgroup calls this with the same plan (one barrier per
- * stage).  t: thread index in the team, T: team size.  Returns the buffer
- * holding the result (x or y). */
+ * thread of the team calls this with the same plan, the team a whole
+ * wavefront (T = 64; one team_sync per stage).  t: thread index in the team,
+ * T: team size.  The caller's workgroup barrier precedes the call when
+ * other teams wrote x or the twiddles.  Returns the buffer holding the
+ * result (x or y). */
 template <int DIR, int MAXP = 47, typename C>
 __device__ C *transform(C *x, C *y, const C *tw, const FftPlan &pl, int t, int T)
 {
@@ -309,11 +325,11 @@ __device__ C *transform(C *x, C *y, const C *tw, const FftPlan &pl, int t, int T
         case 47: if constexpr (MAXP >= 47) stage<47, DIR>(x, y, tw, n, Ns, t, T); break;
         default:
             stage_direct_tw<DIR>(x, tw, n, R, Ns, t, T);
-            __syncthreads();
+            team_sync();
             stage_direct_dft<DIR>(x, y, tw, n, R, Ns, t, T);
             break;
         }
-        __syncthreads();
+        team_sync();
         C *tmp = x;
         x = y;
         y = tmp;
@@ -364,8 +380,8 @@ FFTC_HD void stage_ip(C *__restrict__ x, const C *__restrict__ tw, int n, int Ns
 
 /* In-place transform of a team's buffer x (input in digit-reversed order,
  * output in natural order) for plans of radices 2, 3, 4, 5 and 8 only (the
- * {2,3,5}-smooth lengths of the channel axis); one workgroup barrier per
- * stage, as transform(). */
+ * {2,3,5}-smooth lengths of the channel axis); one team_sync per stage, as
+ * transform(). */
 template <int DIR, typename C>
 __device__ void transform_ip(C *x, const C *tw, const FftPlan &pl, int t, int T)
 {
@@ -380,7 +396,7 @@ __device__ void transform_ip(C *x, const C *tw, const FftPlan &pl, int t, int T)
         case 8: stage_ip<8, DIR>(x, tw, n, Ns, t, T); break;
         default: stage_ip<5, DIR>(x, tw, n, Ns, t, T); break;
         }
-        __syncthreads();
+        team_sync();
         Ns *= R;
     }
 }
