@@ -153,3 +153,53 @@ def test_metric_workload_normalisation_is_exact(ctx, big, prec):
     finally:
         ctx.release_cube(c1)
         ctx.set_precision("bf16x3")
+
+
+def test_metric_workload_symmetric_tilings(ctx, big):
+    """The lag-Fourier form's symmetric tilings at the metric workload's
+    transform length (768 channels, all lags: L = 1536), on its compute
+    grids: an r1 == r2 triangle (sheared tiles with their k3 = L/2 row and
+    fixed-point column), an r2 == r3 one (mode 4: five anti-diagonal corner
+    cells at this L) and an equilateral one.  49 cells each against the FP64
+    oracle, and every cell against the full (k2, k3) tiling of a context
+    with the symmetries off (AMB_NO_SYM), counts equal."""
+    import os
+    gv, samp, _ = big
+    lo, hi = A.log_rings(1, 200, 24)
+    tris = [tuple(int(x) for x in t) for t in A.triangles(lo, hi)]
+    small = [t for t in tris if A.ng(hi[t[0]]) <= 130]
+    pick = [[t for t in small if t[0] == t[1] and t[1] != t[2]][-1],
+            [t for t in small if t[1] == t[2] and t[0] != t[1]][-1],
+            [t for t in small if t[0] == t[1] == t[2]][-1]]
+    os.environ["AMB_NO_SYM"] = "1"
+    try:
+        plain = A.Context(device=0, precision="lagft")
+    finally:
+        del os.environ["AMB_NO_SYM"]
+    ctx.set_precision("lagft")
+    rng = np.random.default_rng(17)
+    try:
+        for c in (ctx, plain):
+            c.set_rings(lo, hi)
+        c1 = ctx.load_cube(N, NC, 1.0, 0, gv, samp)
+        ctx.set_channels(c1, None, NC10)
+        c2 = plain.load_cube(N, NC, 1.0, 0, gv, samp)
+        plain.set_channels(c2, None, NC10)
+        for t in pick:
+            a = ctx.bispec_ex(c1, *t)
+            b = plain.bispec_ex(c2, *t)
+            assert np.array_equal(a["cnt"], b["cnt"])
+            e = np.linalg.norm(a["bl64"] - b["bl64"]) / np.linalg.norm(b["bl64"])
+            d2 = np.append(rng.integers(-NC10, NC10 + 1, 48), 0)
+            d3 = np.append(rng.integers(-NC10, NC10 + 1, 48), 0)
+            b64, c64, _ = O.bispec_cells(N, NC, 1.0, gv, samp, lo, hi, *t, np.ones(NC, np.uint8), NC10, d2, d3)
+            got = a["bl64"][d2 + NC10, d3 + NC10]
+            eo = np.linalg.norm(got - b64) / np.linalg.norm(b64)
+            print("metric workload lagft tiling of %s (grid %d of Ng %d): vs full tiling %.3g, vs oracle %.3g"
+                  % (t, A.compute_grid(hi[t[0]], hi[t[1]], hi[t[2]]), A.ng(hi[t[0]]), e, eo), flush=True)
+            assert np.array_equal(a["cnt"][d2 + NC10, d3 + NC10], c64)
+            assert e <= 2e-6 and eo <= FULL_TOL["lagft"], (t, e, eo)
+        ctx.release_cube(c1)
+    finally:
+        plain.close()
+        ctx.set_precision("bf16x3")
