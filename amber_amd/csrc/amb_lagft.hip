@@ -581,6 +581,191 @@ __device__ __forceinline__ void shear_tile(const LagJob<float> &J, int sg, int t
     }
 }
 
+/* Mode 3 in FP64 (the exact v2 normalisation's sums): the same sheared
+ * (t, k3) tiling with the FP64 kernel's 64 x 128 tiles, lane (li, lj) =
+ * (tid % 16, tid / 16) owning t = a + 4 li + u (u < 4), k3 = b + 8 lj + v.
+ * LDS units of 4 complex doubles (64 B + 16 B pad): per pixel
+ *   G(k2) for k2 = a - b/2 - 64 + r, r in [0, 129)      (units 0-32)
+ *   conj F(s) for s = a + b/2 + r,   r in [0, 129)      (units 33-65)
+ *   H(k3) for k3 = b + r,            r in [0, 129)      (units 66-98)
+ * G window of 8 values from 60 + 4 li - 4 lj (index 4 + u - floor(v/2)),
+ * conj F window of 8 from 4 li + 4 lj (index u + ceil(v/2)), H 8 from
+ * 8 lj: whole units, and lanes an LDS cycle serves one unit apart.  Edge
+ * sums, one per thread, into the column block at index k2:
+ *   k3 = L/2 (tiles with b = L/2 - 128): thread t < 64, t = a + tid,
+ *     k2 = t - L/4: G at r = tid, conj F at tid + 64, H(L/2) at 128;
+ *   t = L/2, even k3 (tiles with a = L/2 - 64): thread 64 + m, k3 = b + 2m,
+ *     k2 = L/2 - k3/2: G at 128 - m, conj F at 64 + m, H at 2m;
+ *   (L/2, L/2) (both): thread 128, k2 = L/4: G at 64, conj F at 128, H at 128. */
+constexpr int SH64_UG = 0, SH64_UF = 33, SH64_UH = 66;
+constexpr int SH64_PXB = 99 * BLKB;
+constexpr int SH64_N = 129;
+
+__device__ __forceinline__ int sh64_off(int unit0, int r) { return (unit0 + (r >> 2)) * BLKB + (r & 3) * 16; }
+
+template <int KPX, int STRIDE>
+__device__ __forceinline__ void shear_tile64(const LagJob<double> &J, int sg, int tile, char *lds0, char *lds1)
+{
+    typedef double2 C;
+    constexpr int SPT = (3 * SH64_N + 255) / 256;
+    const long P = J.P;
+    const int lhp = J.lhp, L = J.L, seg = J.seg, half = L / 2;
+    const int ntb = L / 256;
+    const int a = (tile / ntb) * 64, b = (tile % ntb) * 128;
+    const long xb = (long)sg * seg, xe = xb + seg < P ? xb + seg : P;
+    const int tid = threadIdx.x;
+
+    const C *src[SPT];
+    int kk[SPT], off[SPT];
+    bool ok[SPT], cj[SPT];
+#pragma unroll
+    for (int u = 0; u < SPT; ++u) {
+        const int sl = tid + 256 * u;
+        src[u] = J.G;
+        kk[u] = 0;
+        ok[u] = cj[u] = false;
+        off[u] = -1;
+        int k;
+        bool conj_of_value;
+        const C *x;
+        if (sl < SH64_N) {
+            k = a - b / 2 - 64 + sl;
+            x = J.G;
+            conj_of_value = false;
+            off[u] = sh64_off(SH64_UG, sl);
+        } else if (sl < 2 * SH64_N) {
+            k = a + b / 2 + (sl - SH64_N);
+            x = J.F;
+            conj_of_value = true;
+            off[u] = sh64_off(SH64_UF, sl - SH64_N);
+        } else if (sl < 3 * SH64_N) {
+            k = b + (sl - 2 * SH64_N); /* <= L/2 */
+            x = J.H;
+            conj_of_value = false;
+            off[u] = sh64_off(SH64_UH, sl - 2 * SH64_N);
+        } else {
+            continue;
+        }
+        k %= L;
+        if (k < 0) k += L;
+        const bool m = 2 * k > L;
+        src[u] = x;
+        kk[u] = m ? L - k : k;
+        cj[u] = m != conj_of_value;
+        ok[u] = true;
+    }
+
+    C rg[KPX][SPT];
+    auto fetch = [&](long x0) {
+#pragma unroll
+        for (int pp = 0; pp < KPX; ++pp) {
+            const long x = x0 + pp;
+            const size_t row = (size_t)x * lhp;
+#pragma unroll
+            for (int u = 0; u < SPT; ++u) {
+                C v = make_double2(0.0, 0.0);
+                if (x < xe && ok[u]) {
+                    v = src[u][row + kk[u]];
+                    if (cj[u]) v.y = -v.y;
+                }
+                rg[pp][u] = v;
+            }
+        }
+    };
+    auto stash = [&](char *st) {
+#pragma unroll
+        for (int pp = 0; pp < KPX; ++pp)
+#pragma unroll
+            for (int u = 0; u < SPT; ++u)
+                if (off[u] >= 0) *(C *)(st + pp * STRIDE + off[u]) = rg[pp][u];
+    };
+
+    const int li = tid & 15, lj = tid >> 4;
+    const int gbase = sh64_off(SH64_UG, 60 + 4 * li - 4 * lj), fbase = sh64_off(SH64_UF, 4 * li + 4 * lj);
+    const int hbase = sh64_off(SH64_UH, 8 * lj);
+
+    double acr[4][8], aci[4][8];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 8; ++v) acr[u][v] = aci[u][v] = 0.0;
+
+    int eg = -1, ef = 0, eh = 0, ek2 = 0;
+    if (b == half - 128 && tid < 64) {
+        eg = sh64_off(SH64_UG, tid);
+        ef = sh64_off(SH64_UF, tid + 64);
+        eh = sh64_off(SH64_UH, 128);
+        ek2 = (a + tid - L / 4 + L) % L;
+    } else if (a == half - 64 && tid >= 64 && tid < 128) {
+        const int m = tid - 64;
+        eg = sh64_off(SH64_UG, 128 - m);
+        ef = sh64_off(SH64_UF, 64 + m);
+        eh = sh64_off(SH64_UH, 2 * m);
+        ek2 = half - b / 2 - m;
+    } else if (a == half - 64 && b == half - 128 && tid == 128) {
+        eg = sh64_off(SH64_UG, 64);
+        ef = sh64_off(SH64_UF, 128);
+        eh = sh64_off(SH64_UH, 128);
+        ek2 = L / 4;
+    }
+    const bool ex = eg >= 0;
+    double enr = 0.0, eni = 0.0;
+
+    const long nch = (xe - xb + KPX - 1) / KPX;
+    fetch(xb);
+    stash(lds0);
+    __syncthreads();
+    for (long ch = 0; ch < nch; ++ch) {
+        if (ch + 1 < nch) fetch(xb + (ch + 1) * KPX);
+        const char *st = (ch & 1) ? lds1 : lds0;
+#pragma unroll 1
+        for (int pp = 0; pp < KPX; ++pp) {
+            const char *px = st + pp * STRIDE;
+            /* two halves of v, each with only the window values it reads
+               (a complex double is one 16-byte LDS read): v < 4 reads G
+               3..7, conj F 0..5, H 0..3; v >= 4 reads G 1..5, conj F
+               2..7, H 4..7 */
+#pragma unroll
+            for (int hv2 = 0; hv2 < 2; ++hv2) {
+                const int g0 = hv2 ? 1 : 3, f0 = hv2 ? 2 : 0;
+                C g[5], f[6], hv[4];
+#pragma unroll
+                for (int i = 0; i < 5; ++i) g[i] = *(const C *)(px + gbase + (g0 + i) / 4 * BLKB + (g0 + i) % 4 * 16);
+#pragma unroll
+                for (int i = 0; i < 6; ++i) f[i] = *(const C *)(px + fbase + (f0 + i) / 4 * BLKB + (f0 + i) % 4 * 16);
+                load_unit(px + hbase + hv2 * BLKB, hv);
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int vv = 0; vv < 4; ++vv) {
+                        const int v = 4 * hv2 + vv;
+                        const C gg = g[4 + u - (v >> 1) - g0], ff = f[u + ((v + 1) >> 1) - f0];
+                        const double tr = fma(gg.x, hv[vv].x, -gg.y * hv[vv].y);
+                        const double ti = fma(gg.x, hv[vv].y, gg.y * hv[vv].x);
+                        acr[u][v] = fma(ff.x, tr, fma(-ff.y, ti, acr[u][v]));
+                        aci[u][v] = fma(ff.x, ti, fma(ff.y, tr, aci[u][v]));
+                    }
+            }
+            if (ex) {
+                const C gg = *(const C *)(px + eg), ff = *(const C *)(px + ef), hh = *(const C *)(px + eh);
+                const double tr = fma(gg.x, hh.x, -gg.y * hh.y), ti = fma(gg.x, hh.y, gg.y * hh.x);
+                enr = fma(ff.x, tr, fma(-ff.y, ti, enr));
+                eni = fma(ff.x, ti, fma(ff.y, tr, eni));
+            }
+        }
+        if (ch + 1 < nch) stash((ch & 1) ? lds0 : lds1);
+        __syncthreads();
+    }
+    if (ex) J.nyq[(size_t)sg * L + ek2] = make_double2(enr, eni);
+    C *out = J.part + ((size_t)sg * J.ntile + tile) * (64 * 128);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        C *o = out + (size_t)(4 * li + u) * 128 + 8 * lj;
+#pragma unroll
+        for (int v = 0; v < 8; ++v) o[v] = make_double2(acr[u][v], aci[u][v]);
+    }
+}
+
 template <typename R>
 __global__ __launch_bounds__(256, LagT<R>::WPS) void k_lagft_tiles(const LagJob<R> *__restrict__ jobs, int njob, long total)
 {
@@ -594,7 +779,7 @@ __global__ __launch_bounds__(256, LagT<R>::WPS) void k_lagft_tiles(const LagJob<
     constexpr int NSLOT = 2 * (TA + TB);               /* A, B, then TA + TB of C */
     constexpr int SPT = (NSLOT + (FOLD ? 1 : 0) + 255) / 256; /* slots per thread */
     constexpr int NF = BU + BV - 1, NFU = (NF + VPU - 1) / VPU;
-    constexpr int PXBM = sizeof(R) == 4 && SH_PXB > PXB ? SH_PXB : PXB;
+    constexpr int PXBM = sizeof(R) == 4 ? (SH_PXB > PXB ? SH_PXB : PXB) : (SH64_PXB > PXB ? SH64_PXB : PXB);
     __shared__ __align__(16) char lds[2][KPX * PXBM];
     const long it = xcd_item(blockIdx.x, gridDim.x);
     if (it >= total) return;
@@ -607,6 +792,11 @@ __global__ __launch_bounds__(256, LagT<R>::WPS) void k_lagft_tiles(const LagJob<
     if constexpr (sizeof(R) == 4) {
         if (mode == 3) {
             shear_tile<KPX, SH_PXB>(jobs[jb], sg, tile, lds[0], lds[1]);
+            return;
+        }
+    } else {
+        if (mode == 3) {
+            shear_tile64<1, SH64_PXB>(jobs[jb], sg, tile, lds[0], lds[1]);
             return;
         }
     }
@@ -1391,10 +1581,12 @@ int choose_tiles(amb_ctx *c, int L, bool fp64, bool r12, bool r23, int *mode, in
     *tmap = nullptr;
     *nyql = nullptr;
     *nnyq = 0;
-    if (!fp64 && r12 && !c->no_sym && !c->no_shear && L % 256 == 0) {
-        /* sheared (t, k3) tiles: exactly the distinct sums, edges folded */
+    if (r12 && !c->no_sym && !c->no_shear && L % 256 == 0) {
+        /* sheared (t, k3) tiles: exactly the distinct sums, edges folded
+           (FP32 128 x 128, FP64 64 x 128 tiles) */
         *mode = 3;
-        *ntile = (L / 256) * (L / 256);
+        *ntile = (L / 2 / tile_a<float>()) * (L / 256);
+        if (fp64) *ntile = (L / 2 / tile_a<double>()) * (L / 256);
         *nyq_sep = 0;
         return AMB_OK;
     }
@@ -1624,8 +1816,12 @@ int amb::lagft_run(amb_ctx *c, const amb_tri_req *req, int n, double t0, double 
             const dim3 gr((unsigned)((L + 31) / 32), (unsigned)((L / 2 + 1 + 31) / 32));
             if (j.norm) {
                 const LagJob<double> &g = d64[slot[i]];
-                hipLaunchKernelGGL((k_lagft_reduce<double, 0>), gr, dim3(256), 0, c->stream, g.part, j.nseg,
-                                   j.ntile, L, nullptr, St);
+                if (j.mode == 3)
+                    hipLaunchKernelGGL((k_lagft_reduce<double, 3>), gr, dim3(256), 0, c->stream, g.part, j.nseg,
+                                       j.ntile, L, nullptr, St);
+                else
+                    hipLaunchKernelGGL((k_lagft_reduce<double, 0>), gr, dim3(256), 0, c->stream, g.part, j.nseg,
+                                       j.ntile, L, nullptr, St);
                 if (j.nyq_sep)
                     hipLaunchKernelGGL(k_lagft_reduce_nyq<double>, dim3((unsigned)((L + 255) / 256)), dim3(256), 0,
                                        c->stream, g.nyq, j.nseg, L, nullptr, 0, St);

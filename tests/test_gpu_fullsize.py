@@ -119,16 +119,16 @@ def test_metric_workload_error_vs_fp64(ctx, big, metric_cells, prec):
         ctx.set_precision("bf16x3")
 
 
-@pytest.mark.parametrize("prec", ["bf16x3", "lagft"])
-def test_metric_workload_normalisation_is_exact(ctx, big, prec):
+@pytest.mark.parametrize("prec,t", [("bf16x3", (23, 20, 20)), ("lagft", (23, 20, 20)), ("lagft", (23, 23, 20))])
+def test_metric_workload_normalisation_is_exact(ctx, big, prec, t):
     """The amber_v2 normalisation at the metric workload (768 ch x 512^2,
     Ng = 612): equal bit for bit to the FP64 oracle's sums snapped to their
     1/8 grid of Ng^2, on 49 lag cells, whether the fast path certified the
     triangle or it was recomputed in FP64 (split form), or computed by the
-    FP64 lag-Fourier stages (lagft, certificate margin far inside 1/32)."""
+    FP64 lag-Fourier stages (lagft, certificate margin far inside 1/32; the
+    r1 == r2 triangle on the FP64 sheared tiles)."""
     gv, samp, _ = big
     lo, hi = A.log_rings(1, 200, 24)
-    t = (23, 20, 20)
     ctx.set_precision(prec)
     ctx.set_rings(lo, hi)
     c1 = ctx.load_cube(N, NC, 1.0, 0, gv, samp)

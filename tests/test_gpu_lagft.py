@@ -264,3 +264,43 @@ def test_compute_grids(ctx):
     finally:
         ctx.set_grid(True)
         ctx.set_precision("bf16x3")
+
+
+def test_v2_norm_on_sheared_fp64_tiles(ctx):
+    """The exact v2 normalisation of r1 == r2 triangles at a transform length
+    of 512 (256 channels, all lags), where its FP64 sums run on the sheared
+    (t, k3) tiles: equal bit for bit to the FP64 oracle's sums snapped to
+    their 1/8 grid of Ng^2, and to a context with the symmetries off
+    (AMB_NO_SYM, the full FP64 tiling)."""
+    import os
+    N, nc, nc10 = 64, 256, 255
+    lo, hi = A.log_rings(2, 20, 5)
+    gv, samp, _ = A.generate(N, nc, nflag=0, seed=19)
+    mask = np.ones(nc, np.uint8)
+    tris = [tuple(int(x) for x in t) for t in A.triangles(lo, hi) if t[0] == t[1]]
+    os.environ["AMB_NO_SYM"] = "1"
+    try:
+        plain = A.Context(device=0, precision="lagft")
+    finally:
+        del os.environ["AMB_NO_SYM"]
+    ctx.set_precision("lagft")
+    try:
+        for c in (ctx, plain):
+            c.set_rings(lo, hi)
+        cube = ctx.load_cube(N, nc, 1.0, 0, gv, samp)
+        ctx.set_channels(cube, None, nc10)
+        cube2 = plain.load_cube(N, nc, 1.0, 0, gv, samp)
+        plain.set_channels(cube2, None, nc10)
+        for t in tris[-3:]:
+            print("case lagft v2 norm on sheared FP64 tiles, triangle %s" % (t,), flush=True)
+            a = ctx.bispec_ex(cube, *t, want_norm=True)
+            b = plain.bispec_ex(cube2, *t, want_norm=True)
+            _, _, n64 = O.bispec(N, nc, 1.0, gv, samp, lo, hi, *t, mask, nc10, want_norm=True)
+            P = a["ng"] ** 2
+            snapped = P * np.rint(n64 / P * 8) / 8
+            assert np.array_equal(a["norm64"], snapped), t
+            assert np.array_equal(a["norm64"], b["norm64"]), t
+        ctx.release_cube(cube)
+    finally:
+        plain.close()
+        ctx.set_precision("bf16x3")
