@@ -788,8 +788,9 @@ class Context:
         self._c(_lib.amb_set_cert_strict(self._h, int(bool(strict))))
 
     def set_grid(self, reduced=True):
-        """lag-Fourier sums on each triangle's smallest alias-free grid
-        (compute_grid, default) or on the reference's Ng(r1)"""
+        """sums on each triangle's smallest alias-free grid (compute_grid,
+        default) or on the reference's Ng(r1); the split-BF16 and FP8 forms
+        always use Ng(r1)"""
         self._c(_lib.amb_set_grid(self._h, int(bool(reduced))))
 
 

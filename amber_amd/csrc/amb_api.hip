@@ -645,6 +645,18 @@ static void drop_cube_planes(amb_ctx *c, int id)
         } else ++i;
 }
 
+long compute_grid(const amb_ctx *c, const Cube &cb, int r1, int r2, int r3)
+{
+    const long ng = amb_ng(c->uhigh[r1], cb.Du);
+    /* the split-BF16 and FP8 forms stay on Ng: their error is the operands'
+       representation error per product, which does not shrink with the
+       grid, so against B (n^2 times the closed-triangle sum) it grows as
+       Ng/n; the FP32, FP16x3 and lag-Fourier errors, dominated by the
+       accumulation, shrink with it */
+    if (!c->grid_reduced || c->prec == AMB_PREC_BF16X3 || c->prec == AMB_PREC_FP8) return ng;
+    return amb_compute_grid(c->uhigh[r1], c->uhigh[r2], c->uhigh[r3], cb.Du);
+}
+
 int prepare_tri(amb_ctx *c, const PlaneSrc &src, int id, int r1, int r2, int r3, int want_norm, TriJob &j)
 {
     Cube *cb;
@@ -659,7 +671,10 @@ int prepare_tri(amb_ctx *c, const PlaneSrc &src, int id, int r1, int r2, int r3,
     if (ng <= 0 || ng > 2730) return fail(c, AMB_EARG, "Ng=%ld out of range", ng);
     j = TriJob{};
     j.cube = id; j.r1 = r1; j.r2 = r2; j.r3 = r3; j.want_norm = want_norm;
-    j.ng = ng; j.P = ng * ng; j.pitch = plane_pitch(ng);
+    /* planes and sums on the compute grid (the ring cells and the column
+       cut of Ng), the sums scaled to Ng by the reduction */
+    const long ngc = compute_grid(c, *cb, r1, r2, r3);
+    j.ng = ng; j.ngc = ngc; j.P = ngc * ngc; j.pitch = plane_pitch(ngc);
     j.nact = cb->nact();
     j.W = 2 * cb->nc10 + 1;
     j.cells = (size_t)j.W * j.W;
@@ -671,16 +686,16 @@ int prepare_tri(amb_ctx *c, const PlaneSrc &src, int id, int r1, int r2, int r3,
     const int rr[3] = {r1, r2, r3};
     const int d3kind = prec == AMB_PREC_FP32 ? 0 : prec == AMB_PREC_BF16X3 ? 2 : prec == AMB_PREC_FP16X3 ? 7 : 4;
     for (int k = 0; k < 3; ++k) {
-        if ((rc = get_planes(c, src, id, k == 2 ? d3kind : 0, rr[k], (int)ng, &D[k]))) return rc;
-        if (prec == AMB_PREC_FP8 && (rc = get_planes(c, src, id, 5, rr[k], (int)ng, &SC[k]))) return rc;
+        if ((rc = get_planes(c, src, id, k == 2 ? d3kind : 0, rr[k], (int)ngc, &D[k], (int)ng))) return rc;
+        if (prec == AMB_PREC_FP8 && (rc = get_planes(c, src, id, 5, rr[k], (int)ngc, &SC[k], (int)ng))) return rc;
     }
     /* the FP32 set of r3, for its exponent (built already as the source of
        the derived third operand) */
-    if ((rc = get_planes(c, src, id, 0, r3, (int)ng, &D0))) return rc;
+    if ((rc = get_planes(c, src, id, 0, r3, (int)ngc, &D0, (int)ng))) return rc;
     if (want_norm)
         for (int k = 0; k < 3; ++k) {
             const int kind = (k == 2 && nprec != AMB_PREC_FP32) ? 3 : 1;
-            if ((rc = get_planes(c, src, id, kind, rr[k], (int)ng, &S[k]))) return rc;
+            if ((rc = get_planes(c, src, id, kind, rr[k], (int)ngc, &S[k], (int)ng))) return rc;
         }
     /* r1 == r2 symmetry needs every p3 of p1's window inside p2's window */
     const bool full_window = nact > 0 && cb->act.back() - cb->act.front() <= cb->nc10;
@@ -721,6 +736,8 @@ int prepare_tri(amb_ctx *c, const PlaneSrc &src, int id, int r1, int r2, int r3,
     ReduceArgs &ra = j.ra;
     ra.act = cb->d_act; ra.inv = cb->d_inv;
     ra.nc = (int)cb->nc; ra.nact = nact; ra.nc10 = cb->nc10; ra.P = j.P; ra.sym12 = j.sym12; ra.sym23 = j.sym23;
+    ra.gscale = (double)(ng * ng) / (double)j.P;
+    ra.Pout = (double)(ng * ng);
     j.flop_exec = (double)amb_contract_live_items(nact, cb->wlo.data(), cb->whi.data(), j.sym12, j.sym23) *
                   amb_contract_item_flops(prec, j.pitch);
     return AMB_OK;
@@ -780,15 +797,15 @@ int norm_exact(amb_ctx *c, int id, int r1, int r2, int r3, float *norm, double *
     Cube *cb;
     int rc = find_cube(c, id, &cb);
     if (rc) return rc;
-    const long ng = amb_ng(c->uhigh[r1], cb->Du);
-    const long P = ng * ng, pitch = plane_pitch(ng);
+    const long ng = amb_ng(c->uhigh[r1], cb->Du), ngc = compute_grid(c, *cb, r1, r2, r3);
+    const long P = ngc * ngc, pitch = plane_pitch(ngc);
     const int nact = cb->nact();
     const size_t W = (size_t)(2 * cb->nc10 + 1), cells = W * W, nC = (size_t)nact * nact * nact;
     const PlaneSrc src{&c->planes};
     void *S[3];
     const int rr[3] = {r1, r2, r3};
     for (int k = 0; k < 3; ++k)
-        if ((rc = get_planes(c, src, id, 8, rr[k], (int)ng, &S[k]))) { unpin_all(c); return rc; }
+        if ((rc = get_planes(c, src, id, 8, rr[k], (int)ngc, &S[k], (int)ng))) { unpin_all(c); return rc; }
     void *Cq = c->Cqx, *out = c->outx;
     if ((rc = grow(c, &Cq, &c->Cqx_bytes, sizeof(int) * nC, "exact normalisation partials")) ||
         (rc = grow(c, &out, &c->outx_bytes, cells * (sizeof(float) + sizeof(double)), "exact normalisation"))) {
@@ -812,6 +829,7 @@ int norm_exact(amb_ctx *c, int id, int r1, int r2, int r3, float *norm, double *
     ra.C = nullptr; ra.Cq = c->Cqx;
     ra.act = cb->d_act; ra.inv = cb->d_inv;
     ra.nc = (int)cb->nc; ra.nact = nact; ra.nc10 = cb->nc10; ra.P = P; ra.sym12 = a.sym12; ra.sym23 = a.sym23;
+    ra.Pout = (double)(ng * ng);
     ra.norm = d_n; ra.norm64 = d_n64;
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device);
@@ -908,7 +926,7 @@ int bispec_core(amb_ctx *c, int id, int r1, int r2, int r3, int want_norm, amb_r
         e = (expr);                                                   \
         if (e == hipSuccess && c->sync_debug) e = hipStreamSynchronize(c->stream); \
     }
-    const amb_tune_entry tc = tune_cfg(c, j.ng, j.nact);
+    const amb_tune_entry tc = tune_cfg(c, j.ngc, j.nact);
     timer_start(c, &c->st.ms_contract);
     STEP("contract", amb_launch_contract(c->stream, prec, 0, tune_variant(tc, prec, j.nact, j.pitch), j.a));
     timer_stop(c);

@@ -1456,13 +1456,16 @@ __device__ __forceinline__ void reduce_cell(const ReduceArgs &a, long cell)
     }
     const long mirror = (long)(d3 + a.nc10) * W + (d2 + a.nc10);
     const int nw = (a.sym23 && d2 != d3) ? 2 : 1;
+    /* sums over a compute grid scaled to the reference grid in FP64 */
+    const bool scaled = a.gscale > 0.0 && a.gscale != 1.0;
+    const double sd = ((double)sum + (double)comp) * (scaled ? a.gscale : 1.0);
     for (int w = 0; w < nw; ++w) {
         const long o = w ? mirror : cell;
-        if (a.bl) a.bl[o] = sum + comp;
-        if (a.bl64) a.bl64[o] = (double)sum + (double)comp;
+        if (a.bl) a.bl[o] = scaled ? (float)sd : sum + comp;
+        if (a.bl64) a.bl64[o] = sd;
         if (a.cnt) a.cnt[o] = n;
         if (a.Cq) {
-            const double v = (double)a.P * ((double)q * 0.125);
+            const double v = (a.Pout > 0.0 ? a.Pout : (double)a.P) * ((double)q * 0.125);
             if (a.norm) a.norm[o] = (float)v;
             if (a.norm64) a.norm64[o] = v;
         }

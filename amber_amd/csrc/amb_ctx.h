@@ -113,7 +113,8 @@ struct Timer {
  * filled except the scratch pointers an executor assigns. */
 struct TriJob {
     int cube = 0, r1 = 0, r2 = 0, r3 = 0, want_norm = 0;
-    long ng = 0, P = 0, pitch = 0;
+    long ng = 0, P = 0, pitch = 0; /* Ng(r1); pixels and pitch of the compute grid */
+    long ngc = 0;                  /* the compute grid (compute_grid) */
     int nact = 0, W = 0, sym12 = 0, sym23 = 0;
     size_t cells = 0;
     ContractArgs a{}; /* data contraction */
@@ -128,7 +129,7 @@ struct amb_ctx {
     int device = 0, prec = AMB_PREC_BF16X3, verbose = 0, sync_debug = 0, no_sym = 0, no_shear = 0;
     int batch_mode = AMB_BATCH_GROUPED;
     int cert_strict = 0; /* uncertified normalisation sums: count (0) or fail (1) */
-    int grid_reduced = 1; /* lag-Fourier sums on amb_compute_grid (1) or on Ng(r1) (0) */
+    int grid_reduced = 1; /* lag-Fourier, FP32, FP16x3 sums on amb_compute_grid (1) or on Ng(r1) (0) */
     uint64_t budget = 0, used = 0;
     hipStream_t stream = nullptr;
     std::recursive_mutex mu;
@@ -304,6 +305,9 @@ size_t lag_part_bytes(const LagTri &t);
 long lag_items(const LagTri &t);
 /* pixels of one lag-Fourier work item (a tile over one pixel segment) */
 long lag_item_px();
+/* the grid a triangle's sums run on: amb_compute_grid of its three rings
+   when reduced grids are on (the default), else Ng(r1) */
+long compute_grid(const amb_ctx *c, const Cube &cb, int r1, int r2, int r3);
 size_t lag_desc_bytes();
 size_t lag_st_cap(); /* the spectrum buffer of any transform length */
 void lag_fill_range(amb_ctx *c, const LagTri &t, void *part, long i0, long item0, void *h);

@@ -132,6 +132,9 @@ struct ReduceArgs {
     int sym12;          /* read C[p2][p1][p3] when p2 < p1 */
     int sym23;          /* r2 == r3: sum cells d2 <= d3, write their mirrors */
     long P;
+    double gscale;      /* data sums times (Ng/n)^2, the compute grid's sums as the reference
+                           grid's (0 or 1: as summed) */
+    double Pout;        /* norm: written as eighths of this many pixels, Ng^2 (0: P) */
     float *bl;
     double *bl64;
     int *cnt;

@@ -1558,14 +1558,7 @@ double edge_cells(int mode, int L, const int *nyql, int nnyq)
     return (double)(nyql ? nnyq : L);
 }
 
-/* the grid a triangle's sums run on: the smallest alias-free grid of its
-   three rings (amb_compute_grid), or Ng(r1) when reduced grids are off */
-long compute_grid(const amb_ctx *c, const Cube &cb, int r1, int r2, int r3)
-{
-    const long ng = amb_ng(c->uhigh[r1], cb.Du);
-    if (!c->grid_reduced) return ng;
-    return amb_compute_grid(c->uhigh[r1], c->uhigh[r2], c->uhigh[r3], cb.Du);
-}
+
 
 /* tiling of a triangle: (k2, k3) tiles, or for FP32 jobs the symmetric
    (i, j) tiles of r1 == r2 or the listed (k2, k3) tiles of r2 == r3 where

@@ -546,7 +546,7 @@ bool launch_iter(Srv *s)
     amb_tune_entry t{};
     int var = 0, nvar = 0, blocks = 8;
     if (big) {
-        t = tune_cfg(c, big->j.ng, big->j.nact);
+        t = tune_cfg(c, big->j.ngc, big->j.nact);
         var = nvar = t.variant;
         for (const Range &r : rd) var = std::min(var, tune_variant(t, prec, s->jobs[r.jid].j.nact, r.pitch));
         for (const Range &r : rn) nvar = std::min(nvar, tune_variant(t, nprec, s->jobs[r.jid].j.nact, r.pitch));

@@ -242,7 +242,9 @@ int amb_autotune(amb_ctx *c, const amb_tri_req *req, int n, double max_seconds, 
         if (!q.bl) return fail(c, AMB_EARG, "request %d has no bl buffer", i);
         if (!cb->chans) return fail(c, AMB_ESTATE, "channels not set for cube %d", q.cube_id);
         if (q.r1 < 0 || q.r1 >= (int)c->uhigh.size()) return fail(c, AMB_EARG, "ring %d out of range", q.r1);
-        const long ng = amb_ng(c->uhigh[q.r1], cb->Du);
+        if (q.r2 < 0 || q.r2 >= (int)c->uhigh.size() || q.r3 < 0 || q.r3 >= (int)c->uhigh.size())
+            return fail(c, AMB_EARG, "triangle (%d,%d,%d) out of range", q.r1, q.r2, q.r3);
+        const long ng = compute_grid(c, *cb, q.r1, q.r2, q.r3);
         const int nact = cb->nact();
         const Key k{c->prec, log2_ceil(ng), log2_ceil(std::max(nact, 1))};
         Bucket &b = bk[k];

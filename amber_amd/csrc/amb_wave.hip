@@ -159,7 +159,7 @@ std::vector<char> descriptors(const Wave &w, const WaveBufs &b)
    the tuned wave size of j's shape, at most a quarter of the HBM budget */
 size_t wave_c_cap(amb_ctx *c, const TriJob &j)
 {
-    const amb_tune_entry t = tune_cfg(c, j.ng, j.nact);
+    const amb_tune_entry t = tune_cfg(c, j.ngc, j.nact);
     return std::min<size_t>((size_t)t.wave_gib << 30, c->budget / 4) / sizeof(float);
 }
 
@@ -173,7 +173,7 @@ amb_tune_entry wave_cfg(amb_ctx *c, const Wave &w, int prec, int *var, int *bloc
         const double cost = (double)amb_contract_items(w.jobs[i].nact) * (double)w.jobs[i].pitch;
         if (cost > best) { best = cost; k = i; }
     }
-    const amb_tune_entry t = tune_cfg(c, w.jobs[k].ng, w.jobs[k].nact);
+    const amb_tune_entry t = tune_cfg(c, w.jobs[k].ngc, w.jobs[k].nact);
     *var = t.variant;
     for (const TriJob &j : w.jobs) *var = std::min(*var, tune_variant(t, prec, j.nact, j.pitch));
     /* a multiple of the 8 XCDs, so that every work queue has takers */

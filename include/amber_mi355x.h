@@ -569,13 +569,15 @@ int amb_reset_stats(amb_ctx *ctx);
  * sizes.  A sum beyond the int32 range of eighths fails in either mode. */
 int amb_set_cert_strict(amb_ctx *ctx, int strict);
 
-/* Grids of the lag-Fourier form.  reduced = 1 (default): each triangle's
- * images, channel spectra and products are computed on amb_compute_grid of
- * its rings and the sums scaled by (Ng/n)^2 in FP64 (the normalisation
- * rounded to its 1/8 grid of n^2 first, so it stays exact); the lag arrays
- * are the reference's sums over Ng(r1)^2 pixels (amber.c:193-266) to
- * rounding.  reduced = 0: every triangle on Ng(r1), as the reference.  The
- * MFMA forms always use Ng(r1).  No counterpart in the reference. */
+/* Grids of the lag-Fourier, FP32 and FP16x3 forms.  reduced = 1 (default):
+ * each triangle's images (and channel spectra) and sums are computed on
+ * amb_compute_grid of its rings and the sums scaled by (Ng/n)^2 in FP64 (the
+ * normalisation rounded to its 1/8 grid of n^2 first, so it stays exact);
+ * the lag arrays are the reference's sums over Ng(r1)^2 pixels
+ * (amber.c:193-266) to rounding.  reduced = 0: every triangle on Ng(r1), as
+ * the reference.  The split-BF16 and FP8 forms always use Ng(r1): their
+ * per-product representation error does not shrink with the grid, so their
+ * relative error would grow as Ng/n.  No counterpart in the reference. */
 int amb_set_grid(amb_ctx *ctx, int reduced);
 
 #ifdef __cplusplus
