@@ -1177,6 +1177,7 @@ void amb_destroy(amb_ctx *c)
     hipFree(c->lag_part); hipFree(c->lag_st); hipFree(c->lag_out); hipFree(c->lag_out2);
     for (auto &kv : c->lag_pairs_tab) hipFree(std::get<0>(kv.second));
     for (auto &kv : c->lag_r23_tab) hipFree(std::get<0>(kv.second));
+    for (auto &kv : c->lag_equi_tab) hipFree(std::get<0>(kv.second));
     hipFree(c->wdesc);
     for (void *h : c->hstage)
         if (h) hipHostFree(h);

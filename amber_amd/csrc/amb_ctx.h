@@ -195,6 +195,8 @@ struct amb_ctx {
     /* r2 == r3 tile tables by transform length: [tiles (a, b)] [nt2 x nt3
        tile index], tile count, bytes */
     std::map<int, std::tuple<void *, int, size_t>> lag_r23_tab;
+    /* r1 == r2 == r3 sheared tiles per L: device list + map, tile count, bytes, fixed-point cells */
+    std::map<int, std::tuple<void *, int, size_t, int>> lag_equi_tab;
     /* pinned host staging, two buffers: a grouped wave's results are copied
        out of one while the next wave's land in the other; the lag-space
        post-processing stages alternate triangles in them the same way */

@@ -113,7 +113,9 @@ struct LagJob {
                                    2: r2 == r3, listed (k2, k3) tiles (tiles below the
                                    diagonal of [0, L/2)^2 dropped: S is symmetric there);
                                    3: r1 == r2, sheared (t, k3) tiles; 4: r2 == r3, mode 2's
-                                   tiles without those past the anti-diagonal for k2 >= L/2 */
+                                   tiles without those past the anti-diagonal for k2 >= L/2;
+                                   5: r1 == r2 == r3, the listed sheared tiles of the
+                                   principal domain */
     int ntile;                  /* tiles per segment */
     int nyq_sep;                /* the k3 = L/2 column is summed by k_lagft_nyq */
 };
@@ -418,7 +420,12 @@ __device__ __forceinline__ void shear_tile(const LagJob<float> &J, int sg, int t
     const long P = J.P;
     const int lhp = J.lhp, L = J.L, seg = J.seg, half = L / 2;
     const int nta = L / 256;
-    const int a = (tile / nta) * 128, b = (tile % nta) * 128;
+    int a = (tile / nta) * 128, b = (tile % nta) * 128;
+    if (J.mode == 5) { /* r1 == r2 == r3: the listed tiles of the principal domain */
+        const int2 ij = J.tlist[tile];
+        a = ij.x * 128;
+        b = ij.y * 128;
+    }
     const long xb = (long)sg * seg, xe = xb + seg < P ? xb + seg : P;
     const int tid = threadIdx.x;
 
@@ -790,7 +797,7 @@ __global__ __launch_bounds__(256, LagT<R>::WPS) void k_lagft_tiles(const LagJob<
     const long local = it - jobs[jb].item0 + jobs[jb].ibase;
     const int sg = (int)(local / ntile), tile = (int)(local - (long)sg * ntile);
     if constexpr (sizeof(R) == 4) {
-        if (mode == 3) {
+        if (mode == 3 || mode == 5) {
             shear_tile<KPX, SH_PXB>(jobs[jb], sg, tile, lds[0], lds[1]);
             return;
         }
@@ -1087,6 +1094,11 @@ __global__ __launch_bounds__(256) void k_lagft_nyq(const LagJob<R> *__restrict__
    anti-diagonal (k2 + k3 > L) from their conjugate mirror (L - k3, L - k2),
    the anti-diagonal corners of unlisted tiles and the k3 = L/2 column from
    the column block.
+   MODE 5 (r1 == r2 == r3): S is the same for every permutation of (k1, k2,
+   k3), k1 = -k2 - k3, and conjugate under negation; the cell's orbit meets
+   the principal domain {0 <= k3 <= k2, 2 k2 + k3 <= L} (the bispectrum's
+   inner and outer triangles), whose (t, k3) lie in the listed sheared tiles
+   or on the fixed-point column t = L/2.
    MODE 3: the sheared tiles' (t, k3), t = k2 + floor(k3/2), or its mirror
    (-1 - t for odd k3, -t for even) when t is past L/2; the fixed points t =
    L/2 and the row k3 = L/2 from the column block after the partials. */
@@ -1105,7 +1117,7 @@ __global__ __launch_bounds__(256) void k_lagft_reduce(const typename LagT<R>::C 
     /* k3 = L/2 is a tile column (mode 0, TB does not divide L/2), or in mode
        1 a diagonal of the listed tiles (k_lagft_reduce_nyq then writes the
        k2 that no listed tile holds) */
-    const bool col = MODE == 1 || MODE == 3 || MODE == 4 || half % TB != 0;
+    const bool col = MODE == 1 || MODE >= 3 || half % TB != 0;
     const C *nyq = part + (size_t)nseg * tsz;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -1115,7 +1127,34 @@ __global__ __launch_bounds__(256) void k_lagft_reduce(const typename LagT<R>::C 
             long off = 0;
             int ni = -1;
             bool cj = false;
-            if (MODE == 4) {
+            if (MODE == 5) {
+                const int k1 = (2 * L - k2 - k3) % L;
+                const int cs[6][2] = {{k2, k3}, {k3, k2}, {k1, k3}, {k3, k1}, {k1, k2}, {k2, k1}};
+                int p = -1, q = -1;
+                for (int ng = 0; ng < 2 && p < 0; ++ng)
+                    for (int c = 0; c < 6 && p < 0; ++c) {
+                        int x = cs[c][0], y = cs[c][1];
+                        if (ng) {
+                            x = x ? L - x : 0;
+                            y = y ? L - y : 0;
+                        }
+                        if (y <= x && y <= L - 2 * x) {
+                            p = x;
+                            q = y;
+                            cj = ng;
+                        }
+                    }
+                off = -1;
+                if (p >= 0) {
+                    const int t = p + (q >> 1);
+                    if (t == half) {
+                        ni = half - (q >> 1);
+                    } else {
+                        const int tl = tmap[(t / TA) * (half / TB) + q / TB];
+                        if (tl >= 0) off = (long)tl * (TA * TB) + (t % TA) * TB + (q % TB);
+                    }
+                }
+            } else if (MODE == 4) {
                 int p2 = k2, p3 = k3;
                 if (p2 >= half && p2 + p3 > L) {
                     const int q2 = L - p3;
@@ -1554,6 +1593,7 @@ double lag_distinct(const amb_tri_req &q) { return lag_distinct_r(q.r1, q.r2, q.
 double edge_cells(int mode, int L, const int *nyql, int nnyq)
 {
     if (mode == 3) return (double)(L / 2 + 1 + L / 4);
+    if (mode == 5) return (double)nnyq; /* the fixed points the listed last-t tiles sum */
     if (mode == 4) return (double)((L / 256) * 128 + 128 + (L / 256 - 1)); /* the folded column of the listed last-k3 tiles, corners */
     return (double)(nyql ? nnyq : L);
 }
@@ -1574,6 +1614,47 @@ int choose_tiles(amb_ctx *c, int L, bool fp64, bool r12, bool r23, int *mode, in
     *tmap = nullptr;
     *nyql = nullptr;
     *nnyq = 0;
+    if (!fp64 && r12 && r23 && !c->no_sym && !c->no_shear && L % 256 == 0) {
+        /* r1 == r2 == r3: the sheared tiles that hold a cell of the principal
+           domain {0 <= k3 <= k2, 2 k2 + k3 <= L}, and the last-t tiles whose
+           fixed points t = L/2 lie in it; listed once per L */
+        auto it = c->lag_equi_tab.find(L);
+        if (it == c->lag_equi_tab.end()) {
+            const int half = L / 2, nt = half / 128;
+            auto indom = [&](int p, int q) { return q >= 0 && q <= p && q <= L - 2 * p; };
+            std::vector<int> tm((size_t)nt * nt, -1);
+            for (int k3 = 0; k3 < half; ++k3)
+                for (int k2 = 0; k2 <= half; ++k2) {
+                    const int t = k2 + (k3 >> 1);
+                    if (t < half && indom(k2, k3)) tm[(size_t)(t / 128) * nt + k3 / 128] = 1;
+                }
+            for (int k3 = 0; k3 < half; k3 += 2)
+                if (indom(half - k3 / 2, k3)) tm[(size_t)(nt - 1) * nt + k3 / 128] = 1;
+            std::vector<int2> tl;
+            int fixed = 0;
+            for (int A = 0; A < nt; ++A)
+                for (int B = 0; B < nt; ++B)
+                    if (tm[(size_t)A * nt + B] > 0) {
+                        tm[(size_t)A * nt + B] = (int)tl.size();
+                        tl.push_back(make_int2(A, B));
+                        if (A == nt - 1) fixed += 64;
+                    }
+            void *d = nullptr;
+            const size_t tb = sizeof(int2) * tl.size(), mb = sizeof(int) * tm.size();
+            const int rc = dev_alloc(c, &d, tb + mb, "lag-Fourier tile tables");
+            if (rc) return rc;
+            HIPCHK(c, hipMemcpy(d, tl.data(), tb, hipMemcpyHostToDevice));
+            HIPCHK(c, hipMemcpy((char *)d + tb, tm.data(), mb, hipMemcpyHostToDevice));
+            it = c->lag_equi_tab.emplace(L, std::make_tuple(d, (int)tl.size(), tb + mb, fixed)).first;
+        }
+        *mode = 5;
+        *ntile = std::get<1>(it->second);
+        *tlist = (const int2 *)std::get<0>(it->second);
+        *tmap = (const int *)((const char *)std::get<0>(it->second) + sizeof(int2) * *ntile);
+        *nnyq = std::get<3>(it->second); /* fixed-point cells, for the issued-work count */
+        *nyq_sep = 0;
+        return AMB_OK;
+    }
     if (r12 && !c->no_sym && !c->no_shear && L % 256 == 0) {
         /* sheared (t, k3) tiles: exactly the distinct sums, edges folded
            (FP32 128 x 128, FP64 64 x 128 tiles) */
@@ -1831,6 +1912,9 @@ int amb::lagft_run(amb_ctx *c, const amb_tri_req *req, int n, double t0, double 
                                        j.ntile, L, nullptr, St);
                 else if (j.mode == 4)
                     hipLaunchKernelGGL((k_lagft_reduce<float, 4>), gr, dim3(256), 0, c->stream, g.part, j.nseg,
+                                       j.ntile, L, j.tmap, St);
+                else if (j.mode == 5)
+                    hipLaunchKernelGGL((k_lagft_reduce<float, 5>), gr, dim3(256), 0, c->stream, g.part, j.nseg,
                                        j.ntile, L, j.tmap, St);
                 else
                     hipLaunchKernelGGL((k_lagft_reduce<float, 0>), gr, dim3(256), 0, c->stream, g.part, j.nseg,
@@ -2158,6 +2242,8 @@ hipError_t amb::lag_launch_finish(amb_ctx *c, hipStream_t s, const LagTri &t, co
         hipLaunchKernelGGL((k_lagft_reduce<float, 3>), gr, dim3(256), 0, s, pt, t.nseg, t.ntile, L, nullptr, st);
     else if (t.mode == 4)
         hipLaunchKernelGGL((k_lagft_reduce<float, 4>), gr, dim3(256), 0, s, pt, t.nseg, t.ntile, L, t.tmap, st);
+    else if (t.mode == 5)
+        hipLaunchKernelGGL((k_lagft_reduce<float, 5>), gr, dim3(256), 0, s, pt, t.nseg, t.ntile, L, t.tmap, st);
     else
         hipLaunchKernelGGL((k_lagft_reduce<float, 0>), gr, dim3(256), 0, s, pt, t.nseg, t.ntile, L, nullptr, st);
     const bool folded = (t.mode == 0 || t.mode == 2) && (L / 2) % tile_b<float>() == 0;
