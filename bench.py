@@ -63,7 +63,7 @@ TUNE_TABLE = os.path.join(ROOT, "amber_amd", "tune_gfx950.txt")
 MEASURED_TRAFFIC_STEP = {("cfg3", "bf16x3", "grouped"): 1.909e13,
                          # lag-Fourier form (compute grids, symmetric tilings, 1024-pixel items):
                          # k_lagft_tiles, FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00
-                         ("cfg3", "lagft", "grouped"): 1.481e12}
+                         ("cfg3", "lagft", "grouped"): 1.468e12}
 
 
 def traffic_per_launch(args, st, nsteps):
