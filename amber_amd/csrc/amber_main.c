@@ -85,7 +85,7 @@ int main(int argc, char **argv)
     int Rlzn = atoi(argv[1]);
     const char *params = "input_params_ess", *rings_file = NULL, *res_path = RES_PATH;
     int nr = 22, nc10 = 383, uf_nc = 552, rp_start = 0, rp_end = -1, device = 0, calcr_plane = 5;
-    int prec = AMB_PREC_BF16X3;
+    int prec = AMB_PREC_LAGFT; /* the fastest and most accurate form */
     int wire_legacy = 1, wire_hardened = 0; /* --wire: which result files to write */
     int cert_strict = 0; /* --cert: uncertified fast v2 normalisation sums recomputed exactly, or an error */
     const char *tune_file = NULL; /* --tune: launch-configuration table, loaded, or tuned and saved */
@@ -203,6 +203,10 @@ int main(int argc, char **argv)
 #endif
     int *tri = malloc(sizeof(int) * 3 * blk);
     if (rp_end < 0 || rp_end > rcount) rp_end = rcount;
+    if (tune_file && prec == AMB_PREC_LAGFT) {
+        printf("--tune ignored: the lag-Fourier form has no launch-configuration table\n");
+        tune_file = NULL;
+    }
     if (tune_file) {
         FILE *tf = fopen(tune_file, "r");
         if (tf) {

@@ -111,7 +111,8 @@ def test_hardened_wire_matches_legacy(tmp_path):
     W = 2 * nc10 + 1
     for (r1, r2, r3, _) in rd:
         r1, r2, r3 = int(r1), int(r2), int(r3)
-        for kind, pre, d, prec in ((1, "bsout_final", "v1", A.PREC_FP32), (2, "bsout_final2", "v2", A.PREC_BF16X3)):
+        # amber_v2 ran with the drivers' default form, the lag-Fourier one
+        for kind, pre, d, prec in ((1, "bsout_final", "v1", A.PREC_FP32), (2, "bsout_final2", "v2", A.PREC_LAGFT)):
             legacy = A.read_bsout(str(tmp_path / "results" / d / ("%s_ra%d_rb%d_rc%d_R1" % (pre, r1, r2, r3))))
             h, bl, den = A.read_bsw(str(tmp_path / "results" / d / ("bsw%d_ra%d_rb%d_rc%d_R1" % (kind, r1, r2, r3))))
             assert legacy[0] == kind and h["kind"] == kind
@@ -125,17 +126,18 @@ def test_v2_certificate_modes(tmp_path):
     """amber_v2 on a cube whose fast normalisation sums cannot be certified
     (N = 256, Ng up to 192): by default the run recomputes them in FP64 and
     writes exact normalisations (checked against the oracle's snapped
-    sums); --cert strict makes it an error exit."""
+    sums); --cert strict makes it an error exit.  Run with the split form:
+    the certificate belongs to the MFMA forms."""
     N, nc, nflag, nc10 = 256, 40, 0, 12
     lo, hi = A.log_rings(20, 60, 3)
     gv, samp, _ = _layout(tmp_path, N, nc, nflag, 61, lo, hi)
     print("case v2 certificate modes N=%d nc=%d" % (N, nc), flush=True)
-    _run(tmp_path, "amber", "--nc10", str(nc10), "--uf-nc", str(nc))
+    _run(tmp_path, "amber", "--nc10", str(nc10), "--uf-nc", str(nc), "--precision", "bf16x3")
     for fn in ("rcount", "closed_rings"):
         shutil.copy(tmp_path / "results/v1" / fn, tmp_path / "results/v2" / fn)
     n, rd = A.read_rings(str(tmp_path / "results/v1"))
     assert n > 0 and max(A.ng(hi[int(r1)]) for r1 in rd[:, 0]) >= 134
-    out = _run(tmp_path, "amber_v2", "--nc10", str(nc10), "--uf-nc", str(nc))
+    out = _run(tmp_path, "amber_v2", "--nc10", str(nc10), "--uf-nc", str(nc), "--precision", "bf16x3")
     assert "recomputed in FP64" in out, out[-2000:]
     mask = np.ones(nc, np.uint8)
     W = 2 * nc10 + 1
@@ -146,7 +148,7 @@ def test_v2_certificate_modes(tmp_path):
         P = A.ng(hi[r1]) ** 2
         assert np.array_equal(norm, (P * np.rint(n64 / P * 8) / 8).astype(np.float32)), (r1, r2, r3)
     r = subprocess.run([os.path.join(BIN, "amber_v2"), "1", "--rings", "rings.txt", "--nc10", str(nc10), "--uf-nc",
-                        str(nc), "--cert", "strict"], cwd=tmp_path, stdout=subprocess.PIPE,
+                        str(nc), "--precision", "bf16x3", "--cert", "strict"], cwd=tmp_path, stdout=subprocess.PIPE,
                        stderr=subprocess.STDOUT, text=True, timeout=120)
     assert r.returncode != 0 and "1/8 grid" in r.stdout, r.stdout[-2000:]
 
@@ -159,15 +161,15 @@ def test_tune_option_tunes_then_loads(tmp_path):
     lo, hi = A.log_rings(2, 20, 5)
     _layout(tmp_path, N, nc, nflag, 33, lo, hi)
     print("case driver --tune N=%d nc=%d" % (N, nc), flush=True)
-    _run(tmp_path, "amber", "--nc10", str(nc10), "--uf-nc", str(nc - nflag))
+    _run(tmp_path, "amber", "--nc10", str(nc10), "--uf-nc", str(nc - nflag), "--precision", "bf16x3")
     n, rd = A.read_rings(str(tmp_path / "results/v1"))
     names = ["bsout_final_ra%d_rb%d_rc%d_R1" % tuple(int(x) for x in r[:3]) for r in rd]
     plain = {f: (tmp_path / "results/v1" / f).read_bytes() for f in names}
     table = tmp_path / "tune.txt"
-    out = _run(tmp_path, "amber", "--nc10", str(nc10), "--uf-nc", str(nc - nflag), "--tune", str(table))
+    out = _run(tmp_path, "amber", "--nc10", str(nc10), "--uf-nc", str(nc - nflag), "--precision", "bf16x3", "--tune", str(table))
     assert "tuned on" in out and table.exists()
     assert table.read_text().startswith("amber_mi355x-tune 1 ")
     assert all((tmp_path / "results/v1" / f).read_bytes() == plain[f] for f in names)
-    out = _run(tmp_path, "amber", "--nc10", str(nc10), "--uf-nc", str(nc - nflag), "--tune", str(table))
+    out = _run(tmp_path, "amber", "--nc10", str(nc10), "--uf-nc", str(nc - nflag), "--precision", "bf16x3", "--tune", str(table))
     assert "loaded from" in out
     assert all((tmp_path / "results/v1" / f).read_bytes() == plain[f] for f in names)
