@@ -355,23 +355,48 @@ inline int digit_reversed(int c, const FftPlan &pl)
     return p;
 }
 
+/* Index map of an in-place buffer: position i of the transform is held at
+ * x[ix(i)]; ix is a bijection of [0, n). */
+struct IdxLinear {
+    FFTC_HD int operator()(int i) const { return i; }
+};
+/* Bank swizzle of an 8-byte-element LDS buffer: bits 0-4 of i are XORed
+ * with q = bits 5-7 and with bits 6-7 moved to bits 3-4.  It moves i only
+ * inside its aligned block of 32, so it is a bijection of [0, n) when 32
+ * divides n (otherwise the identity).  Against 64 banks of 4 bytes (32
+ * elements a cycle) it spreads the first two radix-8 stages of a 64-thread
+ * team (stride 8 and blocks of 8 at stride 64) over all banks, where the
+ * identity puts 8 and 4 lanes on one bank; the later stages, whose lanes
+ * read consecutive positions, stay conflict-free. */
+struct IdxXor32 {
+    int m;
+    FFTC_HD explicit IdxXor32(int n) : m(n % 32 == 0 ? 31 : 0) {}
+    FFTC_HD int operator()(int i) const
+    {
+        const int q = (i >> 5) & 7;
+        return i ^ ((q | ((q >> 1) & 3) << 3) & m);
+    }
+};
+
 /* One in-place decimation-in-time stage of radix R: sub-transforms of
  * length Ns are combined R at a time; butterfly (g, k) reads positions
  * g Ns R + k + r Ns, twiddles input r by w^{r k n / (Ns R)} and writes its
  * R outputs back to the same positions (they are in registers first). */
-template <int R, int DIR, typename C>
-FFTC_HD void stage_ip(C *__restrict__ x, const C *__restrict__ tw, int n, int Ns, int t, int T)
+template <int R, int DIR, typename C, typename IX = IdxLinear>
+FFTC_HD void stage_ip(C *__restrict__ x, const C *__restrict__ tw, int n, int Ns, int t, int T, IX ix = IX())
 {
     const int nR = n / R, tstep = n / (Ns * R);
     int g = t / Ns, k = t - g * Ns;
     const int dg = T / Ns, dk = T - dg * Ns;
     for (int j = t; j < nR; j += T) {
-        C *b = x + g * Ns * R + k;
-        C v[R];
-        v[0] = b[0];
+        const int b = g * Ns * R + k;
+        C v[R], o[R];
+        v[0] = x[ix(b)];
 #pragma unroll
-        for (int r = 1; r < R; ++r) v[r] = twmul<DIR>(b[r * Ns], tw[r * k * tstep]);
-        Bfly<R, DIR, C>::run(v, tw, n / R, b, Ns);
+        for (int r = 1; r < R; ++r) v[r] = twmul<DIR>(x[ix(b + r * Ns)], tw[r * k * tstep]);
+        Bfly<R, DIR, C>::run(v, tw, n / R, o, 1);
+#pragma unroll
+        for (int r = 0; r < R; ++r) x[ix(b + r * Ns)] = o[r];
         k += dk;
         g += dg;
         if (k >= Ns) { k -= Ns; ++g; }
@@ -379,22 +404,22 @@ FFTC_HD void stage_ip(C *__restrict__ x, const C *__restrict__ tw, int n, int Ns
 }
 
 /* In-place transform of a team's buffer x (input in digit-reversed order,
- * output in natural order) for plans of radices 2, 3, 4, 5 and 8 only (the
- * {2,3,5}-smooth lengths of the channel axis); one team_sync per stage, as
- * transform(). */
-template <int DIR, typename C>
-__device__ void transform_ip(C *x, const C *tw, const FftPlan &pl, int t, int T)
+ * output in natural order, both through the index map ix) for plans of
+ * radices 2, 3, 4, 5 and 8 only (the {2,3,5}-smooth lengths of the channel
+ * axis); one team_sync per stage, as transform(). */
+template <int DIR, typename C, typename IX = IdxLinear>
+__device__ void transform_ip(C *x, const C *tw, const FftPlan &pl, int t, int T, IX ix = IX())
 {
     const int n = pl.n;
     int Ns = 1;
     for (int s = 0; s < pl.nst; ++s) {
         const int R = pl.rad[s];
         switch (R) {
-        case 2: stage_ip<2, DIR>(x, tw, n, Ns, t, T); break;
-        case 3: stage_ip<3, DIR>(x, tw, n, Ns, t, T); break;
-        case 4: stage_ip<4, DIR>(x, tw, n, Ns, t, T); break;
-        case 8: stage_ip<8, DIR>(x, tw, n, Ns, t, T); break;
-        default: stage_ip<5, DIR>(x, tw, n, Ns, t, T); break;
+        case 2: stage_ip<2, DIR>(x, tw, n, Ns, t, T, ix); break;
+        case 3: stage_ip<3, DIR>(x, tw, n, Ns, t, T, ix); break;
+        case 4: stage_ip<4, DIR>(x, tw, n, Ns, t, T, ix); break;
+        case 8: stage_ip<8, DIR>(x, tw, n, Ns, t, T, ix); break;
+        default: stage_ip<5, DIR>(x, tw, n, Ns, t, T, ix); break;
         }
         team_sync();
         Ns *= R;

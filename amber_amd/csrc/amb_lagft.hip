@@ -284,8 +284,13 @@ __global__ __launch_bounds__(256) void k_chan_spectra(const R *__restrict__ D, l
  * planes tid / 16 + 16 m.  Four rounds then each transform 4 pairs, one
  * per 64-thread team: the holders of a round's pairs write their values at
  * the digit-reversed positions pos[i] of a zeroed team buffer, and the
- * in-place transform leaves the spectrum in natural order.  Dynamic LDS:
- * [tw L if TWL] + 4 L complex + nact positions.  The source is a data set as
+ * in-place transform leaves the spectrum in natural order.  The team
+ * buffers are L + 8 values apart and addressed through fftc::IdxXor32 (a
+ * permutation inside each aligned block of 32): the 16 writers of a
+ * scatter (4 buffers x 4 planes, whose digit-reversed positions are L/8
+ * apart) and the first two radix-8 stages then spread over the banks
+ * instead of landing 4 to 16 on one.  Dynamic LDS: [tw L if TWL] +
+ * 4 (L + 8) complex + nact positions.  The source is a data set as
  * computed (kind 11), read times 2^(7 - E), E = amb_set_exp of its max
  * word; the max word and E follow the spectra. */
 constexpr int SPB_PLANES = 768;
@@ -302,13 +307,15 @@ __global__ __launch_bounds__(256, 2) void k_chan_spectra_blk(const float *__rest
     const int L = pl.n, h = L / 2;
     /* TWL: the twiddles in LDS; otherwise read through the cache, and
        three workgroups fit a CU instead of two */
+    const int LB = L + 8;
+    const fftc::IdxXor32 sw(L);
     float2 *twl = (float2 *)lraw, *buf = twl + (TWL ? L : 0);
     const float2 *tw = TWL ? twl : twg;
-    int *pos = (int *)(buf + 4 * L);
+    int *pos = (int *)(buf + 4 * LB);
     const int tid = threadIdx.x, team = tid >> 6, t = tid & 63;
     if (TWL)
         for (int i = tid; i < L; i += 256) twl[i] = twg[i];
-    for (int i = tid; i < nact; i += 256) pos[i] = posg[i];
+    for (int i = tid; i < nact; i += 256) pos[i] = sw(posg[i]);
     const int E = amb_set_exp(__uint_as_float(slot_in[0]));
     const float sc = ldexpf(1.f, AMB_SET_EXP - E);
     if (blockIdx.x == 0 && tid == 0) {
@@ -331,12 +338,12 @@ __global__ __launch_bounds__(256, 2) void k_chan_spectra_blk(const float *__rest
         }
         v[m] = w;
     }
-    float2 *bx = buf + (size_t)team * L;
+    float2 *bx = buf + (size_t)team * LB;
     for (int r = 0; r < 4; ++r) {
         for (int p = t; p < L; p += 64) bx[p] = make_float2(0.f, 0.f);
         __syncthreads();
         if ((j >> 2) == r) {
-            float2 *bt = buf + (size_t)(j & 3) * L;
+            float2 *bt = buf + (size_t)(j & 3) * LB;
 #pragma unroll
             for (int m = 0; m < NQ; ++m) {
                 const int i = i0 + 16 * m;
@@ -344,13 +351,13 @@ __global__ __launch_bounds__(256, 2) void k_chan_spectra_blk(const float *__rest
             }
         }
         __syncthreads();
-        fftc::transform_ip<-1>(bx, tw, pl, t, 64);
+        fftc::transform_ip<-1>(bx, tw, pl, t, 64, sw);
         const long xa = x0 + 2 * (4 * r + team);
         if (xa < P) {
             float2 *oa = X + (size_t)xa * lhp, *ob = oa + lhp;
             const bool two = xa + 1 < P;
             for (int k = t; k <= h; k += 64) {
-                const float2 a = bx[k], b = bx[k == 0 ? 0 : L - k];
+                const float2 a = bx[sw(k)], b = bx[sw(k == 0 ? 0 : L - k)];
                 /* A = (Z(k) + conj Z(L-k)) / 2, B = (Z(k) - conj Z(L-k)) / 2i */
                 oa[k] = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y - b.y));
                 if (two) ob[k] = make_float2(0.5f * (a.y + b.y), -0.5f * (a.x - b.x));
@@ -1463,7 +1470,8 @@ hipError_t amb::lag_launch_spectra(amb_ctx *c, hipStream_t s, const BuildOp &op)
         /* the in-place transform's radices: 2, 3, 4, 5 and 8 */
         bool smooth5 = true;
         for (int s2 = 0; s2 < pl.nst; ++s2) smooth5 = smooth5 && (pl.rad[s2] <= 5 || pl.rad[s2] == 8);
-        const size_t lds_blk = sizeof(float2) * 4 * (size_t)L + sizeof(int) * (size_t)cb.nact();
+        /* four team buffers of L + 8 values (the bank swizzle's padding) */
+        const size_t lds_blk = sizeof(float2) * 4 * (size_t)(L + 8) + sizeof(int) * (size_t)cb.nact();
         if (cb.nact() <= SPB_PLANES && smooth5 && lds_blk <= 65536) {
             const long nblk = ((P + 31) / 32 + 7) / 8 * 8;
             hipLaunchKernelGGL(k_chan_spectra_blk<false>, dim3((unsigned)nblk), dim3(256), lds_blk, s,

@@ -59,9 +59,10 @@ extern "C" int fftc_host(int n, int dir, int dbl, int team, int maxp, const doub
     return dir > 0 ? run<float, 1>(n, team, maxp, in, out) : run<float, -1>(n, team, maxp, in, out);
 }
 
-/* the in-place form: input placed at its digit-reversed positions */
-template <typename T, int DIR>
-static int run_ip(int n, int TT, const double *in, double *out)
+/* the in-place form: input placed at its digit-reversed positions, the
+   buffer read and written through the index map ix */
+template <typename T, int DIR, typename IX>
+static int run_ip(int n, int TT, const double *in, double *out, IX ix)
 {
     typedef typename fftc::CT<T>::c C;
     FftPlan pl;
@@ -72,32 +73,51 @@ static int run_ip(int n, int TT, const double *in, double *out)
     for (int m = 0; m < n; ++m) {
         const double a = 2.0 * M_PI * (double)m / (double)n;
         tw[m] = fftc::cx((T)cos(a), (T)sin(a));
-        x[fftc::digit_reversed(m, pl)] = fftc::cx((T)in[2 * m], (T)in[2 * m + 1]);
+        x[ix(fftc::digit_reversed(m, pl))] = fftc::cx((T)in[2 * m], (T)in[2 * m + 1]);
     }
     int Ns = 1;
     for (int s = 0; s < pl.nst; ++s) {
         const int R = pl.rad[s];
         for (int t = 0; t < TT; ++t) {
             switch (R) {
-            case 2: fftc::stage_ip<2, DIR>(x.data(), tw.data(), n, Ns, t, TT); break;
-            case 3: fftc::stage_ip<3, DIR>(x.data(), tw.data(), n, Ns, t, TT); break;
-            case 4: fftc::stage_ip<4, DIR>(x.data(), tw.data(), n, Ns, t, TT); break;
-            case 8: fftc::stage_ip<8, DIR>(x.data(), tw.data(), n, Ns, t, TT); break;
-            default: fftc::stage_ip<5, DIR>(x.data(), tw.data(), n, Ns, t, TT); break;
+            case 2: fftc::stage_ip<2, DIR>(x.data(), tw.data(), n, Ns, t, TT, ix); break;
+            case 3: fftc::stage_ip<3, DIR>(x.data(), tw.data(), n, Ns, t, TT, ix); break;
+            case 4: fftc::stage_ip<4, DIR>(x.data(), tw.data(), n, Ns, t, TT, ix); break;
+            case 8: fftc::stage_ip<8, DIR>(x.data(), tw.data(), n, Ns, t, TT, ix); break;
+            default: fftc::stage_ip<5, DIR>(x.data(), tw.data(), n, Ns, t, TT, ix); break;
             }
         }
         Ns *= R;
     }
     for (int m = 0; m < n; ++m) {
-        out[2 * m] = (double)x[m].x;
-        out[2 * m + 1] = (double)x[m].y;
+        out[2 * m] = (double)x[ix(m)].x;
+        out[2 * m + 1] = (double)x[ix(m)].y;
     }
     return pl.nst;
 }
 
-extern "C" int fftc_host_ip(int n, int dir, int dbl, int team, const double *in, double *out)
+template <typename IX>
+static int ip_dispatch(int n, int dir, int dbl, int team, const double *in, double *out, IX ix)
 {
     if (n < 1 || team < 1) return -1;
-    if (dbl) return dir > 0 ? run_ip<double, 1>(n, team, in, out) : run_ip<double, -1>(n, team, in, out);
-    return dir > 0 ? run_ip<float, 1>(n, team, in, out) : run_ip<float, -1>(n, team, in, out);
+    if (dbl) return dir > 0 ? run_ip<double, 1>(n, team, in, out, ix) : run_ip<double, -1>(n, team, in, out, ix);
+    return dir > 0 ? run_ip<float, 1>(n, team, in, out, ix) : run_ip<float, -1>(n, team, in, out, ix);
+}
+
+extern "C" int fftc_host_ip(int n, int dir, int dbl, int team, const double *in, double *out)
+{
+    return ip_dispatch(n, dir, dbl, team, in, out, fftc::IdxLinear());
+}
+
+/* the in-place form through the bank swizzle of the channel-spectra kernel */
+extern "C" int fftc_host_ip_swz(int n, int dir, int dbl, int team, const double *in, double *out)
+{
+    return ip_dispatch(n, dir, dbl, team, in, out, fftc::IdxXor32(n));
+}
+
+/* the swizzle's image of [0, n) (a permutation when it is a bijection) */
+extern "C" void fftc_host_swz_map(int n, int *out)
+{
+    const fftc::IdxXor32 ix(n);
+    for (int i = 0; i < n; ++i) out[i] = ix(i);
 }

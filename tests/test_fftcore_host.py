@@ -29,16 +29,19 @@ def lib(tmp_path_factory):
     L = C.CDLL(out)
     L.fftc_host.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
     L.fftc_host_ip.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+    L.fftc_host_ip_swz.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+    L.fftc_host_swz_map.argtypes = [C.c_int, C.c_void_p]
     return L
 
 
-def run(lib, x, direction, dbl, team, inplace=False, maxp=47):
+def run(lib, x, direction, dbl, team, inplace=False, maxp=47, swz=False):
     n = len(x)
     a = np.empty(2 * n)
     a[0::2], a[1::2] = x.real, x.imag
     o = np.zeros(2 * n)
     if inplace:
-        rc = lib.fftc_host_ip(n, direction, dbl, team, a.ctypes.data, o.ctypes.data)
+        f = lib.fftc_host_ip_swz if swz else lib.fftc_host_ip
+        rc = f(n, direction, dbl, team, a.ctypes.data, o.ctypes.data)
     else:
         rc = lib.fftc_host(n, direction, dbl, team, maxp, a.ctypes.data, o.ctypes.data)
     assert rc > 0 or n == 1
@@ -89,6 +92,37 @@ def test_in_place_against_numpy(lib, n, dbl):
     for team in (64, 7):
         fwd = run(lib, x, -1, dbl, team, inplace=True)
         inv = run(lib, x, +1, dbl, team, inplace=True)
+        assert np.linalg.norm(fwd - np.fft.fft(x)) <= tol * np.linalg.norm(np.fft.fft(x)), (n, team)
+        assert np.linalg.norm(inv - np.fft.ifft(x) * n) <= tol * np.linalg.norm(np.fft.ifft(x) * n), (n, team)
+
+
+@pytest.mark.parametrize("n", [30, 48, 256, 1000, 1536, 2048])
+def test_bank_swizzle_is_a_permutation(lib, n):
+    """The channel-spectra kernel's LDS index map: a permutation of [0, n)
+    that moves indices only inside their aligned block of 32, and the
+    identity where 32 does not divide n."""
+    m = np.empty(n, dtype=np.int32)
+    lib.fftc_host_swz_map(n, m.ctypes.data)
+    assert sorted(m.tolist()) == list(range(n))
+    if n % 32:
+        assert np.array_equal(m, np.arange(n))
+    else:
+        assert np.array_equal(m // 32, np.arange(n) // 32)
+        assert not np.array_equal(m, np.arange(n))
+
+
+@pytest.mark.parametrize("n", [48, 256, 768, 1000, 1536, 2048])
+@pytest.mark.parametrize("dbl", [1, 0])
+def test_in_place_swizzled_against_numpy(lib, n, dbl):
+    """The in-place transform with its buffer read and written through the
+    bank swizzle (input scattered to swizzled digit-reversed positions,
+    output read back through the swizzle) is the same DFT."""
+    rng = np.random.default_rng(11 * n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    tol = 1e-13 if dbl else 2e-6
+    for team in (64, 7):
+        fwd = run(lib, x, -1, dbl, team, inplace=True, swz=True)
+        inv = run(lib, x, +1, dbl, team, inplace=True, swz=True)
         assert np.linalg.norm(fwd - np.fft.fft(x)) <= tol * np.linalg.norm(np.fft.fft(x)), (n, team)
         assert np.linalg.norm(inv - np.fft.ifft(x) * n) <= tol * np.linalg.norm(np.fft.ifft(x) * n), (n, team)
 
