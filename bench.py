@@ -88,17 +88,20 @@ def tri_grids(A, tris, hi, precision="bf16x3", ref_grid=False):
 
 
 def tri_costs(A, tris, hi, precision="bf16x3", ref_grid=False):
-    """Relative contraction cost of each triangle: its grid's pixels times the
-    triplets it computes (half of them where r1 == r2 or r2 == r3 lets the
-    kernel mirror the other half); in the lag-Fourier form the pixels times
-    its tiles."""
+    """Relative contraction cost of each triangle, used to cut the list into
+    slices of near-equal work: its grid's pixels times the triplets it
+    computes (half of them where r1 == r2 or r2 == r3 lets the kernel mirror
+    the other half); in the lag-Fourier form the pixels times the share of
+    the (k2, k3) half plane its tiling sums: half for r1 == r2 (sheared
+    tiles) and for r2 == r3 (the tiles below the diagonal of [0, L/2)^2 and
+    past the anti-diagonal left out), about a quarter for r1 == r2 == r3
+    (the principal domain's tiles)."""
     ng = tri_grids(A, tris, hi, precision, ref_grid)
     sym = np.array([(t[0] == t[1]) or (t[1] == t[2]) for t in tris])
-    if precision == "lagft":  # r1 == r2: the symmetric (i, j) tiles, about 0.63 of the (k2, k3) ones;
-        # r2 == r3: the (k2, k3) tiles without those below the diagonal of [0, L/2)^2, about 0.79
+    if precision == "lagft":
         r12 = np.array([t[0] == t[1] for t in tris])
         r23 = np.array([t[1] == t[2] for t in tris])
-        return ng * ng * np.where(r12, 0.65, np.where(r23, 0.8, 1.0))
+        return ng * ng * np.where(r12 & r23, 0.25, np.where(r12 | r23, 0.5, 1.0))
     return ng * ng * np.where(sym, 0.5, 1.0)
 
 
