@@ -625,7 +625,12 @@ __device__ __forceinline__ void shear_tile64(const LagJob<double> &J, int sg, in
     const long P = J.P;
     const int lhp = J.lhp, L = J.L, seg = J.seg, half = L / 2;
     const int ntb = L / 256;
-    const int a = (tile / ntb) * 64, b = (tile % ntb) * 128;
+    int a = (tile / ntb) * 64, b = (tile % ntb) * 128;
+    if (J.mode == 5) { /* r1 == r2 == r3: the listed tiles of the principal domain */
+        const int2 ij = J.tlist[tile];
+        a = ij.x * 64;
+        b = ij.y * 128;
+    }
     const long xb = (long)sg * seg, xe = xb + seg < P ? xb + seg : P;
     const int tid = threadIdx.x;
 
@@ -809,7 +814,7 @@ __global__ __launch_bounds__(256, LagT<R>::WPS) void k_lagft_tiles(const LagJob<
             return;
         }
     } else {
-        if (mode == 3) {
+        if (mode == 3 || mode == 5) {
             shear_tile64<1, SH64_PXB>(jobs[jb], sg, tile, lds[0], lds[1]);
             return;
         }
@@ -1622,30 +1627,32 @@ int choose_tiles(amb_ctx *c, int L, bool fp64, bool r12, bool r23, int *mode, in
     *tmap = nullptr;
     *nyql = nullptr;
     *nnyq = 0;
-    if (!fp64 && r12 && r23 && !c->no_sym && !c->no_shear && L % 256 == 0) {
-        /* r1 == r2 == r3: the sheared tiles that hold a cell of the principal
-           domain {0 <= k3 <= k2, 2 k2 + k3 <= L}, and the last-t tiles whose
-           fixed points t = L/2 lie in it; listed once per L */
-        auto it = c->lag_equi_tab.find(L);
+    if (r12 && r23 && !c->no_sym && !c->no_shear && L % 256 == 0) {
+        /* r1 == r2 == r3: the sheared tiles (t width 128 in FP32, 64 in
+           FP64; k3 width 128) that hold a cell of the principal domain
+           {0 <= k3 <= k2, 2 k2 + k3 <= L}, and the last-t tiles whose fixed
+           points t = L/2 lie in it; listed once per L and precision */
+        const int key = fp64 ? -L : L;
+        auto it = c->lag_equi_tab.find(key);
         if (it == c->lag_equi_tab.end()) {
-            const int half = L / 2, nt = half / 128;
+            const int half = L / 2, ta = fp64 ? 64 : 128, nta = half / ta, ntb = half / 128;
             auto indom = [&](int p, int q) { return q >= 0 && q <= p && q <= L - 2 * p; };
-            std::vector<int> tm((size_t)nt * nt, -1);
+            std::vector<int> tm((size_t)nta * ntb, -1);
             for (int k3 = 0; k3 < half; ++k3)
                 for (int k2 = 0; k2 <= half; ++k2) {
                     const int t = k2 + (k3 >> 1);
-                    if (t < half && indom(k2, k3)) tm[(size_t)(t / 128) * nt + k3 / 128] = 1;
+                    if (t < half && indom(k2, k3)) tm[(size_t)(t / ta) * ntb + k3 / 128] = 1;
                 }
             for (int k3 = 0; k3 < half; k3 += 2)
-                if (indom(half - k3 / 2, k3)) tm[(size_t)(nt - 1) * nt + k3 / 128] = 1;
+                if (indom(half - k3 / 2, k3)) tm[(size_t)(nta - 1) * ntb + k3 / 128] = 1;
             std::vector<int2> tl;
             int fixed = 0;
-            for (int A = 0; A < nt; ++A)
-                for (int B = 0; B < nt; ++B)
-                    if (tm[(size_t)A * nt + B] > 0) {
-                        tm[(size_t)A * nt + B] = (int)tl.size();
+            for (int A = 0; A < nta; ++A)
+                for (int B = 0; B < ntb; ++B)
+                    if (tm[(size_t)A * ntb + B] > 0) {
+                        tm[(size_t)A * ntb + B] = (int)tl.size();
                         tl.push_back(make_int2(A, B));
-                        if (A == nt - 1) fixed += 64;
+                        if (A == nta - 1) fixed += 64;
                     }
             void *d = nullptr;
             const size_t tb = sizeof(int2) * tl.size(), mb = sizeof(int) * tm.size();
@@ -1653,7 +1660,7 @@ int choose_tiles(amb_ctx *c, int L, bool fp64, bool r12, bool r23, int *mode, in
             if (rc) return rc;
             HIPCHK(c, hipMemcpy(d, tl.data(), tb, hipMemcpyHostToDevice));
             HIPCHK(c, hipMemcpy((char *)d + tb, tm.data(), mb, hipMemcpyHostToDevice));
-            it = c->lag_equi_tab.emplace(L, std::make_tuple(d, (int)tl.size(), tb + mb, fixed)).first;
+            it = c->lag_equi_tab.emplace(key, std::make_tuple(d, (int)tl.size(), tb + mb, fixed)).first;
         }
         *mode = 5;
         *ntile = std::get<1>(it->second);
@@ -1901,6 +1908,9 @@ int amb::lagft_run(amb_ctx *c, const amb_tri_req *req, int n, double t0, double 
                 if (j.mode == 3)
                     hipLaunchKernelGGL((k_lagft_reduce<double, 3>), gr, dim3(256), 0, c->stream, g.part, j.nseg,
                                        j.ntile, L, nullptr, St);
+                else if (j.mode == 5)
+                    hipLaunchKernelGGL((k_lagft_reduce<double, 5>), gr, dim3(256), 0, c->stream, g.part, j.nseg,
+                                       j.ntile, L, j.tmap, St);
                 else
                     hipLaunchKernelGGL((k_lagft_reduce<double, 0>), gr, dim3(256), 0, c->stream, g.part, j.nseg,
                                        j.ntile, L, nullptr, St);

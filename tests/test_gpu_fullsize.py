@@ -119,7 +119,8 @@ def test_metric_workload_error_vs_fp64(ctx, big, metric_cells, prec):
         ctx.set_precision("bf16x3")
 
 
-@pytest.mark.parametrize("prec,t", [("bf16x3", (23, 20, 20)), ("lagft", (23, 20, 20)), ("lagft", (23, 23, 20))])
+@pytest.mark.parametrize("prec,t", [("bf16x3", (23, 20, 20)), ("lagft", (23, 20, 20)), ("lagft", (23, 23, 20)),
+                                    ("lagft", (23, 23, 23))])
 def test_metric_workload_normalisation_is_exact(ctx, big, prec, t):
     """The amber_v2 normalisation at the metric workload (768 ch x 512^2,
     Ng = 612): equal bit for bit to the FP64 oracle's sums snapped to their

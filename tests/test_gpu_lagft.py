@@ -269,9 +269,11 @@ def test_compute_grids(ctx):
 def test_v2_norm_on_sheared_fp64_tiles(ctx):
     """The exact v2 normalisation of r1 == r2 triangles at a transform length
     of 512 (256 channels, all lags), where its FP64 sums run on the sheared
-    (t, k3) tiles: equal bit for bit to the FP64 oracle's sums snapped to
-    their 1/8 grid of Ng^2, and to a context with the symmetries off
-    (AMB_NO_SYM, the full FP64 tiling)."""
+    (t, k3) tiles, and of r1 == r2 == r3 ones, which take only the sheared
+    tiles of the principal domain and read every other cell from its orbit:
+    equal bit for bit to the FP64 oracle's sums snapped to their 1/8 grid of
+    Ng^2, and to a context with the symmetries off (AMB_NO_SYM, the full
+    FP64 tiling)."""
     import os
     N, nc, nc10 = 64, 256, 255
     lo, hi = A.log_rings(2, 20, 5)
@@ -291,7 +293,9 @@ def test_v2_norm_on_sheared_fp64_tiles(ctx):
         ctx.set_channels(cube, None, nc10)
         cube2 = plain.load_cube(N, nc, 1.0, 0, gv, samp)
         plain.set_channels(cube2, None, nc10)
-        for t in tris[-3:]:
+        equi = [t for t in tris if t[1] == t[2]]
+        assert len(equi) >= 2
+        for t in [t for t in tris if t[1] != t[2]][-3:] + equi[-2:]:
             print("case lagft v2 norm on sheared FP64 tiles, triangle %s" % (t,), flush=True)
             a = ctx.bispec_ex(cube, *t, want_norm=True)
             b = plain.bispec_ex(cube2, *t, want_norm=True)
